@@ -1,0 +1,153 @@
+"""Transport throughput on one MI355X: packets/s for the full-abundance
+45^3-octant workload (800 frequency bins, 1e8 packets per sweep).
+
+A step is one transport sweep: zero the estimators, transport all packets of
+the source, keep J_nu and the escape SED resident in HBM (the host copy is
+not part of the step).  Tables are synthetic of the workload's shape
+(mocassin_amd/synth.py).  Prints one JSON line with the roofline of the
+transport kernel (HIP-event timed on its own stream) and a CPU baseline: the
+C restatement of the same method (oracle/) on a bounded sample of the same
+workload, on the host's cores.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--packets N]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(c, nbins):
+    """24 B per segment (4 active + 4 opacity + 16 FP64 J read-modify-write),
+    8 + 4*ceil(log2 nbins) B per absorption (line fraction, u-test operand,
+    binary search of the re-emission CDF), 16 B per escape (FP64 RMW)."""
+    return (24.0 * c["n_seg"] + (8.0 + 4.0 * math.ceil(math.log2(nbins))) * c["n_abs"]
+            + 16.0 * c["n_esc"])
+
+
+def cpu_baseline(p, seed, budget_s=15.0):
+    """The oracle (scalar C restatement, one problem copy per thread) timed
+    on disjoint packet ranges of the same workload, threads = host cores
+    (at most 16)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import mc_oracle as O
+    from concurrent.futures import ThreadPoolExecutor
+    P = O.Problem(xAxis=p["xAxis"], yAxis=p["yAxis"], zAxis=p["zAxis"], active=p["active"],
+                  opacity=p["opacity"], nuArray=p["nuArray"], specCDF=p["specCDF"][:, 0],
+                  R_out=p["R_out"], geoCorr=p["geoCorr"], ionEdge1=p["ionEdge1"],
+                  deltaE=p["deltaE"], starPos=p["starPos"], starIdx=p["starIdx"],
+                  recPDF=p["recPDF"], totalLines=p["totalLines"])
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(16, cores))
+    t = time.perf_counter()
+    P.transport(10 ** 9, 20000, seed, 1, O.MATH_OWN, O.J_FP64)
+    rate1 = 20000 / (time.perf_counter() - t)
+    per_thread = max(20000, int(rate1 * budget_s / cores))
+
+    def work(i):
+        t0 = time.perf_counter()
+        P.transport(2 * 10 ** 9 + i * per_thread, per_thread, seed, 1, O.MATH_OWN, O.J_FP64)
+        return time.perf_counter() - t0
+
+    t = time.perf_counter()
+    with ThreadPoolExecutor(cores) as ex:
+        list(ex.map(work, range(cores)))
+    wall = time.perf_counter() - t
+    return {"value": cores * per_thread / wall, "unit": "packets/s", "cores": cores,
+            "kind": "port",
+            "sample": "%d packets of the same workload (%d threads x %d), scalar C oracle"
+                      % (cores * per_thread, cores, per_thread)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--packets", type=float, default=1e8)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
+              "(the multi-GPU sharding is a design, see the documentation)" % a.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from mocassin_amd import Transport, synth
+
+    npk = int(a.packets)
+    t0 = time.perf_counter()
+    p = synth.workload(a.workload, n_packets=npk)
+    setup_s = time.perf_counter() - t0
+    seed = 20241202
+    T = Transport(p["xAxis"], p["yAxis"], p["zAxis"], p["active"], p["nCells"], p["nbins"],
+                  p["nuArray"], p["R_out"], p["geoCorr"], p["ionEdge1"], [p["starPos"]],
+                  [p["starIdx"]], [p["deltaE"]], symmetric=p["symmetric"], seed=seed, device=0)
+    T.upload_tables(p["opacity"], p["specCDF"], p["recPDF"], p["totalLines"])
+    print("bench: workload %s, %d cells, %d bins, %d packets/step, setup %.1f s"
+          % (a.workload, p["nCells"], p["nbins"], npk, setup_s), file=sys.stderr, flush=True)
+
+    step = 0
+    for _ in range(a.warmup):
+        T.zero_estimators()
+        T.energy_packet_driver(1, npk, step * npk)
+        step += 1
+    torch.cuda.synchronize()
+    counters = []
+    t = time.perf_counter()
+    for _ in range(a.steps):
+        T.zero_estimators()
+        counters.append(T.energy_packet_driver(1, npk, step * npk))
+        step += 1
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t
+    T.close()
+
+    value = a.steps * npk / elapsed
+    kms = sum(c["kernel_ms"] for c in counters) / len(counters)
+    byts = sum(algorithmic_bytes(c, p["nbins"]) for c in counters) / len(counters)
+    achieved = byts / (kms * 1e-3) / 1e9
+    c0 = counters[-1]
+    print("bench: %.3e packets/s, kernel %.1f ms/launch, %.3g seg/packet, %.3g abs/packet"
+          % (value, kms, c0["n_seg"] / npk, c0["n_abs"] / npk), file=sys.stderr, flush=True)
+    traffic = None
+    tp = os.path.join(ROOT, "profiles", "transport_c2_traffic.json")
+    if os.path.exists(tp):
+        try:
+            traffic = json.load(open(tp)).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+    out = {
+        "metric": "packets/s/GPU (transport sweep)",
+        "value": value, "unit": "packets/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": elapsed / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "full-abundance 45^3 octant, 800 bins, %d packets/sweep, "
+                               "tables synthetic of that shape" % npk if a.workload == "c2"
+                   else a.workload,
+                   "cells": p["nCells"], "nbins": p["nbins"], "packets_per_step": npk,
+                   "accumulators": "fp64"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "transport_kernel", "kernel_ms": kms,
+                     "segments": c0["n_seg"], "absorptions": c0["n_abs"], "escapes": c0["n_esc"]},
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(p, seed)
+        out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,143 @@
+/*
+ * mocassin_amd.h - C ABI of the MI355X photon-packet transport and per-cell
+ * balance library (libmocassin_amd.so).
+ *
+ * Drop-in boundary for MOCASSIN 2.02's transport/balance step.  The Fortran
+ * host keeps every array it owns in its own layout; the library owns device
+ * buffers, copies explicitly, and retains no host pointer across calls.
+ * Layout conventions (all Fortran column-major, 1-based on the host side):
+ *   active(1:nx,1:ny,1:nz)                      int32
+ *   opacity/scaOpac/recPDF(0:nCells,1:nbins)    float,  cell index fastest
+ *   totalLines(0:nCells)                        float
+ *   specCDF(1:nbins, 1:nStars)                  float  (inSpectrumProbDen rows)
+ *   J(0:nCells,1:nbins)                         double (Jste, unscaled)
+ *   esc(0:nCells,0:nbins)                       double (escapedPackets(:,:,0))
+ * Row 0 of every (0:nCells,...) array is the inactive-cell sink, as in the
+ * reference.
+ *
+ * Every function returns 0 on success or a positive MC_E* code; the message
+ * of the last failure on a context is available from mc_ctx_last_error().
+ * The library never calls exit(): the host maps a non-zero status to its own
+ * fail-stop ("stop" in the reference).
+ */
+#ifndef MOCASSIN_AMD_H
+#define MOCASSIN_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MC_ABI_VERSION 1
+
+/* flags for mc_desc.flags */
+#define MC_SYMMETRIC   0x01  /* lgSymmetricXYZ  (set_input_mod.f90:333) */
+#define MC_DUST        0x02  /* lgDust */
+#define MC_GAS         0x04  /* lgGas */
+#define MC_ISOTROPIC   0x08  /* lgIsotropic: isotropic dust scattering */
+#define MC_MULTISTARS  0x10  /* lgMultiStars: no octant folding of stellar directions */
+
+/* error codes */
+#define MC_OK          0
+#define MC_EINVAL      1   /* bad argument or descriptor */
+#define MC_ENODEV      2   /* no usable GPU / HIP error */
+#define MC_ENOMEM      3   /* device allocation failed */
+#define MC_ESTATE      4   /* call out of order (e.g. run before tables) */
+#define MC_ETABLE      5   /* table rejected (NaN, non-monotone CDF, ...) */
+#define MC_EPACKET     6   /* a packet met a fail-stop condition of the reference */
+#define MC_EUNSUP      7   /* configuration outside the supported path */
+
+typedef struct mc_ctx mc_ctx;
+
+/* Grid and run description.  Replaces the module globals that
+ * energyPacketDriver reads (photon_mod.f90:26; common_mod.f90:241-302). */
+typedef struct {
+    int32_t abi_version;        /* MC_ABI_VERSION */
+    int32_t nx, ny, nz;         /* grid(1)%nx, ny, nz */
+    const float *xAxis;         /* grid(1)%xAxis(1:nx) [cm] */
+    const float *yAxis;
+    const float *zAxis;
+    const int32_t *active;      /* grid(1)%active, Fortran order */
+    int32_t nCells;             /* grid(1)%nCells */
+    int32_t nbins;              /* nbins */
+    int32_t flags;              /* MC_* flags */
+    float R_out;                /* R_out [cm] */
+    float geoCorr[3];           /* grid(1)%geoCorrX/Y/Z */
+    float ionEdge1;             /* ionEdge(1) [Ryd] */
+    const float *nuArray;       /* nuArray(1:nbins) [Ryd] */
+    int32_t nStars;             /* nStars */
+    const float *starPos;       /* starPosition(1:nStars) as (3,nStars) [cm] */
+    const int32_t *starIdx;     /* starIndeces(1:nStars,1:3) as (3,nStars), 1-based */
+    const float *deltaE;        /* deltaE(1:nStars) [1e36 erg/s] */
+    uint64_t seed;              /* key of the counter-based random streams */
+    int32_t device;             /* HIP device ordinal */
+} mc_desc;
+
+/* Counters of one transport call (summed over its packets). */
+typedef struct {
+    double n_seg;       /* cell-crossing or partial segments walked */
+    double n_abs;       /* absorptions (absInt) */
+    double n_sca;       /* dust scatterings (scaInt) */
+    double n_esc;       /* escapes recorded in esc */
+    double n_lost;      /* packets dropped by the reference's wall-snap return or safeLimit */
+    double n_line;      /* diffuse packets turned into (escaping) line packets */
+    double n_trapped;   /* packets that hit recursionLimit */
+    double n_flights;   /* pathSegment calls */
+    double qphot;       /* Qphot: sum deltaE/(h nu) over stellar packets with nu > 1 Ryd */
+    double kernel_ms;   /* device time of the transport kernel */
+} mc_counters;
+
+/* Create a context on desc->device and upload the grid.
+ * Replaces: grid setup consumed by energyPacketDriver (photon_mod.f90:26). */
+int mc_ctx_create(const mc_desc *desc, mc_ctx **out);
+
+/* Upload the per-sweep tables (frozen during a transport sweep).
+ * scaOpac/gSca may be NULL without dust; recPDF/totalLines NULL without gas.
+ * Replaces: the reads of grid%opacity, grid%scaOpac, grid%recPDF,
+ * grid%totalLines, inSpectrumProbDen, gSca inside photon_mod.f90:768-1812;
+ * the tables are produced by ionizationDriver (ionization_mod.f90:349-484),
+ * emissionDriver (emission_mod.f90:905-1311) and setProbDen
+ * (continuum_mod.f90:418-474). */
+int mc_upload_tables(mc_ctx *ctx, const float *opacity, const float *scaOpac,
+                     const float *recPDF, const float *totalLines,
+                     const float *specCDF, const float *gSca);
+
+/* Zero the device J and escape accumulators.
+ * Replaces: iteration_mod.f90:458-472. */
+int mc_zero_estimators(mc_ctx *ctx);
+
+/* Transport npackets stellar packets of source istar (1-based), packet
+ * indices packet_offset .. packet_offset+npackets-1 (each index selects its
+ * own random stream, so disjoint ranges are independent and results do not
+ * depend on launch shape).  Accumulates into the device estimators; if J_out
+ * / esc_out are non-NULL the running totals are copied back to them
+ * (layouts above).  c may be NULL.
+ * Replaces: energyPacketDriver(iStar, n, grid) (photon_mod.f90:26-284) with
+ * energyPacketRun/newPhotonPacket/pathSegment (289-2872). */
+int mc_transport_run(mc_ctx *ctx, int32_t istar, int64_t npackets, uint64_t packet_offset,
+                     double *J_out, double *esc_out, mc_counters *c);
+
+/* Copy the device estimators to the host (layouts above). */
+int mc_fetch_estimators(mc_ctx *ctx, double *J_out, double *esc_out);
+
+/* Geometry-only replay of pathSegment (photon_mod.f90:1194-2836) with zero
+ * opacity: for each of n packets starting at pos(3,n) with direction
+ * dir(3,n) in cell(3,n) (1-based), writes the visited cells
+ * cell_seq(3,max_len,n) (1-based x,y,z), the FP32 path length of each
+ * segment ds_seq(max_len,n), seq_len(n) and status(n) (1 escaped, 2 dropped
+ * by the upper-wall snap return, 3 safe limit, <0 error). */
+int mc_walk_replay(mc_ctx *ctx, int32_t n, const float *pos, const float *dir,
+                   const int32_t *cell, int32_t *cell_seq, float *ds_seq,
+                   int32_t *seq_len, int32_t *status, int32_t max_len);
+
+/* Message of the last failure on this context (never NULL). */
+const char *mc_ctx_last_error(const mc_ctx *ctx);
+
+/* Release all device memory of the context. */
+void mc_ctx_destroy(mc_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MOCASSIN_AMD_H */

@@ -1,0 +1,76 @@
+"""ctypes declarations of include/mocassin_amd.h (libmocassin_amd.so).
+
+The shared library is built in-tree by `make -C mocassin_amd/csrc` (or
+__graft_entry__.build()).  Loading fails loudly when it is missing: there is
+no fallback path.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmocassin_amd.so")
+
+MC_ABI_VERSION = 1
+MC_SYMMETRIC, MC_DUST, MC_GAS, MC_ISOTROPIC, MC_MULTISTARS = 0x01, 0x02, 0x04, 0x08, 0x10
+ERRORS = {0: "MC_OK", 1: "MC_EINVAL", 2: "MC_ENODEV", 3: "MC_ENOMEM", 4: "MC_ESTATE",
+          5: "MC_ETABLE", 6: "MC_EPACKET", 7: "MC_EUNSUP"}
+
+# every extern "C" symbol the header declares
+EXPORTS = ("mc_ctx_create", "mc_upload_tables", "mc_zero_estimators", "mc_transport_run",
+           "mc_fetch_estimators", "mc_walk_replay", "mc_ctx_last_error", "mc_ctx_destroy")
+
+
+class McDesc(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("xAxis", C.c_void_p), ("yAxis", C.c_void_p), ("zAxis", C.c_void_p),
+        ("active", C.c_void_p), ("nCells", C.c_int32), ("nbins", C.c_int32),
+        ("flags", C.c_int32), ("R_out", C.c_float), ("geoCorr", C.c_float * 3),
+        ("ionEdge1", C.c_float), ("nuArray", C.c_void_p), ("nStars", C.c_int32),
+        ("starPos", C.c_void_p), ("starIdx", C.c_void_p), ("deltaE", C.c_void_p),
+        ("seed", C.c_uint64), ("device", C.c_int32),
+    ]
+
+
+class McCounters(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "n_seg", "n_abs", "n_sca", "n_esc", "n_lost", "n_line", "n_trapped", "n_flights",
+        "qphot", "kernel_ms")]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class McError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s: %s" % (ERRORS.get(code, str(code)), msg))
+        self.code = code
+
+
+_LIB = None
+
+
+def load():
+    """Load libmocassin_amd.so, declaring the C ABI; raises if it is not built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libmocassin_amd.so is not built (%s); run "
+                           "`make -C mocassin_amd/csrc` or __graft_entry__.build()" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, u64 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint64
+    L.mc_ctx_create.argtypes = [C.POINTER(McDesc), C.POINTER(vp)]
+    L.mc_upload_tables.argtypes = [vp] * 7
+    L.mc_zero_estimators.argtypes = [vp]
+    L.mc_transport_run.argtypes = [vp, i32, i64, u64, vp, vp, C.POINTER(McCounters)]
+    L.mc_fetch_estimators.argtypes = [vp, vp, vp]
+    L.mc_walk_replay.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, vp, i32]
+    L.mc_ctx_last_error.argtypes = [vp]
+    L.mc_ctx_last_error.restype = C.c_char_p
+    L.mc_ctx_destroy.argtypes = [vp]
+    L.mc_ctx_destroy.restype = None
+    for name in EXPORTS[:6]:
+        getattr(L, name).restype = C.c_int
+    _LIB = L
+    return L

@@ -1,0 +1,395 @@
+// mc_capi.hip - C ABI (include/mocassin_amd.h) over the HIP kernels.
+//
+// A context owns one device copy of the grid and tables, the FP64
+// accumulators and one HIP stream.  No global mutable state: contexts are
+// independent, so several nebulae can live side by side in one process.
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/mocassin_amd.h"
+#include "mc_device.h"
+#include "mc_kernels.h"
+
+#pragma clang fp contract(off)
+
+struct mc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    mcd::Grid G{};
+    int nStars = 0;
+    std::vector<float> starPos, deltaE;
+    std::vector<int> starIdx;
+    bool tables = false;
+    int blocks = 0, threads = 256;
+    // device allocations
+    float *d_axis = nullptr, *d_mid = nullptr, *d_dfac = nullptr;
+    int *d_active = nullptr;
+    float *d_opacity = nullptr, *d_scaOpac = nullptr, *d_recCDF = nullptr;
+    float *d_totalLines = nullptr, *d_specCDF = nullptr, *d_nuArray = nullptr, *d_gSca = nullptr;
+    double *d_J = nullptr, *d_esc = nullptr;
+    unsigned long long *d_scratch = nullptr;  // next_packet, counts[8]
+    int *d_error = nullptr;
+    double *d_qphot = nullptr;
+    char err[512] = "";
+};
+
+static thread_local char g_create_err[512] = "";
+
+static int fail(mc_ctx *c, int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(c ? c->err : g_create_err, 512, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(c, call)                                                                   \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail((c), e_ == hipErrorOutOfMemory ? MC_ENOMEM : MC_ENODEV, "%s: %s", \
+                        #call, hipGetErrorString(e_));                                    \
+    } while (0)
+
+template <class T>
+static hipError_t dalloc(T **p, size_t n)
+{
+    return hipMalloc((void **)p, n * sizeof(T) ? n * sizeof(T) : sizeof(T));
+}
+
+static int ctx_init(mc_ctx *c, const mc_desc *d);
+
+extern "C" int mc_ctx_create(const mc_desc *d, mc_ctx **out)
+{
+    if (!out) return fail(nullptr, MC_EINVAL, "null output pointer");
+    *out = nullptr;
+    if (!d || d->abi_version != MC_ABI_VERSION)
+        return fail(nullptr, MC_EINVAL, "descriptor missing or ABI version mismatch");
+    if (d->nx < 2 || d->ny < 2 || d->nz < 2 || d->nCells < 1 || d->nbins < 3 || d->nStars < 1 ||
+        !d->xAxis || !d->yAxis || !d->zAxis || !d->active || !d->nuArray || !d->starPos ||
+        !d->starIdx || !d->deltaE)
+        return fail(nullptr, MC_EINVAL, "descriptor has empty sizes or null arrays");
+    mc_ctx *c = new mc_ctx();
+    int rc = ctx_init(c, d);
+    if (rc) {
+        memcpy(g_create_err, c->err, sizeof g_create_err);
+        mc_ctx_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return MC_OK;
+}
+
+static int ctx_init(mc_ctx *c, const mc_desc *d)
+{
+    c->device = d->device;
+    HIPCHK(c, hipSetDevice(d->device));
+    HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(c, hipEventCreate(&c->ev0));
+    HIPCHK(c, hipEventCreate(&c->ev1));
+    const int nx = d->nx, ny = d->ny, nz = d->nz, nb = d->nbins, nc = d->nCells;
+    const int na = nx + ny + nz, nm = na - 3;
+    // axes, midpoints and per-axis volume factors in FP32 exactly as the
+    // reference evaluates them inline (photon_mod.f90:1266, 1448-1500)
+    std::vector<float> axis(na), mid(nm), dfac(na);
+    const float *ax[3] = { d->xAxis, d->yAxis, d->zAxis };
+    const int n3[3] = { nx, ny, nz };
+    const bool sym = (d->flags & MC_SYMMETRIC) != 0;
+    for (int a = 0, oa = 0, om = 0; a < 3; ++a) {
+        const float *v = ax[a];
+        const int n = n3[a];
+        for (int i = 0; i < n; ++i) {
+            axis[oa + i] = v[i];
+            if (std::isnan(v[i])) return fail(c, MC_EINVAL, "NaN in axis %d", a);
+        }
+        for (int i = 0; i + 1 < n; ++i) {
+            mid[om + i] = (v[i + 1] + v[i]) / 2.f;
+            if (!(v[i + 1] > v[i])) return fail(c, MC_EINVAL, "axis %d not increasing", a);
+        }
+        for (int i = 1; i <= n; ++i) {
+            float w;
+            if (i > 1 && i < n) w = std::fabs(v[i] - v[i - 2]) / 2.f;
+            else if (i == 1) w = sym ? std::fabs(v[1] - v[0]) / 2.f : std::fabs(v[1] - v[0]);
+            else w = std::fabs(v[n - 1] - v[n - 2]);
+            dfac[oa + i - 1] = w / 1.e15f;
+        }
+        oa += n;
+        om += n - 1;
+    }
+    // active map: values in [0, nCells]; sub-grids (negative) are outside the path
+    for (size_t i = 0; i < (size_t)nx * ny * nz; ++i)
+        if (d->active[i] < 0 || d->active[i] > nc)
+            return fail(c, d->active[i] < 0 ? MC_EUNSUP : MC_EINVAL,
+                        "active map entry %zu = %d outside [0,%d]", i, d->active[i], nc);
+    c->nStars = d->nStars;
+    c->starPos.assign(d->starPos, d->starPos + 3 * d->nStars);
+    c->starIdx.assign(d->starIdx, d->starIdx + 3 * d->nStars);
+    c->deltaE.assign(d->deltaE, d->deltaE + d->nStars);
+    for (int s = 0; s < d->nStars; ++s) {
+        const int *si = &c->starIdx[3 * s];
+        if (si[0] < 1 || si[0] > nx || si[1] < 1 || si[1] > ny || si[2] < 1 || si[2] > nz)
+            return fail(c, MC_EINVAL, "source %d cell outside the grid", s + 1);
+    }
+
+    HIPCHK(c, dalloc(&c->d_axis, na));
+    HIPCHK(c, dalloc(&c->d_mid, nm));
+    HIPCHK(c, dalloc(&c->d_dfac, na));
+    HIPCHK(c, dalloc(&c->d_active, (size_t)nx * ny * nz));
+    HIPCHK(c, dalloc(&c->d_nuArray, nb));
+    HIPCHK(c, dalloc(&c->d_J, (size_t)(nc + 1) * nb));
+    HIPCHK(c, dalloc(&c->d_esc, (size_t)(nc + 1) * (nb + 1)));
+    HIPCHK(c, dalloc(&c->d_scratch, 9));
+    HIPCHK(c, dalloc(&c->d_error, 1));
+    HIPCHK(c, dalloc(&c->d_qphot, 1));
+    HIPCHK(c, hipMemcpy(c->d_axis, axis.data(), na * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_mid, mid.data(), nm * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_dfac, dfac.data(), na * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_active, d->active, (size_t)nx * ny * nz * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_nuArray, d->nuArray, nb * sizeof(float), hipMemcpyHostToDevice));
+
+    mcd::Grid &G = c->G;
+    G.nx = nx; G.ny = ny; G.nz = nz; G.nCells = nc; G.nbins = nb; G.flags = d->flags;
+    G.nStars = d->nStars;
+    G.R_out = d->R_out;
+    G.geoCorrX = d->geoCorr[0]; G.geoCorrY = d->geoCorr[1]; G.geoCorrZ = d->geoCorr[2];
+    G.ionEdge1 = d->ionEdge1;
+    G.axis = c->d_axis; G.mid = c->d_mid; G.dfac = c->d_dfac; G.active = c->d_active;
+    G.nuArray = c->d_nuArray;
+    G.J = c->d_J; G.esc = c->d_esc;
+    G.seed_lo = (uint32_t)d->seed; G.seed_hi = (uint32_t)(d->seed >> 32);
+
+    int cus = 0;
+    HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->device));
+    c->blocks = 8 * (cus > 0 ? cus : 256);
+    c->threads = 256;
+    int rc = mc_zero_estimators(c);
+    if (rc) return rc;
+    return MC_OK;
+}
+
+// monotone non-decreasing, finite, within [0, 1 + tiny]
+static bool cdf_ok(const float *row, size_t stride, int nb)
+{
+    float prev = 0.f;
+    for (int i = 0; i < nb; ++i) {
+        float v = row[(size_t)i * stride];
+        if (!(v >= prev) || !(v <= 1.0001f)) return false;
+        prev = v;
+    }
+    return true;
+}
+
+extern "C" int mc_upload_tables(mc_ctx *c, const float *opacity, const float *scaOpac,
+                                const float *recPDF, const float *totalLines,
+                                const float *specCDF, const float *gSca)
+{
+    if (!c) return MC_EINVAL;
+    mcd::Grid &G = c->G;
+    const int nc = G.nCells, nb = G.nbins;
+    const size_t nt = (size_t)(nc + 1) * nb;
+    const bool dust = (G.flags & MC_DUST) != 0, gas = (G.flags & MC_GAS) != 0;
+    if (!opacity || !specCDF) return fail(c, MC_EINVAL, "opacity and specCDF are required");
+    if (gas && (!recPDF || !totalLines)) return fail(c, MC_EINVAL, "gas needs recPDF and totalLines");
+    if (dust && !scaOpac) return fail(c, MC_EINVAL, "dust needs scaOpac");
+    if (dust && !(G.flags & MC_ISOTROPIC) && !gSca) return fail(c, MC_EINVAL, "HG scattering needs gSca");
+    HIPCHK(c, hipSetDevice(c->device));
+    for (size_t i = 0; i < nt; ++i)
+        if (!(opacity[i] >= 0.f) || std::isinf(opacity[i]))
+            return fail(c, MC_ETABLE, "opacity entry %zu not finite and >= 0", i);
+    for (int s = 0; s < c->nStars; ++s)
+        if (!cdf_ok(specCDF + (size_t)s * nb, 1, nb))
+            return fail(c, MC_ETABLE, "specCDF of source %d not a monotone CDF", s + 1);
+    std::vector<float> rec;
+    if (gas) {
+        rec.resize(nt);
+        for (int cell = 0; cell <= nc; ++cell) {
+            if (cell > 0 && !cdf_ok(recPDF + cell, (size_t)(nc + 1), nb))
+                return fail(c, MC_ETABLE, "recPDF row of cell %d not a monotone CDF", cell);
+            for (int nu = 0; nu < nb; ++nu) rec[(size_t)cell * nb + nu] = recPDF[(size_t)nu * (nc + 1) + cell];
+        }
+        for (int cell = 0; cell <= nc; ++cell)
+            if (std::isnan(totalLines[cell])) return fail(c, MC_ETABLE, "NaN in totalLines");
+    }
+    if (!c->d_opacity) HIPCHK(c, dalloc(&c->d_opacity, nt));
+    if (!c->d_specCDF) HIPCHK(c, dalloc(&c->d_specCDF, (size_t)nb * c->nStars));
+    HIPCHK(c, hipMemcpy(c->d_opacity, opacity, nt * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_specCDF, specCDF, (size_t)nb * c->nStars * sizeof(float), hipMemcpyHostToDevice));
+    if (gas) {
+        if (!c->d_recCDF) HIPCHK(c, dalloc(&c->d_recCDF, nt));
+        if (!c->d_totalLines) HIPCHK(c, dalloc(&c->d_totalLines, (size_t)nc + 1));
+        HIPCHK(c, hipMemcpy(c->d_recCDF, rec.data(), nt * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_totalLines, totalLines, (nc + 1) * sizeof(float), hipMemcpyHostToDevice));
+    }
+    if (dust) {
+        if (!c->d_scaOpac) HIPCHK(c, dalloc(&c->d_scaOpac, nt));
+        HIPCHK(c, hipMemcpy(c->d_scaOpac, scaOpac, nt * sizeof(float), hipMemcpyHostToDevice));
+        if (gSca) {
+            if (!c->d_gSca) HIPCHK(c, dalloc(&c->d_gSca, nb));
+            HIPCHK(c, hipMemcpy(c->d_gSca, gSca, nb * sizeof(float), hipMemcpyHostToDevice));
+        }
+    }
+    G.opacity = c->d_opacity; G.specCDF = c->d_specCDF;
+    G.recCDF = c->d_recCDF; G.totalLines = c->d_totalLines;
+    G.scaOpac = c->d_scaOpac; G.gSca = c->d_gSca;
+    c->tables = true;
+    return MC_OK;
+}
+
+extern "C" int mc_zero_estimators(mc_ctx *c)
+{
+    if (!c) return MC_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t nc1 = (size_t)c->G.nCells + 1;
+    HIPCHK(c, hipMemsetAsync(c->d_J, 0, nc1 * c->G.nbins * sizeof(double), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_esc, 0, nc1 * (c->G.nbins + 1) * sizeof(double), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MC_OK;
+}
+
+extern "C" int mc_fetch_estimators(mc_ctx *c, double *J_out, double *esc_out)
+{
+    if (!c) return MC_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t nc1 = (size_t)c->G.nCells + 1;
+    if (J_out)
+        HIPCHK(c, hipMemcpyAsync(J_out, c->d_J, nc1 * c->G.nbins * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream));
+    if (esc_out)
+        HIPCHK(c, hipMemcpyAsync(esc_out, c->d_esc, nc1 * (c->G.nbins + 1) * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MC_OK;
+}
+
+static const char *packet_error_text(int code)
+{
+    switch (code) {
+    case 10: return "getNu2: problem with random number generator";
+    case 11: return "newPhotonPacket: insanity in stellar photon nuP assignment";
+    case 13: return "newPhotonPacket: diffuse packet cannot be created in a no gas grid";
+    case 14: return "newPhotonPacket: position of the new diffuse photon packet is inside the inner region";
+    case 15: return "newPhotonPacket: insanity in diffuse photon nuP assignment";
+    case 32: case 35: return "pathSegment: insanity (cell index outside the grid)";
+    case 34: return "pathSegment: insane wall distance (NaN)";
+    case 36: return "pathSegment: dS <= 0";
+    case 37: return "pathSegment: insane position (NaN)";
+    case 39: return "pathSegment: insanity at the scattering/absorption decision";
+    case 41: return "dust-only re-emission (dustEmi) is outside the supported path";
+    default: return "packet fail-stop";
+    }
+}
+
+extern "C" int mc_transport_run(mc_ctx *c, int32_t istar, int64_t npackets, uint64_t packet_offset,
+                                double *J_out, double *esc_out, mc_counters *cnt)
+{
+    if (!c) return MC_EINVAL;
+    if (!c->tables) return fail(c, MC_ESTATE, "mc_upload_tables has not been called");
+    if (istar < 1 || istar > c->nStars) return fail(c, MC_EINVAL, "istar %d outside 1..%d", istar, c->nStars);
+    if (npackets < 0) return fail(c, MC_EINVAL, "negative packet count");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemsetAsync(c->d_scratch, 0, 9 * sizeof(unsigned long long), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_error, 0, sizeof(int), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_qphot, 0, sizeof(double), c->stream));
+    mcd::RunArgs RA;
+    RA.istar = istar;
+    RA.npackets = npackets;
+    RA.packet_offset = packet_offset;
+    RA.next_packet = c->d_scratch;
+    RA.counts = c->d_scratch + 1;
+    RA.error = c->d_error;
+    RA.qphot = c->d_qphot;
+    RA.deltaE = c->deltaE[istar - 1];
+    RA.px = c->starPos[3 * (istar - 1)]; RA.py = c->starPos[3 * (istar - 1) + 1];
+    RA.pz = c->starPos[3 * (istar - 1) + 2];
+    RA.sx = c->starIdx[3 * (istar - 1)]; RA.sy = c->starIdx[3 * (istar - 1) + 1];
+    RA.sz = c->starIdx[3 * (istar - 1) + 2];
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    if (npackets > 0) HIPCHK(c, mcd_launch_transport(c->G, RA, c->blocks, c->threads, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    unsigned long long h[9];
+    int herr = 0;
+    double q = 0.0;
+    HIPCHK(c, hipMemcpyAsync(h, c->d_scratch, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&herr, c->d_error, sizeof herr, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&q, c->d_qphot, sizeof q, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (cnt) {
+        cnt->n_seg = (double)h[1]; cnt->n_abs = (double)h[2]; cnt->n_sca = (double)h[3];
+        cnt->n_esc = (double)h[4]; cnt->n_lost = (double)h[5]; cnt->n_line = (double)h[6];
+        cnt->n_trapped = (double)h[7]; cnt->n_flights = (double)h[8];
+        cnt->qphot = q; cnt->kernel_ms = ms;
+    }
+    if (herr) return fail(c, MC_EPACKET, "%s (code %d)", packet_error_text(herr), herr);
+    if (J_out || esc_out) return mc_fetch_estimators(c, J_out, esc_out);
+    return MC_OK;
+}
+
+extern "C" int mc_walk_replay(mc_ctx *c, int32_t n, const float *pos, const float *dir,
+                              const int32_t *cell, int32_t *cell_seq, float *ds_seq,
+                              int32_t *seq_len, int32_t *status, int32_t max_len)
+{
+    if (!c) return MC_EINVAL;
+    if (n < 0 || max_len < 1 || (n > 0 && (!pos || !dir || !cell || !cell_seq || !ds_seq || !seq_len || !status)))
+        return fail(c, MC_EINVAL, "bad replay arguments");
+    if (n == 0) return MC_OK;
+    for (int i = 0; i < n; ++i) {
+        const int *ce = cell + 3 * i;
+        if (ce[0] < 1 || ce[0] > c->G.nx || ce[1] < 1 || ce[1] > c->G.ny || ce[2] < 1 || ce[2] > c->G.nz)
+            return fail(c, MC_EINVAL, "replay packet %d starts outside the grid", i);
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    float *dp = nullptr, *dd = nullptr, *dds = nullptr;
+    int *dc = nullptr, *dseq = nullptr, *dlen = nullptr, *dst = nullptr;
+    const size_t tot = (size_t)n * max_len;
+    int rc = MC_OK;
+    hipError_t e = hipSuccess;
+    if ((e = dalloc(&dp, 3 * (size_t)n)) == hipSuccess && (e = dalloc(&dd, 3 * (size_t)n)) == hipSuccess &&
+        (e = dalloc(&dc, 3 * (size_t)n)) == hipSuccess && (e = dalloc(&dseq, 3 * tot)) == hipSuccess &&
+        (e = dalloc(&dds, tot)) == hipSuccess && (e = dalloc(&dlen, (size_t)n)) == hipSuccess &&
+        (e = dalloc(&dst, (size_t)n)) == hipSuccess) {
+        e = hipMemcpy(dp, pos, 3 * n * sizeof(float), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dd, dir, 3 * n * sizeof(float), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dc, cell, 3 * n * sizeof(int), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemsetAsync(dseq, 0, 3 * tot * sizeof(int), c->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(dds, 0, tot * sizeof(float), c->stream);
+        if (e == hipSuccess)
+            e = mcd_launch_replay(c->G, n, dp, dd, dc, dseq, dds, dlen, dst, max_len, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipMemcpy(cell_seq, dseq, 3 * tot * sizeof(int), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(ds_seq, dds, tot * sizeof(float), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(seq_len, dlen, n * sizeof(int), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(status, dst, n * sizeof(int), hipMemcpyDeviceToHost);
+    }
+    if (e != hipSuccess) rc = fail(c, e == hipErrorOutOfMemory ? MC_ENOMEM : MC_ENODEV, "replay: %s", hipGetErrorString(e));
+    (void)hipFree(dp); (void)hipFree(dd); (void)hipFree(dc); (void)hipFree(dseq);
+    (void)hipFree(dds); (void)hipFree(dlen); (void)hipFree(dst);
+    return rc;
+}
+
+extern "C" const char *mc_ctx_last_error(const mc_ctx *c)
+{
+    return c ? c->err : g_create_err;
+}
+
+extern "C" void mc_ctx_destroy(mc_ctx *c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    void *bufs[] = { c->d_axis, c->d_mid, c->d_dfac, c->d_active, c->d_opacity, c->d_scaOpac,
+                     c->d_recCDF, c->d_totalLines, c->d_specCDF, c->d_nuArray, c->d_gSca,
+                     c->d_J, c->d_esc, c->d_scratch, c->d_error, c->d_qphot };
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}

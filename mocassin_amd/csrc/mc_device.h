@@ -1,0 +1,379 @@
+// mc_device.h - device-side view of one transport problem and the pieces of
+// MOCASSIN's packet transport that every kernel shares: the Philox stream,
+// the source-frequency draw (getNu2), the isotropic and Henyey-Greenstein
+// direction draws and one step of the Cartesian cell walk (pathSegment).
+//
+// FP32 throughout with the reference's expression order; compiled with
+// contraction off, correctly rounded division and square root and IEEE
+// denormals, so a packet history is bit-identical to the FP32 statement of
+// the reference given the same uniform stream.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "mc_fmath.h"
+
+#pragma clang fp contract(off)
+
+namespace mcd {
+
+constexpr int kSafeLimit = 500000;      // photon_mod.f90:1187-1192
+constexpr int kRecursionLimit = 5000;   // constants_mod.f90:56
+
+enum Flags { F_SYM = 1, F_DUST = 2, F_GAS = 4, F_ISO = 8, F_MULTI = 16 };
+
+// Per-problem constants and device tables.
+struct Grid {
+    int nx, ny, nz, nCells, nbins, flags, nStars;
+    float R_out, geoCorrX, geoCorrY, geoCorrZ, ionEdge1;
+    const float *axis;       // x(1:nx), y(1:ny), z(1:nz) concatenated
+    const float *mid;        // (a(i)+a(i+1))/2, i=1..n-1 per axis, concatenated
+    const float *dfac;       // per-axis cell widths /1e15 (getVolume factors)
+    const int *active;       // active(x,y,z), x fastest
+    const float *opacity;    // [nu][cell]   (reference layout)
+    const float *scaOpac;    // [nu][cell] or null
+    const float *recCDF;     // [cell][nu]   (transposed: one row per cell)
+    const float *totalLines; // [cell]
+    const float *specCDF;    // [star][nu]
+    const float *nuArray;    // [nu]
+    const float *gSca;       // [nu] or null
+    double *J;               // [nu][cell]
+    double *esc;             // [nu 0..nbins][cell]
+    uint32_t seed_lo, seed_hi;
+};
+
+// LDS copy of the small per-axis arrays (axis, midpoints, volume factors).
+struct Axes {
+    const float *x, *y, *z;      // 1-based through the accessors below
+    const float *mx, *my, *mz;
+    const float *dx, *dy, *dz;
+};
+
+__device__ __forceinline__ float AX(const float *a, int i) { return a[i - 1]; }
+
+// --------------------------------------------------------------- RNG
+// Philox4x32-10 (Salmon et al., SC'11).  Counter = (draw/4, packet lo,
+// packet hi, stream), key = seed; uniform = (word >> 8) * 2^-24 in [0,1).
+struct Rng {
+    uint32_t plo, phi, stream, k, blk;
+    uint32_t w0, w1, w2, w3;
+};
+
+__device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                       uint32_t k0, uint32_t k1, uint32_t &o0, uint32_t &o1,
+                                       uint32_t &o2, uint32_t &o3)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    }
+    o0 = c0; o1 = c1; o2 = c2; o3 = c3;
+}
+
+__device__ __forceinline__ void rng_start(Rng &R, uint64_t packet, uint32_t stream)
+{
+    R.plo = (uint32_t)packet; R.phi = (uint32_t)(packet >> 32);
+    R.stream = stream; R.k = 0; R.blk = 0xffffffffu;
+}
+
+__device__ __forceinline__ float draw(Rng &R, const Grid &G)
+{
+    uint32_t b = R.k >> 2;
+    if (b != R.blk) {
+        philox(b, R.plo, R.phi, R.stream, G.seed_lo, G.seed_hi, R.w0, R.w1, R.w2, R.w3);
+        R.blk = b;
+    }
+    uint32_t s = R.k & 3u;
+    uint32_t w = s == 0 ? R.w0 : (s == 1 ? R.w1 : (s == 2 ? R.w2 : R.w3));
+    R.k++;
+    return (float)(w >> 8) * 0x1p-24f;
+}
+
+// --------------------------------------------------------------- sampling
+// getNu2 (photon_mod.f90:720-764): k = number of leading CDF entries <= u,
+// nuP = max(k,1), then +1 unless nuP >= nbins-1.  The linear scan becomes a
+// binary search over the (validated monotone) CDF row; identical result.
+__device__ __forceinline__ int getnu2(Rng &R, const Grid &G, const float *cdf, int nbins, int &err)
+{
+    float u = draw(R, G);
+    int i;
+    for (i = 1; i <= 10000; ++i) {
+        if (u == 0.f || u == 1.f || u == 0.9999999f) u = draw(R, G);
+        else break;
+    }
+    if (i >= 10000) { err = 10; return -1; }
+    int lo = 0, hi = nbins;              // count of entries <= u in [lo, hi]
+    while (lo < hi) {
+        int m = (lo + hi) >> 1;
+        if (u >= cdf[m]) lo = m + 1; else hi = m;
+    }
+    int nuP = lo < 1 ? 1 : lo;
+    if (nuP < nbins - 1) nuP = nuP + 1;
+    return nuP;
+}
+
+// randomUnitVector (vector_mod.f90:303-314)
+__device__ __forceinline__ void random_unit_vector(Rng &R, const Grid &G, float &vx, float &vy, float &vz)
+{
+    float r1 = draw(R, G);
+    float w = 2.f * r1 - 1.f;
+    float t = sqrtf(1.f - w * w);
+    float r2 = draw(R, G);
+    float ang = 3.141592654f * (2.f * r2 - 1.f);
+    float s, c;
+    mcf_sincosf(ang, &s, &c);
+    vx = t * c; vy = t * s; vz = w;
+}
+
+// direction loop of initPhotonPacket (photon_mod.f90:650-675)
+__device__ __forceinline__ void new_direction(Rng &R, const Grid &G, bool fold,
+                                              float &vx, float &vy, float &vz)
+{
+    for (int irep = 1; irep <= 1000000; ++irep) {
+        random_unit_vector(R, G, vx, vy, vz);
+        if (vx != 0.f && vy != 0.f && vz != 0.f) break;
+    }
+    if (fold) {
+        if (vx < 0.f) vx = -vx;
+        if (vy < 0.f) vy = -vy;
+        if (vz < 0.f) vz = -vz;
+    }
+}
+
+// hg (photon_mod.f90:2875-2972); returns ierr, updates d* only on success
+__device__ __forceinline__ int hg(Rng &R, const Grid &G, int nuP, float &dx, float &dy, float &dz)
+{
+    float v0 = dx, v1 = dy, v2 = dz;
+    float hgg = G.gSca[nuP - 1];
+    float random0 = draw(R, G);
+    float s = 2.f * random0 - 1.f;
+    float cost, sint;
+    if (hgg >= 0.0001f) {
+        float q = (1.f - hgg * hgg) / (1.f + hgg * s);
+        cost = 0.5f / hgg * ((1.f + hgg * hgg) - q * q);
+    } else {
+        cost = s;
+    }
+    if (cost >= 1.0f) { cost = 1.0f; sint = 0.f; }
+    else if (cost < -1.0f) { cost = -1.0f; sint = 0.f; }
+    else sint = sqrtf(1.f - cost * cost);
+    float random = draw(R, G);
+    float phi = 6.28318548f * random;
+    float sinp, cosp;
+    mcf_sincosf(phi, &sinp, &cosp);
+    float denom = sqrtf(1.f - v2 * v2);
+    float o0, o1, o2;
+    if (denom > 0.001f) {
+        o0 = sint / denom * (v0 * v2 * cosp - v1 * sinp) + v0 * cost;
+        o1 = sint / denom * (v1 * v2 * cosp + v0 * sinp) + v1 * cost;
+        o2 = -(sint * cosp * denom) + v2 * cost;
+    } else {
+        o0 = sint * cosp;
+        o1 = sint * sinp;
+        o2 = (v2 >= 0.f) ? cost : -cost;
+    }
+    bool finite = !isnan(o0) && !isnan(o1) && !isnan(o2);
+    if (fabsf(o0) <= 1.f && fabsf(o1) <= 1.f && fabsf(o2) <= 1.f && finite) {
+        dx = o0; dy = o1; dz = o2;
+        return 0;
+    }
+    // out-of-range components: the reference normalises a copy it discards
+    if (fabsf(o0) >= 1.f || fabsf(o1) >= 1.f || (fabsf(o2) >= 1.f && finite)) return 0;
+    return 1;
+}
+
+// --------------------------------------------------------------- walk step
+enum StepResult { ST_CONTINUE = 0, ST_ESCAPED = 1, ST_LOST = 2, ST_ABSORBED = 3, ST_ERROR = 4 };
+
+struct Walker {
+    float rx, ry, rz;      // rVec
+    float vx, vy, vz;      // vHat
+    int xP, yP, zP;
+    float absTau, passProb;
+    // outputs of the last step
+    float seg;             // dS or dlLoc of the segment just walked
+    int segCell;           // active index of that segment
+};
+
+__device__ __forceinline__ int act(const Grid &G, int x, int y, int z)
+{
+    return G.active[(x - 1) + G.nx * ((y - 1) + G.ny * (z - 1))];
+}
+
+// One iteration of pathSegment's "do i" loop (photon_mod.f90:1194-2836) on
+// the mother grid.  ZERO_OPACITY replays geometry only.  kappa is looked up
+// here; on ST_ABSORBED the interaction point is in rVec and (xP,yP,zP).
+// err receives the fail-stop code of the reference's "insanity" stops.
+template <bool ZERO_OPACITY>
+__device__ __forceinline__ int walk_step(const Grid &G, const Axes &A, Walker &W, int nuP,
+                                         float &kappa_out, float &dV_out, int &err)
+{
+    const int nx = G.nx, ny = G.ny, nz = G.nz;
+    const bool sym = (G.flags & F_SYM) != 0;
+    float dSx, dSy, dSz, dS;
+    int xP = W.xP, yP = W.yP, zP = W.zP;
+    float rx = W.rx, ry = W.ry, rz = W.rz, vx = W.vx, vy = W.vy, vz = W.vz;
+
+    if (xP > nx || xP < 1 || yP > ny || yP < 1 || zP > nz || zP < 1) { err = 32; return ST_ERROR; }
+
+    if (sym) {                                                   // 1248-1261
+        if (rx <= AX(A.x, 1)) { if (vx < 0.f) vx = -vx; rx = AX(A.x, 1); }
+        if (ry <= AX(A.y, 1)) { if (vy < 0.f) vy = -vy; ry = AX(A.y, 1); }
+        if (rz <= AX(A.z, 1)) { if (vz < 0.f) vz = -vz; rz = AX(A.z, 1); }
+    }
+
+    // wall distances, 1263-1379
+    if (vx > 1.e-10f) {
+        if (xP < nx) {
+            float m = AX(A.mx, xP);
+            dSx = (m - rx) / vx;
+            if (fabsf(dSx) < 1.e-10f) { rx = m; xP = xP + 1; }
+        } else {
+            dSx = (AX(A.x, nx) - rx) / vx;
+            if (fabsf(dSx) < 1.e-10f) { W.rx = AX(A.x, nx); return ST_LOST; }
+        }
+    } else if (vx < -1.e-10f) {
+        if (xP > 1) {
+            float m = AX(A.mx, xP - 1);
+            dSx = (m - rx) / vx;
+            if (fabsf(dSx) < 1.e-10f) { rx = m; xP = xP - 1; }
+        } else {
+            dSx = (AX(A.x, 1) - rx) / vx;
+            if (fabsf(dSx) < 1.e-10f) rx = AX(A.x, 1);
+        }
+    } else {
+        dSx = 1.e35f;
+    }
+    if (isnan(dSx)) { err = 34; return ST_ERROR; }
+
+    if (vy > 1.e-10f) {
+        if (yP < ny) {
+            float m = AX(A.my, yP);
+            dSy = (m - ry) / vy;
+            if (fabsf(dSy) < 1.e-10f) { ry = m; yP = yP + 1; }
+        } else {
+            dSy = (AX(A.y, ny) - ry) / vy;
+            if (fabsf(dSy) < 1.e-10f) { W.ry = AX(A.y, ny); return ST_LOST; }
+        }
+    } else if (vy < -1.e-10f) {
+        if (yP > 1) {
+            float m = AX(A.my, yP - 1);
+            dSy = (m - ry) / vy;
+            if (fabsf(dSy) < 1.e-10f) { ry = m; yP = yP - 1; }
+        } else {
+            dSy = (AX(A.y, 1) - ry) / vy;
+            if (fabsf(dSy) < 1.e-10f) ry = AX(A.y, 1);
+        }
+    } else {
+        dSy = 1.e35f;
+    }
+    if (isnan(dSy)) { err = 34; return ST_ERROR; }
+
+    if (vz > 1.e-10f) {
+        if (zP < nz) {
+            float m = AX(A.mz, zP);
+            dSz = (m - rz) / vz;
+            if (fabsf(dSz) < 1.e-10f) { rz = m; zP = zP + 1; }
+        } else {
+            dSz = (AX(A.z, nz) - rz) / vz;
+            if (fabsf(dSz) < 1.e-10f) { W.rz = AX(A.z, nz); return ST_LOST; }
+        }
+    } else if (vz < -1.e-10f) {
+        if (zP > 1) {
+            float m = AX(A.mz, zP - 1);
+            dSz = (m - rz) / vz;
+            if (fabsf(dSz) < 1.e-10f) { rz = m; zP = zP - 1; }
+        } else {
+            dSz = (AX(A.z, 1) - rz) / vz;
+            if (fabsf(dSz) < 1.e-10f) rz = AX(A.z, 1);
+        }
+    } else {
+        dSz = 1.e35f;
+    }
+    if (isnan(dSz)) { err = 34; return ST_ERROR; }
+
+    // cells on the cell wall, 1382-1384
+    if (fabsf(dSx) < 1.e-10f) dSx = AX(A.x, nx);
+    if (fabsf(dSy) < 1.e-10f) dSy = AX(A.y, ny);
+    if (fabsf(dSz) < 1.e-10f) dSz = AX(A.z, nz);
+    dSx = fabsf(dSx); dSy = fabsf(dSy); dSz = fabsf(dSz);
+    if (dSx <= 0.f) dS = fminf(dSy, dSz);
+    else if (dSy <= 0.f) dS = fminf(dSx, dSz);
+    else if (dSz <= 0.f) dS = fminf(dSx, dSy);
+    else { dS = fminf(dSx, dSy); dS = fminf(dS, dSz); }
+    if (dS <= 0.f) { err = 36; return ST_ERROR; }
+
+    const int a = act(G, xP, yP, zP);
+    float kappa = 0.f;
+    if (!ZERO_OPACITY) kappa = G.opacity[(size_t)(nuP - 1) * (size_t)(G.nCells + 1) + a];
+    const float tauCell = dS * kappa;
+    const float dV = AX(A.dx, xP) * AX(A.dy, yP) * AX(A.dz, zP);  // 1442-1514
+    kappa_out = kappa;
+    dV_out = dV;
+    W.segCell = a;
+
+    if (!ZERO_OPACITY && (W.absTau + tauCell > W.passProb) && a > 0) {
+        // interaction, 1517-1580
+        float dlLoc = (W.passProb - W.absTau) / kappa;
+        rx = rx + dlLoc * vx;
+        ry = ry + dlLoc * vy;
+        rz = rz + dlLoc * vz;
+        if (isnan(rx) || isnan(ry) || isnan(rz)) { err = 37; return ST_ERROR; }
+        if (sym) {
+            if (rx <= AX(A.x, 1)) { if (vx < 0.f) vx = -vx; rx = AX(A.x, 1); }
+            if (ry <= AX(A.y, 1)) { if (vy < 0.f) vy = -vy; ry = AX(A.y, 1); }
+            if (rz <= AX(A.z, 1)) { if (vz < 0.f) vz = -vz; rz = AX(A.z, 1); }
+        }
+        W.seg = dlLoc;
+        W.rx = rx; W.ry = ry; W.rz = rz; W.vx = vx; W.vy = vy; W.vz = vz;
+        W.xP = xP; W.yP = yP; W.zP = zP;
+        return ST_ABSORBED;   // caller: J, R_out escape test, scatter/absorb decision
+    }
+
+    // free flight to the wall, 1818-1976
+    W.seg = dS;
+    W.absTau = W.absTau + tauCell;
+    rx = rx + dS * vx;
+    ry = ry + dS * vy;
+    rz = rz + dS * vz;
+    if (dS == dSx && vx > 0.f) xP = xP + 1;
+    else if (dS == dSx && vx < 0.f) xP = xP - 1;
+    else if (dS == dSy && vy > 0.f) yP = yP + 1;
+    else if (dS == dSy && vy < 0.f) yP = yP - 1;
+    else if (dS == dSz && vz > 0.f) zP = zP + 1;
+    else if (dS == dSz && vz < 0.f) zP = zP - 1;
+
+    int res = ST_CONTINUE;
+    if (!sym) {                                                  // 1986-2194
+        bool ret = false;
+        if (ry <= AX(A.y, 1) - G.geoCorrY || yP < 1) { yP = 1; ret = true; }
+        if (ry > AX(A.y, ny) + G.geoCorrY || yP > ny) { yP = ny; ret = true; }
+        if (rx <= AX(A.x, 1) - G.geoCorrX || xP < 1) { xP = 1; ret = true; }
+        if (rx >= AX(A.x, nx) + G.geoCorrX || xP > nx) { xP = nx; ret = true; }
+        if (rz <= AX(A.z, 1) - G.geoCorrZ || zP < 1) { zP = 1; ret = true; }
+        if (rz >= AX(A.z, nz) + G.geoCorrZ || zP > nz) { zP = nz; ret = true; }
+        if (ret) res = ST_ESCAPED;
+    }
+    if (res == ST_CONTINUE) {                                    // 2421-2554
+        if ((!sym && (rx <= AX(A.x, 1) - G.geoCorrX || ry <= AX(A.y, 1) - G.geoCorrY ||
+                      rz <= AX(A.z, 1) - G.geoCorrZ)) ||
+            rx >= AX(A.x, nx) + G.geoCorrX || ry >= AX(A.y, ny) + G.geoCorrY ||
+            rz >= AX(A.z, nz) + G.geoCorrZ || xP > nx || yP > ny || zP > nz)
+            res = ST_ESCAPED;
+    }
+    if (res == ST_CONTINUE && sym) {                             // 2674-2699
+        if (rx <= AX(A.x, 1) || xP < 1) { if (vx < 0.f) vx = -vx; xP = 1; rx = AX(A.x, 1); }
+        if (ry <= AX(A.y, 1) || yP < 1) { if (vy < 0.f) vy = -vy; yP = 1; ry = AX(A.y, 1); }
+        if (rz <= AX(A.z, 1) || zP < 1) { if (vz < 0.f) vz = -vz; zP = 1; rz = AX(A.z, 1); }
+    }
+    if (res == ST_CONTINUE && (xP > nx || yP > ny || zP > nz)) res = ST_ESCAPED;  // 2728
+    W.rx = rx; W.ry = ry; W.rz = rz; W.vx = vx; W.vy = vy; W.vz = vz;
+    W.xP = xP; W.yP = yP; W.zP = zP;
+    return res;
+}
+
+}  // namespace mcd

@@ -1,0 +1,137 @@
+"""Host-side mirror of MOCASSIN's transport step over the C ABI.
+
+`Transport` plays the role of energyPacketDriver (photon_mod.f90:26) for one
+grid: the host builds it from the same arrays the reference keeps in
+`grid_type` (common_mod.f90:241-302) and module globals, uploads the
+per-sweep tables, and calls `energy_packet_driver(istar, n)` once per source
+per sweep, exactly where iterateMC calls energyPacketDriver
+(iteration_mod.f90:474-496).  The estimator post-processing of iterateMC
+(x1e-9, /8 in symmetric mode; iteration_mod.f90:700-722) is `scale_J`.
+
+All arrays use the reference's Fortran layouts:
+    active(nx,ny,nz) int32 order='F'
+    opacity/recPDF/scaOpac (nCells+1, nbins) float32 order='F'
+    specCDF (nbins, nStars) float32 order='F'
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _capi
+
+
+def _f32(a):
+    return np.ascontiguousarray(np.asarray(a, np.float32).ravel(order="F"))
+
+
+class Transport:
+    """One device context holding a grid, its tables and FP64 estimators."""
+
+    def __init__(self, xAxis, yAxis, zAxis, active, nCells, nbins, nuArray,
+                 R_out, geoCorr, ionEdge1, starPos, starIdx, deltaE,
+                 symmetric=True, dust=False, gas=True, isotropic=False, multistars=False,
+                 seed=12345, device=0):
+        self._lib = _capi.load()
+        self.nCells, self.nbins = int(nCells), int(nbins)
+        active = np.asfortranarray(active, np.int32)
+        self.shape = active.shape
+        starPos = np.asarray(starPos, np.float32).reshape(-1, 3)
+        starIdx = np.asarray(starIdx, np.int32).reshape(-1, 3)
+        deltaE = np.asarray(deltaE, np.float32).reshape(-1)
+        self.nStars = starPos.shape[0]
+        # keep the host copies alive for the duration of the create call
+        self._keep = [_f32(xAxis), _f32(yAxis), _f32(zAxis), active.ravel(order="F").copy(),
+                      _f32(nuArray), _f32(starPos), np.ascontiguousarray(starIdx.ravel()),
+                      _f32(deltaE)]
+        k = self._keep
+        d = _capi.McDesc()
+        d.abi_version = _capi.MC_ABI_VERSION
+        d.nx, d.ny, d.nz = self.shape
+        d.xAxis, d.yAxis, d.zAxis = k[0].ctypes.data, k[1].ctypes.data, k[2].ctypes.data
+        d.active = k[3].ctypes.data
+        d.nCells, d.nbins = self.nCells, self.nbins
+        d.flags = ((_capi.MC_SYMMETRIC if symmetric else 0) | (_capi.MC_DUST if dust else 0) |
+                   (_capi.MC_GAS if gas else 0) | (_capi.MC_ISOTROPIC if isotropic else 0) |
+                   (_capi.MC_MULTISTARS if multistars else 0))
+        d.R_out = float(R_out)
+        d.geoCorr[:] = [float(g) for g in geoCorr]
+        d.ionEdge1 = float(ionEdge1)
+        d.nuArray = k[4].ctypes.data
+        d.nStars = self.nStars
+        d.starPos, d.starIdx, d.deltaE = k[5].ctypes.data, k[6].ctypes.data, k[7].ctypes.data
+        d.seed = int(seed)
+        d.device = int(device)
+        h = C.c_void_p()
+        rc = self._lib.mc_ctx_create(C.byref(d), C.byref(h))
+        if rc:
+            raise _capi.McError(rc, self._lib.mc_ctx_last_error(None).decode())
+        self._h = h
+        self._keep = None
+
+    def _check(self, rc):
+        if rc:
+            raise _capi.McError(rc, self._lib.mc_ctx_last_error(self._h).decode())
+
+    def upload_tables(self, opacity, specCDF, recPDF=None, totalLines=None,
+                      scaOpac=None, gSca=None):
+        arrs = [_f32(opacity), None if scaOpac is None else _f32(scaOpac),
+                None if recPDF is None else _f32(recPDF),
+                None if totalLines is None else _f32(totalLines), _f32(specCDF),
+                None if gSca is None else _f32(gSca)]
+        n = (self.nCells + 1) * self.nbins
+        for a, want in zip(arrs, (n, n, n, self.nCells + 1, self.nbins * self.nStars, self.nbins)):
+            if a is not None and a.size != want:
+                raise ValueError("table of size %d, expected %d" % (a.size, want))
+        ptr = [None if a is None else a.ctypes.data for a in arrs]
+        self._check(self._lib.mc_upload_tables(self._h, *ptr))
+
+    def zero_estimators(self):
+        self._check(self._lib.mc_zero_estimators(self._h))
+
+    def energy_packet_driver(self, istar, npackets, packet_offset=0):
+        """Transport npackets packets of source istar; returns the counters."""
+        c = _capi.McCounters()
+        self._check(self._lib.mc_transport_run(self._h, int(istar), int(npackets),
+                                               int(packet_offset), None, None, C.byref(c)))
+        return c.as_dict()
+
+    def fetch(self):
+        """(J (nCells+1,nbins), esc (nCells+1,nbins+1)) FP64, unscaled, order='F'."""
+        J = np.zeros((self.nCells + 1, self.nbins), np.float64, order="F")
+        E = np.zeros((self.nCells + 1, self.nbins + 1), np.float64, order="F")
+        self._check(self._lib.mc_fetch_estimators(self._h, J.ctypes.data, E.ctypes.data))
+        return J, E
+
+    def walk_replay(self, pos, dirv, cell, max_len=2048):
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        dirv = np.ascontiguousarray(dirv, np.float32).reshape(-1, 3)
+        cell = np.ascontiguousarray(cell, np.int32).reshape(-1, 3)
+        n = pos.shape[0]
+        seq = np.zeros((n, max_len, 3), np.int32)
+        ds = np.zeros((n, max_len), np.float32)
+        ln = np.zeros(n, np.int32)
+        st = np.zeros(n, np.int32)
+        self._check(self._lib.mc_walk_replay(self._h, n, pos.ctypes.data, dirv.ctypes.data,
+                                             cell.ctypes.data, seq.ctypes.data, ds.ctypes.data,
+                                             ln.ctypes.data, st.ctypes.data, int(max_len)))
+        return seq, ds, ln, st
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.mc_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def scale_J(J, symmetric):
+    """iterateMC's estimator post-processing: J *= 1e-9, then /8 in symmetric
+    mode (iteration_mod.f90:706, 717-722); J in units of erg s^-1 cm^-2."""
+    out = J * 1.e-9
+    if symmetric:
+        out = out / 8.
+    return out

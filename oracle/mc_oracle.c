@@ -1,0 +1,665 @@
+/*
+ * mc_oracle.c - CPU restatement of MOCASSIN 2.02 photon-packet transport.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP transport
+ * kernels: it is compiled into oracle/liboracle_mc.so and loaded by tests/,
+ * __graft_entry__.smoke() and the cpu_baseline leg of bench.py.  Nothing in
+ * the product links it.
+ *
+ * Every routine follows the reference statement by statement, in FP32 with
+ * the reference's expression order (build with -ffp-contract=off), so that
+ * packet histories are bit-identical to the reference when it is driven by
+ * the same uniform stream (parity pinned against golden fixtures written by
+ * the unmodified reference, see tests/golden/).
+ *
+ *   energyPacketDriver / energyPacketRun   photon_mod.f90:26-487
+ *   initPhotonPacket                       photon_mod.f90:491-684
+ *   getNu2                                 photon_mod.f90:720-764
+ *   newPhotonPacket                        photon_mod.f90:768-1059
+ *   pathSegment                            photon_mod.f90:1061-2872
+ *   hg                                     photon_mod.f90:2875-2972
+ *   randomUnitVector                       vector_mod.f90:303-314
+ *
+ * Restricted to one mother grid, no plane-parallel or 1-D mode and no
+ * viewing-angle SED bins (nAngleBins = 0), which is what the path supports.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+#include <stdlib.h>
+#include "mc_oracle.h"
+#include "../mocassin_amd/csrc/mc_fmath.h"
+
+/* ------------------------------------------------------------------ RNG */
+/* Philox4x32-10, Salmon, Moraes, Dror & Shaw, SC'11 ("Parallel random
+ * numbers: as easy as 1, 2, 3"), multipliers and Weyl constants as published. */
+void mco_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+    uint32_t k0 = key_in[0], k1 = key_in[1];
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+float mco_uniform(uint64_t seed, uint32_t stream, uint64_t packet, uint32_t k)
+{
+    uint32_t ctr[4] = { k >> 2, (uint32_t)packet, (uint32_t)(packet >> 32), stream };
+    uint32_t key[2] = { (uint32_t)seed, (uint32_t)(seed >> 32) };
+    uint32_t o[4];
+    mco_philox4x32_10(ctr, key, o);
+    return (float)(o[k & 3] >> 8) * 0x1p-24f;
+}
+
+typedef struct {
+    uint64_t seed, packet;
+    uint32_t stream, k;
+} rng_t;
+
+static float draw(rng_t *r)
+{
+    float u = mco_uniform(r->seed, r->stream, r->packet, r->k);
+    r->k++;
+    return u;
+}
+
+/* --------------------------------------------------------- elementary fns */
+float mco_logf(int mode, float x)
+{
+    return mode == MCO_MATH_OWN ? mcf_logf(x) : logf(x);
+}
+
+void mco_sincosf(int mode, float a, float *s, float *c)
+{
+    if (mode == MCO_MATH_OWN) mcf_sincosf(a, s, c);
+    else sincosf(a, s, c);
+}
+
+/* ------------------------------------------------------------- helpers */
+#define AX(a, i) ((a)[(i) - 1])      /* 1-based axis access */
+
+static inline int act(const mco_problem *P, int i, int j, int k)
+{
+    return P->active[(i - 1) + P->nx * ((j - 1) + P->ny * (k - 1))];
+}
+static inline float tab(const mco_problem *P, const float *t, int c, int nu)
+{
+    return t[c + (P->nCells + 1) * (nu - 1)];
+}
+
+typedef struct {
+    const mco_problem *P;
+    rng_t rng;
+    void *J, *esc;
+    mco_counters *cnt;
+} ctx_t;
+
+static void add_J(ctx_t *C, int cell, int nuP, float val)
+{
+    const mco_problem *P = C->P;
+    size_t idx = (size_t)cell + (size_t)(P->nCells + 1) * (size_t)(nuP - 1);
+    if (P->j_mode == MCO_J_FP32) ((float *)C->J)[idx] += val;
+    else ((double *)C->J)[idx] += (double)val;
+}
+
+static void add_esc(ctx_t *C, int origin, int nuP)
+{
+    const mco_problem *P = C->P;
+    size_t idx = (size_t)origin + (size_t)(P->nCells + 1) * (size_t)nuP;
+    if (P->j_mode == MCO_J_FP32) ((float *)C->esc)[idx] += P->deltaE;
+    else ((double *)C->esc)[idx] += (double)P->deltaE;
+    C->cnt->n_esc += 1.0;
+}
+
+/* randomUnitVector, vector_mod.f90:303-314 */
+static void random_unit_vector(ctx_t *C, float v[3])
+{
+    float r1 = draw(&C->rng);
+    float w = 2.f * r1 - 1.f;
+    float t = sqrtf(1.f - w * w);
+    float r2 = draw(&C->rng);
+    float ang = 3.141592654f * (2.f * r2 - 1.f);
+    float s, c;
+    mco_sincosf(C->P->math_mode, ang, &s, &c);
+    v[0] = t * c;
+    v[1] = t * s;
+    v[2] = w;
+}
+
+/* getNu2, photon_mod.f90:720-764, on an explicit uniform array */
+int mco_getnu2(const float *cdf, int nbins, const float *u, int nu_avail, int *used)
+{
+    int n = 0;
+    float random = u[n++];
+    int i;
+    for (i = 1; i <= 10000; ++i) {
+        if (random == 0.f || random == 1.f || random == 0.9999999f) {
+            if (n >= nu_avail) { *used = n; return -1; }
+            random = u[n++];
+        } else break;
+    }
+    *used = n;
+    if (i >= 10000) return -1;
+    int nuP = 1;
+    for (int is = 1; is <= nbins; ++is) {
+        if (random >= cdf[is - 1]) nuP = is;
+        else break;
+    }
+    if (nuP < nbins - 1) nuP = nuP + 1;
+    return nuP;
+}
+
+/* getNu2 drawing from the packet stream; cdf is a strided row */
+static int getnu2_stream(ctx_t *C, const float *cdf, size_t stride, int nbins)
+{
+    float random = draw(&C->rng);
+    int i;
+    for (i = 1; i <= 10000; ++i) {
+        if (random == 0.f || random == 1.f || random == 0.9999999f) random = draw(&C->rng);
+        else break;
+    }
+    if (i >= 10000) { C->cnt->error = 10; return -1; }
+    int nuP = 1;
+    for (int is = 1; is <= nbins; ++is) {
+        if (random >= cdf[(size_t)(is - 1) * stride]) nuP = is;
+        else break;
+    }
+    if (nuP < nbins - 1) nuP = nuP + 1;
+    return nuP;
+}
+
+/* packet state (photon_packet, common_mod.f90:305-321, single grid) */
+typedef struct {
+    int nuP;
+    int xP, yP, zP;
+    int origin;          /* origin(2): active index of the emitting cell */
+    float nu;
+    int lgStellar, lgLine;
+    float pos[3], dir[3];
+} pkt_t;
+
+/* initPhotonPacket, photon_mod.f90:491-684 (no plane-parallel branch) */
+static int init_photon_packet(ctx_t *C, pkt_t *pk, int nuP, const float pos[3],
+                              const float dir[3], int lgLine, int lgStellar,
+                              int xP, int yP, int zP, int lgHG)
+{
+    const mco_problem *P = C->P;
+    pk->dir[0] = dir[0]; pk->dir[1] = dir[1]; pk->dir[2] = dir[2];
+    pk->pos[0] = pos[0]; pk->pos[1] = pos[1]; pk->pos[2] = pos[2];
+    pk->nuP = nuP;
+    pk->lgStellar = lgStellar;
+    if (lgLine) { pk->nu = 0.f; pk->lgLine = 1; }
+    else { pk->nu = P->nuArray[nuP - 1]; pk->lgLine = 0; }
+    pk->xP = xP; pk->yP = yP; pk->zP = zP;
+    if (!lgHG || P->lgIsotropic || pk->lgStellar) {
+        int irepeat;
+        for (irepeat = 1; irepeat <= 1000000; ++irepeat) {
+            random_unit_vector(C, pk->dir);
+            if (isnan(pk->dir[0])) { C->cnt->error = 20; return -1; }
+            if (pk->dir[0] != 0.f && pk->dir[1] != 0.f && pk->dir[2] != 0.f) break;
+        }
+        if (P->lgSymmetricXYZ && pk->lgStellar && !P->lgMultiStars) {
+            if (pk->dir[0] < 0.f) pk->dir[0] = -pk->dir[0];
+            if (pk->dir[1] < 0.f) pk->dir[1] = -pk->dir[1];
+            if (pk->dir[2] < 0.f) pk->dir[2] = -pk->dir[2];
+        }
+    }
+    pk->origin = act(P, pk->xP, pk->yP, pk->zP);
+    return 0;
+}
+
+/* hg, photon_mod.f90:2875-2972; returns ierr */
+static int hg(ctx_t *C, pkt_t *pk)
+{
+    const mco_problem *P = C->P;
+    float vin[3] = { pk->dir[0], pk->dir[1], pk->dir[2] };
+    float vout[3];
+    float hgg = P->gSca[pk->nuP - 1];
+    float random0 = draw(&C->rng);
+    float s = 2.f * random0 - 1.f;
+    float cost, sint;
+    if (hgg >= 0.0001f) {
+        float q = (1.f - hgg * hgg) / (1.f + hgg * s);
+        cost = 0.5f / hgg * ((1.f + hgg * hgg) - q * q);
+    } else {
+        cost = s;
+    }
+    if (cost >= 1.0f) { cost = 1.0f; sint = 0.f; }
+    else if (cost < -1.0f) { cost = -1.0f; sint = 0.f; }
+    else sint = sqrtf(1.f - cost * cost);
+    float random = draw(&C->rng);
+    float phi = 6.28318548f * random;            /* twoPi = 2.*pi in REAL(4) */
+    float sinp, cosp;
+    mco_sincosf(P->math_mode, phi, &sinp, &cosp);
+    float denom = sqrtf(1.f - vin[2] * vin[2]);
+    if (denom > 0.001f) {
+        vout[0] = sint / denom * (vin[0] * vin[2] * cosp - vin[1] * sinp) + vin[0] * cost;
+        vout[1] = sint / denom * (vin[1] * vin[2] * cosp + vin[0] * sinp) + vin[1] * cost;
+        vout[2] = -(sint * cosp * denom) + vin[2] * cost;
+    } else {
+        vout[0] = sint * cosp;
+        vout[1] = sint * sinp;
+        vout[2] = (vin[2] >= 0.f) ? cost : -cost;
+    }
+    int finite = !isnan(vout[0]) && !isnan(vout[1]) && !isnan(vout[2]);
+    if (fabsf(vout[0]) <= 1.f && fabsf(vout[1]) <= 1.f && fabsf(vout[2]) <= 1.f && finite) {
+        pk->dir[0] = vout[0]; pk->dir[1] = vout[1]; pk->dir[2] = vout[2];
+        return 0;
+    }
+    /* The reference normalises vout in a local copy that is then dropped
+     * (photon_mod.f90:2947-2953): the direction is left unchanged, ierr=0. */
+    if (fabsf(vout[0]) >= 1.f || fabsf(vout[1]) >= 1.f ||
+        (fabsf(vout[2]) >= 1.f && finite))
+        return 0;
+    return 1;
+}
+
+/* Outcome of one pathSegment call */
+enum { PS_DONE = 0, PS_REEMIT = 1, PS_ERROR = -1 };
+
+/* pathSegment, photon_mod.f90:1061-2872 (mother grid only).
+ * On PS_REEMIT the absorption position and cell are left in *rpos/*cell.
+ * ws_cells/ws_ds (optional) record the geometry of every segment. */
+typedef struct { int32_t *cells; float *ds; int max, n, status; } seglog_t;
+
+static int path_segment(ctx_t *C, pkt_t *pk, float rpos[3], int cell[3], seglog_t *L)
+{
+    const mco_problem *P = C->P;
+    const int nx = P->nx, ny = P->ny, nz = P->nz;
+    const float *xA = P->xAxis, *yA = P->yAxis, *zA = P->zAxis;
+    float vHat[3], rVec[3];
+    float absTau, passProb, dSx = 0.f, dSy = 0.f, dSz = 0.f, dS, dV, tauCell;
+    int xP, yP, zP;
+    const int safeLimit = 500000;
+
+    if (pk->xP <= 0 || pk->xP > nx || pk->yP <= 0 || pk->yP > ny || pk->zP <= 0 || pk->zP > nz) {
+        C->cnt->error = 30; return PS_ERROR;
+    }
+    rVec[0] = pk->pos[0]; rVec[1] = pk->pos[1]; rVec[2] = pk->pos[2];
+    vHat[0] = pk->dir[0]; vHat[1] = pk->dir[1]; vHat[2] = pk->dir[2];
+    if (isnan(rVec[0]) || isnan(vHat[0])) { C->cnt->error = 31; return PS_ERROR; }
+    xP = pk->xP; yP = pk->yP; zP = pk->zP;
+
+    absTau = 0.f;
+    float random = draw(&C->rng);
+    passProb = -mco_logf(P->math_mode, 1.f - random);
+    C->cnt->n_flights += 1.0;
+
+    int i;
+    for (i = 1; i <= safeLimit; ++i) {
+        /* inner "do j" loop: a single pass on a mother grid (active >= 0) */
+        if (xP > nx || xP < 1 || yP > ny || yP < 1 || zP > nz || zP < 1) {
+            C->cnt->error = 32; return PS_ERROR;
+        }
+        if (act(P, xP, yP, zP) < 0) { C->cnt->error = 33; return PS_ERROR; } /* sub-grids */
+
+        if (P->lgSymmetricXYZ) {                                  /* 1248-1261 */
+            if (rVec[0] <= AX(xA, 1)) { if (vHat[0] < 0.f) vHat[0] = -vHat[0]; rVec[0] = AX(xA, 1); }
+            if (rVec[1] <= AX(yA, 1)) { if (vHat[1] < 0.f) vHat[1] = -vHat[1]; rVec[1] = AX(yA, 1); }
+            if (rVec[2] <= AX(zA, 1)) { if (vHat[2] < 0.f) vHat[2] = -vHat[2]; rVec[2] = AX(zA, 1); }
+        }
+
+        /* wall distances, 1263-1379 */
+        if (vHat[0] > 1.e-10f) {
+            if (xP < nx) {
+                dSx = ((AX(xA, xP + 1) + AX(xA, xP)) / 2.f - rVec[0]) / vHat[0];
+                if (fabsf(dSx) < 1.e-10f) { rVec[0] = (AX(xA, xP + 1) + AX(xA, xP)) / 2.f; xP = xP + 1; }
+            } else {
+                dSx = (AX(xA, nx) - rVec[0]) / vHat[0];
+                if (fabsf(dSx) < 1.e-10f) {
+                    rVec[0] = AX(xA, nx);
+                    C->cnt->n_lost += 1.0; if (L) L->status = 2; return PS_DONE;
+                }
+            }
+        } else if (vHat[0] < -1.e-10f) {
+            if (xP > 1) {
+                dSx = ((AX(xA, xP) + AX(xA, xP - 1)) / 2.f - rVec[0]) / vHat[0];
+                if (fabsf(dSx) < 1.e-10f) { rVec[0] = (AX(xA, xP) + AX(xA, xP - 1)) / 2.f; xP = xP - 1; }
+            } else {
+                dSx = (AX(xA, 1) - rVec[0]) / vHat[0];
+                if (fabsf(dSx) < 1.e-10f) rVec[0] = AX(xA, 1);
+            }
+        } else {
+            dSx = 1.e35f;
+        }
+        if (isnan(dSx)) { C->cnt->error = 34; return PS_ERROR; }
+
+        if (vHat[1] > 1.e-10f) {
+            if (yP < ny) {
+                dSy = ((AX(yA, yP + 1) + AX(yA, yP)) / 2.f - rVec[1]) / vHat[1];
+                if (fabsf(dSy) < 1.e-10f) { rVec[1] = (AX(yA, yP + 1) + AX(yA, yP)) / 2.f; yP = yP + 1; }
+            } else {
+                dSy = (AX(yA, ny) - rVec[1]) / vHat[1];
+                if (fabsf(dSy) < 1.e-10f) {
+                    rVec[1] = AX(yA, ny);
+                    C->cnt->n_lost += 1.0; if (L) L->status = 2; return PS_DONE;
+                }
+            }
+        } else if (vHat[1] < -1.e-10f) {
+            if (yP > 1) {
+                dSy = ((AX(yA, yP) + AX(yA, yP - 1)) / 2.f - rVec[1]) / vHat[1];
+                if (fabsf(dSy) < 1.e-10f) { rVec[1] = (AX(yA, yP) + AX(yA, yP - 1)) / 2.f; yP = yP - 1; }
+            } else {
+                dSy = (AX(yA, 1) - rVec[1]) / vHat[1];
+                if (fabsf(dSy) < 1.e-10f) rVec[1] = AX(yA, 1);
+            }
+        } else {
+            dSy = 1.e35f;
+        }
+        if (isnan(dSy)) { C->cnt->error = 34; return PS_ERROR; }
+
+        if (vHat[2] > 1.e-10f) {
+            if (zP < nz) {
+                dSz = ((AX(zA, zP + 1) + AX(zA, zP)) / 2.f - rVec[2]) / vHat[2];
+                if (fabsf(dSz) < 1.e-10f) { rVec[2] = (AX(zA, zP + 1) + AX(zA, zP)) / 2.f; zP = zP + 1; }
+            } else {
+                dSz = (AX(zA, nz) - rVec[2]) / vHat[2];
+                if (fabsf(dSz) < 1.e-10f) {
+                    rVec[2] = AX(zA, nz);
+                    C->cnt->n_lost += 1.0; if (L) L->status = 2; return PS_DONE;
+                }
+            }
+        } else if (vHat[2] < -1.e-10f) {
+            if (zP > 1) {
+                dSz = ((AX(zA, zP) + AX(zA, zP - 1)) / 2.f - rVec[2]) / vHat[2];
+                if (fabsf(dSz) < 1.e-10f) { rVec[2] = (AX(zA, zP) + AX(zA, zP - 1)) / 2.f; zP = zP - 1; }
+            } else {
+                dSz = (AX(zA, 1) - rVec[2]) / vHat[2];
+                if (fabsf(dSz) < 1.e-10f) rVec[2] = AX(zA, 1);
+            }
+        } else {
+            dSz = 1.e35f;
+        }
+        if (isnan(dSz)) { C->cnt->error = 34; return PS_ERROR; }
+        if (xP > nx || xP < 1 || yP > ny || yP < 1 || zP > nz || zP < 1) {
+            C->cnt->error = 35; return PS_ERROR;
+        }
+
+        /* cells on the cell wall, 1382-1384 */
+        if (fabsf(dSx) < 1.e-10f) dSx = AX(xA, nx);
+        if (fabsf(dSy) < 1.e-10f) dSy = AX(yA, ny);
+        if (fabsf(dSz) < 1.e-10f) dSz = AX(zA, nz);
+        dSx = fabsf(dSx); dSy = fabsf(dSy); dSz = fabsf(dSz);
+
+        if (dSx <= 0.f) dS = fminf(dSy, dSz);
+        else if (dSy <= 0.f) dS = fminf(dSx, dSz);
+        else if (dSz <= 0.f) dS = fminf(dSx, dSy);
+        else { dS = fminf(dSx, dSy); dS = fminf(dS, dSz); }
+        if (dS <= 0.f) { C->cnt->error = 36; return PS_ERROR; }
+
+        int a = act(P, xP, yP, zP);
+        float kappa = tab(P, P->opacity, a, pk->nuP);
+        tauCell = dS * kappa;
+
+        /* cell volume, 1442-1514 (3-D branch) */
+        float dx, dy, dz;
+        if (xP > 1 && xP < nx) dx = fabsf(AX(xA, xP + 1) - AX(xA, xP - 1)) / 2.f;
+        else if (xP == 1) dx = P->lgSymmetricXYZ ? fabsf(AX(xA, xP + 1) - AX(xA, xP)) / 2.f
+                                                 : fabsf(AX(xA, xP + 1) - AX(xA, xP));
+        else dx = fabsf(AX(xA, xP) - AX(xA, xP - 1));
+        if (yP > 1 && yP < ny) dy = fabsf(AX(yA, yP + 1) - AX(yA, yP - 1)) / 2.f;
+        else if (yP == 1) dy = P->lgSymmetricXYZ ? fabsf(AX(yA, yP + 1) - AX(yA, yP)) / 2.f
+                                                 : fabsf(AX(yA, yP + 1) - AX(yA, yP));
+        else dy = fabsf(AX(yA, yP) - AX(yA, yP - 1));
+        if (zP > 1 && zP < nz) dz = fabsf(AX(zA, zP + 1) - AX(zA, zP - 1)) / 2.f;
+        else if (zP == 1) dz = P->lgSymmetricXYZ ? fabsf(AX(zA, zP + 1) - AX(zA, zP)) / 2.f
+                                                 : fabsf(AX(zA, zP + 1) - AX(zA, zP));
+        else dz = fabsf(AX(zA, zP) - AX(zA, zP - 1));
+        dx = dx / 1.e15f; dy = dy / 1.e15f; dz = dz / 1.e15f;
+        dV = dx * dy * dz;
+
+        if ((absTau + tauCell > passProb) && a > 0) {
+            /* interaction, 1517-1816 */
+            float dlLoc = (passProb - absTau) / kappa;
+            rVec[0] = rVec[0] + dlLoc * vHat[0];
+            rVec[1] = rVec[1] + dlLoc * vHat[1];
+            rVec[2] = rVec[2] + dlLoc * vHat[2];
+            if (isnan(rVec[0]) || isnan(rVec[1]) || isnan(rVec[2])) { C->cnt->error = 37; return PS_ERROR; }
+            if (P->lgSymmetricXYZ) {
+                if (rVec[0] <= AX(xA, 1)) { if (vHat[0] < 0.f) vHat[0] = -vHat[0]; rVec[0] = AX(xA, 1); }
+                if (rVec[1] <= AX(yA, 1)) { if (vHat[1] < 0.f) vHat[1] = -vHat[1]; rVec[1] = AX(yA, 1); }
+                if (rVec[2] <= AX(zA, 1)) { if (vHat[2] < 0.f) vHat[2] = -vHat[2]; rVec[2] = AX(zA, 1); }
+            }
+            add_J(C, a, pk->nuP, dlLoc * P->deltaE / dV);
+            if (L && L->n < L->max) {
+                L->cells[3 * L->n] = xP; L->cells[3 * L->n + 1] = yP; L->cells[3 * L->n + 2] = zP;
+                L->ds[L->n] = dlLoc;
+            }
+            if (L) L->n++;
+            C->cnt->n_seg += 1.0;
+
+            float rx = rVec[0] / 1.e10f, ry = rVec[1] / 1.e10f, rz = rVec[2] / 1.e10f;
+            if (sqrtf(rx * rx + ry * ry + rz * rz) * 1.e10f >= P->R_out && P->R_out > 0.f) {
+                add_esc(C, pk->origin, pk->nuP);
+                if (L) L->status = 1;
+                return PS_DONE;
+            }
+            if (P->lgDust) {
+                float probSca = tab(P, P->scaOpac, a, pk->nuP) / kappa;
+                float u = 1.f - draw(&C->rng);
+                if (u > probSca) {
+                    C->cnt->n_abs += 1.0;
+                    break;                      /* "diffuse" (gas) or "dustEmi" */
+                } else if (u <= probSca) {
+                    C->cnt->n_sca += 1.0;
+                    float dirIn[3] = { pk->dir[0], pk->dir[1], pk->dir[2] };
+                    if (init_photon_packet(C, pk, pk->nuP, rVec, dirIn, 0, 0, xP, yP, zP, 1)) return PS_ERROR;
+                    if (!P->lgIsotropic && !pk->lgStellar) {
+                        int ihg;
+                        for (ihg = 1; ihg <= 10; ++ihg) if (hg(C, pk) == 0) break;
+                    }
+                    vHat[0] = pk->dir[0]; vHat[1] = pk->dir[1]; vHat[2] = pk->dir[2];
+                    if (isnan(pk->dir[0])) { C->cnt->error = 38; return PS_ERROR; }
+                    absTau = 0.f;
+                    float r2 = draw(&C->rng);
+                    passProb = -mco_logf(P->math_mode, 1.f - r2);
+                } else {
+                    C->cnt->error = 39; return PS_ERROR;
+                }
+            } else {
+                C->cnt->n_abs += 1.0;
+                if (!P->lgGas) { C->cnt->error = 40; return PS_ERROR; }
+                break;                          /* "diffuse" */
+            }
+        } else {
+            /* no interaction in this cell, 1818-1976 */
+            add_J(C, a, pk->nuP, dS * P->deltaE / dV);
+            if (L && L->n < L->max) {
+                L->cells[3 * L->n] = xP; L->cells[3 * L->n + 1] = yP; L->cells[3 * L->n + 2] = zP;
+                L->ds[L->n] = dS;
+            }
+            if (L) L->n++;
+            C->cnt->n_seg += 1.0;
+            absTau = absTau + tauCell;
+            rVec[0] = rVec[0] + dS * vHat[0];
+            rVec[1] = rVec[1] + dS * vHat[1];
+            rVec[2] = rVec[2] + dS * vHat[2];
+
+            if (dS == dSx && vHat[0] > 0.f) xP = xP + 1;
+            else if (dS == dSx && vHat[0] < 0.f) xP = xP - 1;
+            else if (dS == dSy && vHat[1] > 0.f) yP = yP + 1;
+            else if (dS == dSy && vHat[1] < 0.f) yP = yP - 1;
+            else if (dS == dSz && vHat[2] > 0.f) zP = zP + 1;
+            else if (dS == dSz && vHat[2] < 0.f) zP = zP - 1;
+
+            if (!P->lgSymmetricXYZ) {                    /* 1986-2194 */
+                int ret = 0;
+                if (rVec[1] <= AX(yA, 1) - P->geoCorrY || yP < 1) { yP = 1; ret = 1; }
+                if (rVec[1] > AX(yA, ny) + P->geoCorrY || yP > ny) { yP = ny; ret = 1; }
+                if (rVec[0] <= AX(xA, 1) - P->geoCorrX || xP < 1) { xP = 1; ret = 1; }
+                if (rVec[0] >= AX(xA, nx) + P->geoCorrX || xP > nx) { xP = nx; ret = 1; }
+                if (rVec[2] <= AX(zA, 1) - P->geoCorrZ || zP < 1) { zP = 1; ret = 1; }
+                if (rVec[2] >= AX(zA, nz) + P->geoCorrZ || zP > nz) { zP = nz; ret = 1; }
+                if (ret) { add_esc(C, pk->origin, pk->nuP); if (L) L->status = 1; return PS_DONE; }
+            }
+            /* escape through the grid boundary, 2421-2554 */
+            if ((!P->lgSymmetricXYZ && (rVec[0] <= AX(xA, 1) - P->geoCorrX ||
+                                        rVec[1] <= AX(yA, 1) - P->geoCorrY ||
+                                        rVec[2] <= AX(zA, 1) - P->geoCorrZ)) ||
+                rVec[0] >= AX(xA, nx) + P->geoCorrX ||
+                rVec[1] >= AX(yA, ny) + P->geoCorrY ||
+                rVec[2] >= AX(zA, nz) + P->geoCorrZ ||
+                xP > nx || yP > ny || zP > nz) {
+                add_esc(C, pk->origin, pk->nuP);
+                if (L) L->status = 1;
+                return PS_DONE;
+            }
+            if (P->lgSymmetricXYZ) {                     /* reflections, 2674-2699 */
+                if (rVec[0] <= AX(xA, 1) || xP < 1) { if (vHat[0] < 0.f) vHat[0] = -vHat[0]; xP = 1; rVec[0] = AX(xA, 1); }
+                if (rVec[1] <= AX(yA, 1) || yP < 1) { if (vHat[1] < 0.f) vHat[1] = -vHat[1]; yP = 1; rVec[1] = AX(yA, 1); }
+                if (rVec[2] <= AX(zA, 1) || zP < 1) { if (vHat[2] < 0.f) vHat[2] = -vHat[2]; zP = 1; rVec[2] = AX(zA, 1); }
+            }
+        }
+        if (xP > nx || yP > ny || zP > nz) {             /* 2728-2832 */
+            add_esc(C, pk->origin, pk->nuP);
+            if (L) L->status = 1;
+            return PS_DONE;
+        }
+    }
+    if (i >= safeLimit) {                                 /* 2838-2846 */
+        C->cnt->n_lost += 1.0;
+        if (L) L->status = 3;
+        return PS_DONE;
+    }
+    rpos[0] = rVec[0]; rpos[1] = rVec[1]; rpos[2] = rVec[2];
+    cell[0] = xP; cell[1] = yP; cell[2] = zP;
+    return PS_REEMIT;
+}
+
+/* energyPacketRun, photon_mod.f90:289-487 (+ newPhotonPacket 768-1059).
+ * stellar != 0: create at the source; else "diffuse" at pos/cell. */
+static int packet_run(ctx_t *C, int stellar, float pos[3], int cell[3])
+{
+    const mco_problem *P = C->P;
+    pkt_t pk;
+    const float zero3[3] = { 0.f, 0.f, 0.f };
+    if (stellar) {
+        int nuP = getnu2_stream(C, P->specCDF, 1, P->nbins);
+        if (nuP < 1 || nuP >= P->nbins) { C->cnt->error = 11; return PS_ERROR; }
+        if (act(P, P->starIdx[0], P->starIdx[1], P->starIdx[2]) < 0) { C->cnt->error = 12; return PS_ERROR; }
+        if (init_photon_packet(C, &pk, nuP, P->starPos, zero3, 0, 1,
+                               P->starIdx[0], P->starIdx[1], P->starIdx[2], 0)) return PS_ERROR;
+        if (pk.nu > 1.f) {
+            if (P->j_mode == MCO_J_FP32)
+                C->cnt->qphot = (float)((float)C->cnt->qphot + P->deltaE / (2.1799153e-11f * pk.nu));
+            else
+                C->cnt->qphot += (double)(P->deltaE / (2.1799153e-11f * pk.nu));
+        }
+    } else {
+        if (!P->lgGas) { C->cnt->error = 13; return PS_ERROR; }
+        int a = act(P, cell[0], cell[1], cell[2]);
+        if (a <= 0) { C->cnt->error = 14; return PS_ERROR; }
+        float random = 1.f - draw(&C->rng);
+        if (random <= P->totalLines[a]) {
+            if (init_photon_packet(C, &pk, 0, pos, zero3, 1, 0, cell[0], cell[1], cell[2], 0)) return PS_ERROR;
+        } else {
+            int nuP = getnu2_stream(C, P->recPDF + a, (size_t)(P->nCells + 1), P->nbins);
+            if (nuP < 1 || nuP >= P->nbins) { C->cnt->error = 15; return PS_ERROR; }
+            if (init_photon_packet(C, &pk, nuP, pos, zero3, 0, 0, cell[0], cell[1], cell[2], 0)) return PS_ERROR;
+        }
+    }
+    if (!P->lgDust && pk.nu < P->ionEdge1 && !pk.lgLine) {
+        if (pk.origin < 0) { C->cnt->error = 16; return PS_ERROR; }
+        add_esc(C, pk.origin, pk.nuP);
+        return PS_DONE;
+    }
+    if (pk.lgLine) { C->cnt->n_line += 1.0; return PS_DONE; }
+    return path_segment(C, &pk, pos, cell, NULL);
+}
+
+int mco_transport(const mco_problem *P, uint64_t first, uint64_t n,
+                  void *J, void *esc, mco_counters *cnt)
+{
+    ctx_t C;
+    C.P = P; C.J = J; C.esc = esc; C.cnt = cnt;
+    for (uint64_t p = first; p < first + n; ++p) {
+        C.rng.seed = P->seed; C.rng.stream = P->stream; C.rng.packet = p; C.rng.k = 0;
+        float pos[3] = { P->starPos[0], P->starPos[1], P->starPos[2] };
+        int cell[3] = { P->starIdx[0], P->starIdx[1], P->starIdx[2] };
+        int stellar = 1;
+        int i, rr = PS_DONE;
+        for (i = 1; i <= 5000; ++i) {                 /* recursionLimit */
+            rr = packet_run(&C, stellar, pos, cell);
+            if (rr == PS_ERROR) return cnt->error;
+            if (rr == PS_DONE) break;
+            stellar = 0;
+        }
+        if (i >= 5000) cnt->n_trapped += 1.0;
+    }
+    return cnt->error;
+}
+
+int mco_walk(const mco_problem *P, const float pos[3], const float dir[3],
+             const int cell[3], int32_t *cells_xyz, float *ds, int max_len, int *status)
+{
+    /* zero-opacity replay: a problem copy whose opacity lookups are all 0 */
+    mco_problem Q = *P;
+    size_t ntab = (size_t)(P->nCells + 1) * (size_t)P->nbins;
+    float *zeros = (float *)calloc(ntab, sizeof(float));
+    if (!zeros) { *status = -2; return 0; }
+    Q.opacity = zeros;
+    Q.j_mode = MCO_J_FP64;
+    double *Jd = (double *)calloc(ntab, sizeof(double));
+    double *Ed = (double *)calloc((size_t)(P->nCells + 1) * (size_t)(P->nbins + 1), sizeof(double));
+    mco_counters cnt;
+    memset(&cnt, 0, sizeof cnt);
+    ctx_t C;
+    C.P = &Q; C.J = Jd; C.esc = Ed; C.cnt = &cnt;
+    C.rng.seed = 0; C.rng.stream = 0; C.rng.packet = 0; C.rng.k = 0;
+    pkt_t pk;
+    memset(&pk, 0, sizeof pk);
+    pk.nuP = 1;
+    pk.xP = cell[0]; pk.yP = cell[1]; pk.zP = cell[2];
+    pk.pos[0] = pos[0]; pk.pos[1] = pos[1]; pk.pos[2] = pos[2];
+    pk.dir[0] = dir[0]; pk.dir[1] = dir[1]; pk.dir[2] = dir[2];
+    pk.origin = act(P, cell[0], cell[1], cell[2]);
+    seglog_t L = { cells_xyz, ds, max_len, 0, 0 };
+    float rp[3]; int rc[3];
+    int r = path_segment(&C, &pk, rp, rc, &L);
+    *status = (r == PS_ERROR) ? -cnt.error : L.status;
+    free(zeros); free(Jd); free(Ed);
+    return L.n;
+}
+
+/* Exhaustive comparison of the deterministic elementary functions with the
+ * C library over every argument the transport can produce from a 24-bit
+ * uniform u: log(1-u), sincos(pi_f*(2u-1)) and sincos(2pi_f*u).
+ * out[0..2] = mismatching counts, out[3..5] = max |ulp| differences. */
+void mco_math_mismatch(int64_t out[6])
+{
+    for (int i = 0; i < 6; ++i) out[i] = 0;
+    for (uint32_t k = 0; k < (1u << 24); ++k) {
+        float u = (float)k * 0x1p-24f;
+        float x = 1.f - u;
+        float a = logf(x), b = mcf_logf(x);
+        int32_t ia, ib;
+        memcpy(&ia, &a, 4); memcpy(&ib, &b, 4);
+        if (ia != ib) { out[0]++; int64_t d = llabs((int64_t)ia - ib); if (d > out[3]) out[3] = d; }
+        float ang = 3.141592654f * (2.f * u - 1.f);
+        float s1, c1, s2, c2;
+        sincosf(ang, &s1, &c1); mcf_sincosf(ang, &s2, &c2);
+        int32_t q[4];
+        memcpy(&q[0], &s1, 4); memcpy(&q[1], &s2, 4); memcpy(&q[2], &c1, 4); memcpy(&q[3], &c2, 4);
+        if (q[0] != q[1] || q[2] != q[3]) {
+            out[1]++;
+            int64_t d = llabs((int64_t)q[0] - q[1]); if (llabs((int64_t)q[2] - q[3]) > d) d = llabs((int64_t)q[2] - q[3]);
+            if (d > out[4]) out[4] = d;
+        }
+        float phi = 6.28318548f * u;
+        sincosf(phi, &s1, &c1); mcf_sincosf(phi, &s2, &c2);
+        memcpy(&q[0], &s1, 4); memcpy(&q[1], &s2, 4); memcpy(&q[2], &c1, 4); memcpy(&q[3], &c2, 4);
+        if (q[0] != q[1] || q[2] != q[3]) {
+            out[2]++;
+            int64_t d = llabs((int64_t)q[0] - q[1]); if (llabs((int64_t)q[2] - q[3]) > d) d = llabs((int64_t)q[2] - q[3]);
+            if (d > out[5]) out[5] = d;
+        }
+    }
+}

@@ -1,0 +1,203 @@
+"""ctypes front end of the C restatement in oracle/mc_oracle.c.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py.  The product never imports this module.
+
+`Problem` holds one transport problem in the reference's Fortran layouts:
+    active(nx,ny,nz)            int32, order='F'
+    opacity/recPDF/scaOpac      float32 (nCells+1, nbins), order='F'
+so arrays dumped by the reference harness load without reshuffling.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+MATH_LIBM, MATH_OWN = 0, 1
+J_FP32, J_FP64 = 0, 1
+
+
+class _Problem(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("xAxis", C.c_void_p), ("yAxis", C.c_void_p), ("zAxis", C.c_void_p),
+        ("active", C.c_void_p),
+        ("nCells", C.c_int), ("nbins", C.c_int),
+        ("opacity", C.c_void_p), ("scaOpac", C.c_void_p), ("recPDF", C.c_void_p),
+        ("totalLines", C.c_void_p), ("specCDF", C.c_void_p), ("nuArray", C.c_void_p),
+        ("gSca", C.c_void_p),
+        ("R_out", C.c_float), ("geoCorrX", C.c_float), ("geoCorrY", C.c_float),
+        ("geoCorrZ", C.c_float), ("ionEdge1", C.c_float), ("deltaE", C.c_float),
+        ("starPos", C.c_float * 3), ("starIdx", C.c_int * 3),
+        ("lgSymmetricXYZ", C.c_int), ("lgDust", C.c_int), ("lgGas", C.c_int),
+        ("lgIsotropic", C.c_int), ("lgMultiStars", C.c_int),
+        ("math_mode", C.c_int), ("j_mode", C.c_int),
+        ("seed", C.c_uint64), ("stream", C.c_uint32),
+    ]
+
+
+class _Counters(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "n_seg", "n_abs", "n_sca", "n_esc", "n_lost", "n_line", "n_trapped",
+        "n_flights", "qphot")] + [("error", C.c_int)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle_mc.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run `make -C oracle` (or __graft_entry__.build())")
+        L = C.CDLL(path)
+        L.mco_philox4x32_10.argtypes = [C.POINTER(C.c_uint32)] * 2 + [C.POINTER(C.c_uint32)]
+        L.mco_uniform.restype = C.c_float
+        L.mco_uniform.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32]
+        L.mco_getnu2.restype = C.c_int
+        L.mco_getnu2.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+        L.mco_logf.restype = C.c_float
+        L.mco_logf.argtypes = [C.c_int, C.c_float]
+        L.mco_sincosf.argtypes = [C.c_int, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.mco_walk.restype = C.c_int
+        L.mco_walk.argtypes = [C.POINTER(_Problem), C.c_void_p, C.c_void_p, C.c_void_p,
+                               C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+        L.mco_transport.restype = C.c_int
+        L.mco_transport.argtypes = [C.POINTER(_Problem), C.c_uint64, C.c_uint64,
+                                    C.c_void_p, C.c_void_p, C.POINTER(_Counters)]
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def philox4x32_10(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().mco_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def uniform(seed, stream, packet, k):
+    return lib().mco_uniform(seed, stream, packet, k)
+
+
+def logf(x, mode=MATH_OWN):
+    return lib().mco_logf(mode, x)
+
+
+def sincosf(a, mode=MATH_OWN):
+    s, c = C.c_float(), C.c_float()
+    lib().mco_sincosf(mode, a, C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+def getnu2(cdf, u):
+    cdf = np.ascontiguousarray(cdf, np.float32)
+    u = np.ascontiguousarray(u, np.float32)
+    used = C.c_int()
+    r = lib().mco_getnu2(cdf.ctypes.data, int(cdf.size), u.ctypes.data, int(u.size), C.byref(used))
+    return r, used.value
+
+
+class Problem:
+    """One transport problem (single grid, one source) in Fortran layouts."""
+
+    def __init__(self, xAxis, yAxis, zAxis, active, opacity, nuArray, specCDF,
+                 R_out, geoCorr, ionEdge1, deltaE, starPos, starIdx,
+                 recPDF=None, totalLines=None, scaOpac=None, gSca=None,
+                 symmetric=True, dust=False, gas=True, isotropic=False, multistars=False):
+        f32 = lambda a: None if a is None else np.asarray(a, np.float32)
+        self.xAxis, self.yAxis, self.zAxis = f32(xAxis), f32(yAxis), f32(zAxis)
+        self.active = np.asfortranarray(active, np.int32)
+        self.nx, self.ny, self.nz = self.active.shape
+        self.opacity = np.asfortranarray(opacity, np.float32)
+        self.nCells = self.opacity.shape[0] - 1
+        self.nbins = self.opacity.shape[1]
+        self.nuArray, self.specCDF = f32(nuArray), f32(specCDF)
+        self.recPDF = None if recPDF is None else np.asfortranarray(recPDF, np.float32)
+        self.totalLines = f32(totalLines)
+        self.scaOpac = None if scaOpac is None else np.asfortranarray(scaOpac, np.float32)
+        self.gSca = f32(gSca)
+        self.R_out = float(R_out)
+        self.geoCorr = tuple(float(g) for g in geoCorr)
+        self.ionEdge1 = float(ionEdge1)
+        self.deltaE = float(deltaE)
+        self.starPos = tuple(float(v) for v in starPos)
+        self.starIdx = tuple(int(v) for v in starIdx)
+        self.symmetric, self.dust, self.gas = bool(symmetric), bool(dust), bool(gas)
+        self.isotropic, self.multistars = bool(isotropic), bool(multistars)
+
+    def _struct(self, math_mode, j_mode, seed, stream):
+        s = _Problem()
+        s.nx, s.ny, s.nz = self.nx, self.ny, self.nz
+        s.xAxis, s.yAxis, s.zAxis = _ptr(self.xAxis), _ptr(self.yAxis), _ptr(self.zAxis)
+        s.active = _ptr(self.active)
+        s.nCells, s.nbins = self.nCells, self.nbins
+        s.opacity, s.scaOpac, s.recPDF = _ptr(self.opacity), _ptr(self.scaOpac), _ptr(self.recPDF)
+        s.totalLines, s.specCDF, s.nuArray = _ptr(self.totalLines), _ptr(self.specCDF), _ptr(self.nuArray)
+        s.gSca = _ptr(self.gSca)
+        s.R_out = self.R_out
+        s.geoCorrX, s.geoCorrY, s.geoCorrZ = self.geoCorr
+        s.ionEdge1, s.deltaE = self.ionEdge1, self.deltaE
+        s.starPos[:] = self.starPos
+        s.starIdx[:] = self.starIdx
+        s.lgSymmetricXYZ, s.lgDust, s.lgGas = int(self.symmetric), int(self.dust), int(self.gas)
+        s.lgIsotropic, s.lgMultiStars = int(self.isotropic), int(self.multistars)
+        s.math_mode, s.j_mode = math_mode, j_mode
+        s.seed, s.stream = seed, stream
+        return s
+
+    def transport(self, first, n, seed, stream=1, math_mode=MATH_OWN, j_mode=J_FP64):
+        """Returns (J, esc, counters dict); J (nCells+1, nbins), esc (nCells+1, nbins+1)."""
+        dt = np.float32 if j_mode == J_FP32 else np.float64
+        J = np.zeros((self.nCells + 1, self.nbins), dt, order="F")
+        E = np.zeros((self.nCells + 1, self.nbins + 1), dt, order="F")
+        cnt = _Counters()
+        s = self._struct(math_mode, j_mode, seed, stream)
+        rc = lib().mco_transport(C.byref(s), first, n, J.ctypes.data, E.ctypes.data, C.byref(cnt))
+        if rc:
+            raise RuntimeError("oracle transport stopped with code %d" % rc)
+        return J, E, {f: getattr(cnt, f) for f, _ in _Counters._fields_}
+
+    def walk(self, pos, dirv, cell, max_len=4096):
+        """Zero-opacity geometric walk; returns (cells (n,3) 1-based, ds (n,), status)."""
+        pos = np.ascontiguousarray(pos, np.float32)
+        dirv = np.ascontiguousarray(dirv, np.float32)
+        cell = np.ascontiguousarray(cell, np.int32)
+        cells = np.zeros((max_len, 3), np.int32)
+        ds = np.zeros(max_len, np.float32)
+        st = C.c_int()
+        s = self._struct(MATH_OWN, J_FP64, 0, 0)
+        n = lib().mco_walk(C.byref(s), pos.ctypes.data, dirv.ctypes.data, cell.ctypes.data,
+                           cells.ctypes.data, ds.ctypes.data, max_len, C.byref(st))
+        return cells[:min(n, max_len)].copy(), ds[:min(n, max_len)].copy(), st.value
+
+
+def load_harness_dump(d, **flag_overrides):
+    """Problem from a reference-harness output directory (oracle/ref)."""
+    rd = lambda n, dt=np.float32: np.fromfile(os.path.join(d, n + ".bin"), dt)
+    ints = rd("ints", np.int32)
+    nx, ny, nz, nc, nb = (int(v) for v in ints[:5])
+    si = int(ints[5])
+    star_idx = (si // 100000000, (si // 10000) % 10000, si % 10000)
+    flags = int(ints[6])
+    sc = rd("scalars")
+    has = lambda n: os.path.exists(os.path.join(d, n + ".bin"))
+    tab = lambda n: rd(n).reshape(nb, nc + 1).T if has(n) else None
+    kw = dict(
+        xAxis=rd("xAxis"), yAxis=rd("yAxis"), zAxis=rd("zAxis"),
+        active=rd("active", np.int32).reshape(nz, ny, nx).transpose(2, 1, 0),
+        opacity=tab("opacity"), nuArray=rd("nuArray"), specCDF=rd("specCDF"),
+        R_out=sc[0], geoCorr=sc[1:4], ionEdge1=sc[4], deltaE=sc[5], starPos=sc[6:9],
+        starIdx=star_idx, recPDF=tab("recPDF"),
+        totalLines=rd("totalLines") if has("totalLines") else None,
+        scaOpac=tab("scaOpac"), gSca=rd("gSca") if has("gSca") else None,
+        symmetric=bool(flags & 1), dust=bool(flags & 2), gas=bool(flags & 4),
+        isotropic=bool(flags & 8), multistars=bool(flags & 16))
+    kw.update(flag_overrides)
+    return Problem(**kw)

@@ -1,0 +1,379 @@
+! mcref_harness.f90 - drives the UNMODIFIED reference (MOCASSIN 2.02 modules,
+! compiled from /root/reference/source by oracle/Makefile) to write golden
+! vectors for the transport and balance path.  Runs only in the build
+! container; its outputs are turned into small committed fixtures by
+! tests/golden/make_fixtures.py.
+!
+! The setup replays mocassin.f90:66-104 (readInput .. dustDriver) and the
+! pre-transport part of iterateMC (iteration_mod.f90:81-400: opacity and
+! re-emission tables); transport calls the public energyPacketDriver one
+! packet at a time with the random stream pinned per packet by
+! mcref_set_stream (oracle/ref/mcref_rng.c); balance calls the public
+! updateCell for every cell in the reference's own loop order
+! (iteration_mod.f90:826-837).
+!
+! usage (cwd holds input/input.in):
+!   mcref_harness tables    OUTDIR
+!   mcref_harness transport OUTDIR SEED NPACKETS
+!   mcref_harness walk      OUTDIR SEED NPACKETS NUBIN ORIGINFILE
+!   mcref_harness balance   OUTDIR SEED NPACKETS
+program mcref_harness
+    use common_mod
+    use constants_mod
+    use vector_mod
+    use interpolation_mod
+    use set_input_mod
+    use elements_mod
+    use xSec_mod
+    use continuum_mod
+    use ionization_mod
+    use grid_mod
+    use dust_mod
+    use emission_mod
+    use photon_mod
+    use update_mod
+    use readdata_mod
+    use iso_c_binding
+    implicit none
+    include 'mpif.h'
+
+    interface
+       subroutine mcref_set_stream(seed, stream, packet) bind(C, name="mcref_set_stream")
+         import :: c_int64_t, c_int32_t
+         integer(c_int64_t), intent(in) :: seed
+         integer(c_int32_t), intent(in) :: stream
+         integer(c_int64_t), intent(in) :: packet
+       end subroutine mcref_set_stream
+    end interface
+
+    type(grid_type) :: grid3D(maxGrids)
+    character(len=512) :: mode, outdir, arg, originFile
+    integer(c_int64_t) :: seed, p, npk
+    integer(c_int32_t) :: stream
+    integer :: iGrid, i, j, k, ic, nu, nOrig, io, nuBin, ios
+    real :: orig(3), junk, w(3)
+    integer :: fold
+    type(vector) :: dirv
+
+    call get_command_argument(1, mode)
+    call get_command_argument(2, outdir)
+    seed = 0; npk = 0
+    if (command_argument_count() >= 4) then
+       call get_command_argument(3, arg); read(arg, *) seed
+       call get_command_argument(4, arg); read(arg, *) npk
+    end if
+
+    call mpi_init(ierr)
+    call mpi_comm_rank(MPI_COMM_WORLD, taskid, ierr)
+    call mpi_comm_size(MPI_COMM_WORLD, numtasks, ierr)
+
+    ! --- mocassin.f90:66-104
+    call readInput()
+    do iGrid = 1, nGrids
+       call initCartesianGrid(grid3D(iGrid), nxIn(iGrid), nyIn(iGrid), nzIn(iGrid))
+    end do
+    call initXSecArray()
+    call setContinuum()
+    call fillGrid(grid3D(1:nGrids))
+    call setStarPosition(grid3D(1)%xAxis, grid3D(1)%yAxis, grid3D(1)%zAxis, grid3D(1:nGrids))
+    if (lgGas) then
+       call makeElements()
+       call readData()
+    end if
+    if (lgDust) then
+       do iGrid = 1, nGrids
+          call dustDriver(grid3D(iGrid))
+       end do
+    end if
+
+    call build_tables()
+    ! optional uniform opacity scaling (stress fixtures with many absorptions)
+    call get_environment_variable("MCREF_OPACITY_SCALE", arg, status=ios)
+    if (ios == 0) then
+       read(arg, *) junk
+       grid3D(1)%opacity = grid3D(1)%opacity * junk
+    end if
+    call dump_problem()
+
+    select case (trim(mode))
+    case ("tables")
+    case ("transport")
+       call run_transport()
+    case ("balance")
+       call run_transport()
+       call run_balance()
+    case ("walk")
+       call get_command_argument(5, arg); read(arg, *) nuBin
+       call get_command_argument(6, originFile)
+       call run_walk()
+    case default
+       print*, "mcref_harness: unknown mode ", trim(mode)
+       stop 2
+    end select
+
+    call mpi_finalize(ierr)
+
+contains
+
+    ! iteration_mod.f90:81-400 for one grid and one rank
+    subroutine build_tables()
+       integer :: iG
+       iG = 1
+       grid3D(iG)%lgConverged(0:grid3D(iG)%nCells) = 0
+       grid3D(iG)%lgBlack(0:grid3D(iG)%nCells) = 0
+       if (lgGas) then
+          grid3D(iG)%recPDF(0:grid3D(iG)%nCells, 1:nbins) = 0.
+          grid3D(iG)%totalLines(0:grid3D(iG)%nCells) = 0.
+          BjumpTemp = 0.
+          Bjump = 0.
+       end if
+       grid3D(iG)%opacity(0:grid3D(iG)%nCells, 1:nbins) = 0.
+       if (lgGas) then
+          do i = 1, grid3D(iG)%nx
+             do j = 1, grid3D(iG)%ny
+                do k = 1, grid3D(iG)%nz
+                   call ionizationDriver(grid3D(iG), i, j, k)
+                end do
+             end do
+          end do
+       end if
+       if (lgDust) then
+          grid3D(iG)%scaOpac(0:grid3D(iG)%nCells, 1:nbins) = 0.
+          grid3D(iG)%absOpac(0:grid3D(iG)%nCells, 1:nbins) = 0.
+          do i = 1, grid3D(iG)%nx
+             do j = 1, grid3D(iG)%ny
+                do k = 1, grid3D(iG)%nz
+                   ic = grid3D(iG)%active(i,j,k)
+                   if (ic > 0) then
+                      call add_dust(iG, ic)
+                   end if
+                end do
+             end do
+          end do
+       end if
+       do i = 1, grid3D(iG)%nz
+          do j = 1, grid3D(iG)%ny
+             do k = 1, grid3D(iG)%nx
+                call emissionDriver(grid3D, k, j, i, iG)
+             end do
+          end do
+       end do
+    end subroutine build_tables
+
+    ! dust opacity add, iteration_mod.f90:184-222 (single dust chemistry)
+    subroutine add_dust(iG, ic)
+       integer, intent(in) :: iG, ic
+       integer :: nS, ai, freq, dcp, nsp
+       dcp = dustComPoint(1)
+       nsp = 1
+       do nS = 1, nSpeciesPart(nsp)
+          do ai = 1, nSizes
+             if (grid3D(iG)%Tdust(nS,ai,ic) < TdustSublime(dcp-1+nS)) then
+                do freq = 1, nbins
+                   grid3D(iG)%scaOpac(ic,freq) = grid3D(iG)%scaOpac(ic,freq) + &
+                        & grainAbun(nsp,nS)*grainWeight(ai)*grid3D(iG)%Ndust(ic)* &
+                        & xSecArray(dustScaXsecP(nS+dcp-1,ai)+freq-1)
+                   grid3D(iG)%absOpac(ic,freq) = grid3D(iG)%absOpac(ic,freq) + &
+                        & grainAbun(nsp,nS)*grainWeight(ai)*grid3D(iG)%Ndust(ic)* &
+                        & xSecArray(dustAbsXsecP(nS+dcp-1,ai)+freq-1)
+                end do
+             end if
+          end do
+       end do
+       do freq = 1, nbins
+          grid3D(iG)%opacity(ic,freq) = grid3D(iG)%opacity(ic,freq) + &
+               & (grid3D(iG)%scaOpac(ic,freq) + grid3D(iG)%absOpac(ic,freq))
+       end do
+    end subroutine add_dust
+
+    subroutine dump_r4(name, a, n)
+       character(*), intent(in) :: name
+       integer, intent(in) :: n
+       real, intent(in) :: a(n)
+       open(unit=77, file=trim(outdir)//'/'//name//'.bin', access='stream', &
+            & form='unformatted', status='replace')
+       write(77) a
+       close(77)
+    end subroutine dump_r4
+
+    subroutine dump_i4(name, a, n)
+       character(*), intent(in) :: name
+       integer, intent(in) :: n
+       integer, intent(in) :: a(n)
+       open(unit=77, file=trim(outdir)//'/'//name//'.bin', access='stream', &
+            & form='unformatted', status='replace')
+       write(77) a
+       close(77)
+    end subroutine dump_i4
+
+    subroutine dump_problem()
+       integer :: nc, nfl(8)
+       real :: sc(16)
+       nc = grid3D(1)%nCells
+       call dump_r4('xAxis', grid3D(1)%xAxis, grid3D(1)%nx)
+       call dump_r4('yAxis', grid3D(1)%yAxis, grid3D(1)%ny)
+       call dump_r4('zAxis', grid3D(1)%zAxis, grid3D(1)%nz)
+       call dump_i4('active', grid3D(1)%active, grid3D(1)%nx*grid3D(1)%ny*grid3D(1)%nz)
+       call dump_r4('nuArray', nuArray, nbins)
+       call dump_r4('widFlx', widFlx, nbins)
+       call dump_r4('opacity', grid3D(1)%opacity, (nc+1)*nbins)
+       if (lgGas) then
+          call dump_r4('recPDF', grid3D(1)%recPDF, (nc+1)*nbins)
+          call dump_r4('totalLines', grid3D(1)%totalLines, nc+1)
+          call dump_r4('Te', grid3D(1)%Te, nc+1)
+          call dump_r4('Ne', grid3D(1)%Ne, nc+1)
+          call dump_r4('Hden', grid3D(1)%Hden, nc+1)
+          call dump_r4('ionDen', grid3D(1)%ionDen, (nc+1)*nElementsUsed*nStages)
+          call dump_i4('elementXref', elementXref, nElements)
+       end if
+       if (lgDust) then
+          call dump_r4('scaOpac', grid3D(1)%scaOpac, (nc+1)*nbins)
+          call dump_r4('absOpac', grid3D(1)%absOpac, (nc+1)*nbins)
+          if (allocated(gSca)) call dump_r4('gSca', gSca, nbins)
+       end if
+       call dump_r4('specCDF', inSpectrumProbDen(1,1:nbins), nbins)
+       sc = 0.
+       sc(1) = R_out; sc(2) = grid3D(1)%geoCorrX; sc(3) = grid3D(1)%geoCorrY
+       sc(4) = grid3D(1)%geoCorrZ; sc(5) = ionEdge(1); sc(6) = deltaE(1)
+       sc(7) = starPosition(1)%x; sc(8) = starPosition(1)%y; sc(9) = starPosition(1)%z
+       sc(10) = Lstar(1)
+       call dump_r4('scalars', sc, 16)
+       nfl = 0
+       nfl(1) = grid3D(1)%nx; nfl(2) = grid3D(1)%ny; nfl(3) = grid3D(1)%nz
+       nfl(4) = nc; nfl(5) = nbins
+       nfl(6) = starIndeces(1,1)*10000*10000 + starIndeces(1,2)*10000 + starIndeces(1,3)
+       nfl(7) = 0
+       if (lgSymmetricXYZ) nfl(7) = nfl(7) + 1
+       if (lgDust) nfl(7) = nfl(7) + 2
+       if (lgGas) nfl(7) = nfl(7) + 4
+       if (lgIsotropic) nfl(7) = nfl(7) + 8
+       if (lgMultiStars) nfl(7) = nfl(7) + 16
+       nfl(8) = 0
+       if (lgGas) nfl(8) = nElementsUsed*100 + nStages
+       call dump_i4('ints', nfl, 8)
+    end subroutine dump_problem
+
+    ! iteration_mod.f90:458-496 with one packet per energyPacketDriver call
+    subroutine run_transport()
+       real :: q(4)
+       grid3D(1)%Jste(0:grid3D(1)%nCells, 1:nbins) = 0.
+       grid3D(1)%escapedPackets(0:grid3D(1)%nCells, 0:nbins, 0:nAngleBins) = 0.
+       absInt = 0.; scaInt = 0.
+       stream = 1
+       q = 0.
+       do p = 0, npk - 1
+          call mcref_set_stream(seed, stream, p)
+          call energyPacketDriver(1, 1, grid3D(1:nGrids))
+          q(1) = q(1) + Qphot
+       end do
+       q(2) = absInt; q(3) = scaInt
+       call dump_r4('Jste_raw', grid3D(1)%Jste, (grid3D(1)%nCells+1)*nbins)
+       call dump_r4('esc_raw', grid3D(1)%escapedPackets(:,:,0), (grid3D(1)%nCells+1)*(nbins+1))
+       call dump_r4('transport_counters', q, 4)
+    end subroutine run_transport
+
+    ! iteration_mod.f90:700-722 scaling, then 744-909 (balance, one rank)
+    subroutine run_balance()
+       integer :: nc
+       nc = grid3D(1)%nCells
+       do i = 0, nc
+          grid3D(1)%Jste(i,:) = grid3D(1)%Jste(i,:) * 1.e-9
+          if (lgSymmetricXYZ) grid3D(1)%Jste(i,:) = grid3D(1)%Jste(i,:)/8.
+       end do
+       call dump_r4('Jste_scaled', grid3D(1)%Jste, (nc+1)*nbins)
+       allocate(lgConvergedTemp(0:nc), lgBlackTemp(0:nc))
+       allocate(NeTemp(0:nc), TeTemp(0:nc), ionDenTemp(0:nc, nElementsUsed, nStages))
+       NeTemp = 0.; TeTemp = 0.; ionDenTemp = 0.
+       grid3D(1)%lgConverged(0:nc) = 0
+       grid3D(1)%lgBlack(0:nc) = 0
+       grid3D(1)%noHit = 0.; grid3D(1)%noIonBal = 0.; grid3D(1)%noTeBal = 0.
+       do i = 1, grid3D(1)%nx
+          do j = 1, grid3D(1)%ny
+             do k = 1, grid3D(1)%nz
+                call updateCell(grid3D(1), i, j, k)
+             end do
+          end do
+       end do
+       call dump_r4('bal_Te', TeTemp, nc+1)
+       call dump_r4('bal_Ne', NeTemp, nc+1)
+       call dump_r4('bal_ionDen', ionDenTemp, (nc+1)*nElementsUsed*nStages)
+       call dump_i4('bal_conv', grid3D(1)%lgConverged, nc+1)
+       call dump_i4('bal_black', grid3D(1)%lgBlack, nc+1)
+    end subroutine run_balance
+
+    ! Geometry fixtures: zero opacity, source bin fixed, deltaE = 1.
+    ! For each origin (x y z fold) and packet: record the direction the
+    ! reference's own randomUnitVector returns on that packet's stream
+    ! (draw 0 is getNu2's, photon_mod.f90:727), then run the packet and
+    ! dump every non-zero Jste entry (= sum of dS*1/dV over its visits).
+    subroutine run_walk()
+       integer :: nc, xs, ys, zs, irep, cnt
+       real :: u1
+       nc = grid3D(1)%nCells
+       grid3D(1)%opacity = 0.
+       inSpectrumProbDen(1,:) = 0.
+       inSpectrumProbDen(1,nuBin:nbins) = 1.
+       deltaE(1) = 1.
+       stream = 7
+       open(unit=31, file=trim(originFile), status='old')
+       open(unit=32, file=trim(outdir)//'/walk_dirs.txt', status='replace')
+       open(unit=33, file=trim(outdir)//'/walk_J.txt', status='replace')
+       nOrig = 0
+       do
+          read(31, *, iostat=ios) orig(1), orig(2), orig(3), fold
+          if (ios /= 0) exit
+          nOrig = nOrig + 1
+          starPosition(1)%x = orig(1); starPosition(1)%y = orig(2); starPosition(1)%z = orig(3)
+          ! the cell lookup of setStarPosition (grid_mod.f90:3590-3603)
+          call locate(grid3D(1)%xAxis, orig(1), xs)
+          if (xs < grid3D(1)%nx) then
+             if (orig(1) > (grid3D(1)%xAxis(xs)+grid3D(1)%xAxis(xs+1))/2.) xs = xs+1
+          end if
+          call locate(grid3D(1)%yAxis, orig(2), ys)
+          if (ys < grid3D(1)%ny) then
+             if (orig(2) > (grid3D(1)%yAxis(ys)+grid3D(1)%yAxis(ys+1))/2.) ys = ys+1
+          end if
+          call locate(grid3D(1)%zAxis, orig(3), zs)
+          if (zs < grid3D(1)%nz) then
+             if (orig(3) > (grid3D(1)%zAxis(zs)+grid3D(1)%zAxis(zs+1))/2.) zs = zs+1
+          end if
+          starIndeces(1,1) = xs; starIndeces(1,2) = ys; starIndeces(1,3) = zs; starIndeces(1,4) = 1
+          lgMultiStars = (fold == 0)
+          do p = 0, npk - 1
+             ! direction as initPhotonPacket builds it (photon_mod.f90:650-675)
+             call mcref_set_stream(seed, stream, int(nOrig, c_int64_t)*100000 + p)
+             call random_number(u1)
+             if (u1 == 0. .or. u1 == 1. .or. u1 == 0.9999999) cycle
+             do irep = 1, 1000000
+                dirv = randomUnitVector()
+                if (dirv%x /= 0. .and. dirv%y /= 0. .and. dirv%z /= 0.) exit
+             end do
+             if (lgSymmetricXYZ .and. .not.lgMultiStars) then
+                if (dirv%x < 0.) dirv%x = -dirv%x
+                if (dirv%y < 0.) dirv%y = -dirv%y
+                if (dirv%z < 0.) dirv%z = -dirv%z
+             end if
+             grid3D(1)%Jste = 0.
+             grid3D(1)%escapedPackets = 0.
+             call mcref_set_stream(seed, stream, int(nOrig, c_int64_t)*100000 + p)
+             call energyPacketDriver(1, 1, grid3D(1:nGrids))
+             cnt = 0
+             do ic = 0, nc
+                if (grid3D(1)%Jste(ic, nuBin) /= 0.) cnt = cnt + 1
+             end do
+             write(32, '(I8,1X,I8,3(1X,Z8.8),3(1X,Z8.8),3(1X,I5),1X,I6,1X,Z8.8)') nOrig, p, &
+                  & transfer(orig(1), 1), transfer(orig(2), 1), transfer(orig(3), 1), &
+                  & transfer(dirv%x, 1), transfer(dirv%y, 1), transfer(dirv%z, 1), &
+                  & xs, ys, zs, cnt, transfer(sum(grid3D(1)%escapedPackets(:,nuBin,0)), 1)
+             do ic = 0, nc
+                if (grid3D(1)%Jste(ic, nuBin) /= 0.) then
+                   write(33, '(I8,1X,I8,1X,I8,1X,Z8.8)') nOrig, p, ic, &
+                        & transfer(grid3D(1)%Jste(ic, nuBin), 1)
+                end if
+             end do
+          end do
+       end do
+       close(31); close(32); close(33)
+    end subroutine run_walk
+
+end program mcref_harness

@@ -1,0 +1,82 @@
+"""Input decks (MOCASSIN `input/input.in` keyword files) for the parity and
+benchmark workloads.
+
+The keywords and their meaning are those parsed by the reference's readInput
+(set_input_mod.f90:143-573).  `write_deck(dirpath, name)` writes
+`input/input.in` and the abundance file under `dirpath`; the reference
+harness and the build's host read the same files.
+"""
+import os
+
+# log10(X/H) by number for H, He, C, N, O, Ne, S (other elements absent)
+FULL_GAS = {1: 0.0, 2: -1.0, 6: -3.6, 7: -4.0, 8: -3.3, 10: -4.0, 16: -5.0}
+H_HE = {1: 0.0, 2: -1.0}
+
+
+def abundance_lines(abun):
+    out = []
+    for z in range(1, 31):
+        out.append("%.6e" % (10.0 ** abun[z]) if z in abun else "0.")
+    return "\n".join(out) + "\n"
+
+
+# name -> (keywords, abundances)
+DECKS = {
+    # H+He octant, one sweep: the CPU-runnable reference workload
+    "c1": ([
+        "symmetricXYZ", "nx 19", "ny 19", "nz 19",
+        "edges 3.e18 3.e18 3.e18", "Rin 0.", "Rout 3.e18",
+        "Hdensity 100.", "contShape blackbody", "TStellar 40000.",
+        "LPhot 4.e12", "nbins 400", "nuMin 1.e-5", "nuMax 25.",
+        "nPhotons 1000000", "maxIterateMC 1 100.", "TeStart 10000.",
+    ], H_HE),
+    # small H+He octant used for committed golden vectors
+    "c1mini": ([
+        "symmetricXYZ", "nx 9", "ny 9", "nz 9",
+        "edges 3.e18 3.e18 3.e18", "Rin 0.", "Rout 3.e18",
+        "Hdensity 100.", "contShape blackbody", "TStellar 40000.",
+        "LPhot 4.e12", "nbins 160", "nuMin 1.e-5", "nuMax 25.",
+        "nPhotons 20000", "maxIterateMC 1 100.", "TeStart 10000.",
+    ], H_HE),
+    # full-abundance ionisation-bounded octant (the packets/s workload)
+    "c2": ([
+        "symmetricXYZ", "nx 45", "ny 45", "nz 45",
+        "edges 2.5e18 2.5e18 2.5e18", "Rin 0.", "Rout 2.5e18",
+        "Hdensity 1000.", "contShape blackbody", "TStellar 40000.",
+        "LPhot 1.e13", "nbins 800", "nuMin 1.e-5", "nuMax 25.", "nstages 5",
+        "nPhotons 100000000", "maxIterateMC 30 95.", "TeStart 10000.",
+    ], FULL_GAS),
+    # small full-abundance octant for golden vectors
+    "c2mini": ([
+        "symmetricXYZ", "nx 9", "ny 9", "nz 9",
+        "edges 2.5e18 2.5e18 2.5e18", "Rin 0.", "Rout 2.5e18",
+        "Hdensity 1000.", "contShape blackbody", "TStellar 40000.",
+        "LPhot 1.e13", "nbins 200", "nuMin 1.e-5", "nuMax 25.", "nstages 5",
+        "nPhotons 20000", "maxIterateMC 1 95.", "TeStart 10000.",
+    ], FULL_GAS),
+    # full (non-symmetric) grid, odd size: exercises the non-symmetric walk
+    "full_mini": ([
+        "nx 11", "ny 11", "nz 11",
+        "edges 3.e18 3.e18 3.e18", "Rin 0.", "Rout 3.e18",
+        "Hdensity 100.", "contShape blackbody", "TStellar 40000.",
+        "LPhot 4.e12", "nbins 160", "nuMin 1.e-5", "nuMax 25.",
+        "nPhotons 20000", "maxIterateMC 1 100.", "TeStart 10000.",
+    ], H_HE),
+}
+
+
+def write_deck(dirpath, name, overrides=None):
+    keys, abun = DECKS[name]
+    keys = list(keys)
+    if overrides:
+        for k, v in overrides.items():
+            keys = [x for x in keys if x.split()[0] != k]
+            if v is not None:
+                keys.append("%s %s" % (k, v))
+    os.makedirs(os.path.join(dirpath, "input"), exist_ok=True)
+    os.makedirs(os.path.join(dirpath, "output"), exist_ok=True)
+    with open(os.path.join(dirpath, "input", "abun.in"), "w") as f:
+        f.write(abundance_lines(abun))
+    with open(os.path.join(dirpath, "input", "input.in"), "w") as f:
+        f.write("\n".join(keys + ['nebComposition "input/abun.in"']) + "\n")
+    return os.path.join(dirpath, "input", "input.in")

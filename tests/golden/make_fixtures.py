@@ -1,0 +1,147 @@
+"""Writes the committed golden vectors of the transport path.
+
+Runs ONLY in the build container, where /root/reference exists: it drives the
+unmodified reference (MOCASSIN 2.02, compiled by `make -C oracle ref` into
+oracle/_ref/mcref_harness) on the small decks of tests/golden/decks.py and
+stores inputs and reference outputs as compressed .npz files:
+
+  walk_<deck>.npz       per packet: origin, direction and start cell (as
+                        float/int bits), and the reference's Jste entries of
+                        that packet alone with deltaE = 1 and zero opacity,
+                        i.e. the sum over its visits of dS/dV per cell
+                        (photon_mod.f90:1822-1833), plus its escape flag.
+  transport_<name>.npz  one problem's tables (opacity, recPDF, totalLines,
+                        specCDF, nuArray, axes, active, scalars) and the
+                        reference's raw FP32 Jste and escapedPackets after
+                        N packets with the Philox stream pinned per packet.
+
+usage: python tests/golden/make_fixtures.py [walk|transport|all]
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.abspath(os.path.join(HERE, "..", ".."))
+sys.path.insert(0, HERE)
+import decks  # noqa: E402
+
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "mcref_harness")
+WORK = os.path.join(ROOT, "oracle", "_ref", "work")
+
+
+def run_harness(deck, mode, args, env_extra=None, tag=None):
+    wd = os.path.join(WORK, tag or deck)
+    os.makedirs(os.path.join(wd, "out"), exist_ok=True)
+    decks.write_deck(wd, deck)
+    env = dict(os.environ)
+    env.update(env_extra or {})
+    with open(os.path.join(wd, "log_%s.txt" % mode), "w") as log:
+        subprocess.run([HARNESS, mode, "out"] + [str(a) for a in args], cwd=wd, env=env,
+                       stdout=log, stderr=subprocess.STDOUT, check=True)
+    return os.path.join(wd, "out")
+
+
+def read_problem(d):
+    rd = lambda n, dt=np.float32: np.fromfile(os.path.join(d, n + ".bin"), dt)
+    ints = rd("ints", np.int32)
+    nx, ny, nz, nc, nb = (int(v) for v in ints[:5])
+    out = dict(ints=ints, scalars=rd("scalars"), xAxis=rd("xAxis"), yAxis=rd("yAxis"),
+               zAxis=rd("zAxis"), active=rd("active", np.int32), nuArray=rd("nuArray"),
+               specCDF=rd("specCDF"))
+    for n in ("opacity", "recPDF", "totalLines", "scaOpac", "gSca", "widFlx"):
+        p = os.path.join(d, n + ".bin")
+        if os.path.exists(p):
+            out[n] = rd(n)
+    return out, (nx, ny, nz, nc, nb)
+
+
+def walk_origins(deck, prob):
+    """Origins exercising the walk's special cases (as float32 bits)."""
+    x, y, z = prob["xAxis"], prob["yAxis"], prob["zAxis"]
+    f = np.float32
+    mid = lambda a, i: (a[i + 1] + a[i]) / f(2)        # (a(i+1)+a(i))/2 in FP32
+    rng = np.random.default_rng(2024)
+    o = []
+    if deck.startswith("c1"):                      # symmetric octant, source at (0,0,0)
+        o.append((x[0], y[0], z[0], 1))            # the central source, folded
+        o.append((x[0], y[0], z[0], 0))            # unfolded: immediate reflections
+        o.append((x[3], y[2], z[4], 0))            # on grid points
+        o.append((mid(x, 3), y[5], mid(z, 1), 0))  # on cell walls
+        o.append((x[-1] * f(0.999), y[1], z[6], 0))
+        o.append((x[-1], y[-1], z[-1], 0))         # outer corner
+        for _ in range(3):
+            p = rng.uniform(0, 1, 3).astype(np.float32) * np.array([x[-1], y[-1], z[-1]], np.float32)
+            o.append((p[0], p[1], p[2], 0))
+    else:                                          # full grid, source at the centre
+        c = len(x) // 2
+        o.append((x[c], y[c], z[c], 1))
+        o.append((x[2], y[c], z[7], 1))
+        o.append((mid(x, 4), mid(y, 6), z[3], 1))
+        o.append((x[0], y[0], z[0], 1))            # lower corner
+        o.append((x[-1], y[-1], z[-1], 1))         # upper corner
+        for _ in range(3):
+            p = rng.uniform(-1, 1, 3).astype(np.float32) * np.array([x[-1], y[-1], z[-1]], np.float32)
+            o.append((p[0], p[1], p[2], 1))
+    return o
+
+
+def make_walk(deck, npk=60, seed=777):
+    d = run_harness(deck, "tables", [])
+    prob, (nx, ny, nz, nc, nb) = read_problem(d)
+    origins = walk_origins(deck, prob)
+    wd = os.path.join(WORK, deck)
+    with open(os.path.join(wd, "origins.txt"), "w") as f:
+        for ox, oy, oz, fold in origins:
+            f.write("%.9e %.9e %.9e %d\n" % (ox, oy, oz, fold))
+    # a bin above the H edge so no packet escapes before walking
+    nu_bin = int(np.nonzero(prob["nuArray"] > prob["scalars"][4] * 1.5)[0][0]) + 1
+    d = run_harness(deck, "walk", [seed, npk, nu_bin, os.path.join(wd, "origins.txt")])
+    rows = [l.split() for l in open(os.path.join(d, "walk_dirs.txt"))]
+    jrows = [l.split() for l in open(os.path.join(d, "walk_J.txt"))]
+    hx = lambda s: np.uint32(int(s, 16))
+    dirs = np.array([[int(r[0]), int(r[1])] + [hx(v) for v in r[2:8]] + [int(v) for v in r[8:12]]
+                     + [hx(r[12])] for r in rows], np.int64)
+    jm = np.array([[int(r[0]), int(r[1]), int(r[2]), hx(r[3])] for r in jrows], np.int64)
+    # origins written in decimal: the harness parsed them; keep the bits it used
+    out = os.path.join(HERE, "walk_%s.npz" % deck)
+    np.savez_compressed(
+        out, xAxis=prob["xAxis"], yAxis=prob["yAxis"], zAxis=prob["zAxis"],
+        active=prob["active"], ints=prob["ints"], scalars=prob["scalars"],
+        nu_bin=np.int32(nu_bin),
+        origin_id=dirs[:, 0].astype(np.int32), packet=dirs[:, 1].astype(np.int32),
+        pos_bits=dirs[:, 2:5].astype(np.uint32), dir_bits=dirs[:, 5:8].astype(np.uint32),
+        cell=dirs[:, 8:11].astype(np.int32), n_cells=dirs[:, 11].astype(np.int32),
+        esc_bits=dirs[:, 12].astype(np.uint32),
+        j_origin=jm[:, 0].astype(np.int32), j_packet=jm[:, 1].astype(np.int32),
+        j_cell=jm[:, 2].astype(np.int32), j_bits=jm[:, 3].astype(np.uint32))
+    print("wrote", out, len(rows), "packets", len(jrows), "J entries")
+
+
+def make_transport(deck, npk, seed, name, opacity_scale=None):
+    env = {"MCREF_OPACITY_SCALE": repr(float(opacity_scale))} if opacity_scale else None
+    d = run_harness(deck, "transport", [seed, npk], env_extra=env, tag=name)
+    prob, (nx, ny, nz, nc, nb) = read_problem(d)
+    J = np.fromfile(os.path.join(d, "Jste_raw.bin"), np.float32)
+    E = np.fromfile(os.path.join(d, "esc_raw.bin"), np.float32)
+    q = np.fromfile(os.path.join(d, "transport_counters.bin"), np.float32)
+    nzJ = np.nonzero(J)[0]
+    nzE = np.nonzero(E)[0]
+    out = os.path.join(HERE, "transport_%s.npz" % name)
+    np.savez_compressed(out, seed=np.uint64(seed), npackets=np.int64(npk), **prob,
+                        J_idx=nzJ.astype(np.int32), J_val=J[nzJ],
+                        E_idx=nzE.astype(np.int32), E_val=E[nzE], counters=q)
+    print("wrote", out, "J nonzero", len(nzJ), "E nonzero", len(nzE), "counters", q)
+
+
+if __name__ == "__main__":
+    what = sys.argv[1] if len(sys.argv) > 1 else "all"
+    if what in ("walk", "all"):
+        make_walk("c1mini")
+        make_walk("full_mini")
+    if what in ("transport", "all"):
+        make_transport("c1mini", 20000, 12345, "c1mini")
+        make_transport("c1mini", 8000, 999, "c1mini_thick", opacity_scale=300.0)
+        make_transport("full_mini", 8000, 4242, "full_mini_thick", opacity_scale=100.0)

@@ -1,0 +1,52 @@
+"""The C ABI library loads on any host and exports every symbol the header
+declares; argument validation that needs no device.  No GPU."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+
+
+def header_functions():
+    src = open(os.path.join(ROOT, "include", "mocassin_amd.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s*\*?\s*(mc_\w+)\s*\(", src, re.M)))
+
+
+def test_header_declares_the_boundary():
+    names = header_functions()
+    for want in ("mc_ctx_create", "mc_upload_tables", "mc_transport_run", "mc_walk_replay",
+                 "mc_zero_estimators", "mc_fetch_estimators", "mc_ctx_last_error",
+                 "mc_ctx_destroy"):
+        assert want in names
+
+
+def test_library_exports_every_declared_symbol():
+    from mocassin_amd import _capi
+    L = _capi.load()
+    for name in header_functions():
+        assert hasattr(L, name), name
+    assert set(_capi.EXPORTS) <= set(header_functions())
+
+
+def test_create_rejects_bad_descriptors_without_device():
+    from mocassin_amd import _capi
+    L = _capi.load()
+    h = C.c_void_p()
+    assert L.mc_ctx_create(None, C.byref(h)) == 1 and not h.value
+    d = _capi.McDesc()
+    d.abi_version = 99
+    assert L.mc_ctx_create(C.byref(d), C.byref(h)) == 1
+    assert b"ABI" in L.mc_ctx_last_error(None)
+    d.abi_version = _capi.MC_ABI_VERSION
+    assert L.mc_ctx_create(C.byref(d), C.byref(h)) == 1       # empty sizes
+    assert b"empty" in L.mc_ctx_last_error(None)
+
+
+def test_calls_on_null_context_fail_cleanly():
+    from mocassin_amd import _capi
+    L = _capi.load()
+    assert L.mc_transport_run(None, 1, 10, 0, None, None, None) == 1
+    assert L.mc_upload_tables(None, None, None, None, None, None, None) == 1
+    L.mc_ctx_destroy(None)
