@@ -77,6 +77,7 @@ def main():
     ap.add_argument("--packets", type=float, default=1e8)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -95,6 +96,13 @@ def main():
                   p["nuArray"], p["R_out"], p["geoCorr"], p["ionEdge1"], [p["starPos"]],
                   [p["starIdx"]], [p["deltaE"]], symmetric=p["symmetric"], seed=seed, device=0)
     T.upload_tables(p["opacity"], p["specCDF"], p["recPDF"], p["totalLines"])
+    tuning = T.get_tuning()
+    if not a.no_autotune:
+        from mocassin_amd.autotune import autotune
+        tuning, trials = autotune(T, max(1, min(npk, 4_000_000)))
+        print("bench: autotuned %s (%d candidates, %.1f..%.1f ms)"
+              % (tuning, len(trials), min(t[1] for t in trials), max(t[1] for t in trials)),
+              file=sys.stderr, flush=True)
     print("bench: workload %s, %d cells, %d bins, %d packets/step, setup %.1f s"
           % (a.workload, p["nCells"], p["nbins"], npk, setup_s), file=sys.stderr, flush=True)
 
@@ -137,7 +145,7 @@ def main():
                                "tables synthetic of that shape" % npk if a.workload == "c2"
                    else a.workload,
                    "cells": p["nCells"], "nbins": p["nbins"], "packets_per_step": npk,
-                   "accumulators": "fp64"},
+                   "accumulators": "fp64", "tuning": tuning},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "transport_kernel", "kernel_ms": kms,

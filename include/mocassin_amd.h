@@ -88,6 +88,21 @@ typedef struct {
     double kernel_ms;   /* device time of the transport kernel */
 } mc_counters;
 
+/* Launch shape of the transport kernel (chosen by the autotuner,
+ * mocassin_amd/autotune.py; no counterpart in the reference):
+ *   blocks_per_cu  resident workgroups per compute unit requested (1..32)
+ *   threads        workgroup size, a multiple of 64 (64..256)
+ *   emit_min       an emission pass runs when at least this many lanes of a
+ *                  wavefront wait for one (1..64), or half the live lanes do
+ *   walk_steps     walk passes between two emission checks (1..64)
+ * Results do not depend on these values (each packet has its own stream). */
+typedef struct {
+    int32_t blocks_per_cu;
+    int32_t threads;
+    int32_t emit_min;
+    int32_t walk_steps;
+} mc_tuning;
+
 /* Create a context on desc->device and upload the grid.
  * Replaces: grid setup consumed by energyPacketDriver (photon_mod.f90:26). */
 int mc_ctx_create(const mc_desc *desc, mc_ctx **out);
@@ -130,6 +145,10 @@ int mc_fetch_estimators(mc_ctx *ctx, double *J_out, double *esc_out);
 int mc_walk_replay(mc_ctx *ctx, int32_t n, const float *pos, const float *dir,
                    const int32_t *cell, int32_t *cell_seq, float *ds_seq,
                    int32_t *seq_len, int32_t *status, int32_t max_len);
+
+/* Set / read the transport launch shape (see mc_tuning). */
+int mc_set_tuning(mc_ctx *ctx, const mc_tuning *t);
+int mc_get_tuning(const mc_ctx *ctx, mc_tuning *t);
 
 /* Message of the last failure on this context (never NULL). */
 const char *mc_ctx_last_error(const mc_ctx *ctx);

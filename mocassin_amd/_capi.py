@@ -17,7 +17,8 @@ ERRORS = {0: "MC_OK", 1: "MC_EINVAL", 2: "MC_ENODEV", 3: "MC_ENOMEM", 4: "MC_EST
 
 # every extern "C" symbol the header declares
 EXPORTS = ("mc_ctx_create", "mc_upload_tables", "mc_zero_estimators", "mc_transport_run",
-           "mc_fetch_estimators", "mc_walk_replay", "mc_ctx_last_error", "mc_ctx_destroy")
+           "mc_fetch_estimators", "mc_walk_replay", "mc_set_tuning", "mc_get_tuning",
+           "mc_ctx_last_error", "mc_ctx_destroy")
 
 
 class McDesc(C.Structure):
@@ -36,6 +37,14 @@ class McCounters(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "n_seg", "n_abs", "n_sca", "n_esc", "n_lost", "n_line", "n_trapped", "n_flights",
         "qphot", "kernel_ms")]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class McTuning(C.Structure):
+    _fields_ = [("blocks_per_cu", C.c_int32), ("threads", C.c_int32),
+                ("emit_min", C.c_int32), ("walk_steps", C.c_int32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -66,11 +75,13 @@ def load():
     L.mc_transport_run.argtypes = [vp, i32, i64, u64, vp, vp, C.POINTER(McCounters)]
     L.mc_fetch_estimators.argtypes = [vp, vp, vp]
     L.mc_walk_replay.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, vp, i32]
+    L.mc_set_tuning.argtypes = [vp, C.POINTER(McTuning)]
+    L.mc_get_tuning.argtypes = [vp, C.POINTER(McTuning)]
     L.mc_ctx_last_error.argtypes = [vp]
     L.mc_ctx_last_error.restype = C.c_char_p
     L.mc_ctx_destroy.argtypes = [vp]
     L.mc_ctx_destroy.restype = None
-    for name in EXPORTS[:6]:
+    for name in EXPORTS[:8]:
         getattr(L, name).restype = C.c_int
     _LIB = L
     return L

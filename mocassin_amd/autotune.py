@@ -1,0 +1,45 @@
+"""Autotuner for the transport kernel's launch shape.
+
+The transport kernel has four free parameters (mc_tuning in
+include/mocassin_amd.h): resident workgroups per CU, workgroup size, the
+number of waiting lanes that triggers a batched emission pass, and the
+number of walk passes between emission checks.  Results do not depend on
+them (every packet draws from its own counter-based stream), so the tuner
+only times: it runs a short transport of the real problem for each
+candidate and keeps the fastest by device time.
+"""
+import itertools
+
+DEFAULT_GRID = {
+    "blocks_per_cu": (4, 8, 16),
+    "threads": (256,),
+    "emit_min": (8, 16, 32),
+    "walk_steps": (2, 4, 8),
+}
+
+
+def candidates(grid=None):
+    g = dict(DEFAULT_GRID)
+    g.update(grid or {})
+    keys = list(g)
+    for vals in itertools.product(*(g[k] for k in keys)):
+        yield dict(zip(keys, vals))
+
+
+def autotune(transport, npackets, istar=1, grid=None, packet_offset=1 << 40, repeats=1):
+    """Time every candidate on `npackets` packets of source `istar`; set and
+    return (best tuning, list of (tuning, kernel_ms))."""
+    results = []
+    for i, cand in enumerate(candidates(grid)):
+        transport.set_tuning(**cand)
+        best_ms = None
+        for r in range(repeats):
+            transport.zero_estimators()
+            c = transport.energy_packet_driver(istar, npackets,
+                                               packet_offset + (i * repeats + r) * npackets)
+            best_ms = c["kernel_ms"] if best_ms is None else min(best_ms, c["kernel_ms"])
+        results.append((cand, best_ms))
+    best = min(results, key=lambda t: t[1])[0]
+    transport.set_tuning(**best)
+    transport.zero_estimators()
+    return best, results

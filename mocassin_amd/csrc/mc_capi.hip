@@ -25,13 +25,15 @@ struct mc_ctx {
     std::vector<float> starPos, deltaE;
     std::vector<int> starIdx;
     bool tables = false;
-    int blocks = 0, threads = 256;
+    int cus = 256;
+    mc_tuning tune = { 8, 256, 16, 4 };
     // device allocations
     float *d_axis = nullptr, *d_mid = nullptr, *d_dfac = nullptr;
     int *d_active = nullptr;
-    float *d_opacity = nullptr, *d_scaOpac = nullptr, *d_recCDF = nullptr;
+    float *d_opac3 = nullptr, *d_sca3 = nullptr, *d_recCDF = nullptr;
+    unsigned short *d_recGuide = nullptr;
     float *d_totalLines = nullptr, *d_specCDF = nullptr, *d_nuArray = nullptr, *d_gSca = nullptr;
-    double *d_J = nullptr, *d_esc = nullptr;
+    double *d_J3 = nullptr, *d_J = nullptr, *d_esc = nullptr;
     unsigned long long *d_scratch = nullptr;  // next_packet, counts[8]
     int *d_error = nullptr;
     double *d_qphot = nullptr;
@@ -142,7 +144,8 @@ static int ctx_init(mc_ctx *c, const mc_desc *d)
     HIPCHK(c, dalloc(&c->d_dfac, na));
     HIPCHK(c, dalloc(&c->d_active, (size_t)nx * ny * nz));
     HIPCHK(c, dalloc(&c->d_nuArray, nb));
-    HIPCHK(c, dalloc(&c->d_J, (size_t)(nc + 1) * nb));
+    const size_t ncell3 = (size_t)nx * ny * nz;
+    HIPCHK(c, dalloc(&c->d_J3, ncell3 * nb));
     HIPCHK(c, dalloc(&c->d_esc, (size_t)(nc + 1) * (nb + 1)));
     HIPCHK(c, dalloc(&c->d_scratch, 9));
     HIPCHK(c, dalloc(&c->d_error, 1));
@@ -161,13 +164,13 @@ static int ctx_init(mc_ctx *c, const mc_desc *d)
     G.ionEdge1 = d->ionEdge1;
     G.axis = c->d_axis; G.mid = c->d_mid; G.dfac = c->d_dfac; G.active = c->d_active;
     G.nuArray = c->d_nuArray;
-    G.J = c->d_J; G.esc = c->d_esc;
+    G.ncell3 = (int)ncell3;
+    G.J3 = c->d_J3; G.esc = c->d_esc;
     G.seed_lo = (uint32_t)d->seed; G.seed_hi = (uint32_t)(d->seed >> 32);
 
     int cus = 0;
     HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->device));
-    c->blocks = 8 * (cus > 0 ? cus : 256);
-    c->threads = 256;
+    c->cus = cus > 0 ? cus : 256;
     int rc = mc_zero_estimators(c);
     if (rc) return rc;
     return MC_OK;
@@ -216,27 +219,45 @@ extern "C" int mc_upload_tables(mc_ctx *c, const float *opacity, const float *sc
         for (int cell = 0; cell <= nc; ++cell)
             if (std::isnan(totalLines[cell])) return fail(c, MC_ETABLE, "NaN in totalLines");
     }
-    if (!c->d_opacity) HIPCHK(c, dalloc(&c->d_opacity, nt));
+    for (int nu = 0; nu < nb; ++nu)
+        if (opacity[(size_t)nu * (nc + 1)] != 0.f)
+            return fail(c, MC_ETABLE, "opacity row 0 (inactive cells) must be zero (bin %d)", nu + 1);
+    const size_t ncell3 = (size_t)G.ncell3;
+    float *tmp = nullptr;
+    HIPCHK(c, dalloc(&tmp, nt));
+    int rc = MC_OK;
+    hipError_t e = hipMemcpy(tmp, opacity, nt * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !c->d_opac3) e = dalloc(&c->d_opac3, ncell3 * nb);
+    if (e == hipSuccess) e = mcd_densify(tmp, c->d_active, (int)ncell3, nc + 1, nb, c->d_opac3, c->stream);
+    if (e == hipSuccess && dust) {
+        e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipMemcpy(tmp, scaOpac, nt * sizeof(float), hipMemcpyHostToDevice);
+        if (e == hipSuccess && !c->d_sca3) e = dalloc(&c->d_sca3, ncell3 * nb);
+        if (e == hipSuccess) e = mcd_densify(tmp, c->d_active, (int)ncell3, nc + 1, nb, c->d_sca3, c->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess)
+        return fail(c, e == hipErrorOutOfMemory ? MC_ENOMEM : MC_ENODEV, "table upload: %s", hipGetErrorString(e));
+    (void)rc;
     if (!c->d_specCDF) HIPCHK(c, dalloc(&c->d_specCDF, (size_t)nb * c->nStars));
-    HIPCHK(c, hipMemcpy(c->d_opacity, opacity, nt * sizeof(float), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_specCDF, specCDF, (size_t)nb * c->nStars * sizeof(float), hipMemcpyHostToDevice));
     if (gas) {
         if (!c->d_recCDF) HIPCHK(c, dalloc(&c->d_recCDF, nt));
+        if (!c->d_recGuide) HIPCHK(c, dalloc(&c->d_recGuide, (size_t)(nc + 1) * (mcd::kGuide + 1)));
         if (!c->d_totalLines) HIPCHK(c, dalloc(&c->d_totalLines, (size_t)nc + 1));
         HIPCHK(c, hipMemcpy(c->d_recCDF, rec.data(), nt * sizeof(float), hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(c->d_totalLines, totalLines, (nc + 1) * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(c, mcd_guides(c->d_recCDF, nc + 1, nb, c->d_recGuide, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
     }
-    if (dust) {
-        if (!c->d_scaOpac) HIPCHK(c, dalloc(&c->d_scaOpac, nt));
-        HIPCHK(c, hipMemcpy(c->d_scaOpac, scaOpac, nt * sizeof(float), hipMemcpyHostToDevice));
-        if (gSca) {
-            if (!c->d_gSca) HIPCHK(c, dalloc(&c->d_gSca, nb));
-            HIPCHK(c, hipMemcpy(c->d_gSca, gSca, nb * sizeof(float), hipMemcpyHostToDevice));
-        }
+    if (dust && gSca) {
+        if (!c->d_gSca) HIPCHK(c, dalloc(&c->d_gSca, nb));
+        HIPCHK(c, hipMemcpy(c->d_gSca, gSca, nb * sizeof(float), hipMemcpyHostToDevice));
     }
-    G.opacity = c->d_opacity; G.specCDF = c->d_specCDF;
-    G.recCDF = c->d_recCDF; G.totalLines = c->d_totalLines;
-    G.scaOpac = c->d_scaOpac; G.gSca = c->d_gSca;
+    G.opac3 = c->d_opac3; G.sca3 = c->d_sca3; G.specCDF = c->d_specCDF;
+    G.recCDF = c->d_recCDF; G.recGuide = c->d_recGuide; G.totalLines = c->d_totalLines;
+    G.gSca = c->d_gSca;
     c->tables = true;
     return MC_OK;
 }
@@ -246,7 +267,7 @@ extern "C" int mc_zero_estimators(mc_ctx *c)
     if (!c) return MC_EINVAL;
     HIPCHK(c, hipSetDevice(c->device));
     const size_t nc1 = (size_t)c->G.nCells + 1;
-    HIPCHK(c, hipMemsetAsync(c->d_J, 0, nc1 * c->G.nbins * sizeof(double), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_J3, 0, (size_t)c->G.ncell3 * c->G.nbins * sizeof(double), c->stream));
     HIPCHK(c, hipMemsetAsync(c->d_esc, 0, nc1 * (c->G.nbins + 1) * sizeof(double), c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return MC_OK;
@@ -257,13 +278,35 @@ extern "C" int mc_fetch_estimators(mc_ctx *c, double *J_out, double *esc_out)
     if (!c) return MC_EINVAL;
     HIPCHK(c, hipSetDevice(c->device));
     const size_t nc1 = (size_t)c->G.nCells + 1;
-    if (J_out)
+    if (J_out) {
+        if (!c->d_J) HIPCHK(c, dalloc(&c->d_J, nc1 * c->G.nbins));
+        HIPCHK(c, hipMemsetAsync(c->d_J, 0, nc1 * c->G.nbins * sizeof(double), c->stream));
+        HIPCHK(c, mcd_compact(c->d_J3, c->d_active, c->G.ncell3, (int)nc1, c->G.nbins, c->d_J, c->stream));
         HIPCHK(c, hipMemcpyAsync(J_out, c->d_J, nc1 * c->G.nbins * sizeof(double),
                                  hipMemcpyDeviceToHost, c->stream));
+    }
     if (esc_out)
         HIPCHK(c, hipMemcpyAsync(esc_out, c->d_esc, nc1 * (c->G.nbins + 1) * sizeof(double),
                                  hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MC_OK;
+}
+
+extern "C" int mc_set_tuning(mc_ctx *c, const mc_tuning *t)
+{
+    if (!c || !t) return MC_EINVAL;
+    if (t->blocks_per_cu < 1 || t->blocks_per_cu > 32 || t->threads < 64 || t->threads > 256 ||
+        t->threads % 64 || t->emit_min < 1 || t->emit_min > 64 || t->walk_steps < 1 ||
+        t->walk_steps > 64)
+        return fail(c, MC_EINVAL, "tuning out of range");
+    c->tune = *t;
+    return MC_OK;
+}
+
+extern "C" int mc_get_tuning(const mc_ctx *c, mc_tuning *t)
+{
+    if (!c || !t) return MC_EINVAL;
+    *t = c->tune;
     return MC_OK;
 }
 
@@ -309,8 +352,11 @@ extern "C" int mc_transport_run(mc_ctx *c, int32_t istar, int64_t npackets, uint
     RA.pz = c->starPos[3 * (istar - 1) + 2];
     RA.sx = c->starIdx[3 * (istar - 1)]; RA.sy = c->starIdx[3 * (istar - 1) + 1];
     RA.sz = c->starIdx[3 * (istar - 1) + 2];
+    RA.emit_min = c->tune.emit_min;
+    RA.walk_steps = c->tune.walk_steps;
+    const int blocks = c->tune.blocks_per_cu * c->cus * (256 / c->tune.threads);
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    if (npackets > 0) HIPCHK(c, mcd_launch_transport(c->G, RA, c->blocks, c->threads, c->stream));
+    if (npackets > 0) HIPCHK(c, mcd_launch_transport(c->G, RA, blocks, c->tune.threads, c->stream));
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     unsigned long long h[9];
     int herr = 0;
@@ -383,9 +429,9 @@ extern "C" void mc_ctx_destroy(mc_ctx *c)
 {
     if (!c) return;
     (void)hipSetDevice(c->device);
-    void *bufs[] = { c->d_axis, c->d_mid, c->d_dfac, c->d_active, c->d_opacity, c->d_scaOpac,
-                     c->d_recCDF, c->d_totalLines, c->d_specCDF, c->d_nuArray, c->d_gSca,
-                     c->d_J, c->d_esc, c->d_scratch, c->d_error, c->d_qphot };
+    void *bufs[] = { c->d_axis, c->d_mid, c->d_dfac, c->d_active, c->d_opac3, c->d_sca3,
+                     c->d_recCDF, c->d_recGuide, c->d_totalLines, c->d_specCDF, c->d_nuArray,
+                     c->d_gSca, c->d_J3, c->d_J, c->d_esc, c->d_scratch, c->d_error, c->d_qphot };
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

@@ -28,18 +28,26 @@ struct Grid {
     const float *axis;       // x(1:nx), y(1:ny), z(1:nz) concatenated
     const float *mid;        // (a(i)+a(i+1))/2, i=1..n-1 per axis, concatenated
     const float *dfac;       // per-axis cell widths /1e15 (getVolume factors)
+    int ncell3;              // nx*ny*nz
     const int *active;       // active(x,y,z), x fastest
-    const float *opacity;    // [nu][cell]   (reference layout)
-    const float *scaOpac;    // [nu][cell] or null
-    const float *recCDF;     // [cell][nu]   (transposed: one row per cell)
+    const float *opac3;      // [nu][x,y,z] dense: opacity(active(x,y,z),nu), row 0 = 0
+    const float *sca3;       // [nu][x,y,z] dense scaOpac or null
+    const float *recCDF;     // [cell][nu]   (one row per cell)
+    const unsigned short *recGuide;  // [cell][kGuide+1] guide table of recCDF rows
     const float *totalLines; // [cell]
     const float *specCDF;    // [star][nu]
     const float *nuArray;    // [nu]
     const float *gSca;       // [nu] or null
-    double *J;               // [nu][cell]
+    double *J3;              // [nu][x,y,z] dense FP64 path-length estimator
     double *esc;             // [nu 0..nbins][cell]
     uint32_t seed_lo, seed_hi;
 };
+
+// Guide table (Chen & Asau 1974, "On generating random variates from an
+// empirical distribution"): g[j] = #{i : cdf[i] <= j/K}.  For u in
+// [j/K, (j+1)/K) the count of entries <= u lies in [g[j], g[j+1]], so a
+// search restricted to that range returns exactly the full binary search.
+constexpr int kGuide = 32;
 
 // LDS copy of the small per-axis arrays (axis, midpoints, volume factors).
 struct Axes {
@@ -97,7 +105,8 @@ __device__ __forceinline__ float draw(Rng &R, const Grid &G)
 // getNu2 (photon_mod.f90:720-764): k = number of leading CDF entries <= u,
 // nuP = max(k,1), then +1 unless nuP >= nbins-1.  The linear scan becomes a
 // binary search over the (validated monotone) CDF row; identical result.
-__device__ __forceinline__ int getnu2(Rng &R, const Grid &G, const float *cdf, int nbins, int &err)
+__device__ __forceinline__ int getnu2(Rng &R, const Grid &G, const float *cdf,
+                                      const unsigned short *guide, int nbins, int &err)
 {
     float u = draw(R, G);
     int i;
@@ -106,7 +115,8 @@ __device__ __forceinline__ int getnu2(Rng &R, const Grid &G, const float *cdf, i
         else break;
     }
     if (i >= 10000) { err = 10; return -1; }
-    int lo = 0, hi = nbins;              // count of entries <= u in [lo, hi]
+    const int j = (int)(u * (float)kGuide);          // exact: u has 24 bits, K = 2^5
+    int lo = guide[j], hi = guide[j + 1];            // count of entries <= u in [lo, hi]
     while (lo < hi) {
         int m = (lo + hi) >> 1;
         if (u >= cdf[m]) lo = m + 1; else hi = m;
@@ -114,6 +124,18 @@ __device__ __forceinline__ int getnu2(Rng &R, const Grid &G, const float *cdf, i
     int nuP = lo < 1 ? 1 : lo;
     if (nuP < nbins - 1) nuP = nuP + 1;
     return nuP;
+}
+
+// g[j] = #{i : cdf[i] <= j/kGuide}, j = 0..kGuide, for a monotone row
+__device__ __forceinline__ unsigned short guide_entry(const float *cdf, int nbins, int j)
+{
+    const float t = (float)j / (float)kGuide;
+    int lo = 0, hi = nbins;
+    while (lo < hi) {
+        int m = (lo + hi) >> 1;
+        if (t >= cdf[m]) lo = m + 1; else hi = m;
+    }
+    return (unsigned short)lo;
 }
 
 // randomUnitVector (vector_mod.f90:303-314)
@@ -196,7 +218,8 @@ struct Walker {
     float absTau, passProb;
     // outputs of the last step
     float seg;             // dS or dlLoc of the segment just walked
-    int segCell;           // active index of that segment
+    int segIdx;            // dense index (x-1)+nx*((y-1)+ny*(z-1)) of its cell
+    int segX, segY, segZ;  // its cell indices (1-based)
 };
 
 __device__ __forceinline__ int act(const Grid &G, int x, int y, int z)
@@ -307,16 +330,19 @@ __device__ __forceinline__ int walk_step(const Grid &G, const Axes &A, Walker &W
     else { dS = fminf(dSx, dSy); dS = fminf(dS, dSz); }
     if (dS <= 0.f) { err = 36; return ST_ERROR; }
 
-    const int a = act(G, xP, yP, zP);
+    // dense opacity: inactive cells hold row 0 (= 0), so "active > 0" of the
+    // interaction test (1517) holds whenever tau can exceed passProb
+    const int idx = (xP - 1) + nx * ((yP - 1) + ny * (zP - 1));
     float kappa = 0.f;
-    if (!ZERO_OPACITY) kappa = G.opacity[(size_t)(nuP - 1) * (size_t)(G.nCells + 1) + a];
+    if (!ZERO_OPACITY) kappa = G.opac3[(size_t)(nuP - 1) * (size_t)G.ncell3 + idx];
     const float tauCell = dS * kappa;
     const float dV = AX(A.dx, xP) * AX(A.dy, yP) * AX(A.dz, zP);  // 1442-1514
     kappa_out = kappa;
     dV_out = dV;
-    W.segCell = a;
+    W.segIdx = idx;
+    W.segX = xP; W.segY = yP; W.segZ = zP;
 
-    if (!ZERO_OPACITY && (W.absTau + tauCell > W.passProb) && a > 0) {
+    if (!ZERO_OPACITY && (W.absTau + tauCell > W.passProb)) {
         // interaction, 1517-1580
         float dlLoc = (W.passProb - W.absTau) / kappa;
         rx = rx + dlLoc * vx;

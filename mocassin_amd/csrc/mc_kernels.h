@@ -19,31 +19,19 @@ struct RunArgs {
     float deltaE;
     float px, py, pz;                   // source position [cm]
     int sx, sy, sz;                     // source cell (1-based)
-};
-
-// Per-lane event counters, reduced per wavefront at kernel exit.
-struct Counts {
-    uint32_t seg, abs, sca, esc, lost, line, trapped, flights;
-    __device__ void clear() { seg = abs = sca = esc = lost = line = trapped = flights = 0; }
-    __device__ void reduce_wave()
-    {
-        for (int o = 32; o >= 1; o >>= 1) {
-            seg += __shfl_xor(seg, o);
-            abs += __shfl_xor(abs, o);
-            sca += __shfl_xor(sca, o);
-            esc += __shfl_xor(esc, o);
-            lost += __shfl_xor(lost, o);
-            line += __shfl_xor(line, o);
-            trapped += __shfl_xor(trapped, o);
-            flights += __shfl_xor(flights, o);
-        }
-    }
+    int emit_min;                       // emission pass when >= this many lanes wait
+    int walk_steps;                     // walk passes between emission checks
 };
 
 }  // namespace mcd
 
 hipError_t mcd_launch_transport(const mcd::Grid &G, const mcd::RunArgs &RA, int blocks,
                                 int threads, hipStream_t s);
+hipError_t mcd_densify(const float *compact, const int *active, int ncell3, int nc1, int nbins,
+                       float *dense, hipStream_t s);
+hipError_t mcd_guides(const float *cdf, int rows, int nbins, unsigned short *guide, hipStream_t s);
+hipError_t mcd_compact(const double *dense, const int *active, int ncell3, int nc1, int nbins,
+                       double *out, hipStream_t s);
 hipError_t mcd_launch_replay(const mcd::Grid &G, int n, const float *pos, const float *dir,
                              const int *cell, int *cell_seq, float *ds_seq, int *seq_len,
                              int *status, int max_len, hipStream_t s);

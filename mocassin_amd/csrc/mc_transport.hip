@@ -1,15 +1,19 @@
 // mc_transport.hip - persistent packet-transport kernel and geometry replay
 // kernel for MI355X (gfx950).
 //
-// Transport: one lane owns one packet at a time.  Lanes refill from a
-// global packet counter through one atomic per wavefront (wave-aggregated
-// fetch), so a wavefront keeps all 64 lanes busy while packet lifetimes
-// differ by orders of magnitude (immediate escapes, line packets, long
-// re-emission chains).  Every loop iteration advances each live lane by one
-// event: a packet creation (stellar or diffuse re-emission) or one cell
-// segment of the walk.  The small per-axis arrays and the source CDF are
-// staged in LDS; J_nu and the escape SED are FP64 accumulators updated with
-// hardware FP64 atomics (global_atomic_add_f64).
+// Transport: one lane owns one packet at a time; lanes refill from a global
+// packet counter with one atomic per wavefront.  Packet events are split in
+// two classes and scheduled per wavefront so that SIMD lanes stay coherent:
+//   walk      one cell segment of pathSegment (a few FP32 divisions, one
+//             opacity gather, one FP64 atomic): the common event;
+//   emission  creation of a stellar packet or re-emission of an absorbed one
+//             (Philox draws, CDF search, direction, optical-depth draw):
+//             about ten times the instructions of a walk step.
+// Lanes that need an emission park until at least `emit_min` lanes of the
+// wavefront (or half of its live lanes) are waiting, then all of them emit
+// in one pass; in between the wavefront runs `walk_steps` walk passes.  The
+// per-axis arrays, the source CDF and its guide table live in LDS; J_nu is
+// a dense (x,y,z)-indexed FP64 array updated with hardware FP64 atomics.
 #include "mc_device.h"
 #include "mc_kernels.h"
 
@@ -17,7 +21,7 @@
 
 namespace mcd {
 
-// LDS layout: axes, mids, dV factors for x,y,z followed by the source CDF
+// LDS layout: axes | mids | dV factors | source CDF | source guide (ushort)
 __device__ __forceinline__ void stage_axes(const Grid &G, float *lds, Axes &A, int extra_nu,
                                            const float *extra_src, float **extra_dst)
 {
@@ -34,6 +38,11 @@ __device__ __forceinline__ void stage_axes(const Grid &G, float *lds, Axes &A, i
     A.mx = mi; A.my = mi + (nx - 1); A.mz = mi + (nx - 1) + (ny - 1);
     A.dx = df; A.dy = df + nx; A.dz = df + nx + ny;
     if (extra_dst) *extra_dst = ex;
+}
+
+__host__ __device__ inline size_t lds_floats(int nx, int ny, int nz, int nbins)
+{
+    return (size_t)(2 * (nx + ny + nz) + (nx + ny + nz - 3) + nbins) + (kGuide + 2) / 2 + 1;
 }
 
 // ------------------------------------------------------------ replay kernel
@@ -61,7 +70,7 @@ __global__ void __launch_bounds__(256) walk_replay_kernel(Grid G, int n, const f
         if (r == ST_LOST) { st = 2; break; }
         if (len < max_len) {
             size_t o = (size_t)p * max_len + len;
-            cell_seq[3 * o] = W.xP; cell_seq[3 * o + 1] = W.yP; cell_seq[3 * o + 2] = W.zP;
+            cell_seq[3 * o] = W.segX; cell_seq[3 * o + 1] = W.segY; cell_seq[3 * o + 2] = W.segZ;
             ds_seq[o] = W.seg;
         }
         ++len;
@@ -72,7 +81,7 @@ __global__ void __launch_bounds__(256) walk_replay_kernel(Grid G, int n, const f
 }
 
 // ----------------------------------------------------------- transport kernel
-enum Phase { PH_IDLE = 0, PH_STELLAR = 1, PH_WALK = 2, PH_DIFFUSE = 3, PH_DONE = 4 };
+enum Phase { PH_IDLE = 0, PH_EMIT_STELLAR = 1, PH_EMIT_DIFFUSE = 2, PH_WALK = 3 };
 
 struct Lane {
     Rng R;
@@ -82,23 +91,80 @@ struct Lane {
     int steps, runs, phase;
 };
 
+// wave-uniform event counters (scalar registers)
+struct WaveCounts {
+    unsigned long long seg, abs, sca, esc, lost, line, trapped, flights;
+};
+
+__device__ __forceinline__ unsigned long long wcount(bool p)
+{
+    return (unsigned long long)__popcll(__ballot(p));
+}
+
 __device__ __forceinline__ void esc_add(const Grid &G, int origin, int nuP, float dE)
 {
     unsafeAtomicAdd(&G.esc[(size_t)nuP * (size_t)(G.nCells + 1) + origin], (double)dE);
 }
 
-__device__ __forceinline__ void j_add(const Grid &G, int cell, int nuP, float v)
+__device__ __forceinline__ void j_add(const Grid &G, int idx, int nuP, float v)
 {
-    unsafeAtomicAdd(&G.J[(size_t)(nuP - 1) * (size_t)(G.nCells + 1) + cell], (double)v);
+    unsafeAtomicAdd(&G.J3[(size_t)(nuP - 1) * (size_t)G.ncell3 + idx], (double)v);
 }
 
-__device__ __forceinline__ void begin_flight(Lane &L, const Grid &G, Counts &C)
+// newPhotonPacket + initPhotonPacket + the start of pathSegment
+// (photon_mod.f90:768-1059, 491-684, 1176-1184) for stellar packets
+// (PH_EMIT_STELLAR) and re-emissions at an absorption point (PH_EMIT_DIFFUSE).
+__device__ __forceinline__ void emit(Lane &L, const Grid &G, const RunArgs &RA, const float *spec,
+                                     const unsigned short *spec_guide, bool fold, bool dust,
+                                     bool gas, int star_cell, double &qphot, bool &line_end,
+                                     bool &esc_end, bool &flight, int &err)
 {
+    const bool stellar = (L.phase == PH_EMIT_STELLAR);
+    const float *cdf;
+    const unsigned short *guide;
+    int a;
+    if (stellar) {
+        L.runs = 1;
+        L.W.xP = RA.sx; L.W.yP = RA.sy; L.W.zP = RA.sz;
+        L.W.rx = RA.px; L.W.ry = RA.py; L.W.rz = RA.pz;
+        a = star_cell;
+        cdf = spec;
+        guide = spec_guide;
+    } else {
+        L.runs++;
+        a = act(G, L.W.xP, L.W.yP, L.W.zP);
+        if (!gas) { err = 13; return; }
+        if (a <= 0) { err = 14; return; }
+        float random = 1.f - draw(L.R, G);
+        if (random <= G.totalLines[a]) {                 // line packet (923-956)
+            line_end = true;
+            L.phase = PH_IDLE;
+            return;
+        }
+        cdf = G.recCDF + (size_t)a * G.nbins;
+        guide = G.recGuide + (size_t)a * (kGuide + 1);
+    }
+    int nuP = getnu2(L.R, G, cdf, guide, G.nbins, err);
+    if (err) return;
+    if (nuP < 1 || nuP >= G.nbins) { err = stellar ? 11 : 15; return; }
+    L.nuP = nuP;
+    new_direction(L.R, G, stellar && fold, L.d0x, L.d0y, L.d0z);
+    L.origin = a;
+    const float nu = G.nuArray[nuP - 1];
+    if (stellar && nu > 1.f) qphot += (double)(RA.deltaE / (2.1799153e-11f * nu));
+    if (!dust && nu < G.ionEdge1) {                      // escapes at birth (370-465)
+        esc_add(G, a, nuP, RA.deltaE);
+        esc_end = true;
+        L.phase = PH_IDLE;
+        return;
+    }
+    L.W.vx = L.d0x; L.W.vy = L.d0y; L.W.vz = L.d0z;
     L.W.absTau = 0.f;
     float random = draw(L.R, G);
     L.W.passProb = -mcf_logf(1.f - random);
     L.steps = 0;
-    C.flights++;
+    flight = true;
+    L.phase = PH_WALK;
 }
 
 __global__ void __launch_bounds__(256) transport_kernel(Grid G, RunArgs RA)
@@ -107,6 +173,9 @@ __global__ void __launch_bounds__(256) transport_kernel(Grid G, RunArgs RA)
     Axes A;
     float *spec = nullptr;
     stage_axes(G, lds, A, G.nbins, G.specCDF + (size_t)(RA.istar - 1) * G.nbins, &spec);
+    unsigned short *spec_guide = (unsigned short *)(spec + G.nbins);
+    for (int j = threadIdx.x; j <= kGuide; j += blockDim.x) spec_guide[j] = guide_entry(spec, G.nbins, j);
+    __syncthreads();
 
     const int lane_id = threadIdx.x & 63;
     const bool sym = (G.flags & F_SYM) != 0;
@@ -115,18 +184,18 @@ __global__ void __launch_bounds__(256) transport_kernel(Grid G, RunArgs RA)
     const bool iso = (G.flags & F_ISO) != 0;
     const bool fold = sym && !(G.flags & F_MULTI);
     const float dE = RA.deltaE;
-    const int nbins = G.nbins;
+    const int star_cell = act(G, RA.sx, RA.sy, RA.sz);
 
     Lane L;
     L.phase = PH_IDLE;
     L.runs = 0; L.steps = 0; L.nuP = 1; L.origin = 0;
     bool exhausted = false;
-    Counts C;
-    C.clear();
+    WaveCounts C = {0, 0, 0, 0, 0, 0, 0, 0};
     double qphot = 0.0;
+    int err = 0;
 
     for (;;) {
-        // ---- wave-aggregated refill
+        // ---- refill idle lanes: one atomic per wavefront
         bool need = (L.phase == PH_IDLE) && !exhausted;
         uint64_t m = __ballot(need);
         if (m) {
@@ -140,135 +209,112 @@ __global__ void __launch_bounds__(256) transport_kernel(Grid G, RunArgs RA)
                 unsigned long long id = base + rank;
                 if (id < (unsigned long long)RA.npackets) {
                     rng_start(L.R, RA.packet_offset + id, (uint32_t)RA.istar);
-                    L.phase = PH_STELLAR;
-                    L.runs = 0;
+                    L.phase = PH_EMIT_STELLAR;
                 } else {
                     exhausted = true;
                 }
             }
         }
-        if (!__any(L.phase != PH_IDLE)) break;
+        const bool pend = (L.phase == PH_EMIT_STELLAR || L.phase == PH_EMIT_DIFFUSE);
+        const int npend = (int)wcount(pend);
+        const int nwalk = (int)wcount(L.phase == PH_WALK);
+        if (npend + nwalk == 0) break;
 
-        int err = 0;
-        const bool was_live = L.phase != PH_IDLE;
-        if (L.phase == PH_STELLAR) {
-            // newPhotonPacket("stellar"), photon_mod.f90:800-862
-            L.runs = 1;
-            int nuP = getnu2(L.R, G, spec, nbins, err);
-            if (!err && (nuP < 1 || nuP >= nbins)) err = 11;
-            if (!err) {
-                L.nuP = nuP;
-                new_direction(L.R, G, fold, L.d0x, L.d0y, L.d0z);
-                L.W.xP = RA.sx; L.W.yP = RA.sy; L.W.zP = RA.sz;
-                L.W.rx = RA.px; L.W.ry = RA.py; L.W.rz = RA.pz;
-                L.origin = act(G, RA.sx, RA.sy, RA.sz);
-                float nu = G.nuArray[nuP - 1];
-                if (nu > 1.f) qphot += (double)(dE / (2.1799153e-11f * nu));
-                if (!dust && nu < G.ionEdge1) {          // 370-465
-                    esc_add(G, L.origin, nuP, dE); C.esc++;
-                    L.phase = PH_IDLE;
-                } else {
-                    L.W.vx = L.d0x; L.W.vy = L.d0y; L.W.vz = L.d0z;
-                    begin_flight(L, G, C);
-                    L.phase = PH_WALK;
-                }
+        // ---- emission pass, batched across the wavefront
+        if (npend >= RA.emit_min || npend >= nwalk) {
+            bool line_end = false, esc_end = false, flight = false;
+            if (pend) {
+                emit(L, G, RA, spec, spec_guide, fold, dust, gas, star_cell, qphot, line_end,
+                     esc_end, flight, err);
+                if (err) L.phase = PH_IDLE;
             }
-        } else if (L.phase == PH_DIFFUSE) {
-            // newPhotonPacket("diffuse"), photon_mod.f90:907-992
-            L.runs++;
-            int a = act(G, L.W.xP, L.W.yP, L.W.zP);
-            if (!gas) err = 13;
-            else if (a <= 0) err = 14;
-            if (!err) {
-                float random = 1.f - draw(L.R, G);
-                if (random <= G.totalLines[a]) {
-                    C.line++;                             // line packet: escapes untracked
+            C.line += wcount(line_end);
+            C.esc += wcount(esc_end);
+            C.flights += wcount(flight);
+            // energyPacketDriver counts the packet whose last run is the
+            // recursionLimit-th as trapped (photon_mod.f90:126)
+            C.trapped += wcount(!err && (line_end || esc_end) && L.runs >= kRecursionLimit);
+        }
+
+        // ---- walk passes
+        for (int s = 0; s < RA.walk_steps; ++s) {
+            bool seg = false, escaped = false, lost = false, absorbed = false, scattered = false,
+                 trapped = false;
+            if (L.phase == PH_WALK) {
+                L.steps++;
+                float kappa, dV;
+                int r = walk_step<false>(G, A, L.W, L.nuP, kappa, dV, err);
+                if (r == ST_CONTINUE || r == ST_ESCAPED) {
+                    j_add(G, L.W.segIdx, L.nuP, L.W.seg * dE / dV);
+                    seg = true;
+                    if (r == ST_ESCAPED) { esc_add(G, L.origin, L.nuP, dE); escaped = true; L.phase = PH_IDLE; }
+                } else if (r == ST_LOST) {
+                    lost = true;
                     L.phase = PH_IDLE;
-                } else {
-                    int nuP = getnu2(L.R, G, G.recCDF + (size_t)a * nbins, nbins, err);
-                    if (!err && (nuP < 1 || nuP >= nbins)) err = 15;
-                    if (!err) {
-                        L.nuP = nuP;
-                        new_direction(L.R, G, false, L.d0x, L.d0y, L.d0z);
-                        L.origin = a;
-                        float nu = G.nuArray[nuP - 1];
-                        if (!dust && nu < G.ionEdge1) {
-                            esc_add(G, L.origin, nuP, dE); C.esc++;
-                            L.phase = PH_IDLE;
-                        } else {
+                } else if (r == ST_ABSORBED) {
+                    j_add(G, L.W.segIdx, L.nuP, L.W.seg * dE / dV);
+                    seg = true;
+                    float rx = L.W.rx / 1.e10f, ry = L.W.ry / 1.e10f, rz = L.W.rz / 1.e10f;
+                    if (sqrtf(rx * rx + ry * ry + rz * rz) * 1.e10f >= G.R_out && G.R_out > 0.f) {
+                        esc_add(G, L.origin, L.nuP, dE);        // 1577-1669
+                        escaped = true;
+                        L.phase = PH_IDLE;
+                    } else if (dust) {
+                        const size_t ti = (size_t)(L.nuP - 1) * (size_t)G.ncell3 + L.W.segIdx;
+                        float probSca = G.sca3[ti] / kappa;
+                        float u = 1.f - draw(L.R, G);
+                        if (u > probSca) {
+                            absorbed = true;
+                            if (gas) L.phase = PH_EMIT_DIFFUSE;
+                            else err = 41;                        // dustEmi needs dust PDFs
+                        } else if (u <= probSca) {
+                            scattered = true;                     // 1720-1797
+                            L.origin = act(G, L.W.xP, L.W.yP, L.W.zP);
+                            if (iso) {
+                                new_direction(L.R, G, false, L.d0x, L.d0y, L.d0z);
+                            } else {
+                                for (int ihg = 1; ihg <= 10; ++ihg)
+                                    if (hg(L.R, G, L.nuP, L.d0x, L.d0y, L.d0z) == 0) break;
+                            }
                             L.W.vx = L.d0x; L.W.vy = L.d0y; L.W.vz = L.d0z;
-                            begin_flight(L, G, C);
-                            L.phase = PH_WALK;
-                        }
-                    }
-                }
-            }
-        } else if (L.phase == PH_WALK) {
-            L.steps++;
-            float kappa, dV;
-            int r = walk_step<false>(G, A, L.W, L.nuP, kappa, dV, err);
-            if (r == ST_CONTINUE || r == ST_ESCAPED) {
-                j_add(G, L.W.segCell, L.nuP, L.W.seg * dE / dV);
-                C.seg++;
-                if (r == ST_ESCAPED) { esc_add(G, L.origin, L.nuP, dE); C.esc++; L.phase = PH_IDLE; }
-            } else if (r == ST_LOST) {
-                C.lost++;
-                L.phase = PH_IDLE;
-            } else if (r == ST_ABSORBED) {
-                const int a = L.W.segCell;
-                j_add(G, a, L.nuP, L.W.seg * dE / dV);
-                C.seg++;
-                float rx = L.W.rx / 1.e10f, ry = L.W.ry / 1.e10f, rz = L.W.rz / 1.e10f;
-                if (sqrtf(rx * rx + ry * ry + rz * rz) * 1.e10f >= G.R_out && G.R_out > 0.f) {
-                    esc_add(G, L.origin, L.nuP, dE); C.esc++;     // 1577-1669
-                    L.phase = PH_IDLE;
-                } else if (dust) {
-                    float probSca = G.scaOpac[(size_t)(L.nuP - 1) * (size_t)(G.nCells + 1) + a] / kappa;
-                    float u = 1.f - draw(L.R, G);
-                    if (u > probSca) {
-                        C.abs++;
-                        L.phase = gas ? PH_DIFFUSE : PH_DONE;   // dustEmi needs dust PDFs
-                        if (!gas) err = 41;
-                    } else if (u <= probSca) {
-                        C.sca++;                                  // 1720-1797
-                        L.origin = a;
-                        if (iso) {
-                            new_direction(L.R, G, false, L.d0x, L.d0y, L.d0z);
+                            L.W.absTau = 0.f;
+                            float r2 = draw(L.R, G);
+                            L.W.passProb = -mcf_logf(1.f - r2);
                         } else {
-                            for (int ihg = 1; ihg <= 10; ++ihg)
-                                if (hg(L.R, G, L.nuP, L.d0x, L.d0y, L.d0z) == 0) break;
+                            err = 39;
                         }
-                        L.W.vx = L.d0x; L.W.vy = L.d0y; L.W.vz = L.d0z;
-                        L.W.absTau = 0.f;
-                        float r2 = draw(L.R, G);
-                        L.W.passProb = -mcf_logf(1.f - r2);
                     } else {
-                        err = 39;
+                        absorbed = true;
+                        L.phase = PH_EMIT_DIFFUSE;
                     }
                 } else {
-                    C.abs++;
-                    L.phase = PH_DIFFUSE;
+                    if (!err) err = 50;
                 }
-            } else {
-                err = err ? err : 50;
+                if (!err) {
+                    // safeLimit (2838-2846); recursionLimit (120-126)
+                    if ((L.phase == PH_WALK || L.phase == PH_EMIT_DIFFUSE) && L.steps >= kSafeLimit) {
+                        lost = true;
+                        L.phase = PH_IDLE;
+                    }
+                    if (L.phase == PH_EMIT_DIFFUSE && L.runs >= kRecursionLimit) L.phase = PH_IDLE;
+                    trapped = (L.phase == PH_IDLE) && L.runs >= kRecursionLimit;
+                } else {
+                    L.phase = PH_IDLE;
+                }
             }
-            // safe limit and recursion limit (2838-2846, 120-126)
-            if (!err && L.phase == PH_WALK && L.steps >= kSafeLimit) { C.lost++; L.phase = PH_IDLE; }
-            if (!err && L.phase == PH_DIFFUSE && L.steps >= kSafeLimit) { C.lost++; L.phase = PH_IDLE; }
-            // energyPacketDriver abandons the packet after recursionLimit runs
-            if (!err && L.phase == PH_DIFFUSE && L.runs >= kRecursionLimit) L.phase = PH_IDLE;
+            C.seg += wcount(seg);
+            C.esc += wcount(escaped);
+            C.lost += wcount(lost);
+            C.abs += wcount(absorbed);
+            C.sca += wcount(scattered);
+            C.trapped += wcount(trapped);
         }
         if (err) {
             atomicMax(RA.error, err);
-            L.phase = PH_IDLE;
-        } else if (was_live && L.phase == PH_IDLE && L.runs >= kRecursionLimit) {
-            C.trapped++;                                   // photon_mod.f90:126
+            err = 0;
         }
-        if (L.phase == PH_DONE) L.phase = PH_IDLE;
     }
 
-    // ---- wave reduction of counters
-    C.reduce_wave();
     for (int o = 32; o >= 1; o >>= 1) qphot += __shfl_xor(qphot, o);
     if (lane_id == 0) {
         atomicAdd(&RA.counts[0], C.seg);
@@ -283,13 +329,53 @@ __global__ void __launch_bounds__(256) transport_kernel(Grid G, RunArgs RA)
     }
 }
 
+// ------------------------------------------------------- table preparation
+// opac3[nu][xyz] = opacity(active(xyz), nu)   (reference layout in: [nu][cell])
+__global__ void densify_kernel(const float *compact, const int *active, int ncell3, int nc1,
+                               int nbins, float *dense)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t tot = (size_t)ncell3 * nbins;
+    for (; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+        int nu = (int)(i / ncell3), xyz = (int)(i % ncell3);
+        dense[i] = compact[(size_t)nu * nc1 + active[xyz]];
+    }
+}
+
+// guide[row][j] for every row of a [rows][nbins] CDF table
+__global__ void guide_kernel(const float *cdf, int rows, int nbins, unsigned short *guide)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t tot = (size_t)rows * (kGuide + 1);
+    for (; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+        int r = (int)(i / (kGuide + 1)), j = (int)(i % (kGuide + 1));
+        guide[i] = guide_entry(cdf + (size_t)r * nbins, nbins, j);
+    }
+}
+
+// J(cell,nu) = value at the (x,y,z) of that cell; row 0 collects every
+// inactive position, as the reference's Jste(0,:) does.  out must be zero.
+__global__ void compact_kernel(const double *dense, const int *active, int ncell3, int nc1,
+                               int nbins, double *out)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t tot = (size_t)ncell3 * nbins;
+    for (; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+        int nu = (int)(i / ncell3), xyz = (int)(i % ncell3);
+        double v = dense[i];
+        int a = active[xyz];
+        if (a > 0) out[(size_t)nu * nc1 + a] = v;
+        else if (v != 0.0) unsafeAtomicAdd(&out[(size_t)nu * nc1], v);
+    }
+}
+
 }  // namespace mcd
 
 // ---------------------------------------------------------------- launchers
 hipError_t mcd_launch_transport(const mcd::Grid &G, const mcd::RunArgs &RA, int blocks,
                                 int threads, hipStream_t s)
 {
-    size_t lds = sizeof(float) * (size_t)(2 * (G.nx + G.ny + G.nz) + (G.nx + G.ny + G.nz - 3) + G.nbins);
+    size_t lds = sizeof(float) * mcd::lds_floats(G.nx, G.ny, G.nz, G.nbins);
     hipLaunchKernelGGL(mcd::transport_kernel, dim3(blocks), dim3(threads), lds, s, G, RA);
     return hipGetLastError();
 }
@@ -298,9 +384,31 @@ hipError_t mcd_launch_replay(const mcd::Grid &G, int n, const float *pos, const 
                              const int *cell, int *cell_seq, float *ds_seq, int *seq_len,
                              int *status, int max_len, hipStream_t s)
 {
-    size_t lds = sizeof(float) * (size_t)(2 * (G.nx + G.ny + G.nz) + (G.nx + G.ny + G.nz - 3));
+    size_t lds = sizeof(float) * mcd::lds_floats(G.nx, G.ny, G.nz, 0);
     int threads = 256, blocks = (n + threads - 1) / threads;
     hipLaunchKernelGGL(mcd::walk_replay_kernel, dim3(blocks), dim3(threads), lds, s, G, n, pos,
                        dir, cell, cell_seq, ds_seq, seq_len, status, max_len);
+    return hipGetLastError();
+}
+
+hipError_t mcd_densify(const float *compact, const int *active, int ncell3, int nc1, int nbins,
+                       float *dense, hipStream_t s)
+{
+    hipLaunchKernelGGL(mcd::densify_kernel, dim3(2048), dim3(256), 0, s, compact, active, ncell3,
+                       nc1, nbins, dense);
+    return hipGetLastError();
+}
+
+hipError_t mcd_guides(const float *cdf, int rows, int nbins, unsigned short *guide, hipStream_t s)
+{
+    hipLaunchKernelGGL(mcd::guide_kernel, dim3(2048), dim3(256), 0, s, cdf, rows, nbins, guide);
+    return hipGetLastError();
+}
+
+hipError_t mcd_compact(const double *dense, const int *active, int ncell3, int nc1, int nbins,
+                       double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(mcd::compact_kernel, dim3(2048), dim3(256), 0, s, dense, active, ncell3,
+                       nc1, nbins, out);
     return hipGetLastError();
 }

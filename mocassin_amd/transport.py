@@ -116,6 +116,20 @@ class Transport:
                                              ln.ctypes.data, st.ctypes.data, int(max_len)))
         return seq, ds, ln, st
 
+    def set_tuning(self, blocks_per_cu=None, threads=None, emit_min=None, walk_steps=None):
+        t = _capi.McTuning()
+        self._check(self._lib.mc_get_tuning(self._h, C.byref(t)))
+        for k, v in (("blocks_per_cu", blocks_per_cu), ("threads", threads),
+                     ("emit_min", emit_min), ("walk_steps", walk_steps)):
+            if v is not None:
+                setattr(t, k, int(v))
+        self._check(self._lib.mc_set_tuning(self._h, C.byref(t)))
+
+    def get_tuning(self):
+        t = _capi.McTuning()
+        self._check(self._lib.mc_get_tuning(self._h, C.byref(t)))
+        return t.as_dict()
+
     def close(self):
         if getattr(self, "_h", None):
             self._lib.mc_ctx_destroy(self._h)

@@ -153,3 +153,25 @@ def test_full_size_bookkeeping_and_sampled_parity(c2_problem):
     for k in ("n_seg", "n_abs", "n_esc", "n_line", "n_flights"):
         assert cs[k] == co[k], k
     _cmp_estimators(Js, Es, Jo, Eo)
+
+
+def test_results_independent_of_launch_shape():
+    g = golden("transport_full_mini_thick")
+    P = oracle_problem(g)
+    n, seed = int(g["npackets"]), int(g["seed"])
+    T = transport_from_problem(P, seed)
+    ref = None
+    for tune in ({"blocks_per_cu": 1, "threads": 64, "emit_min": 1, "walk_steps": 1},
+                 {"blocks_per_cu": 8, "threads": 256, "emit_min": 16, "walk_steps": 4},
+                 {"blocks_per_cu": 16, "threads": 128, "emit_min": 64, "walk_steps": 64}):
+        T.set_tuning(**tune)
+        T.zero_estimators()
+        c = T.energy_packet_driver(1, n, 0)
+        J, E = T.fetch()
+        if ref is None:
+            ref = (c, J, E)
+            continue
+        for k in ("n_seg", "n_abs", "n_esc", "n_line", "n_flights"):
+            assert c[k] == ref[0][k], (tune, k)
+        _cmp_estimators(J, E, ref[1], ref[2])
+    T.close()
