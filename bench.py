@@ -78,6 +78,11 @@ def main():
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tuning", default=None,
+                    help="fixed launch shape blocks_per_cu,threads,emit_min,walk_steps")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="also run one device copy of this many MiB (traffic calibration "
+                         "for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -97,7 +102,11 @@ def main():
                   [p["starIdx"]], [p["deltaE"]], symmetric=p["symmetric"], seed=seed, device=0)
     T.upload_tables(p["opacity"], p["specCDF"], p["recPDF"], p["totalLines"])
     tuning = T.get_tuning()
-    if not a.no_autotune:
+    if a.tuning:
+        b, th, e, w = (int(v) for v in a.tuning.split(","))
+        T.set_tuning(blocks_per_cu=b, threads=th, emit_min=e, walk_steps=w)
+        tuning = T.get_tuning()
+    elif not a.no_autotune:
         from mocassin_amd.autotune import autotune
         tuning, trials = autotune(T, max(1, min(npk, 4_000_000)))
         print("bench: autotuned %s (%d candidates, %.1f..%.1f ms)"
@@ -106,6 +115,13 @@ def main():
     print("bench: workload %s, %d cells, %d bins, %d packets/step, setup %.1f s"
           % (a.workload, p["nCells"], p["nbins"], npk, setup_s), file=sys.stderr, flush=True)
 
+    if a.copy_mb:
+        n = a.copy_mb * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     step = 0
     for _ in range(a.warmup):
         T.zero_estimators()

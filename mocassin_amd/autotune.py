@@ -11,10 +11,10 @@ candidate and keeps the fastest by device time.
 import itertools
 
 DEFAULT_GRID = {
-    "blocks_per_cu": (4, 8, 16),
+    "blocks_per_cu": (2, 4, 8),
     "threads": (256,),
-    "emit_min": (8, 16, 32),
-    "walk_steps": (2, 4, 8),
+    "emit_min": (24, 32, 48, 64),
+    "walk_steps": (4, 8, 16, 32),
 }
 
 
