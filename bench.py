@@ -77,7 +77,9 @@ def main():
     ap.add_argument("--packets", type=float, default=1e8)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time the launch-shape candidates on this workload and store the best "
+                         "in mocassin_amd/tuning_db.json (default: use the stored shape)")
     ap.add_argument("--tuning", default=None,
                     help="fixed launch shape blocks_per_cu,threads,emit_min,walk_steps")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -101,17 +103,25 @@ def main():
                   p["nuArray"], p["R_out"], p["geoCorr"], p["ionEdge1"], [p["starPos"]],
                   [p["starIdx"]], [p["deltaE"]], symmetric=p["symmetric"], seed=seed, device=0)
     T.upload_tables(p["opacity"], p["specCDF"], p["recPDF"], p["totalLines"])
-    tuning = T.get_tuning()
+    from mocassin_amd import autotune as at
+    tuning_src = "default"
     if a.tuning:
         b, th, e, w = (int(v) for v in a.tuning.split(","))
         T.set_tuning(blocks_per_cu=b, threads=th, emit_min=e, walk_steps=w)
-        tuning = T.get_tuning()
-    elif not a.no_autotune:
-        from mocassin_amd.autotune import autotune
-        tuning, trials = autotune(T, max(1, min(npk, 4_000_000)))
-        print("bench: autotuned %s (%d candidates, %.1f..%.1f ms)"
-              % (tuning, len(trials), min(t[1] for t in trials), max(t[1] for t in trials)),
+        tuning_src = "command line"
+    elif a.autotune:
+        best, trials = at.autotune(T, max(1, min(npk, 10_000_000)), repeats=2)
+        at.save(a.workload, best, "bench.py --autotune, %d candidates" % len(trials))
+        tuning_src = "autotuned now"
+        print("bench: autotuned %s (%d candidates, %.2f..%.2f ms)"
+              % (best, len(trials), min(t[1] for t in trials), max(t[1] for t in trials)),
               file=sys.stderr, flush=True)
+    elif at.load(a.workload):
+        db = at.load(a.workload)
+        T.set_tuning(**{k: db[k] for k in ("blocks_per_cu", "threads", "emit_min", "walk_steps")})
+        tuning_src = "mocassin_amd/tuning_db.json"
+    tuning = T.get_tuning()
+    tuning["source"] = tuning_src
     print("bench: workload %s, %d cells, %d bins, %d packets/step, setup %.1f s"
           % (a.workload, p["nCells"], p["nbins"], npk, setup_s), file=sys.stderr, flush=True)
 
@@ -145,12 +155,14 @@ def main():
     c0 = counters[-1]
     print("bench: %.3e packets/s, kernel %.1f ms/launch, %.3g seg/packet, %.3g abs/packet"
           % (value, kms, c0["n_seg"] / npk, c0["n_abs"] / npk), file=sys.stderr, flush=True)
+    # HBM bytes per launch: calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE per
+    # packet of this workload (profiles/transport_c2_pmc.md) times this launch
     traffic = None
-    tp = os.path.join(ROOT, "profiles", "transport_c2_traffic.json")
+    tp = os.path.join(ROOT, "profiles", "transport_%s_traffic.json" % a.workload)
     if os.path.exists(tp):
         try:
-            traffic = json.load(open(tp)).get("bytes_per_launch")
-        except Exception:
+            traffic = json.load(open(tp))["hbm_bytes_per_packet"] * npk
+        except (OSError, ValueError, KeyError):
             traffic = None
     out = {
         "metric": "packets/s/GPU (transport sweep)",
@@ -164,6 +176,7 @@ def main():
                    "accumulators": "fp64", "tuning": tuning},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes": byts,
                      "kernel": "transport_kernel", "kernel_ms": kms,
                      "segments": c0["n_seg"], "absorptions": c0["n_abs"], "escapes": c0["n_esc"]},
     }

@@ -9,6 +9,10 @@ only times: it runs a short transport of the real problem for each
 candidate and keeps the fastest by device time.
 """
 import itertools
+import json
+import os
+
+DB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_db.json")
 
 DEFAULT_GRID = {
     "blocks_per_cu": (2, 4, 8),
@@ -43,3 +47,23 @@ def autotune(transport, npackets, istar=1, grid=None, packet_offset=1 << 40, rep
     transport.set_tuning(**best)
     transport.zero_estimators()
     return best, results
+
+
+def load(workload):
+    """Tuning stored for a workload name, or None."""
+    try:
+        return json.load(open(DB_PATH)).get(workload)
+    except (OSError, ValueError):
+        return None
+
+
+def save(workload, tuning, note=""):
+    try:
+        db = json.load(open(DB_PATH))
+    except (OSError, ValueError):
+        db = {}
+    db[workload] = dict(tuning)
+    if note:
+        db[workload]["note"] = note
+    with open(DB_PATH, "w") as f:
+        json.dump(db, f, indent=1, sort_keys=True)
