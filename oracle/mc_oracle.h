@@ -79,6 +79,42 @@ int mco_walk(const mco_problem *P, const float pos[3], const float dir[3],
 int mco_transport(const mco_problem *P, uint64_t first, uint64_t n,
                   void *J, void *esc, mco_counters *cnt);
 
+/* ------------------------------------------------------------- balance */
+#define MCO_MAX_CEL 64
+
+/* Atomic inputs of updateCell as the reference holds them after setup
+ * (arrays dumped by the reference harness; see tests/golden). */
+typedef struct {
+    int nbins, nstages, nElemUsed;
+    const float *nuArray;
+    int32_t elem_on_in[30], xref_in[30];   /* lgElementOn, elementXref (1..30) */
+    float abun_in[30];                     /* elemAbun(1,1:30) */
+    int nch;                               /* photoionisation channels */
+    const int32_t *ch_meta;                /* [nch][8] elem ion g0 g1 IPnuP hiRate hiHeat outShell */
+    const float *ch_sigma;                 /* [nch][nbins] cross sections on nuArray */
+    const float *rrec;                     /* rrec(1:2, z, n)  as [z-1][n-1][2] */
+    const float *rnew;                     /* rnew(1:4, z, n)  as [z-1][n-1][4] */
+    const float *fe;                       /* fe(1:3, 4:13)    as [n-4][3] */
+    const int32_t *dr_int; const float *dr_real;   /* direc_coeffs(25) */
+    const int32_t *ap_int; const float *ap_real;   /* aldropequi_coeffs(167) */
+    int ncel;                              /* CEL model ions in forLines order */
+    const int32_t *cel_meta;               /* [ncel][6] elem ion ii jj nlev ntemp */
+    const double *cel_dbl;                 /* per ion: e(nl), a(nl,nl), qom(nt,nl,nl) */
+    const float *cel_flt;                  /* per ion: logtemp(nt), g(nl), br, z, a_r(4), a_d(5) */
+    float XHILimit;
+    /* filled by mco_atomic_init */
+    int elemOn[31], xref[31];
+    float abun[31];
+    int32_t chan_g[31 * 10 * 2];
+    size_t cel_dbl_off[MCO_MAX_CEL], cel_flt_off[MCO_MAX_CEL];
+} mco_atomic;
+
+int mco_atomic_init(mco_atomic *A);
+int mco_balance_cell_f32(const mco_atomic *A, const float *J, float Hden, float *Te, float *Ne,
+                         float *ionDen, int *nT);
+int mco_balance_cell_f64(const mco_atomic *A, const double *J, double Hden, double *Te, double *Ne,
+                         double *ionDen, int *nT);
+
 #ifdef __cplusplus
 }
 #endif

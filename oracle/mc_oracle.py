@@ -201,3 +201,88 @@ def load_harness_dump(d, **flag_overrides):
         isotropic=bool(flags & 8), multistars=bool(flags & 16))
     kw.update(flag_overrides)
     return Problem(**kw)
+
+
+# ------------------------------------------------------------------ balance
+class _Atomic(C.Structure):
+    _fields_ = [
+        ("nbins", C.c_int), ("nstages", C.c_int), ("nElemUsed", C.c_int),
+        ("nuArray", C.c_void_p),
+        ("elem_on_in", C.c_int32 * 30), ("xref_in", C.c_int32 * 30), ("abun_in", C.c_float * 30),
+        ("nch", C.c_int), ("ch_meta", C.c_void_p), ("ch_sigma", C.c_void_p),
+        ("rrec", C.c_void_p), ("rnew", C.c_void_p), ("fe", C.c_void_p),
+        ("dr_int", C.c_void_p), ("dr_real", C.c_void_p), ("ap_int", C.c_void_p), ("ap_real", C.c_void_p),
+        ("ncel", C.c_int), ("cel_meta", C.c_void_p), ("cel_dbl", C.c_void_p), ("cel_flt", C.c_void_p),
+        ("XHILimit", C.c_float),
+        ("elemOn", C.c_int * 31), ("xref", C.c_int * 31), ("abun", C.c_float * 31),
+        ("chan_g", C.c_int32 * 620),
+        ("cel_dbl_off", C.c_size_t * 64), ("cel_flt_off", C.c_size_t * 64),
+    ]
+
+
+class Balance:
+    """FP32 (the reference's precision) or FP64 restatement of updateCell
+    for every cell of a grid, given an atomic bundle (mocassin_amd.atomic)."""
+
+    def __init__(self, bundle):
+        L = lib()
+        if not hasattr(L, "_bal_declared"):
+            for name, t in (("mco_balance_cell_f32", C.c_float), ("mco_balance_cell_f64", C.c_double)):
+                f = getattr(L, name)
+                f.restype = C.c_int
+                f.argtypes = [C.POINTER(_Atomic), C.c_void_p, t, C.POINTER(t), C.POINTER(t),
+                              C.c_void_p, C.POINTER(C.c_int)]
+            L.mco_atomic_init.argtypes = [C.POINTER(_Atomic)]
+            L.mco_atomic_init.restype = C.c_int
+            L._bal_declared = True
+        b = bundle
+        keep = {}
+
+        def arr(name, dt):
+            a = np.ascontiguousarray(getattr(b, name), dt)
+            keep[name] = a
+            return a.ctypes.data
+        A = _Atomic()
+        A.nbins, A.nstages, A.nElemUsed = b.nbins, b.nstages, b.nElemUsed
+        A.nuArray = arr("nuArray", np.float32)
+        A.elem_on_in[:] = [int(v) for v in b.elem_on]
+        A.xref_in[:] = [int(v) for v in b.xref]
+        A.abun_in[:] = [float(v) for v in b.abun]
+        A.nch, A.ch_meta, A.ch_sigma = b.nch, arr("ch_meta", np.int32), arr("ch_sigma", np.float32)
+        A.rrec, A.rnew, A.fe = arr("rrec", np.float32), arr("rnew", np.float32), arr("fe", np.float32)
+        A.dr_int, A.dr_real = arr("dr_int", np.int32), arr("dr_real", np.float32)
+        A.ap_int, A.ap_real = arr("ap_int", np.int32), arr("ap_real", np.float32)
+        A.ncel, A.cel_meta = b.ncel, arr("cel_meta", np.int32)
+        A.cel_dbl, A.cel_flt = arr("cel_dbl", np.float64), arr("cel_flt", np.float32)
+        A.XHILimit = b.XHILimit
+        rc = L.mco_atomic_init(C.byref(A))
+        if rc:
+            raise RuntimeError("mco_atomic_init failed (%d)" % rc)
+        self.A, self._keep, self.b = A, keep, b
+
+    def run(self, J, Hden, Te, Ne, ionDen, precision="f64", cells=None):
+        """J (nCells+1, nbins) scaled Jste; Hden/Te/Ne (nCells+1,);
+        ionDen (nCells+1, nElemUsed, nstages).  Returns new Te, Ne, ionDen,
+        flags (bit0 converged, bit1 black, bit2 no Te balance, bit3 no ion
+        balance, bit4 not hit) and T-iteration counts."""
+        L = lib()
+        dt = np.float32 if precision == "f32" else np.float64
+        ct = C.c_float if precision == "f32" else C.c_double
+        f = L.mco_balance_cell_f32 if precision == "f32" else L.mco_balance_cell_f64
+        nc1 = J.shape[0]
+        Te = np.array(Te, dt)
+        Ne = np.array(Ne, dt)
+        ion = np.array(ionDen, dt)
+        flags = np.zeros(nc1, np.int32)
+        nT = np.zeros(nc1, np.int32)
+        Jd = np.ascontiguousarray(J, dt)
+        for c in (range(1, nc1) if cells is None else cells):
+            te, ne = ct(Te[c]), ct(Ne[c])
+            sl = np.ascontiguousarray(ion[c].T)          # [stage][elem]
+            n = C.c_int()
+            flags[c] = f(C.byref(self.A), Jd[c].ctypes.data, dt(Hden[c]), C.byref(te), C.byref(ne),
+                         sl.ctypes.data, C.byref(n))
+            Te[c], Ne[c] = te.value, ne.value
+            ion[c] = sl.T
+            nT[c] = n.value
+        return Te, Ne, ion, flags, nT

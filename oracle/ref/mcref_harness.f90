@@ -87,6 +87,7 @@ program mcref_harness
     end if
 
     call build_tables()
+    if (lgGas) call dump_atomic()
     ! optional uniform opacity scaling (stress fixtures with many absorptions)
     call get_environment_variable("MCREF_OPACITY_SCALE", arg, status=ios)
     if (ios == 0) then
@@ -185,6 +186,106 @@ contains
                & (grid3D(iG)%scaOpac(ic,freq) + grid3D(iG)%absOpac(ic,freq))
        end do
     end subroutine add_dust
+
+    ! Atomic inputs of updateCell (update_mod.f90:15-2027): photoionisation
+    ! channels with their cross-section segments and frequency ranges as
+    ! updateCell and thermBalance index them (update_mod.f90:184-215,
+    ! 1150-1185), outer-shell weights (getOuterShell), dielectronic fits
+    ! (direc_coeffs, aldropequi_coeffs) and the CEL model ions that
+    ! forLines/equilibrium use (atomic_data_array).
+    subroutine dump_atomic()
+       integer :: elem, ion, nElec, outShell, g0, g1, IPnuP, hiR, hiH, j, nch, ncel
+       integer :: ii, jj, nl, nt, xs
+       integer, allocatable :: meta(:)
+       real, allocatable :: sig(:)
+       integer :: dri(25*3), api(167*2), hdr(8)
+       real :: drr(25*5), apr(167*4)
+       open(unit=81, file=trim(outdir)//'/ch_meta.bin', access='stream', form='unformatted', status='replace')
+       open(unit=82, file=trim(outdir)//'/ch_sigma.bin', access='stream', form='unformatted', status='replace')
+       allocate(sig(nbins))
+       nch = 0
+       do elem = 1, nElements
+          do ion = 1, min(elem, nStages-1)
+             if (.not.lgElementOn(elem)) exit
+             g0 = 0; g1 = 0; outShell = 0
+             if (elem > 2) then
+                nElec = elem - ion + 1
+                call getOuterShell(elem, nElec, outShell, g0, g1)
+                IPnuP = elementP(elem, ion, outShell, 1)
+                hiR = elementP(elem, ion, outShell, 2) - 1
+                hiH = elementP(elem, ion, outShell, 2)
+                xs = elementP(elem, ion, outShell, 3)
+             else
+                if (elem == 1) then
+                   IPnuP = HlevNuP(1); xs = HlevXSecP(1)
+                else if (ion == 1) then
+                   IPnuP = HeIlevNuP(1); xs = HeISingXSecP(1)
+                else
+                   IPnuP = HeIIlevNuP(1); xs = HeIIXSecP(1)
+                end if
+                hiR = nbins; hiH = nbins
+                nElec = elem - ion + 1
+                call getOuterShell(elem, nElec, outShell, g0, g1)
+             end if
+             sig = 0.
+             do j = IPnuP, min(hiH, nbins)
+                sig(j) = xSecArray(j + xs - IPnuP)
+             end do
+             write(81) elem, ion, g0, g1, IPnuP, hiR, hiH, outShell
+             write(82) sig
+             nch = nch + 1
+          end do
+       end do
+       close(81); close(82)
+       do j = 1, 25
+          dri(3*j-2) = direc_coeffs(j)%elem; dri(3*j-1) = direc_coeffs(j)%n; dri(3*j) = direc_coeffs(j)%g
+          drr(5*j-4) = direc_coeffs(j)%a; drr(5*j-3) = direc_coeffs(j)%b; drr(5*j-2) = direc_coeffs(j)%c
+          drr(5*j-1) = direc_coeffs(j)%d; drr(5*j) = direc_coeffs(j)%f
+       end do
+       do j = 1, 167
+          api(2*j-1) = aldropequi_coeffs(j)%elem; api(2*j) = aldropequi_coeffs(j)%n
+          apr(4*j-3) = aldropequi_coeffs(j)%a; apr(4*j-2) = aldropequi_coeffs(j)%b
+          apr(4*j-1) = aldropequi_coeffs(j)%t0; apr(4*j) = aldropequi_coeffs(j)%t1
+       end do
+       call dump_i4('dr_int', dri, 75); call dump_r4('dr_real', drr, 125)
+       call dump_i4('ap_int', api, 334); call dump_r4('ap_real', apr, 668)
+       ! CEL model ions in forLines order (update_mod.f90:1270-1310)
+       open(unit=83, file=trim(outdir)//'/cel_meta.bin', access='stream', form='unformatted', status='replace')
+       open(unit=84, file=trim(outdir)//'/cel_dbl.bin', access='stream', form='unformatted', status='replace')
+       open(unit=85, file=trim(outdir)//'/cel_flt.bin', access='stream', form='unformatted', status='replace')
+       ncel = 0
+       do elem = 3, nElements
+          do ion = 1, min(elem+1, nstages)
+             if (.not.lgElementOn(elem)) exit
+             if (.not.lgDataAvailable(elem, ion)) cycle
+             ! the entry equilibrium() finds for dataFile(elem, ion) (emission_mod.f90:1960-1968)
+             do ii = 3, nElements
+                do jj = 1, 10
+                   if (trim(atomic_data_array(ii,jj)%ion) == trim(dataFile(elem, ion))) goto 77
+                end do
+             end do
+77           continue
+             nl = atomic_data_array(ii,jj)%nlevs
+             nt = atomic_data_array(ii,jj)%ntemps
+             write(83) elem, ion, ii, jj, nl, nt
+             write(84) atomic_data_array(ii,jj)%e(1:nl), atomic_data_array(ii,jj)%a(1:nl,1:nl), &
+                  & atomic_data_array(ii,jj)%qom(1:nt,1:nl,1:nl)
+             write(85) atomic_data_array(ii,jj)%logtemp(1:nt), real(atomic_data_array(ii,jj)%g(1:nl)), &
+                  & atomic_data_array(ii,jj)%br, atomic_data_array(ii,jj)%z, &
+                  & atomic_data_array(ii,jj)%a_r(1:4), atomic_data_array(ii,jj)%a_d(1:5)
+             ncel = ncel + 1
+          end do
+       end do
+       close(83); close(84); close(85)
+       hdr = 0
+       hdr(1) = nstages; hdr(2) = nElementsUsed; hdr(3) = nAbComponents; hdr(4) = nch; hdr(5) = ncel
+       call dump_i4('at_ints', hdr, 8)
+       call dump_i4('at_elem_on', merge(1, 0, lgElementOn), nElements)
+       call dump_i4('at_xref', elementXref, nElements)
+       call dump_r4('at_abun', grid3D(1)%elemAbun(1, 1:nElements), nElements)
+       call dump_i4('at_abfile', grid3D(1)%abFileIndex, grid3D(1)%nx*grid3D(1)%ny*grid3D(1)%nz)
+       deallocate(sig)
+    end subroutine dump_atomic
 
     subroutine dump_r4(name, a, n)
        character(*), intent(in) :: name

@@ -98,3 +98,18 @@ def gpu_available():
         return torch.cuda.is_available()
     except Exception:
         return False
+
+
+def balance_fixture(name):
+    """(bundle, J, Hden, Te0, Ne0, ion0, ref) of a balance fixture; arrays
+    indexed by cell 0..nCells, ion arrays (nCells+1, nElemUsed, nstages)."""
+    from mocassin_amd.atomic import AtomicBundle
+    g = golden("balance_" + name)
+    b = AtomicBundle.from_npz(g)
+    nc, nb = int(g["ints"][3]), int(g["ints"][4])
+    J = np.zeros((nc + 1) * nb, np.float32)
+    J[g["J_idx"]] = g["J_val"]
+    J = J.reshape(nb, nc + 1).T
+    shp = lambda a: a.reshape(b.nstages, b.nElemUsed, nc + 1).transpose(2, 1, 0)
+    ref = dict(Te=g["Te"], Ne=g["Ne"], ion=shp(g["ionDen"]), conv=g["conv"], black=g["black"])
+    return b, J, g["Hden"], g["Te0"], g["Ne0"], shp(g["ionDen0"]), ref

@@ -136,11 +136,36 @@ def make_transport(deck, npk, seed, name, opacity_scale=None):
     print("wrote", out, "J nonzero", len(nzJ), "E nonzero", len(nzE), "counters", q)
 
 
+def make_balance(deck, npk, seed, name, opacity_scale=None):
+    """Reference transport (J) + one updateCell pass over every cell."""
+    sys.path.insert(0, ROOT)
+    from mocassin_amd.atomic import AtomicBundle
+    env = {"MCREF_OPACITY_SCALE": repr(float(opacity_scale))} if opacity_scale else None
+    d = run_harness(deck, "balance", [seed, npk], env_extra=env, tag="bal_" + name)
+    prob, (nx, ny, nz, nc, nb) = read_problem(d)
+    b = AtomicBundle.from_harness_dump(d, "/root/reference/data/radrec.dat")
+    rd = lambda n, dt=np.float32: np.fromfile(os.path.join(d, n + ".bin"), dt)
+    J = rd("Jste_scaled")
+    nzJ = np.nonzero(J)[0]
+    out = os.path.join(HERE, "balance_%s.npz" % name)
+    np.savez_compressed(
+        out, ints=prob["ints"], nuArray=prob["nuArray"], **b.to_npz_dict(),
+        Hden=rd("Hden"), Te0=rd("Te"), Ne0=rd("Ne"), ionDen0=rd("ionDen"),
+        J_idx=nzJ.astype(np.int32), J_val=J[nzJ],
+        Te=rd("bal_Te"), Ne=rd("bal_Ne"), ionDen=rd("bal_ionDen"),
+        conv=rd("bal_conv", np.int32), black=rd("bal_black", np.int32))
+    print("wrote", out, "cells", nc, "J nonzero", len(nzJ))
+
+
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "all"
     if what in ("walk", "all"):
         make_walk("c1mini")
         make_walk("full_mini")
+    if what in ("balance", "all"):
+        make_balance("c1mini", 20000, 31, "c1mini")
+        make_balance("c2mini", 20000, 4242, "c2mini")
+        make_balance("c2mini", 20000, 77, "c2mini_thick", opacity_scale=300.0)
     if what in ("transport", "all"):
         make_transport("c1mini", 20000, 12345, "c1mini")
         make_transport("c1mini", 8000, 999, "c1mini_thick", opacity_scale=300.0)
