@@ -103,6 +103,49 @@ typedef struct {
     int32_t walk_steps;
 } mc_tuning;
 
+/* Atomic inputs of the per-cell balance, as the host holds them after its
+ * setup (readData, makeElements, initXSecArray; the tables are read by
+ * updateCell and its callees, update_mod.f90:15-2027, emission_mod.f90:
+ * 1913-2190).  All arrays are plain C order as stated:
+ *   elemOn/xref/abun[30]     lgElementOn, elementXref, elemAbun(1,:) by Z-1
+ *   ch_meta[nch][8]          elem, ion, g0, g1, IPnuP, hiRate, hiHeat, outShell
+ *                            of each photoionisation channel: statistical
+ *                            weights of the ground states, first bin above the
+ *                            edge and the last bins of the rate and heating
+ *                            integrals (1-based), outer shell
+ *   ch_sigma[nch][nbins]     its cross section on nuArray [cm^2]
+ *   rrec[30][30][2]          rrec(1:2, z, n) at [z-1][n-1][i-1]
+ *   rnew[30][30][4]          rnew(1:4, z, n) at [z-1][n-1][i-1]
+ *   fe[10][3]                Fe fits of radRecFit for n = 4..13
+ *   dr_int[25][3]/dr_real[25][5]    direc_coeffs: elem, n, Te-range flag / a..f
+ *   ap_int[167][2]/ap_real[167][4]  aldropequi_coeffs: elem, n / A, B, T0, T1
+ *   cel_meta[ncel][6]        elem, ion, ii, jj, nlev, ntemp of each CEL
+ *                            model ion in forLines order
+ *   cel_dbl                  per model ion: e(nl) [cm^-1], a(nl,nl) [s^-1]
+ *                            column-major, qom(nt,nl,nl) column-major
+ *   cel_flt                  per model ion: logtemp(nt), g(nl), br, z,
+ *                            a_r(4), a_d(5) (level recombination fit) */
+typedef struct {
+    int32_t nstages;            /* nstages (<= 9) */
+    int32_t nElemUsed;          /* nElementsUsed */
+    const int32_t *elemOn;
+    const int32_t *xref;
+    const float *abun;
+    int32_t nch;
+    const int32_t *ch_meta;
+    const float *ch_sigma;
+    const float *rrec, *rnew, *fe;
+    const int32_t *dr_int;
+    const float *dr_real;
+    const int32_t *ap_int;
+    const float *ap_real;
+    int32_t ncel;
+    const int32_t *cel_meta;
+    const double *cel_dbl;
+    const float *cel_flt;
+    float XHILimit;             /* XHILimit (convergence of the H0 fraction) */
+} mc_atomic;
+
 /* Create a context on desc->device and upload the grid.
  * Replaces: grid setup consumed by energyPacketDriver (photon_mod.f90:26). */
 int mc_ctx_create(const mc_desc *desc, mc_ctx **out);
@@ -145,6 +188,37 @@ int mc_fetch_estimators(mc_ctx *ctx, double *J_out, double *esc_out);
 int mc_walk_replay(mc_ctx *ctx, int32_t n, const float *pos, const float *dir,
                    const int32_t *cell, int32_t *cell_seq, float *ds_seq,
                    int32_t *seq_len, int32_t *status, int32_t max_len);
+
+/* Overwrite the device estimators with host values (layouts above; esc may
+ * be NULL).  Row 0 of J is not kept: inactive positions read zero.  For a
+ * host that runs the balance on estimators it obtained elsewhere (a restart,
+ * another transport).  Replaces: the reads of grid%Jste and
+ * grid%escapedPackets by iterateMC and updateCell (iteration_mod.f90:700-722,
+ * update_mod.f90:104-154). */
+int mc_set_estimators(mc_ctx *ctx, const double *J, const double *esc);
+
+/* Upload the atomic inputs of the balance (validated; see mc_atomic).
+ * Requires H and He switched on, nstages <= 9, nbins <= 2048, at most 16
+ * elements, 64 channels, 64 CEL model ions of <= 17 levels and <= 64
+ * temperatures.  Replaces: the module data updateCell reads
+ * (update_mod.f90:15-2027). */
+int mc_balance_setup(mc_ctx *ctx, const mc_atomic *atomic);
+
+/* Ionisation and thermal balance of every active cell on the device (FP64),
+ * from the device J times jscale (the host's normalisation of Jste, e.g.
+ * scale_J).  Hden(0:nCells), Te(0:nCells), Ne(0:nCells) and
+ * ionDen(0:nCells, 1:nElemUsed, 1:nstages) (column-major, cell fastest) are
+ * copied in and, with flags(0:nCells) and nIter(0:nCells), back out; cells
+ * that no packet reached keep their inputs.  flags per cell: bit0 H0
+ * fraction converged (lgConverged), bit1 lgBlack, bit2 thermal balance not
+ * reached in 25 iterations, bit3 ion balance not converged, bit4 not hit.
+ * nIter: temperature iterations used.  kernel_ms (nullable): device time.
+ * Replaces: updateCell (update_mod.f90:15-525) with iterateT, ionBalance,
+ * thermBalance, calcAlpha, radRecFit, dielectronic and forLines/equilibrium
+ * (update_mod.f90:339-1834, emission_mod.f90:1913-2190), called per cell by
+ * iterateMC's loop over the grid (iteration_mod.f90:826-838). */
+int mc_balance_run(mc_ctx *ctx, double jscale, const double *Hden, double *Te, double *Ne,
+                   double *ionDen, int32_t *flags, int32_t *nIter, double *kernel_ms);
 
 /* Set / read the transport launch shape (see mc_tuning). */
 int mc_set_tuning(mc_ctx *ctx, const mc_tuning *t);

@@ -18,6 +18,7 @@ ERRORS = {0: "MC_OK", 1: "MC_EINVAL", 2: "MC_ENODEV", 3: "MC_ENOMEM", 4: "MC_EST
 # every extern "C" symbol the header declares
 EXPORTS = ("mc_ctx_create", "mc_upload_tables", "mc_zero_estimators", "mc_transport_run",
            "mc_fetch_estimators", "mc_walk_replay", "mc_set_tuning", "mc_get_tuning",
+           "mc_set_estimators", "mc_balance_setup", "mc_balance_run",
            "mc_ctx_last_error", "mc_ctx_destroy")
 
 
@@ -50,6 +51,17 @@ class McTuning(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class McAtomic(C.Structure):
+    _fields_ = [
+        ("nstages", C.c_int32), ("nElemUsed", C.c_int32), ("elemOn", C.c_void_p),
+        ("xref", C.c_void_p), ("abun", C.c_void_p), ("nch", C.c_int32), ("ch_meta", C.c_void_p),
+        ("ch_sigma", C.c_void_p), ("rrec", C.c_void_p), ("rnew", C.c_void_p), ("fe", C.c_void_p),
+        ("dr_int", C.c_void_p), ("dr_real", C.c_void_p), ("ap_int", C.c_void_p),
+        ("ap_real", C.c_void_p), ("ncel", C.c_int32), ("cel_meta", C.c_void_p),
+        ("cel_dbl", C.c_void_p), ("cel_flt", C.c_void_p), ("XHILimit", C.c_float),
+    ]
+
+
 class McError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("%s: %s" % (ERRORS.get(code, str(code)), msg))
@@ -77,11 +89,14 @@ def load():
     L.mc_walk_replay.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, vp, i32]
     L.mc_set_tuning.argtypes = [vp, C.POINTER(McTuning)]
     L.mc_get_tuning.argtypes = [vp, C.POINTER(McTuning)]
+    L.mc_set_estimators.argtypes = [vp, vp, vp]
+    L.mc_balance_setup.argtypes = [vp, C.POINTER(McAtomic)]
+    L.mc_balance_run.argtypes = [vp, C.c_double, vp, vp, vp, vp, vp, vp, vp]
     L.mc_ctx_last_error.argtypes = [vp]
     L.mc_ctx_last_error.restype = C.c_char_p
     L.mc_ctx_destroy.argtypes = [vp]
     L.mc_ctx_destroy.restype = None
-    for name in EXPORTS[:8]:
+    for name in EXPORTS[:11]:
         getattr(L, name).restype = C.c_int
     _LIB = L
     return L

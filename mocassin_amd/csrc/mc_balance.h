@@ -1,0 +1,92 @@
+// mc_balance.h - device tables of the per-cell balance (updateCell), the
+// host-side preparation that builds them from the C-ABI atomic descriptor,
+// and the launcher of the balance kernel.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "../../include/mocassin_amd.h"
+
+namespace mcb {
+
+constexpr int kMaxCh = 64;      // photoionisation channels (elem, ion)
+constexpr int kMaxElem = 16;    // elements switched on
+constexpr int kMaxStage = 10;   // ionisation stages + 1 (stage index 1..nstages)
+constexpr int kMaxCel = 64;     // CEL model ions
+constexpr int kMaxLev = 17;     // levels per model ion (nForLevels)
+constexpr int kMaxTemp = 64;    // temperatures per collision-strength table
+constexpr int kMaxDr = 4;       // dielectronic fit rows per channel
+constexpr int kMaxNb = 2048;    // frequency bins (one cell's J row is staged in LDS)
+
+// one photoionisation channel (elem, ion) in the reference's loop order
+struct Channel {
+    int elem, ion, eidx;        // atomic number, stage, element index (0..nE-1)
+    int ip, hiR, hiH;           // 1-based bins: rate sum ip..hiR, heat sum ip..hiH
+    int rr_kind;                // 0 Verner-Ferland (4), 1 power law (2), 2 Fe fit (3)
+    double rr[4];
+    int ndr, dr[kMaxDr];        // dielectronic rows (direc_coeffs) filling (elem, ion)
+    int ap;                     // last aldropequi row filling (elem, ion), -1 if none
+    double chex[4], dEk;        // charge-exchange fit and reverse-reaction deltaE/k
+    double g0g1;                // real(g0)/real(g1)
+};
+
+struct Element {
+    int z, nst;                 // atomic number, stages written (min(z+1, nstages))
+    int ch0, nch;               // its channels (stages 1..min(z, nstages-1))
+    double abun;
+};
+
+struct CelIon {
+    int eidx, ion, nl, nt;
+    double br, z, a_r[4], a_d[5];
+    int off_e, off_a, off_q, off_t, off_g;   // offsets: e, A(nl,nl), qom/y2(nt,npair), logT, g
+};
+
+struct Atomic {
+    int nbins, nstages, nE, nch, ncel;
+    float XHILimit;
+    Channel ch[kMaxCh];
+    Element el[kMaxElem];
+    int odr_n[kMaxStage], odr[kMaxStage][kMaxDr], oap[kMaxStage];   // oxygen rows (fallback)
+    double dr_real[25][5];
+    int dr_g[25];
+    double ap_real[167][4];
+    CelIon cel[kMaxCel];
+    // device arrays
+    const double *sigma;        // [nch][nbins]
+    const double *nu;           // [nbins]
+    const double *cel_e, *cel_a, *cel_q, *cel_y2, *cel_t, *cel_g;
+};
+
+struct CellArgs {
+    int ncells;                 // active cells (1..nCells)
+    double jscale;              // J = J3 * jscale
+    const double *J3;           // dense [nu][xyz]
+    int ncell3;
+    const int *cell_xyz;        // [nCells+1] dense index of each active cell
+    const double *Hden;         // [nCells+1]
+    double *Te, *Ne;            // [nCells+1] in/out
+    double *ion;                // [nCells+1][nE][nstages+1], stage 0 unused
+    int *flags, *nIter;         // [nCells+1]
+};
+
+// Host-built tables: `A` with its device pointers unset, and the flat arrays
+// they will point to.  eidx_xref[e] is the elementXref (1-based) of element
+// index e.
+struct Prepared {
+    Atomic A{};
+    std::vector<double> sigma, nu, cel_e, cel_a, cel_q, cel_y2, cel_t, cel_g;
+    std::vector<int> eidx_xref;
+};
+
+// Builds the tables from the descriptor; returns MC_OK or an MC_E* code with
+// a message in err.
+int prepare(const mc_atomic *in, int nbins, const float *nuArray, Prepared &P, char *err,
+            size_t errlen);
+
+}  // namespace mcb
+
+hipError_t mcb_launch_balance(const mcb::Atomic *dA, const mcb::CellArgs &a, hipStream_t s);

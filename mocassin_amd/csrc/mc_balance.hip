@@ -1,0 +1,681 @@
+// mc_balance.hip - per-cell ionisation and thermal balance (MOCASSIN's
+// updateCell, update_mod.f90:15-525) on MI355X, FP64.
+//
+// One 64-lane wavefront per cell (one wavefront per workgroup), cell state
+// in LDS.  Lanes share the work of each stage of the reference's schedule:
+//   * photoionisation rates and heating integrals: lanes over frequency
+//     bins, wave reductions per channel (update_mod.f90:184-269, 1146-1234);
+//   * recombination fits, charge exchange, ion ratios: lanes over the
+//     (elem, ion) channels (calcAlpha, radRecFit, dielectronic, ionBalance);
+//   * CEL cooling: the model ions one after another, each n-level system
+//     built and eliminated cooperatively (rows and pivot updates across
+//     lanes; equilibrium/luSlv of emission_mod.f90:1913-2190);
+//   * the bracketed false-position T iteration and the fixed-point ion
+//     iteration run uniformly on the wavefront (iterateT, ionBalance).
+// The collision-strength spline second derivatives depend only on the
+// atomic data, so they are computed once at setup (the reference recomputes
+// them on every call; the values are the same).
+#include "mc_balance.h"
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+
+#pragma clang fp contract(off)
+
+namespace mcb {
+
+constexpr int W = 64;
+
+struct Lds {
+    double J[kMaxNb];           // this cell's J row (scaled)
+    double Te, Ne, Hden;
+    double ion[kMaxElem][kMaxStage];
+    double nPhot[kMaxCh], heatNu[kMaxCh], alpha[kMaxCh], ratio[kMaxCh], chexv[kMaxCh];
+    double diO[kMaxStage];
+    double red[W];
+    double x[kMaxLev + 1][kMaxLev + 1];
+    double q[kMaxLev + 1][kMaxLev + 1];
+    double y[kMaxLev + 1];
+    double alev[kMaxLev + 1];
+};
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__device__ double rad_rec(const Channel &c, double Te)
+{
+    if (c.rr_kind == 0) {
+        double tt = sqrt(Te / c.rr[2]);
+        return c.rr[0] / (tt * pow(tt + 1.0, 1.0 - c.rr[1]) * pow(1.0 + sqrt(Te / c.rr[3]), 1.0 + c.rr[1]));
+    }
+    double tt = Te * 1.e-4;
+    if (c.rr_kind == 2) return c.rr[0] / pow(tt, c.rr[1] + c.rr[2] * log10(tt));
+    return c.rr[0] / pow(tt, c.rr[1]);
+}
+
+// dielectronic (update_mod.f90:1782-1834) for one (elem, ion): the last
+// satisfying Nussbaumer-Storey row, else (or above 6e4 K) Aldrovandi-Pequignot
+__device__ double di_rec(const Atomic &A, int ndr, const int *dr, int ap, double Te)
+{
+    double v = 0.0;
+    double t = Te / 10000.;
+    for (int r = 0; r < ndr; ++r) {
+        const int i = dr[r];
+        const int g = A.dr_g[i];
+        if (g == 0 || (g == 1 && Te < 20000.) || (g == 2 && Te >= 20000.)) {
+            const double *c = A.dr_real[i];
+            v = 1.e-12 * (c[0] / t + c[1] + c[2] * t + c[3] * (t * t)) * pow(t, -1.5) * exp(-c[4] / t);
+        }
+    }
+    if (Te > 60000. || v == 0.0) {
+        if (ap >= 0) {
+            const double *c = A.ap_real[ap];
+            v = c[0] * pow(Te, -1.5) * exp(-c[2] / Te) * (1.0 + c[1] * exp(-c[3] / Te));
+        } else {
+            v = 0.0;
+        }
+    }
+    return v;
+}
+
+// natural cubic spline evaluation with precomputed second derivatives
+// (splint, interpolation_mod.f90:268-316)
+__device__ double splint(const double *xa, const double *ya, const double *y2, int n, double x)
+{
+    if (x <= xa[0]) return ya[0];
+    if (x >= xa[n - 1]) return ya[n - 1];
+    int klo = 1, khi = n;
+    while (khi - klo > 1) {
+        int k = (khi + klo) / 2;
+        if (xa[k - 1] > x) khi = k; else klo = k;
+    }
+    double h = xa[khi - 1] - xa[klo - 1];
+    double a = (xa[khi - 1] - x) / h;
+    double b = (x - xa[klo - 1]) / h;
+    return a * ya[klo - 1] + b * ya[khi - 1] +
+           ((a * a * a - a) * y2[klo - 1] + (b * b * b - b) * y2[khi - 1]) * (h * h) / 6.;
+}
+
+// CEL cooling of one model ion [erg/s per H], levels 1..10 (forLines +
+// equilibrium + the cooling sum of thermBalance)
+__device__ double cel_cool(const Atomic &A, const CelIon &ci, Lds &S, int lane)
+{
+    const int nl = ci.nl, nt = ci.nt;
+    const double Te = S.Te, Ne = S.Ne;
+    const double *E = A.cel_e + ci.off_e;
+    const double *Aa = A.cel_a + ci.off_a;          // a(i,j) at [(i-1) + nl*(j-1)]
+    const double *lT = A.cel_t + ci.off_t;
+    const double *gg = A.cel_g + ci.off_g;
+    const double xi = S.ion[ci.eidx][ci.ion];
+    const double up = (ci.ion < A.nstages) ? S.ion[ci.eidx][ci.ion + 1] / xi : 0.0;
+    if (up != up) return 0.0;
+    const double sqrTe = sqrt(Te);
+    const double log10Te = log10(Te);
+    // level recombination coefficients (emission_mod.f90:1991-2002)
+    for (int j = lane; j <= kMaxLev; j += W) S.alev[j] = 0.0;
+    __syncthreads();
+    if (ci.br > 0.0) {
+        for (int j = 2 + lane; j <= nl; j += W) {
+            double t4 = Te * 1.e-4;
+            double zz = ci.z;
+            double x1 = t4 / (zz * zz);
+            S.alev[j] = 1.e-13 * ci.br * zz * ci.a_r[0] * pow(x1, ci.a_r[1]) /
+                            (1. + ci.a_r[2] * pow(t4 / (zz * zz), ci.a_r[3])) +
+                        1.0e-12 * ci.br *
+                            (ci.a_d[0] / t4 + ci.a_d[1] + ci.a_d[2] * Te * 1.e-4 + ci.a_d[3] * (t4 * t4)) *
+                            pow(t4, -1.5) * exp(-(ci.a_d[4] * 1.e4 / Te));
+        }
+        if (lane == 0 && nl >= 2) S.alev[1] = 1.0;
+    }
+    // collision rates q(lo,up) excitation, q(up,lo) de-excitation (2004-2045)
+    const int npair = nl * (nl - 1) / 2;
+    for (int p = lane; p < npair; p += W) {
+        int lo = 1, rem = p;
+        while (rem >= nl - lo) { rem -= nl - lo; ++lo; }
+        const int upL = lo + 1 + rem;
+        const double *qq = A.cel_q + ci.off_q + (size_t)p * nt;
+        const double *y2 = A.cel_y2 + ci.off_q + (size_t)p * nt;
+        double cs;
+        if (nt == 1) cs = qq[0];
+        else if (nt == 2) cs = qq[0] + (qq[1] - qq[0]) / (lT[1] - lT[0]) * (log10Te - lT[0]);
+        else cs = splint(lT, qq, y2, nt, log10Te);
+        const double delTeK = (E[lo - 1] - E[upL - 1]) * 1.4388463;
+        const double expFac = exp(delTeK / Te);
+        S.q[lo][upL] = 8.63e-6 * cs * expFac / (gg[lo - 1] * sqrTe);
+        S.q[upL][lo] = 8.63e-6 * cs / (gg[upL - 1] * sqrTe);
+    }
+    __syncthreads();
+    // rate matrix rows 2..nl in the reference's j order, row 1 = ones
+    for (int i = 2 + lane; i <= nl; i += W) {
+        for (int j = 1; j <= nl; ++j) S.x[i][j] = 0.0;
+        for (int j = 1; j <= nl; ++j) {
+            if (j != i) {
+                S.x[i][j] = S.x[i][j] + Ne * S.q[j][i];
+                S.x[i][i] = S.x[i][i] - Ne * S.q[i][j];
+                if (j > i) S.x[i][j] = S.x[i][j] + Aa[(j - 1) + nl * (i - 1)];
+                else S.x[i][i] = S.x[i][i] - Aa[(i - 1) + nl * (j - 1)];
+            }
+        }
+        S.y[i] = -(Ne * up * S.alev[i]);
+    }
+    if (lane == 0) {
+        for (int j = 1; j <= nl; ++j) S.x[1][j] = 1.0;
+        S.y[1] = 1.0;
+    }
+    __syncthreads();
+    // lured: Gaussian elimination without pivoting, rows updated across lanes
+    for (int i = 1; i <= nl - 1; ++i) {
+        for (int k = i + 1 + lane; k <= nl; k += W) {
+            const double factor = S.x[k][i] / S.x[i][i];
+            for (int j = i + 1; j <= nl; ++j) S.x[k][j] = S.x[k][j] - S.x[i][j] * factor;
+        }
+        __syncthreads();
+    }
+    // reslv: forward substitution across lanes, back substitution on lane 0
+    for (int i = 1; i <= nl - 1; ++i) {
+        const double yi = S.y[i], xii = S.x[i][i];
+        for (int j = i + 1 + lane; j <= nl; j += W) S.y[j] = S.y[j] - yi * S.x[j][i] / xii;
+        __syncthreads();
+    }
+    double cool = 0.0;
+    if (lane == 0) {
+        S.y[nl] = S.y[nl] / S.x[nl][nl];
+        for (int i = 1; i <= nl - 1; ++i) {
+            const int k = nl - i;
+            for (int j = k + 1; j <= nl; ++j) S.y[k] = S.y[k] - S.y[j] * S.x[k][j];
+            S.y[k] = S.y[k] / S.x[k][k];
+        }
+        double sumN = 0.0;
+        for (int i = 1; i <= nl; ++i) sumN = sumN + S.y[i];
+        const double s1 = A.el[ci.eidx].abun, s2 = xi;
+        const double kE = 1.9865e-16;
+        for (int i = 1; i <= 10 && i <= nl - 1; ++i)
+            for (int j = i + 1; j <= 10 && j <= nl; ++j) {
+                const double a = Aa[(j - 1) + nl * (i - 1)];
+                if (a != 0.0) {
+                    const double Eji = E[j - 1] - E[i - 1];
+                    cool = cool + a * Eji * (S.y[j] / sumN) * s1 * s2 * kE;
+                }
+            }
+    }
+    __syncthreads();
+    return cool;   // valid on lane 0
+}
+
+__global__ void __launch_bounds__(64) balance_kernel(const Atomic *__restrict__ Ap, CellArgs a)
+{
+    __shared__ Lds S;
+    const Atomic &A = *Ap;
+    const int lane = threadIdx.x;
+    const int cell = blockIdx.x + 1;
+    if (cell > a.ncells) return;
+    const int nb = A.nbins, ns = A.nstages, nE = A.nE, nch = A.nch;
+    const size_t xyz = (size_t)a.cell_xyz[cell];
+    double *ionOut = a.ion + (size_t)cell * nE * (ns + 1);
+
+    // stage the cell's J row; hit test (update_mod.f90:104-154)
+    int hit = 0;
+    for (int j = lane; j < nb; j += W) {
+        const double J = a.J3[(size_t)j * a.ncell3 + xyz] * a.jscale;
+        S.J[j] = J;
+        if (J > 0.0) hit = 1;
+    }
+    if (!__any(hit)) {
+        if (lane == 0) { a.flags[cell] = 16; a.nIter[cell] = 0; }
+        return;
+    }
+    if (lane == 0) {
+        S.Te = a.Te[cell];
+        S.Ne = a.Ne[cell];
+        S.Hden = a.Hden[cell];
+    }
+    for (int i = lane; i < nE * (ns + 1); i += W) S.ion[i / (ns + 1)][i % (ns + 1)] = ionOut[i];
+    __syncthreads();
+    // photoionisation rates and heating integrals per channel
+    // (update_mod.f90:184-269); hiH is already cut at the first bin whose
+    // cross section is below 1e-35, where the reference's heating loop exits
+    const double hcRyd = 2.1799153e-11;
+    for (int k = 0; k < nch; ++k) {
+        const Channel &c = A.ch[k];
+        const double *sig = A.sigma + (size_t)k * nb;
+        double r = 0.0, h = 0.0;
+        const double nuIP = A.nu[c.ip - 1];
+        const int hi = c.hiR > c.hiH ? c.hiR : c.hiH;
+        for (int j = c.ip + lane; j <= hi; j += W) {
+            const double J = S.J[j - 1];
+            if (J > 0.0) {
+                const double ph = sig[j - 1];
+                if (j <= c.hiR && ph >= 1.e-35) r += J * ph / (hcRyd * A.nu[j - 1]);
+                if (j <= c.hiH) h += ph * J * (A.nu[j - 1] - nuIP) / A.nu[j - 1];
+            }
+        }
+        r = wave_sum(r);
+        h = wave_sum(h);
+        if (lane == 0) { S.nPhot[k] = 1.e-20 + r; S.heatNu[k] = h; }
+    }
+    __syncthreads();
+    const double XOldHI = S.ion[0][1];
+
+    int nIterT = 0, flags = 0, lgIBConv = 1;
+    double rHigh = 0., rLow = 0., Thigh = 0., Tlow = 0.;
+    for (;;) {
+        nIterT++;
+        const double Te = S.Te;
+        // calcAlpha: radiative + dielectronic (oxygen rows fill empty entries)
+        for (int i = 1 + lane; i < ns; i += W)
+            S.diO[i] = di_rec(A, A.odr_n[i], A.odr[i], A.oap[i], Te);
+        __syncthreads();
+        for (int k = lane; k < nch; k += W) {
+            const Channel &c = A.ch[k];
+            double al = rad_rec(c, Te);
+            if (c.elem >= 3) {
+                double d = di_rec(A, c.ndr, c.dr, c.ap, Te);
+                if (d == 0.0) d = S.diO[c.ion];
+                al = al + d;
+            }
+            S.alpha[k] = al;
+            // charge-exchange rate coefficient (ionBalance, 1448-1455)
+            const double t4 = Te / 10000.;
+            double cx = c.chex[0] * pow(t4, c.chex[1]) * (1. + c.chex[2] * exp(c.chex[3] * t4));
+            if (cx < 0.) cx = 0.;
+            if (Te < 6000. || Te > 5.e4) cx = 0.;
+            S.chexv[k] = cx;
+        }
+        __syncthreads();
+        // ionBalance fixed point (1320-1622)
+        int nIterX = 1;
+        for (;;) {
+            const double HIOld = S.ion[0][1], HeIOld = S.ion[1][1], HeIIOld = S.ion[1][2];
+            const double xH0 = S.ion[0][1];
+            for (int k = lane; k < nch; k += W) {
+                const Channel &c = A.ch[k];
+                double collIon = 0.0;
+                if (k == 0) {
+                    double expFact = 157893.94 / Te;
+                    if (expFact > 75.) expFact = 75.;
+                    collIon = 2.75E-16 * Te * sqrt(Te) * (157893.94 / Te + 2.) * exp(-expFact);
+                }
+                double revRate = 0.0;
+                if (c.dEk > 1.0)
+                    revRate = S.chexv[k] * (1. - xH0) * S.Hden * 2. * exp(-c.dEk / Te) / c.g0g1;
+                S.ratio[k] = (S.nPhot[k] + collIon + revRate) /
+                             (S.Ne * S.alpha[k] + S.chexv[k] * S.Hden * xH0);
+            }
+            __syncthreads();
+            if (lane < nE) {
+                const Element &e = A.el[lane];
+                double prod[kMaxStage];
+                double den = 1.0, p = 1.0;
+                for (int i = 0; i < e.nch; ++i) { p = 1.0; for (int q = 0; q <= i; ++q) p = p * S.ratio[e.ch0 + q]; prod[i] = p; }
+                for (int i = 0; i < e.nch; ++i) den = den + prod[i];
+                for (int st = 1; st <= e.nst; ++st) {
+                    double v = (st == 1) ? 1.0 / den : prod[st - 2] / den;
+                    if (v > 0.99999) v = 0.99999;
+                    if (v < 1.e-20) v = 1.e-20;
+                    S.ion[lane][st] = v;
+                }
+            }
+            __syncthreads();
+            if (lane == 0) {
+                double Ne = 0.0;
+                for (int ei = 0; ei < nE; ++ei)
+                    for (int st = 2; st <= A.el[ei].nst; ++st)
+                        if (S.ion[ei][st] >= 1.e-10) Ne = Ne + (double)(st - 1) * A.el[ei].abun * S.ion[ei][st];
+                Ne = Ne * S.Hden;
+                if (Ne == 0.0) Ne = 1.0;
+                S.Ne = Ne;
+            }
+            __syncthreads();
+            const double dHI = (S.ion[0][1] - HIOld) / HIOld;
+            const double dHeI = (S.ion[1][1] - HeIOld) / HeIOld;
+            const double dHeII = (S.ion[1][2] - HeIIOld) / HeIIOld;
+            const bool big = fabs(dHI) > 0.01 || fabs(dHeI) > 0.01 || fabs(dHeII) > 0.01;
+            if (big && nIterX < 25) { nIterX++; continue; }
+            lgIBConv = !(big && nIterX == 25);
+            break;
+        }
+        // thermBalance (933-1258)
+        double cool = 0.0, heat = 0.0;
+        for (int k = 0; k < A.ncel; ++k) {
+            double cc = cel_cool(A, A.cel[k], S, lane);
+            cool = cool + cc;
+        }
+        if (lane == 0) {
+            const double T = S.Te, Ne = S.Ne, kB = 1.380658e-16;
+            const double lg = log10(T), Te4 = T / 1.e4;
+            double Np = S.ion[0][2] * A.el[0].abun;
+            double bFF = 1.0108464E-11 + 9.7930778E-13 * lg - 6.6433144E-13 * lg * lg +
+                         2.4793747E-13 * lg * lg * lg - 2.3938215E-14 * lg * lg * lg * lg;
+            double coolFF = Np * Ne * bFF * kB * T / sqrt(T);
+            double bRec = 9.4255985E-11 - 4.04794384E-12 * lg - 1.0055237E-11 * lg * lg +
+                          1.99266862E-12 * lg * lg * lg - 1.06681387E-13 * lg * lg * lg * lg;
+            double coolRec = Np * Ne * bRec * kB * T / sqrt(T);
+            Np = S.ion[1][3] * A.el[1].abun;
+            const double ls = log10(T / 4.);
+            bFF = 2. * (1.0108464E-11 + 9.7930778E-13 * ls - 6.6433144E-13 * ls * ls +
+                        2.4793747E-13 * ls * ls * ls - 2.3938215E-14 * ls * ls * ls * ls);
+            coolFF = coolFF + Np * Ne * bFF * kB * T / sqrt(T / 4.);
+            bRec = 2. * (9.4255985E-11 - 4.04794384E-12 * ls - 1.0055237E-11 * ls * ls +
+                         1.99266862E-12 * ls * ls * ls - 1.06681387E-13 * ls * ls * ls * ls);
+            coolRec = coolRec + Np * Ne * bRec * kB * T / sqrt(T / 4.);
+            Np = S.ion[1][2] * A.el[1].abun;
+            bFF = 1.070073e-11 - 2.5730207e-13 * lg + 2.109134e-13 * lg * lg;
+            coolFF = coolFF + Np * Ne * bFF * kB * T / sqrt(T);
+            bRec = 9.4255985E-11 - 4.04794384E-12 * lg - 1.0055237E-11 * lg * lg +
+                   1.99266862E-12 * lg * lg * lg - 1.06681387E-13 * lg * lg * lg * lg;
+            coolRec = coolRec + Np * Ne * bRec * kB * T / sqrt(T);
+            double coolColl = 0.0;
+            if (T > 5000.)
+                coolColl = (2.47e-8 * exp(-118338. / T) * pow(Te4, -0.228) +
+                            1.32e-8 * exp(-140252. / T) * pow(Te4, -0.460)) *
+                           hcRyd * S.ion[0][1] * Ne;
+            coolColl = coolColl + cool;
+            for (int k = 0; k < nch; ++k) {
+                const Channel &c = A.ch[k];
+                heat = heat + S.heatNu[k] * S.ion[c.eidx][c.ion] * A.el[c.eidx].abun;
+            }
+            S.red[0] = heat;
+            S.red[1] = coolFF + coolRec + coolColl;
+        }
+        __syncthreads();
+        heat = S.red[0];
+        cool = S.red[1];
+        __syncthreads();
+        // iterateT false-position update (398-458)
+        const double res = heat - cool;
+        double Tn = S.Te;
+        bool again = false;
+        if (fabs(res) >= 0.01 * heat) {
+            if (nIterT < 25) {
+                if (res < 0.) {
+                    rLow = res; Tlow = Tn;
+                    Tn = (Thigh != 0.) ? Tlow - (Thigh - Tlow) * rLow / (rHigh - rLow) : Tn / 1.2;
+                } else {
+                    rHigh = res; Thigh = Tn;
+                    Tn = (Tlow != 0.) ? Tlow - (Thigh - Tlow) * rLow / (rHigh - rLow) : Tn * 1.2;
+                }
+                if (nIterT >= 6 && fabs(Thigh - Tlow) <= 0.002 * (Thigh + Tlow)) {
+                    if (res < 0.) { Tn = Thigh - 100.; Thigh = 0.; }
+                    else { Tn = Tlow + 100.; Tlow = 0.; }
+                }
+                if (Tn <= 0.) Tn = 1.;
+                again = true;
+            } else {
+                if (Thigh == 0.) Tn = Tlow;
+                else if (Tlow == 0.) Tn = Thigh;
+                else Tn = (Thigh + Tlow) * 0.5;
+                flags |= 2 | 4;
+            }
+        }
+        __syncthreads();
+        if (lane == 0) S.Te = Tn;
+        __syncthreads();
+        if (!again) break;
+    }
+    if (!lgIBConv) flags |= 2 | 8;
+    const double dXHI = (S.ion[0][1] - XOldHI) / XOldHI;
+    if (fabs(dXHI) <= (double)A.XHILimit) flags |= 1;
+    for (int i = lane; i < nE * (ns + 1); i += W) ionOut[i] = S.ion[i / (ns + 1)][i % (ns + 1)];
+    if (lane == 0) {
+        a.Te[cell] = S.Te;
+        a.Ne[cell] = S.Ne;
+        a.flags[cell] = flags;
+        a.nIter[cell] = nIterT;
+    }
+}
+
+}  // namespace mcb
+
+hipError_t mcb_launch_balance(const mcb::Atomic *dA, const mcb::CellArgs &a, hipStream_t s)
+{
+    if (a.ncells <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mcb::balance_kernel, dim3(a.ncells), dim3(64), 0, s, dA, a);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ host tables
+namespace mcb {
+namespace {
+
+// Charge-exchange fits (elem, ion, a, b, c, d) with a in units of 1e-9
+// cm^3 s^-1, in the order ionBalance assigns them (update_mod.f90:1389-1446):
+// the second Ar II/III pair replaces the first, as there.
+struct ChexRow { int elem, ion; double a, b, c, d; };
+const ChexRow kChex[] = {
+    {2, 1, 7.47e-6, 2.06, 9.93, -3.89},    {2, 2, 1.e-5, 0., 0., 0.},
+    {3, 2, 1.26, 0.96, 3.02, -0.65},       {3, 3, 1.e-5, 0., 0., 0.},
+    {4, 2, 1.e-5, 0., 0., 0.},             {4, 3, 1.e-5, 0., 0., 0.},
+    {4, 4, 5.17, 0.82, -.69, -1.12},       {5, 2, 2.e-2, 0., 0., 0.},
+    {5, 3, 1.e-5, 0., 0., 0.},             {5, 4, 5.27e-1, 0.76, -0.63, -1.17},
+    {6, 1, 1.76e-9, 8.33, 4278.78, -6.41}, {6, 2, 1.67e-4, 2.79, 304.72, -4.07},
+    {6, 3, 3.25, 0.21, 0.19, -3.29},       {6, 4, 332.46, -0.11, -0.995, -1.58e-3},
+    {7, 1, 1.01e-3, -0.29, -0.92, -8.38},  {7, 2, 3.05e-1, 0.60, 2.65, -0.93},
+    {7, 3, 4.54, 0.57, -0.65, -0.89},      {7, 4, 3.28, 0.52, -0.52, -0.19},
+    {8, 1, 1.04, 3.15e-2, -0.61, -9.73},   {8, 2, 1.04, 0.27, 2.02, -5.92},
+    {8, 3, 3.98, 0.26, 0.56, -2.62},       {8, 4, 2.52e-1, 0.63, 2.08, -4.16},
+    {9, 2, 1.e-5, 0., 0., 0.},             {9, 3, 9.86, 0.29, -0.21, -1.15},
+    {9, 4, 7.15e-1, 1.21, -0.70, -0.85},   {10, 2, 1.e-5, 0., 0., 0.},
+    {10, 3, 14.73, 4.52e-2, -0.84, -0.31}, {10, 4, 6.47, 0.54, 3.59, -5.22},
+    {11, 2, 1.e-5, 0., 0., 0.},            {11, 3, 1.33, 1.15, 1.20, -0.32},
+    {11, 4, 1.01e-1, 1.34, 10.05, -6.41},  {12, 2, 8.58e-5, 2.49e-3, 2.93e-2, -4.33},
+    {12, 3, 6.49, 0.53, 2.82, -7.63},      {12, 4, 6.36, 0.55, 3.86, -5.19},
+    {13, 2, 1.e-5, 0., 0., 0.},            {13, 3, 7.11e-5, 4.12, 1.72e4, -22.24},
+    {13, 4, 7.52e-1, 0.77, 6.24, -5.67},   {14, 2, 1.23, 0.24, 3.17, 4.18e-3},
+    {14, 3, 4.900e-1, -8.74e-2, -0.36, -0.79}, {14, 4, 7.58, 0.37, 1.06, -4.09},
+    {16, 1, 3.82e-7, 11.10, 2.57e4, -8.22}, {16, 2, 1.e-5, 0., 0., 0.},
+    {16, 3, 2.29, 4.02e-2, 1.59, -6.06},   {16, 4, 6.44, 0.13, 2.69, -5.69},
+    {18, 2, 1.e-5, 0., 0., 0.},            {18, 3, 4.57, 0.27, -0.18, -1.57},
+    {18, 4, 6.37, 2.12, 10.21, -6.22},     {18, 3, 3.17e-2, 2.12, 12.06, -0.40},
+    {18, 4, 2.68, 0.69, -0.68, -4.47},     {26, 2, 1.26, 7.72e-2, -0.41, -7.31},
+    {26, 3, 3.42, 0.51, -2.06, -8.99},
+};
+
+int err(char *e, size_t n, int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(e, n, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+// natural cubic spline second derivatives (spline, interpolation_mod.f90:
+// 182-266, with zero end derivatives of second order)
+void spline_nat(const double *xa, const double *ya, int n, double *y2)
+{
+    double u[kMaxTemp];
+    y2[0] = 0.0;
+    u[0] = 0.0;
+    for (int i = 1; i < n - 1; ++i) {
+        const double sig = (xa[i] - xa[i - 1]) / (xa[i + 1] - xa[i - 1]);
+        const double p = sig * y2[i - 1] + 2.0;
+        y2[i] = (sig - 1.0) / p;
+        double v = ya[i + 1];
+        v = v - ya[i];
+        v = v / (xa[i + 1] - xa[i]);
+        v = v - (ya[i] - ya[i - 1]) / (xa[i] - xa[i - 1]);
+        u[i] = (6.0 * v / (xa[i + 1] - xa[i - 1]) - sig * u[i - 1]) / p;
+    }
+    y2[n - 1] = 0.0;
+    for (int k = n - 2; k >= 0; --k) y2[k] = y2[k] * y2[k + 1] + u[k];
+}
+
+// dielectronic rows (direc_coeffs, aldropequi_coeffs) that fill (z, ion)
+// (dielectronic, update_mod.f90:1782-1834); returns their number, or -1
+// with a message in e
+int dr_rows(const mc_atomic *in, int ns, int z, int ion, int *rows, int *ap, char *e, size_t n)
+{
+    int nr = 0;
+    for (int i = 0; i < 25; ++i) {
+        const int el = in->dr_int[3 * i], nn = in->dr_int[3 * i + 1];
+        const int io = el + 1 - nn;
+        if (el != z || io != ion || io > ns || io < 1) continue;
+        if (nr == kMaxDr) {
+            err(e, n, MC_ETABLE, "more than %d dielectronic rows for Z=%d ion %d", kMaxDr, z, ion);
+            return -1;
+        }
+        rows[nr++] = i;
+    }
+    *ap = -1;
+    for (int i = 0; i < 167; ++i) {
+        const int el = in->ap_int[2 * i], nn = in->ap_int[2 * i + 1];
+        const int io = el + 1 - nn;
+        if (el == z && io == ion && io >= 1 && io <= ns) *ap = i;
+    }
+    return nr;
+}
+
+}  // namespace
+
+int prepare(const mc_atomic *in, int nb, const float *nuArray, Prepared &P, char *e, size_t n)
+{
+    if (!in || !in->elemOn || !in->xref || !in->abun || !in->ch_meta || !in->ch_sigma || !in->rrec ||
+        !in->rnew || !in->fe || !in->dr_int || !in->dr_real || !in->ap_int || !in->ap_real ||
+        (in->ncel > 0 && (!in->cel_meta || !in->cel_dbl || !in->cel_flt)))
+        return err(e, n, MC_EINVAL, "atomic descriptor has null arrays");
+    const int ns = in->nstages;
+    if (ns < 2 || ns > kMaxStage - 1) return err(e, n, MC_EUNSUP, "nstages %d outside 2..%d", ns, kMaxStage - 1);
+    if (nb > kMaxNb) return err(e, n, MC_EUNSUP, "nbins %d above %d", nb, kMaxNb);
+    if (!in->elemOn[0] || !in->elemOn[1]) return err(e, n, MC_EUNSUP, "H and He must be switched on");
+    Atomic &A = P.A;
+    A = Atomic{};
+    A.nbins = nb;
+    A.nstages = ns;
+    A.XHILimit = in->XHILimit;
+    P.nu.assign(nuArray, nuArray + nb);
+    // elements in atomic-number order, their channels ion = 1..min(z, ns-1)
+    int eidx_of_z[31];
+    for (int z = 0; z <= 30; ++z) eidx_of_z[z] = -1;
+    int nE = 0, nch = 0;
+    P.eidx_xref.clear();
+    for (int z = 1; z <= 30; ++z) {
+        if (!in->elemOn[z - 1]) continue;
+        if (nE == kMaxElem) return err(e, n, MC_EUNSUP, "more than %d elements", kMaxElem);
+        const int xr = in->xref[z - 1];
+        if (xr < 1 || xr > in->nElemUsed) return err(e, n, MC_ETABLE, "elementXref(%d) = %d", z, xr);
+        Element &E = A.el[nE];
+        E.z = z;
+        E.nst = std::min(z + 1, ns);
+        E.ch0 = nch;
+        E.nch = std::min(z, ns - 1);
+        E.abun = (double)in->abun[z - 1];
+        if (nch + E.nch > kMaxCh) return err(e, n, MC_EUNSUP, "more than %d channels", kMaxCh);
+        for (int ion = 1; ion <= E.nch; ++ion) {
+            Channel &c = A.ch[nch];
+            c.elem = z; c.ion = ion; c.eidx = nE;
+            c.ip = 1; c.hiR = 0; c.hiH = 0;
+            int g0 = 0, g1 = 0, row = -1;
+            for (int k = 0; k < in->nch; ++k)
+                if (in->ch_meta[8 * k] == z && in->ch_meta[8 * k + 1] == ion) row = k;
+            std::vector<double> sig(nb, 0.0);
+            if (row >= 0) {
+                const int32_t *m = in->ch_meta + 8 * row;
+                g0 = m[2]; g1 = m[3];
+                c.ip = m[4]; c.hiR = m[5]; c.hiH = m[6];
+                if (c.ip < 1 || c.ip > nb || c.hiR > nb || c.hiH > nb)
+                    return err(e, n, MC_ETABLE, "channel Z=%d ion %d: bins %d..%d/%d outside 1..%d", z, ion,
+                               c.ip, c.hiR, c.hiH, nb);
+                for (int j = 0; j < nb; ++j) sig[j] = (double)in->ch_sigma[(size_t)row * nb + j];
+                // the heating integral stops at the first cross section below 1e-35
+                for (int j = c.ip; j <= c.hiH; ++j)
+                    if (sig[j - 1] < 1.e-35) { c.hiH = j - 1; break; }
+            }
+            P.sigma.insert(P.sigma.end(), sig.begin(), sig.end());
+            c.g0g1 = (double)g0 / (double)g1;
+            // radiative recombination fit (radRecFit, update_mod.f90:1674-1780)
+            const int nn = z - ion + 1;
+            if (nn <= 3 || nn == 11 || (z > 5 && z < 9) || z == 10 || (z == 26 && nn > 11)) {
+                c.rr_kind = 0;
+                for (int i = 0; i < 4; ++i) c.rr[i] = (double)in->rnew[(z - 1) * 120 + (nn - 1) * 4 + i];
+            } else if (z == 26 && nn <= 13) {
+                c.rr_kind = 2;
+                for (int i = 0; i < 3; ++i) c.rr[i] = (double)in->fe[(nn - 4) * 3 + i];
+            } else {
+                c.rr_kind = 1;
+                for (int i = 0; i < 2; ++i) c.rr[i] = (double)in->rrec[(z - 1) * 60 + (nn - 1) * 2 + i];
+            }
+            const int nr = dr_rows(in, ns, z, ion, c.dr, &c.ap, e, n);
+            if (nr < 0) return MC_ETABLE;
+            c.ndr = nr;
+            // charge exchange
+            for (const ChexRow &r : kChex)
+                if (r.elem == z && r.ion == ion) {
+                    c.chex[0] = r.a * 1.e-9; c.chex[1] = r.b; c.chex[2] = r.c; c.chex[3] = r.d;
+                }
+            c.dEk = (z == 7 && ion == 1) ? 10863. : (z == 8 && ion == 1) ? 2205. : 0.;
+            ++nch;
+        }
+        eidx_of_z[z] = nE;
+        P.eidx_xref.push_back(xr);
+        ++nE;
+    }
+    A.nE = nE;
+    A.nch = nch;
+    if (A.ch[0].elem != 1 || A.ch[0].ion != 1) return err(e, n, MC_EUNSUP, "first channel is not H I");
+    for (int ion = 1; ion < ns; ++ion) {
+        const int nr = dr_rows(in, ns, 8, ion, A.odr[ion], &A.oap[ion], e, n);
+        if (nr < 0) return MC_ETABLE;
+        A.odr_n[ion] = nr;
+    }
+    for (int i = 0; i < 25; ++i) {
+        for (int k = 0; k < 5; ++k) A.dr_real[i][k] = (double)in->dr_real[5 * i + k];
+        A.dr_g[i] = in->dr_int[3 * i + 2];
+    }
+    for (int i = 0; i < 167; ++i)
+        for (int k = 0; k < 4; ++k) A.ap_real[i][k] = (double)in->ap_real[4 * i + k];
+    // CEL model ions
+    if (in->ncel < 0 || in->ncel > kMaxCel) return err(e, n, MC_EUNSUP, "more than %d CEL ions", kMaxCel);
+    A.ncel = in->ncel;
+    size_t od = 0, of = 0;
+    for (int k = 0; k < in->ncel; ++k) {
+        const int32_t *m = in->cel_meta + 6 * k;
+        const int z = m[0], ion = m[1], nl = m[4], nt = m[5];
+        if (z < 1 || z > 30 || eidx_of_z[z] < 0 || ion < 1 || ion > ns)
+            return err(e, n, MC_ETABLE, "CEL ion %d: Z=%d ion %d not switched on", k, z, ion);
+        if (nl < 2 || nl > kMaxLev || nt < 1 || nt > kMaxTemp)
+            return err(e, n, MC_EUNSUP, "CEL ion %d: %d levels, %d temperatures", k, nl, nt);
+        const double *ev = in->cel_dbl + od;
+        const double *av = ev + nl;
+        const double *qv = av + (size_t)nl * nl;
+        const float *lt = in->cel_flt + of;
+        const float *gv = lt + nt;
+        CelIon &ci = A.cel[k];
+        ci.eidx = eidx_of_z[z]; ci.ion = ion; ci.nl = nl; ci.nt = nt;
+        ci.br = (double)gv[nl];
+        ci.z = (double)gv[nl + 1];
+        for (int i = 0; i < 4; ++i) ci.a_r[i] = (double)gv[nl + 2 + i];
+        for (int i = 0; i < 5; ++i) ci.a_d[i] = (double)gv[nl + 6 + i];
+        ci.off_e = (int)P.cel_e.size();
+        ci.off_a = (int)P.cel_a.size();
+        ci.off_q = (int)P.cel_q.size();
+        ci.off_t = (int)P.cel_t.size();
+        ci.off_g = (int)P.cel_g.size();
+        P.cel_e.insert(P.cel_e.end(), ev, ev + nl);
+        P.cel_a.insert(P.cel_a.end(), av, av + (size_t)nl * nl);
+        std::vector<double> xa(nt);
+        for (int t = 0; t < nt; ++t) xa[t] = (double)lt[t];
+        for (int t = 1; t < nt; ++t)
+            if (!(xa[t] > xa[t - 1])) return err(e, n, MC_ETABLE, "CEL ion %d: log T not increasing", k);
+        P.cel_t.insert(P.cel_t.end(), xa.begin(), xa.end());
+        for (int i = 0; i < nl; ++i) P.cel_g.push_back((double)gv[i]);
+        // collision strengths of each pair (lo < up) and their spline curvature
+        double qq[kMaxTemp], y2[kMaxTemp];
+        for (int lo = 1; lo <= nl - 1; ++lo)
+            for (int up = lo + 1; up <= nl; ++up) {
+                for (int t = 0; t < nt; ++t) qq[t] = qv[t + (size_t)nt * ((lo - 1) + (size_t)nl * (up - 1))];
+                if (nt >= 3) spline_nat(xa.data(), qq, nt, y2);
+                else for (int t = 0; t < nt; ++t) y2[t] = 0.0;
+                P.cel_q.insert(P.cel_q.end(), qq, qq + nt);
+                P.cel_y2.insert(P.cel_y2.end(), y2, y2 + nt);
+            }
+        od += (size_t)nl + (size_t)nl * nl + (size_t)nt * nl * nl;
+        of += (size_t)nt + nl + 2 + 4 + 5;
+    }
+    return MC_OK;
+}
+
+}  // namespace mcb

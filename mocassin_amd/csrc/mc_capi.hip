@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/mocassin_amd.h"
+#include "mc_balance.h"
 #include "mc_device.h"
 #include "mc_kernels.h"
 
@@ -37,6 +38,15 @@ struct mc_ctx {
     unsigned long long *d_scratch = nullptr;  // next_packet, counts[8]
     int *d_error = nullptr;
     double *d_qphot = nullptr;
+    // per-cell balance
+    std::vector<int> active, cell_xyz;      // host copies: active map, xyz of each cell
+    int *d_cell_xyz = nullptr;
+    bool balance = false;
+    mcb::Prepared bal;
+    mcb::Atomic *d_atomic = nullptr;
+    double *d_bal_tab = nullptr;             // sigma, nu, CEL arrays (one allocation)
+    double *d_Hden = nullptr, *d_Te = nullptr, *d_Ne = nullptr, *d_ion = nullptr;
+    int *d_flags = nullptr, *d_nIter = nullptr;
     char err[512] = "";
 };
 
@@ -129,6 +139,10 @@ static int ctx_init(mc_ctx *c, const mc_desc *d)
         if (d->active[i] < 0 || d->active[i] > nc)
             return fail(c, d->active[i] < 0 ? MC_EUNSUP : MC_EINVAL,
                         "active map entry %zu = %d outside [0,%d]", i, d->active[i], nc);
+    c->active.assign(d->active, d->active + (size_t)nx * ny * nz);
+    c->cell_xyz.assign((size_t)nc + 1, -1);
+    for (size_t i = 0; i < (size_t)nx * ny * nz; ++i)
+        if (d->active[i] > 0) c->cell_xyz[d->active[i]] = (int)i;
     c->nStars = d->nStars;
     c->starPos.assign(d->starPos, d->starPos + 3 * d->nStars);
     c->starIdx.assign(d->starIdx, d->starIdx + 3 * d->nStars);
@@ -292,6 +306,133 @@ extern "C" int mc_fetch_estimators(mc_ctx *c, double *J_out, double *esc_out)
     return MC_OK;
 }
 
+extern "C" int mc_set_estimators(mc_ctx *c, const double *J, const double *esc)
+{
+    if (!c) return MC_EINVAL;
+    if (!J) return fail(c, MC_EINVAL, "J is required");
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t nc1 = (size_t)c->G.nCells + 1, nb = c->G.nbins, n3 = c->G.ncell3;
+    std::vector<double> dense(n3 * nb);
+    for (size_t nu = 0; nu < nb; ++nu)
+        for (size_t i = 0; i < n3; ++i) {
+            const int a = c->active[i];
+            dense[nu * n3 + i] = a > 0 ? J[nu * nc1 + a] : 0.0;
+        }
+    HIPCHK(c, hipMemcpy(c->d_J3, dense.data(), n3 * nb * sizeof(double), hipMemcpyHostToDevice));
+    if (esc) HIPCHK(c, hipMemcpy(c->d_esc, esc, nc1 * (nb + 1) * sizeof(double), hipMemcpyHostToDevice));
+    return MC_OK;
+}
+
+extern "C" int mc_balance_setup(mc_ctx *c, const mc_atomic *at)
+{
+    if (!c) return MC_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    for (int cell = 1; cell <= c->G.nCells; ++cell)
+        if (c->cell_xyz[cell] < 0) return fail(c, MC_EINVAL, "active map has no position for cell %d", cell);
+    c->balance = false;
+    std::vector<float> nu(c->G.nbins);
+    HIPCHK(c, hipMemcpy(nu.data(), c->d_nuArray, nu.size() * sizeof(float), hipMemcpyDeviceToHost));
+    mcb::Prepared &P = c->bal;
+    c->err[0] = 0;
+    int rc = mcb::prepare(at, c->G.nbins, nu.data(), P, c->err, sizeof c->err);
+    if (rc) return rc;
+    if (at->nElemUsed != P.A.nE) return fail(c, MC_ETABLE, "nElemUsed %d but %d elements on", at->nElemUsed, P.A.nE);
+    // one device block for every table, in this order
+    const std::vector<double> *parts[] = { &P.sigma, &P.nu, &P.cel_e, &P.cel_a, &P.cel_q,
+                                           &P.cel_y2, &P.cel_t, &P.cel_g };
+    size_t tot = 0;
+    for (auto *v : parts) tot += v->size();
+    if (c->d_bal_tab) { (void)hipFree(c->d_bal_tab); c->d_bal_tab = nullptr; }
+    HIPCHK(c, dalloc(&c->d_bal_tab, tot));
+    const double *ptr[8];
+    size_t off = 0;
+    for (int k = 0; k < 8; ++k) {
+        ptr[k] = c->d_bal_tab + off;
+        if (!parts[k]->empty())
+            HIPCHK(c, hipMemcpy(c->d_bal_tab + off, parts[k]->data(), parts[k]->size() * sizeof(double),
+                                hipMemcpyHostToDevice));
+        off += parts[k]->size();
+    }
+    mcb::Atomic A = P.A;
+    A.sigma = ptr[0]; A.nu = ptr[1]; A.cel_e = ptr[2]; A.cel_a = ptr[3]; A.cel_q = ptr[4];
+    A.cel_y2 = ptr[5]; A.cel_t = ptr[6]; A.cel_g = ptr[7];
+    if (!c->d_atomic) HIPCHK(c, dalloc(&c->d_atomic, 1));
+    HIPCHK(c, hipMemcpy(c->d_atomic, &A, sizeof A, hipMemcpyHostToDevice));
+    const size_t nc1 = (size_t)c->G.nCells + 1;
+    const size_t nion = nc1 * P.A.nE * (P.A.nstages + 1);
+    if (!c->d_cell_xyz) {
+        HIPCHK(c, dalloc(&c->d_cell_xyz, nc1));
+        HIPCHK(c, hipMemcpy(c->d_cell_xyz, c->cell_xyz.data(), nc1 * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(c, dalloc(&c->d_Hden, nc1));
+        HIPCHK(c, dalloc(&c->d_Te, nc1));
+        HIPCHK(c, dalloc(&c->d_Ne, nc1));
+        HIPCHK(c, dalloc(&c->d_flags, nc1));
+        HIPCHK(c, dalloc(&c->d_nIter, nc1));
+    }
+    if (c->d_ion) { (void)hipFree(c->d_ion); c->d_ion = nullptr; }
+    HIPCHK(c, dalloc(&c->d_ion, nion));
+    c->balance = true;
+    return MC_OK;
+}
+
+extern "C" int mc_balance_run(mc_ctx *c, double jscale, const double *Hden, double *Te, double *Ne,
+                              double *ionDen, int32_t *flags, int32_t *nIter, double *kernel_ms)
+{
+    if (!c) return MC_EINVAL;
+    if (!c->balance) return fail(c, MC_ESTATE, "mc_balance_setup has not been called");
+    if (!Hden || !Te || !Ne || !ionDen || !flags || !nIter) return fail(c, MC_EINVAL, "null array");
+    if (!(jscale > 0.0) || std::isinf(jscale)) return fail(c, MC_EINVAL, "jscale must be finite and > 0");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int nc = c->G.nCells, nE = c->bal.A.nE, ns = c->bal.A.nstages;
+    const size_t nc1 = (size_t)nc + 1;
+    for (int cell = 1; cell <= nc; ++cell)
+        if (!(Te[cell] > 0.0) || !(Hden[cell] > 0.0) || !(Ne[cell] > 0.0))
+            return fail(c, MC_EINVAL, "cell %d: Te, Ne and Hden must be > 0", cell);
+    // ionDen(0:nCells, xref, stage) -> [cell][element index][stage 0..ns]
+    std::vector<double> ion(nc1 * nE * (ns + 1), 0.0);
+    for (size_t cell = 0; cell < nc1; ++cell)
+        for (int ei = 0; ei < nE; ++ei) {
+            const size_t xr = (size_t)c->bal.eidx_xref[ei] - 1;
+            for (int st = 1; st <= ns; ++st)
+                ion[(cell * nE + ei) * (ns + 1) + st] = ionDen[cell + nc1 * (xr + (size_t)nE * (st - 1))];
+        }
+    HIPCHK(c, hipMemcpyAsync(c->d_Hden, Hden, nc1 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_Te, Te, nc1 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_Ne, Ne, nc1 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_ion, ion.data(), ion.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_flags, 0, nc1 * sizeof(int), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_nIter, 0, nc1 * sizeof(int), c->stream));
+    mcb::CellArgs a;
+    a.ncells = nc;
+    a.jscale = jscale;
+    a.J3 = c->d_J3;
+    a.ncell3 = c->G.ncell3;
+    a.cell_xyz = c->d_cell_xyz;
+    a.Hden = c->d_Hden; a.Te = c->d_Te; a.Ne = c->d_Ne; a.ion = c->d_ion;
+    a.flags = c->d_flags; a.nIter = c->d_nIter;
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, mcb_launch_balance(c->d_atomic, a, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipMemcpyAsync(Te, c->d_Te, nc1 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(Ne, c->d_Ne, nc1 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(ion.data(), c->d_ion, ion.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(flags, c->d_flags, nc1 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(nIter, c->d_nIter, nc1 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (kernel_ms) {
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        *kernel_ms = ms;
+    }
+    for (size_t cell = 0; cell < nc1; ++cell)
+        for (int ei = 0; ei < nE; ++ei) {
+            const size_t xr = (size_t)c->bal.eidx_xref[ei] - 1;
+            for (int st = 1; st <= ns; ++st)
+                ionDen[cell + nc1 * (xr + (size_t)nE * (st - 1))] = ion[(cell * nE + ei) * (ns + 1) + st];
+        }
+    return MC_OK;
+}
+
 extern "C" int mc_set_tuning(mc_ctx *c, const mc_tuning *t)
 {
     if (!c || !t) return MC_EINVAL;
@@ -431,7 +572,9 @@ extern "C" void mc_ctx_destroy(mc_ctx *c)
     (void)hipSetDevice(c->device);
     void *bufs[] = { c->d_axis, c->d_mid, c->d_dfac, c->d_active, c->d_opac3, c->d_sca3,
                      c->d_recCDF, c->d_recGuide, c->d_totalLines, c->d_specCDF, c->d_nuArray,
-                     c->d_gSca, c->d_J3, c->d_J, c->d_esc, c->d_scratch, c->d_error, c->d_qphot };
+                     c->d_gSca, c->d_J3, c->d_J, c->d_esc, c->d_scratch, c->d_error, c->d_qphot,
+                     c->d_cell_xyz, c->d_atomic, c->d_bal_tab, c->d_Hden, c->d_Te, c->d_Ne,
+                     c->d_ion, c->d_flags, c->d_nIter };
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

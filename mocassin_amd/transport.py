@@ -102,6 +102,61 @@ class Transport:
         self._check(self._lib.mc_fetch_estimators(self._h, J.ctypes.data, E.ctypes.data))
         return J, E
 
+    def set_estimators(self, J, esc=None):
+        """Overwrite the device J (nCells+1, nbins) [and esc (nCells+1, nbins+1)]."""
+        J = np.asarray(J, np.float64)
+        if J.shape != (self.nCells + 1, self.nbins):
+            raise ValueError("J of shape %s" % (J.shape,))
+        Jf = np.asfortranarray(J)
+        Ef = None if esc is None else np.asfortranarray(np.asarray(esc, np.float64))
+        if Ef is not None and Ef.shape != (self.nCells + 1, self.nbins + 1):
+            raise ValueError("esc of shape %s" % (Ef.shape,))
+        self._check(self._lib.mc_set_estimators(self._h, Jf.ctypes.data,
+                                                None if Ef is None else Ef.ctypes.data))
+
+    def balance_setup(self, bundle):
+        """Upload the atomic inputs of the balance (mocassin_amd.atomic.AtomicBundle)."""
+        b = bundle
+        keep = {}
+
+        def arr(name, dt):
+            a = np.ascontiguousarray(getattr(b, name), dt)
+            keep[name] = a
+            return a.ctypes.data
+        A = _capi.McAtomic()
+        A.nstages, A.nElemUsed = b.nstages, b.nElemUsed
+        A.elemOn, A.xref, A.abun = arr("elem_on", np.int32), arr("xref", np.int32), arr("abun", np.float32)
+        A.nch, A.ch_meta, A.ch_sigma = b.nch, arr("ch_meta", np.int32), arr("ch_sigma", np.float32)
+        A.rrec, A.rnew, A.fe = arr("rrec", np.float32), arr("rnew", np.float32), arr("fe", np.float32)
+        A.dr_int, A.dr_real = arr("dr_int", np.int32), arr("dr_real", np.float32)
+        A.ap_int, A.ap_real = arr("ap_int", np.int32), arr("ap_real", np.float32)
+        A.ncel, A.cel_meta = b.ncel, arr("cel_meta", np.int32)
+        A.cel_dbl, A.cel_flt = arr("cel_dbl", np.float64), arr("cel_flt", np.float32)
+        A.XHILimit = b.XHILimit
+        if keep["ch_sigma"].size != b.nch * self.nbins:
+            raise ValueError("cross sections not on this frequency mesh")
+        self._nE, self._ns = b.nElemUsed, b.nstages
+        self._check(self._lib.mc_balance_setup(self._h, C.byref(A)))
+
+    def update_cells(self, Hden, Te, Ne, ionDen, jscale=1.0):
+        """updateCell for every active cell from the device J times jscale.
+        Hden/Te/Ne (nCells+1,), ionDen (nCells+1, nElemUsed, nstages).
+        Returns (Te, Ne, ionDen, flags, nIter, kernel_ms); flags bit0
+        converged, bit1 black, bit2 no Te balance, bit3 no ion balance, bit4
+        not hit."""
+        nc1 = self.nCells + 1
+        Te = np.array(Te, np.float64).reshape(nc1)
+        Ne = np.array(Ne, np.float64).reshape(nc1)
+        Hd = np.ascontiguousarray(Hden, np.float64).reshape(nc1)
+        ion = np.asfortranarray(np.array(ionDen, np.float64).reshape(nc1, self._nE, self._ns))
+        flags = np.zeros(nc1, np.int32)
+        nIt = np.zeros(nc1, np.int32)
+        ms = C.c_double()
+        self._check(self._lib.mc_balance_run(self._h, float(jscale), Hd.ctypes.data, Te.ctypes.data,
+                                             Ne.ctypes.data, ion.ctypes.data, flags.ctypes.data,
+                                             nIt.ctypes.data, C.byref(ms)))
+        return Te, Ne, np.ascontiguousarray(ion), flags, nIt, ms.value
+
     def walk_replay(self, pos, dirv, cell, max_len=2048):
         pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
         dirv = np.ascontiguousarray(dirv, np.float32).reshape(-1, 3)

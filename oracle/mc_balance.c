@@ -48,6 +48,7 @@ int mco_atomic_init(mco_atomic *A)
 #define SQRTR sqrtf
 #define POWR powf
 #define FABSR fabsf
+#define QCAST float   /* qq(iT) = real(qom(...)), emission_mod.f90:2008 */
 static R pow10m12_f32(void) { return 1.e-12f; }
 #include "mc_balance_impl.h"
 #undef R
@@ -58,6 +59,7 @@ static R pow10m12_f32(void) { return 1.e-12f; }
 #undef SQRTR
 #undef POWR
 #undef FABSR
+#undef QCAST
 #undef NFOR
 
 /* ---- FP64 oracle ---- */
@@ -69,5 +71,6 @@ static R pow10m12_f32(void) { return 1.e-12f; }
 #define SQRTR sqrt
 #define POWR pow
 #define FABSR fabs
+#define QCAST double
 static R pow10m12_f64(void) { return 1.e-12; }
 #include "mc_balance_impl.h"

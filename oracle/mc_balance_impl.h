@@ -265,9 +265,9 @@ static void FN(equilibrium)(const mco_atomic *A, int k, R ionDenUp, R Te, R Ne, 
         for (int j = i; j <= nl; ++j) {
             R qomInt;
             if (nt == 1) {
-                qomInt = (R)(float)QOM(1, i - 1, j);
+                qomInt = (R)(QCAST)QOM(1, i - 1, j);
             } else if (nt == 2) {
-                R q1 = (R)(float)QOM(1, i - 1, j), q2 = (R)(float)QOM(2, i - 1, j);
+                R q1 = (R)(QCAST)QOM(1, i - 1, j), q2 = (R)(QCAST)QOM(2, i - 1, j);
                 qomInt = q1 + (q2 - q1) / ((R)logT[1] - (R)logT[0]) * (log10Te - (R)logT[0]);
             } else if (sizeof(R) == sizeof(float)) {
                 float qq[64], qq2[64];
