@@ -1,0 +1,233 @@
+! mocassin_amd_mod.f90 - Fortran host interface to libmocassin_amd.so
+! (include/mocassin_amd.h) through ISO_C_BINDING.
+!
+! Two layers:
+!   * the raw bind(C) interfaces and interoperable types, one per C entry
+!     point, for hosts that manage their own c_ptr/c_loc plumbing;
+!   * host-side wrappers shaped like the reference's call sites, which take
+!     Fortran arrays in the reference's own layouts and turn a non-zero
+!     status into the reference's fail-stop (print + stop):
+!       gpu_energy_packet_driver  ~ energyPacketDriver(iStar, n, grid)
+!                                    (photon_mod.f90:26-284, called from
+!                                    iterateMC, iteration_mod.f90:488-493)
+!       gpu_update_cells          ~ the per-cell updateCell loop of iterateMC
+!                                    (iteration_mod.f90:826-837) with its
+!                                    lgConverged/lgBlack bookkeeping
+!                                    (iteration_mod.f90:848-876)
+! REAL arrays of the host are passed to the C side as they are; the balance
+! state is widened to double on the way in and narrowed back on the way out.
+module mocassin_amd_mod
+    use, intrinsic :: iso_c_binding
+    implicit none
+
+    integer(c_int32_t), parameter :: MC_ABI_VERSION = 1
+    integer(c_int32_t), parameter :: MC_SYMMETRIC = 1, MC_DUST = 2, MC_GAS = 4, &
+                                     MC_ISOTROPIC = 8, MC_MULTISTARS = 16
+    integer(c_int), parameter :: MC_OK = 0, MC_EINVAL = 1, MC_ENODEV = 2, MC_ENOMEM = 3, &
+                                 MC_ESTATE = 4, MC_ETABLE = 5, MC_EPACKET = 6, MC_EUNSUP = 7
+
+    type, bind(C) :: mc_desc
+        integer(c_int32_t) :: abi_version, nx, ny, nz
+        type(c_ptr) :: xAxis, yAxis, zAxis, active
+        integer(c_int32_t) :: nCells, nbins, flags
+        real(c_float) :: R_out, geoCorr(3), ionEdge1
+        type(c_ptr) :: nuArray
+        integer(c_int32_t) :: nStars
+        type(c_ptr) :: starPos, starIdx, deltaE
+        integer(c_int64_t) :: seed          ! uint64 on the C side (same bits)
+        integer(c_int32_t) :: device
+    end type mc_desc
+
+    type, bind(C) :: mc_counters
+        real(c_double) :: n_seg, n_abs, n_sca, n_esc, n_lost, n_line, n_trapped, n_flights, &
+                          qphot, kernel_ms
+    end type mc_counters
+
+    type, bind(C) :: mc_tuning
+        integer(c_int32_t) :: blocks_per_cu, threads, emit_min, walk_steps
+    end type mc_tuning
+
+    type, bind(C) :: mc_atomic
+        integer(c_int32_t) :: nstages, nElemUsed
+        type(c_ptr) :: elemOn, xref, abun
+        integer(c_int32_t) :: nch
+        type(c_ptr) :: ch_meta, ch_sigma, rrec, rnew, fe, dr_int, dr_real, ap_int, ap_real
+        integer(c_int32_t) :: ncel
+        type(c_ptr) :: cel_meta, cel_dbl, cel_flt
+        real(c_float) :: XHILimit
+    end type mc_atomic
+
+    interface
+        integer(c_int) function mc_ctx_create(desc, ctx) bind(C, name="mc_ctx_create")
+            import :: c_int, c_ptr, mc_desc
+            type(mc_desc), intent(in) :: desc
+            type(c_ptr), intent(out) :: ctx
+        end function mc_ctx_create
+
+        integer(c_int) function mc_upload_tables(ctx, opacity, scaOpac, recPDF, totalLines, &
+                                                 specCDF, gSca) bind(C, name="mc_upload_tables")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, opacity, scaOpac, recPDF, totalLines, specCDF, gSca
+        end function mc_upload_tables
+
+        integer(c_int) function mc_zero_estimators(ctx) bind(C, name="mc_zero_estimators")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function mc_zero_estimators
+
+        integer(c_int) function mc_transport_run(ctx, istar, npackets, packet_offset, J_out, &
+                                                 esc_out, cnt) bind(C, name="mc_transport_run")
+            import :: c_int, c_int32_t, c_int64_t, c_ptr, mc_counters
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: istar
+            integer(c_int64_t), value :: npackets, packet_offset
+            type(c_ptr), value :: J_out, esc_out
+            type(mc_counters), intent(out) :: cnt
+        end function mc_transport_run
+
+        integer(c_int) function mc_fetch_estimators(ctx, J_out, esc_out) &
+                bind(C, name="mc_fetch_estimators")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, J_out, esc_out
+        end function mc_fetch_estimators
+
+        integer(c_int) function mc_set_estimators(ctx, J, esc) bind(C, name="mc_set_estimators")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, J, esc
+        end function mc_set_estimators
+
+        integer(c_int) function mc_walk_replay(ctx, n, pos, dir, cell, cell_seq, ds_seq, &
+                                               seq_len, status, max_len) bind(C, name="mc_walk_replay")
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: n
+            type(c_ptr), value :: pos, dir, cell, cell_seq, ds_seq, seq_len, status
+            integer(c_int32_t), value :: max_len
+        end function mc_walk_replay
+
+        integer(c_int) function mc_balance_setup(ctx, atomic) bind(C, name="mc_balance_setup")
+            import :: c_int, c_ptr, mc_atomic
+            type(c_ptr), value :: ctx
+            type(mc_atomic), intent(in) :: atomic
+        end function mc_balance_setup
+
+        integer(c_int) function mc_balance_run(ctx, jscale, Hden, Te, Ne, ionDen, flags, nIter, &
+                                               kernel_ms) bind(C, name="mc_balance_run")
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: ctx
+            real(c_double), value :: jscale
+            type(c_ptr), value :: Hden, Te, Ne, ionDen, flags, nIter, kernel_ms
+        end function mc_balance_run
+
+        integer(c_int) function mc_set_tuning(ctx, t) bind(C, name="mc_set_tuning")
+            import :: c_int, c_ptr, mc_tuning
+            type(c_ptr), value :: ctx
+            type(mc_tuning), intent(in) :: t
+        end function mc_set_tuning
+
+        integer(c_int) function mc_get_tuning(ctx, t) bind(C, name="mc_get_tuning")
+            import :: c_int, c_ptr, mc_tuning
+            type(c_ptr), value :: ctx
+            type(mc_tuning), intent(out) :: t
+        end function mc_get_tuning
+
+        type(c_ptr) function mc_ctx_last_error(ctx) bind(C, name="mc_ctx_last_error")
+            import :: c_ptr
+            type(c_ptr), value :: ctx
+        end function mc_ctx_last_error
+
+        subroutine mc_ctx_destroy(ctx) bind(C, name="mc_ctx_destroy")
+            import :: c_ptr
+            type(c_ptr), value :: ctx
+        end subroutine mc_ctx_destroy
+
+        integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
+            import :: c_size_t, c_ptr
+            type(c_ptr), value :: s
+        end function c_strlen
+    end interface
+
+contains
+
+    ! message of the last failure on ctx (c_null_ptr: of the last failed create)
+    function mc_error_text(ctx) result(msg)
+        type(c_ptr), intent(in) :: ctx
+        character(len=512) :: msg
+        type(c_ptr) :: p
+        character(kind=c_char), pointer :: s(:)
+        integer :: i, n
+        msg = ' '
+        p = mc_ctx_last_error(ctx)
+        if (.not. c_associated(p)) return
+        n = int(min(c_strlen(p), int(len(msg), c_size_t)))
+        call c_f_pointer(p, s, [max(n, 1)])
+        do i = 1, n
+            msg(i:i) = s(i)
+        end do
+    end function mc_error_text
+
+    ! the reference's fail-stop: print the message and stop
+    subroutine mc_check(rc, ctx, where)
+        integer(c_int), intent(in) :: rc
+        type(c_ptr), intent(in) :: ctx
+        character(len=*), intent(in) :: where
+        if (rc /= MC_OK) then
+            print *, '! ', where, ': ', trim(mc_error_text(ctx)), ' (code ', rc, ')'
+            stop
+        end if
+    end subroutine mc_check
+
+    ! energyPacketDriver(iStar, n, grid): transport n packets of star iStar
+    ! (packet indices offset+1 .. offset+n), then copy the running Jste
+    ! (0:nCells,1:nbins) and escapedPackets(0:nCells,0:nbins) totals, in
+    ! double, to J and esc.
+    subroutine gpu_energy_packet_driver(ctx, iStar, n, offset, J, esc, cnt)
+        type(c_ptr), intent(in) :: ctx
+        integer, intent(in) :: iStar
+        integer(c_int64_t), intent(in) :: n, offset
+        real(c_double), intent(inout), target :: J(:, :), esc(:, :)
+        type(mc_counters), intent(out) :: cnt
+        call mc_check(mc_transport_run(ctx, int(iStar, c_int32_t), n, offset, c_loc(J), &
+                                       c_loc(esc), cnt), ctx, 'energyPacketDriver')
+    end subroutine gpu_energy_packet_driver
+
+    ! The updateCell loop over every active cell: Te, Ne (0:nCells) and
+    ! ionDen(0:nCells, nElementsUsed, nstages) are REAL as the host keeps
+    ! them.  lgBlack(0:nCells) receives 1/0 per cell; lgConverged keeps its
+    ! value in cells no packet reached, as updateCell leaves it there
+    ! (update_mod.f90:100-154, 510-515); noHit, noIonBal, noTeBal count the
+    ! cells the reference counts (iteration_mod.f90:848-855).
+    subroutine gpu_update_cells(ctx, jscale, Hden, Te, Ne, ionDen, lgConverged, lgBlack, &
+                                noHit, noIonBal, noTeBal)
+        type(c_ptr), intent(in) :: ctx
+        real(c_double), intent(in) :: jscale
+        real, intent(in) :: Hden(0:)
+        real, intent(inout) :: Te(0:), Ne(0:), ionDen(0:, :, :)
+        integer, intent(inout) :: lgConverged(0:)
+        integer, intent(out) :: lgBlack(0:)
+        integer, intent(out) :: noHit, noIonBal, noTeBal
+        real(c_double), allocatable, target :: dH(:), dT(:), dN(:), dI(:, :, :)
+        integer(c_int32_t), allocatable, target :: fl(:), nIt(:)
+        integer :: nc1
+        nc1 = size(Te)
+        allocate (dH(0:nc1 - 1), dT(0:nc1 - 1), dN(0:nc1 - 1), fl(0:nc1 - 1), nIt(0:nc1 - 1))
+        allocate (dI(0:nc1 - 1, size(ionDen, 2), size(ionDen, 3)))
+        dH = real(Hden, c_double)
+        dT = real(Te, c_double)
+        dN = real(Ne, c_double)
+        dI = real(ionDen, c_double)
+        call mc_check(mc_balance_run(ctx, jscale, c_loc(dH), c_loc(dT), c_loc(dN), c_loc(dI), &
+                                     c_loc(fl), c_loc(nIt), c_null_ptr), ctx, 'updateCell')
+        Te = real(dT)
+        Ne = real(dN)
+        ionDen = real(dI)
+        where (iand(fl, 16) == 0) lgConverged = iand(fl, 1)
+        lgBlack = ishft(iand(fl, 2), -1)
+        lgBlack(0) = 0
+        noHit = count(iand(fl(1:), 16) /= 0)
+        noIonBal = count(iand(fl(1:), 8) /= 0)
+        noTeBal = count(iand(fl(1:), 4) /= 0)
+        deallocate (dH, dT, dN, dI, fl, nIt)
+    end subroutine gpu_update_cells
+
+end module mocassin_amd_mod
