@@ -204,6 +204,21 @@ int mc_set_estimators(mc_ctx *ctx, const double *J, const double *esc);
  * (update_mod.f90:15-2027). */
 int mc_balance_setup(mc_ctx *ctx, const mc_atomic *atomic);
 
+/* Temperature solver and warm start of mc_balance_run (no counterpart in
+ * the reference; NULL options mean the reference's own schedule). */
+#define MC_SOLVER_REFERENCE 0  /* iterateT: false position, |heat-cool| < 0.01 heat, 25 iterations */
+#define MC_SOLVER_SECANT    1  /* secant steps kept inside the bracket, |heat-cool| <= tol heat */
+typedef struct {
+    int32_t solver;             /* MC_SOLVER_* */
+    int32_t max_iter;           /* secant: temperature iterations allowed */
+    double tol;                 /* secant: relative thermal residual accepted */
+    double ion_tol;             /* ion fixed point: relative change of X(H0), X(He0), X(He+)
+                                   that ends it (the reference uses 0.01) */
+    int32_t warm_start;         /* 1: reached cells start from the network's prediction
+                                   (mc_mlp_set first); a cell whose solve from there fails
+                                   restarts from the state passed in (flags bit5) */
+} mc_balance_opts;
+
 /* Ionisation and thermal balance of every active cell on the device (FP64),
  * from the device J times jscale (the host's normalisation of Jste, e.g.
  * scale_J).  Hden(0:nCells), Te(0:nCells), Ne(0:nCells) and
@@ -211,14 +226,56 @@ int mc_balance_setup(mc_ctx *ctx, const mc_atomic *atomic);
  * copied in and, with flags(0:nCells) and nIter(0:nCells), back out; cells
  * that no packet reached keep their inputs.  flags per cell: bit0 H0
  * fraction converged (lgConverged), bit1 lgBlack, bit2 thermal balance not
- * reached in 25 iterations, bit3 ion balance not converged, bit4 not hit.
- * nIter: temperature iterations used.  kernel_ms (nullable): device time.
+ * reached, bit3 ion balance not converged, bit4 not hit, bit5 restarted
+ * from the given state after a failed warm-started solve.  nIter:
+ * temperature iterations used.  kernel_ms (nullable): device time of the
+ * balance (and of the network's inference when warm-starting).
+ * opts NULL = MC_SOLVER_REFERENCE, ion_tol 0.01, no warm start.
  * Replaces: updateCell (update_mod.f90:15-525) with iterateT, ionBalance,
  * thermBalance, calcAlpha, radRecFit, dielectronic and forLines/equilibrium
  * (update_mod.f90:339-1834, emission_mod.f90:1913-2190), called per cell by
  * iterateMC's loop over the grid (iteration_mod.f90:826-838). */
-int mc_balance_run(mc_ctx *ctx, double jscale, const double *Hden, double *Te, double *Ne,
-                   double *ionDen, int32_t *flags, int32_t *nIter, double *kernel_ms);
+int mc_balance_run(mc_ctx *ctx, double jscale, const mc_balance_opts *opts, const double *Hden,
+                   double *Te, double *Ne, double *ionDen, int32_t *flags, int32_t *nIter,
+                   double *kernel_ms);
+
+/* ---- warm-start network (no counterpart in the reference) -------------
+ * A 72 -> 128 -> 128 -> 128 -> 31 ReLU perceptron with BF16 weights and
+ * activations and FP32 accumulation on the matrix cores, trained on the
+ * device.  Inputs per cell (72): log10 of J summed over 64 contiguous groups
+ * of frequency bins, log10 nH, log10 abundances of H, He, C, N, O, Ne, S
+ * (-30 when absent).  Outputs (31): log10 Te, then log10 of the ionic
+ * fractions of those elements, stages 1..min(Z+1,5).  Parameters: a flat
+ * FP32 vector, per layer W[out][in] then b[out]; layer 0 has 96 inputs (72
+ * used, the rest zero) and layer 3 has 32 outputs (31 used).  scale holds
+ * x_mean[72], x_std[72], y_mean[31], y_std[31]: the network sees
+ * (x - x_mean)/x_std and predicts (y - y_mean)/y_std. */
+#define MC_MLP_NIN     72
+#define MC_MLP_NOUT    31
+#define MC_MLP_NPARAMS 49568
+#define MC_MLP_NSCALE  206
+
+/* Set parameters and scaling (resets the optimiser state). */
+int mc_mlp_set(mc_ctx *ctx, const float *params, const float *scale);
+/* Read them back (either may be NULL). */
+int mc_mlp_get(mc_ctx *ctx, float *params, float *scale);
+/* Network inputs of every active cell from the device J times jscale:
+ * X[nCells][72] (row c-1 for cell c), hit(0:nCells) = 1 where a packet
+ * reached the cell.  Needs mc_balance_setup. */
+int mc_mlp_features(mc_ctx *ctx, double jscale, const double *Hden, float *X, int32_t *hit);
+/* Y[n][31] = network(X[n][72]), unscaled. */
+int mc_mlp_predict(mc_ctx *ctx, const float *X, int64_t n, float *Y);
+/* Loss L = sum M (y - t)^2 / sum M over scaled outputs y and targets t
+ * (T[n][31] unscaled, mask M[n][31]) and its gradient dL/dparams. */
+int mc_mlp_loss_grad(mc_ctx *ctx, const float *X, const float *T, const float *M, int64_t n,
+                     float *grad, double *loss);
+/* Adam (lr, 0.9, 0.999, 1e-8) for `steps` steps of `batch` rows; the rows
+ * of step s are order[s*batch .. s*batch+batch) (indices into X, T, M).
+ * loss[s] (nullable) receives each step's L; kernel_ms (nullable) the
+ * device time. */
+int mc_mlp_train(mc_ctx *ctx, const float *X, const float *T, const float *M, int64_t n,
+                 const int32_t *order, int32_t steps, int32_t batch, float lr, float *loss,
+                 double *kernel_ms);
 
 /* Set / read the transport launch shape (see mc_tuning). */
 int mc_set_tuning(mc_ctx *ctx, const mc_tuning *t);

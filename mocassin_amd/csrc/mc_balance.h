@@ -71,6 +71,12 @@ struct CellArgs {
     double *Te, *Ne;            // [nCells+1] in/out
     double *ion;                // [nCells+1][nE][nstages+1], stage 0 unused
     int *flags, *nIter;         // [nCells+1]
+    // temperature solver: 0 the reference's schedule (iterateT), 1 secant
+    // iteration to |heat - cool| <= tol * heat in at most max_iter steps
+    int solver, max_iter;
+    double tol, ion_tol;        // ion_tol: the ion fixed point's relative change (reference 0.01)
+    // state to restart from when the solve from Te/Ne/ion fails (nullable)
+    const double *Te0, *Ne0, *ion0;
 };
 
 // Host-built tables: `A` with its device pointers unset, and the flat arrays
@@ -90,3 +96,9 @@ int prepare(const mc_atomic *in, int nbins, const float *nuArray, Prepared &P, c
 }  // namespace mcb
 
 hipError_t mcb_launch_balance(const mcb::Atomic *dA, const mcb::CellArgs &a, hipStream_t s);
+// warm-start network inputs of every active cell (X [nCells][72], hit [nCells+1])
+hipError_t mcb_launch_features(const mcb::Atomic *dA, const mcb::CellArgs &a, float *X, int *hit,
+                               hipStream_t s);
+// Te, Ne, ion of reached cells from the network's outputs Y [nCells][ldy]
+hipError_t mcb_launch_warm(const mcb::Atomic *dA, const mcb::CellArgs &a, const float *Y, int ldy,
+                           const int *hit, hipStream_t s);

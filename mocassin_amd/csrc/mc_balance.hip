@@ -16,6 +16,7 @@
 // atomic data, so they are computed once at setup (the reference recomputes
 // them on every call; the values are the same).
 #include "mc_balance.h"
+#include "mc_mlp.h"
 
 #include <algorithm>
 #include <cstdarg>
@@ -258,163 +259,211 @@ __global__ void __launch_bounds__(64) balance_kernel(const Atomic *__restrict__ 
         if (lane == 0) { S.nPhot[k] = 1.e-20 + r; S.heatNu[k] = h; }
     }
     __syncthreads();
-    const double XOldHI = S.ion[0][1];
+    // lgConverged compares with the caller's H0 fraction (update_mod.f90:510-515)
+    const double XOldHI = a.ion0 ? a.ion0[(size_t)cell * nE * (ns + 1) + 1] : S.ion[0][1];
 
-    int nIterT = 0, flags = 0, lgIBConv = 1;
-    double rHigh = 0., rLow = 0., Thigh = 0., Tlow = 0.;
-    for (;;) {
-        nIterT++;
-        const double Te = S.Te;
-        // calcAlpha: radiative + dielectronic (oxygen rows fill empty entries)
-        for (int i = 1 + lane; i < ns; i += W)
-            S.diO[i] = di_rec(A, A.odr_n[i], A.odr[i], A.oap[i], Te);
-        __syncthreads();
-        for (int k = lane; k < nch; k += W) {
-            const Channel &c = A.ch[k];
-            double al = rad_rec(c, Te);
-            if (c.elem >= 3) {
-                double d = di_rec(A, c.ndr, c.dr, c.ap, Te);
-                if (d == 0.0) d = S.diO[c.ion];
-                al = al + d;
-            }
-            S.alpha[k] = al;
-            // charge-exchange rate coefficient (ionBalance, 1448-1455)
-            const double t4 = Te / 10000.;
-            double cx = c.chex[0] * pow(t4, c.chex[1]) * (1. + c.chex[2] * exp(c.chex[3] * t4));
-            if (cx < 0.) cx = 0.;
-            if (Te < 6000. || Te > 5.e4) cx = 0.;
-            S.chexv[k] = cx;
-        }
-        __syncthreads();
-        // ionBalance fixed point (1320-1622)
-        int nIterX = 1;
+    int nIterT = 0, flags = 0, lgIBConv = 1, nTot = 0;
+    for (int attempt = 0;; ++attempt) {
+        nIterT = 0;
+        flags = 0;
+        lgIBConv = 1;
+        double rHigh = 0., rLow = 0., Thigh = 0., Tlow = 0.;        // reference schedule
+        double Tp = 0., fp = 0., Tlo = 0., Thi = 0.;                // secant iteration
+        bool havep = false, haveLo = false, haveHi = false;
         for (;;) {
-            const double HIOld = S.ion[0][1], HeIOld = S.ion[1][1], HeIIOld = S.ion[1][2];
-            const double xH0 = S.ion[0][1];
+            nIterT++;
+            const double Te = S.Te;
+            // calcAlpha: radiative + dielectronic (oxygen rows fill empty entries)
+            for (int i = 1 + lane; i < ns; i += W)
+                S.diO[i] = di_rec(A, A.odr_n[i], A.odr[i], A.oap[i], Te);
+            __syncthreads();
             for (int k = lane; k < nch; k += W) {
                 const Channel &c = A.ch[k];
-                double collIon = 0.0;
-                if (k == 0) {
-                    double expFact = 157893.94 / Te;
-                    if (expFact > 75.) expFact = 75.;
-                    collIon = 2.75E-16 * Te * sqrt(Te) * (157893.94 / Te + 2.) * exp(-expFact);
+                double al = rad_rec(c, Te);
+                if (c.elem >= 3) {
+                    double d = di_rec(A, c.ndr, c.dr, c.ap, Te);
+                    if (d == 0.0) d = S.diO[c.ion];
+                    al = al + d;
                 }
-                double revRate = 0.0;
-                if (c.dEk > 1.0)
-                    revRate = S.chexv[k] * (1. - xH0) * S.Hden * 2. * exp(-c.dEk / Te) / c.g0g1;
-                S.ratio[k] = (S.nPhot[k] + collIon + revRate) /
-                             (S.Ne * S.alpha[k] + S.chexv[k] * S.Hden * xH0);
+                S.alpha[k] = al;
+                // charge-exchange rate coefficient (ionBalance, 1448-1455)
+                const double t4 = Te / 10000.;
+                double cx = c.chex[0] * pow(t4, c.chex[1]) * (1. + c.chex[2] * exp(c.chex[3] * t4));
+                if (cx < 0.) cx = 0.;
+                if (Te < 6000. || Te > 5.e4) cx = 0.;
+                S.chexv[k] = cx;
             }
             __syncthreads();
-            if (lane < nE) {
-                const Element &e = A.el[lane];
-                double prod[kMaxStage];
-                double den = 1.0, p = 1.0;
-                for (int i = 0; i < e.nch; ++i) { p = 1.0; for (int q = 0; q <= i; ++q) p = p * S.ratio[e.ch0 + q]; prod[i] = p; }
-                for (int i = 0; i < e.nch; ++i) den = den + prod[i];
-                for (int st = 1; st <= e.nst; ++st) {
-                    double v = (st == 1) ? 1.0 / den : prod[st - 2] / den;
-                    if (v > 0.99999) v = 0.99999;
-                    if (v < 1.e-20) v = 1.e-20;
-                    S.ion[lane][st] = v;
+            // ionBalance fixed point (1320-1622)
+            int nIterX = 1;
+            for (;;) {
+                const double HIOld = S.ion[0][1], HeIOld = S.ion[1][1], HeIIOld = S.ion[1][2];
+                const double xH0 = S.ion[0][1];
+                for (int k = lane; k < nch; k += W) {
+                    const Channel &c = A.ch[k];
+                    double collIon = 0.0;
+                    if (k == 0) {
+                        double expFact = 157893.94 / Te;
+                        if (expFact > 75.) expFact = 75.;
+                        collIon = 2.75E-16 * Te * sqrt(Te) * (157893.94 / Te + 2.) * exp(-expFact);
+                    }
+                    double revRate = 0.0;
+                    if (c.dEk > 1.0)
+                        revRate = S.chexv[k] * (1. - xH0) * S.Hden * 2. * exp(-c.dEk / Te) / c.g0g1;
+                    S.ratio[k] = (S.nPhot[k] + collIon + revRate) /
+                                 (S.Ne * S.alpha[k] + S.chexv[k] * S.Hden * xH0);
                 }
+                __syncthreads();
+                if (lane < nE) {
+                    const Element &e = A.el[lane];
+                    double prod[kMaxStage];
+                    double den = 1.0, p = 1.0;
+                    for (int i = 0; i < e.nch; ++i) { p = 1.0; for (int q = 0; q <= i; ++q) p = p * S.ratio[e.ch0 + q]; prod[i] = p; }
+                    for (int i = 0; i < e.nch; ++i) den = den + prod[i];
+                    for (int st = 1; st <= e.nst; ++st) {
+                        double v = (st == 1) ? 1.0 / den : prod[st - 2] / den;
+                        if (v > 0.99999) v = 0.99999;
+                        if (v < 1.e-20) v = 1.e-20;
+                        S.ion[lane][st] = v;
+                    }
+                }
+                __syncthreads();
+                if (lane == 0) {
+                    double Ne = 0.0;
+                    for (int ei = 0; ei < nE; ++ei)
+                        for (int st = 2; st <= A.el[ei].nst; ++st)
+                            if (S.ion[ei][st] >= 1.e-10) Ne = Ne + (double)(st - 1) * A.el[ei].abun * S.ion[ei][st];
+                    Ne = Ne * S.Hden;
+                    if (Ne == 0.0) Ne = 1.0;
+                    S.Ne = Ne;
+                }
+                __syncthreads();
+                const double dHI = (S.ion[0][1] - HIOld) / HIOld;
+                const double dHeI = (S.ion[1][1] - HeIOld) / HeIOld;
+                const double dHeII = (S.ion[1][2] - HeIIOld) / HeIIOld;
+                const bool big = fabs(dHI) > a.ion_tol || fabs(dHeI) > a.ion_tol || fabs(dHeII) > a.ion_tol;
+                if (big && nIterX < 25) { nIterX++; continue; }
+                lgIBConv = !(big && nIterX == 25);
+                break;
             }
-            __syncthreads();
+            // thermBalance (933-1258)
+            double cool = 0.0, heat = 0.0;
+            for (int k = 0; k < A.ncel; ++k) {
+                double cc = cel_cool(A, A.cel[k], S, lane);
+                cool = cool + cc;
+            }
             if (lane == 0) {
-                double Ne = 0.0;
-                for (int ei = 0; ei < nE; ++ei)
-                    for (int st = 2; st <= A.el[ei].nst; ++st)
-                        if (S.ion[ei][st] >= 1.e-10) Ne = Ne + (double)(st - 1) * A.el[ei].abun * S.ion[ei][st];
-                Ne = Ne * S.Hden;
-                if (Ne == 0.0) Ne = 1.0;
-                S.Ne = Ne;
+                const double T = S.Te, Ne = S.Ne, kB = 1.380658e-16;
+                const double lg = log10(T), Te4 = T / 1.e4;
+                double Np = S.ion[0][2] * A.el[0].abun;
+                double bFF = 1.0108464E-11 + 9.7930778E-13 * lg - 6.6433144E-13 * lg * lg +
+                             2.4793747E-13 * lg * lg * lg - 2.3938215E-14 * lg * lg * lg * lg;
+                double coolFF = Np * Ne * bFF * kB * T / sqrt(T);
+                double bRec = 9.4255985E-11 - 4.04794384E-12 * lg - 1.0055237E-11 * lg * lg +
+                              1.99266862E-12 * lg * lg * lg - 1.06681387E-13 * lg * lg * lg * lg;
+                double coolRec = Np * Ne * bRec * kB * T / sqrt(T);
+                Np = S.ion[1][3] * A.el[1].abun;
+                const double ls = log10(T / 4.);
+                bFF = 2. * (1.0108464E-11 + 9.7930778E-13 * ls - 6.6433144E-13 * ls * ls +
+                            2.4793747E-13 * ls * ls * ls - 2.3938215E-14 * ls * ls * ls * ls);
+                coolFF = coolFF + Np * Ne * bFF * kB * T / sqrt(T / 4.);
+                bRec = 2. * (9.4255985E-11 - 4.04794384E-12 * ls - 1.0055237E-11 * ls * ls +
+                             1.99266862E-12 * ls * ls * ls - 1.06681387E-13 * ls * ls * ls * ls);
+                coolRec = coolRec + Np * Ne * bRec * kB * T / sqrt(T / 4.);
+                Np = S.ion[1][2] * A.el[1].abun;
+                bFF = 1.070073e-11 - 2.5730207e-13 * lg + 2.109134e-13 * lg * lg;
+                coolFF = coolFF + Np * Ne * bFF * kB * T / sqrt(T);
+                bRec = 9.4255985E-11 - 4.04794384E-12 * lg - 1.0055237E-11 * lg * lg +
+                       1.99266862E-12 * lg * lg * lg - 1.06681387E-13 * lg * lg * lg * lg;
+                coolRec = coolRec + Np * Ne * bRec * kB * T / sqrt(T);
+                double coolColl = 0.0;
+                if (T > 5000.)
+                    coolColl = (2.47e-8 * exp(-118338. / T) * pow(Te4, -0.228) +
+                                1.32e-8 * exp(-140252. / T) * pow(Te4, -0.460)) *
+                               hcRyd * S.ion[0][1] * Ne;
+                coolColl = coolColl + cool;
+                for (int k = 0; k < nch; ++k) {
+                    const Channel &c = A.ch[k];
+                    heat = heat + S.heatNu[k] * S.ion[c.eidx][c.ion] * A.el[c.eidx].abun;
+                }
+                S.red[0] = heat;
+                S.red[1] = coolFF + coolRec + coolColl;
             }
             __syncthreads();
-            const double dHI = (S.ion[0][1] - HIOld) / HIOld;
-            const double dHeI = (S.ion[1][1] - HeIOld) / HeIOld;
-            const double dHeII = (S.ion[1][2] - HeIIOld) / HeIIOld;
-            const bool big = fabs(dHI) > 0.01 || fabs(dHeI) > 0.01 || fabs(dHeII) > 0.01;
-            if (big && nIterX < 25) { nIterX++; continue; }
-            lgIBConv = !(big && nIterX == 25);
-            break;
-        }
-        // thermBalance (933-1258)
-        double cool = 0.0, heat = 0.0;
-        for (int k = 0; k < A.ncel; ++k) {
-            double cc = cel_cool(A, A.cel[k], S, lane);
-            cool = cool + cc;
-        }
-        if (lane == 0) {
-            const double T = S.Te, Ne = S.Ne, kB = 1.380658e-16;
-            const double lg = log10(T), Te4 = T / 1.e4;
-            double Np = S.ion[0][2] * A.el[0].abun;
-            double bFF = 1.0108464E-11 + 9.7930778E-13 * lg - 6.6433144E-13 * lg * lg +
-                         2.4793747E-13 * lg * lg * lg - 2.3938215E-14 * lg * lg * lg * lg;
-            double coolFF = Np * Ne * bFF * kB * T / sqrt(T);
-            double bRec = 9.4255985E-11 - 4.04794384E-12 * lg - 1.0055237E-11 * lg * lg +
-                          1.99266862E-12 * lg * lg * lg - 1.06681387E-13 * lg * lg * lg * lg;
-            double coolRec = Np * Ne * bRec * kB * T / sqrt(T);
-            Np = S.ion[1][3] * A.el[1].abun;
-            const double ls = log10(T / 4.);
-            bFF = 2. * (1.0108464E-11 + 9.7930778E-13 * ls - 6.6433144E-13 * ls * ls +
-                        2.4793747E-13 * ls * ls * ls - 2.3938215E-14 * ls * ls * ls * ls);
-            coolFF = coolFF + Np * Ne * bFF * kB * T / sqrt(T / 4.);
-            bRec = 2. * (9.4255985E-11 - 4.04794384E-12 * ls - 1.0055237E-11 * ls * ls +
-                         1.99266862E-12 * ls * ls * ls - 1.06681387E-13 * ls * ls * ls * ls);
-            coolRec = coolRec + Np * Ne * bRec * kB * T / sqrt(T / 4.);
-            Np = S.ion[1][2] * A.el[1].abun;
-            bFF = 1.070073e-11 - 2.5730207e-13 * lg + 2.109134e-13 * lg * lg;
-            coolFF = coolFF + Np * Ne * bFF * kB * T / sqrt(T);
-            bRec = 9.4255985E-11 - 4.04794384E-12 * lg - 1.0055237E-11 * lg * lg +
-                   1.99266862E-12 * lg * lg * lg - 1.06681387E-13 * lg * lg * lg * lg;
-            coolRec = coolRec + Np * Ne * bRec * kB * T / sqrt(T);
-            double coolColl = 0.0;
-            if (T > 5000.)
-                coolColl = (2.47e-8 * exp(-118338. / T) * pow(Te4, -0.228) +
-                            1.32e-8 * exp(-140252. / T) * pow(Te4, -0.460)) *
-                           hcRyd * S.ion[0][1] * Ne;
-            coolColl = coolColl + cool;
-            for (int k = 0; k < nch; ++k) {
-                const Channel &c = A.ch[k];
-                heat = heat + S.heatNu[k] * S.ion[c.eidx][c.ion] * A.el[c.eidx].abun;
-            }
-            S.red[0] = heat;
-            S.red[1] = coolFF + coolRec + coolColl;
-        }
-        __syncthreads();
-        heat = S.red[0];
-        cool = S.red[1];
-        __syncthreads();
-        // iterateT false-position update (398-458)
-        const double res = heat - cool;
-        double Tn = S.Te;
-        bool again = false;
-        if (fabs(res) >= 0.01 * heat) {
-            if (nIterT < 25) {
-                if (res < 0.) {
-                    rLow = res; Tlow = Tn;
-                    Tn = (Thigh != 0.) ? Tlow - (Thigh - Tlow) * rLow / (rHigh - rLow) : Tn / 1.2;
+            heat = S.red[0];
+            cool = S.red[1];
+            __syncthreads();
+            // iterateT false-position update (398-458)
+            const double res = heat - cool;
+            double Tn = S.Te;
+            bool again = false;
+            if (a.solver == 0) {
+                if (fabs(res) >= 0.01 * heat) {
+                    if (nIterT < 25) {
+                        if (res < 0.) {
+                            rLow = res; Tlow = Tn;
+                            Tn = (Thigh != 0.) ? Tlow - (Thigh - Tlow) * rLow / (rHigh - rLow) : Tn / 1.2;
+                        } else {
+                            rHigh = res; Thigh = Tn;
+                            Tn = (Tlow != 0.) ? Tlow - (Thigh - Tlow) * rLow / (rHigh - rLow) : Tn * 1.2;
+                        }
+                        if (nIterT >= 6 && fabs(Thigh - Tlow) <= 0.002 * (Thigh + Tlow)) {
+                            if (res < 0.) { Tn = Thigh - 100.; Thigh = 0.; }
+                            else { Tn = Tlow + 100.; Tlow = 0.; }
+                        }
+                        if (Tn <= 0.) Tn = 1.;
+                        again = true;
+                    } else {
+                        if (Thigh == 0.) Tn = Tlow;
+                        else if (Tlow == 0.) Tn = Thigh;
+                        else Tn = (Thigh + Tlow) * 0.5;
+                        flags |= 2 | 4;
+                    }
+                }
+            } else if (fabs(res) > a.tol * heat) {
+                // secant steps on f(T) = heat - cool, kept inside the bracket once
+                // both signs have been seen (f > 0: heating wins, T must rise)
+                const double f = res, T = Tn;
+                if (f > 0.) { Tlo = T; haveLo = true; } else { Thi = T; haveHi = true; }
+                const double bl = fmin(Tlo, Thi), bh = fmax(Tlo, Thi);
+                if (nIterT >= a.max_iter) {
+                    if (haveLo && haveHi) Tn = 0.5 * (bl + bh);
+                    flags |= 2 | 4;
+                } else if (haveLo && haveHi && bh - bl <= 1.e-7 * T) {
+                    Tn = 0.5 * (bl + bh);
                 } else {
-                    rHigh = res; Thigh = Tn;
-                    Tn = (Tlow != 0.) ? Tlow - (Thigh - Tlow) * rLow / (rHigh - rLow) : Tn * 1.2;
+                    double Tc = (havep && f != fp) ? T - f * (T - Tp) / (f - fp) : (f > 0. ? T * 1.05 : T / 1.05);
+                    if (haveLo && haveHi) {
+                        if (!(Tc > bl && Tc < bh)) Tc = 0.5 * (bl + bh);
+                    } else {
+                        if (f > 0. && !(Tc > T)) Tc = T * 1.2;
+                        if (f < 0. && !(Tc < T)) Tc = T / 1.2;
+                        Tc = fmin(fmax(Tc, T / 1.5), T * 1.5);
+                    }
+                    Tp = T;
+                    fp = f;
+                    havep = true;
+                    Tn = Tc;
+                    again = true;
                 }
-                if (nIterT >= 6 && fabs(Thigh - Tlow) <= 0.002 * (Thigh + Tlow)) {
-                    if (res < 0.) { Tn = Thigh - 100.; Thigh = 0.; }
-                    else { Tn = Tlow + 100.; Tlow = 0.; }
-                }
-                if (Tn <= 0.) Tn = 1.;
-                again = true;
-            } else {
-                if (Thigh == 0.) Tn = Tlow;
-                else if (Tlow == 0.) Tn = Thigh;
-                else Tn = (Thigh + Tlow) * 0.5;
-                flags |= 2 | 4;
             }
+            __syncthreads();
+            if (lane == 0) S.Te = Tn;
+            __syncthreads();
+            if (!again) break;
         }
-        __syncthreads();
-        if (lane == 0) S.Te = Tn;
-        __syncthreads();
-        if (!again) break;
+        nTot += nIterT;
+        const bool failed = (flags & 4) != 0 || !lgIBConv;
+        if (failed && attempt == 0 && a.Te0) {
+            // restart from the caller's state (the reference's starting point)
+            const double *src = a.ion0 + (size_t)cell * nE * (ns + 1);
+            for (int i = lane; i < nE * (ns + 1); i += W) S.ion[i / (ns + 1)][i % (ns + 1)] = src[i];
+            if (lane == 0) { S.Te = a.Te0[cell]; S.Ne = a.Ne0[cell]; }
+            __syncthreads();
+            continue;
+        }
+        if (attempt > 0) flags |= 32;
+        break;
     }
     if (!lgIBConv) flags |= 2 | 8;
     const double dXHI = (S.ion[0][1] - XOldHI) / XOldHI;
@@ -424,8 +473,82 @@ __global__ void __launch_bounds__(64) balance_kernel(const Atomic *__restrict__ 
         a.Te[cell] = S.Te;
         a.Ne[cell] = S.Ne;
         a.flags[cell] = flags;
-        a.nIter[cell] = nIterT;
+        a.nIter[cell] = nTot;
     }
+}
+
+// Features of the warm-start network for cell blockIdx.x + 1 (row
+// blockIdx.x of X [nCells][72]): log10 of J summed over 64 contiguous groups
+// of bins, log10 nH, log10 of the abundances of the network's seven elements
+// (-30 when absent); hit[cell] = 1 when a packet reached the cell.
+__global__ void __launch_bounds__(64) features_kernel(const Atomic *__restrict__ Ap, CellArgs a,
+                                                      float *__restrict__ X, int *__restrict__ hit)
+{
+    const Atomic &A = *Ap;
+    const int lane = threadIdx.x, cell = blockIdx.x + 1;
+    if (cell > a.ncells) return;
+    const size_t xyz = (size_t)a.cell_xyz[cell];
+    const int nb = A.nbins;
+    const int j0 = lane * nb / mcm::kBands, j1 = (lane + 1) * nb / mcm::kBands;
+    double sum = 0.0;
+    for (int j = j0; j < j1; ++j) sum += a.J3[(size_t)j * a.ncell3 + xyz] * a.jscale;
+    float *row = X + (size_t)(cell - 1) * mcm::kIn;
+    row[lane] = (float)log10(fmax(sum, 1.e-30));
+    const int any = __any(sum > 0.0);
+    if (lane < 8) {
+        float v;
+        if (lane == 0) {
+            v = (float)log10(a.Hden[cell]);
+        } else {
+            const int z = mcm::kSlotsZ[lane - 1];
+            v = -30.f;
+            for (int e = 0; e < A.nE; ++e)
+                if (A.el[e].z == z && A.el[e].abun > 0.0) v = (float)log10(A.el[e].abun);
+        }
+        row[mcm::kBands + lane] = v;
+    }
+    if (lane == 0) hit[cell] = any;
+}
+
+// Warm start of reached cells from the network's (unscaled) outputs Y
+// [nCells][ldy]: Te = 10^y0, the seven elements' fractions 10^y normalised
+// per element and clamped as ionBalance clamps them, Ne from them.
+__global__ void warm_kernel(const Atomic *__restrict__ Ap, CellArgs a, const float *__restrict__ Y, int ldy,
+                            const int *__restrict__ hit)
+{
+    const Atomic &A = *Ap;
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (cell > a.ncells || !hit[cell]) return;
+    const int ns = A.nstages, nE = A.nE;
+    const float *y = Y + (size_t)(cell - 1) * ldy;
+    double *ion = a.ion + (size_t)cell * nE * (ns + 1);
+    a.Te[cell] = fmin(fmax(pow(10.0, (double)y[0]), 500.0), 2.e5);
+    int base = 1;
+    for (int k = 0; k < 7; ++k) {
+        const int z = mcm::kSlotsZ[k], nslot = z + 1 < mcm::kStageMax ? z + 1 : mcm::kStageMax;
+        for (int e = 0; e < nE; ++e) {
+            if (A.el[e].z != z) continue;
+            const int nst = A.el[e].nst;
+            double v[kMaxStage], sum = 0.0;
+            for (int st = 1; st <= nst; ++st) {
+                v[st] = st <= nslot ? pow(10.0, fmin((double)y[base + st - 1], 0.0)) : 0.0;
+                sum += v[st];
+            }
+            for (int st = 1; st <= nst; ++st) {
+                double x = sum > 0.0 ? v[st] / sum : (st == 1 ? 1.0 : 0.0);
+                x = fmin(fmax(x, 1.e-20), 0.99999);
+                ion[e * (ns + 1) + st] = x;
+            }
+        }
+        base += nslot;
+    }
+    double Ne = 0.0;
+    for (int e = 0; e < nE; ++e)
+        for (int st = 2; st <= A.el[e].nst; ++st) {
+            const double x = ion[e * (ns + 1) + st];
+            if (x >= 1.e-10) Ne = Ne + (double)(st - 1) * A.el[e].abun * x;
+        }
+    a.Ne[cell] = Ne * a.Hden[cell] > 0.0 ? Ne * a.Hden[cell] : 1.0;
 }
 
 }  // namespace mcb
@@ -434,6 +557,21 @@ hipError_t mcb_launch_balance(const mcb::Atomic *dA, const mcb::CellArgs &a, hip
 {
     if (a.ncells <= 0) return hipSuccess;
     hipLaunchKernelGGL(mcb::balance_kernel, dim3(a.ncells), dim3(64), 0, s, dA, a);
+    return hipGetLastError();
+}
+
+hipError_t mcb_launch_features(const mcb::Atomic *dA, const mcb::CellArgs &a, float *X, int *hit, hipStream_t s)
+{
+    if (a.ncells <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mcb::features_kernel, dim3(a.ncells), dim3(64), 0, s, dA, a, X, hit);
+    return hipGetLastError();
+}
+
+hipError_t mcb_launch_warm(const mcb::Atomic *dA, const mcb::CellArgs &a, const float *Y, int ldy, const int *hit,
+                           hipStream_t s)
+{
+    if (a.ncells <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mcb::warm_kernel, dim3((a.ncells + 255) / 256), dim3(256), 0, s, dA, a, Y, ldy, hit);
     return hipGetLastError();
 }
 

@@ -14,6 +14,7 @@
 #include "mc_balance.h"
 #include "mc_device.h"
 #include "mc_kernels.h"
+#include "mc_mlp.h"
 
 #pragma clang fp contract(off)
 
@@ -47,6 +48,15 @@ struct mc_ctx {
     double *d_bal_tab = nullptr;             // sigma, nu, CEL arrays (one allocation)
     double *d_Hden = nullptr, *d_Te = nullptr, *d_Ne = nullptr, *d_ion = nullptr;
     int *d_flags = nullptr, *d_nIter = nullptr;
+    double *d_cold = nullptr;                // Te0, Ne0, ion0 of a warm-started run
+    // warm-start network
+    bool mlp = false;
+    int mlp_step = 0;
+    float *d_mlp = nullptr;                  // P | grad | m | v (kParams each) | scale
+    uint16_t *d_mlpW = nullptr;              // bf16 copy of P
+    float *d_mX = nullptr, *d_mY = nullptr;  // inference rows [nCells][72], [nCells][32]
+    uint16_t *d_mX0 = nullptr;
+    int *d_hit = nullptr;
     char err[512] = "";
 };
 
@@ -375,13 +385,62 @@ extern "C" int mc_balance_setup(mc_ctx *c, const mc_atomic *at)
     return MC_OK;
 }
 
-extern "C" int mc_balance_run(mc_ctx *c, double jscale, const double *Hden, double *Te, double *Ne,
-                              double *ionDen, int32_t *flags, int32_t *nIter, double *kernel_ms)
+static mcm::Net mlp_net(mc_ctx *c) { return mcm::Net{ c->d_mlpW, c->d_mlp }; }
+static mcm::Scale mlp_scale(mc_ctx *c)
+{
+    const float *sc = c->d_mlp + 4 * (size_t)mcm::kParams;
+    return mcm::Scale{ sc, sc + mcm::kIn, sc + 2 * mcm::kIn, sc + 2 * mcm::kIn + mcm::kOut };
+}
+
+// inference buffers for one row per active cell
+static int mlp_rows(mc_ctx *c)
+{
+    if (c->d_mX) return MC_OK;
+    const size_t nc = (size_t)c->G.nCells;
+    HIPCHK(c, dalloc(&c->d_mX, nc * mcm::kIn));
+    HIPCHK(c, dalloc(&c->d_mY, nc * mcm::kOutP));
+    HIPCHK(c, dalloc(&c->d_mX0, nc * mcm::kInP));
+    HIPCHK(c, dalloc(&c->d_hit, nc + 1));
+    return MC_OK;
+}
+
+static mcb::CellArgs cell_args(mc_ctx *c, double jscale)
+{
+    mcb::CellArgs a{};
+    a.ncells = c->G.nCells;
+    a.jscale = jscale;
+    a.J3 = c->d_J3;
+    a.ncell3 = c->G.ncell3;
+    a.cell_xyz = c->d_cell_xyz;
+    a.Hden = c->d_Hden; a.Te = c->d_Te; a.Ne = c->d_Ne; a.ion = c->d_ion;
+    a.flags = c->d_flags; a.nIter = c->d_nIter;
+    a.solver = MC_SOLVER_REFERENCE; a.max_iter = 25; a.tol = 0.01; a.ion_tol = 0.01;
+    return a;
+}
+
+extern "C" int mc_balance_run(mc_ctx *c, double jscale, const mc_balance_opts *opts, const double *Hden,
+                              double *Te, double *Ne, double *ionDen, int32_t *flags, int32_t *nIter,
+                              double *kernel_ms)
 {
     if (!c) return MC_EINVAL;
     if (!c->balance) return fail(c, MC_ESTATE, "mc_balance_setup has not been called");
     if (!Hden || !Te || !Ne || !ionDen || !flags || !nIter) return fail(c, MC_EINVAL, "null array");
     if (!(jscale > 0.0) || std::isinf(jscale)) return fail(c, MC_EINVAL, "jscale must be finite and > 0");
+    mcb::CellArgs a = cell_args(c, jscale);
+    bool warm = false;
+    if (opts) {
+        if (opts->solver != MC_SOLVER_REFERENCE && opts->solver != MC_SOLVER_SECANT)
+            return fail(c, MC_EINVAL, "unknown solver %d", opts->solver);
+        if (!(opts->ion_tol > 0.0 && opts->ion_tol < 1.0)) return fail(c, MC_EINVAL, "ion_tol outside (0,1)");
+        if (opts->solver == MC_SOLVER_SECANT &&
+            (!(opts->tol > 0.0 && opts->tol < 1.0) || opts->max_iter < 1 || opts->max_iter > 1000))
+            return fail(c, MC_EINVAL, "secant solver needs 0 < tol < 1 and 1 <= max_iter <= 1000");
+        a.solver = opts->solver;
+        a.ion_tol = opts->ion_tol;
+        if (opts->solver == MC_SOLVER_SECANT) { a.tol = opts->tol; a.max_iter = opts->max_iter; }
+        warm = opts->warm_start != 0;
+        if (warm && !c->mlp) return fail(c, MC_ESTATE, "warm start needs mc_mlp_set");
+    }
     HIPCHK(c, hipSetDevice(c->device));
     const int nc = c->G.nCells, nE = c->bal.A.nE, ns = c->bal.A.nstages;
     const size_t nc1 = (size_t)nc + 1;
@@ -389,7 +448,8 @@ extern "C" int mc_balance_run(mc_ctx *c, double jscale, const double *Hden, doub
         if (!(Te[cell] > 0.0) || !(Hden[cell] > 0.0) || !(Ne[cell] > 0.0))
             return fail(c, MC_EINVAL, "cell %d: Te, Ne and Hden must be > 0", cell);
     // ionDen(0:nCells, xref, stage) -> [cell][element index][stage 0..ns]
-    std::vector<double> ion(nc1 * nE * (ns + 1), 0.0);
+    const size_t nion = nc1 * nE * (ns + 1);
+    std::vector<double> ion(nion, 0.0);
     for (size_t cell = 0; cell < nc1; ++cell)
         for (int ei = 0; ei < nE; ++ei) {
             const size_t xr = (size_t)c->bal.eidx_xref[ei] - 1;
@@ -399,23 +459,34 @@ extern "C" int mc_balance_run(mc_ctx *c, double jscale, const double *Hden, doub
     HIPCHK(c, hipMemcpyAsync(c->d_Hden, Hden, nc1 * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->d_Te, Te, nc1 * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->d_Ne, Ne, nc1 * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->d_ion, ion.data(), ion.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_ion, ion.data(), nion * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemsetAsync(c->d_flags, 0, nc1 * sizeof(int), c->stream));
     HIPCHK(c, hipMemsetAsync(c->d_nIter, 0, nc1 * sizeof(int), c->stream));
-    mcb::CellArgs a;
-    a.ncells = nc;
-    a.jscale = jscale;
-    a.J3 = c->d_J3;
-    a.ncell3 = c->G.ncell3;
-    a.cell_xyz = c->d_cell_xyz;
-    a.Hden = c->d_Hden; a.Te = c->d_Te; a.Ne = c->d_Ne; a.ion = c->d_ion;
-    a.flags = c->d_flags; a.nIter = c->d_nIter;
+    if (warm) {
+        if (!c->d_cold) HIPCHK(c, dalloc(&c->d_cold, 2 * nc1 + nion));
+        int rc = mlp_rows(c);
+        if (rc) return rc;
+        HIPCHK(c, hipMemcpyAsync(c->d_cold, c->d_Te, nc1 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_cold + nc1, c->d_Ne, nc1 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_cold + 2 * nc1, c->d_ion, nion * sizeof(double), hipMemcpyDeviceToDevice,
+                                 c->stream));
+        a.Te0 = c->d_cold;
+        a.Ne0 = c->d_cold + nc1;
+        a.ion0 = c->d_cold + 2 * nc1;
+    }
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    if (warm) {
+        HIPCHK(c, mcb_launch_features(c->d_atomic, a, c->d_mX, c->d_hit, c->stream));
+        HIPCHK(c, mcm_standardise(c->d_mX, nc, mlp_scale(c), c->d_mX0, c->stream));
+        HIPCHK(c, mcm_forward(c->d_mX0, nc, mlp_net(c), c->d_mY, mcm::kOutP, nullptr, 2 * c->cus, c->stream));
+        HIPCHK(c, mcm_unscale(c->d_mY, nc, mcm::kOutP, mlp_scale(c), c->stream));
+        HIPCHK(c, mcb_launch_warm(c->d_atomic, a, c->d_mY, mcm::kOutP, c->d_hit, c->stream));
+    }
     HIPCHK(c, mcb_launch_balance(c->d_atomic, a, c->stream));
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipMemcpyAsync(Te, c->d_Te, nc1 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(Ne, c->d_Ne, nc1 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(ion.data(), c->d_ion, ion.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(ion.data(), c->d_ion, nion * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(flags, c->d_flags, nc1 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(nIter, c->d_nIter, nc1 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -430,6 +501,231 @@ extern "C" int mc_balance_run(mc_ctx *c, double jscale, const double *Hden, doub
             for (int st = 1; st <= ns; ++st)
                 ionDen[cell + nc1 * (xr + (size_t)nE * (st - 1))] = ion[(cell * nE + ei) * (ns + 1) + st];
         }
+    return MC_OK;
+}
+
+// ------------------------------------------------------- warm-start network
+extern "C" int mc_mlp_set(mc_ctx *c, const float *params, const float *scale)
+{
+    if (!c) return MC_EINVAL;
+    if (!params || !scale) return fail(c, MC_EINVAL, "params and scale are required");
+    for (int i = 0; i < mcm::kParams; ++i)
+        if (!std::isfinite(params[i])) return fail(c, MC_EINVAL, "parameter %d not finite", i);
+    for (int i = 0; i < MC_MLP_NSCALE; ++i)
+        if (!std::isfinite(scale[i])) return fail(c, MC_EINVAL, "scale entry %d not finite", i);
+    for (int i = 0; i < mcm::kIn; ++i)
+        if (!(scale[mcm::kIn + i] > 0.f)) return fail(c, MC_EINVAL, "x_std[%d] must be > 0", i);
+    for (int i = 0; i < mcm::kOut; ++i)
+        if (!(scale[2 * mcm::kIn + mcm::kOut + i] > 0.f))
+            return fail(c, MC_EINVAL, "y_std[%d] must be > 0", i);
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t P = mcm::kParams;
+    if (!c->d_mlp) {
+        HIPCHK(c, dalloc(&c->d_mlp, 4 * P + MC_MLP_NSCALE));
+        HIPCHK(c, dalloc(&c->d_mlpW, P));
+    }
+    HIPCHK(c, hipMemcpy(c->d_mlp, params, P * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(c->d_mlp + P, 0, 3 * P * sizeof(float)));
+    HIPCHK(c, hipMemcpy(c->d_mlp + 4 * P, scale, MC_MLP_NSCALE * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, mcm_publish(c->d_mlp, c->d_mlpW, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->mlp = true;
+    c->mlp_step = 0;
+    return MC_OK;
+}
+
+extern "C" int mc_mlp_get(mc_ctx *c, float *params, float *scale)
+{
+    if (!c) return MC_EINVAL;
+    if (!c->mlp) return fail(c, MC_ESTATE, "mc_mlp_set has not been called");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (params) HIPCHK(c, hipMemcpy(params, c->d_mlp, mcm::kParams * sizeof(float), hipMemcpyDeviceToHost));
+    if (scale)
+        HIPCHK(c, hipMemcpy(scale, c->d_mlp + 4 * (size_t)mcm::kParams, MC_MLP_NSCALE * sizeof(float),
+                            hipMemcpyDeviceToHost));
+    return MC_OK;
+}
+
+extern "C" int mc_mlp_features(mc_ctx *c, double jscale, const double *Hden, float *X, int32_t *hit)
+{
+    if (!c) return MC_EINVAL;
+    if (!c->balance) return fail(c, MC_ESTATE, "mc_balance_setup has not been called");
+    if (!Hden || !X || !hit) return fail(c, MC_EINVAL, "null array");
+    if (!(jscale > 0.0) || std::isinf(jscale)) return fail(c, MC_EINVAL, "jscale must be finite and > 0");
+    const int nc = c->G.nCells;
+    for (int cell = 1; cell <= nc; ++cell)
+        if (!(Hden[cell] > 0.0)) return fail(c, MC_EINVAL, "cell %d: Hden must be > 0", cell);
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = mlp_rows(c);
+    if (rc) return rc;
+    const size_t nc1 = (size_t)nc + 1;
+    HIPCHK(c, hipMemcpyAsync(c->d_Hden, Hden, nc1 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_hit, 0, nc1 * sizeof(int), c->stream));
+    mcb::CellArgs a = cell_args(c, jscale);
+    HIPCHK(c, mcb_launch_features(c->d_atomic, a, c->d_mX, c->d_hit, c->stream));
+    HIPCHK(c, hipMemcpyAsync(X, c->d_mX, (size_t)nc * mcm::kIn * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(hit, c->d_hit, nc1 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MC_OK;
+}
+
+// device copies of a host data set and the batch buffers of a training call
+struct MlpBatch {
+    float *X = nullptr, *T = nullptr, *M = nullptr, *Yb = nullptr, *Tb = nullptr, *Mb = nullptr;
+    uint16_t *X0 = nullptr, *H = nullptr, *dA = nullptr, *dB = nullptr;
+    int *idx = nullptr;
+    double *sums = nullptr;
+    void release()
+    {
+        void *b[] = { X, T, M, Yb, Tb, Mb, X0, H, dA, dB, idx, sums };
+        for (void *p : b)
+            if (p) (void)hipFree(p);
+    }
+};
+
+static hipError_t batch_alloc(MlpBatch &B, int64_t n, int batch, int64_t nidx, int nsums)
+{
+    hipError_t e;
+    if ((e = dalloc(&B.X, (size_t)n * mcm::kIn)) != hipSuccess) return e;
+    if ((e = dalloc(&B.T, (size_t)n * mcm::kOut)) != hipSuccess) return e;
+    if ((e = dalloc(&B.M, (size_t)n * mcm::kOut)) != hipSuccess) return e;
+    if ((e = dalloc(&B.Yb, (size_t)batch * mcm::kOutP)) != hipSuccess) return e;
+    if ((e = dalloc(&B.Tb, (size_t)batch * mcm::kOut)) != hipSuccess) return e;
+    if ((e = dalloc(&B.Mb, (size_t)batch * mcm::kOut)) != hipSuccess) return e;
+    if ((e = dalloc(&B.X0, (size_t)batch * mcm::kInP)) != hipSuccess) return e;
+    if ((e = dalloc(&B.H, (size_t)3 * batch * mcm::kHid)) != hipSuccess) return e;
+    if ((e = dalloc(&B.dA, (size_t)batch * mcm::kHid)) != hipSuccess) return e;
+    if ((e = dalloc(&B.dB, (size_t)batch * mcm::kHid)) != hipSuccess) return e;
+    if ((e = dalloc(&B.idx, (size_t)nidx)) != hipSuccess) return e;
+    return dalloc(&B.sums, (size_t)nsums);
+}
+
+static int check_data(mc_ctx *c, const float *X, const float *T, const float *M, int64_t n)
+{
+    if (!c->mlp) return fail(c, MC_ESTATE, "mc_mlp_set has not been called");
+    if (!X || !T || !M || n < 1 || n > (1LL << 26)) return fail(c, MC_EINVAL, "bad data set (n = %lld)", (long long)n);
+    for (int64_t i = 0; i < n * mcm::kIn; ++i)
+        if (!std::isfinite(X[i])) return fail(c, MC_EINVAL, "X entry %lld not finite", (long long)i);
+    for (int64_t i = 0; i < n * mcm::kOut; ++i)
+        if (!std::isfinite(T[i]) || !std::isfinite(M[i]) || M[i] < 0.f)
+            return fail(c, MC_EINVAL, "T or M entry %lld not finite (or M < 0)", (long long)i);
+    return MC_OK;
+}
+
+static hipError_t upload_data(mc_ctx *c, MlpBatch &B, const float *X, const float *T, const float *M, int64_t n)
+{
+    hipError_t e = hipMemcpyAsync(B.X, X, (size_t)n * mcm::kIn * sizeof(float), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(B.T, T, (size_t)n * mcm::kOut * sizeof(float), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(B.M, M, (size_t)n * mcm::kOut * sizeof(float), hipMemcpyHostToDevice, c->stream);
+    return e;
+}
+
+// forward + backward of one batch (rows idx[0..nb)); sums must be zero
+static hipError_t train_pass(mc_ctx *c, MlpBatch &B, const int *idx, int nb, double *sums)
+{
+    hipError_t e = mcm_gather(B.X, B.T, B.M, idx, nb, mlp_scale(c), B.X0, B.Tb, B.Mb, c->stream);
+    if (e == hipSuccess) e = mcm_forward(B.X0, nb, mlp_net(c), B.Yb, mcm::kOutP, B.H, 2 * c->cus, c->stream);
+    if (e == hipSuccess)
+        e = mcm_backward(B.X0, B.H, B.Yb, B.Tb, B.Mb, nb, mlp_net(c), B.dA, B.dB,
+                         c->d_mlp + mcm::kParams, sums, c->stream);
+    return e;
+}
+
+extern "C" int mc_mlp_predict(mc_ctx *c, const float *X, int64_t n, float *Y)
+{
+    if (!c) return MC_EINVAL;
+    if (!c->mlp) return fail(c, MC_ESTATE, "mc_mlp_set has not been called");
+    if (!X || !Y || n < 1 || n > (1LL << 26)) return fail(c, MC_EINVAL, "bad rows (n = %lld)", (long long)n);
+    HIPCHK(c, hipSetDevice(c->device));
+    float *dX = nullptr, *dY = nullptr;
+    uint16_t *dX0 = nullptr;
+    hipError_t e = dalloc(&dX, (size_t)n * mcm::kIn);
+    if (e == hipSuccess) e = dalloc(&dY, (size_t)n * mcm::kOut);
+    if (e == hipSuccess) e = dalloc(&dX0, (size_t)n * mcm::kInP);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dX, X, (size_t)n * mcm::kIn * sizeof(float), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = mcm_standardise(dX, (int)n, mlp_scale(c), dX0, c->stream);
+    if (e == hipSuccess) e = mcm_forward(dX0, (int)n, mlp_net(c), dY, mcm::kOut, nullptr, 2 * c->cus, c->stream);
+    if (e == hipSuccess) e = mcm_unscale(dY, (int)n, mcm::kOut, mlp_scale(c), c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(Y, dY, (size_t)n * mcm::kOut * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(dX); (void)hipFree(dY); (void)hipFree(dX0);
+    if (e != hipSuccess)
+        return fail(c, e == hipErrorOutOfMemory ? MC_ENOMEM : MC_ENODEV, "mlp predict: %s", hipGetErrorString(e));
+    return MC_OK;
+}
+
+extern "C" int mc_mlp_loss_grad(mc_ctx *c, const float *X, const float *T, const float *M, int64_t n,
+                                float *grad, double *loss)
+{
+    if (!c) return MC_EINVAL;
+    int rc = check_data(c, X, T, M, n);
+    if (rc) return rc;
+    if (!grad || n > 65536) return fail(c, MC_EINVAL, "gradient needs grad and at most 65536 rows");
+    HIPCHK(c, hipSetDevice(c->device));
+    MlpBatch B;
+    std::vector<int> idx(n);
+    for (int64_t i = 0; i < n; ++i) idx[i] = (int)i;
+    double sums[2] = { 0.0, 0.0 };
+    hipError_t e = batch_alloc(B, n, (int)n, n, 2);
+    if (e == hipSuccess) e = upload_data(c, B, X, T, M, n);
+    if (e == hipSuccess) e = hipMemcpyAsync(B.idx, idx.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(B.sums, 0, 2 * sizeof(double), c->stream);
+    if (e == hipSuccess) e = train_pass(c, B, B.idx, (int)n, B.sums);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(grad, c->d_mlp + mcm::kParams, mcm::kParams * sizeof(float), hipMemcpyDeviceToHost,
+                           c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(sums, B.sums, sizeof sums, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    B.release();
+    if (e != hipSuccess)
+        return fail(c, e == hipErrorOutOfMemory ? MC_ENOMEM : MC_ENODEV, "mlp gradient: %s", hipGetErrorString(e));
+    const double inv = sums[1] > 0.0 ? 1.0 / sums[1] : 0.0;
+    for (int i = 0; i < mcm::kParams; ++i) grad[i] = (float)(grad[i] * inv);
+    if (loss) *loss = sums[0] * inv;
+    return MC_OK;
+}
+
+extern "C" int mc_mlp_train(mc_ctx *c, const float *X, const float *T, const float *M, int64_t n,
+                            const int32_t *order, int32_t steps, int32_t batch, float lr, float *loss,
+                            double *kernel_ms)
+{
+    if (!c) return MC_EINVAL;
+    int rc = check_data(c, X, T, M, n);
+    if (rc) return rc;
+    if (!order || steps < 1 || batch < 1 || batch > 65536 || !(lr > 0.f) || (int64_t)steps * batch > (1LL << 28))
+        return fail(c, MC_EINVAL, "bad training schedule");
+    for (int64_t i = 0; i < (int64_t)steps * batch; ++i)
+        if (order[i] < 0 || order[i] >= n) return fail(c, MC_EINVAL, "order[%lld] outside the data set", (long long)i);
+    HIPCHK(c, hipSetDevice(c->device));
+    MlpBatch B;
+    const int64_t nidx = (int64_t)steps * batch;
+    hipError_t e = batch_alloc(B, n, batch, nidx, 2 * steps);
+    if (e == hipSuccess) e = upload_data(c, B, X, T, M, n);
+    if (e == hipSuccess) e = hipMemcpyAsync(B.idx, order, nidx * sizeof(int), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(B.sums, 0, 2 * steps * sizeof(double), c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+    float *P = c->d_mlp, *G = P + mcm::kParams, *m1 = G + mcm::kParams, *m2 = m1 + mcm::kParams;
+    for (int s = 0; s < steps && e == hipSuccess; ++s) {
+        e = train_pass(c, B, B.idx + (size_t)s * batch, batch, B.sums + 2 * s);
+        if (e == hipSuccess) e = mcm_adam(P, G, m1, m2, B.sums + 2 * s, ++c->mlp_step, lr, c->d_mlpW, c->stream);
+    }
+    if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+    std::vector<double> sums(2 * (size_t)steps);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(sums.data(), B.sums, sums.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    B.release();
+    if (e != hipSuccess)
+        return fail(c, e == hipErrorOutOfMemory ? MC_ENOMEM : MC_ENODEV, "mlp training: %s", hipGetErrorString(e));
+    if (loss)
+        for (int s = 0; s < steps; ++s) loss[s] = (float)(sums[2 * s + 1] > 0.0 ? sums[2 * s] / sums[2 * s + 1] : 0.0);
+    if (kernel_ms) *kernel_ms = ms;
     return MC_OK;
 }
 
@@ -574,7 +870,8 @@ extern "C" void mc_ctx_destroy(mc_ctx *c)
                      c->d_recCDF, c->d_recGuide, c->d_totalLines, c->d_specCDF, c->d_nuArray,
                      c->d_gSca, c->d_J3, c->d_J, c->d_esc, c->d_scratch, c->d_error, c->d_qphot,
                      c->d_cell_xyz, c->d_atomic, c->d_bal_tab, c->d_Hden, c->d_Te, c->d_Ne,
-                     c->d_ion, c->d_flags, c->d_nIter };
+                     c->d_ion, c->d_flags, c->d_nIter, c->d_cold, c->d_mlp, c->d_mlpW, c->d_mX,
+                     c->d_mY, c->d_mX0, c->d_hit };
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

@@ -47,6 +47,14 @@ module mocassin_amd_mod
         integer(c_int32_t) :: blocks_per_cu, threads, emit_min, walk_steps
     end type mc_tuning
 
+    integer(c_int32_t), parameter :: MC_SOLVER_REFERENCE = 0, MC_SOLVER_SECANT = 1
+
+    type, bind(C) :: mc_balance_opts
+        integer(c_int32_t) :: solver, max_iter
+        real(c_double) :: tol, ion_tol
+        integer(c_int32_t) :: warm_start
+    end type mc_balance_opts
+
     type, bind(C) :: mc_atomic
         integer(c_int32_t) :: nstages, nElemUsed
         type(c_ptr) :: elemOn, xref, abun
@@ -111,13 +119,18 @@ module mocassin_amd_mod
             type(mc_atomic), intent(in) :: atomic
         end function mc_balance_setup
 
-        integer(c_int) function mc_balance_run(ctx, jscale, Hden, Te, Ne, ionDen, flags, nIter, &
-                                               kernel_ms) bind(C, name="mc_balance_run")
+        integer(c_int) function mc_balance_run(ctx, jscale, opts, Hden, Te, Ne, ionDen, flags, &
+                                               nIter, kernel_ms) bind(C, name="mc_balance_run")
             import :: c_int, c_double, c_ptr
             type(c_ptr), value :: ctx
             real(c_double), value :: jscale
-            type(c_ptr), value :: Hden, Te, Ne, ionDen, flags, nIter, kernel_ms
+            type(c_ptr), value :: opts, Hden, Te, Ne, ionDen, flags, nIter, kernel_ms
         end function mc_balance_run
+
+        integer(c_int) function mc_mlp_set(ctx, params, scale) bind(C, name="mc_mlp_set")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, params, scale
+        end function mc_mlp_set
 
         integer(c_int) function mc_set_tuning(ctx, t) bind(C, name="mc_set_tuning")
             import :: c_int, c_ptr, mc_tuning
@@ -216,8 +229,8 @@ contains
         dT = real(Te, c_double)
         dN = real(Ne, c_double)
         dI = real(ionDen, c_double)
-        call mc_check(mc_balance_run(ctx, jscale, c_loc(dH), c_loc(dT), c_loc(dN), c_loc(dI), &
-                                     c_loc(fl), c_loc(nIt), c_null_ptr), ctx, 'updateCell')
+        call mc_check(mc_balance_run(ctx, jscale, c_null_ptr, c_loc(dH), c_loc(dT), c_loc(dN), &
+                                     c_loc(dI), c_loc(fl), c_loc(nIt), c_null_ptr), ctx, 'updateCell')
         Te = real(dT)
         Ne = real(dN)
         ionDen = real(dI)

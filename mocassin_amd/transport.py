@@ -138,12 +138,16 @@ class Transport:
         self._nE, self._ns = b.nElemUsed, b.nstages
         self._check(self._lib.mc_balance_setup(self._h, C.byref(A)))
 
-    def update_cells(self, Hden, Te, Ne, ionDen, jscale=1.0):
+    def update_cells(self, Hden, Te, Ne, ionDen, jscale=1.0, solver=None, tol=1e-3, max_iter=60,
+                     ion_tol=0.01, warm_start=False):
         """updateCell for every active cell from the device J times jscale.
         Hden/Te/Ne (nCells+1,), ionDen (nCells+1, nElemUsed, nstages).
+        solver None: the reference's schedule; "secant": secant iteration to
+        |heat-cool| <= tol heat.  warm_start: start reached cells from the
+        network's prediction (set_mlp first), falling back to the given state.
         Returns (Te, Ne, ionDen, flags, nIter, kernel_ms); flags bit0
         converged, bit1 black, bit2 no Te balance, bit3 no ion balance, bit4
-        not hit."""
+        not hit, bit5 restarted from the given state."""
         nc1 = self.nCells + 1
         Te = np.array(Te, np.float64).reshape(nc1)
         Ne = np.array(Ne, np.float64).reshape(nc1)
@@ -152,10 +156,71 @@ class Transport:
         flags = np.zeros(nc1, np.int32)
         nIt = np.zeros(nc1, np.int32)
         ms = C.c_double()
-        self._check(self._lib.mc_balance_run(self._h, float(jscale), Hd.ctypes.data, Te.ctypes.data,
+        opts = None
+        if solver is not None or warm_start or ion_tol != 0.01:
+            opts = _capi.McBalanceOpts()
+            opts.solver = _capi.MC_SOLVER_SECANT if solver == "secant" else _capi.MC_SOLVER_REFERENCE
+            opts.max_iter, opts.tol, opts.ion_tol = int(max_iter), float(tol), float(ion_tol)
+            opts.warm_start = 1 if warm_start else 0
+        self._check(self._lib.mc_balance_run(self._h, float(jscale), None if opts is None else C.byref(opts),
+                                             Hd.ctypes.data, Te.ctypes.data,
                                              Ne.ctypes.data, ion.ctypes.data, flags.ctypes.data,
                                              nIt.ctypes.data, C.byref(ms)))
         return Te, Ne, np.ascontiguousarray(ion), flags, nIt, ms.value
+
+    # ---- warm-start network
+    def set_mlp(self, params, scale):
+        p = np.ascontiguousarray(params, np.float32).ravel()
+        sc = np.ascontiguousarray(scale, np.float32).ravel()
+        if p.size != _capi.MC_MLP_NPARAMS or sc.size != _capi.MC_MLP_NSCALE:
+            raise ValueError("params %d (want %d), scale %d (want %d)"
+                             % (p.size, _capi.MC_MLP_NPARAMS, sc.size, _capi.MC_MLP_NSCALE))
+        self._check(self._lib.mc_mlp_set(self._h, p.ctypes.data, sc.ctypes.data))
+
+    def get_mlp(self):
+        p = np.zeros(_capi.MC_MLP_NPARAMS, np.float32)
+        sc = np.zeros(_capi.MC_MLP_NSCALE, np.float32)
+        self._check(self._lib.mc_mlp_get(self._h, p.ctypes.data, sc.ctypes.data))
+        return p, sc
+
+    def mlp_features(self, Hden, jscale=1.0):
+        """(X [nCells, 72], hit [nCells+1]) from the device J."""
+        Hd = np.ascontiguousarray(Hden, np.float64).reshape(self.nCells + 1)
+        X = np.zeros((self.nCells, _capi.MC_MLP_NIN), np.float32)
+        hit = np.zeros(self.nCells + 1, np.int32)
+        self._check(self._lib.mc_mlp_features(self._h, float(jscale), Hd.ctypes.data, X.ctypes.data,
+                                              hit.ctypes.data))
+        return X, hit
+
+    def mlp_predict(self, X):
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, _capi.MC_MLP_NIN)
+        Y = np.zeros((X.shape[0], _capi.MC_MLP_NOUT), np.float32)
+        self._check(self._lib.mc_mlp_predict(self._h, X.ctypes.data, X.shape[0], Y.ctypes.data))
+        return Y
+
+    def mlp_loss_grad(self, X, T, M):
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, _capi.MC_MLP_NIN)
+        T = np.ascontiguousarray(T, np.float32).reshape(-1, _capi.MC_MLP_NOUT)
+        M = np.ascontiguousarray(M, np.float32).reshape(-1, _capi.MC_MLP_NOUT)
+        g = np.zeros(_capi.MC_MLP_NPARAMS, np.float32)
+        loss = C.c_double()
+        self._check(self._lib.mc_mlp_loss_grad(self._h, X.ctypes.data, T.ctypes.data, M.ctypes.data,
+                                               X.shape[0], g.ctypes.data, C.byref(loss)))
+        return loss.value, g
+
+    def mlp_train(self, X, T, M, order, batch, lr):
+        """Adam steps over batches order.reshape(-1, batch); returns (loss per step, device ms)."""
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, _capi.MC_MLP_NIN)
+        T = np.ascontiguousarray(T, np.float32).reshape(-1, _capi.MC_MLP_NOUT)
+        M = np.ascontiguousarray(M, np.float32).reshape(-1, _capi.MC_MLP_NOUT)
+        order = np.ascontiguousarray(order, np.int32).ravel()
+        steps = order.size // int(batch)
+        loss = np.zeros(steps, np.float32)
+        ms = C.c_double()
+        self._check(self._lib.mc_mlp_train(self._h, X.ctypes.data, T.ctypes.data, M.ctypes.data,
+                                           X.shape[0], order.ctypes.data, steps, int(batch), float(lr),
+                                           loss.ctypes.data, C.byref(ms)))
+        return loss, ms.value
 
     def walk_replay(self, pos, dirv, cell, max_len=2048):
         pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)

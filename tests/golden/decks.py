@@ -54,6 +54,22 @@ DECKS = {
         "LPhot 1.e13", "nbins 200", "nuMin 1.e-5", "nuMax 25.", "nstages 5",
         "nPhotons 20000", "maxIterateMC 1 95.", "TeStart 10000.",
     ], FULL_GAS),
+    # the atomic data of the c1 and c2 workloads (frequency mesh, composition,
+    # stages) taken from a 5^3 grid of the same keywords
+    "c1atom": ([
+        "symmetricXYZ", "nx 5", "ny 5", "nz 5",
+        "edges 3.e18 3.e18 3.e18", "Rin 0.", "Rout 3.e18",
+        "Hdensity 100.", "contShape blackbody", "TStellar 40000.",
+        "LPhot 4.e12", "nbins 400", "nuMin 1.e-5", "nuMax 25.",
+        "nPhotons 2000", "maxIterateMC 1 100.", "TeStart 10000.",
+    ], H_HE),
+    "c2atom": ([
+        "symmetricXYZ", "nx 5", "ny 5", "nz 5",
+        "edges 2.5e18 2.5e18 2.5e18", "Rin 0.", "Rout 2.5e18",
+        "Hdensity 1000.", "contShape blackbody", "TStellar 40000.",
+        "LPhot 1.e13", "nbins 800", "nuMin 1.e-5", "nuMax 25.", "nstages 5",
+        "nPhotons 2000", "maxIterateMC 1 95.", "TeStart 10000.",
+    ], FULL_GAS),
     # full (non-symmetric) grid, odd size: exercises the non-symmetric walk
     "full_mini": ([
         "nx 11", "ny 11", "nz 11",

@@ -157,6 +157,22 @@ def make_balance(deck, npk, seed, name, opacity_scale=None):
     print("wrote", out, "cells", nc, "J nonzero", len(nzJ))
 
 
+def make_atomic(deck, name):
+    """Atomic inputs of the balance for a workload's frequency mesh and
+    composition, with the reference's initial cell state (TeStart and its
+    initial ionisation fractions of one cell) as the cold start."""
+    sys.path.insert(0, ROOT)
+    from mocassin_amd.atomic import AtomicBundle
+    d = run_harness(deck, "balance", [5, 2000], tag="atom_" + name)
+    prob, (nx, ny, nz, nc, nb) = read_problem(d)
+    b = AtomicBundle.from_harness_dump(d, "/root/reference/data/radrec.dat")
+    rd = lambda n, dt=np.float32: np.fromfile(os.path.join(d, n + ".bin"), dt)
+    ion0 = rd("ionDen").reshape(b.nstages, b.nElemUsed, nc + 1)[:, :, 1].T.copy()
+    out = os.path.join(HERE, "atomic_%s.npz" % name)
+    np.savez_compressed(out, **b.to_npz_dict(), Te0=rd("Te")[1:2], ion0=ion0)
+    print("wrote", out, "nbins", nb, "channels", b.nch, "CEL ions", b.ncel)
+
+
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "all"
     if what in ("walk", "all"):
@@ -166,6 +182,9 @@ if __name__ == "__main__":
         make_balance("c1mini", 20000, 31, "c1mini")
         make_balance("c2mini", 20000, 4242, "c2mini")
         make_balance("c2mini", 20000, 77, "c2mini_thick", opacity_scale=300.0)
+    if what in ("atomic", "all"):
+        make_atomic("c1atom", "c1")
+        make_atomic("c2atom", "c2")
     if what in ("transport", "all"):
         make_transport("c1mini", 20000, 12345, "c1mini")
         make_transport("c1mini", 8000, 999, "c1mini_thick", opacity_scale=300.0)
