@@ -176,6 +176,15 @@ int mc_zero_estimators(mc_ctx *ctx);
 int mc_transport_run(mc_ctx *ctx, int32_t istar, int64_t npackets, uint64_t packet_offset,
                      double *J_out, double *esc_out, mc_counters *c);
 
+/* Count the path segments walked in each cell during transport (1) or stop
+ * counting (0); counts are zeroed with the estimators.  Off by default: it
+ * costs one extra atomic per segment.  No counterpart in the reference; it
+ * gives the per-cell packet crossings that decide which cells are sampled
+ * well enough to judge convergence. */
+int mc_count_crossings(mc_ctx *ctx, int32_t enable);
+/* crossings(0:nCells): the counts per cell (row 0: all inactive points). */
+int mc_fetch_crossings(mc_ctx *ctx, uint32_t *crossings);
+
 /* Copy the device estimators to the host (layouts above). */
 int mc_fetch_estimators(mc_ctx *ctx, double *J_out, double *esc_out);
 
@@ -217,6 +226,9 @@ typedef struct {
     int32_t warm_start;         /* 1: reached cells start from the network's prediction
                                    (mc_mlp_set first); a cell whose solve from there fails
                                    restarts from the state passed in (flags bit5) */
+    double first_step;          /* secant: relative size of the first trial step when the
+                                   start gives no bracket (0: 0.05); a good warm start
+                                   wants a step of the order of its error */
 } mc_balance_opts;
 
 /* Ionisation and thermal balance of every active cell on the device (FP64),

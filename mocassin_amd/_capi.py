@@ -20,7 +20,8 @@ EXPORTS = ("mc_ctx_create", "mc_upload_tables", "mc_zero_estimators", "mc_transp
            "mc_fetch_estimators", "mc_walk_replay", "mc_set_tuning", "mc_get_tuning",
            "mc_set_estimators", "mc_balance_setup", "mc_balance_run",
            "mc_mlp_set", "mc_mlp_get", "mc_mlp_features", "mc_mlp_predict", "mc_mlp_loss_grad",
-           "mc_mlp_train", "mc_ctx_last_error", "mc_ctx_destroy")
+           "mc_mlp_train", "mc_count_crossings", "mc_fetch_crossings",
+           "mc_ctx_last_error", "mc_ctx_destroy")
 
 MC_SOLVER_REFERENCE, MC_SOLVER_SECANT = 0, 1
 MC_MLP_NIN, MC_MLP_NOUT, MC_MLP_NPARAMS, MC_MLP_NSCALE = 72, 31, 49568, 206
@@ -68,7 +69,7 @@ class McAtomic(C.Structure):
 
 class McBalanceOpts(C.Structure):
     _fields_ = [("solver", C.c_int32), ("max_iter", C.c_int32), ("tol", C.c_double),
-                ("ion_tol", C.c_double), ("warm_start", C.c_int32)]
+                ("ion_tol", C.c_double), ("warm_start", C.c_int32), ("first_step", C.c_double)]
 
 
 class McError(RuntimeError):
@@ -107,11 +108,13 @@ def load():
     L.mc_mlp_predict.argtypes = [vp, vp, i64, vp]
     L.mc_mlp_loss_grad.argtypes = [vp, vp, vp, vp, i64, vp, vp]
     L.mc_mlp_train.argtypes = [vp, vp, vp, vp, i64, vp, i32, i32, C.c_float, vp, vp]
+    L.mc_count_crossings.argtypes = [vp, i32]
+    L.mc_fetch_crossings.argtypes = [vp, vp]
     L.mc_ctx_last_error.argtypes = [vp]
     L.mc_ctx_last_error.restype = C.c_char_p
     L.mc_ctx_destroy.argtypes = [vp]
     L.mc_ctx_destroy.restype = None
-    for name in EXPORTS[:17]:
+    for name in EXPORTS[:19]:
         getattr(L, name).restype = C.c_int
     _LIB = L
     return L

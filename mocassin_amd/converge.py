@@ -1,0 +1,171 @@
+"""Wall time to converged Te, plain balance versus the warm-started one, on
+the clumpy 100^3 H+He workload (density: lognormal field plus 200 seeded
+clumps, mocassin_amd/synth.py; source 40,000 K, Q(H) = 4e48 s^-1).
+
+For each of two seeded variants of the grid: one transport sweep with
+per-cell crossing counts, then a tight balance solve (secant, |heat - cool|
+<= 1e-5 heat, ion fixed point to 1e-4) from the reference's cold start
+(TeStart = 10,000 K and its initial ionisation) gives Te_ref.  The network is
+trained on the device on the reached cells of variant 1 and judged on
+variant 0:
+
+  plain      secant balance (tolerance 1e-3) from the cold start
+  warm       the network's prediction, then the same balance from there,
+             its first trial step sized to the network's error (1 % instead
+             of 5 %), restarting from the cold start where that solve fails:
+             the prediction is kept where its residual is within 1e-3
+  reference  the reference's own schedule (iterateT) from the cold start
+and the plain and warm pair again at the reference's own tolerance (1e-2).
+
+"Converged" means at least 95 % of the cells with at least 1e3 packet
+crossings have |Te - Te_ref| / Te_ref <= 1 %.  Times are device times of
+the balance (plus the network's inference when warm-starting), measured
+with HIP events, and the host wall time of the whole call.
+"""
+import time
+
+import numpy as np
+
+from . import mlp as MP
+from . import synth
+from .atomic import AtomicBundle
+from .transport import Transport
+
+
+def cold_state(bundle, Te0, ion0, Hden):
+    nc1 = Hden.size
+    Te = np.full(nc1, float(Te0), np.float64)
+    ion = np.broadcast_to(np.asarray(ion0, np.float64), (nc1,) + np.shape(ion0)).copy()
+    Ne = np.zeros(nc1)
+    for z in range(1, 31):
+        if not bundle.elem_on[z - 1]:
+            continue
+        k = int(bundle.xref[z - 1]) - 1
+        for st in range(2, min(z + 1, bundle.nstages) + 1):
+            x = ion[:, k, st - 1]
+            Ne += np.where(x >= 1e-10, (st - 1) * float(bundle.abun[z - 1]) * x, 0.0)
+    Ne = Ne * Hden
+    Ne[Ne <= 0] = 1.0
+    Hd = Hden.copy()
+    Hd[0] = 1.0
+    Te[0], Ne[0] = 1.0, 1.0
+    return Te, Ne, ion, Hd
+
+
+def _variant(seed, bundle, atom, packets, log):
+    t = time.perf_counter()
+    p = synth.workload("c4", n_packets=packets, seed=seed, nu=bundle.nuArray)
+    log("c4 seed %d: %d cells, built in %.1f s" % (seed, p["nCells"], time.perf_counter() - t))
+    T = Transport(p["xAxis"], p["yAxis"], p["zAxis"], p["active"], p["nCells"], p["nbins"], p["nuArray"],
+                  p["R_out"], p["geoCorr"], p["ionEdge1"], [p["starPos"]], [p["starIdx"]], [p["deltaE"]],
+                  symmetric=p["symmetric"], seed=1000 + seed, device=0)
+    T.upload_tables(p["opacity"], p["specCDF"], p["recPDF"], p["totalLines"])
+    T.count_crossings(True)
+    T.zero_estimators()
+    t = time.perf_counter()
+    c = T.energy_packet_driver(1, int(packets), 0)
+    tr_s = time.perf_counter() - t
+    cross = T.crossings()
+    T.balance_setup(bundle)
+    Te0, Ne0, ion0, Hd = cold_state(bundle, atom["Te0"][0], atom["ion0"], p["Hden"])
+    jscale = 1e-9 / (8.0 if p["symmetric"] else 1.0)
+    ref = T.update_cells(Hd, Te0, Ne0, ion0, jscale=jscale, solver="secant", tol=1e-5, max_iter=100,
+                         ion_tol=1e-4)
+    log("c4 seed %d: transport %.2f s (%.3g packets/s), tight balance %.1f ms"
+        % (seed, tr_s, packets / tr_s, ref[5]))
+    return dict(T=T, cross=cross, cold=(Hd, Te0, Ne0, ion0), jscale=jscale, ref=ref, counters=c)
+
+
+def _judge(Te, Te_ref, well):
+    err = np.abs(Te[well] - Te_ref[well]) / Te_ref[well]
+    return float(np.mean(err <= 0.01)), float(np.max(err))
+
+
+def te_convergence(packets=1e8, train_steps=6000, batch=1024, lr=2e-3, log=print):
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    atom = dict(np.load(os.path.join(here, "..", "tests", "golden", "atomic_c1.npz")))
+    bundle = AtomicBundle.from_npz(atom)
+    v_eval = _variant(0, bundle, atom, packets, log)
+    v_train = _variant(1, bundle, atom, packets, log)
+    out = {"workload": "clumpy 100^3 full grid, H+He, 400 bins, %.0e packets per sweep" % packets,
+           "transport_counters": {k: v_eval["counters"][k] for k in ("n_seg", "n_abs", "kernel_ms")}}
+    try:
+        # training pairs from variant 1's reached, balanced cells
+        Ttr = v_train["T"]
+        Hd, Te0, Ne0, ion0 = v_train["cold"]
+        Te_r, Ne_r, ion_r, fl_r, _, _ = v_train["ref"]
+        X, hit = Ttr.mlp_features(Hd, jscale=v_train["jscale"])
+        cells = np.nonzero(hit[1:] & ((fl_r[1:] & (4 | 8)) == 0))[0] + 1
+        Tg, M = MP.targets(bundle, Te_r, ion_r, cells)
+        loss, train_ms = MP.train(Ttr, X[cells - 1], Tg, M, steps=train_steps, batch=batch, lr=lr)
+        params, scale = Ttr.get_mlp()
+        log("network trained on %d cells: loss %.3g -> %.3g in %.0f ms (%d Adam steps of %d)"
+            % (cells.size, loss[:20].mean(), loss[-50:].mean(), train_ms, train_steps, batch))
+        out["training"] = {"cells": int(cells.size), "steps": train_steps, "batch": batch,
+                           "device_ms": train_ms, "loss_first": float(loss[:20].mean()),
+                           "loss_last": float(loss[-50:].mean())}
+
+        T = v_eval["T"]
+        Hd, Te0, Ne0, ion0 = v_eval["cold"]
+        Te_ref, fl_ref = v_eval["ref"][0], v_eval["ref"][3]
+        well = np.zeros(Te_ref.size, bool)
+        well[1:] = (v_eval["cross"][1:] >= 1000) & ((fl_ref[1:] & (4 | 16)) == 0)
+        T.set_mlp(params, scale)
+        js = v_eval["jscale"]
+        X, hit = T.mlp_features(Hd, jscale=js)
+        pred = T.mlp_predict(X)
+        Te_pred = Te0.copy()
+        Te_pred[1:] = 10.0 ** pred[:, 0]
+        runs = {}
+        kw = dict(jscale=js, solver="secant", max_iter=60, ion_tol=1e-3)
+        for name, extra in (("plain", {"tol": 1e-3}),
+                            ("warm", {"tol": 1e-3, "warm_start": True, "first_step": 0.01}),
+                            ("plain_tol_0.01", {"tol": 1e-2}),
+                            ("warm_tol_0.01", {"tol": 1e-2, "warm_start": True, "first_step": 0.01})):
+            best = None
+            for _ in range(3):
+                t = time.perf_counter()
+                r = T.update_cells(Hd, Te0, Ne0, ion0, **kw, **extra)
+                wall = time.perf_counter() - t
+                if best is None or r[5] < best[1]:
+                    best = (r, r[5], wall)
+            r, ms, wall = best
+            frac, worst = _judge(r[0], Te_ref, well)
+            runs[name] = {"device_ms": ms, "host_wall_ms": wall * 1e3, "frac_within_1pct": frac,
+                          "max_rel_err": worst, "mean_iterations": float(np.mean(r[4][well])),
+                          "restarted_cells": int(np.sum((r[3][1:] & 32) != 0)),
+                          "converged": frac >= 0.95}
+        t = time.perf_counter()
+        r = T.update_cells(Hd, Te0, Ne0, ion0, jscale=js)
+        wall = time.perf_counter() - t
+        frac, worst = _judge(r[0], Te_ref, well)
+        runs["reference_schedule"] = {"device_ms": r[5], "host_wall_ms": wall * 1e3, "frac_within_1pct": frac,
+                                      "max_rel_err": worst, "mean_iterations": float(np.mean(r[4][well])),
+                                      "converged": frac >= 0.95}
+        frac, worst = _judge(Te_pred, Te_ref, well)
+        runs["network_alone"] = {"frac_within_1pct": frac, "max_rel_err": worst}
+        out["well_sampled_cells"] = int(well.sum())
+        out["runs"] = runs
+        p, w = runs["plain"], runs["warm"]
+        out["speedup_device"] = p["device_ms"] / w["device_ms"] if p["converged"] and w["converged"] else None
+        out["speedup_host_wall"] = (p["host_wall_ms"] / w["host_wall_ms"]
+                                    if p["converged"] and w["converged"] else None)
+        p2, w2 = runs["plain_tol_0.01"], runs["warm_tol_0.01"]
+        out["speedup_device_tol_0.01"] = (p2["device_ms"] / w2["device_ms"]
+                                          if p2["converged"] and w2["converged"] else None)
+        log("plain %.1f ms (%.3f within 1%%), warm %.1f ms (%.3f), reference schedule %.1f ms (%.3f), "
+            "network alone %.3f within 1%%"
+            % (p["device_ms"], p["frac_within_1pct"], w["device_ms"], w["frac_within_1pct"],
+               runs["reference_schedule"]["device_ms"], runs["reference_schedule"]["frac_within_1pct"],
+               runs["network_alone"]["frac_within_1pct"]))
+    finally:
+        v_eval["T"].close()
+        v_train["T"].close()
+    return out
+
+
+if __name__ == "__main__":
+    import json
+    import sys
+    print(json.dumps(te_convergence(log=lambda m: print(m, file=sys.stderr, flush=True))))

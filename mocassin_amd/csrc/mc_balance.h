@@ -19,7 +19,7 @@ constexpr int kMaxCel = 64;     // CEL model ions
 constexpr int kMaxLev = 17;     // levels per model ion (nForLevels)
 constexpr int kMaxTemp = 64;    // temperatures per collision-strength table
 constexpr int kMaxDr = 4;       // dielectronic fit rows per channel
-constexpr int kMaxNb = 2048;    // frequency bins (one cell's J row is staged in LDS)
+constexpr int kMaxNb = 65536;   // frequency bins
 
 // one photoionisation channel (elem, ion) in the reference's loop order
 struct Channel {
@@ -58,15 +58,15 @@ struct Atomic {
     // device arrays
     const double *sigma;        // [nch][nbins]
     const double *nu;           // [nbins]
+    const double *cR, *cH;      // [nch][nbins] rate and heating coefficients per bin
     const double *cel_e, *cel_a, *cel_q, *cel_y2, *cel_t, *cel_g;
 };
 
 struct CellArgs {
     int ncells;                 // active cells (1..nCells)
     double jscale;              // J = J3 * jscale
-    const double *J3;           // dense [nu][xyz]
-    int ncell3;
-    const int *cell_xyz;        // [nCells+1] dense index of each active cell
+    const double *Rint, *Hint;  // [nCells+1][nch] channel integrals (integrals_kernel)
+    const int *hit;             // [nCells+1] reached by a packet
     const double *Hden;         // [nCells+1]
     double *Te, *Ne;            // [nCells+1] in/out
     double *ion;                // [nCells+1][nE][nstages+1], stage 0 unused
@@ -75,6 +75,7 @@ struct CellArgs {
     // iteration to |heat - cool| <= tol * heat in at most max_iter steps
     int solver, max_iter;
     double tol, ion_tol;        // ion_tol: the ion fixed point's relative change (reference 0.01)
+    double step0;               // secant: relative size of the first trial step
     // state to restart from when the solve from Te/Ne/ion fails (nullable)
     const double *Te0, *Ne0, *ion0;
 };
@@ -84,7 +85,7 @@ struct CellArgs {
 // index e.
 struct Prepared {
     Atomic A{};
-    std::vector<double> sigma, nu, cel_e, cel_a, cel_q, cel_y2, cel_t, cel_g;
+    std::vector<double> sigma, nu, cR, cH, cel_e, cel_a, cel_q, cel_y2, cel_t, cel_g;
     std::vector<int> eidx_xref;
 };
 
@@ -96,9 +97,12 @@ int prepare(const mc_atomic *in, int nbins, const float *nuArray, Prepared &P, c
 }  // namespace mcb
 
 hipError_t mcb_launch_balance(const mcb::Atomic *dA, const mcb::CellArgs &a, hipStream_t s);
-// warm-start network inputs of every active cell (X [nCells][72], hit [nCells+1])
-hipError_t mcb_launch_features(const mcb::Atomic *dA, const mcb::CellArgs &a, float *X, int *hit,
-                               hipStream_t s);
+// channel integrals Rint/Hint [nCells+1][nch], hit flags [nCells+1] and
+// (X non-null) the warm-start network's inputs X [nCells][72] of every
+// active cell, from the dense J3[nu][xyz] times a.jscale
+hipError_t mcb_launch_integrals(const mcb::Atomic *dA, const mcb::CellArgs &a, const double *J3,
+                                const int *active, int ncell3, double *Rint, double *Hint, int *hit,
+                                float *X, hipStream_t s);
 // Te, Ne, ion of reached cells from the network's outputs Y [nCells][ldy]
 hipError_t mcb_launch_warm(const mcb::Atomic *dA, const mcb::CellArgs &a, const float *Y, int ldy,
                            const int *hit, hipStream_t s);

@@ -16,6 +16,7 @@
 // atomic data, so they are computed once at setup (the reference recomputes
 // them on every call; the values are the same).
 #include "mc_balance.h"
+#include "mc_device.h"
 #include "mc_mlp.h"
 
 #include <algorithm>
@@ -29,7 +30,6 @@ namespace mcb {
 constexpr int W = 64;
 
 struct Lds {
-    double J[kMaxNb];           // this cell's J row (scaled)
     double Te, Ne, Hden;
     double ion[kMaxElem][kMaxStage];
     double nPhot[kMaxCh], heatNu[kMaxCh], alpha[kMaxCh], ratio[kMaxCh], chexv[kMaxCh];
@@ -103,10 +103,10 @@ __device__ double splint(const double *xa, const double *ya, const double *y2, i
 
 // CEL cooling of one model ion [erg/s per H], levels 1..10 (forLines +
 // equilibrium + the cooling sum of thermBalance)
-__device__ double cel_cool(const Atomic &A, const CelIon &ci, Lds &S, int lane)
+__device__ double cel_cool(const Atomic &A, const CelIon &ci, Lds &S, int lane, double Te)
 {
     const int nl = ci.nl, nt = ci.nt;
-    const double Te = S.Te, Ne = S.Ne;
+    const double Ne = S.Ne;
     const double *E = A.cel_e + ci.off_e;
     const double *Aa = A.cel_a + ci.off_a;          // a(i,j) at [(i-1) + nl*(j-1)]
     const double *lT = A.cel_t + ci.off_t;
@@ -207,25 +207,62 @@ __device__ double cel_cool(const Atomic &A, const CelIon &ci, Lds &S, int lane)
     return cool;   // valid on lane 0
 }
 
-__global__ void __launch_bounds__(64) balance_kernel(const Atomic *__restrict__ Ap, CellArgs a)
+// total cooling at temperature T for the cell's current ionisation and Ne
+// (thermBalance, update_mod.f90:933-1113: H and He free-free and
+// recombination fits, H collisional excitation, CEL lines); valid on lane 0
+__device__ double cooling(const Atomic &A, Lds &S, int lane, double T)
+{
+    double cool = 0.0;
+    for (int k = 0; k < A.ncel; ++k) {
+        double cc = cel_cool(A, A.cel[k], S, lane, T);
+        cool = cool + cc;
+    }
+    if (lane != 0) return 0.0;
+    const double Ne = S.Ne, kB = 1.380658e-16, hcRyd = 2.1799153e-11;
+    const double lg = log10(T), Te4 = T / 1.e4;
+    double Np = S.ion[0][2] * A.el[0].abun;
+    double bFF = 1.0108464E-11 + 9.7930778E-13 * lg - 6.6433144E-13 * lg * lg +
+                 2.4793747E-13 * lg * lg * lg - 2.3938215E-14 * lg * lg * lg * lg;
+    double coolFF = Np * Ne * bFF * kB * T / sqrt(T);
+    double bRec = 9.4255985E-11 - 4.04794384E-12 * lg - 1.0055237E-11 * lg * lg +
+                  1.99266862E-12 * lg * lg * lg - 1.06681387E-13 * lg * lg * lg * lg;
+    double coolRec = Np * Ne * bRec * kB * T / sqrt(T);
+    Np = S.ion[1][3] * A.el[1].abun;
+    const double ls = log10(T / 4.);
+    bFF = 2. * (1.0108464E-11 + 9.7930778E-13 * ls - 6.6433144E-13 * ls * ls +
+                2.4793747E-13 * ls * ls * ls - 2.3938215E-14 * ls * ls * ls * ls);
+    coolFF = coolFF + Np * Ne * bFF * kB * T / sqrt(T / 4.);
+    bRec = 2. * (9.4255985E-11 - 4.04794384E-12 * ls - 1.0055237E-11 * ls * ls +
+                 1.99266862E-12 * ls * ls * ls - 1.06681387E-13 * ls * ls * ls * ls);
+    coolRec = coolRec + Np * Ne * bRec * kB * T / sqrt(T / 4.);
+    Np = S.ion[1][2] * A.el[1].abun;
+    bFF = 1.070073e-11 - 2.5730207e-13 * lg + 2.109134e-13 * lg * lg;
+    coolFF = coolFF + Np * Ne * bFF * kB * T / sqrt(T);
+    bRec = 9.4255985E-11 - 4.04794384E-12 * lg - 1.0055237E-11 * lg * lg +
+           1.99266862E-12 * lg * lg * lg - 1.06681387E-13 * lg * lg * lg * lg;
+    coolRec = coolRec + Np * Ne * bRec * kB * T / sqrt(T);
+    double coolColl = 0.0;
+    if (T > 5000.)
+        coolColl = (2.47e-8 * exp(-118338. / T) * pow(Te4, -0.228) +
+                    1.32e-8 * exp(-140252. / T) * pow(Te4, -0.460)) *
+                   hcRyd * S.ion[0][1] * Ne;
+    coolColl = coolColl + cool;
+    return coolFF + coolRec + coolColl;
+}
+
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) balance_kernel(const Atomic *__restrict__ Ap, CellArgs a)
 {
     __shared__ Lds S;
     const Atomic &A = *Ap;
     const int lane = threadIdx.x;
     const int cell = blockIdx.x + 1;
     if (cell > a.ncells) return;
-    const int nb = A.nbins, ns = A.nstages, nE = A.nE, nch = A.nch;
-    const size_t xyz = (size_t)a.cell_xyz[cell];
+    const int ns = A.nstages, nE = A.nE, nch = A.nch;
     double *ionOut = a.ion + (size_t)cell * nE * (ns + 1);
 
-    // stage the cell's J row; hit test (update_mod.f90:104-154)
-    int hit = 0;
-    for (int j = lane; j < nb; j += W) {
-        const double J = a.J3[(size_t)j * a.ncell3 + xyz] * a.jscale;
-        S.J[j] = J;
-        if (J > 0.0) hit = 1;
-    }
-    if (!__any(hit)) {
+    // hit test (update_mod.f90:104-154) and the channel integrals, both
+    // from integrals_kernel
+    if (!a.hit[cell]) {
         if (lane == 0) { a.flags[cell] = 16; a.nIter[cell] = 0; }
         return;
     }
@@ -235,28 +272,9 @@ __global__ void __launch_bounds__(64) balance_kernel(const Atomic *__restrict__ 
         S.Hden = a.Hden[cell];
     }
     for (int i = lane; i < nE * (ns + 1); i += W) S.ion[i / (ns + 1)][i % (ns + 1)] = ionOut[i];
-    __syncthreads();
-    // photoionisation rates and heating integrals per channel
-    // (update_mod.f90:184-269); hiH is already cut at the first bin whose
-    // cross section is below 1e-35, where the reference's heating loop exits
-    const double hcRyd = 2.1799153e-11;
-    for (int k = 0; k < nch; ++k) {
-        const Channel &c = A.ch[k];
-        const double *sig = A.sigma + (size_t)k * nb;
-        double r = 0.0, h = 0.0;
-        const double nuIP = A.nu[c.ip - 1];
-        const int hi = c.hiR > c.hiH ? c.hiR : c.hiH;
-        for (int j = c.ip + lane; j <= hi; j += W) {
-            const double J = S.J[j - 1];
-            if (J > 0.0) {
-                const double ph = sig[j - 1];
-                if (j <= c.hiR && ph >= 1.e-35) r += J * ph / (hcRyd * A.nu[j - 1]);
-                if (j <= c.hiH) h += ph * J * (A.nu[j - 1] - nuIP) / A.nu[j - 1];
-            }
-        }
-        r = wave_sum(r);
-        h = wave_sum(h);
-        if (lane == 0) { S.nPhot[k] = 1.e-20 + r; S.heatNu[k] = h; }
+    for (int k = lane; k < nch; k += W) {
+        S.nPhot[k] = 1.e-20 + a.Rint[(size_t)cell * nch + k];
+        S.heatNu[k] = a.Hint[(size_t)cell * nch + k];
     }
     __syncthreads();
     // lgConverged compares with the caller's H0 fraction (update_mod.f90:510-515)
@@ -347,47 +365,14 @@ __global__ void __launch_bounds__(64) balance_kernel(const Atomic *__restrict__ 
                 break;
             }
             // thermBalance (933-1258)
-            double cool = 0.0, heat = 0.0;
-            for (int k = 0; k < A.ncel; ++k) {
-                double cc = cel_cool(A, A.cel[k], S, lane);
-                cool = cool + cc;
-            }
+            double cool = cooling(A, S, lane, S.Te), heat = 0.0;
             if (lane == 0) {
-                const double T = S.Te, Ne = S.Ne, kB = 1.380658e-16;
-                const double lg = log10(T), Te4 = T / 1.e4;
-                double Np = S.ion[0][2] * A.el[0].abun;
-                double bFF = 1.0108464E-11 + 9.7930778E-13 * lg - 6.6433144E-13 * lg * lg +
-                             2.4793747E-13 * lg * lg * lg - 2.3938215E-14 * lg * lg * lg * lg;
-                double coolFF = Np * Ne * bFF * kB * T / sqrt(T);
-                double bRec = 9.4255985E-11 - 4.04794384E-12 * lg - 1.0055237E-11 * lg * lg +
-                              1.99266862E-12 * lg * lg * lg - 1.06681387E-13 * lg * lg * lg * lg;
-                double coolRec = Np * Ne * bRec * kB * T / sqrt(T);
-                Np = S.ion[1][3] * A.el[1].abun;
-                const double ls = log10(T / 4.);
-                bFF = 2. * (1.0108464E-11 + 9.7930778E-13 * ls - 6.6433144E-13 * ls * ls +
-                            2.4793747E-13 * ls * ls * ls - 2.3938215E-14 * ls * ls * ls * ls);
-                coolFF = coolFF + Np * Ne * bFF * kB * T / sqrt(T / 4.);
-                bRec = 2. * (9.4255985E-11 - 4.04794384E-12 * ls - 1.0055237E-11 * ls * ls +
-                             1.99266862E-12 * ls * ls * ls - 1.06681387E-13 * ls * ls * ls * ls);
-                coolRec = coolRec + Np * Ne * bRec * kB * T / sqrt(T / 4.);
-                Np = S.ion[1][2] * A.el[1].abun;
-                bFF = 1.070073e-11 - 2.5730207e-13 * lg + 2.109134e-13 * lg * lg;
-                coolFF = coolFF + Np * Ne * bFF * kB * T / sqrt(T);
-                bRec = 9.4255985E-11 - 4.04794384E-12 * lg - 1.0055237E-11 * lg * lg +
-                       1.99266862E-12 * lg * lg * lg - 1.06681387E-13 * lg * lg * lg * lg;
-                coolRec = coolRec + Np * Ne * bRec * kB * T / sqrt(T);
-                double coolColl = 0.0;
-                if (T > 5000.)
-                    coolColl = (2.47e-8 * exp(-118338. / T) * pow(Te4, -0.228) +
-                                1.32e-8 * exp(-140252. / T) * pow(Te4, -0.460)) *
-                               hcRyd * S.ion[0][1] * Ne;
-                coolColl = coolColl + cool;
                 for (int k = 0; k < nch; ++k) {
                     const Channel &c = A.ch[k];
                     heat = heat + S.heatNu[k] * S.ion[c.eidx][c.ion] * A.el[c.eidx].abun;
                 }
                 S.red[0] = heat;
-                S.red[1] = coolFF + coolRec + coolColl;
+                S.red[1] = cool;
             }
             __syncthreads();
             heat = S.red[0];
@@ -421,8 +406,8 @@ __global__ void __launch_bounds__(64) balance_kernel(const Atomic *__restrict__ 
                     }
                 }
             } else if (fabs(res) > a.tol * heat) {
-                // secant steps on f(T) = heat - cool, kept inside the bracket once
-                // both signs have been seen (f > 0: heating wins, T must rise)
+                // secant steps on f(T) = heat - cool, kept inside the bracket
+                // once both signs have been seen (f > 0: heating wins, T rises)
                 const double f = res, T = Tn;
                 if (f > 0.) { Tlo = T; haveLo = true; } else { Thi = T; haveHi = true; }
                 const double bl = fmin(Tlo, Thi), bh = fmax(Tlo, Thi);
@@ -432,7 +417,11 @@ __global__ void __launch_bounds__(64) balance_kernel(const Atomic *__restrict__ 
                 } else if (haveLo && haveHi && bh - bl <= 1.e-7 * T) {
                     Tn = 0.5 * (bl + bh);
                 } else {
-                    double Tc = (havep && f != fp) ? T - f * (T - Tp) / (f - fp) : (f > 0. ? T * 1.05 : T / 1.05);
+                    // first trial: a step of the relative residual (cooling
+                    // grows about linearly with T near balance), at most step0
+                    const double d0 = fmin(a.step0, fmax(fabs(f) / heat, 1.e-4));
+                    double Tc = (havep && f != fp) ? T - f * (T - Tp) / (f - fp)
+                                                   : (f > 0. ? T * (1. + d0) : T / (1. + d0));
                     if (haveLo && haveHi) {
                         if (!(Tc > bl && Tc < bh)) Tc = 0.5 * (bl + bh);
                     } else {
@@ -477,37 +466,101 @@ __global__ void __launch_bounds__(64) balance_kernel(const Atomic *__restrict__ 
     }
 }
 
-// Features of the warm-start network for cell blockIdx.x + 1 (row
-// blockIdx.x of X [nCells][72]): log10 of J summed over 64 contiguous groups
-// of bins, log10 nH, log10 of the abundances of the network's seven elements
-// (-30 when absent); hit[cell] = 1 when a packet reached the cell.
-__global__ void __launch_bounds__(64) features_kernel(const Atomic *__restrict__ Ap, CellArgs a,
-                                                      float *__restrict__ X, int *__restrict__ hit)
+// Per lattice point (one lane each, so that every read of J3[nu][xyz] is
+// coalesced across the wavefront): the photoionisation rate and heating
+// integrals of every channel (update_mod.f90:184-269, 1146-1234), the hit
+// flag (104-154) and, when X is given, the warm-start network's features of
+// the cell (row cell-1 of X [nCells][72]): log10 of J summed over 64
+// contiguous groups of bins, log10 nH, log10 abundances of its seven
+// elements (-30 when absent).  Channels are accumulated eight at a time.
+__global__ void __launch_bounds__(64) integrals_kernel(const Atomic *__restrict__ Ap, CellArgs a,
+                                                       const double *__restrict__ J3,
+                                                       const int *__restrict__ active, int ncell3,
+                                                       double *__restrict__ Rint, double *__restrict__ Hint,
+                                                       int *__restrict__ hit, float *__restrict__ X)
 {
+    constexpr int G = 8, C = 64, U = 16;
+    __shared__ double cr[G][C], ch[G][C];              // coefficients of one chunk of bins
     const Atomic &A = *Ap;
-    const int lane = threadIdx.x, cell = blockIdx.x + 1;
-    if (cell > a.ncells) return;
-    const size_t xyz = (size_t)a.cell_xyz[cell];
-    const int nb = A.nbins;
-    const int j0 = lane * nb / mcm::kBands, j1 = (lane + 1) * nb / mcm::kBands;
-    double sum = 0.0;
-    for (int j = j0; j < j1; ++j) sum += a.J3[(size_t)j * a.ncell3 + xyz] * a.jscale;
-    float *row = X + (size_t)(cell - 1) * mcm::kIn;
-    row[lane] = (float)log10(fmax(sum, 1.e-30));
-    const int any = __any(sum > 0.0);
-    if (lane < 8) {
-        float v;
-        if (lane == 0) {
-            v = (float)log10(a.Hden[cell]);
-        } else {
-            const int z = mcm::kSlotsZ[lane - 1];
-            v = -30.f;
-            for (int e = 0; e < A.nE; ++e)
-                if (A.el[e].z == z && A.el[e].abun > 0.0) v = (float)log10(A.el[e].abun);
+    const int lane = threadIdx.x;
+    const int xyz = blockIdx.x * 64 + lane;
+    const int cell = xyz < ncell3 ? active[xyz] : 0;
+    if (!__any(cell > 0)) return;
+    const int nb = A.nbins, nch = A.nch;
+    // this wavefront's tile of J3: [nu][64 points], contiguous
+    const double *Jp = J3 + mcd::j3_index(nb, 0, blockIdx.x * 64) + lane;
+    float *brow = (X && cell > 0) ? X + (size_t)(cell - 1) * mcm::kIn : nullptr;
+    int any = 0;
+    for (int g0 = 0; g0 < nch || g0 == 0; g0 += G) {
+        const int ng = nch - g0 < G ? nch - g0 : G;
+        double r[G], h[G];
+        for (int k = 0; k < G; ++k) { r[k] = 0.0; h[k] = 0.0; }
+        // the first pass visits every bin (hit flag, band sums)
+        int jlo = nb + 1, jhi = 0;
+        for (int k = 0; k < ng; ++k) {
+            const Channel &c = A.ch[g0 + k];
+            jlo = c.ip < jlo ? c.ip : jlo;
+            const int hi = c.hiR > c.hiH ? c.hiR : c.hiH;
+            jhi = hi > jhi ? hi : jhi;
         }
-        row[mcm::kBands + lane] = v;
+        if (g0 == 0) { jlo = 1; jhi = nb; }
+        double band = 0.0;
+        int b = 0, bend = nb / mcm::kBands;
+        for (int c0 = jlo; c0 <= jhi; c0 += C) {
+            const int cn = jhi - c0 + 1 < C ? jhi - c0 + 1 : C;
+            __syncthreads();
+            for (int k = 0; k < ng; ++k) {
+                const size_t o = (size_t)(g0 + k) * nb + c0 - 1 + lane;
+                cr[k][lane] = lane < cn ? A.cR[o] : 0.0;
+                ch[k][lane] = lane < cn ? A.cH[o] : 0.0;
+            }
+            __syncthreads();
+            for (int u0 = 0; u0 < cn; u0 += U) {
+                double Jv[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    Jv[u] = u0 + u < cn ? Jp[(size_t)(c0 + u0 + u - 1) * mcd::kJTile] * a.jscale : 0.0;
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (u0 + u >= cn) break;
+                    const double J = Jv[u];
+                    for (int k = 0; k < ng; ++k) {
+                        r[k] = r[k] + J * cr[k][u0 + u];
+                        h[k] = h[k] + J * ch[k][u0 + u];
+                    }
+                    if (g0 == 0) {
+                        const int j = c0 + u0 + u;
+                        any |= J > 0.0;
+                        band += J;
+                        if (j == bend) {
+                            if (brow) brow[b] = (float)log10(fmax(band, 1.e-30));
+                            band = 0.0;
+                            ++b;
+                            bend = (b + 1) * nb / mcm::kBands;
+                        }
+                    }
+                }
+            }
+        }
+        if (cell > 0)
+            for (int k = 0; k < ng; ++k) {
+                Rint[(size_t)cell * nch + g0 + k] = r[k];
+                Hint[(size_t)cell * nch + g0 + k] = h[k];
+            }
+        if (nch == 0) break;
     }
-    if (lane == 0) hit[cell] = any;
+    if (cell <= 0) return;
+    hit[cell] = any;
+    if (brow) {
+        float *row = brow;
+        row[mcm::kBands] = (float)log10(a.Hden[cell]);
+        for (int k = 0; k < 7; ++k) {
+            float v = -30.f;
+            for (int e = 0; e < A.nE; ++e)
+                if (A.el[e].z == mcm::kSlotsZ[k] && A.el[e].abun > 0.0) v = (float)log10(A.el[e].abun);
+            row[mcm::kBands + 1 + k] = v;
+        }
+    }
 }
 
 // Warm start of reached cells from the network's (unscaled) outputs Y
@@ -560,10 +613,11 @@ hipError_t mcb_launch_balance(const mcb::Atomic *dA, const mcb::CellArgs &a, hip
     return hipGetLastError();
 }
 
-hipError_t mcb_launch_features(const mcb::Atomic *dA, const mcb::CellArgs &a, float *X, int *hit, hipStream_t s)
+hipError_t mcb_launch_integrals(const mcb::Atomic *dA, const mcb::CellArgs &a, const double *J3, const int *active,
+                               int ncell3, double *Rint, double *Hint, int *hit, float *X, hipStream_t s)
 {
-    if (a.ncells <= 0) return hipSuccess;
-    hipLaunchKernelGGL(mcb::features_kernel, dim3(a.ncells), dim3(64), 0, s, dA, a, X, hit);
+    hipLaunchKernelGGL(mcb::integrals_kernel, dim3((ncell3 + 63) / 64), dim3(64), 0, s, dA, a, J3, active, ncell3,
+                       Rint, Hint, hit, X);
     return hipGetLastError();
 }
 
@@ -677,7 +731,8 @@ int prepare(const mc_atomic *in, int nb, const float *nuArray, Prepared &P, char
         return err(e, n, MC_EINVAL, "atomic descriptor has null arrays");
     const int ns = in->nstages;
     if (ns < 2 || ns > kMaxStage - 1) return err(e, n, MC_EUNSUP, "nstages %d outside 2..%d", ns, kMaxStage - 1);
-    if (nb > kMaxNb) return err(e, n, MC_EUNSUP, "nbins %d above %d", nb, kMaxNb);
+    if (nb > kMaxNb || nb < mcm::kBands)
+        return err(e, n, MC_EUNSUP, "nbins %d outside %d..%d", nb, mcm::kBands, kMaxNb);
     if (!in->elemOn[0] || !in->elemOn[1]) return err(e, n, MC_EUNSUP, "H and He must be switched on");
     Atomic &A = P.A;
     A = Atomic{};
@@ -721,6 +776,18 @@ int prepare(const mc_atomic *in, int nb, const float *nuArray, Prepared &P, char
                 // the heating integral stops at the first cross section below 1e-35
                 for (int j = c.ip; j <= c.hiH; ++j)
                     if (sig[j - 1] < 1.e-35) { c.hiH = j - 1; break; }
+            }
+            // coefficients of the rate and heating integrals per bin:
+            // rate J sigma / (h c Ryd nu) over ip..hiRate where sigma >= 1e-35,
+            // heating J sigma (nu - nu_ip) / nu over ip..hiHeat
+            {
+                const double hcRyd = 2.1799153e-11;
+                const double nuIP = (double)nuArray[c.ip - 1];
+                for (int j = 1; j <= nb; ++j) {
+                    const double nu = (double)nuArray[j - 1], ph = sig[j - 1];
+                    P.cR.push_back(j >= c.ip && j <= c.hiR && ph >= 1.e-35 ? ph / (hcRyd * nu) : 0.0);
+                    P.cH.push_back(j >= c.ip && j <= c.hiH ? ph * (nu - nuIP) / nu : 0.0);
+                }
             }
             P.sigma.insert(P.sigma.end(), sig.begin(), sig.end());
             c.g0g1 = (double)g0 / (double)g1;

@@ -49,6 +49,8 @@ struct mc_ctx {
     double *d_Hden = nullptr, *d_Te = nullptr, *d_Ne = nullptr, *d_ion = nullptr;
     int *d_flags = nullptr, *d_nIter = nullptr;
     double *d_cold = nullptr;                // Te0, Ne0, ion0 of a warm-started run
+    double *d_Rint = nullptr, *d_Hint = nullptr;  // [nCells+1][nch] channel integrals
+    unsigned *d_cross = nullptr;             // segment counts per lattice point (when enabled)
     // warm-start network
     bool mlp = false;
     int mlp_step = 0;
@@ -169,7 +171,7 @@ static int ctx_init(mc_ctx *c, const mc_desc *d)
     HIPCHK(c, dalloc(&c->d_active, (size_t)nx * ny * nz));
     HIPCHK(c, dalloc(&c->d_nuArray, nb));
     const size_t ncell3 = (size_t)nx * ny * nz;
-    HIPCHK(c, dalloc(&c->d_J3, ncell3 * nb));
+    HIPCHK(c, dalloc(&c->d_J3, mcd::j3_size((int)ncell3, nb)));
     HIPCHK(c, dalloc(&c->d_esc, (size_t)(nc + 1) * (nb + 1)));
     HIPCHK(c, dalloc(&c->d_scratch, 9));
     HIPCHK(c, dalloc(&c->d_error, 1));
@@ -291,9 +293,35 @@ extern "C" int mc_zero_estimators(mc_ctx *c)
     if (!c) return MC_EINVAL;
     HIPCHK(c, hipSetDevice(c->device));
     const size_t nc1 = (size_t)c->G.nCells + 1;
-    HIPCHK(c, hipMemsetAsync(c->d_J3, 0, (size_t)c->G.ncell3 * c->G.nbins * sizeof(double), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_J3, 0, mcd::j3_size(c->G.ncell3, c->G.nbins) * sizeof(double), c->stream));
     HIPCHK(c, hipMemsetAsync(c->d_esc, 0, nc1 * (c->G.nbins + 1) * sizeof(double), c->stream));
+    if (c->G.cross) HIPCHK(c, hipMemsetAsync(c->G.cross, 0, (size_t)c->G.ncell3 * sizeof(unsigned), c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MC_OK;
+}
+
+extern "C" int mc_count_crossings(mc_ctx *c, int32_t enable)
+{
+    if (!c) return MC_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (enable && !c->d_cross) {
+        HIPCHK(c, dalloc(&c->d_cross, (size_t)c->G.ncell3));
+        HIPCHK(c, hipMemset(c->d_cross, 0, (size_t)c->G.ncell3 * sizeof(unsigned)));
+    }
+    c->G.cross = enable ? c->d_cross : nullptr;
+    return MC_OK;
+}
+
+extern "C" int mc_fetch_crossings(mc_ctx *c, uint32_t *out)
+{
+    if (!c) return MC_EINVAL;
+    if (!out) return fail(c, MC_EINVAL, "null output");
+    if (!c->d_cross) return fail(c, MC_ESTATE, "crossing counts are not enabled");
+    HIPCHK(c, hipSetDevice(c->device));
+    std::vector<uint32_t> dense(c->G.ncell3);
+    HIPCHK(c, hipMemcpy(dense.data(), c->d_cross, dense.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (int cell = 0; cell <= c->G.nCells; ++cell) out[cell] = 0;
+    for (size_t i = 0; i < dense.size(); ++i) out[c->active[i]] += dense[i];
     return MC_OK;
 }
 
@@ -322,13 +350,13 @@ extern "C" int mc_set_estimators(mc_ctx *c, const double *J, const double *esc)
     if (!J) return fail(c, MC_EINVAL, "J is required");
     HIPCHK(c, hipSetDevice(c->device));
     const size_t nc1 = (size_t)c->G.nCells + 1, nb = c->G.nbins, n3 = c->G.ncell3;
-    std::vector<double> dense(n3 * nb);
+    std::vector<double> dense(mcd::j3_size((int)n3, (int)nb), 0.0);
     for (size_t nu = 0; nu < nb; ++nu)
         for (size_t i = 0; i < n3; ++i) {
             const int a = c->active[i];
-            dense[nu * n3 + i] = a > 0 ? J[nu * nc1 + a] : 0.0;
+            if (a > 0) dense[mcd::j3_index((int)nb, (int)nu, (int)i)] = J[nu * nc1 + a];
         }
-    HIPCHK(c, hipMemcpy(c->d_J3, dense.data(), n3 * nb * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_J3, dense.data(), dense.size() * sizeof(double), hipMemcpyHostToDevice));
     if (esc) HIPCHK(c, hipMemcpy(c->d_esc, esc, nc1 * (nb + 1) * sizeof(double), hipMemcpyHostToDevice));
     return MC_OK;
 }
@@ -349,14 +377,14 @@ extern "C" int mc_balance_setup(mc_ctx *c, const mc_atomic *at)
     if (at->nElemUsed != P.A.nE) return fail(c, MC_ETABLE, "nElemUsed %d but %d elements on", at->nElemUsed, P.A.nE);
     // one device block for every table, in this order
     const std::vector<double> *parts[] = { &P.sigma, &P.nu, &P.cel_e, &P.cel_a, &P.cel_q,
-                                           &P.cel_y2, &P.cel_t, &P.cel_g };
+                                           &P.cel_y2, &P.cel_t, &P.cel_g, &P.cR, &P.cH };
     size_t tot = 0;
     for (auto *v : parts) tot += v->size();
     if (c->d_bal_tab) { (void)hipFree(c->d_bal_tab); c->d_bal_tab = nullptr; }
     HIPCHK(c, dalloc(&c->d_bal_tab, tot));
-    const double *ptr[8];
+    const double *ptr[10];
     size_t off = 0;
-    for (int k = 0; k < 8; ++k) {
+    for (int k = 0; k < 10; ++k) {
         ptr[k] = c->d_bal_tab + off;
         if (!parts[k]->empty())
             HIPCHK(c, hipMemcpy(c->d_bal_tab + off, parts[k]->data(), parts[k]->size() * sizeof(double),
@@ -365,7 +393,7 @@ extern "C" int mc_balance_setup(mc_ctx *c, const mc_atomic *at)
     }
     mcb::Atomic A = P.A;
     A.sigma = ptr[0]; A.nu = ptr[1]; A.cel_e = ptr[2]; A.cel_a = ptr[3]; A.cel_q = ptr[4];
-    A.cel_y2 = ptr[5]; A.cel_t = ptr[6]; A.cel_g = ptr[7];
+    A.cel_y2 = ptr[5]; A.cel_t = ptr[6]; A.cel_g = ptr[7]; A.cR = ptr[8]; A.cH = ptr[9];
     if (!c->d_atomic) HIPCHK(c, dalloc(&c->d_atomic, 1));
     HIPCHK(c, hipMemcpy(c->d_atomic, &A, sizeof A, hipMemcpyHostToDevice));
     const size_t nc1 = (size_t)c->G.nCells + 1;
@@ -381,6 +409,11 @@ extern "C" int mc_balance_setup(mc_ctx *c, const mc_atomic *at)
     }
     if (c->d_ion) { (void)hipFree(c->d_ion); c->d_ion = nullptr; }
     HIPCHK(c, dalloc(&c->d_ion, nion));
+    if (c->d_Rint) { (void)hipFree(c->d_Rint); c->d_Rint = nullptr; }
+    if (c->d_Hint) { (void)hipFree(c->d_Hint); c->d_Hint = nullptr; }
+    HIPCHK(c, dalloc(&c->d_Rint, nc1 * (P.A.nch > 0 ? P.A.nch : 1)));
+    HIPCHK(c, dalloc(&c->d_Hint, nc1 * (P.A.nch > 0 ? P.A.nch : 1)));
+    if (!c->d_hit) HIPCHK(c, dalloc(&c->d_hit, nc1));
     c->balance = true;
     return MC_OK;
 }
@@ -400,7 +433,6 @@ static int mlp_rows(mc_ctx *c)
     HIPCHK(c, dalloc(&c->d_mX, nc * mcm::kIn));
     HIPCHK(c, dalloc(&c->d_mY, nc * mcm::kOutP));
     HIPCHK(c, dalloc(&c->d_mX0, nc * mcm::kInP));
-    HIPCHK(c, dalloc(&c->d_hit, nc + 1));
     return MC_OK;
 }
 
@@ -409,13 +441,22 @@ static mcb::CellArgs cell_args(mc_ctx *c, double jscale)
     mcb::CellArgs a{};
     a.ncells = c->G.nCells;
     a.jscale = jscale;
-    a.J3 = c->d_J3;
-    a.ncell3 = c->G.ncell3;
-    a.cell_xyz = c->d_cell_xyz;
+    a.Rint = c->d_Rint;
+    a.Hint = c->d_Hint;
+    a.hit = c->d_hit;
     a.Hden = c->d_Hden; a.Te = c->d_Te; a.Ne = c->d_Ne; a.ion = c->d_ion;
     a.flags = c->d_flags; a.nIter = c->d_nIter;
-    a.solver = MC_SOLVER_REFERENCE; a.max_iter = 25; a.tol = 0.01; a.ion_tol = 0.01;
+    a.solver = MC_SOLVER_REFERENCE; a.max_iter = 25; a.tol = 0.01; a.ion_tol = 0.01; a.step0 = 0.05;
     return a;
+}
+
+// channel integrals, hit flags and (X non-null) network inputs of every cell
+static int integrals(mc_ctx *c, const mcb::CellArgs &a, float *X)
+{
+    HIPCHK(c, hipMemsetAsync(c->d_hit, 0, ((size_t)c->G.nCells + 1) * sizeof(int), c->stream));
+    HIPCHK(c, mcb_launch_integrals(c->d_atomic, a, c->d_J3, c->d_active, c->G.ncell3, c->d_Rint, c->d_Hint,
+                                   c->d_hit, X, c->stream));
+    return MC_OK;
 }
 
 extern "C" int mc_balance_run(mc_ctx *c, double jscale, const mc_balance_opts *opts, const double *Hden,
@@ -437,7 +478,15 @@ extern "C" int mc_balance_run(mc_ctx *c, double jscale, const mc_balance_opts *o
             return fail(c, MC_EINVAL, "secant solver needs 0 < tol < 1 and 1 <= max_iter <= 1000");
         a.solver = opts->solver;
         a.ion_tol = opts->ion_tol;
-        if (opts->solver == MC_SOLVER_SECANT) { a.tol = opts->tol; a.max_iter = opts->max_iter; }
+        if (opts->solver == MC_SOLVER_SECANT) {
+            a.tol = opts->tol;
+            a.max_iter = opts->max_iter;
+            if (opts->first_step != 0.0) {
+                if (!(opts->first_step > 0.0 && opts->first_step <= 1.0))
+                    return fail(c, MC_EINVAL, "first_step outside (0,1]");
+                a.step0 = opts->first_step;
+            }
+        }
         warm = opts->warm_start != 0;
         if (warm && !c->mlp) return fail(c, MC_ESTATE, "warm start needs mc_mlp_set");
     }
@@ -475,8 +524,11 @@ extern "C" int mc_balance_run(mc_ctx *c, double jscale, const mc_balance_opts *o
         a.ion0 = c->d_cold + 2 * nc1;
     }
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    {
+        int rc = integrals(c, a, warm ? c->d_mX : nullptr);
+        if (rc) return rc;
+    }
     if (warm) {
-        HIPCHK(c, mcb_launch_features(c->d_atomic, a, c->d_mX, c->d_hit, c->stream));
         HIPCHK(c, mcm_standardise(c->d_mX, nc, mlp_scale(c), c->d_mX0, c->stream));
         HIPCHK(c, mcm_forward(c->d_mX0, nc, mlp_net(c), c->d_mY, mcm::kOutP, nullptr, 2 * c->cus, c->stream));
         HIPCHK(c, mcm_unscale(c->d_mY, nc, mcm::kOutP, mlp_scale(c), c->stream));
@@ -560,9 +612,9 @@ extern "C" int mc_mlp_features(mc_ctx *c, double jscale, const double *Hden, flo
     if (rc) return rc;
     const size_t nc1 = (size_t)nc + 1;
     HIPCHK(c, hipMemcpyAsync(c->d_Hden, Hden, nc1 * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->d_hit, 0, nc1 * sizeof(int), c->stream));
     mcb::CellArgs a = cell_args(c, jscale);
-    HIPCHK(c, mcb_launch_features(c->d_atomic, a, c->d_mX, c->d_hit, c->stream));
+    rc = integrals(c, a, c->d_mX);
+    if (rc) return rc;
     HIPCHK(c, hipMemcpyAsync(X, c->d_mX, (size_t)nc * mcm::kIn * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(hit, c->d_hit, nc1 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -871,7 +923,7 @@ extern "C" void mc_ctx_destroy(mc_ctx *c)
                      c->d_gSca, c->d_J3, c->d_J, c->d_esc, c->d_scratch, c->d_error, c->d_qphot,
                      c->d_cell_xyz, c->d_atomic, c->d_bal_tab, c->d_Hden, c->d_Te, c->d_Ne,
                      c->d_ion, c->d_flags, c->d_nIter, c->d_cold, c->d_mlp, c->d_mlpW, c->d_mX,
-                     c->d_mY, c->d_mX0, c->d_hit };
+                     c->d_mY, c->d_mX0, c->d_hit, c->d_cross, c->d_Rint, c->d_Hint };
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

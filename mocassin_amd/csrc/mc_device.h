@@ -38,10 +38,26 @@ struct Grid {
     const float *specCDF;    // [star][nu]
     const float *nuArray;    // [nu]
     const float *gSca;       // [nu] or null
-    double *J3;              // [nu][x,y,z] dense FP64 path-length estimator
+    double *J3;              // FP64 path-length estimator, tiled: see j3_index
     double *esc;             // [nu 0..nbins][cell]
+    unsigned *cross;         // [x,y,z] segments walked in each cell, or null
     uint32_t seed_lo, seed_hi;
 };
+
+// J3 layout: lattice points in tiles of 64 consecutive (x,y,z) indices; a
+// tile holds [nu][64].  A packet's x-steps stay in one tile row, and one
+// wavefront reading a tile's spectra (64 lanes, one point each) streams
+// contiguous memory.
+constexpr int kJTile = 64;
+__host__ __device__ __forceinline__ size_t j3_index(int nb, int nu0, int xyz)
+{
+    const unsigned u = (unsigned)xyz;
+    return ((size_t)(u / kJTile) * (unsigned)nb + (unsigned)nu0) * kJTile + (size_t)(u % kJTile);
+}
+__host__ __device__ __forceinline__ size_t j3_size(int ncell3, int nb)
+{
+    return (size_t)((ncell3 + kJTile - 1) / kJTile) * kJTile * nb;
+}
 
 // Guide table (Chen & Asau 1974, "On generating random variates from an
 // empirical distribution"): g[j] = #{i : cdf[i] <= j/K}.  For u in

@@ -117,13 +117,14 @@ __global__ void __launch_bounds__(256) forward_kernel(const uint16_t *__restrict
 // ---------------------------------------------------------------- GEMM
 // C[M][N] = sum_k A(m,k) B(k,n), A(m,k) = A[m*sam + k*sak],
 // B(k,n) = B[k*sbk + n*sbn], BF16 operands staged in LDS (k contiguous),
-// 64x64 tile per workgroup, K step 32.
-enum { EPI_F32 = 0, EPI_RELU_MASK = 1 };
+// 64x64 tile per workgroup, K step 32.  EPI_F32_ADD splits K over
+// blockIdx.z (kc rows each) and adds into a zeroed C32.
+enum { EPI_F32_ADD = 0, EPI_RELU_MASK = 1 };
 
 template <int EPI>
 __global__ void __launch_bounds__(256) gemm_kernel(const uint16_t *__restrict__ A, long sam, long sak,
                                                    const uint16_t *__restrict__ B, long sbk, long sbn,
-                                                   int M, int N, int K, float *__restrict__ C32,
+                                                   int M, int N, int K, int kc, float *__restrict__ C32,
                                                    uint16_t *__restrict__ C16, int ldc,
                                                    const uint16_t *__restrict__ Hm, int ldh)
 {
@@ -135,11 +136,12 @@ __global__ void __launch_bounds__(256) gemm_kernel(const uint16_t *__restrict__ 
     f32x4 acc[4];
     for (int t = 0; t < 4; ++t) acc[t] = f32x4{ 0.f, 0.f, 0.f, 0.f };
     const int rr = tid >> 2, kk = (tid & 3) * 8;
-    for (int k0 = 0; k0 < K; k0 += 32) {
+    const int kb = blockIdx.z * kc, ke = kb + kc < K ? kb + kc : K;
+    for (int k0 = kb; k0 < ke; k0 += 32) {
         for (int i = 0; i < 8; ++i) {
             const int k = k0 + kk + i, m = m0 + rr, nn = n0 + rr;
-            As[rr * S + kk + i] = (m < M && k < K) ? A[(long)m * sam + (long)k * sak] : (uint16_t)0;
-            Bs[rr * S + kk + i] = (nn < N && k < K) ? B[(long)k * sbk + (long)nn * sbn] : (uint16_t)0;
+            As[rr * S + kk + i] = (m < M && k < ke) ? A[(long)m * sam + (long)k * sak] : (uint16_t)0;
+            Bs[rr * S + kk + i] = (nn < N && k < ke) ? B[(long)k * sbk + (long)nn * sbn] : (uint16_t)0;
         }
         __syncthreads();
         const bf16x8 a = ld8(As + (16 * w + lr) * S + lk);
@@ -150,8 +152,8 @@ __global__ void __launch_bounds__(256) gemm_kernel(const uint16_t *__restrict__ 
         for (int i = 0; i < 4; ++i) {
             const int row = m0 + 16 * w + lrow + i, col = n0 + 16 * t + lr;
             if (row >= M || col >= N) continue;
-            if (EPI == EPI_F32) {
-                C32[(long)row * ldc + col] = acc[t][i];
+            if (EPI == EPI_F32_ADD) {
+                atomicAdd(&C32[(long)row * ldc + col], acc[t][i]);
             } else {
                 const float v = bf2f(Hm[(long)row * ldh + col]) > 0.f ? acc[t][i] : 0.f;
                 C16[(long)row * ldc + col] = f2bf(v);
@@ -298,8 +300,10 @@ static hipError_t gemm(const uint16_t *A, long sam, long sak, const uint16_t *B,
                        int N, int K, float *C32, uint16_t *C16, int ldc, const uint16_t *Hm, int ldh,
                        hipStream_t s)
 {
-    hipLaunchKernelGGL(gemm_kernel<EPI>, dim3(grid1(N, 64), grid1(M, 64)), dim3(256), 0, s, A, sam, sak, B,
-                       sbk, sbn, M, N, K, C32, C16, ldc, Hm, ldh);
+    // split K in chunks of 128 rows when accumulating (the weight gradients)
+    const int kc = EPI == EPI_F32_ADD ? 128 : K;
+    hipLaunchKernelGGL(gemm_kernel<EPI>, dim3(grid1(N, 64), grid1(M, 64), grid1(K, kc)), dim3(256), 0, s, A,
+                       sam, sak, B, sbk, sbn, M, N, K, kc, C32, C16, ldc, Hm, ldh);
     return hipGetLastError();
 }
 
@@ -311,6 +315,7 @@ hipError_t mcm_backward(const uint16_t *X0, const uint16_t *H, const float *Y, c
                        dim3(256), 0, s, Y, T, M, n, dA, sums);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if ((e = hipMemsetAsync(grad, 0, kParams * sizeof(float), s)) != hipSuccess) return e;
     // gradient rows of layer l flow in cur (ld = its width); prev activation
     // of layer l: H[l-1] (or X0)
     uint16_t *cur = dA, *nxt = dB;
@@ -318,7 +323,7 @@ hipError_t mcm_backward(const uint16_t *X0, const uint16_t *H, const float *Y, c
         const int K = layer_in(l), N = layer_out(l);
         const uint16_t *prev = l == 0 ? X0 : H + (size_t)(l - 1) * n * kHid;
         // dW_l [N][K] = cur^T prev
-        e = gemm<EPI_F32>(cur, 1, N, prev, K, 1, N, K, n, grad + w_off(l), nullptr, K, nullptr, 0, s);
+        e = gemm<EPI_F32_ADD>(cur, 1, N, prev, K, 1, N, K, n, grad + w_off(l), nullptr, K, nullptr, 0, s);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(colsum_kernel, dim3(N), dim3(256), 0, s, cur, n, N, grad + b_off(l));
         if ((e = hipGetLastError()) != hipSuccess) return e;

@@ -108,7 +108,8 @@ __device__ __forceinline__ void esc_add(const Grid &G, int origin, int nuP, floa
 
 __device__ __forceinline__ void j_add(const Grid &G, int idx, int nuP, float v)
 {
-    unsafeAtomicAdd(&G.J3[(size_t)(nuP - 1) * (size_t)G.ncell3 + idx], (double)v);
+    unsafeAtomicAdd(&G.J3[j3_index(G.nbins, nuP - 1, idx)], (double)v);
+    if (G.cross) atomicAdd(&G.cross[idx], 1u);
 }
 
 // newPhotonPacket + initPhotonPacket + the start of pathSegment
@@ -362,7 +363,7 @@ __global__ void compact_kernel(const double *dense, const int *active, int ncell
     size_t tot = (size_t)ncell3 * nbins;
     for (; i < tot; i += (size_t)gridDim.x * blockDim.x) {
         int nu = (int)(i / ncell3), xyz = (int)(i % ncell3);
-        double v = dense[i];
+        double v = dense[j3_index(nbins, nu, xyz)];
         int a = active[xyz];
         if (a > 0) out[(size_t)nu * nc1 + a] = v;
         else if (v != 0.0) unsafeAtomicAdd(&out[(size_t)nu * nc1], v);

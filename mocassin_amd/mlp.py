@@ -83,8 +83,10 @@ def fit_scale(X, T, M):
     return np.concatenate([xm, xs, ym, ys]).astype(np.float32)
 
 
-def train(ctx, X, T, M, steps=2000, batch=1024, lr=2e-3, seed=0, init_seed=1, params=None):
-    """Fit the network on (X, T, M) with Adam on the device.  ctx: a
+def train(ctx, X, T, M, steps=2000, batch=1024, lr=2e-3, seed=0, init_seed=1, params=None,
+          decay=(1.0,)):
+    """Fit the network on (X, T, M) with Adam on the device: `steps` steps
+    in equal phases at lr times each factor of `decay`.  ctx: a
     mocassin_amd.Transport.  Returns (loss per step, device ms)."""
     X = np.asarray(X, np.float32)
     scale = fit_scale(X, T, M)
@@ -93,4 +95,12 @@ def train(ctx, X, T, M, steps=2000, batch=1024, lr=2e-3, seed=0, init_seed=1, pa
     n = X.shape[0]
     order = np.concatenate([rng.permutation(n) for _ in range((steps * batch + n - 1) // n)])
     order = order[:steps * batch].astype(np.int32)
-    return ctx.mlp_train(X, T, M, order, batch, lr)
+    losses, ms = [], 0.0
+    per = steps // len(decay)
+    for i, f in enumerate(decay):
+        s0 = i * per
+        s1 = steps if i == len(decay) - 1 else s0 + per
+        l, m = ctx.mlp_train(X, T, M, order[s0 * batch:s1 * batch], batch, lr * f)
+        losses.append(l)
+        ms += m
+    return np.concatenate(losses), ms

@@ -122,16 +122,27 @@ def _sigma(nu, e):
 
 def build_problem(n, edge, R_out, nH, Q_H, T_star, nbins, abundances, symmetric=True,
                   nu_min=1.e-5, nu_max=25., Te=1.e4, line_fraction=0.35, n_packets=1e8,
-                  density=None, seed=0):
+                  density=None, seed=0, nu=None):
     """Returns a dict with the grid (Fortran layouts) and tables of one
-    synthetic problem.  abundances: {Z: X/H by number}."""
+    synthetic problem.  abundances: {Z: X/H by number}; density: nH per
+    lattice point [x, y, z] (default uniform nH); nu: a frequency mesh to use
+    instead of the generated one (e.g. the one the host's atomic data are
+    tabulated on)."""
     if symmetric:
         x = axes_symmetric(n, edge)
     else:
         x = axes_full(n, edge)
     y, z = x.copy(), x.copy()
     active, nc = active_map(x, y, z, R_out, dens=density)
-    nu, wid = frequency_mesh(nbins, nu_min, nu_max, [e[0] for e in EDGES if e[3] in abundances])
+    if nu is None:
+        nu, wid = frequency_mesh(nbins, nu_min, nu_max, [e[0] for e in EDGES if e[3] in abundances])
+    else:
+        nu = np.asarray(nu, np.float32)
+        nbins = nu.size
+        wid = np.empty_like(nu)
+        wid[0] = nu[1] - nu[0]
+        wid[1:-1] = (nu[2:] - nu[:-2]) / f32(2.)
+        wid[-1] = nu[-1] - nu[-2]
     cdf = blackbody_cdf(nu, wid, T_star)
     # source luminosity from Q(H): L = Q <h nu>_ion / (fraction of energy above 1 Ryd)
     nu64 = nu.astype(np.float64)
@@ -177,37 +188,45 @@ def build_problem(n, edge, R_out, nH, Q_H, T_star, nbins, abundances, symmetric=
     xH0 = np.clip(1 - xion_c, 1e-6, 1.0)
     xH0[0] = 0.0
 
-    opac = np.zeros((nc + 1, nbins))
-    emis = np.zeros((nc + 1, nbins))
+    # tables, a block of cells at a time (elementwise in the cell index, so
+    # the blocking does not change any value)
+    opacity = np.zeros((nc + 1, nbins), np.float32, order="F")
+    rec = np.zeros((nc + 1, nbins), np.float32, order="F")
     kT = Te / HC_K_RYD
-    for e in EDGES:
-        Zel, stage = e[3], e[4]
-        if Zel not in abundances:
-            continue
-        ab = abundances[Zel]
-        # neutral stage follows H0; the next stages share the ionised part
-        frac_lo = xH0 if stage == 1 else (1 - xH0) * (0.6 if stage == 2 else 0.3)
-        sg = _sigma(nu64, e)
-        opac += (nden * ab * frac_lo)[:, None] * sg[None, :]
-        # Milne recombination continuum of the stage above
-        rec_den = nden * ab * (1 - xH0) * (1.0 if stage == 1 else 0.5)
-        shape = nu64 ** 3 * sg * np.exp(-np.clip((nu64 - e[0]) / kT, 0, 700)) * (nu64 >= e[0])
-        emis += (rec_den * nden * (1 - xH0))[:, None] * shape[None, :] * 1e3
-    # free-free emission and absorption (hydrogenic, Gaunt factor 1)
-    ne = nden * (1 - xH0)
+    edges = [(e, _sigma(nu64, e)) for e in EDGES if e[3] in abundances]
+    shapes = [nu64 ** 3 * sg * np.exp(-np.clip((nu64 - e[0]) / kT, 0, 700)) * (nu64 >= e[0])
+              for e, sg in edges]
     ff_e = np.exp(-np.clip(nu64 / kT, 0, 700))
-    emis += (ne * ne)[:, None] * ff_e[None, :] * 1e-17
-    opac += (ne * ne * 3.7e8 / np.sqrt(Te))[:, None] * (1 - np.exp(-nu64 / kT))[None, :] * \
-        (1.0 / (nu64 * 3.28984e15) ** 3)[None, :]
-    opac[0] = 0.0
-    emis[0] = 0.0
-    w = emis * wid[None, :]
-    tot = w.sum(axis=1, keepdims=True)
-    tot[tot == 0] = 1.0
-    rec = np.cumsum(w, axis=1) / tot
-    rec = rec.astype(np.float32)
-    rec[rec > f32(0.999998)] = 1.
-    rec[0] = 0.
+    ff_a = (1 - np.exp(-nu64 / kT)) * (1.0 / (nu64 * 3.28984e15) ** 3)
+    def block(c0):
+        sl = slice(c0, min(nc + 1, c0 + 16384))
+        nd, x0 = nden[sl], xH0[sl]
+        opac = np.zeros((nd.size, nbins))
+        emis = np.zeros((nd.size, nbins))
+        for (e, sg), shape in zip(edges, shapes):
+            ab = abundances[e[3]]
+            stage = e[4]
+            # neutral stage follows H0; the next stages share the ionised part
+            frac_lo = x0 if stage == 1 else (1 - x0) * (0.6 if stage == 2 else 0.3)
+            opac += (nd * ab * frac_lo)[:, None] * sg[None, :]
+            # Milne recombination continuum of the stage above
+            rec_den = nd * ab * (1 - x0) * (1.0 if stage == 1 else 0.5)
+            emis += (rec_den * nd * (1 - x0))[:, None] * shape[None, :] * 1e3
+        # free-free emission and absorption (hydrogenic, Gaunt factor 1)
+        ne = nd * (1 - x0)
+        emis += (ne * ne)[:, None] * ff_e[None, :] * 1e-17
+        opac += (ne * ne * 3.7e8 / np.sqrt(Te))[:, None] * ff_a[None, :]
+        w = emis * wid[None, :]
+        tot = w.sum(axis=1, keepdims=True)
+        tot[tot == 0] = 1.0
+        r = (np.cumsum(w, axis=1) / tot).astype(np.float32)
+        r[r > f32(0.999998)] = 1.
+        opacity[sl] = opac.astype(np.float32)
+        rec[sl] = r
+
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(8) as ex:
+        list(ex.map(block, range(1, nc + 1, 16384)))
     totalLines = np.full(nc + 1, line_fraction, np.float32)
     totalLines[0] = 0.
     geo = (x[-1] - x[-2]) / f32(2.)
@@ -218,26 +237,54 @@ def build_problem(n, edge, R_out, nH, Q_H, T_star, nbins, abundances, symmetric=
     sidx = tuple(locate_cell(a, 0.0) for a in (x, y, z))
     return dict(
         xAxis=x, yAxis=y, zAxis=z, active=active, nCells=nc, nbins=nbins, nuArray=nu,
-        widFlx=wid, opacity=np.asfortranarray(opac.astype(np.float32)),
-        recPDF=np.asfortranarray(rec), totalLines=totalLines,
+        widFlx=wid, opacity=opacity, recPDF=rec, totalLines=totalLines,
         specCDF=np.asfortranarray(cdf.reshape(nbins, 1)),
         R_out=f32(R_out), geoCorr=(geo, geo, geo), ionEdge1=f32(0.999434), deltaE=deltaE,
-        starPos=(0.0, 0.0, 0.0), starIdx=sidx, symmetric=symmetric, L36=L36)
+        starPos=(0.0, 0.0, 0.0), starIdx=sidx, symmetric=symmetric, L36=L36, Hden=nden)
 
 
 FULL_GAS = {1: 1.0, 2: 0.1, 6: 10 ** -3.6, 7: 1e-4, 8: 10 ** -3.3, 10: 1e-4, 16: 1e-5}
 H_HE = {1: 1.0, 2: 0.1}
 
 
-def workload(name, n_packets=1e8):
-    """Named workloads (edge length, density, source and mesh of the decks)."""
+def clumpy_density(n, seed, mean=300.0, sigma_ln=0.5, corr=8.0, n_clumps=200, contrast=10.0,
+                   clump_width=2.0):
+    """Lognormal nH field on an n^3 lattice [x, y, z]: Gaussian-filtered white
+    noise (correlation length `corr` cells, periodic), scaled to standard
+    deviation sigma_ln in ln nH and mean `mean`, times `n_clumps` seeded
+    Gaussian clumps of peak contrast `contrast` and width `clump_width` cells."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(seed)
+    g = gaussian_filter(rng.standard_normal((n, n, n)), corr, mode="wrap")
+    g = (g - g.mean()) / g.std()
+    mu = np.log(mean) - 0.5 * sigma_ln ** 2
+    dens = np.exp(mu + sigma_ln * g)
+    w = clump_width
+    r = int(np.ceil(4 * w))
+    off = np.arange(-r, r + 1)
+    bump = np.exp(-(off[:, None, None] ** 2 + off[None, :, None] ** 2 + off[None, None, :] ** 2) / (2 * w * w))
+    boost = np.ones_like(dens)
+    for c in rng.integers(0, n, (n_clumps, 3)):
+        sl = tuple(slice(max(0, ci - r), min(n, ci + r + 1)) for ci in c)
+        bs = tuple(slice(s_.start - (ci - r), s_.stop - (ci - r)) for s_, ci in zip(sl, c))
+        boost[sl] += (contrast - 1.0) * bump[bs]
+    return dens * boost
+
+
+def workload(name, n_packets=1e8, seed=0, nu=None):
+    """Named workloads (edge length, density, source and mesh of the decks).
+    nu: frequency mesh to use (default: generated)."""
     if name == "c2":
         return build_problem(45, 2.5e18, 2.5e18, 1.e3, 1.e49, 4.e4, 800, FULL_GAS,
-                             n_packets=n_packets)
+                             n_packets=n_packets, nu=nu)
     if name == "c1":
         return build_problem(19, 3.e18, 3.e18, 100., 4.e48, 4.e4, 400, H_HE,
-                             n_packets=n_packets)
+                             n_packets=n_packets, nu=nu)
     if name == "c2small":
         return build_problem(15, 2.5e18, 2.5e18, 1.e3, 1.e49, 4.e4, 200, FULL_GAS,
-                             n_packets=n_packets)
+                             n_packets=n_packets, nu=nu)
+    if name == "c4":
+        # clumpy 100^3 full grid, H+He gas and the 40,000 K, Q(H) = 4e48 source
+        return build_problem(100, 3.e18, 3.e18, 300., 4.e48, 4.e4, 400, H_HE, symmetric=False,
+                             n_packets=n_packets, density=clumpy_density(100, seed), nu=nu)
     raise KeyError(name)

@@ -102,6 +102,15 @@ class Transport:
         self._check(self._lib.mc_fetch_estimators(self._h, J.ctypes.data, E.ctypes.data))
         return J, E
 
+    def count_crossings(self, enable=True):
+        self._check(self._lib.mc_count_crossings(self._h, 1 if enable else 0))
+
+    def crossings(self):
+        """Segments walked in each cell (nCells+1,) since the last zeroing."""
+        out = np.zeros(self.nCells + 1, np.uint32)
+        self._check(self._lib.mc_fetch_crossings(self._h, out.ctypes.data))
+        return out
+
     def set_estimators(self, J, esc=None):
         """Overwrite the device J (nCells+1, nbins) [and esc (nCells+1, nbins+1)]."""
         J = np.asarray(J, np.float64)
@@ -139,7 +148,7 @@ class Transport:
         self._check(self._lib.mc_balance_setup(self._h, C.byref(A)))
 
     def update_cells(self, Hden, Te, Ne, ionDen, jscale=1.0, solver=None, tol=1e-3, max_iter=60,
-                     ion_tol=0.01, warm_start=False):
+                     ion_tol=0.01, warm_start=False, first_step=0.0):
         """updateCell for every active cell from the device J times jscale.
         Hden/Te/Ne (nCells+1,), ionDen (nCells+1, nElemUsed, nstages).
         solver None: the reference's schedule; "secant": secant iteration to
@@ -162,6 +171,7 @@ class Transport:
             opts.solver = _capi.MC_SOLVER_SECANT if solver == "secant" else _capi.MC_SOLVER_REFERENCE
             opts.max_iter, opts.tol, opts.ion_tol = int(max_iter), float(tol), float(ion_tol)
             opts.warm_start = 1 if warm_start else 0
+            opts.first_step = float(first_step)
         self._check(self._lib.mc_balance_run(self._h, float(jscale), None if opts is None else C.byref(opts),
                                              Hd.ctypes.data, Te.ctypes.data,
                                              Ne.ctypes.data, ion.ctypes.data, flags.ctypes.data,

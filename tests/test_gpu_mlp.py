@@ -116,6 +116,27 @@ def test_training_fits_a_smooth_map():
     print("training: loss %.3g -> %.3g, %.1f ms device" % (loss[:20].mean(), loss[-50:].mean(), ms))
 
 
+def test_features_match_their_definition():
+    """Network inputs of each cell from the device J: log10 of J summed over
+    64 contiguous groups of bins, log10 nH, log10 abundances (-30 absent)."""
+    print("mlp features", flush=True)
+    T, b, J, Hden, Te0, Ne0, ion0 = _balance_ctx("c2mini")
+    try:
+        X, hit = T.mlp_features(Hden, jscale=2.0)
+    finally:
+        T.close()
+    nb = b.nbins
+    Jd = J.astype(np.float64) * 2.0
+    for c in range(1, J.shape[0]):
+        want = [np.log10(max(Jd[c, i * nb // 64:(i + 1) * nb // 64].sum(), 1e-30)) for i in range(64)]
+        assert np.allclose(X[c - 1, :64], want, rtol=0, atol=1e-5)
+        assert hit[c] == int(np.any(Jd[c] > 0))
+        assert abs(X[c - 1, 64] - np.log10(Hden[c])) < 1e-5
+    for k, z in enumerate(MP.SLOTS_Z):
+        want = np.log10(b.abun[z - 1]) if b.elem_on[z - 1] else -30.0
+        assert np.allclose(X[:, 65 + k], want, atol=1e-5)
+
+
 def _balance_ctx(name):
     from test_gpu_balance import _context
     b, J, Hden, Te0, Ne0, ion0, ref = balance_fixture(name)
