@@ -1,13 +1,19 @@
 """Transport throughput on one MI355X: packets/s for the full-abundance
-45^3-octant workload (800 frequency bins, 1e8 packets per sweep).
+45^3-octant workload (800 frequency bins, 1e8 packets per sweep), and the
+wall time to converged Te of the plain and the warm-started balance.
 
 A step is one transport sweep: zero the estimators, transport all packets of
 the source, keep J_nu and the escape SED resident in HBM (the host copy is
 not part of the step).  Tables are synthetic of the workload's shape
-(mocassin_amd/synth.py).  Prints one JSON line with the roofline of the
-transport kernel (HIP-event timed on its own stream) and a CPU baseline: the
-C restatement of the same method (oracle/) on a bounded sample of the same
-workload, on the host's cores.
+(mocassin_amd/synth.py) on the frequency mesh of the workload's atomic data.
+Prints one JSON line with
+  roofline      of the transport kernel (HIP-event timed on its own stream)
+  cpu_baseline  the C restatement of the same method (oracle/) on a bounded
+                sample of the same workload, on the host's cores
+  balance       the per-cell balance of the 45^3 workload on the last
+                sweep's J (reference schedule, FP64): cells/s
+  te_convergence  plain vs warm-started balance on the clumpy 100^3 H+He
+                workload (mocassin_amd/converge.py; --no-te skips it)
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--packets N]
 """
@@ -77,6 +83,8 @@ def main():
     ap.add_argument("--packets", type=float, default=1e8)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-te", action="store_true", help="skip the Te-convergence measurement")
+    ap.add_argument("--no-balance", action="store_true", help="skip the balance measurement")
     ap.add_argument("--autotune", action="store_true",
                     help="time the launch-shape candidates on this workload and store the best "
                          "in mocassin_amd/tuning_db.json (default: use the stored shape)")
@@ -91,12 +99,17 @@ def main():
               "(the multi-GPU sharding is a design, see the documentation)" % a.gpus,
               file=sys.stderr)
         sys.exit(2)
+    import numpy as np
     import torch
     from mocassin_amd import Transport, synth
+    from mocassin_amd.atomic import AtomicBundle
 
     npk = int(a.packets)
     t0 = time.perf_counter()
-    p = synth.workload(a.workload, n_packets=npk)
+    atom_path = os.path.join(ROOT, "tests", "golden", "atomic_%s.npz" % a.workload)
+    atom = dict(np.load(atom_path)) if os.path.exists(atom_path) else None
+    bundle = AtomicBundle.from_npz(atom) if atom else None
+    p = synth.workload(a.workload, n_packets=npk, nu=None if bundle is None else bundle.nuArray)
     setup_s = time.perf_counter() - t0
     seed = 20241202
     T = Transport(p["xAxis"], p["yAxis"], p["zAxis"], p["active"], p["nCells"], p["nbins"],
@@ -146,6 +159,25 @@ def main():
         step += 1
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t
+    balance = None
+    if bundle is not None and not a.no_balance:
+        from mocassin_amd.converge import cold_state
+        T.balance_setup(bundle)
+        Te0, Ne0, ion0, Hd = cold_state(bundle, atom["Te0"][0], atom["ion0"], p["Hden"])
+        js = 1e-9 / (8.0 if p["symmetric"] else 1.0)
+        runs = []
+        for _ in range(2):
+            tb = time.perf_counter()
+            r = T.update_cells(Hd, Te0, Ne0, ion0, jscale=js)
+            runs.append((r[5], time.perf_counter() - tb, r))
+        ms_b, wall_b, r = min(runs, key=lambda x: x[0])
+        hit = (r[3][1:] & 16) == 0
+        balance = {"cells": int(p["nCells"]), "reached": int(hit.sum()), "device_ms": ms_b,
+                   "host_wall_ms": wall_b * 1e3, "cells_per_s": float(hit.sum()) / (ms_b * 1e-3),
+                   "mean_iterations": float(r[4][1:][hit].mean()),
+                   "solver": "reference schedule (iterateT), FP64, cold start TeStart=1e4 K"}
+        print("bench: balance %d cells in %.1f ms (%.2f T-iterations per cell)"
+              % (hit.sum(), ms_b, balance["mean_iterations"]), file=sys.stderr, flush=True)
     T.close()
 
     value = a.steps * npk / elapsed
@@ -165,7 +197,7 @@ def main():
         except (OSError, ValueError, KeyError):
             traffic = None
     out = {
-        "metric": "packets/s/GPU (transport sweep)",
+        "metric": "packets/s/GPU; wall-time to converged Te (\u00b11%) plain vs MLP warm-start",
         "value": value, "unit": "packets/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": elapsed / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
@@ -180,9 +212,24 @@ def main():
                      "kernel": "transport_kernel", "kernel_ms": kms,
                      "segments": c0["n_seg"], "absorptions": c0["n_abs"], "escapes": c0["n_esc"]},
     }
+    if balance is not None:
+        out["balance"] = balance
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, seed)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    if not a.no_te:
+        from mocassin_amd.converge import te_convergence
+        tc = te_convergence(log=lambda m: print("bench: " + m, file=sys.stderr, flush=True))
+        out["te_convergence"] = {
+            "workload": tc["workload"], "well_sampled_cells": tc["well_sampled_cells"],
+            "plain_ms": tc["runs"]["plain"]["device_ms"], "warm_ms": tc["runs"]["warm"]["device_ms"],
+            "speedup": tc["speedup_device"], "speedup_at_tol_0.01": tc["speedup_device_tol_0.01"],
+            "plain_iterations": tc["runs"]["plain"]["mean_iterations"],
+            "warm_iterations": tc["runs"]["warm"]["mean_iterations"],
+            "frac_within_1pct": {k: v["frac_within_1pct"] for k, v in tc["runs"].items()},
+            "training_ms": tc["training"]["device_ms"],
+            "note": "device time of the balance step (warm: + network inference); tolerance 1e-3 "
+                    "on |heat-cool|/heat; Te_ref from a 1e-5 solve"}
     print(json.dumps(out), flush=True)
 
 
