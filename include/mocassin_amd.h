@@ -185,6 +185,32 @@ int mc_count_crossings(mc_ctx *ctx, int32_t enable);
 /* crossings(0:nCells): the counts per cell (row 0: all inactive points). */
 int mc_fetch_crossings(mc_ctx *ctx, uint32_t *crossings);
 
+/* Launch shape of a batched transport (chosen by the autotuner; no
+ * counterpart in the reference):
+ *   waves_per_cu  resident one-wavefront workgroups requested per compute
+ *                 unit (1..32)
+ *   emit_min, walk_steps  as in mc_tuning
+ *   chunk         packets handed to a wavefront at a time (64..2^24)
+ * Results do not depend on these values. */
+typedef struct {
+    int32_t waves_per_cu;
+    int32_t emit_min;
+    int32_t walk_steps;
+    int64_t chunk;
+} mc_batch_tuning;
+
+/* Transport the packets of n independent models (contexts on one device)
+ * in one persistent launch: for each i, npackets[i] packets of source
+ * istar[i] of ctx[i], packet indices packet_offset[i] ... (NULL: 0).
+ * Each context accumulates into its own estimators and gets its own
+ * counters c[i] (nullable; kernel_ms is the whole launch); the results equal
+ * those of n separate mc_transport_run calls.  tuning may be NULL.
+ * Replaces: n independent runs of energyPacketDriver (photon_mod.f90:26-284),
+ * one per nebula. */
+int mc_transport_batch(mc_ctx *const *ctx, int32_t n, const int32_t *istar, const int64_t *npackets,
+                       const uint64_t *packet_offset, const mc_batch_tuning *tuning, mc_counters *c,
+                       double *kernel_ms);
+
 /* Copy the device estimators to the host (layouts above). */
 int mc_fetch_estimators(mc_ctx *ctx, double *J_out, double *esc_out);
 

@@ -20,7 +20,7 @@ EXPORTS = ("mc_ctx_create", "mc_upload_tables", "mc_zero_estimators", "mc_transp
            "mc_fetch_estimators", "mc_walk_replay", "mc_set_tuning", "mc_get_tuning",
            "mc_set_estimators", "mc_balance_setup", "mc_balance_run",
            "mc_mlp_set", "mc_mlp_get", "mc_mlp_features", "mc_mlp_predict", "mc_mlp_loss_grad",
-           "mc_mlp_train", "mc_count_crossings", "mc_fetch_crossings",
+           "mc_mlp_train", "mc_count_crossings", "mc_fetch_crossings", "mc_transport_batch",
            "mc_ctx_last_error", "mc_ctx_destroy")
 
 MC_SOLVER_REFERENCE, MC_SOLVER_SECANT = 0, 1
@@ -67,6 +67,14 @@ class McAtomic(C.Structure):
     ]
 
 
+class McBatchTuning(C.Structure):
+    _fields_ = [("waves_per_cu", C.c_int32), ("emit_min", C.c_int32), ("walk_steps", C.c_int32),
+                ("chunk", C.c_int64)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class McBalanceOpts(C.Structure):
     _fields_ = [("solver", C.c_int32), ("max_iter", C.c_int32), ("tol", C.c_double),
                 ("ion_tol", C.c_double), ("warm_start", C.c_int32), ("first_step", C.c_double)]
@@ -108,13 +116,15 @@ def load():
     L.mc_mlp_predict.argtypes = [vp, vp, i64, vp]
     L.mc_mlp_loss_grad.argtypes = [vp, vp, vp, vp, i64, vp, vp]
     L.mc_mlp_train.argtypes = [vp, vp, vp, vp, i64, vp, i32, i32, C.c_float, vp, vp]
+    L.mc_transport_batch.argtypes = [C.POINTER(vp), i32, vp, vp, vp, C.POINTER(McBatchTuning),
+                                     C.POINTER(McCounters), vp]
     L.mc_count_crossings.argtypes = [vp, i32]
     L.mc_fetch_crossings.argtypes = [vp, vp]
     L.mc_ctx_last_error.argtypes = [vp]
     L.mc_ctx_last_error.restype = C.c_char_p
     L.mc_ctx_destroy.argtypes = [vp]
     L.mc_ctx_destroy.restype = None
-    for name in EXPORTS[:19]:
+    for name in EXPORTS[:20]:
         getattr(L, name).restype = C.c_int
     _LIB = L
     return L

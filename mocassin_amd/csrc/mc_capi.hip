@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -865,6 +866,131 @@ extern "C" int mc_transport_run(mc_ctx *c, int32_t istar, int64_t npackets, uint
     if (herr) return fail(c, MC_EPACKET, "%s (code %d)", packet_error_text(herr), herr);
     if (J_out || esc_out) return mc_fetch_estimators(c, J_out, esc_out);
     return MC_OK;
+}
+
+extern "C" int mc_transport_batch(mc_ctx *const *ctx, int32_t n, const int32_t *istar, const int64_t *npackets,
+                                  const uint64_t *packet_offset, const mc_batch_tuning *tuning, mc_counters *cnt,
+                                  double *kernel_ms)
+{
+    if (!ctx || n < 1 || !ctx[0]) return fail(nullptr, MC_EINVAL, "no contexts");
+    mc_ctx *c0 = ctx[0];
+    if (!istar || !npackets) return fail(c0, MC_EINVAL, "istar and npackets are required");
+    mc_batch_tuning tu = { 16, 32, 8, 16384 };
+    if (tuning) {
+        if (tuning->waves_per_cu < 1 || tuning->waves_per_cu > 32 || tuning->emit_min < 1 ||
+            tuning->emit_min > 64 || tuning->walk_steps < 1 || tuning->walk_steps > 64 || tuning->chunk < 64 ||
+            tuning->chunk > (1LL << 24))
+            return fail(c0, MC_EINVAL, "batch tuning out of range");
+        tu = *tuning;
+    }
+    size_t lds = 0, nchunks = 0;
+    for (int i = 0; i < n; ++i) {
+        mc_ctx *c = ctx[i];
+        if (!c) return fail(c0, MC_EINVAL, "context %d is null", i);
+        if (c->device != c0->device) return fail(c0, MC_EINVAL, "context %d is on another device", i);
+        if (!c->tables) return fail(c0, MC_ESTATE, "context %d has no tables", i);
+        if (istar[i] < 1 || istar[i] > c->nStars) return fail(c0, MC_EINVAL, "context %d: istar %d", i, istar[i]);
+        if (npackets[i] < 0) return fail(c0, MC_EINVAL, "context %d: negative packet count", i);
+        for (int j = 0; j < i; ++j)
+            if (ctx[j] == c) return fail(c0, MC_EINVAL, "context %d appears twice", i);
+        lds = std::max(lds, mcd_lds_bytes(c->G.nx, c->G.ny, c->G.nz, c->G.nbins));
+        nchunks += (size_t)((npackets[i] + tu.chunk - 1) / tu.chunk);
+    }
+    if (nchunks > (1u << 30)) return fail(c0, MC_EINVAL, "too many chunks");
+    HIPCHK(c0, hipSetDevice(c0->device));
+    // order every context's earlier work before the launch
+    for (int i = 0; i < n; ++i) HIPCHK(c0, hipStreamSynchronize(ctx[i]->stream));
+    std::vector<mcd::Grid> G(n);
+    std::vector<mcd::RunArgs> RA(n);
+    std::vector<int> cm;
+    std::vector<long long> cf;
+    std::vector<int> cc;
+    cm.reserve(nchunks); cf.reserve(nchunks); cc.reserve(nchunks);
+    for (int i = 0; i < n; ++i) {
+        mc_ctx *c = ctx[i];
+        HIPCHK(c0, hipMemsetAsync(c->d_scratch, 0, 9 * sizeof(unsigned long long), c0->stream));
+        HIPCHK(c0, hipMemsetAsync(c->d_error, 0, sizeof(int), c0->stream));
+        HIPCHK(c0, hipMemsetAsync(c->d_qphot, 0, sizeof(double), c0->stream));
+        G[i] = c->G;
+        mcd::RunArgs &R = RA[i];
+        const int s = istar[i] - 1;
+        R.istar = istar[i];
+        R.npackets = npackets[i];
+        R.packet_offset = packet_offset ? packet_offset[i] : 0;
+        R.next_packet = c->d_scratch;
+        R.counts = c->d_scratch + 1;
+        R.error = c->d_error;
+        R.qphot = c->d_qphot;
+        R.deltaE = c->deltaE[s];
+        R.px = c->starPos[3 * s]; R.py = c->starPos[3 * s + 1]; R.pz = c->starPos[3 * s + 2];
+        R.sx = c->starIdx[3 * s]; R.sy = c->starIdx[3 * s + 1]; R.sz = c->starIdx[3 * s + 2];
+        R.emit_min = tu.emit_min;
+        R.walk_steps = tu.walk_steps;
+        for (long long f = 0; f < npackets[i]; f += tu.chunk) {
+            cm.push_back(i);
+            cf.push_back(f);
+            cc.push_back((int)std::min<long long>(tu.chunk, npackets[i] - f));
+        }
+    }
+    // one device block: grids | run arguments | chunk table | chunk counter
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t oG = 0, oR = up(n * sizeof(mcd::Grid)), oM = oR + up(n * sizeof(mcd::RunArgs));
+    const size_t oF = oM + up(cm.size() * sizeof(int)), oC = oF + up(cf.size() * sizeof(long long));
+    const size_t oN = oC + up(cc.size() * sizeof(int)), tot = oN + 256;
+    char *buf = nullptr;
+    HIPCHK(c0, dalloc(&buf, tot));
+    hipError_t e = hipMemcpyAsync(buf + oG, G.data(), n * sizeof(mcd::Grid), hipMemcpyHostToDevice, c0->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(buf + oR, RA.data(), n * sizeof(mcd::RunArgs), hipMemcpyHostToDevice, c0->stream);
+    if (e == hipSuccess && !cm.empty())
+        e = hipMemcpyAsync(buf + oM, cm.data(), cm.size() * sizeof(int), hipMemcpyHostToDevice, c0->stream);
+    if (e == hipSuccess && !cf.empty())
+        e = hipMemcpyAsync(buf + oF, cf.data(), cf.size() * sizeof(long long), hipMemcpyHostToDevice, c0->stream);
+    if (e == hipSuccess && !cc.empty())
+        e = hipMemcpyAsync(buf + oC, cc.data(), cc.size() * sizeof(int), hipMemcpyHostToDevice, c0->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(buf + oN, 0, sizeof(unsigned), c0->stream);
+    mcd::BatchArgs B;
+    B.G = reinterpret_cast<const mcd::Grid *>(buf + oG);
+    B.RA = reinterpret_cast<const mcd::RunArgs *>(buf + oR);
+    B.chunk_model = reinterpret_cast<const int *>(buf + oM);
+    B.chunk_first = reinterpret_cast<const long long *>(buf + oF);
+    B.chunk_count = reinterpret_cast<const int *>(buf + oC);
+    B.nchunks = (int)cm.size();
+    B.next_chunk = reinterpret_cast<unsigned *>(buf + oN);
+    B.emit_min = tu.emit_min;
+    B.walk_steps = tu.walk_steps;
+    if (e == hipSuccess) e = hipEventRecord(c0->ev0, c0->stream);
+    if (e == hipSuccess && B.nchunks > 0) e = mcd_launch_batch(B, lds, tu.waves_per_cu * c0->cus, c0->stream);
+    if (e == hipSuccess) e = hipEventRecord(c0->ev1, c0->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c0->stream);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, c0->ev0, c0->ev1);
+    (void)hipFree(buf);
+    if (e != hipSuccess)
+        return fail(c0, e == hipErrorOutOfMemory ? MC_ENOMEM : MC_ENODEV, "batch: %s", hipGetErrorString(e));
+    if (kernel_ms) *kernel_ms = ms;
+    int rc = MC_OK;
+    for (int i = 0; i < n; ++i) {
+        mc_ctx *c = ctx[i];
+        unsigned long long h[9];
+        int herr = 0;
+        double q = 0.0;
+        HIPCHK(c0, hipMemcpy(h, c->d_scratch, sizeof h, hipMemcpyDeviceToHost));
+        HIPCHK(c0, hipMemcpy(&herr, c->d_error, sizeof herr, hipMemcpyDeviceToHost));
+        HIPCHK(c0, hipMemcpy(&q, c->d_qphot, sizeof q, hipMemcpyDeviceToHost));
+        if (cnt) {
+            mc_counters &k = cnt[i];
+            k.n_seg = (double)h[1]; k.n_abs = (double)h[2]; k.n_sca = (double)h[3];
+            k.n_esc = (double)h[4]; k.n_lost = (double)h[5]; k.n_line = (double)h[6];
+            k.n_trapped = (double)h[7]; k.n_flights = (double)h[8];
+            k.qphot = q; k.kernel_ms = ms;
+        }
+        if (herr && rc == MC_OK) {
+            fail(c, MC_EPACKET, "%s (code %d)", packet_error_text(herr), herr);
+            rc = fail(c0, MC_EPACKET, "context %d: %s (code %d)", i, packet_error_text(herr), herr);
+        }
+    }
+    return rc;
 }
 
 extern "C" int mc_walk_replay(mc_ctx *c, int32_t n, const float *pos, const float *dir,

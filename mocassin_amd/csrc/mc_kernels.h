@@ -23,8 +23,22 @@ struct RunArgs {
     int walk_steps;                     // walk passes between emission checks
 };
 
+// Chunk table and per-model arguments of a batched launch (device memory)
+struct BatchArgs {
+    const Grid *G;                      // [models]
+    const RunArgs *RA;                  // [models] (istar, packet_offset, source, counters)
+    const int *chunk_model;             // [nchunks], non-decreasing
+    const long long *chunk_first;       // [nchunks] first packet index within the model
+    const int *chunk_count;             // [nchunks]
+    int nchunks;
+    unsigned *next_chunk;               // zeroed per launch
+    int emit_min, walk_steps;
+};
+
 }  // namespace mcd
 
+hipError_t mcd_launch_batch(const mcd::BatchArgs &B, size_t lds_bytes, int blocks, hipStream_t s);
+size_t mcd_lds_bytes(int nx, int ny, int nz, int nbins);
 hipError_t mcd_launch_transport(const mcd::Grid &G, const mcd::RunArgs &RA, int blocks,
                                 int threads, hipStream_t s);
 hipError_t mcd_densify(const float *compact, const int *active, int ncell3, int nc1, int nbins,

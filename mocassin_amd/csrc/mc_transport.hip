@@ -168,24 +168,168 @@ __device__ __forceinline__ void emit(Lane &L, const Grid &G, const RunArgs &RA, 
     L.phase = PH_WALK;
 }
 
+// Model-wide constants of one grid and source, read by the wavefront loop.
+struct Model {
+    bool fold, dust, gas, iso;
+    float dE;
+    int star_cell;
+};
+
+__device__ __forceinline__ Model model_of(const Grid &G, const RunArgs &RA)
+{
+    Model M;
+    const bool sym = (G.flags & F_SYM) != 0;
+    M.dust = (G.flags & F_DUST) != 0;
+    M.gas = (G.flags & F_GAS) != 0;
+    M.iso = (G.flags & F_ISO) != 0;
+    M.fold = sym && !(G.flags & F_MULTI);
+    M.dE = RA.deltaE;
+    M.star_cell = act(G, RA.sx, RA.sy, RA.sz);
+    return M;
+}
+
+// Stage the grid's axes, the source CDF and its guide table in LDS.
+__device__ __forceinline__ void stage_model(const Grid &G, const RunArgs &RA, float *lds, Axes &A,
+                                            float *&spec, unsigned short *&spec_guide)
+{
+    stage_axes(G, lds, A, G.nbins, G.specCDF + (size_t)(RA.istar - 1) * G.nbins, &spec);
+    spec_guide = (unsigned short *)(spec + G.nbins);
+    for (int j = threadIdx.x; j <= kGuide; j += blockDim.x) spec_guide[j] = guide_entry(spec, G.nbins, j);
+    __syncthreads();
+}
+
+// One scheduling round of a wavefront: an emission pass when at least
+// emit_min lanes (or as many as are walking) wait for one, then walk_steps
+// walk passes.  Counts events into C; a fail-stop code goes to RA.error.
+__device__ __forceinline__ void advance(Lane &L, const Grid &G, const Axes &A, const RunArgs &RA,
+                                        const Model &M, const float *spec,
+                                        const unsigned short *spec_guide, int emit_min, int walk_steps,
+                                        int npend, int nwalk, WaveCounts &C, double &qphot, int &err)
+{
+    const bool pend = (L.phase == PH_EMIT_STELLAR || L.phase == PH_EMIT_DIFFUSE);
+    // ---- emission pass, batched across the wavefront
+    if (npend >= emit_min || npend >= nwalk) {
+        bool line_end = false, esc_end = false, flight = false;
+        if (pend) {
+            emit(L, G, RA, spec, spec_guide, M.fold, M.dust, M.gas, M.star_cell, qphot, line_end,
+                 esc_end, flight, err);
+            if (err) L.phase = PH_IDLE;
+        }
+        C.line += wcount(line_end);
+        C.esc += wcount(esc_end);
+        C.flights += wcount(flight);
+        // energyPacketDriver counts the packet whose last run is the
+        // recursionLimit-th as trapped (photon_mod.f90:126)
+        C.trapped += wcount(!err && (line_end || esc_end) && L.runs >= kRecursionLimit);
+    }
+
+    // ---- walk passes
+    for (int s = 0; s < walk_steps; ++s) {
+        bool seg = false, escaped = false, lost = false, absorbed = false, scattered = false,
+             trapped = false;
+        if (L.phase == PH_WALK) {
+            L.steps++;
+            float kappa, dV;
+            int r = walk_step<false>(G, A, L.W, L.nuP, kappa, dV, err);
+            if (r == ST_CONTINUE || r == ST_ESCAPED) {
+                j_add(G, L.W.segIdx, L.nuP, L.W.seg * M.dE / dV);
+                seg = true;
+                if (r == ST_ESCAPED) { esc_add(G, L.origin, L.nuP, M.dE); escaped = true; L.phase = PH_IDLE; }
+            } else if (r == ST_LOST) {
+                lost = true;
+                L.phase = PH_IDLE;
+            } else if (r == ST_ABSORBED) {
+                j_add(G, L.W.segIdx, L.nuP, L.W.seg * M.dE / dV);
+                seg = true;
+                float rx = L.W.rx / 1.e10f, ry = L.W.ry / 1.e10f, rz = L.W.rz / 1.e10f;
+                if (sqrtf(rx * rx + ry * ry + rz * rz) * 1.e10f >= G.R_out && G.R_out > 0.f) {
+                    esc_add(G, L.origin, L.nuP, M.dE);        // 1577-1669
+                    escaped = true;
+                    L.phase = PH_IDLE;
+                } else if (M.dust) {
+                    const size_t ti = (size_t)(L.nuP - 1) * (size_t)G.ncell3 + L.W.segIdx;
+                    float probSca = G.sca3[ti] / kappa;
+                    float u = 1.f - draw(L.R, G);
+                    if (u > probSca) {
+                        absorbed = true;
+                        if (M.gas) L.phase = PH_EMIT_DIFFUSE;
+                        else err = 41;                        // dustEmi needs dust PDFs
+                    } else if (u <= probSca) {
+                        scattered = true;                     // 1720-1797
+                        L.origin = act(G, L.W.xP, L.W.yP, L.W.zP);
+                        if (M.iso) {
+                            new_direction(L.R, G, false, L.d0x, L.d0y, L.d0z);
+                        } else {
+                            for (int ihg = 1; ihg <= 10; ++ihg)
+                                if (hg(L.R, G, L.nuP, L.d0x, L.d0y, L.d0z) == 0) break;
+                        }
+                        L.W.vx = L.d0x; L.W.vy = L.d0y; L.W.vz = L.d0z;
+                        L.W.absTau = 0.f;
+                        float r2 = draw(L.R, G);
+                        L.W.passProb = -mcf_logf(1.f - r2);
+                    } else {
+                        err = 39;
+                    }
+                } else {
+                    absorbed = true;
+                    L.phase = PH_EMIT_DIFFUSE;
+                }
+            } else {
+                if (!err) err = 50;
+            }
+            if (!err) {
+                // safeLimit (2838-2846); recursionLimit (120-126)
+                if ((L.phase == PH_WALK || L.phase == PH_EMIT_DIFFUSE) && L.steps >= kSafeLimit) {
+                    lost = true;
+                    L.phase = PH_IDLE;
+                }
+                if (L.phase == PH_EMIT_DIFFUSE && L.runs >= kRecursionLimit) L.phase = PH_IDLE;
+                trapped = (L.phase == PH_IDLE) && L.runs >= kRecursionLimit;
+            } else {
+                L.phase = PH_IDLE;
+            }
+        }
+        C.seg += wcount(seg);
+        C.esc += wcount(escaped);
+        C.lost += wcount(lost);
+        C.abs += wcount(absorbed);
+        C.sca += wcount(scattered);
+        C.trapped += wcount(trapped);
+    }
+    if (err) {
+        atomicMax(RA.error, err);
+        err = 0;
+    }
+}
+
+// Add a wavefront's counters to its model's totals and clear them.
+__device__ __forceinline__ void flush_counts(const RunArgs &RA, WaveCounts &C, double &qphot)
+{
+    for (int o = 32; o >= 1; o >>= 1) qphot += __shfl_xor(qphot, o);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&RA.counts[0], C.seg);
+        atomicAdd(&RA.counts[1], C.abs);
+        atomicAdd(&RA.counts[2], C.sca);
+        atomicAdd(&RA.counts[3], C.esc);
+        atomicAdd(&RA.counts[4], C.lost);
+        atomicAdd(&RA.counts[5], C.line);
+        atomicAdd(&RA.counts[6], C.trapped);
+        atomicAdd(&RA.counts[7], C.flights);
+        unsafeAtomicAdd(RA.qphot, qphot);
+    }
+    C = WaveCounts{0, 0, 0, 0, 0, 0, 0, 0};
+    qphot = 0.0;
+}
+
 __global__ void __launch_bounds__(256) transport_kernel(Grid G, RunArgs RA)
 {
     extern __shared__ float lds[];
     Axes A;
     float *spec = nullptr;
-    stage_axes(G, lds, A, G.nbins, G.specCDF + (size_t)(RA.istar - 1) * G.nbins, &spec);
-    unsigned short *spec_guide = (unsigned short *)(spec + G.nbins);
-    for (int j = threadIdx.x; j <= kGuide; j += blockDim.x) spec_guide[j] = guide_entry(spec, G.nbins, j);
-    __syncthreads();
-
+    unsigned short *spec_guide = nullptr;
+    stage_model(G, RA, lds, A, spec, spec_guide);
     const int lane_id = threadIdx.x & 63;
-    const bool sym = (G.flags & F_SYM) != 0;
-    const bool dust = (G.flags & F_DUST) != 0;
-    const bool gas = (G.flags & F_GAS) != 0;
-    const bool iso = (G.flags & F_ISO) != 0;
-    const bool fold = sym && !(G.flags & F_MULTI);
-    const float dE = RA.deltaE;
-    const int star_cell = act(G, RA.sx, RA.sy, RA.sz);
+    const Model M = model_of(G, RA);
 
     Lane L;
     L.phase = PH_IDLE;
@@ -220,114 +364,93 @@ __global__ void __launch_bounds__(256) transport_kernel(Grid G, RunArgs RA)
         const int npend = (int)wcount(pend);
         const int nwalk = (int)wcount(L.phase == PH_WALK);
         if (npend + nwalk == 0) break;
-
-        // ---- emission pass, batched across the wavefront
-        if (npend >= RA.emit_min || npend >= nwalk) {
-            bool line_end = false, esc_end = false, flight = false;
-            if (pend) {
-                emit(L, G, RA, spec, spec_guide, fold, dust, gas, star_cell, qphot, line_end,
-                     esc_end, flight, err);
-                if (err) L.phase = PH_IDLE;
-            }
-            C.line += wcount(line_end);
-            C.esc += wcount(esc_end);
-            C.flights += wcount(flight);
-            // energyPacketDriver counts the packet whose last run is the
-            // recursionLimit-th as trapped (photon_mod.f90:126)
-            C.trapped += wcount(!err && (line_end || esc_end) && L.runs >= kRecursionLimit);
-        }
-
-        // ---- walk passes
-        for (int s = 0; s < RA.walk_steps; ++s) {
-            bool seg = false, escaped = false, lost = false, absorbed = false, scattered = false,
-                 trapped = false;
-            if (L.phase == PH_WALK) {
-                L.steps++;
-                float kappa, dV;
-                int r = walk_step<false>(G, A, L.W, L.nuP, kappa, dV, err);
-                if (r == ST_CONTINUE || r == ST_ESCAPED) {
-                    j_add(G, L.W.segIdx, L.nuP, L.W.seg * dE / dV);
-                    seg = true;
-                    if (r == ST_ESCAPED) { esc_add(G, L.origin, L.nuP, dE); escaped = true; L.phase = PH_IDLE; }
-                } else if (r == ST_LOST) {
-                    lost = true;
-                    L.phase = PH_IDLE;
-                } else if (r == ST_ABSORBED) {
-                    j_add(G, L.W.segIdx, L.nuP, L.W.seg * dE / dV);
-                    seg = true;
-                    float rx = L.W.rx / 1.e10f, ry = L.W.ry / 1.e10f, rz = L.W.rz / 1.e10f;
-                    if (sqrtf(rx * rx + ry * ry + rz * rz) * 1.e10f >= G.R_out && G.R_out > 0.f) {
-                        esc_add(G, L.origin, L.nuP, dE);        // 1577-1669
-                        escaped = true;
-                        L.phase = PH_IDLE;
-                    } else if (dust) {
-                        const size_t ti = (size_t)(L.nuP - 1) * (size_t)G.ncell3 + L.W.segIdx;
-                        float probSca = G.sca3[ti] / kappa;
-                        float u = 1.f - draw(L.R, G);
-                        if (u > probSca) {
-                            absorbed = true;
-                            if (gas) L.phase = PH_EMIT_DIFFUSE;
-                            else err = 41;                        // dustEmi needs dust PDFs
-                        } else if (u <= probSca) {
-                            scattered = true;                     // 1720-1797
-                            L.origin = act(G, L.W.xP, L.W.yP, L.W.zP);
-                            if (iso) {
-                                new_direction(L.R, G, false, L.d0x, L.d0y, L.d0z);
-                            } else {
-                                for (int ihg = 1; ihg <= 10; ++ihg)
-                                    if (hg(L.R, G, L.nuP, L.d0x, L.d0y, L.d0z) == 0) break;
-                            }
-                            L.W.vx = L.d0x; L.W.vy = L.d0y; L.W.vz = L.d0z;
-                            L.W.absTau = 0.f;
-                            float r2 = draw(L.R, G);
-                            L.W.passProb = -mcf_logf(1.f - r2);
-                        } else {
-                            err = 39;
-                        }
-                    } else {
-                        absorbed = true;
-                        L.phase = PH_EMIT_DIFFUSE;
-                    }
-                } else {
-                    if (!err) err = 50;
-                }
-                if (!err) {
-                    // safeLimit (2838-2846); recursionLimit (120-126)
-                    if ((L.phase == PH_WALK || L.phase == PH_EMIT_DIFFUSE) && L.steps >= kSafeLimit) {
-                        lost = true;
-                        L.phase = PH_IDLE;
-                    }
-                    if (L.phase == PH_EMIT_DIFFUSE && L.runs >= kRecursionLimit) L.phase = PH_IDLE;
-                    trapped = (L.phase == PH_IDLE) && L.runs >= kRecursionLimit;
-                } else {
-                    L.phase = PH_IDLE;
-                }
-            }
-            C.seg += wcount(seg);
-            C.esc += wcount(escaped);
-            C.lost += wcount(lost);
-            C.abs += wcount(absorbed);
-            C.sca += wcount(scattered);
-            C.trapped += wcount(trapped);
-        }
-        if (err) {
-            atomicMax(RA.error, err);
-            err = 0;
-        }
+        advance(L, G, A, RA, M, spec, spec_guide, RA.emit_min, RA.walk_steps, npend, nwalk, C, qphot, err);
     }
+    flush_counts(RA, C, qphot);
+}
 
-    for (int o = 32; o >= 1; o >>= 1) qphot += __shfl_xor(qphot, o);
-    if (lane_id == 0) {
-        atomicAdd(&RA.counts[0], C.seg);
-        atomicAdd(&RA.counts[1], C.abs);
-        atomicAdd(&RA.counts[2], C.sca);
-        atomicAdd(&RA.counts[3], C.esc);
-        atomicAdd(&RA.counts[4], C.lost);
-        atomicAdd(&RA.counts[5], C.line);
-        atomicAdd(&RA.counts[6], C.trapped);
-        atomicAdd(&RA.counts[7], C.flights);
-        unsafeAtomicAdd(RA.qphot, qphot);
+// Many models in one persistent launch.  The packets of every model are cut
+// into chunks, listed model after model; a one-wavefront workgroup takes
+// chunks from a global counter, refills its lanes from the chunk it holds,
+// and takes the next chunk when that one is used up.  A chunk of another
+// model is taken only after the wavefront's packets of the current model
+// have finished: then its grid and source are staged in LDS and its
+// counters flushed.  Each packet keeps its own random stream, so results
+// equal those of the models' separate runs.
+__global__ void __launch_bounds__(64) batch_kernel(BatchArgs B)
+{
+    extern __shared__ float lds[];
+    const int lane_id = threadIdx.x;
+    Axes A;
+    float *spec = nullptr;
+    unsigned short *spec_guide = nullptr;
+    // the current model's grid and arguments are read in place (wave-uniform
+    // addresses: scalar loads), not copied into registers
+    const Grid *Gp = B.G;
+    const RunArgs *RAp = B.RA;
+    Model M;
+    int cur = -1;                       // model of the staged tables
+    long long cnext = 0, cend = 0;      // packets left in the held chunk (model cur)
+    int pending = -1;                   // chunk of another model, taken after draining
+    bool done = false;                  // no chunks left
+
+    Lane L;
+    L.phase = PH_IDLE;
+    L.runs = 0; L.steps = 0; L.nuP = 1; L.origin = 0;
+    WaveCounts C = {0, 0, 0, 0, 0, 0, 0, 0};
+    double qphot = 0.0;
+    int err = 0;
+
+    for (;;) {
+        // ---- refill idle lanes from the held chunk, taking chunks of the same model
+        uint64_t m = __ballot(L.phase == PH_IDLE);
+        while (m && pending < 0 && !done) {
+            if (cnext >= cend) {
+                int c = 0;
+                if (lane_id == 0) c = (int)atomicAdd(B.next_chunk, 1u);
+                c = __builtin_amdgcn_readfirstlane(c);     // wave-uniform: scalar registers
+                if (c >= B.nchunks) { done = true; break; }
+                if (B.chunk_model[c] != cur) { pending = c; break; }
+                cnext = B.chunk_first[c];
+                cend = cnext + B.chunk_count[c];
+                continue;
+            }
+            const long long avail = cend - cnext;
+            const unsigned long long rank = __popcll(m & ((1ull << lane_id) - 1ull));
+            const bool mine = ((m >> lane_id) & 1ull) && (long long)rank < avail;
+            if (mine) {
+                rng_start(L.R, RAp->packet_offset + (unsigned long long)(cnext + (long long)rank),
+                          (uint32_t)RAp->istar);
+                L.phase = PH_EMIT_STELLAR;
+            }
+            const uint64_t took = __ballot(mine);
+            cnext += __popcll(took);
+            m &= ~took;
+        }
+        const bool pend = (L.phase == PH_EMIT_STELLAR || L.phase == PH_EMIT_DIFFUSE);
+        const int npend = (int)wcount(pend);
+        const int nwalk = (int)wcount(L.phase == PH_WALK);
+        if (npend + nwalk == 0) {
+            if (pending >= 0) {
+                // all lanes idle: switch to the pending chunk's model
+                if (cur >= 0) flush_counts(*RAp, C, qphot);
+                cur = __builtin_amdgcn_readfirstlane(B.chunk_model[pending]);
+                Gp = B.G + cur;
+                RAp = B.RA + cur;
+                __syncthreads();
+                stage_model(*Gp, *RAp, lds, A, spec, spec_guide);
+                M = model_of(*Gp, *RAp);
+                cnext = B.chunk_first[pending];
+                cend = cnext + B.chunk_count[pending];
+                pending = -1;
+                continue;
+            }
+            if (done) break;
+            continue;
+        }
+        advance(L, *Gp, A, *RAp, M, spec, spec_guide, B.emit_min, B.walk_steps, npend, nwalk, C, qphot, err);
     }
+    if (cur >= 0) flush_counts(*RAp, C, qphot);
 }
 
 // ------------------------------------------------------- table preparation
@@ -373,6 +496,17 @@ __global__ void compact_kernel(const double *dense, const int *active, int ncell
 }  // namespace mcd
 
 // ---------------------------------------------------------------- launchers
+hipError_t mcd_launch_batch(const mcd::BatchArgs &B, size_t lds_bytes, int blocks, hipStream_t s)
+{
+    hipLaunchKernelGGL(mcd::batch_kernel, dim3(blocks), dim3(64), lds_bytes, s, B);
+    return hipGetLastError();
+}
+
+size_t mcd_lds_bytes(int nx, int ny, int nz, int nbins)
+{
+    return sizeof(float) * mcd::lds_floats(nx, ny, nz, nbins);
+}
+
 hipError_t mcd_launch_transport(const mcd::Grid &G, const mcd::RunArgs &RA, int blocks,
                                 int threads, hipStream_t s)
 {

@@ -272,6 +272,32 @@ class Transport:
             pass
 
 
+def transport_batch(models, istar, npackets, offsets=None, tuning=None):
+    """One persistent launch over several Transport contexts (one nebula
+    each): npackets[i] packets of source istar[i] of models[i].  tuning: dict
+    with waves_per_cu, emit_min, walk_steps, chunk (None: defaults).  Returns
+    (counters per model, device ms of the launch)."""
+    n = len(models)
+    lib = _capi.load()
+    hs = (C.c_void_p * n)(*[m._h for m in models])
+    ist = np.ascontiguousarray(istar, np.int32).reshape(n)
+    npk = np.ascontiguousarray(npackets, np.int64).reshape(n)
+    off = None if offsets is None else np.ascontiguousarray(offsets, np.uint64).reshape(n)
+    tu = None
+    if tuning is not None:
+        tu = _capi.McBatchTuning()
+        for k, v in tuning.items():
+            setattr(tu, k, int(v))
+    cnt = (_capi.McCounters * n)()
+    ms = C.c_double()
+    rc = lib.mc_transport_batch(hs, n, ist.ctypes.data, npk.ctypes.data,
+                                None if off is None else off.ctypes.data,
+                                None if tu is None else C.byref(tu), cnt, C.byref(ms))
+    if rc:
+        raise _capi.McError(rc, lib.mc_ctx_last_error(models[0]._h).decode())
+    return [c.as_dict() for c in cnt], ms.value
+
+
 def scale_J(J, symmetric):
     """iterateMC's estimator post-processing: J *= 1e-9, then /8 in symmetric
     mode (iteration_mod.f90:706, 717-722); J in units of erg s^-1 cm^-2."""
