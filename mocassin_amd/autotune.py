@@ -1,12 +1,14 @@
-"""Autotuner for the transport kernel's launch shape.
+"""Autotuner for the transport kernels' launch shapes.
 
 The transport kernel has four free parameters (mc_tuning in
 include/mocassin_amd.h): resident workgroups per CU, workgroup size, the
 number of waiting lanes that triggers a batched emission pass, and the
-number of walk passes between emission checks.  Results do not depend on
-them (every packet draws from its own counter-based stream), so the tuner
-only times: it runs a short transport of the real problem for each
-candidate and keeps the fastest by device time.
+number of walk passes between emission checks.  The batched kernel
+(mc_batch_tuning) has resident wavefronts per CU, the same two scheduling
+parameters, and the packet-batch size handed to a wavefront at a time.
+Results do not depend on any of them (every packet draws from its own
+counter-based stream), so the tuner only times: it runs a short transport
+of the real problem for each candidate and keeps the fastest by device time.
 """
 import itertools
 import json
@@ -47,6 +49,34 @@ def autotune(transport, npackets, istar=1, grid=None, packet_offset=1 << 40, rep
     transport.set_tuning(**best)
     transport.zero_estimators()
     return best, results
+
+
+BATCH_GRID = {
+    "waves_per_cu": (8, 16),
+    "emit_min": (32,),
+    "walk_steps": (8,),
+    "chunk": (512, 1024, 4096, 16384),
+}
+
+
+def autotune_batch(models, npackets, istar=None, grid=None, packet_offset=1 << 40):
+    """Time every batched launch shape (mc_batch_tuning) on npackets[i]
+    packets of each model in one launch; return (best, [(tuning, ms)])."""
+    from .transport import transport_batch
+    g = dict(BATCH_GRID)
+    g.update(grid or {})
+    keys = list(g)
+    ist = istar if istar is not None else [1] * len(models)
+    results = []
+    for i, vals in enumerate(itertools.product(*(g[k] for k in keys))):
+        cand = dict(zip(keys, vals))
+        for m in models:
+            m.zero_estimators()
+        _, ms = transport_batch(models, ist, npackets, [packet_offset + i * max(npackets)] * len(models), cand)
+        results.append((cand, ms))
+    for m in models:
+        m.zero_estimators()
+    return min(results, key=lambda t: t[1])[0], results
 
 
 def load(workload):
