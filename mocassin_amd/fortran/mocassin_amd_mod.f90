@@ -56,6 +56,11 @@ module mocassin_amd_mod
         real(c_double) :: first_step
     end type mc_balance_opts
 
+    type, bind(C) :: mc_batch_tuning
+        integer(c_int32_t) :: waves_per_cu, emit_min, walk_steps
+        integer(c_int64_t) :: chunk
+    end type mc_batch_tuning
+
     type, bind(C) :: mc_atomic
         integer(c_int32_t) :: nstages, nElemUsed
         type(c_ptr) :: elemOn, xref, abun
@@ -132,6 +137,62 @@ module mocassin_amd_mod
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx, params, scale
         end function mc_mlp_set
+
+        integer(c_int) function mc_mlp_get(ctx, params, scale) bind(C, name="mc_mlp_get")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, params, scale
+        end function mc_mlp_get
+
+        integer(c_int) function mc_mlp_features(ctx, jscale, Hden, X, hit) bind(C, name="mc_mlp_features")
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: ctx
+            real(c_double), value :: jscale
+            type(c_ptr), value :: Hden, X, hit
+        end function mc_mlp_features
+
+        integer(c_int) function mc_mlp_predict(ctx, X, n, Y) bind(C, name="mc_mlp_predict")
+            import :: c_int, c_int64_t, c_ptr
+            type(c_ptr), value :: ctx, X
+            integer(c_int64_t), value :: n
+            type(c_ptr), value :: Y
+        end function mc_mlp_predict
+
+        integer(c_int) function mc_mlp_loss_grad(ctx, X, T, M, n, grad, loss) bind(C, name="mc_mlp_loss_grad")
+            import :: c_int, c_int64_t, c_ptr
+            type(c_ptr), value :: ctx, X, T, M
+            integer(c_int64_t), value :: n
+            type(c_ptr), value :: grad, loss
+        end function mc_mlp_loss_grad
+
+        integer(c_int) function mc_mlp_train(ctx, X, T, M, n, order, steps, batch, lr, loss, kernel_ms) &
+                bind(C, name="mc_mlp_train")
+            import :: c_int, c_int32_t, c_int64_t, c_float, c_ptr
+            type(c_ptr), value :: ctx, X, T, M
+            integer(c_int64_t), value :: n
+            type(c_ptr), value :: order
+            integer(c_int32_t), value :: steps, batch
+            real(c_float), value :: lr
+            type(c_ptr), value :: loss, kernel_ms
+        end function mc_mlp_train
+
+        integer(c_int) function mc_transport_batch(ctx, n, istar, npackets, packet_offset, tuning, cnt, &
+                                                   kernel_ms) bind(C, name="mc_transport_batch")
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx           ! array of n context handles
+            integer(c_int32_t), value :: n
+            type(c_ptr), value :: istar, npackets, packet_offset, tuning, cnt, kernel_ms
+        end function mc_transport_batch
+
+        integer(c_int) function mc_count_crossings(ctx, enable) bind(C, name="mc_count_crossings")
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: enable
+        end function mc_count_crossings
+
+        integer(c_int) function mc_fetch_crossings(ctx, crossings) bind(C, name="mc_fetch_crossings")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, crossings
+        end function mc_fetch_crossings
 
         integer(c_int) function mc_set_tuning(ctx, t) bind(C, name="mc_set_tuning")
             import :: c_int, c_ptr, mc_tuning

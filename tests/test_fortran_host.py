@@ -21,6 +21,14 @@ def _exe():
     return EXE
 
 
+def test_fortran_module_declares_every_entry_point():
+    import re
+    from test_capi_cpu import header_functions
+    src = open(os.path.join(ROOT, "mocassin_amd", "fortran", "mocassin_amd_mod.f90")).read()
+    bound = set(re.findall(r'bind\(C,\s*name="(mc_\w+)"\)', src))
+    assert set(header_functions()) == bound
+
+
 def test_fortran_selftest_reports_c_errors():
     r = subprocess.run([_exe(), "selftest"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stdout + r.stderr
