@@ -8,6 +8,8 @@ harness and the build's host read the same files.
 """
 import os
 
+import numpy as np
+
 # log10(X/H) by number for H, He, C, N, O, Ne, S (other elements absent)
 FULL_GAS = {1: 0.0, 2: -1.0, 6: -3.6, 7: -4.0, 8: -3.3, 10: -4.0, 16: -5.0}
 H_HE = {1: 0.0, 2: -1.0}
@@ -81,8 +83,40 @@ DECKS = {
 }
 
 
+def grain_sizes(n=10, amin=0.005, amax=0.25, p=3.5):
+    """Grain-size file (makeDustXsec reads: n, then index, radius [um], weight):
+    n log-spaced radii with weights a^-p."""
+    step = (np.log10(amax) - np.log10(amin)) / (n - 1)
+    rows = ["%d sizes" % n]
+    for i in range(n):
+        a = 10.0 ** (np.log10(amin) + i * step)
+        rows.append("%d %.6e %.6e" % (i + 1, a, a ** (-p)))
+    return "\n".join(rows) + "\n"
+
+
+SILICATE = "1\n'dustData/sil-dlaor.nk' 1.0\n"
+
+# dust + gas decks: keywords, abundances, extra input files
+DUST_DECKS = {
+    # small full grid, H+He gas with astronomical silicate (10 sizes, a^-3.5),
+    # dust-to-gas mass 0.01, Henyey-Greenstein scattering
+    "dust_mini": ([
+        "nx 9", "ny 9", "nz 9",
+        "edges 3.e18 3.e18 3.e18", "Rin 0.", "Rout 3.e18",
+        "Hdensity 300.", "contShape blackbody", "TStellar 50000.",
+        "LPhot 4.e12", "nbins 160", "nuMin 1.e-5", "nuMax 25.",
+        "nPhotons 20000", "maxIterateMC 1 100.", "TeStart 10000.",
+        "dustFile 'input/species.dat' 'input/sizes.dat'", "MdMg constant 0.01",
+    ], H_HE, {"species.dat": SILICATE, "sizes.dat": None}),
+}
+
+
 def write_deck(dirpath, name, overrides=None):
-    keys, abun = DECKS[name]
+    files = {}
+    if name in DUST_DECKS:
+        keys, abun, files = DUST_DECKS[name]
+    else:
+        keys, abun = DECKS[name]
     keys = list(keys)
     if overrides:
         for k, v in overrides.items():
@@ -95,4 +129,7 @@ def write_deck(dirpath, name, overrides=None):
         f.write(abundance_lines(abun))
     with open(os.path.join(dirpath, "input", "input.in"), "w") as f:
         f.write("\n".join(keys + ['nebComposition "input/abun.in"']) + "\n")
+    for fn, text in files.items():
+        with open(os.path.join(dirpath, "input", fn), "w") as f:
+            f.write(grain_sizes() if text is None else text)
     return os.path.join(dirpath, "input", "input.in")

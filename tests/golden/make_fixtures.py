@@ -185,6 +185,9 @@ if __name__ == "__main__":
     if what in ("atomic", "all"):
         make_atomic("c1atom", "c1")
         make_atomic("c2atom", "c2")
+    if what in ("dust", "all"):
+        make_transport("dust_mini", 8000, 2718, "dust_mini")
+        make_transport("dust_mini", 4000, 3141, "dust_mini_thick", opacity_scale=30.0)
     if what in ("transport", "all"):
         make_transport("c1mini", 20000, 12345, "c1mini")
         make_transport("c1mini", 8000, 999, "c1mini_thick", opacity_scale=300.0)

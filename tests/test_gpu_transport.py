@@ -71,7 +71,8 @@ def _cmp_estimators(J, E, Jo, Eo, rtol=1e-12):
     assert np.array_equal(J > 0, Jo > 0)
 
 
-@pytest.mark.parametrize("name", ["c1mini", "c1mini_thick", "full_mini_thick"])
+@pytest.mark.parametrize("name", ["c1mini", "c1mini_thick", "full_mini_thick", "dust_mini",
+                                  "dust_mini_thick"])
 def test_transport_equals_oracle(name):
     g = golden("transport_" + name)
     P = oracle_problem(g)
@@ -82,13 +83,13 @@ def test_transport_equals_oracle(name):
     c = T.energy_packet_driver(1, n, 0)
     J, E = T.fetch()
     T.close()
-    for k in ("n_seg", "n_abs", "n_esc", "n_lost", "n_line", "n_trapped", "n_flights"):
+    for k in ("n_seg", "n_abs", "n_sca", "n_esc", "n_lost", "n_line", "n_trapped", "n_flights"):
         assert c[k] == co[k], k
     assert np.isclose(c["qphot"], co["qphot"], rtol=1e-12)
     _cmp_estimators(J, E, Jo, Eo)
 
 
-@pytest.mark.parametrize("name", ["c1mini_thick", "full_mini_thick"])
+@pytest.mark.parametrize("name", ["c1mini_thick", "full_mini_thick", "dust_mini"])
 def test_transport_vs_reference_within_noise(name):
     g = golden("transport_" + name)
     P = oracle_problem(g)

@@ -118,7 +118,8 @@ def test_oracle_walk_matches_reference_bit_exact(deck):
 
 
 # ------------------------------------------------------------ full transport
-@pytest.mark.parametrize("name", ["c1mini", "c1mini_thick", "full_mini_thick"])
+@pytest.mark.parametrize("name", ["c1mini", "c1mini_thick", "full_mini_thick", "dust_mini",
+                                  "dust_mini_thick"])
 def test_oracle_transport_matches_reference_bit_exact(name):
     """libm mode + FP32 accumulation: identical bits to the reference's Jste,
     escapedPackets, Qphot and absInt under the same per-packet stream."""
@@ -130,6 +131,9 @@ def test_oracle_transport_matches_reference_bit_exact(name):
     assert np.array_equal(E.view(np.uint32), Er.view(np.uint32))
     assert np.float32(cnt["qphot"]) == g["counters"][0]
     assert cnt["n_abs"] == g["counters"][1]
+    assert cnt["n_sca"] == g["counters"][2]
+    if name.startswith("dust"):
+        assert cnt["n_sca"] > 50            # Henyey-Greenstein scatterings exercised
 
 
 @pytest.mark.parametrize("name", ["c1mini_thick", "full_mini_thick"])
