@@ -286,3 +286,25 @@ class Balance:
             ion[c] = sl.T
             nT[c] = n.value
         return Te, Ne, ion, flags, nT
+
+
+def dust_temps(J, Tdust, g, precision="f64"):
+    """getDustT for every cell (oracle/mc_dust.c).  J: scaled Jste
+    (nCells+1, nbins); Tdust: (nCells+1, nSizes+1, nSpeciesMax+1) input
+    temperatures; g: dict with dust_ints (nS, nSizes, nTemps, nsMax,
+    nSpecies), dust_abs, dust_weight, dust_abun, dust_emint.  Returns the new
+    Tdust in the same shape."""
+    L = lib()
+    dt = np.float32 if precision == "f32" else np.float64
+    f = L.mco_dust_temps_f32 if precision == "f32" else L.mco_dust_temps_f64
+    nS, nSz, nT, nsm, nSp = (int(v) for v in g["dust_ints"][:5])
+    nc1, nb = J.shape
+    Jf = np.asfortranarray(np.asarray(J, dt))
+    T = np.ascontiguousarray(np.asarray(Tdust, dt).reshape(nc1, nSz + 1, nsm + 1))
+    keep = [np.ascontiguousarray(g[k], np.float32) for k in ("dust_abs", "dust_weight", "dust_abun", "dust_emint")]
+    f.restype = None
+    f.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+    f(Jf.ctypes.data, nc1 - 1, nb, nS, nsm, nSz, nT, keep[0].ctypes.data, keep[1].ctypes.data,
+      keep[2].ctypes.data, keep[3].ctypes.data, nSp, T.ctypes.data)
+    return T

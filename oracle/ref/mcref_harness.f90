@@ -384,6 +384,12 @@ contains
        call dump_r4('Jste_scaled', grid3D(1)%Jste, (nc+1)*nbins)
        allocate(lgConvergedTemp(0:nc), lgBlackTemp(0:nc))
        allocate(NeTemp(0:nc), TeTemp(0:nc), ionDenTemp(0:nc, nElementsUsed, nStages))
+       if (lgDust) then
+          ! iteration_mod.f90:797-803
+          allocate(TdustTemp(0:nSpeciesMax, 0:nSizes, 0:nc))
+          TdustTemp = 0.
+          call dump_dust()
+       end if
        NeTemp = 0.; TeTemp = 0.; ionDenTemp = 0.
        grid3D(1)%lgConverged(0:nc) = 0
        grid3D(1)%lgBlack(0:nc) = 0
@@ -400,7 +406,35 @@ contains
        call dump_r4('bal_ionDen', ionDenTemp, (nc+1)*nElementsUsed*nStages)
        call dump_i4('bal_conv', grid3D(1)%lgConverged, nc+1)
        call dump_i4('bal_black', grid3D(1)%lgBlack, nc+1)
+       if (lgDust) call dump_r4('bal_Tdust', TdustTemp, (nSpeciesMax+1)*(nSizes+1)*(nc+1))
     end subroutine run_balance
+
+    ! Inputs of getDustT (update_mod.f90:1836-1945) for one dust chemistry:
+    ! absorption cross sections of every species and size on nuArray, size
+    ! weights, species abundances, the emission integrals dustEmIntegral
+    ! (dust_mod.f90:141-183) and the grid's initial Tdust.
+    subroutine dump_dust()
+       integer :: hdr(5), nS, ai, nsp, nc
+       real, allocatable :: xs(:)
+       nsp = 1
+       nc = grid3D(1)%nCells
+       hdr(1) = nSpeciesPart(nsp); hdr(2) = nSizes; hdr(3) = nTemps; hdr(4) = nSpeciesMax
+       hdr(5) = nSpecies
+       call dump_i4('dust_ints', hdr, 5)
+       allocate(xs(nbins * nSizes * nSpeciesPart(nsp)))
+       do nS = 1, nSpeciesPart(nsp)
+          do ai = 1, nSizes
+             do i = 1, nbins
+                xs(i + nbins * ((ai - 1) + nSizes * (nS - 1))) = xSecArray(dustAbsXsecP(nS,ai) + i - 1)
+             end do
+          end do
+       end do
+       call dump_r4('dust_abs', xs, size(xs))
+       call dump_r4('dust_weight', grainWeight, nSizes)
+       call dump_r4('dust_abun', grainAbun(nsp, 1:nSpeciesPart(nsp)), nSpeciesPart(nsp))
+       call dump_r4('dust_emint', dustEmIntegral, nSpecies * nSizes * nTemps)
+       call dump_r4('Tdust0', grid3D(1)%Tdust, (nSpeciesMax+1)*(nSizes+1)*(nc+1))
+    end subroutine dump_dust
 
     ! Geometry fixtures: zero opacity, source bin fixed, deltaE = 1.
     ! For each origin (x y z fold) and packet: record the direction the

@@ -148,8 +148,13 @@ def make_balance(deck, npk, seed, name, opacity_scale=None):
     J = rd("Jste_scaled")
     nzJ = np.nonzero(J)[0]
     out = os.path.join(HERE, "balance_%s.npz" % name)
+    dust = {}
+    if os.path.exists(os.path.join(d, "dust_ints.bin")):
+        dust = dict(dust_ints=rd("dust_ints", np.int32), dust_abs=rd("dust_abs"),
+                    dust_weight=rd("dust_weight"), dust_abun=rd("dust_abun"),
+                    dust_emint=rd("dust_emint"), Tdust0=rd("Tdust0"), Tdust=rd("bal_Tdust"))
     np.savez_compressed(
-        out, ints=prob["ints"], nuArray=prob["nuArray"], **b.to_npz_dict(),
+        out, ints=prob["ints"], nuArray=prob["nuArray"], **b.to_npz_dict(), **dust,
         Hden=rd("Hden"), Te0=rd("Te"), Ne0=rd("Ne"), ionDen0=rd("ionDen"),
         J_idx=nzJ.astype(np.int32), J_val=J[nzJ],
         Te=rd("bal_Te"), Ne=rd("bal_Ne"), ionDen=rd("bal_ionDen"),
@@ -186,6 +191,7 @@ if __name__ == "__main__":
         make_atomic("c1atom", "c1")
         make_atomic("c2atom", "c2")
     if what in ("dust", "all"):
+        make_balance("dust_bal", 20000, 1618, "dust_bal")
         make_transport("dust_mini", 8000, 2718, "dust_mini")
         make_transport("dust_mini", 4000, 3141, "dust_mini_thick", opacity_scale=30.0)
     if what in ("transport", "all"):
