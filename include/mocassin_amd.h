@@ -277,6 +277,44 @@ int mc_balance_run(mc_ctx *ctx, double jscale, const mc_balance_opts *opts, cons
                    double *Te, double *Ne, double *ionDen, int32_t *flags, int32_t *nIter,
                    double *kernel_ms);
 
+/* ---- grains ---------------------------------------------------------------
+ * One dust chemistry: the tables getDustT reads, as the host holds them after
+ * dustDriver (dust_mod.f90:14-185) and makeDustXsec (ph_mod.f90:808-1544). */
+typedef struct {
+    int32_t nSpecies;           /* grain species (nSpeciesPart(1)); nSpecies*nSizes <= 32 */
+    int32_t nSpeciesMax;        /* Tdust's first dimension is 0:nSpeciesMax (>= nSpecies) */
+    int32_t nSizes;             /* grain sizes */
+    int32_t nTemps;             /* emission integrals at 1..nTemps K (<= 8192) */
+    int32_t photoelectric;      /* lgPhotoelectric with gas: grain charge, photoelectric
+                                   heating and gas-grain collisions; 1 is refused (MC_EUNSUP) */
+    const float *absXsec;       /* [nSpecies][nSizes][nbins]: xSecArray(dustAbsXsecP(nS,ai)+i-1) */
+    const float *grainWeight;   /* (1:nSizes) */
+    const float *grainAbun;     /* (1:nSpecies): grainAbun(1,nS) */
+    const float *emIntegral;    /* dustEmIntegral(1:nSpecies, 1:nSizes, 1:nTemps), column-major */
+    float XHILimit;             /* grains without gas: |dTdust(0,0)|/Tdust(0,0) <= XHILimit
+                                   marks a cell converged (update_mod.f90:316-324) */
+} mc_dust_tables;
+
+/* Upload the grain tables (validated).  Replaces: the module data getDustT
+ * reads (update_mod.f90:1836-1945). */
+int mc_dust_setup(mc_ctx *ctx, const mc_dust_tables *dust);
+
+/* updateCell over every active cell: the gas balance of mc_balance_run when
+ * mc_balance_setup was called, then, when mc_dust_setup was called, the grain
+ * temperatures of every reached cell (FP64): the absorption integral
+ * sum_i absXsec(i) J(i)/pi of each species and size located in its emission
+ * integrals and interpolated linearly, then the size- and species-weighted
+ * means Tdust(nS,0,c) and Tdust(0,0,c).  Tdust(0:nSpeciesMax, 0:nSizes,
+ * 0:nCells) is copied in and back out (column-major); unreached cells keep
+ * it.  The gas arrays may be NULL without gas, Tdust without grains.  flags
+ * and nIter as mc_balance_run; grains without gas set flags bit0 from
+ * Tdust(0,0) and bit4 for unreached cells.  Replaces: the updateCell loop of
+ * iterateMC (iteration_mod.f90:826-838; update_mod.f90:15-335 with getDustT,
+ * 1836-1945). */
+int mc_update_cells(mc_ctx *ctx, double jscale, const mc_balance_opts *opts, const double *Hden,
+                    double *Te, double *Ne, double *ionDen, double *Tdust, int32_t *flags,
+                    int32_t *nIter, double *kernel_ms);
+
 /* ---- warm-start network (no counterpart in the reference) -------------
  * A 72 -> 128 -> 128 -> 128 -> 31 ReLU perceptron with BF16 weights and
  * activations and FP32 accumulation on the matrix cores, trained on the

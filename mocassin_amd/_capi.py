@@ -21,7 +21,7 @@ EXPORTS = ("mc_ctx_create", "mc_upload_tables", "mc_zero_estimators", "mc_transp
            "mc_set_estimators", "mc_balance_setup", "mc_balance_run",
            "mc_mlp_set", "mc_mlp_get", "mc_mlp_features", "mc_mlp_predict", "mc_mlp_loss_grad",
            "mc_mlp_train", "mc_count_crossings", "mc_fetch_crossings", "mc_transport_batch",
-           "mc_ctx_last_error", "mc_ctx_destroy")
+           "mc_dust_setup", "mc_update_cells", "mc_ctx_last_error", "mc_ctx_destroy")
 
 MC_SOLVER_REFERENCE, MC_SOLVER_SECANT = 0, 1
 MC_MLP_NIN, MC_MLP_NOUT, MC_MLP_NPARAMS, MC_MLP_NSCALE = 72, 31, 49568, 206
@@ -80,6 +80,13 @@ class McBalanceOpts(C.Structure):
                 ("ion_tol", C.c_double), ("warm_start", C.c_int32), ("first_step", C.c_double)]
 
 
+class McDustTables(C.Structure):
+    _fields_ = [("nSpecies", C.c_int32), ("nSpeciesMax", C.c_int32), ("nSizes", C.c_int32),
+                ("nTemps", C.c_int32), ("photoelectric", C.c_int32), ("absXsec", C.c_void_p),
+                ("grainWeight", C.c_void_p), ("grainAbun", C.c_void_p), ("emIntegral", C.c_void_p),
+                ("XHILimit", C.c_float)]
+
+
 class McError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("%s: %s" % (ERRORS.get(code, str(code)), msg))
@@ -118,13 +125,15 @@ def load():
     L.mc_mlp_train.argtypes = [vp, vp, vp, vp, i64, vp, i32, i32, C.c_float, vp, vp]
     L.mc_transport_batch.argtypes = [C.POINTER(vp), i32, vp, vp, vp, C.POINTER(McBatchTuning),
                                      C.POINTER(McCounters), vp]
+    L.mc_dust_setup.argtypes = [vp, C.POINTER(McDustTables)]
+    L.mc_update_cells.argtypes = [vp, C.c_double, C.POINTER(McBalanceOpts)] + [vp] * 8
     L.mc_count_crossings.argtypes = [vp, i32]
     L.mc_fetch_crossings.argtypes = [vp, vp]
     L.mc_ctx_last_error.argtypes = [vp]
     L.mc_ctx_last_error.restype = C.c_char_p
     L.mc_ctx_destroy.argtypes = [vp]
     L.mc_ctx_destroy.restype = None
-    for name in EXPORTS[:20]:
+    for name in EXPORTS[:-2]:
         getattr(L, name).restype = C.c_int
     _LIB = L
     return L

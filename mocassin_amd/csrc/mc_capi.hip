@@ -14,6 +14,7 @@
 #include "../../include/mocassin_amd.h"
 #include "mc_balance.h"
 #include "mc_device.h"
+#include "mc_dust.h"
 #include "mc_kernels.h"
 #include "mc_mlp.h"
 
@@ -60,6 +61,11 @@ struct mc_ctx {
     float *d_mX = nullptr, *d_mY = nullptr;  // inference rows [nCells][72], [nCells][32]
     uint16_t *d_mX0 = nullptr;
     int *d_hit = nullptr;
+    // grains
+    bool dust = false;
+    mcg::DustArgs dust_args{};
+    double *d_dust_tab = nullptr, *d_Tdust = nullptr;
+    int *d_dust_mono = nullptr;
     char err[512] = "";
 };
 
@@ -460,56 +466,134 @@ static int integrals(mc_ctx *c, const mcb::CellArgs &a, float *X)
     return MC_OK;
 }
 
-extern "C" int mc_balance_run(mc_ctx *c, double jscale, const mc_balance_opts *opts, const double *Hden,
-                              double *Te, double *Ne, double *ionDen, int32_t *flags, int32_t *nIter,
-                              double *kernel_ms)
+// grains: tables of getDustT (rows per (species, size)) and the state buffer
+extern "C" int mc_dust_setup(mc_ctx *c, const mc_dust_tables *d)
 {
     if (!c) return MC_EINVAL;
-    if (!c->balance) return fail(c, MC_ESTATE, "mc_balance_setup has not been called");
-    if (!Hden || !Te || !Ne || !ionDen || !flags || !nIter) return fail(c, MC_EINVAL, "null array");
-    if (!(jscale > 0.0) || std::isinf(jscale)) return fail(c, MC_EINVAL, "jscale must be finite and > 0");
-    mcb::CellArgs a = cell_args(c, jscale);
-    bool warm = false;
-    if (opts) {
-        if (opts->solver != MC_SOLVER_REFERENCE && opts->solver != MC_SOLVER_SECANT)
-            return fail(c, MC_EINVAL, "unknown solver %d", opts->solver);
-        if (!(opts->ion_tol > 0.0 && opts->ion_tol < 1.0)) return fail(c, MC_EINVAL, "ion_tol outside (0,1)");
-        if (opts->solver == MC_SOLVER_SECANT &&
-            (!(opts->tol > 0.0 && opts->tol < 1.0) || opts->max_iter < 1 || opts->max_iter > 1000))
-            return fail(c, MC_EINVAL, "secant solver needs 0 < tol < 1 and 1 <= max_iter <= 1000");
-        a.solver = opts->solver;
-        a.ion_tol = opts->ion_tol;
-        if (opts->solver == MC_SOLVER_SECANT) {
-            a.tol = opts->tol;
-            a.max_iter = opts->max_iter;
-            if (opts->first_step != 0.0) {
-                if (!(opts->first_step > 0.0 && opts->first_step <= 1.0))
-                    return fail(c, MC_EINVAL, "first_step outside (0,1]");
-                a.step0 = opts->first_step;
+    if (!d) return fail(c, MC_EINVAL, "null dust description");
+    if (d->nSpecies < 1 || d->nSizes < 1 || d->nSpecies * d->nSizes > mcg::kMaxRows)
+        return fail(c, MC_EUNSUP, "nSpecies * nSizes must be 1..%d", mcg::kMaxRows);
+    if (d->nSpeciesMax < d->nSpecies) return fail(c, MC_EINVAL, "nSpeciesMax < nSpecies");
+    if (d->nTemps < 2 || d->nTemps > mcg::kMaxTemps) return fail(c, MC_EUNSUP, "nTemps must be 2..%d", mcg::kMaxTemps);
+    if (d->photoelectric) return fail(c, MC_EUNSUP, "the grain charge and photoelectric balance is not supported");
+    if (!d->absXsec || !d->grainWeight || !d->grainAbun || !d->emIntegral)
+        return fail(c, MC_EINVAL, "null dust array");
+    if (!(d->XHILimit > 0.0)) return fail(c, MC_EINVAL, "XHILimit must be > 0");
+    const int nS = d->nSpecies, nSz = d->nSizes, nT = d->nTemps, nb = c->G.nbins, K = nS * nSz;
+    // one FP64 allocation: absx rows | emission rows | weight | abun
+    std::vector<double> tab((size_t)K * nb + (size_t)K * nT + nSz + nS);
+    std::vector<int> mono(K, 1);
+    double *absx = tab.data(), *em = absx + (size_t)K * nb, *w = em + (size_t)K * nT, *ab = w + nSz;
+    for (size_t i = 0; i < (size_t)K * nb; ++i) {
+        if (!std::isfinite(d->absXsec[i])) return fail(c, MC_ETABLE, "absXsec entry %zu not finite", i);
+        absx[i] = d->absXsec[i];
+    }
+    for (int s = 0; s < nS; ++s)
+        for (int ai = 0; ai < nSz; ++ai) {
+            const int k = s * nSz + ai;
+            for (int t = 0; t < nT; ++t) {
+                const float v = d->emIntegral[s + (size_t)nS * (ai + (size_t)nSz * t)];
+                if (!std::isfinite(v)) return fail(c, MC_ETABLE, "emIntegral(%d,%d,%d) not finite", s + 1, ai + 1, t + 1);
+                em[(size_t)k * nT + t] = v;
+                if (t > 0 && em[(size_t)k * nT + t] < em[(size_t)k * nT + t - 1]) mono[k] = 0;
             }
         }
-        warm = opts->warm_start != 0;
-        if (warm && !c->mlp) return fail(c, MC_ESTATE, "warm start needs mc_mlp_set");
+    for (int ai = 0; ai < nSz; ++ai) w[ai] = d->grainWeight[ai];
+    for (int s = 0; s < nS; ++s) ab[s] = d->grainAbun[s];
+    HIPCHK(c, hipSetDevice(c->device));
+    if (c->d_dust_tab) { (void)hipFree(c->d_dust_tab); c->d_dust_tab = nullptr; }
+    if (c->d_dust_mono) { (void)hipFree(c->d_dust_mono); c->d_dust_mono = nullptr; }
+    if (c->d_Tdust) { (void)hipFree(c->d_Tdust); c->d_Tdust = nullptr; }
+    HIPCHK(c, dalloc(&c->d_dust_tab, tab.size()));
+    HIPCHK(c, dalloc(&c->d_dust_mono, (size_t)K));
+    const size_t nc1 = (size_t)c->G.nCells + 1;
+    HIPCHK(c, dalloc(&c->d_Tdust, nc1 * (d->nSpeciesMax + 1) * (nSz + 1)));
+    HIPCHK(c, hipMemcpy(c->d_dust_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_dust_mono, mono.data(), K * sizeof(int), hipMemcpyHostToDevice));
+    if (!c->d_flags) {
+        HIPCHK(c, dalloc(&c->d_flags, nc1));
+        HIPCHK(c, dalloc(&c->d_nIter, nc1));
+    }
+    mcg::DustArgs &a = c->dust_args;
+    a = mcg::DustArgs{};
+    a.ncell3 = c->G.ncell3; a.nb = nb; a.nc = c->G.nCells;
+    a.nS = nS; a.nsMax = d->nSpeciesMax; a.nSizes = nSz; a.nTemps = nT;
+    a.absx = c->d_dust_tab; a.em = c->d_dust_tab + (size_t)K * nb;
+    a.weight = a.em + (size_t)K * nT;
+    a.abun = a.weight + nSz;
+    a.mono = c->d_dust_mono;
+    a.Tdust = c->d_Tdust;
+    a.XHILimit = d->XHILimit;
+    c->dust = true;
+    return MC_OK;
+}
+
+// updateCell over every cell: the gas balance when mc_balance_setup was
+// called, then the grain temperatures when mc_dust_setup was called
+static int update_cells(mc_ctx *c, double jscale, const mc_balance_opts *opts, const double *Hden, double *Te,
+                        double *Ne, double *ionDen, double *Tdust, int32_t *flags, int32_t *nIter,
+                        double *kernel_ms)
+{
+    const bool gas = c->balance, dust = c->dust;
+    if (!gas && !dust) return fail(c, MC_ESTATE, "neither mc_balance_setup nor mc_dust_setup has been called");
+    if (gas && (!Hden || !Te || !Ne || !ionDen)) return fail(c, MC_EINVAL, "null gas array");
+    if (dust && !Tdust) return fail(c, MC_EINVAL, "null Tdust");
+    if (!flags || !nIter) return fail(c, MC_EINVAL, "null array");
+    if (!(jscale > 0.0) || std::isinf(jscale)) return fail(c, MC_EINVAL, "jscale must be finite and > 0");
+    const int nc = c->G.nCells;
+    const size_t nc1 = (size_t)nc + 1;
+    mcb::CellArgs a{};
+    bool warm = false;
+    int nE = 0, ns = 0;
+    std::vector<double> ion;
+    if (gas) {
+        a = cell_args(c, jscale);
+        if (opts) {
+            if (opts->solver != MC_SOLVER_REFERENCE && opts->solver != MC_SOLVER_SECANT)
+                return fail(c, MC_EINVAL, "unknown solver %d", opts->solver);
+            if (!(opts->ion_tol > 0.0 && opts->ion_tol < 1.0)) return fail(c, MC_EINVAL, "ion_tol outside (0,1)");
+            if (opts->solver == MC_SOLVER_SECANT &&
+                (!(opts->tol > 0.0 && opts->tol < 1.0) || opts->max_iter < 1 || opts->max_iter > 1000))
+                return fail(c, MC_EINVAL, "secant solver needs 0 < tol < 1 and 1 <= max_iter <= 1000");
+            a.solver = opts->solver;
+            a.ion_tol = opts->ion_tol;
+            if (opts->solver == MC_SOLVER_SECANT) {
+                a.tol = opts->tol;
+                a.max_iter = opts->max_iter;
+                if (opts->first_step != 0.0) {
+                    if (!(opts->first_step > 0.0 && opts->first_step <= 1.0))
+                        return fail(c, MC_EINVAL, "first_step outside (0,1]");
+                    a.step0 = opts->first_step;
+                }
+            }
+            warm = opts->warm_start != 0;
+            if (warm && !c->mlp) return fail(c, MC_ESTATE, "warm start needs mc_mlp_set");
+        }
+        nE = c->bal.A.nE;
+        ns = c->bal.A.nstages;
+        for (int cell = 1; cell <= nc; ++cell)
+            if (!(Te[cell] > 0.0) || !(Hden[cell] > 0.0) || !(Ne[cell] > 0.0))
+                return fail(c, MC_EINVAL, "cell %d: Te, Ne and Hden must be > 0", cell);
     }
     HIPCHK(c, hipSetDevice(c->device));
-    const int nc = c->G.nCells, nE = c->bal.A.nE, ns = c->bal.A.nstages;
-    const size_t nc1 = (size_t)nc + 1;
-    for (int cell = 1; cell <= nc; ++cell)
-        if (!(Te[cell] > 0.0) || !(Hden[cell] > 0.0) || !(Ne[cell] > 0.0))
-            return fail(c, MC_EINVAL, "cell %d: Te, Ne and Hden must be > 0", cell);
-    // ionDen(0:nCells, xref, stage) -> [cell][element index][stage 0..ns]
     const size_t nion = nc1 * nE * (ns + 1);
-    std::vector<double> ion(nion, 0.0);
-    for (size_t cell = 0; cell < nc1; ++cell)
-        for (int ei = 0; ei < nE; ++ei) {
-            const size_t xr = (size_t)c->bal.eidx_xref[ei] - 1;
-            for (int st = 1; st <= ns; ++st)
-                ion[(cell * nE + ei) * (ns + 1) + st] = ionDen[cell + nc1 * (xr + (size_t)nE * (st - 1))];
-        }
-    HIPCHK(c, hipMemcpyAsync(c->d_Hden, Hden, nc1 * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->d_Te, Te, nc1 * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->d_Ne, Ne, nc1 * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->d_ion, ion.data(), nion * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    const mcg::DustArgs &da0 = c->dust_args;
+    const size_t ntd = dust ? nc1 * (da0.nsMax + 1) * (da0.nSizes + 1) : 0;
+    if (gas) {
+        // ionDen(0:nCells, xref, stage) -> [cell][element index][stage 0..ns]
+        ion.assign(nion, 0.0);
+        for (size_t cell = 0; cell < nc1; ++cell)
+            for (int ei = 0; ei < nE; ++ei) {
+                const size_t xr = (size_t)c->bal.eidx_xref[ei] - 1;
+                for (int st = 1; st <= ns; ++st)
+                    ion[(cell * nE + ei) * (ns + 1) + st] = ionDen[cell + nc1 * (xr + (size_t)nE * (st - 1))];
+            }
+        HIPCHK(c, hipMemcpyAsync(c->d_Hden, Hden, nc1 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_Te, Te, nc1 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_Ne, Ne, nc1 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_ion, ion.data(), nion * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    }
+    if (dust) HIPCHK(c, hipMemcpyAsync(c->d_Tdust, Tdust, ntd * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemsetAsync(c->d_flags, 0, nc1 * sizeof(int), c->stream));
     HIPCHK(c, hipMemsetAsync(c->d_nIter, 0, nc1 * sizeof(int), c->stream));
     if (warm) {
@@ -525,21 +609,30 @@ extern "C" int mc_balance_run(mc_ctx *c, double jscale, const mc_balance_opts *o
         a.ion0 = c->d_cold + 2 * nc1;
     }
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    {
+    if (gas) {
         int rc = integrals(c, a, warm ? c->d_mX : nullptr);
         if (rc) return rc;
+        if (warm) {
+            HIPCHK(c, mcm_standardise(c->d_mX, nc, mlp_scale(c), c->d_mX0, c->stream));
+            HIPCHK(c, mcm_forward(c->d_mX0, nc, mlp_net(c), c->d_mY, mcm::kOutP, nullptr, 2 * c->cus, c->stream));
+            HIPCHK(c, mcm_unscale(c->d_mY, nc, mcm::kOutP, mlp_scale(c), c->stream));
+            HIPCHK(c, mcb_launch_warm(c->d_atomic, a, c->d_mY, mcm::kOutP, c->d_hit, c->stream));
+        }
+        HIPCHK(c, mcb_launch_balance(c->d_atomic, a, c->stream));
     }
-    if (warm) {
-        HIPCHK(c, mcm_standardise(c->d_mX, nc, mlp_scale(c), c->d_mX0, c->stream));
-        HIPCHK(c, mcm_forward(c->d_mX0, nc, mlp_net(c), c->d_mY, mcm::kOutP, nullptr, 2 * c->cus, c->stream));
-        HIPCHK(c, mcm_unscale(c->d_mY, nc, mcm::kOutP, mlp_scale(c), c->stream));
-        HIPCHK(c, mcb_launch_warm(c->d_atomic, a, c->d_mY, mcm::kOutP, c->d_hit, c->stream));
+    if (dust) {
+        mcg::DustArgs da = da0;
+        da.jscale = jscale;
+        da.flags = gas ? nullptr : c->d_flags;      // grains alone: Tdust(0,0) decides convergence
+        HIPCHK(c, mcg_launch_dust(da, c->d_J3, c->d_active, c->stream));
     }
-    HIPCHK(c, mcb_launch_balance(c->d_atomic, a, c->stream));
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-    HIPCHK(c, hipMemcpyAsync(Te, c->d_Te, nc1 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(Ne, c->d_Ne, nc1 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(ion.data(), c->d_ion, nion * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (gas) {
+        HIPCHK(c, hipMemcpyAsync(Te, c->d_Te, nc1 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(Ne, c->d_Ne, nc1 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(ion.data(), c->d_ion, nion * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    }
+    if (dust) HIPCHK(c, hipMemcpyAsync(Tdust, c->d_Tdust, ntd * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(flags, c->d_flags, nc1 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(nIter, c->d_nIter, nc1 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -548,13 +641,35 @@ extern "C" int mc_balance_run(mc_ctx *c, double jscale, const mc_balance_opts *o
         HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
         *kernel_ms = ms;
     }
-    for (size_t cell = 0; cell < nc1; ++cell)
-        for (int ei = 0; ei < nE; ++ei) {
-            const size_t xr = (size_t)c->bal.eidx_xref[ei] - 1;
-            for (int st = 1; st <= ns; ++st)
-                ionDen[cell + nc1 * (xr + (size_t)nE * (st - 1))] = ion[(cell * nE + ei) * (ns + 1) + st];
-        }
+    if (gas)
+        for (size_t cell = 0; cell < nc1; ++cell)
+            for (int ei = 0; ei < nE; ++ei) {
+                const size_t xr = (size_t)c->bal.eidx_xref[ei] - 1;
+                for (int st = 1; st <= ns; ++st)
+                    ionDen[cell + nc1 * (xr + (size_t)nE * (st - 1))] = ion[(cell * nE + ei) * (ns + 1) + st];
+            }
     return MC_OK;
+}
+
+extern "C" int mc_balance_run(mc_ctx *c, double jscale, const mc_balance_opts *opts, const double *Hden,
+                              double *Te, double *Ne, double *ionDen, int32_t *flags, int32_t *nIter,
+                              double *kernel_ms)
+{
+    if (!c) return MC_EINVAL;
+    if (!c->balance) return fail(c, MC_ESTATE, "mc_balance_setup has not been called");
+    const bool dust = c->dust;
+    c->dust = false;                                 // the gas alone
+    const int rc = update_cells(c, jscale, opts, Hden, Te, Ne, ionDen, nullptr, flags, nIter, kernel_ms);
+    c->dust = dust;
+    return rc;
+}
+
+extern "C" int mc_update_cells(mc_ctx *c, double jscale, const mc_balance_opts *opts, const double *Hden,
+                               double *Te, double *Ne, double *ionDen, double *Tdust, int32_t *flags,
+                               int32_t *nIter, double *kernel_ms)
+{
+    if (!c) return MC_EINVAL;
+    return update_cells(c, jscale, opts, Hden, Te, Ne, ionDen, Tdust, flags, nIter, kernel_ms);
 }
 
 // ------------------------------------------------------- warm-start network
@@ -1049,7 +1164,8 @@ extern "C" void mc_ctx_destroy(mc_ctx *c)
                      c->d_gSca, c->d_J3, c->d_J, c->d_esc, c->d_scratch, c->d_error, c->d_qphot,
                      c->d_cell_xyz, c->d_atomic, c->d_bal_tab, c->d_Hden, c->d_Te, c->d_Ne,
                      c->d_ion, c->d_flags, c->d_nIter, c->d_cold, c->d_mlp, c->d_mlpW, c->d_mX,
-                     c->d_mY, c->d_mX0, c->d_hit, c->d_cross, c->d_Rint, c->d_Hint };
+                     c->d_mY, c->d_mX0, c->d_hit, c->d_cross, c->d_Rint, c->d_Hint,
+                     c->d_dust_tab, c->d_Tdust, c->d_dust_mono };
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

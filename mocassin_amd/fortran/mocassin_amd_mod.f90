@@ -71,6 +71,12 @@ module mocassin_amd_mod
         real(c_float) :: XHILimit
     end type mc_atomic
 
+    type, bind(C) :: mc_dust_tables
+        integer(c_int32_t) :: nSpecies, nSpeciesMax, nSizes, nTemps, photoelectric
+        type(c_ptr) :: absXsec, grainWeight, grainAbun, emIntegral
+        real(c_float) :: XHILimit
+    end type mc_dust_tables
+
     interface
         integer(c_int) function mc_ctx_create(desc, ctx) bind(C, name="mc_ctx_create")
             import :: c_int, c_ptr, mc_desc
@@ -132,6 +138,20 @@ module mocassin_amd_mod
             real(c_double), value :: jscale
             type(c_ptr), value :: opts, Hden, Te, Ne, ionDen, flags, nIter, kernel_ms
         end function mc_balance_run
+
+        integer(c_int) function mc_dust_setup(ctx, dust) bind(C, name="mc_dust_setup")
+            import :: c_int, c_ptr, mc_dust_tables
+            type(c_ptr), value :: ctx
+            type(mc_dust_tables), intent(in) :: dust
+        end function mc_dust_setup
+
+        integer(c_int) function mc_update_cells(ctx, jscale, opts, Hden, Te, Ne, ionDen, Tdust, flags, &
+                                                nIter, kernel_ms) bind(C, name="mc_update_cells")
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: ctx
+            real(c_double), value :: jscale
+            type(c_ptr), value :: opts, Hden, Te, Ne, ionDen, Tdust, flags, nIter, kernel_ms
+        end function mc_update_cells
 
         integer(c_int) function mc_mlp_set(ctx, params, scale) bind(C, name="mc_mlp_set")
             import :: c_int, c_ptr
@@ -304,5 +324,57 @@ contains
         noTeBal = count(iand(fl(1:), 4) /= 0)
         deallocate (dH, dT, dN, dI, fl, nIt)
     end subroutine gpu_update_cells
+
+    ! The updateCell loop with grains (mc_dust_setup first): as
+    ! gpu_update_cells for the gas, and Tdust(0:nSpeciesMax, 0:nSizes,
+    ! 0:nCells), REAL as the host keeps it, rewritten in every reached cell
+    ! (getDustT, update_mod.f90:1836-1945).  Without gas (lgGas false) the
+    ! gas arrays are not read and lgConverged follows Tdust(0,0)
+    ! (update_mod.f90:306-324).
+    subroutine gpu_update_cells_dust(ctx, jscale, lgGas, Hden, Te, Ne, ionDen, Tdust, lgConverged, &
+                                     lgBlack, noHit, noIonBal, noTeBal)
+        type(c_ptr), intent(in) :: ctx
+        real(c_double), intent(in) :: jscale
+        logical, intent(in) :: lgGas
+        real, intent(in) :: Hden(0:)
+        real, intent(inout) :: Te(0:), Ne(0:), ionDen(0:, :, :), Tdust(0:, 0:, 0:)
+        integer, intent(inout) :: lgConverged(0:)
+        integer, intent(out) :: lgBlack(0:)
+        integer, intent(out) :: noHit, noIonBal, noTeBal
+        real(c_double), allocatable, target :: dH(:), dT(:), dN(:), dI(:, :, :), dD(:, :, :)
+        integer(c_int32_t), allocatable, target :: fl(:), nIt(:)
+        integer :: nc1
+        nc1 = size(Tdust, 3)
+        allocate (fl(0:nc1 - 1), nIt(0:nc1 - 1))
+        allocate (dD(0:size(Tdust, 1) - 1, 0:size(Tdust, 2) - 1, 0:nc1 - 1))
+        dD = real(Tdust, c_double)
+        if (lgGas) then
+            allocate (dH(0:nc1 - 1), dT(0:nc1 - 1), dN(0:nc1 - 1))
+            allocate (dI(0:nc1 - 1, size(ionDen, 2), size(ionDen, 3)))
+            dH = real(Hden, c_double)
+            dT = real(Te, c_double)
+            dN = real(Ne, c_double)
+            dI = real(ionDen, c_double)
+            call mc_check(mc_update_cells(ctx, jscale, c_null_ptr, c_loc(dH), c_loc(dT), c_loc(dN), &
+                                          c_loc(dI), c_loc(dD), c_loc(fl), c_loc(nIt), c_null_ptr), &
+                          ctx, 'updateCell')
+            Te = real(dT)
+            Ne = real(dN)
+            ionDen = real(dI)
+            deallocate (dH, dT, dN, dI)
+        else
+            call mc_check(mc_update_cells(ctx, jscale, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, &
+                                          c_null_ptr, c_loc(dD), c_loc(fl), c_loc(nIt), c_null_ptr), &
+                          ctx, 'updateCell')
+        end if
+        Tdust = real(dD)
+        where (iand(fl, 16) == 0) lgConverged = iand(fl, 1)
+        lgBlack = ishft(iand(fl, 2), -1)
+        lgBlack(0) = 0
+        noHit = count(iand(fl(1:), 16) /= 0)
+        noIonBal = count(iand(fl(1:), 8) /= 0)
+        noTeBal = count(iand(fl(1:), 4) /= 0)
+        deallocate (fl, nIt, dD)
+    end subroutine gpu_update_cells_dust
 
 end module mocassin_amd_mod

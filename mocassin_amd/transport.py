@@ -147,6 +147,17 @@ class Transport:
         self._nE, self._ns = b.nElemUsed, b.nstages
         self._check(self._lib.mc_balance_setup(self._h, C.byref(A)))
 
+    @staticmethod
+    def _opts(solver=None, tol=1e-3, max_iter=60, ion_tol=0.01, warm_start=False, first_step=0.0):
+        if solver is None and not warm_start and ion_tol == 0.01:
+            return None
+        opts = _capi.McBalanceOpts()
+        opts.solver = _capi.MC_SOLVER_SECANT if solver == "secant" else _capi.MC_SOLVER_REFERENCE
+        opts.max_iter, opts.tol, opts.ion_tol = int(max_iter), float(tol), float(ion_tol)
+        opts.warm_start = 1 if warm_start else 0
+        opts.first_step = float(first_step)
+        return opts
+
     def update_cells(self, Hden, Te, Ne, ionDen, jscale=1.0, solver=None, tol=1e-3, max_iter=60,
                      ion_tol=0.01, warm_start=False, first_step=0.0):
         """updateCell for every active cell from the device J times jscale.
@@ -165,18 +176,59 @@ class Transport:
         flags = np.zeros(nc1, np.int32)
         nIt = np.zeros(nc1, np.int32)
         ms = C.c_double()
-        opts = None
-        if solver is not None or warm_start or ion_tol != 0.01:
-            opts = _capi.McBalanceOpts()
-            opts.solver = _capi.MC_SOLVER_SECANT if solver == "secant" else _capi.MC_SOLVER_REFERENCE
-            opts.max_iter, opts.tol, opts.ion_tol = int(max_iter), float(tol), float(ion_tol)
-            opts.warm_start = 1 if warm_start else 0
-            opts.first_step = float(first_step)
+        opts = self._opts(solver, tol, max_iter, ion_tol, warm_start, first_step)
         self._check(self._lib.mc_balance_run(self._h, float(jscale), None if opts is None else C.byref(opts),
                                              Hd.ctypes.data, Te.ctypes.data,
                                              Ne.ctypes.data, ion.ctypes.data, flags.ctypes.data,
                                              nIt.ctypes.data, C.byref(ms)))
         return Te, Ne, np.ascontiguousarray(ion), flags, nIt, ms.value
+
+    # ---- grains
+    def dust_setup(self, absXsec, grainWeight, grainAbun, emIntegral, nTemps, nSpeciesMax=None,
+                   XHILimit=0.01, photoelectric=False):
+        """Upload the tables of getDustT for one dust chemistry.  absXsec
+        (nSpecies, nSizes, nbins); emIntegral (nSpecies, nSizes, nTemps)."""
+        keep = [np.ascontiguousarray(absXsec, np.float32), np.ascontiguousarray(grainWeight, np.float32),
+                np.ascontiguousarray(grainAbun, np.float32),
+                np.asfortranarray(np.asarray(emIntegral, np.float32))]
+        nS, nSz, nb = keep[0].shape
+        if nb != self.nbins or keep[3].shape != (nS, nSz, int(nTemps)):
+            raise ValueError("dust tables do not match nbins / (nSpecies, nSizes, nTemps)")
+        D = _capi.McDustTables()
+        D.nSpecies, D.nSizes, D.nTemps = nS, nSz, int(nTemps)
+        D.nSpeciesMax = nS if nSpeciesMax is None else int(nSpeciesMax)
+        D.photoelectric = 1 if photoelectric else 0
+        D.absXsec, D.grainWeight, D.grainAbun, D.emIntegral = (k.ctypes.data for k in keep)
+        D.XHILimit = float(XHILimit)
+        self._check(self._lib.mc_dust_setup(self._h, C.byref(D)))
+        self._dust_shape = (D.nSpeciesMax + 1, nSz + 1)
+
+    def update_grains(self, Tdust, jscale=1.0, Hden=None, Te=None, Ne=None, ionDen=None, **balance):
+        """updateCell with grains (mc_update_cells): Tdust (nCells+1, nSizes+1,
+        nSpeciesMax+1) in, the new one out; with gas, also the balance of
+        update_cells (its keyword options apply).  Returns (Tdust, flags,
+        nIter, kernel_ms) and, with gas, (Te, Ne, ionDen) after them."""
+        nc1 = self.nCells + 1
+        ld, nsz = self._dust_shape
+        T = np.array(Tdust, np.float64).reshape(nc1, nsz, ld)
+        flags = np.zeros(nc1, np.int32)
+        nIt = np.zeros(nc1, np.int32)
+        ms = C.c_double()
+        gas = [None] * 4
+        opts = None
+        if Hden is not None:
+            gas = [np.ascontiguousarray(Hden, np.float64).reshape(nc1), np.array(Te, np.float64).reshape(nc1),
+                   np.array(Ne, np.float64).reshape(nc1),
+                   np.asfortranarray(np.array(ionDen, np.float64).reshape(nc1, self._nE, self._ns))]
+            opts = self._opts(**balance)
+        ptr = [None if g is None else g.ctypes.data for g in gas]
+        self._check(self._lib.mc_update_cells(self._h, float(jscale), None if opts is None else C.byref(opts),
+                                              *ptr, T.ctypes.data, flags.ctypes.data, nIt.ctypes.data,
+                                              C.byref(ms)))
+        out = (T, flags, nIt, ms.value)
+        if Hden is not None:
+            out += (gas[1], gas[2], np.ascontiguousarray(gas[3]))
+        return out
 
     # ---- warm-start network
     def set_mlp(self, params, scale):

@@ -7,7 +7,7 @@ import pytest
 import mc_oracle as O
 from conftest import balance_fixture
 
-NAMES = ["c1mini", "c2mini", "c2mini_thick"]
+NAMES = ["c1mini", "c2mini", "c2mini_thick", "dust_bal"]
 
 
 @pytest.mark.parametrize("name", NAMES)

@@ -7,7 +7,7 @@ from conftest import balance_fixture, gpu_available
 
 pytestmark = pytest.mark.gpu
 
-NAMES = ["c1mini", "c2mini", "c2mini_thick"]
+NAMES = ["c1mini", "c2mini", "c2mini_thick", "dust_bal"]
 
 
 def _context(nc, nb, nu, permute=False, seed=0):
