@@ -268,7 +268,9 @@ typedef struct {
  * from the given state after a failed warm-started solve.  nIter:
  * temperature iterations used.  kernel_ms (nullable): device time of the
  * balance (and of the network's inference when warm-starting).
- * opts NULL = MC_SOLVER_REFERENCE, ion_tol 0.01, no warm start.
+ * opts NULL = MC_SOLVER_REFERENCE, ion_tol 0.01, no warm start.  Refused
+ * (MC_ESTATE) when grains with the photoelectric effect are set up: that
+ * balance needs Tdust, through mc_update_cells.
  * Replaces: updateCell (update_mod.f90:15-525) with iterateT, ionBalance,
  * thermBalance, calcAlpha, radRecFit, dielectronic and forLines/equilibrium
  * (update_mod.f90:339-1834, emission_mod.f90:1913-2190), called per cell by
@@ -285,18 +287,26 @@ typedef struct {
     int32_t nSpeciesMax;        /* Tdust's first dimension is 0:nSpeciesMax (>= nSpecies) */
     int32_t nSizes;             /* grain sizes */
     int32_t nTemps;             /* emission integrals at 1..nTemps K (<= 8192) */
-    int32_t photoelectric;      /* lgPhotoelectric with gas: grain charge, photoelectric
-                                   heating and gas-grain collisions; 1 is refused (MC_EUNSUP) */
+    int32_t photoelectric;      /* lgPhotoelectric: with gas, the grain charge, photoelectric
+                                   heating and gas-grain collisions (update_mod.f90:527-932)
+                                   enter the gas balance and the grain temperatures; needs
+                                   nbins <= 4096 and the five arrays below */
     const float *absXsec;       /* [nSpecies][nSizes][nbins]: xSecArray(dustAbsXsecP(nS,ai)+i-1) */
     const float *grainWeight;   /* (1:nSizes) */
     const float *grainAbun;     /* (1:nSpecies): grainAbun(1,nS) */
     const float *emIntegral;    /* dustEmIntegral(1:nSpecies, 1:nSizes, 1:nTemps), column-major */
+    const float *grainRadius;   /* (1:nSizes) [um]                        photoelectric only */
+    const float *grainVn;       /* (1:nSpecies) [Ryd]                      photoelectric only */
+    const float *MsurfAtom;     /* (1:nSpecies), as the species file gives it  photoelectric only */
+    const float *aWeight;       /* (1:30) atomic weights                   photoelectric only */
+    const float *Ndust;         /* (0:nCells) grain number density         photoelectric only */
     float XHILimit;             /* grains without gas: |dTdust(0,0)|/Tdust(0,0) <= XHILimit
                                    marks a cell converged (update_mod.f90:316-324) */
 } mc_dust_tables;
 
 /* Upload the grain tables (validated).  Replaces: the module data getDustT
- * reads (update_mod.f90:1836-1945). */
+ * and the grain charge and photoelectric routines read (update_mod.f90:
+ * 527-932, 1836-1945). */
 int mc_dust_setup(mc_ctx *ctx, const mc_dust_tables *dust);
 
 /* updateCell over every active cell: the gas balance of mc_balance_run when
@@ -304,7 +314,12 @@ int mc_dust_setup(mc_ctx *ctx, const mc_dust_tables *dust);
  * temperatures of every reached cell (FP64): the absorption integral
  * sum_i absXsec(i) J(i)/pi of each species and size located in its emission
  * integrals and interpolated linearly, then the size- and species-weighted
- * means Tdust(nS,0,c) and Tdust(0,0,c).  Tdust(0:nSpeciesMax, 0:nSizes,
+ * means Tdust(nS,0,c) and Tdust(0,0,c).  With gas and photoelectric = 1, every
+ * temperature iteration of the gas solves each grain's potential
+ * (setGrainPotential), adds the photoelectric heating and the gas-grain
+ * collision cooling to the thermal balance, and the grain temperatures take
+ * the last iteration's collision heating and photoelectric loss
+ * (update_mod.f90:367-388, 1252-1255, 1886-1889).  Tdust(0:nSpeciesMax, 0:nSizes,
  * 0:nCells) is copied in and back out (column-major); unreached cells keep
  * it.  The gas arrays may be NULL without gas, Tdust without grains.  flags
  * and nIter as mc_balance_run; grains without gas set flags bit0 from

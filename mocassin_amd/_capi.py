@@ -84,7 +84,8 @@ class McDustTables(C.Structure):
     _fields_ = [("nSpecies", C.c_int32), ("nSpeciesMax", C.c_int32), ("nSizes", C.c_int32),
                 ("nTemps", C.c_int32), ("photoelectric", C.c_int32), ("absXsec", C.c_void_p),
                 ("grainWeight", C.c_void_p), ("grainAbun", C.c_void_p), ("emIntegral", C.c_void_p),
-                ("XHILimit", C.c_float)]
+                ("grainRadius", C.c_void_p), ("grainVn", C.c_void_p), ("MsurfAtom", C.c_void_p),
+                ("aWeight", C.c_void_p), ("Ndust", C.c_void_p), ("XHILimit", C.c_float)]
 
 
 class McError(RuntimeError):

@@ -62,6 +62,26 @@ struct Atomic {
     const double *cel_e, *cel_a, *cel_q, *cel_y2, *cel_t, *cel_g;
 };
 
+// Grains of one dust chemistry for the grain charge, photoelectric heating
+// and gas-grain collision terms (update_mod.f90:527-932); rows k = (species
+// - 1) * nSizes + size - 1.
+constexpr int kMaxGrainRows = 32;
+struct Grains {
+    int nS, nSizes, nsMax, K;
+    double vn[kMaxGrainRows], msurf[kMaxGrainRows];       // grainVn, MsurfAtom per species
+    double gabun[kMaxGrainRows];                           // grainAbun per species
+    double radius[kMaxGrainRows], weight[kMaxGrainRows];   // grainRadius [um], grainWeight per size
+    double aweight[30];                                    // aWeight by atomic number - 1
+    const double *absx;         // [K][nbins] absorption cross-sections
+    const double *qa;           // [K][nbins] absx / (1e-8 radius^2)
+    const double *yn;           // [nS][nbins] min(0.5 (1 - vn/nu), 0.2)
+    const double *ndust;        // Ndust(0:nCells)
+    const double *Tdust;        // Tdust(0:nsMax, 0:nSizes, 0:nCells) before the update
+    double *gdc, *peh;          // [nCells+1][K]: gasDustColl_d, photoelHeat_d of the last T-iteration
+    const double *J3;           // the estimator, tiled (mcd::j3_index)
+    const int *cell_xyz;        // lattice point of each cell
+};
+
 struct CellArgs {
     int ncells;                 // active cells (1..nCells)
     double jscale;              // J = J3 * jscale
@@ -78,6 +98,10 @@ struct CellArgs {
     double step0;               // secant: relative size of the first trial step
     // state to restart from when the solve from Te/Ne/ion fails (nullable)
     const double *Te0, *Ne0, *ion0;
+    // grains with the photoelectric effect (nullable: gas alone); device
+    // pointer, with the frequency-bin count for the launch's LDS size
+    const Grains *grains;
+    int grains_nb;
 };
 
 // Host-built tables: `A` with its device pointers unset, and the flat arrays

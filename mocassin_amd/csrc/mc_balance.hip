@@ -103,7 +103,7 @@ __device__ double splint(const double *xa, const double *ya, const double *y2, i
 
 // CEL cooling of one model ion [erg/s per H], levels 1..10 (forLines +
 // equilibrium + the cooling sum of thermBalance)
-__device__ double cel_cool(const Atomic &A, const CelIon &ci, Lds &S, int lane, double Te)
+__device__ __forceinline__ double cel_cool(const Atomic &A, const CelIon &ci, Lds &S, int lane, double Te)
 {
     const int nl = ci.nl, nt = ci.nt;
     const double Ne = S.Ne;
@@ -210,7 +210,7 @@ __device__ double cel_cool(const Atomic &A, const CelIon &ci, Lds &S, int lane, 
 // total cooling at temperature T for the cell's current ionisation and Ne
 // (thermBalance, update_mod.f90:933-1113: H and He free-free and
 // recombination fits, H collisional excitation, CEL lines); valid on lane 0
-__device__ double cooling(const Atomic &A, Lds &S, int lane, double T)
+__device__ __forceinline__ double cooling(const Atomic &A, Lds &S, int lane, double T)
 {
     double cool = 0.0;
     for (int k = 0; k < A.ncel; ++k) {
@@ -250,9 +250,201 @@ __device__ double cooling(const Atomic &A, Lds &S, int lane, double T)
     return coolFF + coolRec + coolColl;
 }
 
+// ---- grains: charge, photoelectric heating, gas-grain collisions ---------
+// (update_mod.f90:527-932).  Per temperature iteration, after the ion fixed
+// point: every grain row's potential (setGrainPotential's secant iteration,
+// at most 100 steps), then the photoelectric heating and cooling
+// (setPhotoelHeatCool) and the collision terms (setDustGasCollHeatCool).
+// The cell's photon flux J/(h c Ryd nu) per bin is staged in LDS once per
+// cell; the bin sums run across lanes, the collision sums with one lane per
+// grain row in the reference's order of elements and stages.
+constexpr double kPiR = (double)3.141592654f;             // constants_mod.f90:32 (REAL)
+constexpr double kFourPiR = (double)(4.f * 3.141592654f);
+constexpr double kHcRyd = 2.1799153e-11, kRyd2erg = 2.1799e-11, kKB = 1.380658e-16;
+constexpr double kAmu = 1.66053e-24, kMe = 9.109382e-28;
+
+// dynamic LDS of the grain variant, after the cell's photon flux pf[nbins]
+struct GrainLds {
+    double w[kMaxElem][kMaxStage];   // n(ion) v_mean of each (element, stage >= 2)
+    double pot[kMaxGrainRows], gdc[kMaxGrainRows], peh[kMaxGrainRows];
+    double gdcg, pehg;
+};
+
+__device__ __forceinline__ double minr(double a, double b) { return a < b ? a : b; }
+__device__ __forceinline__ double maxr(double a, double b) { return a > b ? a : b; }
+
+// locate(nuArray, x) on the increasing frequency mesh (interpolation_mod.f90:
+// 48-81): nb when x > nu(nb), 0 when x < nu(1), else the last i with
+// nu(i) <= x, at least 1 (x == nu(nb) gives 1)
+__device__ int locate_nu(const double *nu, int n, double x)
+{
+    if (x > nu[n - 1]) return n;
+    if (x < nu[0]) return 0;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (nu[mid] > x) hi = mid; else lo = mid + 1;
+    }
+    const int p = lo < n ? lo + 1 : 0;
+    return p - 1 > 1 ? p - 1 : 1;
+}
+
+// getGrainEmission (update_mod.f90:601-651) of row k at potential Vg, on every lane
+__device__ double grain_emission(const Atomic &A, const Grains &G, const double *pf, int k, double Vg, int lane)
+{
+    const int nb = A.nbins, s = k / G.nSizes;
+    const double vn = G.vn[s];
+    const int ip = locate_nu(A.nu, nb, maxr(vn + Vg, vn)) + 1;
+    const double *qa = G.qa + (size_t)k * nb, *yn = G.yn + (size_t)s * nb;
+    double sum = 0.0;
+    for (int f = ip + lane; f <= nb; f += W) {
+        const double Yhat = yn[f - 1] * minr(1., maxr(0., 1. - Vg / (A.nu[f - 1] - vn)));
+        const double photFlux = pf[f - 1] * kFourPiR;
+        sum = sum + Yhat * photFlux * qa[f - 1];
+    }
+    return wave_sum(sum);
+}
+
+// getGrainRecombination (update_mod.f90:656-733), on every lane
+__device__ double grain_recombination(const Atomic &A, const GrainLds &L, double Te, double eNe, double Vg, int lane)
+{
+    const double kT = 6.336e-6 * Te;
+    double eta = -(Vg / kT);
+    eta = eta <= 0. ? 1. - eta : exp(-eta);
+    const double rec = eNe * eta;
+    double ions = 0.0;
+    for (int i = lane; i < A.nE * kMaxStage; i += W) {
+        const int e = i / kMaxStage, st = i % kMaxStage;
+        if (st < 2 || st > A.el[e].nst) continue;
+        double et = (double)(st - 1) * Vg / kT;
+        et = et <= 0. ? 1. - et : exp(-et);
+        ions = ions + L.w[e][st] * et;
+    }
+    return rec - wave_sum(ions);
+}
+
+// the grain block of iterateT after ionBalance (update_mod.f90:367-388) and
+// the two heat/cool terms it adds to thermBalance (1252-1255)
+__device__ void grain_terms(const Atomic &A, const Grains &G, const double *pf, GrainLds &L, Lds &S, int lane,
+                            int cell)
+{
+    const int nb = A.nbins, K = G.K;
+    const double Te = S.Te, Ne = S.Ne, Hden = S.Hden;
+    const double kT = 6.336e-6 * Te, e8 = 1.1045e-15 * Te / kPiR;
+    // n(ion) v_mean of the ions; electrons ((Ne vmean) S) with S = 1
+    for (int i = lane; i < A.nE * kMaxStage; i += W) {
+        const int e = i / kMaxStage, st = i % kMaxStage;
+        if (st < 2 || st > A.el[e].nst) continue;
+        const double cpDen = S.ion[e][st] * A.el[e].abun * Hden;
+        const double vm = sqrt(e8 / (G.aweight[A.el[e].z - 1] * kAmu));
+        L.w[e][st] = cpDen * vm * 1.;
+    }
+    __syncthreads();
+    const double eNe = Ne * sqrt(e8 / kMe) * 1.;
+    // setGrainPotential: secant on emission - recombination, from 0 V
+    for (int k = 0; k < K; ++k) {
+        const double vn = G.vn[k / G.nSizes];
+        double slope = 0., potOld = 0.;
+        double emiOld = grain_emission(A, G, pf, k, 0., lane);
+        double recOld = grain_recombination(A, L, Te, eNe, 0., lane);
+        double pot = potOld + 0.05;
+        for (int n = 1;; ++n) {
+            const double threshold = maxr(vn + pot, vn);
+            const double emi = grain_emission(A, G, pf, k, pot, lane);
+            const double rec = grain_recombination(A, L, Te, eNe, pot, lane);
+            double delta = emi - rec;
+            double delta1 = fabs(delta / (0.5 * maxr(1.e-35, emi + rec)));
+            if (delta1 < 0.005) break;
+            if (pot != potOld) {
+                const double fac = (emi - emiOld) - (rec - recOld);
+                if (fac != 0.) slope = fac / (pot - potOld);
+            }
+            potOld = pot;
+            recOld = rec;
+            emiOld = emi;
+            delta1 = -(delta / slope);
+            delta = minr(fabs(delta1), 0.05 * threshold);
+            delta = signbit(delta1) ? -fabs(delta) : fabs(delta);
+            pot = pot + delta;
+            if (n >= 100) break;
+        }
+        if (lane == 0) L.pot[k] = pot;
+    }
+    __syncthreads();
+    // setPhotoelHeatCool: lanes over bins, one row after another
+    double pehg = 0.0;
+    for (int k = 0; k < K; ++k) {
+        const int s = k / G.nSizes, ai = k % G.nSizes;
+        const double pot = L.pot[k], vn = G.vn[s];
+        int thP = locate_nu(A.nu, nb, maxr(vn + pot, vn));
+        if (thP <= 0) thP = 1;
+        const double *ax = G.absx + (size_t)k * nb, *yn = G.yn + (size_t)s * nb;
+        double hd = 0.0, hg = 0.0;
+        for (int f = thP + lane; f <= nb; f += W) {
+            const double nu = A.nu[f - 1], Yn = yn[f - 1];
+            const double Yhat = Yn * minr(1., maxr(0., 1. - pot / (nu - vn)));
+            const double d = nu - vn;
+            const double EY = Yn * 0.5 * minr(d, maxr(0., (d * d - pot * pot) / d));
+            const double q = ax[f - 1] * pf[f - 1];
+            hd = hd + q * EY * kHcRyd / kPiR;
+            hg = hg + q * (EY - Yhat * pot) * G.gabun[s] * G.weight[ai];
+        }
+        hd = wave_sum(hd);
+        hg = wave_sum(hg);
+        if (lane == 0) L.peh[k] = hd;
+        pehg = pehg + hg;
+    }
+    pehg = pehg * G.ndust[cell] * kHcRyd / Hden;
+    // setDustGasCollHeatCool: one lane per grain row, elements and stages in
+    // the reference's order; IPerg is set for ionised stages and kept for
+    // the neutral stage that follows (it starts at 0)
+    double gd = 0.0, gg = 0.0;
+    if (lane < K) {
+        const int k = lane, s = k / G.nSizes, ai = k % G.nSizes;
+        const double pot = L.pot[k], rad = G.radius[ai], ms = G.msurf[s];
+        const double Td = G.Tdust[(size_t)cell * (G.nsMax + 1) * (G.nSizes + 1) + (s + 1) + (G.nsMax + 1) * (ai + 1)];
+        const double nd = G.ndust[cell];
+        double IPerg = 0.0;
+        for (int e = 0; e < A.nE; ++e) {
+            const int z = A.el[e].z;
+            const double mcp = G.aweight[z - 1] * kAmu;
+            const double vmean = sqrt(e8 / mcp);
+            for (int st = 1; st <= A.el[e].nst; ++st) {
+                const double Z = (double)(st - 1);
+                if (st > 1) IPerg = A.nu[A.ch[A.el[e].ch0 + st - 2].ip - 1] * kRyd2erg;
+                const double S_ = st == 1 ? 2. * mcp * ms / ((mcp + ms) * (mcp + ms)) : 1.;
+                const double psi = Z * pot / kT;
+                double eta, xi;
+                if (psi <= 0.) { eta = 1. - psi; xi = 1. - psi / 2.; }
+                else { eta = exp(-psi); xi = (1. + psi / 2.) * eta; }
+                const double x = S.ion[e][st], ab = A.el[e].abun;
+                gd = gd + x * ab * Hden * kPiR * rad * rad * 1.e-8 * S_ * vmean *
+                          (2. * kT * kRyd2erg * xi - eta * (Z * pot * kRyd2erg - IPerg + 2. * kKB * Td));
+                gg = gg + x * G.weight[ai] * G.gabun[s] * nd * ab * kPiR * rad * rad * 1.e-8 * S_ * vmean *
+                          (2. * kT * kRyd2erg * xi - eta * 2. * kKB * Td);
+            }
+        }
+        // electrons
+        const double vme = sqrt(e8 / kMe), Z = -1.;
+        const double psi = Z * pot / kT;
+        double eta, xi;
+        if (psi <= 0.) { eta = 1. - psi; xi = 1. - psi / 2.; }
+        else { eta = exp(-psi); xi = (1. + psi / 2.) * eta; }
+        gd = gd + Ne * kPiR * rad * rad * 1.e-8 * 1. * vme * (2. * kT * kRyd2erg * xi - eta * (Z * pot * kRyd2erg));
+        gg = gg + Ne * G.weight[ai] * G.gabun[s] * nd * kPiR * rad * rad * 1.e-8 * 1. * vme *
+                  (2. * kT * kRyd2erg * xi) / Hden;
+        L.gdc[k] = gd / kFourPiR;
+    }
+    gg = wave_sum(gg) / kFourPiR;
+    if (lane == 0) { L.gdcg = gg; L.pehg = pehg; }
+    __syncthreads();
+}
+
+template <bool PE>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) balance_kernel(const Atomic *__restrict__ Ap, CellArgs a)
 {
     __shared__ Lds S;
+    extern __shared__ double dyn[];           // grain variant: photon flux per bin, then GrainLds
     const Atomic &A = *Ap;
     const int lane = threadIdx.x;
     const int cell = blockIdx.x + 1;
@@ -275,6 +467,15 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) ba
     for (int k = lane; k < nch; k += W) {
         S.nPhot[k] = 1.e-20 + a.Rint[(size_t)cell * nch + k];
         S.heatNu[k] = a.Hint[(size_t)cell * nch + k];
+    }
+    double *pf = dyn;
+    GrainLds &GL = *reinterpret_cast<GrainLds *>(dyn + A.nbins);
+    if (PE) {
+        // the cell's photon flux J/(h c Ryd nu) (getGrainEmission, setPhotoelHeatCool)
+        const Grains &G = *a.grains;
+        const int xyz = G.cell_xyz[cell];
+        for (int f = lane; f < A.nbins; f += W)
+            pf[f] = G.J3[mcd::j3_index(A.nbins, f, xyz)] * a.jscale / (kHcRyd * A.nu[f]);
     }
     __syncthreads();
     // lgConverged compares with the caller's H0 fraction (update_mod.f90:510-515)
@@ -364,12 +565,18 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) ba
                 lgIBConv = !(big && nIterX == 25);
                 break;
             }
+            if (PE) grain_terms(A, *a.grains, pf, GL, S, lane, cell);
             // thermBalance (933-1258)
             double cool = cooling(A, S, lane, S.Te), heat = 0.0;
             if (lane == 0) {
                 for (int k = 0; k < nch; ++k) {
                     const Channel &c = A.ch[k];
                     heat = heat + S.heatNu[k] * S.ion[c.eidx][c.ion] * A.el[c.eidx].abun;
+                }
+                if (PE) {
+                    // the grain terms of thermBalance (1252-1255)
+                    cool = cool + GL.gdcg;
+                    heat = heat + GL.pehg;
                 }
                 S.red[0] = heat;
                 S.red[1] = cool;
@@ -458,6 +665,12 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) ba
     const double dXHI = (S.ion[0][1] - XOldHI) / XOldHI;
     if (fabs(dXHI) <= (double)A.XHILimit) flags |= 1;
     for (int i = lane; i < nE * (ns + 1); i += W) ionOut[i] = S.ion[i / (ns + 1)][i % (ns + 1)];
+    if (PE && lane < a.grains->K) {
+        // terms of the last temperature iteration, for getDustT
+        const Grains &G = *a.grains;
+        G.gdc[(size_t)cell * G.K + lane] = GL.gdc[lane];
+        G.peh[(size_t)cell * G.K + lane] = GL.peh[lane];
+    }
     if (lane == 0) {
         a.Te[cell] = S.Te;
         a.Ne[cell] = S.Ne;
@@ -609,7 +822,13 @@ __global__ void warm_kernel(const Atomic *__restrict__ Ap, CellArgs a, const flo
 hipError_t mcb_launch_balance(const mcb::Atomic *dA, const mcb::CellArgs &a, hipStream_t s)
 {
     if (a.ncells <= 0) return hipSuccess;
-    hipLaunchKernelGGL(mcb::balance_kernel, dim3(a.ncells), dim3(64), 0, s, dA, a);
+    if (a.grains) {
+        // the caller passes the frequency-bin count through grains_nb
+        const size_t lds = (size_t)a.grains_nb * sizeof(double) + sizeof(mcb::GrainLds);
+        hipLaunchKernelGGL(mcb::balance_kernel<true>, dim3(a.ncells), dim3(64), lds, s, dA, a);
+    } else {
+        hipLaunchKernelGGL(mcb::balance_kernel<false>, dim3(a.ncells), dim3(64), 0, s, dA, a);
+    }
     return hipGetLastError();
 }
 

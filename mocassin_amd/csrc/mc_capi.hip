@@ -66,6 +66,11 @@ struct mc_ctx {
     mcg::DustArgs dust_args{};
     double *d_dust_tab = nullptr, *d_Tdust = nullptr;
     int *d_dust_mono = nullptr;
+    bool dust_pe = false;                    // grain charge and photoelectric terms (with gas)
+    double *d_pe_tab = nullptr;              // Qa, Yn, Ndust, gasDustColl_d, photoelHeat_d
+    double *pe_gdc = nullptr, *pe_peh = nullptr;
+    mcb::Grains *d_grains = nullptr;
+    std::vector<float> nuArray;
     char err[512] = "";
 };
 
@@ -162,6 +167,7 @@ static int ctx_init(mc_ctx *c, const mc_desc *d)
     c->cell_xyz.assign((size_t)nc + 1, -1);
     for (size_t i = 0; i < (size_t)nx * ny * nz; ++i)
         if (d->active[i] > 0) c->cell_xyz[d->active[i]] = (int)i;
+    c->nuArray.assign(d->nuArray, d->nuArray + d->nbins);
     c->nStars = d->nStars;
     c->starPos.assign(d->starPos, d->starPos + 3 * d->nStars);
     c->starIdx.assign(d->starIdx, d->starIdx + 3 * d->nStars);
@@ -368,12 +374,28 @@ extern "C" int mc_set_estimators(mc_ctx *c, const double *J, const double *esc)
     return MC_OK;
 }
 
+// the lattice point of every cell on the device (per-cell kernels that read
+// the cell's spectrum from J3)
+static int cell_positions(mc_ctx *c)
+{
+    for (int cell = 1; cell <= c->G.nCells; ++cell)
+        if (c->cell_xyz[cell] < 0) return fail(c, MC_EINVAL, "active map has no position for cell %d", cell);
+    if (!c->d_cell_xyz) {
+        const size_t nc1 = (size_t)c->G.nCells + 1;
+        HIPCHK(c, dalloc(&c->d_cell_xyz, nc1));
+        HIPCHK(c, hipMemcpy(c->d_cell_xyz, c->cell_xyz.data(), nc1 * sizeof(int), hipMemcpyHostToDevice));
+    }
+    return MC_OK;
+}
+
 extern "C" int mc_balance_setup(mc_ctx *c, const mc_atomic *at)
 {
     if (!c) return MC_EINVAL;
     HIPCHK(c, hipSetDevice(c->device));
-    for (int cell = 1; cell <= c->G.nCells; ++cell)
-        if (c->cell_xyz[cell] < 0) return fail(c, MC_EINVAL, "active map has no position for cell %d", cell);
+    {
+        int rc = cell_positions(c);
+        if (rc) return rc;
+    }
     c->balance = false;
     std::vector<float> nu(c->G.nbins);
     HIPCHK(c, hipMemcpy(nu.data(), c->d_nuArray, nu.size() * sizeof(float), hipMemcpyDeviceToHost));
@@ -405,12 +427,12 @@ extern "C" int mc_balance_setup(mc_ctx *c, const mc_atomic *at)
     HIPCHK(c, hipMemcpy(c->d_atomic, &A, sizeof A, hipMemcpyHostToDevice));
     const size_t nc1 = (size_t)c->G.nCells + 1;
     const size_t nion = nc1 * P.A.nE * (P.A.nstages + 1);
-    if (!c->d_cell_xyz) {
-        HIPCHK(c, dalloc(&c->d_cell_xyz, nc1));
-        HIPCHK(c, hipMemcpy(c->d_cell_xyz, c->cell_xyz.data(), nc1 * sizeof(int), hipMemcpyHostToDevice));
+    if (!c->d_Hden) {
         HIPCHK(c, dalloc(&c->d_Hden, nc1));
         HIPCHK(c, dalloc(&c->d_Te, nc1));
         HIPCHK(c, dalloc(&c->d_Ne, nc1));
+    }
+    if (!c->d_flags) {
         HIPCHK(c, dalloc(&c->d_flags, nc1));
         HIPCHK(c, dalloc(&c->d_nIter, nc1));
     }
@@ -475,9 +497,17 @@ extern "C" int mc_dust_setup(mc_ctx *c, const mc_dust_tables *d)
         return fail(c, MC_EUNSUP, "nSpecies * nSizes must be 1..%d", mcg::kMaxRows);
     if (d->nSpeciesMax < d->nSpecies) return fail(c, MC_EINVAL, "nSpeciesMax < nSpecies");
     if (d->nTemps < 2 || d->nTemps > mcg::kMaxTemps) return fail(c, MC_EUNSUP, "nTemps must be 2..%d", mcg::kMaxTemps);
-    if (d->photoelectric) return fail(c, MC_EUNSUP, "the grain charge and photoelectric balance is not supported");
     if (!d->absXsec || !d->grainWeight || !d->grainAbun || !d->emIntegral)
         return fail(c, MC_EINVAL, "null dust array");
+    const bool pe = d->photoelectric != 0;
+    if (pe) {
+        if (!d->grainRadius || !d->grainVn || !d->MsurfAtom || !d->aWeight || !d->Ndust)
+            return fail(c, MC_EINVAL, "null photoelectric array");
+        if (c->G.nbins > 4096) return fail(c, MC_EUNSUP, "the photoelectric balance needs nbins <= 4096");
+        for (int i = 1; i < c->G.nbins; ++i)
+            if (!(c->nuArray[i] > c->nuArray[i - 1]))
+                return fail(c, MC_ETABLE, "nuArray not increasing at bin %d", i + 1);
+    }
     if (!(d->XHILimit > 0.0)) return fail(c, MC_EINVAL, "XHILimit must be > 0");
     const int nS = d->nSpecies, nSz = d->nSizes, nT = d->nTemps, nb = c->G.nbins, K = nS * nSz;
     // one FP64 allocation: absx rows | emission rows | weight | abun
@@ -500,14 +530,78 @@ extern "C" int mc_dust_setup(mc_ctx *c, const mc_dust_tables *d)
         }
     for (int ai = 0; ai < nSz; ++ai) w[ai] = d->grainWeight[ai];
     for (int s = 0; s < nS; ++s) ab[s] = d->grainAbun[s];
+    // photoelectric tables: Qa = absx / (1e-8 a^2) per row and bin, Yn per
+    // species and bin, as getGrainEmission and setPhotoelHeatCool form them
+    std::vector<double> pet;
+    mcb::Grains GH{};
+    const size_t nc1 = (size_t)c->G.nCells + 1;
+    if (pe) {
+        pet.assign((size_t)K * nb + (size_t)nS * nb + nc1, 0.0);
+        double *qa = pet.data(), *yn = qa + (size_t)K * nb, *nd = yn + (size_t)nS * nb;
+        for (int s = 0; s < nS; ++s) {
+            const double vn = d->grainVn[s];
+            if (!std::isfinite(vn) || !std::isfinite(d->MsurfAtom[s]))
+                return fail(c, MC_ETABLE, "grainVn/MsurfAtom of species %d not finite", s + 1);
+            for (int i = 0; i < nb; ++i) {
+                const double v = 0.5 * (1. - vn / (double)c->nuArray[i]);
+                yn[(size_t)s * nb + i] = v < 0.2 ? v : 0.2;
+            }
+            GH.vn[s] = vn;
+            GH.msurf[s] = d->MsurfAtom[s];
+            GH.gabun[s] = d->grainAbun[s];
+        }
+        for (int ai = 0; ai < nSz; ++ai) {
+            const double r = d->grainRadius[ai];
+            if (!(r > 0.0) || !std::isfinite(r)) return fail(c, MC_ETABLE, "grainRadius(%d) must be > 0", ai + 1);
+            GH.radius[ai] = r;
+            GH.weight[ai] = d->grainWeight[ai];
+            for (int s = 0; s < nS; ++s)
+                for (int i = 0; i < nb; ++i) {
+                    const size_t q = ((size_t)s * nSz + ai) * nb + i;
+                    qa[q] = absx[q] / (1.e-8 * (r * r));
+                }
+        }
+        for (int z = 0; z < 30; ++z) {
+            if (!(d->aWeight[z] > 0.0f)) return fail(c, MC_ETABLE, "aWeight(%d) must be > 0", z + 1);
+            GH.aweight[z] = d->aWeight[z];
+        }
+        for (size_t i = 0; i < nc1; ++i) {
+            if (!std::isfinite(d->Ndust[i]) || d->Ndust[i] < 0.0f) return fail(c, MC_ETABLE, "Ndust(%zu) invalid", i);
+            nd[i] = d->Ndust[i];
+        }
+        GH.nS = nS; GH.nSizes = nSz; GH.nsMax = d->nSpeciesMax; GH.K = K;
+    }
     HIPCHK(c, hipSetDevice(c->device));
     if (c->d_dust_tab) { (void)hipFree(c->d_dust_tab); c->d_dust_tab = nullptr; }
     if (c->d_dust_mono) { (void)hipFree(c->d_dust_mono); c->d_dust_mono = nullptr; }
     if (c->d_Tdust) { (void)hipFree(c->d_Tdust); c->d_Tdust = nullptr; }
+    if (c->d_pe_tab) { (void)hipFree(c->d_pe_tab); c->d_pe_tab = nullptr; }
+    if (c->d_grains) { (void)hipFree(c->d_grains); c->d_grains = nullptr; }
+    c->dust = false;
     HIPCHK(c, dalloc(&c->d_dust_tab, tab.size()));
     HIPCHK(c, dalloc(&c->d_dust_mono, (size_t)K));
-    const size_t nc1 = (size_t)c->G.nCells + 1;
     HIPCHK(c, dalloc(&c->d_Tdust, nc1 * (d->nSpeciesMax + 1) * (nSz + 1)));
+    if (pe) {
+        // qa | yn | Ndust | gdc | peh
+        HIPCHK(c, dalloc(&c->d_pe_tab, pet.size() + 2 * nc1 * K));
+        HIPCHK(c, hipMemcpy(c->d_pe_tab, pet.data(), pet.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemset(c->d_pe_tab + pet.size(), 0, 2 * nc1 * K * sizeof(double)));
+        int rc = cell_positions(c);
+        if (rc) return rc;
+        GH.absx = c->d_dust_tab;
+        GH.qa = c->d_pe_tab;
+        GH.yn = GH.qa + (size_t)K * nb;
+        GH.ndust = GH.yn + (size_t)nS * nb;
+        GH.gdc = c->d_pe_tab + pet.size();
+        GH.peh = GH.gdc + nc1 * K;
+        GH.Tdust = c->d_Tdust;
+        GH.J3 = c->d_J3;
+        GH.cell_xyz = c->d_cell_xyz;
+        HIPCHK(c, dalloc(&c->d_grains, 1));
+        HIPCHK(c, hipMemcpy(c->d_grains, &GH, sizeof GH, hipMemcpyHostToDevice));
+        c->pe_gdc = GH.gdc;
+        c->pe_peh = GH.peh;
+    }
     HIPCHK(c, hipMemcpy(c->d_dust_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_dust_mono, mono.data(), K * sizeof(int), hipMemcpyHostToDevice));
     if (!c->d_flags) {
@@ -525,6 +619,7 @@ extern "C" int mc_dust_setup(mc_ctx *c, const mc_dust_tables *d)
     a.Tdust = c->d_Tdust;
     a.XHILimit = d->XHILimit;
     c->dust = true;
+    c->dust_pe = pe;
     return MC_OK;
 }
 
@@ -618,12 +713,20 @@ static int update_cells(mc_ctx *c, double jscale, const mc_balance_opts *opts, c
             HIPCHK(c, mcm_unscale(c->d_mY, nc, mcm::kOutP, mlp_scale(c), c->stream));
             HIPCHK(c, mcb_launch_warm(c->d_atomic, a, c->d_mY, mcm::kOutP, c->d_hit, c->stream));
         }
+        if (dust && c->dust_pe) {
+            a.grains = c->d_grains;
+            a.grains_nb = c->G.nbins;
+        }
         HIPCHK(c, mcb_launch_balance(c->d_atomic, a, c->stream));
     }
     if (dust) {
         mcg::DustArgs da = da0;
         da.jscale = jscale;
         da.flags = gas ? nullptr : c->d_flags;      // grains alone: Tdust(0,0) decides convergence
+        if (gas && c->dust_pe) {                    // the gas balance left gasDustColl_d, photoelHeat_d
+            da.gdc = c->pe_gdc;
+            da.peh = c->pe_peh;
+        }
         HIPCHK(c, mcg_launch_dust(da, c->d_J3, c->d_active, c->stream));
     }
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
@@ -657,6 +760,9 @@ extern "C" int mc_balance_run(mc_ctx *c, double jscale, const mc_balance_opts *o
 {
     if (!c) return MC_EINVAL;
     if (!c->balance) return fail(c, MC_ESTATE, "mc_balance_setup has not been called");
+    if (c->dust && c->dust_pe)
+        return fail(c, MC_ESTATE, "grains with the photoelectric effect are set up: the gas balance needs "
+                                  "Tdust, call mc_update_cells");
     const bool dust = c->dust;
     c->dust = false;                                 // the gas alone
     const int rc = update_cells(c, jscale, opts, Hden, Te, Ne, ionDen, nullptr, flags, nIter, kernel_ms);
@@ -1165,7 +1271,7 @@ extern "C" void mc_ctx_destroy(mc_ctx *c)
                      c->d_cell_xyz, c->d_atomic, c->d_bal_tab, c->d_Hden, c->d_Te, c->d_Ne,
                      c->d_ion, c->d_flags, c->d_nIter, c->d_cold, c->d_mlp, c->d_mlpW, c->d_mX,
                      c->d_mY, c->d_mX0, c->d_hit, c->d_cross, c->d_Rint, c->d_Hint,
-                     c->d_dust_tab, c->d_Tdust, c->d_dust_mono };
+                     c->d_dust_tab, c->d_Tdust, c->d_dust_mono, c->d_pe_tab, c->d_grains };
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

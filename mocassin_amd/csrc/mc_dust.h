@@ -17,8 +17,9 @@ struct DustArgs {
     const double *em;                 // [nS*nSizes][nTemps]: emission integral at 1..nTemps K
     const int *mono;                  // [nS*nSizes]: 1 where the row never decreases
     const double *weight, *abun;      // grainWeight(1:nSizes), grainAbun(1:nS)
-    const double *add;                // nullable [nc+1][nS*nSizes]: added to the absorption
-                                      // integral (gas-grain collisions minus photoelectric loss)
+    const double *gdc, *peh;          // nullable [nc+1][nS*nSizes]: the absorption integral
+                                      // becomes (integral + gdc) - peh (gas-grain collisions,
+                                      // photoelectric loss; update_mod.f90:1886-1889)
     double *Tdust;                    // in/out; reached cells rewritten
     int *flags;                       // nullable (0:nc): bit0 converged, bit4 not reached
     double XHILimit;                  // convergence of Tdust(0,0) (grains without gas)

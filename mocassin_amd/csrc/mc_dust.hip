@@ -94,7 +94,7 @@ __global__ void __launch_bounds__(64) dust_kernel(DustArgs a, const double *__re
     for (int k = 0; k < K; ++k) {
         const int s = k / a.nSizes + 1, ai = k % a.nSizes + 1;
         double x = sx[k][lane];
-        if (a.add) x = x + a.add[(size_t)cell * K + k];
+        if (a.gdc) x = x + a.gdc[(size_t)cell * K + k] - a.peh[(size_t)cell * K + k];
         const double *xa = a.em + (size_t)k * a.nTemps;
         const int n = a.nTemps;
         const int iT = locate_row(xa, n, a.mono[k], x);

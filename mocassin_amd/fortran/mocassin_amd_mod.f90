@@ -74,6 +74,7 @@ module mocassin_amd_mod
     type, bind(C) :: mc_dust_tables
         integer(c_int32_t) :: nSpecies, nSpeciesMax, nSizes, nTemps, photoelectric
         type(c_ptr) :: absXsec, grainWeight, grainAbun, emIntegral
+        type(c_ptr) :: grainRadius, grainVn, MsurfAtom, aWeight, Ndust
         real(c_float) :: XHILimit
     end type mc_dust_tables
 

@@ -185,12 +185,22 @@ class Transport:
 
     # ---- grains
     def dust_setup(self, absXsec, grainWeight, grainAbun, emIntegral, nTemps, nSpeciesMax=None,
-                   XHILimit=0.01, photoelectric=False):
+                   XHILimit=0.01, photoelectric=False, grainRadius=None, grainVn=None, MsurfAtom=None,
+                   aWeight=None, Ndust=None):
         """Upload the tables of getDustT for one dust chemistry.  absXsec
-        (nSpecies, nSizes, nbins); emIntegral (nSpecies, nSizes, nTemps)."""
+        (nSpecies, nSizes, nbins); emIntegral (nSpecies, nSizes, nTemps).
+        photoelectric: the grain charge and photoelectric terms, which need
+        grainRadius (nSizes), grainVn, MsurfAtom (nSpecies), aWeight (30)
+        and Ndust (nCells+1)."""
         keep = [np.ascontiguousarray(absXsec, np.float32), np.ascontiguousarray(grainWeight, np.float32),
                 np.ascontiguousarray(grainAbun, np.float32),
                 np.asfortranarray(np.asarray(emIntegral, np.float32))]
+        pe = [None] * 5
+        if photoelectric:
+            pe = [np.ascontiguousarray(v, np.float32) for v in (grainRadius, grainVn, MsurfAtom, aWeight, Ndust)]
+            if pe[3].size != 30 or pe[4].size != self.nCells + 1:
+                raise ValueError("aWeight needs 30 entries and Ndust nCells+1")
+        self._dust_keep = keep + pe
         nS, nSz, nb = keep[0].shape
         if nb != self.nbins or keep[3].shape != (nS, nSz, int(nTemps)):
             raise ValueError("dust tables do not match nbins / (nSpecies, nSizes, nTemps)")
@@ -199,6 +209,8 @@ class Transport:
         D.nSpeciesMax = nS if nSpeciesMax is None else int(nSpeciesMax)
         D.photoelectric = 1 if photoelectric else 0
         D.absXsec, D.grainWeight, D.grainAbun, D.emIntegral = (k.ctypes.data for k in keep)
+        D.grainRadius, D.grainVn, D.MsurfAtom, D.aWeight, D.Ndust = (None if v is None else v.ctypes.data
+                                                                     for v in pe)
         D.XHILimit = float(XHILimit)
         self._check(self._lib.mc_dust_setup(self._h, C.byref(D)))
         self._dust_shape = (D.nSpeciesMax + 1, nSz + 1)

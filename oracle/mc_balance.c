@@ -19,11 +19,13 @@ int mco_atomic_init(mco_atomic *A)
     for (int z = 1; z <= 30; ++z) A->xref[z] = A->xref_in[z - 1];
     for (int z = 1; z <= 30; ++z) A->abun[z] = A->abun_in[z - 1];
     memset(A->chan_g, 0, sizeof A->chan_g);
+    memset(A->chan_ip, 0, sizeof A->chan_ip);
     for (int k = 0; k < A->nch; ++k) {
         const int32_t *m = A->ch_meta + 8 * k;
         if (m[0] < 1 || m[0] > 30 || m[1] < 1 || m[1] > 9) return 1;
         A->chan_g[2 * (m[0] * 10 + m[1])] = m[2];
         A->chan_g[2 * (m[0] * 10 + m[1]) + 1] = m[3];
+        A->chan_ip[m[0] * 10 + m[1]] = m[4];
     }
     if (A->ncel > MCO_MAX_CEL) return 2;
     size_t od = 0, of = 0;
@@ -49,6 +51,7 @@ int mco_atomic_init(mco_atomic *A)
 #define POWR powf
 #define FABSR fabsf
 #define QCAST float   /* qq(iT) = real(qom(...)), emission_mod.f90:2008 */
+#define GC_SAVED_SLOPE 1   /* setGrainPotential's SAVEd slope carries over between calls */
 static R pow10m12_f32(void) { return 1.e-12f; }
 #include "mc_balance_impl.h"
 #undef R
@@ -61,6 +64,7 @@ static R pow10m12_f32(void) { return 1.e-12f; }
 #undef FABSR
 #undef QCAST
 #undef NFOR
+#undef GC_SAVED_SLOPE
 
 /* ---- FP64 oracle ---- */
 #define R double
@@ -72,5 +76,6 @@ static R pow10m12_f32(void) { return 1.e-12f; }
 #define POWR pow
 #define FABSR fabs
 #define QCAST double
+#define GC_SAVED_SLOPE 0   /* each grain-potential solve starts from slope 0 */
 static R pow10m12_f64(void) { return 1.e-12; }
 #include "mc_balance_impl.h"

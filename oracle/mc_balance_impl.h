@@ -480,12 +480,249 @@ static void FN(therm_balance)(const mco_atomic *A, FN(cell_t) *c, R *heatInt, R 
     *coolInt = coolFF + coolRec + coolColl;
 }
 
+/* ---- grain charge, photoelectric heating and gas-grain collisions ------
+ * (update_mod.f90:527-932), one dust chemistry.  Constants as the reference
+ * declares them: Pi a REAL (constants_mod.f90:32), Y0 = 0.5, Y1 = 0.2,
+ * errorLim 0.005, dm 0.05, maxIterateGC 100 (update_mod.f90:20,81,533-535). */
+typedef struct {
+    const mco_grains *G;
+    const R *J;              /* the cell's scaled Jste */
+    R ndust;                 /* Ndust(cell) */
+    const R *Tdust;          /* Tdust(0:nsMax, 0:nSizes, cell) before the update */
+    int ldT;                 /* nsMax + 1 */
+    R pot[MCO_MAX_GROWS];    /* grainPot(isp, ai) at (isp-1)*nSizes + ai-1 */
+    R gdc[MCO_MAX_GROWS], peh[MCO_MAX_GROWS];   /* gasDustColl_d, photoelHeat_d */
+    R gdcg, pehg;            /* gasDustColl_g, photoelHeat_g */
+} FN(grain_t);
+
+#define PI_R ((R)3.141592654f)
+#define FOURPI_R ((R)(4.f * 3.141592654f))
+#define MINR(a, b) ((a) < (b) ? (a) : (b))
+#define MAXR(a, b) ((a) > (b) ? (a) : (b))
+
+/* locate(nuArray, x), interpolation_mod.f90:48-81, for the increasing
+ * frequency mesh: n when x > nu(n), 0 when x < nu(1), else the last i with
+ * nu(i) <= x (at least 1); x == nu(n) gives 1 */
+static int FN(locate_nu)(const float *nu, int n, R x)
+{
+    if (x > (R)nu[n - 1]) return n;
+    if (x < (R)nu[0]) return 0;
+    int p = 0;
+    R bd = L(0.);
+    for (int i = 0; i < n; ++i) {
+        const R d = (R)nu[i] - x;
+        if (d > L(0.) && (p == 0 || d < bd)) { p = i + 1; bd = d; }
+    }
+    return p - 1 > 1 ? p - 1 : 1;
+}
+
+/* getGrainEmission, update_mod.f90:601-651 */
+static R FN(grain_emission)(const mco_atomic *A, const FN(grain_t) *g, R Vg, int isp, int ai)
+{
+    const mco_grains *G = g->G;
+    const int nb = A->nbins;
+    const R hcRyd = L(2.1799153e-11);
+    const R vn = (R)G->vn[isp - 1];
+    const R thres = MAXR(vn + Vg, vn);
+    const int ip = FN(locate_nu)(A->nuArray, nb, thres) + 1;
+    const R rad = (R)G->radius[ai - 1];
+    const R qdiv = L(1.e-8) * (rad * rad);
+    const float *ax = G->absx + ((size_t)(isp - 1) * G->nSizes + (ai - 1)) * nb;
+    R sum = L(0.);
+    for (int f = ip; f <= nb; ++f) {
+        const R nu = (R)A->nuArray[f - 1];
+        const R Yn = MINR(L(0.5) * (L(1.) - vn / nu), L(0.2));
+        const R Yhat = Yn * MINR(L(1.), MAXR(L(0.), L(1.) - Vg / (nu - vn)));
+        R photFlux = g->J[f - 1] / (hcRyd * nu);
+        photFlux = photFlux * FOURPI_R;
+        const R Qa = (R)ax[f - 1] / qdiv;
+        sum = sum + Yhat * photFlux * Qa;
+    }
+    return sum;
+}
+
+/* getGrainRecombination, update_mod.f90:656-733 */
+static R FN(grain_recombination)(const mco_atomic *A, const FN(cell_t) *c, const FN(grain_t) *g, R Vg)
+{
+    const R kT = L(6.336e-6) * c->Te;
+    const R e8 = L(1.1045e-15) * c->Te / PI_R;
+    R vmean = SQRTR(e8 / L(9.109382e-28));
+    const R S = L(1.);
+    R eta = -(Vg / kT);
+    if (eta <= L(0.)) eta = L(1.) - eta;
+    else eta = EXPR(-eta);
+    R rec = c->Ne * vmean * S * eta;
+    for (int elem = 1; elem <= 30; ++elem) {
+        if (!A->elemOn[elem]) continue;
+        const int top = elem + 1 < A->nstages ? elem + 1 : A->nstages;
+        for (int ist = 2; ist <= top; ++ist) {
+            const R cpDen = c->ion[elem][ist] * (R)A->abun[elem] * c->Hden;
+            const R mcp = (R)g->G->aweight[elem - 1] * L(1.66053e-24);
+            vmean = SQRTR(e8 / mcp);
+            const R Z = (R)(ist - 1);
+            eta = Z * Vg / kT;
+            if (eta <= L(0.)) eta = L(1.) - eta;
+            else eta = EXPR(-eta);
+            rec = rec - cpDen * vmean * S * eta;
+        }
+    }
+    return rec;
+}
+
+#if GC_SAVED_SLOPE
+static R FN(gc_slope) = L(0.);
+#endif
+
+/* setGrainPotential, update_mod.f90:527-597 (its recursion as a loop) */
+static R FN(grain_potential)(const mco_atomic *A, const FN(cell_t) *c, const FN(grain_t) *g, int isp, int ai)
+{
+#if GC_SAVED_SLOPE
+    R slope = FN(gc_slope);
+#else
+    R slope = L(0.);
+#endif
+    const R vn = (R)g->G->vn[isp - 1];
+    R potOld = L(0.);
+    R emiOld = FN(grain_emission)(A, g, L(0.), isp, ai);
+    R recOld = FN(grain_recombination)(A, c, g, L(0.));
+    R pot = potOld + L(0.05);
+    for (int n = 1;; ++n) {
+        const R threshold = MAXR(vn + pot, vn);
+        const R emi = FN(grain_emission)(A, g, pot, isp, ai);
+        const R rec = FN(grain_recombination)(A, c, g, pot);
+        R delta = emi - rec;
+        R delta1 = FABSR(delta / (L(0.5) * MAXR(L(1.e-35), emi + rec)));
+        if (delta1 < L(0.005)) break;
+        if (pot != potOld) {
+            const R fac = (emi - emiOld) - (rec - recOld);
+            if (fac != L(0.)) slope = fac / (pot - potOld);
+        }
+        potOld = pot;
+        recOld = rec;
+        emiOld = emi;
+        delta1 = -(delta / slope);
+        delta = FABSR(delta1);
+        delta = MINR(delta, L(0.05) * threshold);
+        delta = signbit(delta1) ? -FABSR(delta) : FABSR(delta);
+        pot = pot + delta;
+        if (n >= 100) break;
+    }
+#if GC_SAVED_SLOPE
+    FN(gc_slope) = slope;
+#endif
+    return pot;
+}
+
+/* setPhotoelHeatCool, update_mod.f90:736-804 */
+static void FN(photoel_heat_cool)(const mco_atomic *A, const FN(cell_t) *c, FN(grain_t) *g)
+{
+    const mco_grains *G = g->G;
+    const int nb = A->nbins;
+    const R hcRyd = L(2.1799153e-11);
+    g->pehg = L(0.);
+    for (int k = 0; k < G->nS * G->nSizes; ++k) g->peh[k] = L(0.);
+    for (int ns = 1; ns <= G->nS; ++ns)
+        for (int na = 1; na <= G->nSizes; ++na) {
+            const int k = (ns - 1) * G->nSizes + na - 1;
+            const R pot = g->pot[k], vn = (R)G->vn[ns - 1];
+            const R th = MAXR(vn + pot, vn);
+            int thP = FN(locate_nu)(A->nuArray, nb, th);
+            if (thP <= 0) thP = 1;
+            const float *ax = G->absx + (size_t)k * nb;
+            for (int f = thP; f <= nb; ++f) {
+                const R nu = (R)A->nuArray[f - 1];
+                const R Yn = MINR(L(0.5) * (L(1.) - vn / nu), L(0.2));
+                const R Yhat = Yn * MINR(L(1.), MAXR(L(0.), L(1.) - pot / (nu - vn)));
+                const R photFlux = g->J[f - 1] / (hcRyd * nu);
+                const R Qa = (R)ax[f - 1];
+                const R d = nu - vn;
+                const R EY = Yn * L(0.5) * MINR(d, MAXR(L(0.), (d * d - pot * pot) / d));
+                g->peh[k] = g->peh[k] + Qa * photFlux * EY * hcRyd / PI_R;
+                g->pehg = g->pehg + Qa * photFlux * (EY - Yhat * pot) * (R)G->gabun[ns - 1] * (R)G->weight[na - 1];
+            }
+        }
+    g->pehg = g->pehg * g->ndust * hcRyd / c->Hden;
+}
+
+/* setDustGasCollHeatCool, update_mod.f90:806-930.  IPerg is assigned for
+ * ionised stages only and keeps its last value for the neutral stage that
+ * follows (the reference's own order); it starts at 0. */
+static void FN(dust_gas_coll)(const mco_atomic *A, const FN(cell_t) *c, FN(grain_t) *g)
+{
+    const mco_grains *G = g->G;
+    const R Ryd2erg = L(2.1799e-11), kB = L(1.380658e-16);
+    const R kT = L(6.336e-6) * c->Te;
+    const R e8 = L(1.1045e-15) * c->Te / PI_R;
+    R IPerg = L(0.);
+    g->gdcg = L(0.);
+    for (int k = 0; k < G->nS * G->nSizes; ++k) g->gdc[k] = L(0.);
+    for (int elem = 1; elem <= 30; ++elem) {
+        if (!A->elemOn[elem]) continue;
+        const int top = A->nstages < elem + 1 ? A->nstages : elem + 1;
+        for (int ist = 1; ist <= top; ++ist) {
+            const R Z = (R)(ist - 1);
+            const R mcp = (R)G->aweight[elem - 1] * L(1.66053e-24);
+            const R vmean = SQRTR(e8 / mcp);
+            if (ist > 1) IPerg = (R)A->nuArray[A->chan_ip[elem * 10 + ist - 1] - 1] * Ryd2erg;
+            for (int ns = 1; ns <= G->nS; ++ns) {
+                const R ms = (R)G->msurf[ns - 1];
+                const R S = (ist == 1) ? L(2.) * mcp * ms / ((mcp + ms) * (mcp + ms)) : L(1.);
+                for (int na = 1; na <= G->nSizes; ++na) {
+                    const int k = (ns - 1) * G->nSizes + na - 1;
+                    const R pot = g->pot[k], rad = (R)G->radius[na - 1];
+                    const R Td = g->Tdust[ns + g->ldT * na];
+                    const R psi = Z * pot / kT;
+                    R eta, xi;
+                    if (psi <= L(0.)) { eta = L(1.) - psi; xi = L(1.) - psi / L(2.); }
+                    else { eta = EXPR(-psi); xi = (L(1.) + psi / L(2.)) * eta; }
+                    const R x = c->ion[elem][ist];
+                    g->gdc[k] = g->gdc[k] + x * (R)A->abun[elem] * c->Hden * PI_R * rad * rad * L(1.e-8) * S *
+                                vmean * (L(2.) * kT * Ryd2erg * xi - eta * (Z * pot * Ryd2erg - IPerg + L(2.) * kB * Td));
+                    g->gdcg = g->gdcg + x * (R)G->weight[na - 1] * (R)G->gabun[ns - 1] * g->ndust * (R)A->abun[elem] *
+                              PI_R * rad * rad * L(1.e-8) * S * vmean *
+                              (L(2.) * kT * Ryd2erg * xi - eta * L(2.) * kB * Td);
+                }
+            }
+        }
+    }
+    const R vme = SQRTR(e8 / L(9.109382e-28));
+    for (int ns = 1; ns <= G->nS; ++ns) {
+        const R S = L(1.), Z = L(-1.);
+        for (int na = 1; na <= G->nSizes; ++na) {
+            const int k = (ns - 1) * G->nSizes + na - 1;
+            const R pot = g->pot[k], rad = (R)G->radius[na - 1];
+            const R psi = Z * pot / kT;
+            R eta, xi;
+            if (psi <= L(0.)) { eta = L(1.) - psi; xi = L(1.) - psi / L(2.); }
+            else { eta = EXPR(-psi); xi = (L(1.) + psi / L(2.)) * eta; }
+            g->gdc[k] = g->gdc[k] + c->Ne * PI_R * rad * rad * L(1.e-8) * S * vme *
+                        (L(2.) * kT * Ryd2erg * xi - eta * (Z * pot * Ryd2erg));
+            g->gdcg = g->gdcg + c->Ne * (R)G->weight[na - 1] * (R)G->gabun[ns - 1] * g->ndust * PI_R * rad * rad *
+                      L(1.e-8) * S * vme * (L(2.) * kT * Ryd2erg * xi) / c->Hden;
+        }
+    }
+    for (int k = 0; k < G->nS * G->nSizes; ++k) g->gdc[k] = g->gdc[k] / FOURPI_R;
+    g->gdcg = g->gdcg / FOURPI_R;
+}
+
+/* the grain block of iterateT after ionBalance (update_mod.f90:367-388) */
+static void FN(grain_terms)(const mco_atomic *A, const FN(cell_t) *c, FN(grain_t) *g)
+{
+    const mco_grains *G = g->G;
+    for (int isp = 1; isp <= G->nS; ++isp)
+        for (int ai = 1; ai <= G->nSizes; ++ai)
+            g->pot[(isp - 1) * G->nSizes + ai - 1] = FN(grain_potential)(A, c, g, isp, ai);
+    FN(photoel_heat_cool)(A, c, g);
+    FN(dust_gas_coll)(A, c, g);
+}
+
 /* updateCell + iterateT for one active cell (update_mod.f90:15-525).
  * J: the cell's scaled Jste(cell,1:nbins).  ionDen: [stage-1][xref-1]
  * slice of ionDen(cell,:,:) (in/out).  Returns flags: bit0 converged,
  * bit1 black, bit2 Te not balanced, bit3 ion balance not converged,
  * bit4 not hit; *nT = T-iterations used. */
-int FN(mco_balance_cell)(const mco_atomic *A, const R *J, R Hden, R *Te, R *Ne, R *ionDen, int *nT)
+static int FN(balance_cell)(const mco_atomic *A, const R *J, R Hden, R *Te, R *Ne, R *ionDen, int *nT,
+                            FN(grain_t) *g)
 {
     const int ns = A->nstages, nE = A->nElemUsed, nb = A->nbins;
     int hit = 0;
@@ -531,8 +768,14 @@ int FN(mco_balance_cell)(const mco_atomic *A, const R *J, R Hden, R *Te, R *Ne, 
         FN(calc_alpha)(A, ns, c.Te, c.alphaTot);
         c.nIterateX = 1;
         lgIBConv = FN(ion_balance)(A, &c, chex0, dEk);
+        if (g) FN(grain_terms)(A, &c, g);
         R heatInt, coolInt;
         FN(therm_balance)(A, &c, &heatInt, &coolInt, fl);
+        if (g) {
+            /* thermBalance's dust terms (update_mod.f90:1252-1255) */
+            coolInt = coolInt + g->gdcg;
+            heatInt = heatInt + g->pehg;
+        }
         R thResidual = heatInt - coolInt;
         if (FABSR(thResidual) >= L(0.01) * heatInt) {
             if (nIterateT < 25) {
@@ -572,4 +815,29 @@ int FN(mco_balance_cell)(const mco_atomic *A, const R *J, R Hden, R *Te, R *Ne, 
             for (int s = 1; s <= ns; ++s) ionDen[(s - 1) * nE + A->xref[elem] - 1] = c.ion[elem][s];
     if (nT) *nT = nIterateT;
     return flags;
+}
+
+int FN(mco_balance_cell)(const mco_atomic *A, const R *J, R Hden, R *Te, R *Ne, R *ionDen, int *nT)
+{
+    return FN(balance_cell)(A, J, Hden, Te, Ne, ionDen, nT, NULL);
+}
+
+int FN(mco_balance_cell_pe)(const mco_atomic *A, const mco_grains *G, int nsMax, const R *J, R Hden, R *Te,
+                            R *Ne, R *ionDen, int *nT, R ndust, const R *Tdust, R *gdc, R *peh, R *pot)
+{
+    if (G->nS * G->nSizes > MCO_MAX_GROWS) return -1;
+    FN(grain_t) g;
+    memset(&g, 0, sizeof g);
+    g.G = G;
+    g.J = J;
+    g.ndust = ndust;
+    g.Tdust = Tdust;
+    g.ldT = nsMax + 1;
+    const int fl = FN(balance_cell)(A, J, Hden, Te, Ne, ionDen, nT, &g);
+    for (int k = 0; k < G->nS * G->nSizes; ++k) {
+        gdc[k] = g.gdc[k];
+        peh[k] = g.peh[k];
+        if (pot) pot[k] = g.pot[k];
+    }
+    return fl;
 }

@@ -10,7 +10,10 @@
  * the absorption integral sum_i sigma_abs(i) radField(i) is located in the
  * emission integrals E(T), T = 1..nTemps K, and interpolated linearly;
  * Tdust(nS,0) = sum_ai Tdust(nS,ai) weight(ai), Tdust(0,0) = sum_nS
- * Tdust(nS,0) abun(nS).  Unreached cells keep their temperatures.
+ * Tdust(nS,0) abun(nS).  Unreached cells keep their temperatures.  With
+ * the photoelectric effect (gdc, peh non-null, [cell][nS][ai]) the
+ * absorption integral gains gasDustColl_d - photoelHeat_d of the cell's last
+ * temperature iteration (update_mod.f90:1886-1889).
  */
 #include <stdint.h>
 #include <string.h>
@@ -42,7 +45,7 @@ LOCATE(double)
 #define DUST(R, NAME)                                                                          \
     void NAME(const R *J, int nc, int nb, int nS, int nsMax, int nSizes, int nTemps,          \
               const float *absx, const float *weight, const float *abun, const float *em,     \
-              int nSpeciesEm, R *Tdust)                                                        \
+              int nSpeciesEm, const R *gdc, const R *peh, R *Tdust)                            \
     {                                                                                          \
         R xa[4096];                                                                            \
         if (nTemps > 4096) return;                                                             \
@@ -61,6 +64,10 @@ LOCATE(double)
                     for (int i = 0; i < nb; ++i) {                                             \
                         R rf = J[c + (size_t)(nc + 1) * i] / (R)PI_REF;                        \
                         integ = integ + (R)sx[i] * rf;                                         \
+                    }                                                                          \
+                    if (gdc) {                                                                 \
+                        const size_t q = (size_t)c * nS * nSizes + (size_t)(s - 1) * nSizes + ai - 1; \
+                        integ = integ + gdc[q] - peh[q];                                       \
                     }                                                                          \
                     for (int t = 0; t < nTemps; ++t)                                           \
                         xa[t] = (R)em[(s - 1) + (size_t)nSpeciesEm * ((ai - 1) + (size_t)nSizes * t)]; \

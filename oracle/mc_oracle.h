@@ -106,14 +106,38 @@ typedef struct {
     int elemOn[31], xref[31];
     float abun[31];
     int32_t chan_g[31 * 10 * 2];
+    int32_t chan_ip[31 * 10];              /* IPnuP of channel (elem, ion) */
     size_t cel_dbl_off[MCO_MAX_CEL], cel_flt_off[MCO_MAX_CEL];
 } mco_atomic;
+
+/* Grains of one dust chemistry for the grain charge, photoelectric heating
+ * and gas-grain collision terms of updateCell (update_mod.f90:527-932). */
+#define MCO_MAX_GROWS 32
+typedef struct {
+    int nS, nSizes;
+    const float *absx;      /* [nS][nSizes][nbins]: xSecArray(dustAbsXsecP(nS,ai)+i-1) */
+    const float *weight;    /* grainWeight(1:nSizes) */
+    const float *gabun;     /* grainAbun(1,1:nS) */
+    const float *vn;        /* grainVn(1:nS) [Ryd] */
+    const float *msurf;     /* MsurfAtom(1:nS) */
+    const float *radius;    /* grainRadius(1:nSizes) [um] */
+    const float *aweight;   /* aWeight(1:30) */
+} mco_grains;
 
 int mco_atomic_init(mco_atomic *A);
 int mco_balance_cell_f32(const mco_atomic *A, const float *J, float Hden, float *Te, float *Ne,
                          float *ionDen, int *nT);
 int mco_balance_cell_f64(const mco_atomic *A, const double *J, double Hden, double *Te, double *Ne,
                          double *ionDen, int *nT);
+/* The same with the grain terms: ndust = Ndust(cell), Tdust = Tdust(0:nsMax,
+ * 0:nSizes, cell) before the update; out: gdc/peh[(nS-1)*nSizes + ai-1] =
+ * gasDustColl_d, photoelHeat_d and pot = grainPot of the last T-iteration. */
+int mco_balance_cell_pe_f32(const mco_atomic *A, const mco_grains *G, int nsMax, const float *J,
+                            float Hden, float *Te, float *Ne, float *ionDen, int *nT, float ndust,
+                            const float *Tdust, float *gdc, float *peh, float *pot);
+int mco_balance_cell_pe_f64(const mco_atomic *A, const mco_grains *G, int nsMax, const double *J,
+                            double Hden, double *Te, double *Ne, double *ionDen, int *nT, double ndust,
+                            const double *Tdust, double *gdc, double *peh, double *pot);
 
 #ifdef __cplusplus
 }

@@ -215,9 +215,15 @@ class _Atomic(C.Structure):
         ("ncel", C.c_int), ("cel_meta", C.c_void_p), ("cel_dbl", C.c_void_p), ("cel_flt", C.c_void_p),
         ("XHILimit", C.c_float),
         ("elemOn", C.c_int * 31), ("xref", C.c_int * 31), ("abun", C.c_float * 31),
-        ("chan_g", C.c_int32 * 620),
+        ("chan_g", C.c_int32 * 620), ("chan_ip", C.c_int32 * 310),
         ("cel_dbl_off", C.c_size_t * 64), ("cel_flt_off", C.c_size_t * 64),
     ]
+
+
+class _Grains(C.Structure):
+    _fields_ = [("nS", C.c_int), ("nSizes", C.c_int), ("absx", C.c_void_p), ("weight", C.c_void_p),
+                ("gabun", C.c_void_p), ("vn", C.c_void_p), ("msurf", C.c_void_p), ("radius", C.c_void_p),
+                ("aweight", C.c_void_p)]
 
 
 class Balance:
@@ -287,8 +293,48 @@ class Balance:
             nT[c] = n.value
         return Te, Ne, ion, flags, nT
 
+    def run_grains(self, J, Hden, Te, Ne, ionDen, g, Tdust, precision="f64", cells=None):
+        """run() with the grain charge, photoelectric and gas-grain collision
+        terms of a dust chemistry (g: a balance fixture's dust_* arrays;
+        Tdust (nCells+1, nSizes+1, nSpeciesMax+1) before the update).
+        Returns run()'s outputs and gdc, peh, pot [nCells+1][nS*nSizes]."""
+        L = lib()
+        dt = np.float32 if precision == "f32" else np.float64
+        ct = C.c_float if precision == "f32" else C.c_double
+        f = L.mco_balance_cell_pe_f32 if precision == "f32" else L.mco_balance_cell_pe_f64
+        f.restype = C.c_int
+        f.argtypes = [C.POINTER(_Atomic), C.POINTER(_Grains), C.c_int, C.c_void_p, ct, C.POINTER(ct),
+                      C.POINTER(ct), C.c_void_p, C.POINTER(C.c_int), ct, C.c_void_p, C.c_void_p, C.c_void_p,
+                      C.c_void_p]
+        nS, nSz, nT_, nsm = (int(v) for v in g["dust_ints"][:4])
+        keep = [np.ascontiguousarray(g[k], np.float32) for k in
+                ("dust_abs", "dust_weight", "dust_abun", "dust_vn", "dust_msurf", "dust_radius", "dust_aweight")]
+        G = _Grains(nS, nSz, *(k.ctypes.data for k in keep))
+        nc1 = J.shape[0]
+        Te = np.array(Te, dt)
+        Ne = np.array(Ne, dt)
+        ion = np.array(ionDen, dt)
+        nd = np.asarray(g["dust_ndust"], dt)
+        Td = np.ascontiguousarray(np.asarray(Tdust, dt).reshape(nc1, nSz + 1, nsm + 1))
+        flags = np.zeros(nc1, np.int32)
+        nT = np.zeros(nc1, np.int32)
+        K = nS * nSz
+        gdc, peh, pot = (np.zeros((nc1, K), dt) for _ in range(3))
+        Jd = np.ascontiguousarray(J, dt)
+        for c in (range(1, nc1) if cells is None else cells):
+            te, ne = ct(Te[c]), ct(Ne[c])
+            sl = np.ascontiguousarray(ion[c].T)
+            n = C.c_int()
+            flags[c] = f(C.byref(self.A), C.byref(G), nsm, Jd[c].ctypes.data, dt(Hden[c]), C.byref(te),
+                         C.byref(ne), sl.ctypes.data, C.byref(n), dt(nd[c]), Td[c].ctypes.data,
+                         gdc[c].ctypes.data, peh[c].ctypes.data, pot[c].ctypes.data)
+            Te[c], Ne[c] = te.value, ne.value
+            ion[c] = sl.T
+            nT[c] = n.value
+        return Te, Ne, ion, flags, nT, gdc, peh, pot
 
-def dust_temps(J, Tdust, g, precision="f64"):
+
+def dust_temps(J, Tdust, g, precision="f64", gdc=None, peh=None):
     """getDustT for every cell (oracle/mc_dust.c).  J: scaled Jste
     (nCells+1, nbins); Tdust: (nCells+1, nSizes+1, nSpeciesMax+1) input
     temperatures; g: dict with dust_ints (nS, nSizes, nTemps, nsMax,
@@ -304,7 +350,12 @@ def dust_temps(J, Tdust, g, precision="f64"):
     keep = [np.ascontiguousarray(g[k], np.float32) for k in ("dust_abs", "dust_weight", "dust_abun", "dust_emint")]
     f.restype = None
     f.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
-                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                  C.c_void_p]
+    add = [None, None]
+    if gdc is not None:
+        add = [np.ascontiguousarray(gdc, dt), np.ascontiguousarray(peh, dt)]
     f(Jf.ctypes.data, nc1 - 1, nb, nS, nsm, nSz, nT, keep[0].ctypes.data, keep[1].ctypes.data,
-      keep[2].ctypes.data, keep[3].ctypes.data, nSp, T.ctypes.data)
+      keep[2].ctypes.data, keep[3].ctypes.data, nSp, None if add[0] is None else add[0].ctypes.data,
+      None if add[1] is None else add[1].ctypes.data, T.ctypes.data)
     return T

@@ -414,7 +414,8 @@ contains
     ! weights, species abundances, the emission integrals dustEmIntegral
     ! (dust_mod.f90:141-183) and the grid's initial Tdust.
     subroutine dump_dust()
-       integer :: hdr(5), nS, ai, nsp, nc
+       integer :: hdr(5), nS, ai, nsp, nc, elem, ist
+       integer :: ipc(nElements, nstages)
        real, allocatable :: xs(:)
        nsp = 1
        nc = grid3D(1)%nCells
@@ -434,6 +435,34 @@ contains
        call dump_r4('dust_abun', grainAbun(nsp, 1:nSpeciesPart(nsp)), nSpeciesPart(nsp))
        call dump_r4('dust_emint', dustEmIntegral, nSpecies * nSizes * nTemps)
        call dump_r4('Tdust0', grid3D(1)%Tdust, (nSpeciesMax+1)*(nSizes+1)*(nc+1))
+       ! grain charge, photoelectric and gas-grain collision inputs
+       ! (update_mod.f90:527-932): flag, grainVn, MsurfAtom, grainRadius,
+       ! Ndust, aWeight and the nuArray index of the ionisation potential
+       ! setDustGasCollHeatCool uses for (elem, istage >= 2)
+       hdr = 0
+       hdr(1) = merge(1, 0, lgPhotoelectric)
+       call dump_i4('dust_pe', hdr, 1)
+       call dump_r4('dust_vn', grainVn, nSpecies)
+       call dump_r4('dust_msurf', MsurfAtom, nSpecies)
+       call dump_r4('dust_radius', grainRadius, nSizes)
+       call dump_r4('dust_ndust', grid3D(1)%Ndust, nc+1)
+       call dump_r4('dust_aweight', aWeight, nElements)
+       ipc = 0
+       do elem = 1, nElements
+          if (.not.lgElementOn(elem)) cycle
+          do ist = 2, min(elem+1, nstages)
+             if (elem == 1) then
+                ipc(elem, ist) = HlevNuP(1)
+             else if (elem == 2 .and. ist == 2) then
+                ipc(elem, ist) = HeIlevNuP(1)
+             else if (elem == 2 .and. ist == 3) then
+                ipc(elem, ist) = HeIIlevNuP(1)
+             else
+                ipc(elem, ist) = elementP(elem, ist-1, nShells(elem, ist-1), 1)
+             end if
+          end do
+       end do
+       call dump_i4('dust_ipcoll', ipc, nElements*nstages)
     end subroutine dump_dust
 
     ! Geometry fixtures: zero opacity, source bin fixed, deltaE = 1.

@@ -118,6 +118,16 @@ DUST_DECKS = {
         "nPhotons 20000", "maxIterateMC 1 100.", "TeStart 10000.",
         "dustFile 'input/species.dat' 'input/sizes.dat'", "MdMg constant 0.01", "noPhotoelectric",
     ], H_HE, {"species.dat": SILICATE, "sizes.dat": None}),
+    # the same with the reference's default grain photoelectric effect: grain
+    # charge, photoelectric heating and gas-grain collisions
+    "dust_pe": ([
+        "nx 9", "ny 9", "nz 9",
+        "edges 3.e18 3.e18 3.e18", "Rin 0.", "Rout 3.e18",
+        "Hdensity 300.", "contShape blackbody", "TStellar 50000.",
+        "LPhot 4.e12", "nbins 160", "nuMin 1.e-5", "nuMax 25.",
+        "nPhotons 20000", "maxIterateMC 1 100.", "TeStart 10000.",
+        "dustFile 'input/species.dat' 'input/sizes.dat'", "MdMg constant 0.01",
+    ], H_HE, {"species.dat": SILICATE, "sizes.dat": None}),
 }
 
 

@@ -152,7 +152,10 @@ def make_balance(deck, npk, seed, name, opacity_scale=None):
     if os.path.exists(os.path.join(d, "dust_ints.bin")):
         dust = dict(dust_ints=rd("dust_ints", np.int32), dust_abs=rd("dust_abs"),
                     dust_weight=rd("dust_weight"), dust_abun=rd("dust_abun"),
-                    dust_emint=rd("dust_emint"), Tdust0=rd("Tdust0"), Tdust=rd("bal_Tdust"))
+                    dust_emint=rd("dust_emint"), Tdust0=rd("Tdust0"), Tdust=rd("bal_Tdust"),
+                    dust_pe=rd("dust_pe", np.int32), dust_vn=rd("dust_vn"), dust_msurf=rd("dust_msurf"),
+                    dust_radius=rd("dust_radius"), dust_ndust=rd("dust_ndust"),
+                    dust_aweight=rd("dust_aweight"), dust_ipcoll=rd("dust_ipcoll", np.int32))
     np.savez_compressed(
         out, ints=prob["ints"], nuArray=prob["nuArray"], **b.to_npz_dict(), **dust,
         Hden=rd("Hden"), Te0=rd("Te"), Ne0=rd("Ne"), ionDen0=rd("ionDen"),
@@ -192,6 +195,7 @@ if __name__ == "__main__":
         make_atomic("c2atom", "c2")
     if what in ("dust", "all"):
         make_balance("dust_bal", 20000, 1618, "dust_bal")
+        make_balance("dust_pe", 20000, 1618, "dust_pe")
         make_transport("dust_mini", 8000, 2718, "dust_mini")
         make_transport("dust_mini", 4000, 3141, "dust_mini_thick", opacity_scale=30.0)
     if what in ("transport", "all"):

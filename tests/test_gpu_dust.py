@@ -25,14 +25,19 @@ def _tables(g):
                 nTemps=nT, nSpeciesMax=nsm)
 
 
+def _pe_tables(g):
+    return dict(_tables(g), photoelectric=True, grainRadius=g["dust_radius"], grainVn=g["dust_vn"],
+                MsurfAtom=g["dust_msurf"], aWeight=g["dust_aweight"], Ndust=g["dust_ndust"])
+
+
 def _context(nc, nb, nu, permute=False):
     from test_gpu_balance import _context as ctx
     return ctx(nc, nb, nu, permute=permute)
 
 
-def _fixture():
-    g = golden("balance_dust_bal")
-    b, J, Hden, Te0, Ne0, ion0, ref = balance_fixture("dust_bal")
+def _fixture(name="dust_bal"):
+    g = golden("balance_" + name)
+    b, J, Hden, Te0, Ne0, ion0, ref = balance_fixture(name)
     nc = J.shape[0] - 1
     nS, nSz, nT, nsm = (int(v) for v in g["dust_ints"][:4])
     shp = lambda a: a.reshape(nc + 1, nSz + 1, nsm + 1)
@@ -195,8 +200,12 @@ def test_rejects_bad_tables():
             bad = ok["emIntegral"].copy()
             bad[0, 1, 3] = np.nan
             T.dust_setup(**dict(ok, emIntegral=bad))
-        with pytest.raises(McError, match="MC_EUNSUP"):
-            T.dust_setup(**ok, photoelectric=True)
+        with pytest.raises(McError, match="MC_ETABLE"):
+            T.dust_setup(**ok, photoelectric=True, grainRadius=[0.1, 0.2], grainVn=[0.5], MsurfAtom=[20.],
+                         aWeight=np.ones(30), Ndust=np.ones(9) * np.nan)
+        with pytest.raises(McError, match="MC_ETABLE"):
+            T.dust_setup(**ok, photoelectric=True, grainRadius=[0.1, 0.0], grainVn=[0.5], MsurfAtom=[20.],
+                         aWeight=np.ones(30), Ndust=np.ones(9))
         fl = np.zeros(9, np.int32)
         assert T._lib.mc_update_cells(T._h, 1.0, None, None, None, None, None, None, fl.ctypes.data,
                                       fl.ctypes.data, None) == 4        # MC_ESTATE: nothing set up
@@ -205,3 +214,59 @@ def test_rejects_bad_tables():
             T.update_grains(np.zeros((9, 3, 3)))
     finally:
         T.close()
+
+
+@pytest.mark.parametrize("permute", [False, True])
+def test_photoelectric_matches_oracle_and_reference(permute):
+    """Gas and grains with the grain charge, photoelectric heating and
+    gas-grain collisions: the GPU against the FP64 restatement (sums across
+    lanes in another order: 1e-9) and against the reference's FP32 updateCell
+    (Te within 1 %, fractions above 1e-4 within 2 %, Tdust within 1e-4)."""
+    print("photoelectric grains with gas, permute=%s" % permute, flush=True)
+    g, b, J, Hden, Te0, Ne0, ion0, ref, T0, Tref = _fixture("dust_pe")
+    T = _context(J.shape[0] - 1, b.nbins, b.nuArray, permute=permute)
+    try:
+        T.set_estimators(J.astype(np.float64))
+        T.balance_setup(b)
+        T.dust_setup(**_pe_tables(g))
+        Tg, fl, nI, ms, Te, Ne, ion = T.update_grains(T0, Hden=Hden, Te=Te0, Ne=Ne0, ionDen=ion0)
+    finally:
+        T.close()
+    B = O.Balance(b)
+    Teo, Neo, iono, flo, nIo, gdc, peh, pot = B.run_grains(J.astype(np.float64), Hden, Te0, Ne0, ion0, g, T0, "f64")
+    To = O.dust_temps(J.astype(np.float64), T0, g, "f64", gdc=gdc, peh=peh)
+    c = slice(1, None)
+    assert np.array_equal(fl[c], flo[c]) and np.array_equal(nI[c], nIo[c])
+    assert np.max(np.abs(Te[c] - Teo[c]) / Teo[c]) < 1e-9
+    assert np.max(np.abs(Ne[c] - Neo[c]) / Neo[c]) < 1e-9
+    on = iono[c] > 0                                     # stages an element has
+    assert np.max(np.abs(ion[c][on] - iono[c][on]) / iono[c][on]) < 1e-8
+    assert np.all(ion[c][~on] == 0)
+    assert np.max(np.abs(Tg[c] - To[c]) / To[c].clip(1)) < 1e-9
+    # against the reference
+    assert np.max(np.abs(Te[c] - ref["Te"][c]) / ref["Te"][c]) < 1e-2
+    big = ref["ion"][c] > 1e-4
+    assert np.max(np.abs(ion[c][big] - ref["ion"][c][big]) / ref["ion"][c][big]) < 2e-2
+    assert np.max(np.abs(Tg[c] - Tref[c]) / Tref[c].clip(1)) < 1e-4
+    print("photoelectric: max |dTe|/Te vs reference %.2e, %.2f ms"
+          % (np.max(np.abs(Te[c] - ref["Te"][c]) / ref["Te"][c]), ms))
+
+
+def test_photoelectric_needs_the_combined_call_and_gas():
+    """With the photoelectric terms set up the gas alone is refused (its
+    balance needs Tdust); grains without gas ignore the terms, as getDustT
+    does when lgGas is false."""
+    from mocassin_amd._capi import McError
+    print("photoelectric call rules", flush=True)
+    g, b, J, Hden, Te0, Ne0, ion0, ref, T0, Tref = _fixture("dust_pe")
+    T = _context(J.shape[0] - 1, b.nbins, b.nuArray)
+    try:
+        T.set_estimators(J.astype(np.float64))
+        T.dust_setup(**_pe_tables(g))
+        Tg = T.update_grains(T0)[0]
+        T.balance_setup(b)
+        with pytest.raises(McError, match="MC_ESTATE"):
+            T.update_cells(Hden, Te0, Ne0, ion0)
+    finally:
+        T.close()
+    assert np.array_equal(Tg, O.dust_temps(J.astype(np.float64), T0, g, "f64"))
