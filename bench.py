@@ -11,7 +11,9 @@ Prints one JSON line with
   cpu_baseline  the C restatement of the same method (oracle/) on a bounded
                 sample of the same workload, on the host's cores
   balance       the per-cell balance of the 45^3 workload on the last
-                sweep's J (reference schedule, FP64): cells/s
+                sweep's J (reference schedule, FP64): cells/s; with_grains:
+                the same with silicate grains (grain charge, photoelectric
+                heating, gas-grain collisions) and their temperatures
   te_convergence  plain vs warm-started balance on the clumpy 100^3 H+He
                 workload (mocassin_amd/converge.py; --no-te skips it)
 
@@ -85,6 +87,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-te", action="store_true", help="skip the Te-convergence measurement")
     ap.add_argument("--no-balance", action="store_true", help="skip the balance measurement")
+    ap.add_argument("--no-grains", action="store_true", help="skip the balance with grains")
     ap.add_argument("--autotune", action="store_true",
                     help="time the launch-shape candidates on this workload and store the best "
                          "in mocassin_amd/tuning_db.json (default: use the stored shape)")
@@ -178,6 +181,27 @@ def main():
                    "solver": "reference schedule (iterateT), FP64, cold start TeStart=1e4 K"}
         print("bench: balance %d cells in %.1f ms (%.2f T-iterations per cell)"
               % (hit.sum(), ms_b, balance["mean_iterations"]), file=sys.stderr, flush=True)
+        if not a.no_grains:
+            # the same balance with silicate grains, their charge and the
+            # photoelectric and collision terms, then the grain temperatures
+            T.dust_setup(**synth.grain_tables(p["nuArray"], Hd))
+            Td0 = np.full((p["nCells"] + 1, 11, 2), 50.0)
+            runs = []
+            for _ in range(2):
+                tb = time.perf_counter()
+                r = T.update_grains(Td0, jscale=js, Hden=Hd, Te=Te0, Ne=Ne0, ionDen=ion0)
+                runs.append((r[3], time.perf_counter() - tb, r))
+            ms_g, wall_g, r = min(runs, key=lambda x: x[0])
+            hit = (r[1][1:] & 16) == 0
+            Tg = r[0][1:, 0, 0][hit]
+            balance["with_grains"] = {
+                "device_ms": ms_g, "host_wall_ms": wall_g * 1e3, "cells_per_s": float(hit.sum()) / (ms_g * 1e-3),
+                "mean_iterations": float(r[2][1:][hit].mean()),
+                "grains": "astronomical silicate, 10 sizes, photoelectric charge and gas-grain collisions",
+                "Tdust_K": [float(Tg.min()), float(np.median(Tg)), float(Tg.max())]}
+            print("bench: balance with grains %.1f ms (%.2f T-iterations per cell), Tdust %.0f-%.0f K"
+                  % (ms_g, balance["with_grains"]["mean_iterations"], Tg.min(), Tg.max()),
+                  file=sys.stderr, flush=True)
     T.close()
 
     value = a.steps * npk / elapsed

@@ -288,3 +288,35 @@ def workload(name, n_packets=1e8, seed=0, nu=None):
         return build_problem(100, 3.e18, 3.e18, 300., 4.e48, 4.e4, 400, H_HE, symmetric=False,
                              n_packets=n_packets, density=clumpy_density(100, seed), nu=nu)
     raise KeyError(name)
+
+
+def grain_tables(nu, Hden, path=None):
+    """Grain tables of astronomical silicate (10 sizes 0.005-0.25 um, a^-3.5)
+    on the frequency mesh nu, for timing the grain steps on a workload: the
+    absorption cross-sections, emission integrals and grain constants the
+    reference dumps for its dust deck (tests/golden/balance_dust_pe.npz),
+    the cross-sections interpolated in log nu; Ndust in the deck's
+    proportion to nH.  Returns the keyword arguments of
+    Transport.dust_setup (photoelectric on)."""
+    import os
+    if path is None:
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "tests", "golden",
+                            "balance_dust_pe.npz")
+    g = dict(np.load(path))
+    nS, nSz, nT, nsm, nSp = (int(v) for v in g["dust_ints"][:5])
+    nu0 = np.asarray(g["nuArray"], np.float64)
+    ab0 = g["dust_abs"].reshape(nS, nSz, -1).astype(np.float64)
+    lnu = np.log(np.asarray(nu, np.float64))
+    absx = np.empty((nS, nSz, lnu.size), np.float32)
+    for s in range(nS):
+        for ai in range(nSz):
+            absx[s, ai] = np.exp(np.interp(lnu, np.log(nu0), np.log(np.maximum(ab0[s, ai], 1e-40))))
+    Hd = np.asarray(g["Hden"], np.float64)
+    nd = np.asarray(g["dust_ndust"], np.float64)
+    ratio = float(np.median(nd[1:] / Hd[1:]))
+    Ndust = (np.asarray(Hden, np.float64) * ratio).astype(np.float32)
+    Ndust[0] = 0.0
+    return dict(absXsec=absx, grainWeight=g["dust_weight"], grainAbun=g["dust_abun"],
+                emIntegral=g["dust_emint"].reshape(nT, nSz, nSp).transpose(2, 1, 0), nTemps=nT,
+                nSpeciesMax=nsm, photoelectric=True, grainRadius=g["dust_radius"], grainVn=g["dust_vn"],
+                MsurfAtom=g["dust_msurf"], aWeight=g["dust_aweight"], Ndust=Ndust)
