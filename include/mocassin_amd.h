@@ -250,11 +250,14 @@ typedef struct {
     double ion_tol;             /* ion fixed point: relative change of X(H0), X(He0), X(He+)
                                    that ends it (the reference uses 0.01) */
     int32_t warm_start;         /* 1: reached cells start from the network's prediction
-                                   (mc_mlp_set first); a cell whose solve from there fails
-                                   restarts from the state passed in (flags bit5) */
+                                   (mc_mlp_set first); a cell whose secant solve from there
+                                   fails, or has not converged after 8 iterations, restarts
+                                   from the state passed in (flags bit5) */
     double first_step;          /* secant: relative size of the first trial step when the
                                    start gives no bracket (0: 0.05); a good warm start
                                    wants a step of the order of its error */
+    double first_slope;         /* secant: that step is the relative residual divided by
+                                   this expected |d((heat-cool)/heat)/d ln Te| (0: 1) */
 } mc_balance_opts;
 
 /* Ionisation and thermal balance of every active cell on the device (FP64),

@@ -77,7 +77,8 @@ class McBatchTuning(C.Structure):
 
 class McBalanceOpts(C.Structure):
     _fields_ = [("solver", C.c_int32), ("max_iter", C.c_int32), ("tol", C.c_double),
-                ("ion_tol", C.c_double), ("warm_start", C.c_int32), ("first_step", C.c_double)]
+                ("ion_tol", C.c_double), ("warm_start", C.c_int32), ("first_step", C.c_double),
+                ("first_slope", C.c_double)]
 
 
 class McDustTables(C.Structure):

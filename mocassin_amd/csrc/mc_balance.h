@@ -96,6 +96,7 @@ struct CellArgs {
     int solver, max_iter;
     double tol, ion_tol;        // ion_tol: the ion fixed point's relative change (reference 0.01)
     double step0;               // secant: relative size of the first trial step
+    double slope0;              // secant: expected |d((heat - cool)/heat)/d ln T| for that step
     // state to restart from when the solve from Te/Ne/ion fails (nullable)
     const double *Te0, *Ne0, *ion0;
     // grains with the photoelectric effect (nullable: gas alone); device

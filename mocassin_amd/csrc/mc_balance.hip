@@ -28,6 +28,10 @@
 namespace mcb {
 
 constexpr int W = 64;
+// temperature iterations of a warm-started secant solve before it restarts
+// from the caller's state: a good start converges in two or three, and one
+// slow cell holds its wavefront for the whole launch's tail
+constexpr int kWarmIter = 8;
 
 struct Lds {
     double Te, Ne, Hden;
@@ -618,15 +622,19 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) ba
                 const double f = res, T = Tn;
                 if (f > 0.) { Tlo = T; haveLo = true; } else { Thi = T; haveHi = true; }
                 const double bl = fmin(Tlo, Thi), bh = fmax(Tlo, Thi);
-                if (nIterT >= a.max_iter) {
+                // a warm-started attempt gets kWarmIter iterations before
+                // the restart from the caller's state
+                const int cap = (attempt == 0 && a.Te0) ? min(a.max_iter, kWarmIter) : a.max_iter;
+                if (nIterT >= cap) {
                     if (haveLo && haveHi) Tn = 0.5 * (bl + bh);
                     flags |= 2 | 4;
                 } else if (haveLo && haveHi && bh - bl <= 1.e-7 * T) {
                     Tn = 0.5 * (bl + bh);
                 } else {
                     // first trial: a step of the relative residual (cooling
-                    // grows about linearly with T near balance), at most step0
-                    const double d0 = fmin(a.step0, fmax(fabs(f) / heat, 1.e-4));
+                    // grows about linearly with T near balance), divided by
+                    // the expected slope d(f/heat)/d ln T, at most step0
+                    const double d0 = fmin(a.step0, fmax(fabs(f) / heat / a.slope0, 1.e-4));
                     double Tc = (havep && f != fp) ? T - f * (T - Tp) / (f - fp)
                                                    : (f > 0. ? T * (1. + d0) : T / (1. + d0));
                     if (haveLo && haveHi) {

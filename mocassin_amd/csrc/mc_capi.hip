@@ -476,6 +476,7 @@ static mcb::CellArgs cell_args(mc_ctx *c, double jscale)
     a.Hden = c->d_Hden; a.Te = c->d_Te; a.Ne = c->d_Ne; a.ion = c->d_ion;
     a.flags = c->d_flags; a.nIter = c->d_nIter;
     a.solver = MC_SOLVER_REFERENCE; a.max_iter = 25; a.tol = 0.01; a.ion_tol = 0.01; a.step0 = 0.05;
+    a.slope0 = 1.0;
     return a;
 }
 
@@ -659,6 +660,11 @@ static int update_cells(mc_ctx *c, double jscale, const mc_balance_opts *opts, c
                     if (!(opts->first_step > 0.0 && opts->first_step <= 1.0))
                         return fail(c, MC_EINVAL, "first_step outside (0,1]");
                     a.step0 = opts->first_step;
+                }
+                if (opts->first_slope != 0.0) {
+                    if (!(opts->first_slope >= 0.1 && opts->first_slope <= 100.0))
+                        return fail(c, MC_EINVAL, "first_slope outside [0.1,100]");
+                    a.slope0 = opts->first_slope;
                 }
             }
             warm = opts->warm_start != 0;

@@ -53,7 +53,7 @@ module mocassin_amd_mod
         integer(c_int32_t) :: solver, max_iter
         real(c_double) :: tol, ion_tol
         integer(c_int32_t) :: warm_start
-        real(c_double) :: first_step
+        real(c_double) :: first_step, first_slope
     end type mc_balance_opts
 
     type, bind(C) :: mc_batch_tuning

@@ -33,7 +33,9 @@ def slot_layout():
 
 def targets(bundle, Te, ion, cells):
     """Targets T [n][31] and mask M [n][31] for `cells` (1-based ids) from a
-    balance state Te (nCells+1,), ion (nCells+1, nElemUsed, nstages)."""
+    balance state Te (nCells+1,), ion (nCells+1, nElemUsed, nstages).  The
+    loss is sum M (y - t)^2 / sum M, so scaling a column of M weighs that
+    output."""
     cells = np.asarray(cells)
     T = np.zeros((cells.size, N_OUT), np.float32)
     M = np.zeros((cells.size, N_OUT), np.float32)

@@ -148,7 +148,8 @@ class Transport:
         self._check(self._lib.mc_balance_setup(self._h, C.byref(A)))
 
     @staticmethod
-    def _opts(solver=None, tol=1e-3, max_iter=60, ion_tol=0.01, warm_start=False, first_step=0.0):
+    def _opts(solver=None, tol=1e-3, max_iter=60, ion_tol=0.01, warm_start=False, first_step=0.0,
+              first_slope=0.0):
         if solver is None and not warm_start and ion_tol == 0.01:
             return None
         opts = _capi.McBalanceOpts()
@@ -156,10 +157,11 @@ class Transport:
         opts.max_iter, opts.tol, opts.ion_tol = int(max_iter), float(tol), float(ion_tol)
         opts.warm_start = 1 if warm_start else 0
         opts.first_step = float(first_step)
+        opts.first_slope = float(first_slope)
         return opts
 
     def update_cells(self, Hden, Te, Ne, ionDen, jscale=1.0, solver=None, tol=1e-3, max_iter=60,
-                     ion_tol=0.01, warm_start=False, first_step=0.0):
+                     ion_tol=0.01, warm_start=False, first_step=0.0, first_slope=0.0):
         """updateCell for every active cell from the device J times jscale.
         Hden/Te/Ne (nCells+1,), ionDen (nCells+1, nElemUsed, nstages).
         solver None: the reference's schedule; "secant": secant iteration to
@@ -176,7 +178,7 @@ class Transport:
         flags = np.zeros(nc1, np.int32)
         nIt = np.zeros(nc1, np.int32)
         ms = C.c_double()
-        opts = self._opts(solver, tol, max_iter, ion_tol, warm_start, first_step)
+        opts = self._opts(solver, tol, max_iter, ion_tol, warm_start, first_step, first_slope)
         self._check(self._lib.mc_balance_run(self._h, float(jscale), None if opts is None else C.byref(opts),
                                              Hd.ctypes.data, Te.ctypes.data,
                                              Ne.ctypes.data, ion.ctypes.data, flags.ctypes.data,
