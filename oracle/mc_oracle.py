@@ -359,3 +359,60 @@ def dust_temps(J, Tdust, g, precision="f64", gdc=None, peh=None):
       keep[2].ctypes.data, keep[3].ctypes.data, nSp, None if add[0] is None else add[0].ctypes.data,
       None if add[1] is None else add[1].ctypes.data, T.ctypes.data)
     return T
+
+
+def opacity_tables(g, gaunt_cache=True, Te_cache=None):
+    """Gas opacity (0:nCells, 1:nbins) of an opacity fixture's state in the
+    reference's FP32 (oracle/mc_opacity.c), cells 1..nCells in order.
+    gaunt_cache: the reference's Gaunt-factor reuse (starting from the
+    uniform initial Te0, as after the initial table step), else exact per
+    cell.  Returns (nCells+1, nbins) float32."""
+    L = lib()
+    f = L.mco_opacity_f32
+    f.restype = None
+    f.argtypes = [C.c_int] * 4 + [C.c_void_p] * 8 + [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float,
+                                                     C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    ints = g["ints"]
+    nc, nb = int(ints[3]), int(ints[4])
+    nE, ns = int(ints[7]) // 100, int(ints[7]) % 100
+    a = lambda k, dt=np.float32: np.ascontiguousarray(g[k], dt)
+    keep = [a("nuArray"), a("Hden"), a("Te"), a("Ne"), a("ionDen"), a("elem_on", np.int32), a("xref", np.int32),
+            a("abun"), a("seg_meta", np.int32), a("seg_sigma"), a("seg_off", np.int64)]
+    out = np.zeros(((nc + 1) * nb), np.float32)
+    tc = C.c_float(float(g["Te0"][1]) if Te_cache is None else Te_cache)
+    gv = C.c_float(0.0)
+    if gaunt_cache:
+        # the initial table step: every cell at Te0, so one recomputation
+        gv = C.c_float(L_gaunt(float(g["Te0"][1]), float(g["nuArray"][0])))
+    f(nc, nb, ns, nE, *(k.ctypes.data for k in keep[:8]), int(g["seg_meta"].shape[0]), keep[8].ctypes.data,
+      keep[9].ctypes.data, keep[10].ctypes.data, float(g["brems1"][0]), 1 if gaunt_cache else 0, C.byref(tc),
+      C.byref(gv), out.ctypes.data)
+    return out.reshape(nb, nc + 1).T.copy()
+
+
+def L_gaunt(Te, nu1, z=0.0, g_prev=0.0):
+    """getGauntFF at one frequency (oracle/mc_opacity.c), FP32."""
+    L = lib()
+    f = L.mco_gaunt_ff
+    f.restype = C.c_float
+    f.argtypes = [C.c_float] * 4
+    return f(z, float(np.log10(np.float32(Te))), float(np.log10(np.float32(nu1))), g_prev)
+
+
+def dust_opacity(g, opacity):
+    """The dust add on top of a gas opacity (nCells+1, nbins); returns
+    (opacity, scaOpac) in FP32."""
+    L = lib()
+    f = L.mco_dust_opacity_f32
+    f.restype = None
+    f.argtypes = [C.c_int] * 5 + [C.c_void_p] * 9
+    ints = g["ints"]
+    nc, nb = int(ints[3]), int(ints[4])
+    nS, nSz, nT, nsm = (int(v) for v in g["dust_ints"][:4])
+    a = lambda k: np.ascontiguousarray(g[k], np.float32)
+    keep = [a("dust_abs"), a("dust_sca"), a("dust_weight"), a("dust_abun"), a("dust_tsub"), a("dust_ndust"),
+            a("Tdust")]
+    op = np.asfortranarray(np.asarray(opacity, np.float32)).copy(order="F")
+    sca = np.zeros_like(op, order="F")
+    f(nc, nb, nS, nsm, nSz, *(k.ctypes.data for k in keep), sca.ctypes.data, op.ctypes.data)
+    return np.ascontiguousarray(op), np.ascontiguousarray(sca)

@@ -17,6 +17,7 @@
 !   mcref_harness transport OUTDIR SEED NPACKETS
 !   mcref_harness walk      OUTDIR SEED NPACKETS NUBIN ORIGINFILE
 !   mcref_harness balance   OUTDIR SEED NPACKETS
+!   mcref_harness opacity   OUTDIR SEED NPACKETS
 program mcref_harness
     use common_mod
     use constants_mod
@@ -103,6 +104,10 @@ program mcref_harness
     case ("balance")
        call run_transport()
        call run_balance()
+    case ("opacity")
+       call run_transport()
+       call run_balance()
+       call run_opacity()
     case ("walk")
        call get_command_argument(5, arg); read(arg, *) nuBin
        call get_command_argument(6, originFile)
@@ -408,6 +413,88 @@ contains
        call dump_i4('bal_black', grid3D(1)%lgBlack, nc+1)
        if (lgDust) call dump_r4('bal_Tdust', TdustTemp, (nSpeciesMax+1)*(nSizes+1)*(nc+1))
     end subroutine run_balance
+
+    ! The opacity tables after the balance pass (iteration_mod.f90:117-228
+    ! on the updated Te, Ne, ionDen, Tdust): ionizationDriver per cell in
+    ! iterateMC's loop order, then the dust add; and the cross-section
+    ! segments addOpacity adds, in its order: (elem, ion, shell, nuLowP,
+    ! nuHighP, xSecP) with the whole xSecArray and bremsXSecP.
+    subroutine run_opacity()
+       integer :: nc, elem, ion, sh, nseg
+       integer, allocatable :: meta(:)
+       nc = grid3D(1)%nCells
+       call dump_r4('op_Te', grid3D(1)%Te, nc+1)
+       call dump_r4('op_Ne', grid3D(1)%Ne, nc+1)
+       call dump_r4('op_ionDen', grid3D(1)%ionDen, (nc+1)*nElementsUsed*nStages)
+       if (lgDust) call dump_r4('op_Tdust', grid3D(1)%Tdust, (nSpeciesMax+1)*(nSizes+1)*(nc+1))
+       grid3D(1)%opacity(0:nc, 1:nbins) = 0.
+       do i = 1, grid3D(1)%nx
+          do j = 1, grid3D(1)%ny
+             do k = 1, grid3D(1)%nz
+                call ionizationDriver(grid3D(1), i, j, k)
+             end do
+          end do
+       end do
+       call dump_r4('op_gas', grid3D(1)%opacity, (nc+1)*nbins)
+       if (lgDust) then
+          grid3D(1)%scaOpac(0:nc, 1:nbins) = 0.
+          grid3D(1)%absOpac(0:nc, 1:nbins) = 0.
+          do i = 1, grid3D(1)%nx
+             do j = 1, grid3D(1)%ny
+                do k = 1, grid3D(1)%nz
+                   ic = grid3D(1)%active(i,j,k)
+                   if (ic > 0) call add_dust(1, ic)
+                end do
+             end do
+          end do
+          call dump_r4('op_total', grid3D(1)%opacity, (nc+1)*nbins)
+          call dump_r4('op_sca', grid3D(1)%scaOpac, (nc+1)*nbins)
+          call dump_dust_sca()
+       end if
+       allocate(meta(6 * 4000))
+       nseg = 0
+       call seg(1, 1, 0, HlevNuP(1), nbins, HlevXSecP(1))
+       call seg(2, 1, 0, HeIlevNuP(1), nbins, HeISingXSecP(1))
+       call seg(2, 2, 0, HeIIlevNuP(1), nbins, HeIIXSecP(1))
+       do elem = 3, nElements
+          if (.not.lgElementOn(elem)) cycle
+          do ion = 1, min(elem, nstages)
+             do sh = 1, nShells(elem, ion)
+                call seg(elem, ion, sh, elementP(elem, ion, sh, 1), elementP(elem, ion, sh, 2), &
+                         elementP(elem, ion, sh, 3))
+             end do
+          end do
+       end do
+       call dump_i4('op_seg', meta, 6 * nseg)
+       call dump_r4('op_xsec', xSecArray, size(xSecArray))
+       meta(1) = bremsXSecP; meta(2) = nseg; meta(3) = size(xSecArray)
+       call dump_i4('op_ints', meta, 3)
+    contains
+       subroutine seg(e, n, s, lo, hi, xp)
+          integer, intent(in) :: e, n, s, lo, hi, xp
+          meta(6*nseg+1) = e; meta(6*nseg+2) = n; meta(6*nseg+3) = s
+          meta(6*nseg+4) = lo; meta(6*nseg+5) = hi; meta(6*nseg+6) = xp
+          nseg = nseg + 1
+       end subroutine seg
+    end subroutine run_opacity
+
+    ! scattering cross-sections of every species and size on nuArray and the
+    ! sublimation temperatures (the dust add of iteration_mod.f90:184-222)
+    subroutine dump_dust_sca()
+       integer :: nS, ai, nsp
+       real, allocatable :: xs(:)
+       nsp = 1
+       allocate(xs(nbins * nSizes * nSpeciesPart(nsp)))
+       do nS = 1, nSpeciesPart(nsp)
+          do ai = 1, nSizes
+             do i = 1, nbins
+                xs(i + nbins * ((ai - 1) + nSizes * (nS - 1))) = xSecArray(dustScaXsecP(nS,ai) + i - 1)
+             end do
+          end do
+       end do
+       call dump_r4('dust_sca', xs, size(xs))
+       call dump_r4('dust_tsub', TdustSublime, nSpecies)
+    end subroutine dump_dust_sca
 
     ! Inputs of getDustT (update_mod.f90:1836-1945) for one dust chemistry:
     ! absorption cross sections of every species and size on nuArray, size

@@ -165,6 +165,40 @@ def make_balance(deck, npk, seed, name, opacity_scale=None):
     print("wrote", out, "cells", nc, "J nonzero", len(nzJ))
 
 
+def make_opacity(deck, npk, seed, name, keep_cells=96):
+    """Reference transport + one updateCell pass, then the opacity tables of
+    the updated state (ionizationDriver in iterateMC's order, and the dust
+    add): the state, the cross-section segments addOpacity adds, and the
+    resulting opacities of `keep_cells` evenly spaced cells."""
+    d = run_harness(deck, "opacity", [seed, npk], tag="op_" + name)
+    prob, (nx, ny, nz, nc, nb) = read_problem(d)
+    rd = lambda n, dt=np.float32: np.fromfile(os.path.join(d, n + ".bin"), dt)
+    seg = rd("op_seg", np.int32).reshape(-1, 6)
+    xs = rd("op_xsec")
+    bp, nseg, _ = (int(v) for v in rd("op_ints", np.int32))
+    # the entries inOpacity reads: bins lo..max(lo, min(hi, nbins))
+    sig, off = [], [0]
+    for e, n, sh, lo, hi, xp in seg:
+        iup = max(lo, min(hi, nb))
+        sig.append(xs[xp - 1:xp - 1 + iup - lo + 1])
+        off.append(off[-1] + iup - lo + 1)
+    cells = np.unique(np.linspace(1, nc, min(keep_cells, nc)).astype(np.int32))
+    pick = lambda a: a.reshape(nb, nc + 1).T[cells].copy()
+    out = dict(ints=prob["ints"], nuArray=prob["nuArray"], Hden=rd("Hden"), Te0=rd("Te"),
+               Te=rd("op_Te"), Ne=rd("op_Ne"), ionDen=rd("op_ionDen"),
+               elem_on=rd("at_elem_on", np.int32), xref=rd("at_xref", np.int32), abun=rd("at_abun"),
+               at_ints=rd("at_ints", np.int32), seg_meta=seg, seg_sigma=np.concatenate(sig),
+               seg_off=np.array(off, np.int64), brems1=xs[bp - 1:bp], cells=cells, op_gas=pick(rd("op_gas")))
+    if os.path.exists(os.path.join(d, "op_total.bin")):
+        out.update(Tdust=rd("op_Tdust"), op_total=pick(rd("op_total")), op_sca=pick(rd("op_sca")),
+                   dust_ints=rd("dust_ints", np.int32), dust_abs=rd("dust_abs"), dust_sca=rd("dust_sca"),
+                   dust_weight=rd("dust_weight"), dust_abun=rd("dust_abun"), dust_tsub=rd("dust_tsub"),
+                   dust_ndust=rd("dust_ndust"))
+    path = os.path.join(HERE, "opacity_%s.npz" % name)
+    np.savez_compressed(path, **out)
+    print("wrote", path, "cells", nc, "segments", nseg, "bins", nb)
+
+
 def make_atomic(deck, name):
     """Atomic inputs of the balance for a workload's frequency mesh and
     composition, with the reference's initial cell state (TeStart and its
@@ -198,6 +232,9 @@ if __name__ == "__main__":
         make_balance("dust_pe", 20000, 1618, "dust_pe")
         make_transport("dust_mini", 8000, 2718, "dust_mini")
         make_transport("dust_mini", 4000, 3141, "dust_mini_thick", opacity_scale=30.0)
+    if what in ("opacity", "all"):
+        make_opacity("c2mini", 20000, 4243, "c2mini")
+        make_opacity("dust_pe", 20000, 1619, "dust_pe")
     if what in ("transport", "all"):
         make_transport("c1mini", 20000, 12345, "c1mini")
         make_transport("c1mini", 8000, 999, "c1mini_thick", opacity_scale=300.0)
