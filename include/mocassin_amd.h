@@ -152,6 +152,7 @@ int mc_ctx_create(const mc_desc *desc, mc_ctx **out);
 
 /* Upload the per-sweep tables (frozen during a transport sweep).
  * scaOpac/gSca may be NULL without dust; recPDF/totalLines NULL without gas.
+ * opacity (and scaOpac) NULL keep the tables mc_opacity_build wrote.
  * Replaces: the reads of grid%opacity, grid%scaOpac, grid%recPDF,
  * grid%totalLines, inSpectrumProbDen, gSca inside photon_mod.f90:768-1812;
  * the tables are produced by ionizationDriver (ionization_mod.f90:349-484),
@@ -302,7 +303,11 @@ typedef struct {
     const float *grainVn;       /* (1:nSpecies) [Ryd]                      photoelectric only */
     const float *MsurfAtom;     /* (1:nSpecies), as the species file gives it  photoelectric only */
     const float *aWeight;       /* (1:30) atomic weights                   photoelectric only */
-    const float *Ndust;         /* (0:nCells) grain number density         photoelectric only */
+    const float *Ndust;         /* (0:nCells) grain number density   photoelectric, opacity */
+    const float *scaXsec;       /* [nSpecies][nSizes][nbins]: xSecArray(dustScaXsecP(nS,ai)+i-1);
+                                   with TdustSublime and Ndust, the dust add of
+                                   mc_opacity_build (NULL: no dust opacity on the device) */
+    const float *TdustSublime;  /* (1:nSpecies) sublimation temperatures [K] */
     float XHILimit;             /* grains without gas: |dTdust(0,0)|/Tdust(0,0) <= XHILimit
                                    marks a cell converged (update_mod.f90:316-324) */
 } mc_dust_tables;
@@ -332,6 +337,46 @@ int mc_dust_setup(mc_ctx *ctx, const mc_dust_tables *dust);
 int mc_update_cells(mc_ctx *ctx, double jscale, const mc_balance_opts *opts, const double *Hden,
                     double *Te, double *Ne, double *ionDen, double *Tdust, int32_t *flags,
                     int32_t *nIter, double *kernel_ms);
+
+/* ---- opacity tables -------------------------------------------------------
+ * The photoionisation cross-sections addOpacity adds (ionization_mod.f90:
+ * 329-484), in its order: the H, He I and He II ground levels, then every
+ * element switched on from Z = 3 by atomic number, ions 1..min(Z, nstages),
+ * shells 1..nShells (elementP).  Segment s covers bins seg_meta[s][2] ..
+ * seg_meta[s][3] (1-based; the last is max(nuLowP, min(nuHighP, nbins)), the
+ * range inOpacity loops over) with those bins' cross-sections in seg_sigma,
+ * segment after segment. */
+typedef struct {
+    int32_t nseg;
+    const int32_t *seg_meta;    /* [nseg][4]: elem, ion, first bin, last bin */
+    const float *seg_sigma;     /* the cross-sections, in the units xSecArray holds them */
+    float brems1;               /* xSecArray(bremsXSecP): free-free factor of bin 1 */
+} mc_opacity_tables;
+
+/* Upload the segment tables (validated against the atomic data of
+ * mc_balance_setup, which must come first).  Replaces: the module data
+ * ionizationDriver reads (ionization_mod.f90:26-484). */
+int mc_opacity_setup(mc_ctx *ctx, const mc_opacity_tables *tables);
+
+/* The opacity tables of every active cell on the device, in FP32, written
+ * into the transport's own tables (the next sweep uses them; pass opacity =
+ * NULL to mc_upload_tables).  The gas part is ionizationDriver per cell:
+ * ion densities, the free-free term of bin 1 (Hummer's Gaunt factor at the
+ * cell's own Te; the reference reuses one for cells within 10 % of the Te it
+ * was computed at) and the photoionisation segments.  With grains set up
+ * with scaXsec, the dust add follows: grains below their sublimation
+ * temperature add grainAbun grainWeight Ndust sigma to scaOpac and absOpac,
+ * and opacity gains both.  The state is Hden, Te, Ne (0:nCells) and
+ * ionDen(0:nCells, 1:nElemUsed, 1:nstages) as the host holds them (REAL);
+ * all four NULL take the device state of the last mc_update_cells /
+ * mc_balance_run, rounded to FP32.  Tdust (0:nSpeciesMax, 0:nSizes,
+ * 0:nCells), NULL: the device Tdust of the last mc_update_cells.  opacity
+ * and scaOpac (nullable): host copies (0:nCells, 1:nbins).  kernel_ms
+ * (nullable): device time.  Replaces: the table step of iterateMC
+ * (iteration_mod.f90:117-228) with ionizationDriver (ionization_mod.f90:
+ * 26-484). */
+int mc_opacity_build(mc_ctx *ctx, const float *Hden, const float *Te, const float *Ne, const float *ionDen,
+                     const float *Tdust, float *opacity, float *scaOpac, double *kernel_ms);
 
 /* ---- warm-start network (no counterpart in the reference) -------------
  * A 72 -> 128 -> 128 -> 128 -> 31 ReLU perceptron with BF16 weights and

@@ -21,7 +21,8 @@ EXPORTS = ("mc_ctx_create", "mc_upload_tables", "mc_zero_estimators", "mc_transp
            "mc_set_estimators", "mc_balance_setup", "mc_balance_run",
            "mc_mlp_set", "mc_mlp_get", "mc_mlp_features", "mc_mlp_predict", "mc_mlp_loss_grad",
            "mc_mlp_train", "mc_count_crossings", "mc_fetch_crossings", "mc_transport_batch",
-           "mc_dust_setup", "mc_update_cells", "mc_ctx_last_error", "mc_ctx_destroy")
+           "mc_dust_setup", "mc_update_cells", "mc_opacity_setup", "mc_opacity_build",
+           "mc_ctx_last_error", "mc_ctx_destroy")
 
 MC_SOLVER_REFERENCE, MC_SOLVER_SECANT = 0, 1
 MC_MLP_NIN, MC_MLP_NOUT, MC_MLP_NPARAMS, MC_MLP_NSCALE = 72, 31, 49568, 206
@@ -86,7 +87,12 @@ class McDustTables(C.Structure):
                 ("nTemps", C.c_int32), ("photoelectric", C.c_int32), ("absXsec", C.c_void_p),
                 ("grainWeight", C.c_void_p), ("grainAbun", C.c_void_p), ("emIntegral", C.c_void_p),
                 ("grainRadius", C.c_void_p), ("grainVn", C.c_void_p), ("MsurfAtom", C.c_void_p),
-                ("aWeight", C.c_void_p), ("Ndust", C.c_void_p), ("XHILimit", C.c_float)]
+                ("aWeight", C.c_void_p), ("Ndust", C.c_void_p), ("scaXsec", C.c_void_p),
+                ("TdustSublime", C.c_void_p), ("XHILimit", C.c_float)]
+
+
+class McOpacityTables(C.Structure):
+    _fields_ = [("nseg", C.c_int32), ("seg_meta", C.c_void_p), ("seg_sigma", C.c_void_p), ("brems1", C.c_float)]
 
 
 class McError(RuntimeError):
@@ -129,6 +135,8 @@ def load():
                                      C.POINTER(McCounters), vp]
     L.mc_dust_setup.argtypes = [vp, C.POINTER(McDustTables)]
     L.mc_update_cells.argtypes = [vp, C.c_double, C.POINTER(McBalanceOpts)] + [vp] * 8
+    L.mc_opacity_setup.argtypes = [vp, C.POINTER(McOpacityTables)]
+    L.mc_opacity_build.argtypes = [vp] * 9
     L.mc_count_crossings.argtypes = [vp, i32]
     L.mc_fetch_crossings.argtypes = [vp, vp]
     L.mc_ctx_last_error.argtypes = [vp]

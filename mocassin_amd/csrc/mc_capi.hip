@@ -17,6 +17,7 @@
 #include "mc_dust.h"
 #include "mc_kernels.h"
 #include "mc_mlp.h"
+#include "mc_opacity.h"
 
 #pragma clang fp contract(off)
 
@@ -71,6 +72,18 @@ struct mc_ctx {
     double *pe_gdc = nullptr, *pe_peh = nullptr;
     mcb::Grains *d_grains = nullptr;
     std::vector<float> nuArray;
+    // opacity tables on the device
+    bool opac = false, opac_built = false, opac_sca_built = false;
+    bool state_dev = false, tdust_dev = false;   // device state written by a balance / grain run
+    bool dust_opac = false;                      // grains carry the dust-add tables
+    int dust_rows = 0;
+    float *d_op_dust = nullptr;                  // dsca | dabs | dw | tsub | Ndust
+    int *d_op_i = nullptr;                       // slot_kind | bin_start | ent_slot | slot_e | slot_stage
+    float *d_op_f = nullptr;                     // slot_abun | slot_efac | ent_sigma
+    float *d_op_st = nullptr, *d_op_Td = nullptr, *d_op_out = nullptr;
+    mco::OpacArgs op{};
+    int *op_slot_e = nullptr, *op_slot_st = nullptr;   // into d_op_i
+    std::vector<int> op_sl_e, op_sl_st;                // slot -> element index, stage
     char err[512] = "";
 };
 
@@ -236,14 +249,17 @@ extern "C" int mc_upload_tables(mc_ctx *c, const float *opacity, const float *sc
     const int nc = G.nCells, nb = G.nbins;
     const size_t nt = (size_t)(nc + 1) * nb;
     const bool dust = (G.flags & MC_DUST) != 0, gas = (G.flags & MC_GAS) != 0;
-    if (!opacity || !specCDF) return fail(c, MC_EINVAL, "opacity and specCDF are required");
+    if (!specCDF) return fail(c, MC_EINVAL, "specCDF is required");
+    if (!opacity && !c->opac_built) return fail(c, MC_EINVAL, "opacity is required (mc_opacity_build has not run)");
     if (gas && (!recPDF || !totalLines)) return fail(c, MC_EINVAL, "gas needs recPDF and totalLines");
-    if (dust && !scaOpac) return fail(c, MC_EINVAL, "dust needs scaOpac");
+    if (dust && !scaOpac && !(c->opac_built && c->opac_sca_built && !opacity))
+        return fail(c, MC_EINVAL, "dust needs scaOpac");
     if (dust && !(G.flags & MC_ISOTROPIC) && !gSca) return fail(c, MC_EINVAL, "HG scattering needs gSca");
     HIPCHK(c, hipSetDevice(c->device));
-    for (size_t i = 0; i < nt; ++i)
-        if (!(opacity[i] >= 0.f) || std::isinf(opacity[i]))
-            return fail(c, MC_ETABLE, "opacity entry %zu not finite and >= 0", i);
+    if (opacity)
+        for (size_t i = 0; i < nt; ++i)
+            if (!(opacity[i] >= 0.f) || std::isinf(opacity[i]))
+                return fail(c, MC_ETABLE, "opacity entry %zu not finite and >= 0", i);
     for (int s = 0; s < c->nStars; ++s)
         if (!cdf_ok(specCDF + (size_t)s * nb, 1, nb))
             return fail(c, MC_ETABLE, "specCDF of source %d not a monotone CDF", s + 1);
@@ -258,17 +274,22 @@ extern "C" int mc_upload_tables(mc_ctx *c, const float *opacity, const float *sc
         for (int cell = 0; cell <= nc; ++cell)
             if (std::isnan(totalLines[cell])) return fail(c, MC_ETABLE, "NaN in totalLines");
     }
-    for (int nu = 0; nu < nb; ++nu)
-        if (opacity[(size_t)nu * (nc + 1)] != 0.f)
-            return fail(c, MC_ETABLE, "opacity row 0 (inactive cells) must be zero (bin %d)", nu + 1);
+    if (opacity)
+        for (int nu = 0; nu < nb; ++nu)
+            if (opacity[(size_t)nu * (nc + 1)] != 0.f)
+                return fail(c, MC_ETABLE, "opacity row 0 (inactive cells) must be zero (bin %d)", nu + 1);
     const size_t ncell3 = (size_t)G.ncell3;
     float *tmp = nullptr;
     HIPCHK(c, dalloc(&tmp, nt));
     int rc = MC_OK;
-    hipError_t e = hipMemcpy(tmp, opacity, nt * sizeof(float), hipMemcpyHostToDevice);
-    if (e == hipSuccess && !c->d_opac3) e = dalloc(&c->d_opac3, ncell3 * nb);
-    if (e == hipSuccess) e = mcd_densify(tmp, c->d_active, (int)ncell3, nc + 1, nb, c->d_opac3, c->stream);
-    if (e == hipSuccess && dust) {
+    hipError_t e = hipSuccess;
+    if (opacity) {
+        e = hipMemcpy(tmp, opacity, nt * sizeof(float), hipMemcpyHostToDevice);
+        if (e == hipSuccess && !c->d_opac3) e = dalloc(&c->d_opac3, ncell3 * nb);
+        if (e == hipSuccess) e = mcd_densify(tmp, c->d_active, (int)ncell3, nc + 1, nb, c->d_opac3, c->stream);
+        c->opac_built = false;
+    }
+    if (e == hipSuccess && dust && scaOpac) {
         e = hipStreamSynchronize(c->stream);
         if (e == hipSuccess) e = hipMemcpy(tmp, scaOpac, nt * sizeof(float), hipMemcpyHostToDevice);
         if (e == hipSuccess && !c->d_sca3) e = dalloc(&c->d_sca3, ncell3 * nb);
@@ -619,6 +640,34 @@ extern "C" int mc_dust_setup(mc_ctx *c, const mc_dust_tables *d)
     a.mono = c->d_dust_mono;
     a.Tdust = c->d_Tdust;
     a.XHILimit = d->XHILimit;
+    // the dust add of the opacity build: per row the scattering and
+    // absorption cross-sections, grainAbun * grainWeight (FP32, as the
+    // reference multiplies them), the sublimation temperature; Ndust
+    c->dust_opac = false;
+    if (c->d_op_dust) { (void)hipFree(c->d_op_dust); c->d_op_dust = nullptr; }
+    if (d->scaXsec) {
+        if (!d->TdustSublime || !d->Ndust) return fail(c, MC_EINVAL, "scaXsec needs TdustSublime and Ndust");
+        std::vector<float> t((size_t)2 * K * nb + 2 * K + nc1);
+        float *sc = t.data(), *ab_ = sc + (size_t)K * nb, *dw = ab_ + (size_t)K * nb, *ts = dw + K, *nd = ts + K;
+        for (size_t i = 0; i < (size_t)K * nb; ++i) {
+            if (!std::isfinite(d->scaXsec[i])) return fail(c, MC_ETABLE, "scaXsec entry %zu not finite", i);
+            sc[i] = d->scaXsec[i];
+            ab_[i] = d->absXsec[i];
+        }
+        for (int s_ = 0; s_ < nS; ++s_)
+            for (int ai = 0; ai < nSz; ++ai) {
+                dw[s_ * nSz + ai] = d->grainAbun[s_] * d->grainWeight[ai];
+                ts[s_ * nSz + ai] = d->TdustSublime[s_];
+            }
+        for (size_t i = 0; i < nc1; ++i) {
+            if (!std::isfinite(d->Ndust[i]) || d->Ndust[i] < 0.0f) return fail(c, MC_ETABLE, "Ndust(%zu) invalid", i);
+            nd[i] = d->Ndust[i];
+        }
+        HIPCHK(c, dalloc(&c->d_op_dust, t.size()));
+        HIPCHK(c, hipMemcpy(c->d_op_dust, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+        c->dust_opac = true;
+        c->dust_rows = K;
+    }
     c->dust = true;
     c->dust_pe = pe;
     return MC_OK;
@@ -724,6 +773,7 @@ static int update_cells(mc_ctx *c, double jscale, const mc_balance_opts *opts, c
             a.grains_nb = c->G.nbins;
         }
         HIPCHK(c, mcb_launch_balance(c->d_atomic, a, c->stream));
+        c->state_dev = true;
     }
     if (dust) {
         mcg::DustArgs da = da0;
@@ -734,6 +784,7 @@ static int update_cells(mc_ctx *c, double jscale, const mc_balance_opts *opts, c
             da.peh = c->pe_peh;
         }
         HIPCHK(c, mcg_launch_dust(da, c->d_J3, c->d_active, c->stream));
+        c->tdust_dev = true;
     }
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     if (gas) {
@@ -782,6 +833,191 @@ extern "C" int mc_update_cells(mc_ctx *c, double jscale, const mc_balance_opts *
 {
     if (!c) return MC_EINVAL;
     return update_cells(c, jscale, opts, Hden, Te, Ne, ionDen, Tdust, flags, nIter, kernel_ms);
+}
+
+// --------------------------------------------------------- opacity tables
+extern "C" int mc_opacity_setup(mc_ctx *c, const mc_opacity_tables *t)
+{
+    if (!c) return MC_EINVAL;
+    if (!t || !t->seg_meta || !t->seg_sigma || t->nseg < 1) return fail(c, MC_EINVAL, "empty segment tables");
+    if (!c->balance) return fail(c, MC_ESTATE, "mc_balance_setup has not been called");
+    if (!std::isfinite(t->brems1)) return fail(c, MC_ETABLE, "brems1 not finite");
+    const mcb::Atomic &A = c->bal.A;
+    const int nb = c->G.nbins, ns = A.nstages;
+    // slots: (element, ion 1..min(Z, nstages)) in atomic-number order
+    std::vector<int> kind, sl_e, sl_st, slot_of(31 * 10, -1);
+    std::vector<float> sabun, efac;
+    int he2 = -1;
+    for (int e = 0; e < A.nE; ++e) {
+        const int z = A.el[e].z;
+        for (int st = 1; st <= std::min(z, ns); ++st) {
+            slot_of[z * 10 + st] = (int)kind.size();
+            if (z == 2 && st == 2) he2 = (int)kind.size();
+            kind.push_back(z > 2 ? 1 : 0);
+            sl_e.push_back(e);
+            sl_st.push_back(st);
+            sabun.push_back((float)A.el[e].abun);
+            efac.push_back(z > 2 && st >= 2 ? (float)((st - 1) * (st - 1)) : 0.f);
+        }
+    }
+    const int nslot = (int)kind.size();
+    if (nslot > mco::kMaxSlots) return fail(c, MC_EUNSUP, "more than %d ion slots", mco::kMaxSlots);
+    // entries of every bin, in segment order
+    std::vector<std::vector<std::pair<int, float>>> bins(nb);
+    size_t off = 0;
+    for (int s = 0; s < t->nseg; ++s) {
+        const int32_t *m = t->seg_meta + 4 * s;
+        const int z = m[0], ion = m[1], lo = m[2], hi = m[3];
+        if (z < 1 || z > 30 || ion < 1 || ion > 9 || slot_of[z * 10 + ion] < 0)
+            return fail(c, MC_ETABLE, "segment %d: (Z=%d, ion %d) not among the ions switched on", s, z, ion);
+        if (lo < 1 || hi < lo || hi > nb) return fail(c, MC_ETABLE, "segment %d: bins %d..%d outside 1..%d", s, lo, hi, nb);
+        for (int i = lo; i <= hi; ++i) {
+            const float v = t->seg_sigma[off + (size_t)(i - lo)];
+            if (!std::isfinite(v)) return fail(c, MC_ETABLE, "segment %d: cross-section not finite", s);
+            bins[i - 1].push_back({ slot_of[z * 10 + ion], v });
+        }
+        off += (size_t)(hi - lo + 1);
+    }
+    std::vector<int> bstart(nb + 1, 0), eslot;
+    std::vector<float> esig;
+    for (int i = 0; i < nb; ++i) {
+        bstart[i + 1] = bstart[i] + (int)bins[i].size();
+        for (auto &p : bins[i]) { eslot.push_back(p.first); esig.push_back(p.second); }
+    }
+    const size_t nnz = eslot.size();
+    std::vector<int> iv;
+    iv.insert(iv.end(), kind.begin(), kind.end());
+    iv.insert(iv.end(), bstart.begin(), bstart.end());
+    iv.insert(iv.end(), eslot.begin(), eslot.end());
+    iv.insert(iv.end(), sl_e.begin(), sl_e.end());
+    iv.insert(iv.end(), sl_st.begin(), sl_st.end());
+    std::vector<float> fv;
+    fv.insert(fv.end(), sabun.begin(), sabun.end());
+    fv.insert(fv.end(), efac.begin(), efac.end());
+    fv.insert(fv.end(), esig.begin(), esig.end());
+    HIPCHK(c, hipSetDevice(c->device));
+    c->opac = false;
+    if (c->d_op_i) { (void)hipFree(c->d_op_i); c->d_op_i = nullptr; }
+    if (c->d_op_f) { (void)hipFree(c->d_op_f); c->d_op_f = nullptr; }
+    if (c->d_op_st) { (void)hipFree(c->d_op_st); c->d_op_st = nullptr; }
+    HIPCHK(c, dalloc(&c->d_op_i, iv.size()));
+    HIPCHK(c, dalloc(&c->d_op_f, fv.size()));
+    HIPCHK(c, hipMemcpy(c->d_op_i, iv.data(), iv.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_op_f, fv.data(), fv.size() * sizeof(float), hipMemcpyHostToDevice));
+    const size_t nc1 = (size_t)c->G.nCells + 1;
+    const int ldst = 3 + nslot;
+    HIPCHK(c, dalloc(&c->d_op_st, nc1 * ldst));
+    mco::OpacArgs &o = c->op;
+    o = mco::OpacArgs{};
+    o.ncell3 = c->G.ncell3; o.nb = nb; o.nc = c->G.nCells;
+    o.active = c->d_active;
+    o.st = c->d_op_st; o.ldst = ldst; o.nslot = nslot;
+    o.slot_kind = c->d_op_i;
+    o.bin_start = c->d_op_i + nslot;
+    o.ent_slot = o.bin_start + nb + 1;
+    o.slot_abun = c->d_op_f;
+    o.slot_efac = c->d_op_f + nslot;
+    o.ent_sigma = c->d_op_f + 2 * nslot;
+    o.slot_he2 = he2;
+    o.brems1 = t->brems1;
+    o.nu1 = c->nuArray[0];
+    o.xlf1 = log10f(c->nuArray[0]);
+    c->op_slot_e = c->d_op_i + nslot + (nb + 1) + nnz;
+    c->op_slot_st = c->op_slot_e + nslot;
+    c->op_sl_e = sl_e;
+    c->op_sl_st = sl_st;
+    c->opac = true;
+    return MC_OK;
+}
+
+extern "C" int mc_opacity_build(mc_ctx *c, const float *Hden, const float *Te, const float *Ne, const float *ionDen,
+                                const float *Tdust, float *opacity, float *scaOpac, double *kernel_ms)
+{
+    if (!c) return MC_EINVAL;
+    if (!c->opac) return fail(c, MC_ESTATE, "mc_opacity_setup has not been called");
+    const int nc = c->G.nCells, nb = c->G.nbins;
+    const size_t nc1 = (size_t)nc + 1, n3 = (size_t)c->G.ncell3;
+    mco::OpacArgs o = c->op;
+    const int nslot = o.nslot, ldst = o.ldst;
+    const bool host_state = Hden || Te || Ne || ionDen;
+    if (host_state && !(Hden && Te && Ne && ionDen))
+        return fail(c, MC_EINVAL, "Hden, Te, Ne and ionDen go together (or all NULL for the device state)");
+    if (!host_state && !c->state_dev)
+        return fail(c, MC_ESTATE, "no device state yet: pass the state or run mc_update_cells first");
+    const bool dust = c->dust && c->dust_opac;
+    if (dust && !Tdust && !c->tdust_dev)
+        return fail(c, MC_ESTATE, "no device Tdust yet: pass Tdust or run mc_update_cells first");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (host_state) {
+        const int nE = c->bal.A.nE;
+        std::vector<float> st(nc1 * ldst, 0.f);
+        const std::vector<int> &sl_e = c->op_sl_e, &sl_st = c->op_sl_st;
+        for (int cell = 1; cell <= nc; ++cell) {
+            if (!(Te[cell] > 0.f) || !(Hden[cell] >= 0.f) || !std::isfinite(Ne[cell]))
+                return fail(c, MC_EINVAL, "cell %d: Te must be > 0, Hden >= 0, Ne finite", cell);
+        }
+        for (size_t cell = 0; cell < nc1; ++cell) {
+            float *r = st.data() + cell * ldst;
+            r[0] = Te[cell]; r[1] = Ne[cell]; r[2] = Hden[cell];
+            for (int k = 0; k < nslot; ++k) {
+                const size_t xr = (size_t)c->bal.eidx_xref[sl_e[k]] - 1;
+                r[3 + k] = ionDen[cell + nc1 * (xr + (size_t)nE * (sl_st[k] - 1))];
+            }
+        }
+        HIPCHK(c, hipMemcpyAsync(c->d_op_st, st.data(), st.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    if (dust) {
+        const mcg::DustArgs &da = c->dust_args;
+        const size_t ntd = nc1 * (da.nsMax + 1) * (da.nSizes + 1);
+        if (!c->d_op_Td) HIPCHK(c, dalloc(&c->d_op_Td, ntd));
+        if (Tdust) HIPCHK(c, hipMemcpy(c->d_op_Td, Tdust, ntd * sizeof(float), hipMemcpyHostToDevice));
+        const int K = c->dust_rows;
+        o.ndrow = K; o.nSizes = da.nSizes; o.nsMax = da.nsMax;
+        o.dsca = c->d_op_dust;
+        o.dabs = c->d_op_dust + (size_t)K * nb;
+        o.dw = c->d_op_dust + 2 * (size_t)K * nb;
+        o.tsub = o.dw + K;
+        o.ndust = o.tsub + K;
+        o.Tdust = c->d_op_Td;
+        if (!c->d_sca3) HIPCHK(c, dalloc(&c->d_sca3, n3 * nb));
+        o.sca3 = c->d_sca3;
+    }
+    if (!c->d_opac3) HIPCHK(c, dalloc(&c->d_opac3, n3 * nb));
+    o.opac3 = c->d_opac3;
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    if (!host_state)
+        HIPCHK(c, mco_launch_state(c->d_Te, c->d_Ne, c->d_Hden, c->d_ion, c->bal.A.nE, c->bal.A.nstages + 1,
+                                   c->op_slot_e, c->op_slot_st, nslot, nc, c->d_op_st, ldst, c->stream));
+    if (dust && !Tdust) {
+        const mcg::DustArgs &da = c->dust_args;
+        HIPCHK(c, mco_launch_f32(c->d_Tdust, nc1 * (da.nsMax + 1) * (da.nSizes + 1), c->d_op_Td, c->stream));
+    }
+    const size_t lds = (size_t)(nslot + o.ndrow) * 64 * sizeof(float);
+    HIPCHK(c, mco_launch_opacity(o, lds, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    if (opacity || scaOpac) {
+        if (!c->d_op_out) HIPCHK(c, dalloc(&c->d_op_out, nc1 * nb));
+        if (opacity) {
+            HIPCHK(c, mco_launch_gather(c->d_opac3, n3, c->d_cell_xyz, nc, nb, c->d_op_out, c->stream));
+            HIPCHK(c, hipMemcpyAsync(opacity, c->d_op_out, nc1 * nb * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
+        if (scaOpac) {
+            if (dust) HIPCHK(c, mco_launch_gather(c->d_sca3, n3, c->d_cell_xyz, nc, nb, c->d_op_out, c->stream));
+            else HIPCHK(c, hipMemsetAsync(c->d_op_out, 0, nc1 * nb * sizeof(float), c->stream));
+            HIPCHK(c, hipMemcpyAsync(scaOpac, c->d_op_out, nc1 * nb * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        }
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (kernel_ms) {
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        *kernel_ms = ms;
+    }
+    c->opac_built = true;
+    c->opac_sca_built = dust;
+    return MC_OK;
 }
 
 // ------------------------------------------------------- warm-start network
@@ -1277,7 +1513,8 @@ extern "C" void mc_ctx_destroy(mc_ctx *c)
                      c->d_cell_xyz, c->d_atomic, c->d_bal_tab, c->d_Hden, c->d_Te, c->d_Ne,
                      c->d_ion, c->d_flags, c->d_nIter, c->d_cold, c->d_mlp, c->d_mlpW, c->d_mX,
                      c->d_mY, c->d_mX0, c->d_hit, c->d_cross, c->d_Rint, c->d_Hint,
-                     c->d_dust_tab, c->d_Tdust, c->d_dust_mono, c->d_pe_tab, c->d_grains };
+                     c->d_dust_tab, c->d_Tdust, c->d_dust_mono, c->d_pe_tab, c->d_grains, c->d_op_dust,
+                     c->d_op_i, c->d_op_f, c->d_op_st, c->d_op_Td, c->d_op_out };
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

@@ -74,9 +74,15 @@ module mocassin_amd_mod
     type, bind(C) :: mc_dust_tables
         integer(c_int32_t) :: nSpecies, nSpeciesMax, nSizes, nTemps, photoelectric
         type(c_ptr) :: absXsec, grainWeight, grainAbun, emIntegral
-        type(c_ptr) :: grainRadius, grainVn, MsurfAtom, aWeight, Ndust
+        type(c_ptr) :: grainRadius, grainVn, MsurfAtom, aWeight, Ndust, scaXsec, TdustSublime
         real(c_float) :: XHILimit
     end type mc_dust_tables
+
+    type, bind(C) :: mc_opacity_tables
+        integer(c_int32_t) :: nseg
+        type(c_ptr) :: seg_meta, seg_sigma
+        real(c_float) :: brems1
+    end type mc_opacity_tables
 
     interface
         integer(c_int) function mc_ctx_create(desc, ctx) bind(C, name="mc_ctx_create")
@@ -153,6 +159,18 @@ module mocassin_amd_mod
             real(c_double), value :: jscale
             type(c_ptr), value :: opts, Hden, Te, Ne, ionDen, Tdust, flags, nIter, kernel_ms
         end function mc_update_cells
+
+        integer(c_int) function mc_opacity_setup(ctx, tables) bind(C, name="mc_opacity_setup")
+            import :: c_int, c_ptr, mc_opacity_tables
+            type(c_ptr), value :: ctx
+            type(mc_opacity_tables), intent(in) :: tables
+        end function mc_opacity_setup
+
+        integer(c_int) function mc_opacity_build(ctx, Hden, Te, Ne, ionDen, Tdust, opacity, scaOpac, &
+                                                 kernel_ms) bind(C, name="mc_opacity_build")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, Hden, Te, Ne, ionDen, Tdust, opacity, scaOpac, kernel_ms
+        end function mc_opacity_build
 
         integer(c_int) function mc_mlp_set(ctx, params, scale) bind(C, name="mc_mlp_set")
             import :: c_int, c_ptr

@@ -188,12 +188,13 @@ class Transport:
     # ---- grains
     def dust_setup(self, absXsec, grainWeight, grainAbun, emIntegral, nTemps, nSpeciesMax=None,
                    XHILimit=0.01, photoelectric=False, grainRadius=None, grainVn=None, MsurfAtom=None,
-                   aWeight=None, Ndust=None):
+                   aWeight=None, Ndust=None, scaXsec=None, TdustSublime=None):
         """Upload the tables of getDustT for one dust chemistry.  absXsec
         (nSpecies, nSizes, nbins); emIntegral (nSpecies, nSizes, nTemps).
         photoelectric: the grain charge and photoelectric terms, which need
         grainRadius (nSizes), grainVn, MsurfAtom (nSpecies), aWeight (30)
-        and Ndust (nCells+1)."""
+        and Ndust (nCells+1).  scaXsec (nSpecies, nSizes, nbins) with
+        TdustSublime (nSpecies) and Ndust: the dust add of opacity_build."""
         keep = [np.ascontiguousarray(absXsec, np.float32), np.ascontiguousarray(grainWeight, np.float32),
                 np.ascontiguousarray(grainAbun, np.float32),
                 np.asfortranarray(np.asarray(emIntegral, np.float32))]
@@ -202,7 +203,12 @@ class Transport:
             pe = [np.ascontiguousarray(v, np.float32) for v in (grainRadius, grainVn, MsurfAtom, aWeight, Ndust)]
             if pe[3].size != 30 or pe[4].size != self.nCells + 1:
                 raise ValueError("aWeight needs 30 entries and Ndust nCells+1")
-        self._dust_keep = keep + pe
+        op = [None, None]
+        if scaXsec is not None:
+            op = [np.ascontiguousarray(scaXsec, np.float32), np.ascontiguousarray(TdustSublime, np.float32)]
+            if pe[4] is None:
+                pe[4] = np.ascontiguousarray(Ndust, np.float32)
+        self._dust_keep = keep + pe + op
         nS, nSz, nb = keep[0].shape
         if nb != self.nbins or keep[3].shape != (nS, nSz, int(nTemps)):
             raise ValueError("dust tables do not match nbins / (nSpecies, nSizes, nTemps)")
@@ -213,6 +219,7 @@ class Transport:
         D.absXsec, D.grainWeight, D.grainAbun, D.emIntegral = (k.ctypes.data for k in keep)
         D.grainRadius, D.grainVn, D.MsurfAtom, D.aWeight, D.Ndust = (None if v is None else v.ctypes.data
                                                                      for v in pe)
+        D.scaXsec, D.TdustSublime = (None if v is None else v.ctypes.data for v in op)
         D.XHILimit = float(XHILimit)
         self._check(self._lib.mc_dust_setup(self._h, C.byref(D)))
         self._dust_shape = (D.nSpeciesMax + 1, nSz + 1)
@@ -243,6 +250,42 @@ class Transport:
         if Hden is not None:
             out += (gas[1], gas[2], np.ascontiguousarray(gas[3]))
         return out
+
+    # ---- opacity tables
+    def opacity_setup(self, seg_meta, seg_sigma, brems1):
+        """Cross-section segments in addOpacity's order: seg_meta [nseg][4]
+        (elem, ion, first bin, last bin), seg_sigma concatenated."""
+        m = np.ascontiguousarray(seg_meta, np.int32).reshape(-1, 4)
+        sg = np.ascontiguousarray(seg_sigma, np.float32)
+        T = _capi.McOpacityTables(m.shape[0], m.ctypes.data, sg.ctypes.data, float(brems1))
+        self._op_keep = (m, sg)
+        self._check(self._lib.mc_opacity_setup(self._h, C.byref(T)))
+
+    def opacity_build(self, Hden=None, Te=None, Ne=None, ionDen=None, Tdust=None, host=True):
+        """Opacity tables on the device (they become the transport's).  The
+        state as the host holds it (FP32), or None for the device state of
+        the last update.  Returns (opacity, scaOpac, kernel_ms); the tables
+        (nCells+1, nbins) are None when host is False."""
+        nc1 = self.nCells + 1
+        st = [None] * 4
+        if Te is not None:
+            st = [np.ascontiguousarray(Hden, np.float32).reshape(nc1), np.ascontiguousarray(Te, np.float32).reshape(nc1),
+                  np.ascontiguousarray(Ne, np.float32).reshape(nc1),
+                  np.asfortranarray(np.asarray(ionDen, np.float32).reshape(nc1, self._nE, self._ns))]
+        td = None if Tdust is None else np.ascontiguousarray(Tdust, np.float32).ravel()
+        if td is not None and td.size != nc1 * int(np.prod(getattr(self, "_dust_shape", (0,)))):
+            raise ValueError("Tdust needs (nCells+1) x (nSizes+1) x (nSpeciesMax+1) entries (dust_setup first)")
+        op = sca = None
+        if host:
+            op = np.zeros((self.nbins, nc1), np.float32)
+            sca = np.zeros((self.nbins, nc1), np.float32)
+        ms = C.c_double()
+        ptr = lambda a: None if a is None else a.ctypes.data
+        self._check(self._lib.mc_opacity_build(self._h, *(ptr(a) for a in st), ptr(td), ptr(op), ptr(sca),
+                                               C.byref(ms)))
+        if host:
+            op, sca = op.T.copy(), sca.T.copy()
+        return op, sca, ms.value
 
     # ---- warm-start network
     def set_mlp(self, params, scale):
