@@ -77,6 +77,33 @@ def cpu_baseline(p, seed, budget_s=15.0):
                       % (cores * per_thread, cores, per_thread)}
 
 
+def opacity_cpu_baseline(atom, bundle, nb, nu, Hden, Te, Ne, ion, cells=3000):
+    """ionizationDriver/addOpacity in the reference's FP32 (the C
+    restatement, oracle/mc_opacity.c) on the first `cells` cells of the same
+    state, one thread: cells per second."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import mc_oracle as O
+    import numpy as np
+    n = min(cells, Te.size - 1)
+    sl = slice(0, n + 1)
+    m4 = np.asarray(atom["op_seg"], np.int32)
+    m6 = np.zeros((m4.shape[0], 6), np.int32)
+    m6[:, 0], m6[:, 1], m6[:, 3], m6[:, 4] = m4[:, 0], m4[:, 1], m4[:, 2], m4[:, 3]
+    lens = m4[:, 3] - m4[:, 2] + 1
+    g = dict(ints=np.array([0, 0, 0, n, nb, 0, 0, bundle.nElemUsed * 100 + bundle.nstages]), nuArray=nu,
+             Hden=np.asarray(Hden[sl], np.float32), Te=np.asarray(Te[sl], np.float32), Te0=np.asarray(Te[sl], np.float32),
+             Ne=np.asarray(Ne[sl], np.float32),
+             ionDen=np.asfortranarray(np.asarray(ion[sl], np.float32)).ravel(order="F"),
+             elem_on=np.asarray(bundle.elem_on, np.int32), xref=np.asarray(bundle.xref, np.int32),
+             abun=np.asarray(bundle.abun, np.float32), seg_meta=m6, seg_sigma=atom["op_sigma"],
+             seg_off=np.concatenate([[0], np.cumsum(lens)]).astype(np.int64), brems1=atom["op_brems1"])
+    t = time.perf_counter()
+    O.opacity_tables(g, gaunt_cache=True)
+    dt = time.perf_counter() - t
+    return {"value": n / dt, "unit": "cells/s", "cores": 1, "kind": "port",
+            "sample": "%d cells of the workload's state after the balance, scalar C restatement" % n}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +115,7 @@ def main():
     ap.add_argument("--no-te", action="store_true", help="skip the Te-convergence measurement")
     ap.add_argument("--no-balance", action="store_true", help="skip the balance measurement")
     ap.add_argument("--no-grains", action="store_true", help="skip the balance with grains")
+    ap.add_argument("--no-opacity", action="store_true", help="skip the opacity-table build")
     ap.add_argument("--autotune", action="store_true",
                     help="time the launch-shape candidates on this workload and store the best "
                          "in mocassin_amd/tuning_db.json (default: use the stored shape)")
@@ -181,6 +209,25 @@ def main():
                    "solver": "reference schedule (iterateT), FP64, cold start TeStart=1e4 K"}
         print("bench: balance %d cells in %.1f ms (%.2f T-iterations per cell)"
               % (hit.sum(), ms_b, balance["mean_iterations"]), file=sys.stderr, flush=True)
+        if not a.no_opacity and "op_seg" in atom:
+            # the opacity tables of the balanced state, built where it lies
+            T.opacity_setup(atom["op_seg"], atom["op_sigma"], float(atom["op_brems1"][0]))
+            ms_o = min(T.opacity_build(host=False)[2] for _ in range(3))
+            n3 = int(np.prod(np.shape(p["active"])))
+            nslot = sum(min(z, bundle.nstages) for z in range(1, 31) if bundle.elem_on[z - 1])
+            byts = 4.0 * n3 * p["nbins"] + 4.0 * n3 + 4.0 * (p["nCells"] + 1) * (3 + nslot)
+            opacity = {"device_ms": ms_o, "cells_per_s": p["nCells"] / (ms_o * 1e-3),
+                       "roofline": {"bound": "hbm", "achieved": byts / (ms_o * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                                    "unit": "GB/s", "frac": byts / (ms_o * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                    "algorithmic_bytes": byts, "kernel": "opacity_kernel"}}
+            if not a.no_cpu:
+                r2 = r
+                opacity["cpu_baseline"] = opacity_cpu_baseline(atom, bundle, p["nbins"], p["nuArray"], Hd, r2[0],
+                                                               r2[1], r2[2])
+                opacity["gpu_over_cpu"] = opacity["cells_per_s"] / opacity["cpu_baseline"]["value"]
+            balance["opacity"] = opacity
+            print("bench: opacity tables %.3f ms (%.2f TB/s of table writes)"
+                  % (ms_o, opacity["roofline"]["achieved"] / 1e3), file=sys.stderr, flush=True)
         if not a.no_grains:
             # the same balance with silicate grains, their charge and the
             # photoelectric and collision terms, then the grain temperatures

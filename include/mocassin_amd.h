@@ -152,7 +152,8 @@ int mc_ctx_create(const mc_desc *desc, mc_ctx **out);
 
 /* Upload the per-sweep tables (frozen during a transport sweep).
  * scaOpac/gSca may be NULL without dust; recPDF/totalLines NULL without gas.
- * opacity (and scaOpac) NULL keep the tables mc_opacity_build wrote.
+ * opacity (and scaOpac) NULL keep the device's current tables: those
+ * mc_opacity_build wrote, or an earlier upload's.
  * Replaces: the reads of grid%opacity, grid%scaOpac, grid%recPDF,
  * grid%totalLines, inSpectrumProbDen, gSca inside photon_mod.f90:768-1812;
  * the tables are produced by ionizationDriver (ionization_mod.f90:349-484),

@@ -78,8 +78,9 @@ struct mc_ctx {
     bool dust_opac = false;                      // grains carry the dust-add tables
     int dust_rows = 0;
     float *d_op_dust = nullptr;                  // dsca | dabs | dw | tsub | Ndust
-    int *d_op_i = nullptr;                       // slot_kind | bin_start | ent_slot | slot_e | slot_stage
-    float *d_op_f = nullptr;                     // slot_abun | slot_efac | ent_sigma
+    int *d_op_i = nullptr;                       // bin_start | slot_e | slot_stage
+    float *d_op_f = nullptr;                     // slot_abun | slot_efac
+    mco::Ent *d_op_ent = nullptr;                // cross-section entries of every bin
     float *d_op_st = nullptr, *d_op_Td = nullptr, *d_op_out = nullptr;
     mco::OpacArgs op{};
     int *op_slot_e = nullptr, *op_slot_st = nullptr;   // into d_op_i
@@ -250,10 +251,11 @@ extern "C" int mc_upload_tables(mc_ctx *c, const float *opacity, const float *sc
     const size_t nt = (size_t)(nc + 1) * nb;
     const bool dust = (G.flags & MC_DUST) != 0, gas = (G.flags & MC_GAS) != 0;
     if (!specCDF) return fail(c, MC_EINVAL, "specCDF is required");
-    if (!opacity && !c->opac_built) return fail(c, MC_EINVAL, "opacity is required (mc_opacity_build has not run)");
+    if (!opacity && !c->opac_built)
+        return fail(c, MC_EINVAL, "opacity is required (no tables on the device yet)");
     if (gas && (!recPDF || !totalLines)) return fail(c, MC_EINVAL, "gas needs recPDF and totalLines");
-    if (dust && !scaOpac && !(c->opac_built && c->opac_sca_built && !opacity))
-        return fail(c, MC_EINVAL, "dust needs scaOpac");
+    if (dust && !scaOpac && !c->opac_sca_built)
+        return fail(c, MC_EINVAL, "dust needs scaOpac (no scattering table on the device yet)");
     if (dust && !(G.flags & MC_ISOTROPIC) && !gSca) return fail(c, MC_EINVAL, "HG scattering needs gSca");
     HIPCHK(c, hipSetDevice(c->device));
     if (opacity)
@@ -287,13 +289,14 @@ extern "C" int mc_upload_tables(mc_ctx *c, const float *opacity, const float *sc
         e = hipMemcpy(tmp, opacity, nt * sizeof(float), hipMemcpyHostToDevice);
         if (e == hipSuccess && !c->d_opac3) e = dalloc(&c->d_opac3, ncell3 * nb);
         if (e == hipSuccess) e = mcd_densify(tmp, c->d_active, (int)ncell3, nc + 1, nb, c->d_opac3, c->stream);
-        c->opac_built = false;
+        c->opac_built = e == hipSuccess;
     }
     if (e == hipSuccess && dust && scaOpac) {
         e = hipStreamSynchronize(c->stream);
         if (e == hipSuccess) e = hipMemcpy(tmp, scaOpac, nt * sizeof(float), hipMemcpyHostToDevice);
         if (e == hipSuccess && !c->d_sca3) e = dalloc(&c->d_sca3, ncell3 * nb);
         if (e == hipSuccess) e = mcd_densify(tmp, c->d_active, (int)ncell3, nc + 1, nb, c->d_sca3, c->stream);
+        c->opac_sca_built = e == hipSuccess;
     }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(tmp);
@@ -878,32 +881,32 @@ extern "C" int mc_opacity_setup(mc_ctx *c, const mc_opacity_tables *t)
         }
         off += (size_t)(hi - lo + 1);
     }
-    std::vector<int> bstart(nb + 1, 0), eslot;
-    std::vector<float> esig;
+    std::vector<int> bstart(nb + 1, 0);
+    std::vector<mco::Ent> ent;
     for (int i = 0; i < nb; ++i) {
         bstart[i + 1] = bstart[i] + (int)bins[i].size();
-        for (auto &p : bins[i]) { eslot.push_back(p.first); esig.push_back(p.second); }
+        for (auto &p : bins[i]) ent.push_back(mco::Ent{ p.first * 2 + kind[p.first], p.second });
     }
-    const size_t nnz = eslot.size();
     std::vector<int> iv;
-    iv.insert(iv.end(), kind.begin(), kind.end());
     iv.insert(iv.end(), bstart.begin(), bstart.end());
-    iv.insert(iv.end(), eslot.begin(), eslot.end());
     iv.insert(iv.end(), sl_e.begin(), sl_e.end());
     iv.insert(iv.end(), sl_st.begin(), sl_st.end());
     std::vector<float> fv;
     fv.insert(fv.end(), sabun.begin(), sabun.end());
     fv.insert(fv.end(), efac.begin(), efac.end());
-    fv.insert(fv.end(), esig.begin(), esig.end());
     HIPCHK(c, hipSetDevice(c->device));
     c->opac = false;
     if (c->d_op_i) { (void)hipFree(c->d_op_i); c->d_op_i = nullptr; }
     if (c->d_op_f) { (void)hipFree(c->d_op_f); c->d_op_f = nullptr; }
+    if (c->d_op_ent) { (void)hipFree(c->d_op_ent); c->d_op_ent = nullptr; }
     if (c->d_op_st) { (void)hipFree(c->d_op_st); c->d_op_st = nullptr; }
     HIPCHK(c, dalloc(&c->d_op_i, iv.size()));
     HIPCHK(c, dalloc(&c->d_op_f, fv.size()));
+    HIPCHK(c, dalloc(&c->d_op_ent, ent.size()));
     HIPCHK(c, hipMemcpy(c->d_op_i, iv.data(), iv.size() * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_op_f, fv.data(), fv.size() * sizeof(float), hipMemcpyHostToDevice));
+    if (!ent.empty())
+        HIPCHK(c, hipMemcpy(c->d_op_ent, ent.data(), ent.size() * sizeof(mco::Ent), hipMemcpyHostToDevice));
     const size_t nc1 = (size_t)c->G.nCells + 1;
     const int ldst = 3 + nslot;
     HIPCHK(c, dalloc(&c->d_op_st, nc1 * ldst));
@@ -912,17 +915,15 @@ extern "C" int mc_opacity_setup(mc_ctx *c, const mc_opacity_tables *t)
     o.ncell3 = c->G.ncell3; o.nb = nb; o.nc = c->G.nCells;
     o.active = c->d_active;
     o.st = c->d_op_st; o.ldst = ldst; o.nslot = nslot;
-    o.slot_kind = c->d_op_i;
-    o.bin_start = c->d_op_i + nslot;
-    o.ent_slot = o.bin_start + nb + 1;
+    o.bin_start = c->d_op_i;
+    o.ent = c->d_op_ent;
     o.slot_abun = c->d_op_f;
     o.slot_efac = c->d_op_f + nslot;
-    o.ent_sigma = c->d_op_f + 2 * nslot;
     o.slot_he2 = he2;
     o.brems1 = t->brems1;
     o.nu1 = c->nuArray[0];
     o.xlf1 = log10f(c->nuArray[0]);
-    c->op_slot_e = c->d_op_i + nslot + (nb + 1) + nnz;
+    c->op_slot_e = c->d_op_i + (nb + 1);
     c->op_slot_st = c->op_slot_e + nslot;
     c->op_sl_e = sl_e;
     c->op_sl_st = sl_st;
@@ -1514,7 +1515,7 @@ extern "C" void mc_ctx_destroy(mc_ctx *c)
                      c->d_ion, c->d_flags, c->d_nIter, c->d_cold, c->d_mlp, c->d_mlpW, c->d_mX,
                      c->d_mY, c->d_mX0, c->d_hit, c->d_cross, c->d_Rint, c->d_Hint,
                      c->d_dust_tab, c->d_Tdust, c->d_dust_mono, c->d_pe_tab, c->d_grains, c->d_op_dust,
-                     c->d_op_i, c->d_op_f, c->d_op_st, c->d_op_Td, c->d_op_out };
+                     c->d_op_i, c->d_op_f, c->d_op_st, c->d_op_Td, c->d_op_out, c->d_op_ent };
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

@@ -9,6 +9,14 @@ namespace mco {
 
 constexpr int kMaxSlots = 160;      // (element, ion) densities per cell
 constexpr int kMaxDustRows = 32;    // grain (species, size) rows
+constexpr int kSplit = 4;           // wavefronts per tile of 64 lattice points
+
+// one cross-section entry of a bin: qk = slot * 2 + (1 for a heavy ion,
+// added only when its density is > 0)
+struct Ent {
+    int qk;
+    float sigma;
+};
 
 struct OpacArgs {
     int ncell3, nb, nc;
@@ -18,14 +26,13 @@ struct OpacArgs {
     const float *st;
     int ldst, nslot;
     const float *slot_abun;          // [nslot] elemental abundance of the slot's element
-    const int *slot_kind;            // [nslot]: 0 H or He, 1 heavy (added only when its density > 0)
     const float *slot_efac;          // [nslot]: (stage-1)^2 for heavy ions above neutral, else 0
                                      // (eDenSum), in eDenSum's order
     int slot_he2;                    // slot of He+ (density(2,2) of fac1), -1 if absent
     // bins: entries [bin_start[i], bin_start[i+1]) of bin i (0-based), in
     // addOpacity's order
-    const int *bin_start, *ent_slot;
-    const float *ent_sigma;
+    const int *bin_start;
+    const Ent *ent;
     float brems1, nu1, xlf1;         // free-free factor, nu(1), log10 nu(1)
     // grains (nullable): rows k = (species-1)*nSizes + size-1
     int ndrow, nSizes, nsMax;

@@ -103,7 +103,10 @@ __global__ void __launch_bounds__(64) opacity_kernel(OpacArgs a)
 {
     extern __shared__ float lds[];               // [nslot][64] densities, then [ndrow][64] weights
     const int lane = threadIdx.x;
-    const int xyz = blockIdx.x * 64 + lane;
+    // kSplit wavefronts share a tile of 64 lattice points, each writing one
+    // contiguous range of bins (more wavefronts in flight to hide latency)
+    const int tile = blockIdx.x / kSplit, part = blockIdx.x % kSplit;
+    const int xyz = tile * 64 + lane;
     const bool in = xyz < a.ncell3;
     const int cell = in ? a.active[xyz] : 0;
     float *dens = lds, *wr = lds + a.nslot * 64;
@@ -118,20 +121,22 @@ __global__ void __launch_bounds__(64) opacity_kernel(OpacArgs a)
             if (a.slot_efac[q] != 0.f) eSum = eSum + a.slot_efac[q] * d;
             if (q == a.slot_he2) d22 = d;
         }
-        const float log10Te = log10f(T);
-        const float sqrTe = sqrtf(T);
-        const float expo = (kTe1Ryd / T) + a.nu1;
-        const float cb1 = expo > 84.f ? 0.f : expf(-expo);
-        const float g1 = gaunt_ff(0.f, log10Te, a.xlf1);
-        // (density(1,2) + density(2,2) + 4 density(2,3) + eDenFFSum): the
-        // H+ and He++ densities are never set in ionizationDriver's loop
-        float t = 0.f + d22;
-        t = t + 4.f * 0.f;
-        t = t + eSum;
-        const float fac1 = (N / 1.e10f) * (t / 1.e10f) / sqrTe;
-        const float fac2 = fac1 * kTe1Ryd / T;
-        if (cb1 < 0.995f) ff1 = (a.brems1 * (1.f - cb1) * g1) * fac1;
-        else ff1 = (a.brems1 * a.nu1 * g1) * fac2;
+        if (part == 0) {                          // bin 1's free-free term
+            const float log10Te = log10f(T);
+            const float sqrTe = sqrtf(T);
+            const float expo = (kTe1Ryd / T) + a.nu1;
+            const float cb1 = expo > 84.f ? 0.f : expf(-expo);
+            const float g1 = gaunt_ff(0.f, log10Te, a.xlf1);
+            // (density(1,2) + density(2,2) + 4 density(2,3) + eDenFFSum): the
+            // H+ and He++ densities are never set in ionizationDriver's loop
+            float t = 0.f + d22;
+            t = t + 4.f * 0.f;
+            t = t + eSum;
+            const float fac1 = (N / 1.e10f) * (t / 1.e10f) / sqrTe;
+            const float fac2 = fac1 * kTe1Ryd / T;
+            if (cb1 < 0.995f) ff1 = (a.brems1 * (1.f - cb1) * g1) * fac1;
+            else ff1 = (a.brems1 * a.nu1 * g1) * fac2;
+        }
         if (a.ndrow) {
             const float *Td = a.Tdust + (size_t)cell * (a.nsMax + 1) * (a.nSizes + 1);
             for (int r = 0; r < a.ndrow; ++r) {
@@ -143,16 +148,33 @@ __global__ void __launch_bounds__(64) opacity_kernel(OpacArgs a)
     }
     if (!in) return;
     const size_t n3 = (size_t)a.ncell3;
-    for (int i = 0; i < a.nb; ++i) {
+    const int per = (a.nb + kSplit - 1) / kSplit;
+    const int i0 = part * per, i1 = min(a.nb, i0 + per);
+    const Ent *ent = a.ent;
+    for (int i = i0; i < i1; ++i) {
         float acc = 0.f, sca = 0.f;
         if (cell > 0) {
             acc = acc + (i == 0 ? ff1 : 0.f);
             const int e1 = a.bin_start[i + 1];
-            for (int e = a.bin_start[i]; e < e1; ++e) {
-                const int q = a.ent_slot[e];
-                const float d = dens[q * 64 + lane];
-                if (a.slot_kind[q] && !(d > 0.f)) continue;
-                acc = acc + a.ent_sigma[e] * d;
+            int e = a.bin_start[i];
+            // four entries' (slot, kind, sigma) fetched ahead of the
+            // dependent FP32 sum, which stays in the reference's order
+            for (; e + 4 <= e1; e += 4) {
+                Ent en[4];
+                float d[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) en[k] = ent[e + k];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) d[k] = dens[(en[k].qk >> 1) * 64 + lane];
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (!((en[k].qk & 1) && !(d[k] > 0.f))) acc = acc + en[k].sigma * d[k];
+            }
+            for (; e < e1; ++e) {
+                const Ent en = ent[e];
+                const float d = dens[(en.qk >> 1) * 64 + lane];
+                if ((en.qk & 1) && !(d > 0.f)) continue;
+                acc = acc + en.sigma * d;
             }
             if (a.ndrow) {
                 float ab = 0.f;
@@ -194,7 +216,7 @@ __global__ void f32_kernel(const double *in, size_t n, float *out)
 
 hipError_t mco_launch_opacity(const mco::OpacArgs &a, size_t lds_bytes, hipStream_t s)
 {
-    hipLaunchKernelGGL(mco::opacity_kernel, dim3((a.ncell3 + 63) / 64), dim3(64), lds_bytes, s, a);
+    hipLaunchKernelGGL(mco::opacity_kernel, dim3((a.ncell3 + 63) / 64 * mco::kSplit), dim3(64), lds_bytes, s, a);
     return hipGetLastError();
 }
 

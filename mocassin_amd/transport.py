@@ -74,7 +74,7 @@ class Transport:
 
     def upload_tables(self, opacity, specCDF, recPDF=None, totalLines=None,
                       scaOpac=None, gSca=None):
-        arrs = [_f32(opacity), None if scaOpac is None else _f32(scaOpac),
+        arrs = [None if opacity is None else _f32(opacity), None if scaOpac is None else _f32(scaOpac),
                 None if recPDF is None else _f32(recPDF),
                 None if totalLines is None else _f32(totalLines), _f32(specCDF),
                 None if gSca is None else _f32(gSca)]

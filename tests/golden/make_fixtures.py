@@ -205,13 +205,21 @@ def make_atomic(deck, name):
     initial ionisation fractions of one cell) as the cold start."""
     sys.path.insert(0, ROOT)
     from mocassin_amd.atomic import AtomicBundle
-    d = run_harness(deck, "balance", [5, 2000], tag="atom_" + name)
+    d = run_harness(deck, "opacity", [5, 2000], tag="atom_" + name)
     prob, (nx, ny, nz, nc, nb) = read_problem(d)
     b = AtomicBundle.from_harness_dump(d, "/root/reference/data/radrec.dat")
     rd = lambda n, dt=np.float32: np.fromfile(os.path.join(d, n + ".bin"), dt)
     ion0 = rd("ionDen").reshape(b.nstages, b.nElemUsed, nc + 1)[:, :, 1].T.copy()
+    # the opacity segments of this mesh and composition (addOpacity's order)
+    seg = rd("op_seg", np.int32).reshape(-1, 6)
+    xs = rd("op_xsec")
+    bp = int(rd("op_ints", np.int32)[0])
+    last = np.maximum(seg[:, 3], np.minimum(seg[:, 4], nb))
+    sig = np.concatenate([xs[xp - 1:xp - 1 + l - lo + 1] for (_, _, _, lo, _, xp), l in zip(seg, last)])
     out = os.path.join(HERE, "atomic_%s.npz" % name)
-    np.savez_compressed(out, **b.to_npz_dict(), Te0=rd("Te")[1:2], ion0=ion0)
+    np.savez_compressed(out, **b.to_npz_dict(), Te0=rd("Te")[1:2], ion0=ion0,
+                        op_seg=np.stack([seg[:, 0], seg[:, 1], seg[:, 3], last], axis=1).astype(np.int32),
+                        op_sigma=sig, op_brems1=xs[bp - 1:bp])
     print("wrote", out, "nbins", nb, "channels", b.nch, "CEL ions", b.ncel)
 
 

@@ -98,3 +98,32 @@ def test_device_state_after_the_balance():
         T.close()
     assert np.array_equal(op_dev, op_host)
     assert np.all(op_dev[1:, 1:].sum(axis=1) > 0)
+
+
+def test_transport_runs_on_the_device_tables():
+    """A sweep on the tables mc_opacity_build wrote (opacity NULL in
+    mc_upload_tables) equals a sweep on the same tables passed through host
+    memory: same counters, J to the FP64 atomics' order."""
+    print("transport on device-built opacity", flush=True)
+    T, g, b = _ctx("c2mini")
+    nc, nb = int(g["ints"][3]), int(g["ints"][4])
+    spec = np.linspace(0.0, 1.0, nb, dtype=np.float32)
+    rec = np.tile(spec, (nc + 1, 1)).astype(np.float32)
+    rec[0] = 0.0
+    tl = np.zeros(nc + 1, np.float32)
+    try:
+        op, _, _ = T.opacity_build(*_state(g, b))
+        out = []
+        for host in (False, True):
+            T.upload_tables(op if host else None, spec, rec, tl)
+            T.zero_estimators()
+            c = T.energy_packet_driver(1, 200000, 0)
+            J, E = T.fetch()
+            out.append((c, J))
+    finally:
+        T.close()
+    (c1, J1), (c2, J2) = out
+    for k in ("n_seg", "n_abs", "n_esc", "n_flights"):
+        assert c1[k] == c2[k], k
+    assert c1["n_seg"] > 1e5
+    assert np.allclose(J1, J2, rtol=1e-12, atol=0)
