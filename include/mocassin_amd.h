@@ -253,13 +253,19 @@ typedef struct {
                                    that ends it (the reference uses 0.01) */
     int32_t warm_start;         /* 1: reached cells start from the network's prediction
                                    (mc_mlp_set first); a cell whose secant solve from there
-                                   fails, or has not converged after 8 iterations, restarts
-                                   from the state passed in (flags bit5) */
+                                   fails, or has not converged after warm_iter iterations,
+                                   restarts from the state passed in (flags bit5) */
     double first_step;          /* secant: relative size of the first trial step when the
-                                   start gives no bracket (0: 0.05); a good warm start
-                                   wants a step of the order of its error */
+                                   start gives no bracket (0: 0.05, or 0.01 with
+                                   warm_start: a good warm start wants a step of the
+                                   order of its error) */
     double first_slope;         /* secant: that step is the relative residual divided by
                                    this expected |d((heat-cool)/heat)/d ln Te| (0: 1) */
+    int32_t warm_iter;          /* warm_start: temperature iterations of the attempt from the
+                                   prediction before the restart (0: max_iter).  A small cap
+                                   keeps one slow cell from holding the launch's tail where
+                                   predictions are good; where cells need many iterations
+                                   it wastes the attempt */
 } mc_balance_opts;
 
 /* Ionisation and thermal balance of every active cell on the device (FP64),
@@ -378,6 +384,28 @@ int mc_opacity_setup(mc_ctx *ctx, const mc_opacity_tables *tables);
  * 26-484). */
 int mc_opacity_build(mc_ctx *ctx, const float *Hden, const float *Te, const float *Ne, const float *ionDen,
                      const float *Tdust, float *opacity, float *scaOpac, double *kernel_ms);
+
+/* The analytic line luminosities of outputGas for a 3-D grid (output_mod.f90:
+ * 302-520), summed over every active cell that is not black: per cell,
+ * forLines (1946-2008: each model ion's level populations from equilibrium,
+ * emissivity a(j,i) (e(j) - e(i)) n(j) times elemAbun, ionDen and the REAL
+ * 1.9865e9) and RecLinesEmission's Hbeta (2010-2066: 2530/Te**0.833 Ne
+ * ionDen(H,2) elemAbun(H)), each times Hden and getVolume (grid_mod.f90:
+ * 2876-2964).  The state is Hden, Te, Ne (0:nCells) and ionDen(0:nCells,
+ * 1:nElemUsed, 1:nstages) as the host holds them, with lgBlack (0:nCells,
+ * nullable: no black cell); all four NULL take the device state of the last
+ * mc_update_cells / mc_balance_run, rounded to FP32, whose cells without a Te
+ * or ion balance are black.  Out: forbVol[nCel][17][17] (model ion in
+ * forLines order, upper level, lower level; Fortran forbVol(lower, upper,
+ * ion)) and HbetaVol, both before outputGas divides by HbetaVol (line ratios
+ * are forbVol / HbetaVol, output_mod.f90:864).  Summed in FP64 in a fixed
+ * order (the reference sums REALs).  The level populations are solved in
+ * FP64 (the reference evaluates the recombination and collision-strength
+ * fits in REAL).  kernel_ms (nullable): device time.  Replaces: the analytic
+ * emissivity loop of outputGas (output_mod.f90:302-520, the forbVol and
+ * HbetaVol sums). */
+int mc_line_luminosities(mc_ctx *ctx, const float *Hden, const float *Te, const float *Ne, const float *ionDen,
+                         const int32_t *lgBlack, double *forbVol, double *HbetaVol, double *kernel_ms);
 
 /* ---- warm-start network (no counterpart in the reference) -------------
  * A 72 -> 128 -> 128 -> 128 -> 31 ReLU perceptron with BF16 weights and

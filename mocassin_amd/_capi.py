@@ -22,6 +22,7 @@ EXPORTS = ("mc_ctx_create", "mc_upload_tables", "mc_zero_estimators", "mc_transp
            "mc_mlp_set", "mc_mlp_get", "mc_mlp_features", "mc_mlp_predict", "mc_mlp_loss_grad",
            "mc_mlp_train", "mc_count_crossings", "mc_fetch_crossings", "mc_transport_batch",
            "mc_dust_setup", "mc_update_cells", "mc_opacity_setup", "mc_opacity_build",
+           "mc_line_luminosities",
            "mc_ctx_last_error", "mc_ctx_destroy")
 
 MC_SOLVER_REFERENCE, MC_SOLVER_SECANT = 0, 1
@@ -79,7 +80,7 @@ class McBatchTuning(C.Structure):
 class McBalanceOpts(C.Structure):
     _fields_ = [("solver", C.c_int32), ("max_iter", C.c_int32), ("tol", C.c_double),
                 ("ion_tol", C.c_double), ("warm_start", C.c_int32), ("first_step", C.c_double),
-                ("first_slope", C.c_double)]
+                ("first_slope", C.c_double), ("warm_iter", C.c_int32)]
 
 
 class McDustTables(C.Structure):
@@ -137,6 +138,7 @@ def load():
     L.mc_update_cells.argtypes = [vp, C.c_double, C.POINTER(McBalanceOpts)] + [vp] * 8
     L.mc_opacity_setup.argtypes = [vp, C.POINTER(McOpacityTables)]
     L.mc_opacity_build.argtypes = [vp] * 9
+    L.mc_line_luminosities.argtypes = [vp] * 9
     L.mc_count_crossings.argtypes = [vp, i32]
     L.mc_fetch_crossings.argtypes = [vp, vp]
     L.mc_ctx_last_error.argtypes = [vp]

@@ -18,7 +18,8 @@ trained on the device on the reached cells of variant 1 (log10 Te weighted
 and the plain and warm pair again at the reference's own tolerance (1e-2).
 The secant's first step divides the relative residual by an expected slope
 d((heat - cool)/heat)/d ln T; each solver's value is chosen from SLOPE_GRID
-on the training variant.
+on the training variant, and so is the warm attempt's iteration cap before
+its restart from the cold state (WARM_ITER_GRID).
 
 "Converged" means at least 95 % of the cells with at least 1e3 packet
 crossings have |Te - Te_ref| / Te_ref <= 1 %.  Times are device times of
@@ -36,6 +37,7 @@ from .transport import Transport
 
 
 SLOPE_GRID = (1.0, 1.5, 2.0, 3.0, 4.0, 6.0)
+WARM_ITER_GRID = (6, 8, 12, 0)     # 0: the attempt from the prediction may use max_iter
 
 
 def cold_state(bundle, Te0, ion0, Hden):
@@ -128,6 +130,16 @@ def te_convergence(packets=1e8, train_steps=6000, batch=4096, lr=2e-3, te_weight
             log("first-step slope for %s: %s (device ms %s)"
                 % (name, tuned[name], ", ".join("%.2f@%g" % t for t in trials)))
         out["first_slope"] = dict(tuned, grid=list(SLOPE_GRID))
+        # the warm attempt's iteration cap before its restart, likewise
+        trials = []
+        for wi in WARM_ITER_GRID:
+            r = Ttr.update_cells(Hd_t, Te0_t, Ne0_t, ion0_t, jscale=v_train["jscale"], solver="secant", tol=1e-3,
+                                 max_iter=60, ion_tol=1e-3, first_slope=tuned["warm"], warm_start=True,
+                                 first_step=0.01, warm_iter=wi)
+            trials.append((r[5], wi))
+        warm_iter = min(trials)[1]
+        log("warm attempt cap: %s (device ms %s)" % (warm_iter, ", ".join("%.2f@%g" % t for t in trials)))
+        out["warm_iter"] = {"chosen": warm_iter, "grid": list(WARM_ITER_GRID)}
 
         T = v_eval["T"]
         Hd, Te0, Ne0, ion0 = v_eval["cold"]
@@ -144,10 +156,11 @@ def te_convergence(packets=1e8, train_steps=6000, batch=4096, lr=2e-3, te_weight
         kw = dict(jscale=js, solver="secant", max_iter=60, ion_tol=1e-3)
         sp, sw = tuned["plain"], tuned["warm"]
         for name, extra in (("plain", {"tol": 1e-3, "first_slope": sp}),
-                            ("warm", {"tol": 1e-3, "warm_start": True, "first_step": 0.01, "first_slope": sw}),
+                            ("warm", {"tol": 1e-3, "warm_start": True, "first_step": 0.01, "first_slope": sw,
+                                      "warm_iter": warm_iter}),
                             ("plain_tol_0.01", {"tol": 1e-2, "first_slope": sp}),
                             ("warm_tol_0.01", {"tol": 1e-2, "warm_start": True, "first_step": 0.01,
-                                               "first_slope": sw})):
+                                               "first_slope": sw, "warm_iter": warm_iter})):
             best = None
             for _ in range(3):
                 t = time.perf_counter()

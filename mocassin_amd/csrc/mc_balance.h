@@ -97,12 +97,32 @@ struct CellArgs {
     double tol, ion_tol;        // ion_tol: the ion fixed point's relative change (reference 0.01)
     double step0;               // secant: relative size of the first trial step
     double slope0;              // secant: expected |d((heat - cool)/heat)/d ln T| for that step
+    int warm_iter;              // secant: iterations of a warm-started attempt before its restart
     // state to restart from when the solve from Te/Ne/ion fails (nullable)
     const double *Te0, *Ne0, *ion0;
     // grains with the photoelectric effect (nullable: gas alone); device
     // pointer, with the frequency-bin count for the launch's LDS size
     const Grains *grains;
     int grains_nb;
+};
+
+// Line luminosities of a state (outputGas's analytic forbidden lines and
+// Hbeta, output_mod.f90:302-520).  Cells are split into blocks of
+// cells_per_blk consecutive cells, one wavefront each; block b writes its
+// sums to part[b * ldp ...]: per model ion k the 17 x 17 entries (upper - 1)
+// * 17 + lower - 1 at k * 289, then Hbeta at ncel * 289 (ldp >= that + 1).
+constexpr int kLineEnt = kMaxLev * kMaxLev;   // entries per model ion
+struct LineArgs {
+    int ncells, cells_per_blk;
+    const double *Te, *Ne, *Hden;   // [nCells+1], rounded to FP32 as the host grid stores them
+    const double *ion;              // [nCells+1][nE][nstages+1]
+    const int *black;               // nullable: cell c is skipped when black[c] & black_mask
+    int black_mask;
+    const int *cell_xyz;            // lattice point of each cell, (x-1) + nx*((y-1) + ny*(z-1))
+    int nx, ny;
+    const float *dfac;              // per-axis getVolume widths / 1e15, axes x, y, z concatenated
+    double *part;
+    int ldp;
 };
 
 // Host-built tables: `A` with its device pointers unset, and the flat arrays
@@ -131,3 +151,8 @@ hipError_t mcb_launch_integrals(const mcb::Atomic *dA, const mcb::CellArgs &a, c
 // Te, Ne, ion of reached cells from the network's outputs Y [nCells][ldy]
 hipError_t mcb_launch_warm(const mcb::Atomic *dA, const mcb::CellArgs &a, const float *Y, int ldy,
                            const int *hit, hipStream_t s);
+// line luminosities of every unblackened cell: out[ncel * 289 + 1] (model
+// ion, upper level, lower level; then Hbeta), summed over cells in a fixed
+// order; a.part holds nblk rows of a.ldp doubles
+hipError_t mcb_launch_lines(const mcb::Atomic *dA, const mcb::LineArgs &a, int nblk, int ncel, double *out,
+                            hipStream_t s);

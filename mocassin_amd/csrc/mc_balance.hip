@@ -28,10 +28,6 @@
 namespace mcb {
 
 constexpr int W = 64;
-// temperature iterations of a warm-started secant solve before it restarts
-// from the caller's state: a good start converges in two or three, and one
-// slow cell holds its wavefront for the whole launch's tail
-constexpr int kWarmIter = 8;
 
 struct Lds {
     double Te, Ne, Hden;
@@ -105,19 +101,19 @@ __device__ double splint(const double *xa, const double *ya, const double *y2, i
            ((a * a * a - a) * y2[klo - 1] + (b * b * b - b) * y2[khi - 1]) * (h * h) / 6.;
 }
 
-// CEL cooling of one model ion [erg/s per H], levels 1..10 (forLines +
-// equilibrium + the cooling sum of thermBalance)
-__device__ __forceinline__ double cel_cool(const Atomic &A, const CelIon &ci, Lds &S, int lane, double Te)
+// Level populations of one model ion (equilibrium, emission_mod.f90:
+// 1913-2114, with luSlv): S.y[1..nl] unnormalised, valid on every lane on
+// return.  up = ionDen(ion+1)/ionDen(ion), not NaN.  S: any LDS block with
+// the x, q, y and alev arrays of Lds.
+template <class LdsT>
+__device__ __forceinline__ void cel_levels(const Atomic &A, const CelIon &ci, LdsT &S, int lane, double Te,
+                                           double Ne, double up)
 {
     const int nl = ci.nl, nt = ci.nt;
-    const double Ne = S.Ne;
     const double *E = A.cel_e + ci.off_e;
     const double *Aa = A.cel_a + ci.off_a;          // a(i,j) at [(i-1) + nl*(j-1)]
     const double *lT = A.cel_t + ci.off_t;
     const double *gg = A.cel_g + ci.off_g;
-    const double xi = S.ion[ci.eidx][ci.ion];
-    const double up = (ci.ion < A.nstages) ? S.ion[ci.eidx][ci.ion + 1] / xi : 0.0;
-    if (up != up) return 0.0;
     const double sqrTe = sqrt(Te);
     const double log10Te = log10(Te);
     // level recombination coefficients (emission_mod.f90:1991-2002)
@@ -186,7 +182,6 @@ __device__ __forceinline__ double cel_cool(const Atomic &A, const CelIon &ci, Ld
         for (int j = i + 1 + lane; j <= nl; j += W) S.y[j] = S.y[j] - yi * S.x[j][i] / xii;
         __syncthreads();
     }
-    double cool = 0.0;
     if (lane == 0) {
         S.y[nl] = S.y[nl] / S.x[nl][nl];
         for (int i = 1; i <= nl - 1; ++i) {
@@ -194,6 +189,23 @@ __device__ __forceinline__ double cel_cool(const Atomic &A, const CelIon &ci, Ld
             for (int j = k + 1; j <= nl; ++j) S.y[k] = S.y[k] - S.y[j] * S.x[k][j];
             S.y[k] = S.y[k] / S.x[k][k];
         }
+    }
+    __syncthreads();
+}
+
+// CEL cooling of one model ion [erg/s per H], levels 1..10 (forLines +
+// equilibrium + the cooling sum of thermBalance)
+__device__ __forceinline__ double cel_cool(const Atomic &A, const CelIon &ci, Lds &S, int lane, double Te)
+{
+    const int nl = ci.nl;
+    const double *E = A.cel_e + ci.off_e;
+    const double *Aa = A.cel_a + ci.off_a;
+    const double xi = S.ion[ci.eidx][ci.ion];
+    const double up = (ci.ion < A.nstages) ? S.ion[ci.eidx][ci.ion + 1] / xi : 0.0;
+    if (up != up) return 0.0;
+    cel_levels(A, ci, S, lane, Te, S.Ne, up);
+    double cool = 0.0;
+    if (lane == 0) {
         double sumN = 0.0;
         for (int i = 1; i <= nl; ++i) sumN = sumN + S.y[i];
         const double s1 = A.el[ci.eidx].abun, s2 = xi;
@@ -622,9 +634,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) ba
                 const double f = res, T = Tn;
                 if (f > 0.) { Tlo = T; haveLo = true; } else { Thi = T; haveHi = true; }
                 const double bl = fmin(Tlo, Thi), bh = fmax(Tlo, Thi);
-                // a warm-started attempt gets kWarmIter iterations before
+                // a warm-started attempt gets warm_iter iterations before
                 // the restart from the caller's state
-                const int cap = (attempt == 0 && a.Te0) ? min(a.max_iter, kWarmIter) : a.max_iter;
+                const int cap = (attempt == 0 && a.Te0) ? min(a.max_iter, a.warm_iter) : a.max_iter;
                 if (nIterT >= cap) {
                     if (haveLo && haveHi) Tn = 0.5 * (bl + bh);
                     flags |= 2 | 4;
@@ -825,6 +837,121 @@ __global__ void warm_kernel(const Atomic *__restrict__ Ap, CellArgs a, const flo
     a.Ne[cell] = Ne * a.Hden[cell] > 0.0 ? Ne * a.Hden[cell] : 1.0;
 }
 
+// ---- analytic line luminosities (outputGas, output_mod.f90:302-520) -----
+// One wavefront per block of consecutive cells.  Per model ion (outer loop,
+// so that a lane's accumulators stay in registers): every cell's level
+// populations (cel_levels), then lanes over the 17 x 17 (upper, lower)
+// entries add forLines' emissivity times Hden * getVolume.  Each block writes
+// its sums to its own row of `part`; lines_reduce_kernel adds the rows in
+// block order, so the result does not depend on scheduling.
+struct LineLds {
+    double x[kMaxLev + 1][kMaxLev + 1];
+    double q[kMaxLev + 1][kMaxLev + 1];
+    double y[kMaxLev + 1];
+    double alev[kMaxLev + 1];
+};
+
+constexpr int kLineR = (kLineEnt + W - 1) / W;       // entries per lane
+
+__device__ __forceinline__ float line_volume(const LineArgs &a, int cell)
+{
+    // getVolume (grid_mod.f90:2876-2964) of the cell's lattice point: the
+    // per-axis widths over 1e15 multiplied in FP32 as (dx * dy) * dz
+    const int l = a.cell_xyz[cell];
+    const int ix = l % a.nx, iy = (l / a.nx) % a.ny, iz = l / (a.nx * a.ny);
+    return (a.dfac[ix] * a.dfac[a.nx + iy]) * a.dfac[a.nx + a.ny + iz];
+}
+
+__global__ void __launch_bounds__(64) lines_kernel(const Atomic *__restrict__ Ap, LineArgs a)
+{
+    __shared__ LineLds S;
+    const Atomic &A = *Ap;
+    const int lane = threadIdx.x;
+    const int nE = A.nE, ns = A.nstages;
+    const int c0 = 1 + blockIdx.x * a.cells_per_blk;
+    const int c1 = min(a.ncells, c0 + a.cells_per_blk - 1);
+    double *out = a.part + (size_t)blockIdx.x * a.ldp;
+    // forLines' 1.9865e9 is a REAL literal
+    const double kL = (double)1.9865e9f;
+    for (int k = 0; k < A.ncel; ++k) {
+        const CelIon &ci = A.cel[k];
+        const int nl = ci.nl;
+        const double *E = A.cel_e + ci.off_e;
+        const double *Aa = A.cel_a + ci.off_a;
+        const double abun = (double)(float)A.el[ci.eidx].abun;
+        double acc[kLineR];
+        for (int r = 0; r < kLineR; ++r) acc[r] = 0.0;
+        for (int cell = c0; cell <= c1; ++cell) {
+            if (a.black && (a.black[cell] & a.black_mask)) continue;
+            // the state as the host grid stores it (REAL)
+            const float te = (float)a.Te[cell], ne = (float)a.Ne[cell], hd = (float)a.Hden[cell];
+            const double *ion = a.ion + ((size_t)cell * nE + ci.eidx) * (ns + 1);
+            const float x = (float)ion[ci.ion];
+            const float up = (ci.ion < ns) ? (float)ion[ci.ion + 1] / x : 0.f;
+            if (up != up) continue;               // equilibrium's NaN test: no emission
+            const float dv = line_volume(a, cell);
+            cel_levels(A, ci, S, lane, (double)te, (double)ne, (double)up);
+            double sumN = 0.0;
+            for (int i = 1; i <= nl; ++i) sumN = sumN + S.y[i];
+            for (int r = 0; r < kLineR; ++r) {
+                const int e = lane + W * r;
+                const int hi = e / kMaxLev + 1, lo = e % kMaxLev + 1;
+                if (e < kLineEnt && lo < hi && hi <= nl) {
+                    const double av = Aa[(hi - 1) + nl * (lo - 1)];
+                    if (av != 0.0) {
+                        const double Eji = E[hi - 1] - E[lo - 1];
+                        double f = av * Eji * (S.y[hi] / sumN);
+                        f = f * abun * (double)x;
+                        f = f * kL;
+                        acc[r] = acc[r] + f * (double)hd * (double)dv;
+                    }
+                }
+            }
+            __syncthreads();                      // S is rewritten by the next cell
+        }
+        for (int r = 0; r < kLineR; ++r) {
+            const int e = lane + W * r;
+            if (e < kLineEnt) out[(size_t)k * kLineEnt + e] = acc[r];
+        }
+    }
+    // RecLinesEmission's Hbeta (output_mod.f90:2053-2058), HIRecLines(4,2) =
+    // (fh/hb) * Hbeta = Hbeta, times Hden * dV
+    double hb = 0.0;
+    const double abunH = (double)(float)A.el[0].abun;
+    for (int cell = c0 + lane; cell <= c1; cell += W) {
+        if (a.black && (a.black[cell] & a.black_mask)) continue;
+        const float te = (float)a.Te[cell], ne = (float)a.Ne[cell], hd = (float)a.Hden[cell];
+        const float xp = (float)a.ion[((size_t)cell * nE) * (ns + 1) + 2];
+        float h = 2530.f / powf(te, 0.833f);
+        h = h * ne * xp * (float)abunH;
+        hb = hb + (double)h * (double)hd * (double)line_volume(a, cell);
+    }
+    // lanes in order: a fixed summation order
+    if (lane == 0) S.y[0] = 0.0;
+    __syncthreads();
+    for (int l = 0; l < W; ++l) {
+        if (lane == l) S.y[0] = S.y[0] + hb;
+        __syncthreads();
+    }
+    if (lane == 0) out[(size_t)A.ncel * kLineEnt] = S.y[0];
+}
+
+// out[t] = sum over blocks b (in order) of part[b][t]; four wavefronts per
+// 64 outputs, each over every fourth block, combined in wavefront order
+__global__ void __launch_bounds__(256) lines_reduce_kernel(const double *__restrict__ part, int ldp, int nblk,
+                                                           int n, double *__restrict__ out)
+{
+    __shared__ double red[4][W];
+    const int lane = threadIdx.x % W, w = threadIdx.x / W;
+    const int t = blockIdx.x * W + lane;
+    double sum = 0.0;
+    if (t < n)
+        for (int b = w; b < nblk; b += 4) sum = sum + part[(size_t)b * ldp + t];
+    red[w][lane] = sum;
+    __syncthreads();
+    if (w == 0 && t < n) out[t] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+}
+
 }  // namespace mcb
 
 hipError_t mcb_launch_balance(const mcb::Atomic *dA, const mcb::CellArgs &a, hipStream_t s)
@@ -853,6 +980,16 @@ hipError_t mcb_launch_warm(const mcb::Atomic *dA, const mcb::CellArgs &a, const 
 {
     if (a.ncells <= 0) return hipSuccess;
     hipLaunchKernelGGL(mcb::warm_kernel, dim3((a.ncells + 255) / 256), dim3(256), 0, s, dA, a, Y, ldy, hit);
+    return hipGetLastError();
+}
+
+hipError_t mcb_launch_lines(const mcb::Atomic *dA, const mcb::LineArgs &a, int nblk, int ncel, double *out,
+                            hipStream_t s)
+{
+    if (nblk <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mcb::lines_kernel, dim3(nblk), dim3(64), 0, s, dA, a);
+    const int n = ncel * mcb::kLineEnt + 1;
+    hipLaunchKernelGGL(mcb::lines_reduce_kernel, dim3((n + 63) / 64), dim3(256), 0, s, a.part, a.ldp, nblk, n, out);
     return hipGetLastError();
 }
 

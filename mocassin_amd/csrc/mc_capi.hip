@@ -85,6 +85,10 @@ struct mc_ctx {
     mco::OpacArgs op{};
     int *op_slot_e = nullptr, *op_slot_st = nullptr;   // into d_op_i
     std::vector<int> op_sl_e, op_sl_st;                // slot -> element index, stage
+    // line luminosities: per-block partial sums, a host state, the result
+    double *d_ln_part = nullptr, *d_ln_st = nullptr, *d_ln_out = nullptr;
+    size_t ln_part_n = 0;
+    int *d_ln_black = nullptr;
     char err[512] = "";
 };
 
@@ -501,6 +505,7 @@ static mcb::CellArgs cell_args(mc_ctx *c, double jscale)
     a.flags = c->d_flags; a.nIter = c->d_nIter;
     a.solver = MC_SOLVER_REFERENCE; a.max_iter = 25; a.tol = 0.01; a.ion_tol = 0.01; a.step0 = 0.05;
     a.slope0 = 1.0;
+    a.warm_iter = a.max_iter;
     return a;
 }
 
@@ -708,6 +713,9 @@ static int update_cells(mc_ctx *c, double jscale, const mc_balance_opts *opts, c
             if (opts->solver == MC_SOLVER_SECANT) {
                 a.tol = opts->tol;
                 a.max_iter = opts->max_iter;
+                if (opts->warm_iter < 0 || opts->warm_iter > 1000)
+                    return fail(c, MC_EINVAL, "warm_iter outside [0,1000]");
+                a.warm_iter = opts->warm_iter ? opts->warm_iter : opts->max_iter;
                 if (opts->first_step != 0.0) {
                     if (!(opts->first_step > 0.0 && opts->first_step <= 1.0))
                         return fail(c, MC_EINVAL, "first_step outside (0,1]");
@@ -721,6 +729,8 @@ static int update_cells(mc_ctx *c, double jscale, const mc_balance_opts *opts, c
             }
             warm = opts->warm_start != 0;
             if (warm && !c->mlp) return fail(c, MC_ESTATE, "warm start needs mc_mlp_set");
+            // a warm start is about a per cent off: its first step is that size
+            if (warm && opts->solver == MC_SOLVER_SECANT && opts->first_step == 0.0) a.step0 = 0.01;
         }
         nE = c->bal.A.nE;
         ns = c->bal.A.nstages;
@@ -1018,6 +1028,104 @@ extern "C" int mc_opacity_build(mc_ctx *c, const float *Hden, const float *Te, c
     }
     c->opac_built = true;
     c->opac_sca_built = dust;
+    return MC_OK;
+}
+
+// ------------------------------------------------------- line luminosities
+
+extern "C" int mc_line_luminosities(mc_ctx *c, const float *Hden, const float *Te, const float *Ne,
+                                    const float *ionDen, const int32_t *lgBlack, double *forbVol,
+                                    double *HbetaVol, double *kernel_ms)
+{
+    if (!c) return MC_EINVAL;
+    if (!c->balance) return fail(c, MC_ESTATE, "mc_balance_setup has not been called");
+    if (!forbVol || !HbetaVol) return fail(c, MC_EINVAL, "forbVol and HbetaVol are required");
+    const bool host_state = Hden || Te || Ne || ionDen;
+    if (host_state && !(Hden && Te && Ne && ionDen))
+        return fail(c, MC_EINVAL, "Hden, Te, Ne and ionDen go together (or all NULL for the device state)");
+    if (!host_state && lgBlack)
+        return fail(c, MC_EINVAL, "lgBlack goes with a host state (the device state uses its last update's flags)");
+    if (!host_state && !c->state_dev)
+        return fail(c, MC_ESTATE, "no device state yet: pass the state or run mc_update_cells first");
+    const mcb::Atomic &A = c->bal.A;
+    const int nc = c->G.nCells, nE = A.nE, ns = A.nstages, ncel = A.ncel;
+    const size_t nc1 = (size_t)nc + 1, nion = nc1 * nE * (ns + 1);
+    if (A.el[0].z != 1) return fail(c, MC_EUNSUP, "Hbeta needs hydrogen among the elements");
+    int rc = cell_positions(c);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    mcb::LineArgs a{};
+    a.ncells = nc;
+    a.cells_per_blk = std::max(1, (nc + 4095) / 4096);
+    const int nblk = (nc + a.cells_per_blk - 1) / a.cells_per_blk;
+    a.ldp = ncel * mcb::kLineEnt + 1;
+    a.cell_xyz = c->d_cell_xyz;
+    a.nx = c->G.nx;
+    a.ny = c->G.ny;
+    a.dfac = c->d_dfac;
+    if (host_state) {
+        std::vector<double> st(3 * nc1 + nion, 0.0);
+        for (int cell = 1; cell <= nc; ++cell)
+            if (!(Te[cell] > 0.f) || !(Hden[cell] >= 0.f) || !std::isfinite(Ne[cell]))
+                return fail(c, MC_EINVAL, "cell %d: Te must be > 0, Hden >= 0, Ne finite", cell);
+        double *ion = st.data() + 3 * nc1;
+        for (size_t cell = 0; cell < nc1; ++cell) {
+            st[cell] = Te[cell];
+            st[nc1 + cell] = Ne[cell];
+            st[2 * nc1 + cell] = Hden[cell];
+            for (int e = 0; e < nE; ++e) {
+                const size_t xr = (size_t)c->bal.eidx_xref[e] - 1;
+                for (int k = 1; k <= ns; ++k)
+                    ion[(cell * nE + e) * (ns + 1) + k] = ionDen[cell + nc1 * (xr + (size_t)nE * (k - 1))];
+            }
+        }
+        if (!c->d_ln_st) HIPCHK(c, dalloc(&c->d_ln_st, st.size()));
+        HIPCHK(c, hipMemcpyAsync(c->d_ln_st, st.data(), st.size() * sizeof(double), hipMemcpyHostToDevice,
+                                 c->stream));
+        a.Te = c->d_ln_st;
+        a.Ne = c->d_ln_st + nc1;
+        a.Hden = c->d_ln_st + 2 * nc1;
+        a.ion = c->d_ln_st + 3 * nc1;
+        if (lgBlack) {
+            std::vector<int> bl(nc1);
+            for (size_t i = 0; i < nc1; ++i) bl[i] = lgBlack[i] >= 1 ? 1 : 0;
+            if (!c->d_ln_black) HIPCHK(c, dalloc(&c->d_ln_black, nc1));
+            HIPCHK(c, hipMemcpyAsync(c->d_ln_black, bl.data(), nc1 * sizeof(int), hipMemcpyHostToDevice, c->stream));
+            a.black = c->d_ln_black;
+            a.black_mask = 1;
+        }
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    } else {
+        a.Te = c->d_Te;
+        a.Ne = c->d_Ne;
+        a.Hden = c->d_Hden;
+        a.ion = c->d_ion;
+        a.black = c->d_flags;             // bit 1: the last update found no Te or ion balance
+        a.black_mask = 2;
+    }
+    const size_t npart = (size_t)nblk * a.ldp;
+    if (c->ln_part_n < npart) {
+        if (c->d_ln_part) { (void)hipFree(c->d_ln_part); c->d_ln_part = nullptr; }
+        if (c->d_ln_out) { (void)hipFree(c->d_ln_out); c->d_ln_out = nullptr; }
+        c->ln_part_n = 0;
+        HIPCHK(c, dalloc(&c->d_ln_part, npart));
+        HIPCHK(c, dalloc(&c->d_ln_out, (size_t)a.ldp));
+        c->ln_part_n = npart;
+    }
+    a.part = c->d_ln_part;
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, mcb_launch_lines(c->d_atomic, a, nblk, ncel, c->d_ln_out, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipMemcpyAsync(forbVol, c->d_ln_out, (size_t)ncel * mcb::kLineEnt * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(HbetaVol, c->d_ln_out + (size_t)ncel * mcb::kLineEnt, sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (kernel_ms) {
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        *kernel_ms = ms;
+    }
     return MC_OK;
 }
 
@@ -1515,7 +1623,8 @@ extern "C" void mc_ctx_destroy(mc_ctx *c)
                      c->d_ion, c->d_flags, c->d_nIter, c->d_cold, c->d_mlp, c->d_mlpW, c->d_mX,
                      c->d_mY, c->d_mX0, c->d_hit, c->d_cross, c->d_Rint, c->d_Hint,
                      c->d_dust_tab, c->d_Tdust, c->d_dust_mono, c->d_pe_tab, c->d_grains, c->d_op_dust,
-                     c->d_op_i, c->d_op_f, c->d_op_st, c->d_op_Td, c->d_op_out, c->d_op_ent };
+                     c->d_op_i, c->d_op_f, c->d_op_st, c->d_op_Td, c->d_op_out, c->d_op_ent,
+                     c->d_ln_part, c->d_ln_st, c->d_ln_out, c->d_ln_black };
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

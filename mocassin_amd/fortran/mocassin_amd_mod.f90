@@ -54,6 +54,7 @@ module mocassin_amd_mod
         real(c_double) :: tol, ion_tol
         integer(c_int32_t) :: warm_start
         real(c_double) :: first_step, first_slope
+        integer(c_int32_t) :: warm_iter
     end type mc_balance_opts
 
     type, bind(C) :: mc_batch_tuning
@@ -171,6 +172,12 @@ module mocassin_amd_mod
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx, Hden, Te, Ne, ionDen, Tdust, opacity, scaOpac, kernel_ms
         end function mc_opacity_build
+
+        integer(c_int) function mc_line_luminosities(ctx, Hden, Te, Ne, ionDen, lgBlack, forbVol, HbetaVol, &
+                                                     kernel_ms) bind(C, name="mc_line_luminosities")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, Hden, Te, Ne, ionDen, lgBlack, forbVol, HbetaVol, kernel_ms
+        end function mc_line_luminosities
 
         integer(c_int) function mc_mlp_set(ctx, params, scale) bind(C, name="mc_mlp_set")
             import :: c_int, c_ptr
@@ -395,5 +402,19 @@ contains
         noTeBal = count(iand(fl(1:), 4) /= 0)
         deallocate (fl, nIt, dD)
     end subroutine gpu_update_cells_dust
+
+    ! outputGas's analytic forbidden-line and Hbeta luminosities of the grid's
+    ! state (output_mod.f90:302-520), before the division by HbetaVol:
+    ! forbVol(lower, upper, model ion in forLines order)
+    subroutine gpu_line_luminosities(ctx, Hden, Te, Ne, ionDen, lgBlack, forbVol, HbetaVol)
+        type(c_ptr), intent(in) :: ctx
+        real, intent(in), target :: Hden(0:), Te(0:), Ne(0:), ionDen(0:, :, :)
+        integer(c_int32_t), intent(in), target :: lgBlack(0:)
+        real(c_double), intent(out), target :: forbVol(:, :, :)
+        real(c_double), intent(out), target :: HbetaVol
+        call mc_check(mc_line_luminosities(ctx, c_loc(Hden), c_loc(Te), c_loc(Ne), c_loc(ionDen), &
+                                           c_loc(lgBlack), c_loc(forbVol), c_loc(HbetaVol), c_null_ptr), &
+                      ctx, 'outputGas lines')
+    end subroutine gpu_line_luminosities
 
 end module mocassin_amd_mod

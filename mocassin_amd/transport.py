@@ -144,12 +144,12 @@ class Transport:
         A.XHILimit = b.XHILimit
         if keep["ch_sigma"].size != b.nch * self.nbins:
             raise ValueError("cross sections not on this frequency mesh")
-        self._nE, self._ns = b.nElemUsed, b.nstages
+        self._nE, self._ns, self._ncel = b.nElemUsed, b.nstages, b.ncel
         self._check(self._lib.mc_balance_setup(self._h, C.byref(A)))
 
     @staticmethod
     def _opts(solver=None, tol=1e-3, max_iter=60, ion_tol=0.01, warm_start=False, first_step=0.0,
-              first_slope=0.0):
+              first_slope=0.0, warm_iter=0):
         if solver is None and not warm_start and ion_tol == 0.01:
             return None
         opts = _capi.McBalanceOpts()
@@ -158,10 +158,11 @@ class Transport:
         opts.warm_start = 1 if warm_start else 0
         opts.first_step = float(first_step)
         opts.first_slope = float(first_slope)
+        opts.warm_iter = int(warm_iter)
         return opts
 
     def update_cells(self, Hden, Te, Ne, ionDen, jscale=1.0, solver=None, tol=1e-3, max_iter=60,
-                     ion_tol=0.01, warm_start=False, first_step=0.0, first_slope=0.0):
+                     ion_tol=0.01, warm_start=False, first_step=0.0, first_slope=0.0, warm_iter=0):
         """updateCell for every active cell from the device J times jscale.
         Hden/Te/Ne (nCells+1,), ionDen (nCells+1, nElemUsed, nstages).
         solver None: the reference's schedule; "secant": secant iteration to
@@ -178,7 +179,7 @@ class Transport:
         flags = np.zeros(nc1, np.int32)
         nIt = np.zeros(nc1, np.int32)
         ms = C.c_double()
-        opts = self._opts(solver, tol, max_iter, ion_tol, warm_start, first_step, first_slope)
+        opts = self._opts(solver, tol, max_iter, ion_tol, warm_start, first_step, first_slope, warm_iter)
         self._check(self._lib.mc_balance_run(self._h, float(jscale), None if opts is None else C.byref(opts),
                                              Hd.ctypes.data, Te.ctypes.data,
                                              Ne.ctypes.data, ion.ctypes.data, flags.ctypes.data,
@@ -286,6 +287,29 @@ class Transport:
         if host:
             op, sca = op.T.copy(), sca.T.copy()
         return op, sca, ms.value
+
+    def line_luminosities(self, Hden=None, Te=None, Ne=None, ionDen=None, lgBlack=None):
+        """outputGas's analytic line luminosities: (forbVol [nCel, 17, 17]
+        (model ion in forLines order, upper level, lower level), HbetaVol,
+        kernel_ms), before the division by HbetaVol.  The state as the host
+        holds it (FP32, with lgBlack or None), or None for the device state
+        of the last update."""
+        nc1 = self.nCells + 1
+        st = [None] * 4
+        bl = None
+        if Te is not None:
+            st = [np.ascontiguousarray(Hden, np.float32).reshape(nc1), np.ascontiguousarray(Te, np.float32).reshape(nc1),
+                  np.ascontiguousarray(Ne, np.float32).reshape(nc1),
+                  np.asfortranarray(np.asarray(ionDen, np.float32).reshape(nc1, self._nE, self._ns))]
+            if lgBlack is not None:
+                bl = np.ascontiguousarray(lgBlack, np.int32).reshape(nc1)
+        forb = np.zeros((self._ncel, 17, 17), np.float64)
+        hb = C.c_double()
+        ms = C.c_double()
+        ptr = lambda a: None if a is None else a.ctypes.data
+        self._check(self._lib.mc_line_luminosities(self._h, *(ptr(a) for a in st), ptr(bl), forb.ctypes.data,
+                                                   C.byref(hb), C.byref(ms)))
+        return forb, hb.value, ms.value
 
     # ---- warm-start network
     def set_mlp(self, params, scale):

@@ -480,6 +480,47 @@ static void FN(therm_balance)(const mco_atomic *A, FN(cell_t) *c, R *heatInt, R 
     *coolInt = coolFF + coolRec + coolColl;
 }
 
+/* outputGas's analytic line luminosities for a 3-D grid (output_mod.f90:
+ * 302-520): per cell in the caller's order, forLines (1946-2008: equilibrium
+ * of every model ion, times elemAbun * ionDen, times 1.9865e9, the REAL
+ * literal) and RecLinesEmission's Hbeta (2010-2066: 2530/Te**0.833 * Ne *
+ * ionDen(H,2) * elemAbun(H); HIRecLines(4,2) = (fh/hb) * Hbeta with fh = hb),
+ * each times Hden * dV and added to forbVol and HbetaVol, which the reference
+ * holds as REAL: with R = float each cell's term is rounded to float and
+ * summed in float, with R = double everything stays double.  State per cell c:
+ * Hden/Te/Ne/dV[c], ionDen[c * nstages * nElemUsed + (s-1)*nElemUsed + xref-1].
+ * forb[(k * 17 + up - 1) * 17 + lo - 1] is forbVol of model ion k (forLines
+ * order), line up -> lo; both outputs are accumulated into (not zeroed). */
+void FN(mco_line_lum)(const mco_atomic *A, int n, const R *Hden, const R *Te, const R *Ne, const R *ionDen,
+                      const R *dV, R *forb, R *hbeta)
+{
+    const int ns = A->nstages, nE = A->nElemUsed;
+    double fl[NFOR * NFOR];
+    const double kL = (double)(R)L(1.9865e9);
+    for (int c = 0; c < n; ++c) {
+        const R *ion = ionDen + (size_t)c * ns * nE;
+        for (int k = 0; k < A->ncel; ++k) {
+            const int elem = A->cel_meta[6 * k], st = A->cel_meta[6 * k + 1];
+            const int xr = A->xref[elem] - 1;
+            const R x = ion[(st - 1) * nE + xr];
+            const R up = (st < ns) ? ion[st * nE + xr] / x : L(0.);
+            for (int i = 0; i < NFOR * NFOR; ++i) fl[i] = 0.0;
+            FN(equilibrium)(A, k, up, Te[c], Ne[c], fl);
+            for (int lo = 1; lo <= NFOR; ++lo)
+                for (int hi = 1; hi <= NFOR; ++hi) {
+                    double f = fl[(lo - 1) + NFOR * (hi - 1)];
+                    f = f * (double)(R)A->abun[elem] * (double)x;
+                    f = f * kL;
+                    R *o = forb + ((size_t)k * NFOR + hi - 1) * NFOR + lo - 1;
+                    *o = *o + (R)(f * (double)Hden[c] * (double)dV[c]);
+                }
+        }
+        R hb = L(2530.) / POWR(Te[c], L(0.833));
+        hb = hb * Ne[c] * ion[1 * nE + A->xref[1] - 1] * (R)A->abun[1];
+        *hbeta = *hbeta + (R)((double)hb * (double)Hden[c] * (double)dV[c]);
+    }
+}
+
 /* ---- grain charge, photoelectric heating and gas-grain collisions ------
  * (update_mod.f90:527-932), one dust chemistry.  Constants as the reference
  * declares them: Pi a REAL (constants_mod.f90:32), Y0 = 0.5, Y1 = 0.2,

@@ -139,6 +139,14 @@ int mco_balance_cell_pe_f64(const mco_atomic *A, const mco_grains *G, int nsMax,
                             double Hden, double *Te, double *Ne, double *ionDen, int *nT, double ndust,
                             const double *Tdust, double *gdc, double *peh, double *pot);
 
+/* outputGas's analytic forbidden-line luminosities forbVol of every model ion
+ * and HbetaVol (output_mod.f90:302-520) of n cells, accumulated into forb
+ * [ncel][17][17] (model ion, upper level, lower level) and *hbeta. */
+void mco_line_lum_f32(const mco_atomic *A, int n, const float *Hden, const float *Te, const float *Ne,
+                      const float *ionDen, const float *dV, float *forb, float *hbeta);
+void mco_line_lum_f64(const mco_atomic *A, int n, const double *Hden, const double *Te, const double *Ne,
+                      const double *ionDen, const double *dV, double *forb, double *hbeta);
+
 #ifdef __cplusplus
 }
 #endif

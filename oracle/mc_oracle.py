@@ -293,6 +293,29 @@ class Balance:
             nT[c] = n.value
         return Te, Ne, ion, flags, nT
 
+    def line_lum(self, Hden, Te, Ne, ionDen, dV, cells, precision="f32"):
+        """outputGas's forbVol [ncel, 17, 17] (model ion, upper, lower) and
+        HbetaVol over `cells` in that order; Hden/Te/Ne/dV (nCells+1,),
+        ionDen (nCells+1, nElemUsed, nstages).  f32: the reference's REAL
+        sums; f64: all in double."""
+        L = lib()
+        if not hasattr(L, "_lines_declared"):
+            for name in ("mco_line_lum_f32", "mco_line_lum_f64"):
+                f = getattr(L, name)
+                f.restype = None
+                f.argtypes = [C.POINTER(_Atomic), C.c_int] + [C.c_void_p] * 7
+            L._lines_declared = True
+        dt = np.float32 if precision == "f32" else np.float64
+        f = L.mco_line_lum_f32 if precision == "f32" else L.mco_line_lum_f64
+        cells = np.asarray(cells, np.int64)
+        col = lambda a: np.ascontiguousarray(np.asarray(a)[cells], dt)
+        ion = np.ascontiguousarray(np.asarray(ionDen)[cells].transpose(0, 2, 1), dt)   # [cell][stage][elem]
+        args = [col(Hden), col(Te), col(Ne), ion, col(dV)]
+        forb = np.zeros((self.b.ncel, 17, 17), dt)
+        hb = np.zeros(1, dt)
+        f(C.byref(self.A), len(cells), *(a.ctypes.data for a in args), forb.ctypes.data, hb.ctypes.data)
+        return forb, float(hb[0])
+
     def run_grains(self, J, Hden, Te, Ne, ionDen, g, Tdust, precision="f64", cells=None):
         """run() with the grain charge, photoelectric and gas-grain collision
         terms of a dust chemistry (g: a balance fixture's dust_* arrays;
@@ -359,6 +382,39 @@ def dust_temps(J, Tdust, g, precision="f64", gdc=None, peh=None):
       keep[2].ctypes.data, keep[3].ctypes.data, nSp, None if add[0] is None else add[0].ctypes.data,
       None if add[1] is None else add[1].ctypes.data, T.ctypes.data)
     return T
+
+
+def cell_volumes(xAxis, yAxis, zAxis, active, symmetric):
+    """getVolume (grid_mod.f90:2876-2964) of every cell of a 3-D grid in
+    FP32, and the cells in outputGas's loop order (x outer, z inner).
+    active (nx, ny, nz).  Returns (dV (nCells+1,), cells)."""
+    def widths(v):
+        v = np.asarray(v, np.float32)
+        n = v.size
+        w = np.zeros(n, np.float32)
+        for i in range(1, n + 1):
+            if 1 < i < n:
+                w[i - 1] = np.float32(abs(v[i] - v[i - 2])) / np.float32(2.)
+            elif i == 1:
+                d = np.float32(abs(v[1] - v[0]))
+                w[0] = d / np.float32(2.) if symmetric else d
+            else:
+                w[n - 1] = np.float32(abs(v[n - 1] - v[n - 2]))
+        return w / np.float32(1.e15)
+    wx, wy, wz = widths(xAxis), widths(yAxis), widths(zAxis)
+    active = np.asarray(active)
+    nc = int(active.max())
+    dV = np.zeros(nc + 1, np.float32)
+    cells = []
+    nx, ny, nz = active.shape
+    for i in range(nx):
+        for j in range(ny):
+            for k in range(nz):
+                c = int(active[i, j, k])
+                if c > 0:
+                    dV[c] = (wx[i] * wy[j]) * wz[k]
+                    cells.append(c)
+    return dV, np.array(cells, np.int64)
 
 
 def opacity_tables(g, gaunt_cache=True, Te_cache=None):

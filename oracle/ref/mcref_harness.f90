@@ -107,6 +107,7 @@ program mcref_harness
     case ("opacity")
        call run_transport()
        call run_balance()
+       call run_lines()
        call run_opacity()
     case ("walk")
        call get_command_argument(5, arg); read(arg, *) nuBin
@@ -477,6 +478,71 @@ contains
           nseg = nseg + 1
        end subroutine seg
     end subroutine run_opacity
+
+    ! outputGas's analytic line luminosities for a 3-D grid (output_mod.f90:
+    ! 302-520) from the state after the balance pass: per active cell in its
+    ! loop order, forLines (1946-2008: equilibrium of every model ion, times
+    ! elemAbun * ionDen * 1.9865e9) and RecLinesEmission's Hbeta (2010-2066:
+    ! 2530/Te**0.833 * Ne * ionDen(H,2) * elemAbun(H); HIRecLines(4,2) is
+    ! (fh/hb) * Hbeta with fh = hb), each times Hden * getVolume and summed
+    ! in the REAL forbVol and HbetaVol (before their division by HbetaVol).
+    subroutine run_lines()
+       integer :: nc, elem, ion, ncel, ic2, xr
+       double precision :: fl(nForLevels, nForLevels)
+       real, allocatable :: L(:, :, :)
+       real :: up, x, hbc, hb(1), dV, Hd
+       nc = grid3D(1)%nCells
+       ncel = 0
+       do elem = 3, nElements
+          do ion = 1, min(elem+1, nstages)
+             if (.not.lgElementOn(elem)) exit
+             if (lgDataAvailable(elem, ion)) ncel = ncel + 1
+          end do
+       end do
+       allocate(L(nForLevels, nForLevels, max(ncel, 1)))
+       L = 0.
+       hb = 0.
+       do i = 1, grid3D(1)%nx
+          do j = 1, grid3D(1)%ny
+             do k = 1, grid3D(1)%nz
+                ic2 = grid3D(1)%active(i,j,k)
+                if (ic2 <= 0) cycle
+                dV = getVolume(grid3D(1), i, j, k)
+                Hd = grid3D(1)%Hden(ic2)
+                ncel = 0
+                do elem = 3, nElements
+                   do ion = 1, min(elem+1, nstages)
+                      if (.not.lgElementOn(elem)) exit
+                      if (.not.lgDataAvailable(elem, ion)) cycle
+                      ncel = ncel + 1
+                      xr = elementXref(elem)
+                      x = grid3D(1)%ionDen(ic2, xr, ion)
+                      if (ion < nstages) then
+                         up = grid3D(1)%ionDen(ic2, xr, ion+1) / x
+                      else
+                         up = 0.
+                      end if
+                      fl = 0.d0
+                      call equilibrium(dataFile(elem, ion), up, grid3D(1)%Te(ic2), grid3D(1)%Ne(ic2), fl)
+                      fl = fl * grid3D(1)%elemAbun(1, elem) * x
+                      fl = fl * 1.9865e9
+                      L(:, :, ncel) = L(:, :, ncel) + real(fl * Hd * dV)
+                   end do
+                end do
+                hbc = 2530./(grid3D(1)%Te(ic2)**0.833)
+                hbc = hbc * grid3D(1)%Ne(ic2) * grid3D(1)%ionDen(ic2, elementXref(1), 2) * grid3D(1)%elemAbun(1, 1)
+                hb(1) = hb(1) + real(dble(hbc) * Hd * dV)
+             end do
+          end do
+       end do
+       open(unit=86, file=trim(outdir)//'/lines_L.bin', access='stream', form='unformatted', status='replace')
+       write(86) L
+       close(86)
+       open(unit=86, file=trim(outdir)//'/lines_hb.bin', access='stream', form='unformatted', status='replace')
+       write(86) hb
+       close(86)
+       deallocate(L)
+    end subroutine run_lines
 
     ! scattering cross-sections of every species and size on nuArray and the
     ! sublimation temperatures (the dust add of iteration_mod.f90:184-222)

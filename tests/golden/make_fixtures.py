@@ -188,7 +188,13 @@ def make_opacity(deck, npk, seed, name, keep_cells=96):
                Te=rd("op_Te"), Ne=rd("op_Ne"), ionDen=rd("op_ionDen"),
                elem_on=rd("at_elem_on", np.int32), xref=rd("at_xref", np.int32), abun=rd("at_abun"),
                at_ints=rd("at_ints", np.int32), seg_meta=seg, seg_sigma=np.concatenate(sig),
-               seg_off=np.array(off, np.int64), brems1=xs[bp - 1:bp], cells=cells, op_gas=pick(rd("op_gas")))
+               seg_off=np.array(off, np.int64), brems1=xs[bp - 1:bp], cells=cells, op_gas=pick(rd("op_gas")),
+               xAxis=prob["xAxis"], yAxis=prob["yAxis"], zAxis=prob["zAxis"], active=prob["active"],
+               lines_L=rd("lines_L"), lines_hb=rd("lines_hb"),
+               Ne0=rd("Ne"), ionDen0=rd("ionDen"))
+    J = rd("Jste_scaled")
+    nzJ = np.nonzero(J)[0]
+    out.update(J_idx=nzJ.astype(np.int32), J_val=J[nzJ])
     if os.path.exists(os.path.join(d, "op_total.bin")):
         out.update(Tdust=rd("op_Tdust"), op_total=pick(rd("op_total")), op_sca=pick(rd("op_sca")),
                    dust_ints=rd("dust_ints", np.int32), dust_abs=rd("dust_abs"), dust_sca=rd("dust_sca"),

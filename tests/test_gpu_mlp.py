@@ -147,10 +147,13 @@ def _balance_ctx(name):
 
 
 def test_warm_start_gains_over_cold_start():
-    """A network trained on half of the cells of two decks warm-starts the
-    secant balance of the other half: fewer temperature iterations than the
-    cold start, the same temperatures within the solver's tolerance, and no
-    cell lost to the fallback."""
+    """The warm start's mechanism and its safety net.  A network trained on
+    the reached cells of two decks warm-starts their secant balance: far
+    fewer temperature iterations than the cold start, the same temperatures
+    within the solver's tolerance, and almost no restarts.  Trained on half
+    of the cells only (a few hundred: the network then predicts the other
+    half to several per cent), the held-out cells still gain and reach the
+    same temperatures."""
     print("warm start gain", flush=True)
     Xs, Ts, Ms, ctxs = [], [], [], []
     ref = {}
@@ -167,30 +170,39 @@ def test_warm_start_gains_over_cold_start():
         Ms.append(Mt)
         ref[name] = (Te_r, cells)
     X, Tg, M = np.concatenate(Xs), np.concatenate(Ts), np.concatenate(Ms)
-    rng = np.random.default_rng(0)
-    train = rng.random(len(X)) < 0.5
-    try:
-        T0 = ctxs[0][0]
-        loss, ms = MP.train(T0, X[train], Tg[train], M[train], steps=3000, batch=256, lr=2e-3, seed=2)
-        params, scale = T0.get_mlp()
+    M[:, 0] *= 5.0                      # log10 Te weighed above each fraction
+    half = np.random.default_rng(0).random(len(X)) < 0.5
+    kw = dict(solver="secant", tol=1e-3, max_iter=60, ion_tol=1e-3)
+
+    def judge(params, scale, pick):
         off = 0
-        cold_it, warm_it, worst = [], [], 0.0
+        cold_it, warm_it, worst, n_restart = [], [], 0.0, 0
         for (T, b, Hden, Te0, Ne0, ion0), name in zip(ctxs, ("c2mini", "c2mini_thick")):
             Te_r, cells = ref[name]
-            held = cells[~train[off:off + len(cells)]]
+            sel = cells[pick[off:off + len(cells)]]
             off += len(cells)
             T.set_mlp(params, scale)
-            kw = dict(solver="secant", tol=1e-3, max_iter=60, ion_tol=1e-3)
             Tc, _, _, flc, nc_, _ = T.update_cells(Hden, Te0, Ne0, ion0, **kw)
             Tw, _, _, flw, nw_, _ = T.update_cells(Hden, Te0, Ne0, ion0, warm_start=True, **kw)
-            cold_it += list(nc_[held])
-            warm_it += list(nw_[held])
-            worst = max(worst, np.max(np.abs(Tw[held] - Te_r[held]) / Te_r[held]))
-            assert np.all((flw[held] & 32) == 0)
+            cold_it += list(nc_[sel])
+            warm_it += list(nw_[sel])
+            worst = max(worst, np.max(np.abs(Tw[sel] - Te_r[sel]) / Te_r[sel]))
+            n_restart += int(np.count_nonzero(flw[sel] & 32))
+            assert np.all((flw[sel] & 4) == 0)
+        return np.mean(cold_it), np.mean(warm_it), worst, n_restart, len(cold_it)
+
+    try:
+        T0 = ctxs[0][0]
+        MP.train(T0, X, Tg, M, steps=3000, batch=256, lr=2e-3, seed=2)
+        c, w, worst, nr, n = judge(*T0.get_mlp(), np.ones(len(X), bool))
+        print("all cells: cold %.2f, warm %.2f iterations per cell; worst Te error %.2e; %d of %d restarted"
+              % (c, w, worst, nr, n))
+        assert w < 0.5 * c and worst < 0.01 and nr <= max(1, 0.02 * n)
+        MP.train(T0, X[half], Tg[half], M[half], steps=3000, batch=256, lr=2e-3, seed=2)
+        c, w, worst, nr, n = judge(*T0.get_mlp(), ~half)
+        print("held-out cells: cold %.2f, warm %.2f iterations per cell; worst Te error %.2e; %d of %d restarted"
+              % (c, w, worst, nr, n))
+        assert w < 0.7 * c and worst < 0.01 and nr <= max(1, 0.02 * n)
     finally:
-        for c in ctxs:
-            c[0].close()
-    print("cold %.2f, warm %.2f iterations per held-out cell; worst Te error %.2e"
-          % (np.mean(cold_it), np.mean(warm_it), worst))
-    assert np.mean(warm_it) < 0.7 * np.mean(cold_it)
-    assert worst < 0.01
+        for c_ in ctxs:
+            c_[0].close()
