@@ -104,6 +104,23 @@ def opacity_cpu_baseline(atom, bundle, nb, nu, Hden, Te, Ne, ion, cells=3000):
             "sample": "%d cells of the workload's state after the balance, scalar C restatement" % n}
 
 
+def lines_cpu_baseline(bundle, Hden, Te, Ne, ion, dV, cells=500):
+    """outputGas's forbVol/HbetaVol sums (the FP64 C restatement the GPU
+    kernel follows, oracle mco_line_lum_f64) over the first `cells` cells of
+    the same state, one thread: cells per second."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import mc_oracle as O
+    import numpy as np
+    n = min(cells, Te.size - 1)
+    f32 = lambda v: np.asarray(v, np.float32).astype(np.float64)
+    B = O.Balance(bundle)
+    t = time.perf_counter()
+    B.line_lum(f32(Hden), f32(Te), f32(Ne), f32(ion), f32(dV), np.arange(1, n + 1), "f64")
+    dt = time.perf_counter() - t
+    return {"value": n / dt, "unit": "cells/s", "cores": 1, "kind": "port",
+            "sample": "%d cells of the workload's state after the balance, scalar C restatement" % n}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +132,7 @@ def main():
     ap.add_argument("--no-te", action="store_true", help="skip the Te-convergence measurement")
     ap.add_argument("--no-balance", action="store_true", help="skip the balance measurement")
     ap.add_argument("--no-grains", action="store_true", help="skip the balance with grains")
+    ap.add_argument("--no-lines", action="store_true", help="skip the line luminosities")
     ap.add_argument("--no-opacity", action="store_true", help="skip the opacity-table build")
     ap.add_argument("--autotune", action="store_true",
                     help="time the launch-shape candidates on this workload and store the best "
@@ -228,6 +246,23 @@ def main():
             balance["opacity"] = opacity
             print("bench: opacity tables %.3f ms (%.2f TB/s of table writes)"
                   % (ms_o, opacity["roofline"]["achieved"] / 1e3), file=sys.stderr, flush=True)
+        if not a.no_lines and bundle.ncel > 0:
+            # outputGas's analytic line luminosities of the balanced state
+            ms_l = min(T.line_luminosities()[2] for _ in range(3))
+            forb, hb, _ = T.line_luminosities()
+            lines = {"device_ms": ms_l, "cells_per_s": p["nCells"] / (ms_l * 1e-3), "model_ions": int(bundle.ncel),
+                     "lines_above_1e-3_hbeta": int(np.count_nonzero(forb > 1e-3 * hb)),
+                     "kernel": "lines_kernel + lines_reduce_kernel"}
+            if not a.no_cpu:
+                sys.path.insert(0, os.path.join(ROOT, "oracle"))
+                import mc_oracle as O
+                dV, _ = O.cell_volumes(p["xAxis"], p["yAxis"], p["zAxis"], np.asarray(p["active"]),
+                                       p["symmetric"])
+                lines["cpu_baseline"] = lines_cpu_baseline(bundle, Hd, r[0], r[1], r[2], dV)
+                lines["gpu_over_cpu"] = lines["cells_per_s"] / lines["cpu_baseline"]["value"]
+            balance["lines"] = lines
+            print("bench: line luminosities %.3f ms (%d model ions, %d lines above 1e-3 Hbeta)"
+                  % (ms_l, bundle.ncel, lines["lines_above_1e-3_hbeta"]), file=sys.stderr, flush=True)
         if not a.no_grains:
             # the same balance with silicate grains, their charge and the
             # photoelectric and collision terms, then the grain temperatures

@@ -389,7 +389,7 @@ int mc_opacity_build(mc_ctx *ctx, const float *Hden, const float *Te, const floa
  * 302-520), summed over every active cell that is not black: per cell,
  * forLines (1946-2008: each model ion's level populations from equilibrium,
  * emissivity a(j,i) (e(j) - e(i)) n(j) times elemAbun, ionDen and the REAL
- * 1.9865e9) and RecLinesEmission's Hbeta (2010-2066: 2530/Te**0.833 Ne
+ * 1.9865e9) and RecLinesEmission's Hbeta (2065-2079: 2530/Te**0.833 Ne
  * ionDen(H,2) elemAbun(H)), each times Hden and getVolume (grid_mod.f90:
  * 2876-2964).  The state is Hden, Te, Ne (0:nCells) and ionDen(0:nCells,
  * 1:nElemUsed, 1:nstages) as the host holds them, with lgBlack (0:nCells,

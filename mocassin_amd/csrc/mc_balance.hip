@@ -914,7 +914,7 @@ __global__ void __launch_bounds__(64) lines_kernel(const Atomic *__restrict__ Ap
             if (e < kLineEnt) out[(size_t)k * kLineEnt + e] = acc[r];
         }
     }
-    // RecLinesEmission's Hbeta (output_mod.f90:2053-2058), HIRecLines(4,2) =
+    // RecLinesEmission's Hbeta (output_mod.f90:2065-2079), HIRecLines(4,2) =
     // (fh/hb) * Hbeta = Hbeta, times Hden * dV
     double hb = 0.0;
     const double abunH = (double)(float)A.el[0].abun;

@@ -483,7 +483,7 @@ static void FN(therm_balance)(const mco_atomic *A, FN(cell_t) *c, R *heatInt, R 
 /* outputGas's analytic line luminosities for a 3-D grid (output_mod.f90:
  * 302-520): per cell in the caller's order, forLines (1946-2008: equilibrium
  * of every model ion, times elemAbun * ionDen, times 1.9865e9, the REAL
- * literal) and RecLinesEmission's Hbeta (2010-2066: 2530/Te**0.833 * Ne *
+ * literal) and RecLinesEmission's Hbeta (2065-2079: 2530/Te**0.833 * Ne *
  * ionDen(H,2) * elemAbun(H); HIRecLines(4,2) = (fh/hb) * Hbeta with fh = hb),
  * each times Hden * dV and added to forbVol and HbetaVol, which the reference
  * holds as REAL: with R = float each cell's term is rounded to float and

@@ -482,7 +482,7 @@ contains
     ! outputGas's analytic line luminosities for a 3-D grid (output_mod.f90:
     ! 302-520) from the state after the balance pass: per active cell in its
     ! loop order, forLines (1946-2008: equilibrium of every model ion, times
-    ! elemAbun * ionDen * 1.9865e9) and RecLinesEmission's Hbeta (2010-2066:
+    ! elemAbun * ionDen * 1.9865e9) and RecLinesEmission's Hbeta (2065-2079:
     ! 2530/Te**0.833 * Ne * ionDen(H,2) * elemAbun(H); HIRecLines(4,2) is
     ! (fh/hb) * Hbeta with fh = hb), each times Hden * getVolume and summed
     ! in the REAL forbVol and HbetaVol (before their division by HbetaVol).
