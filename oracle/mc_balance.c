@@ -54,6 +54,7 @@ int mco_atomic_init(mco_atomic *A)
 #define GC_SAVED_SLOPE 1   /* setGrainPotential's SAVEd slope carries over between calls */
 static R pow10m12_f32(void) { return 1.e-12f; }
 #include "mc_balance_impl.h"
+#include "mc_emission_impl.h"
 #undef R
 #undef L
 #undef FN
@@ -79,3 +80,4 @@ static R pow10m12_f32(void) { return 1.e-12f; }
 #define GC_SAVED_SLOPE 0   /* each grain-potential solve starts from slope 0 */
 static R pow10m12_f64(void) { return 1.e-12; }
 #include "mc_balance_impl.h"
+#include "mc_emission_impl.h"

@@ -139,6 +139,45 @@ int mco_balance_cell_pe_f64(const mco_atomic *A, const mco_grains *G, int nsMax,
                             double Hden, double *Te, double *Ne, double *ionDen, int *nT, double ndust,
                             const double *Tdust, double *gdc, double *peh, double *pot);
 
+/* The module data emissionDriver reads for the gas re-emission of a cell
+ * (set up by the reference's ph_mod, hydro_mod and readData): the log gamma
+ * tables of H I, He I and He II (ntk, nlim) column-major with their edge
+ * bins, the free-free Gaunt factors, the ground-state cross-sections from
+ * the Milne start bins (sHI from ipHI, sHeI from i0HeI, sHeII from i0HeII),
+ * the heavy-ion free-bound channels of fb_ff ([nfb][4] elem, ion, IPnuP,
+ * highNuP; g0/g1; cross-sections over IPnuP..highNuP-1, channels
+ * concatenated), the He I two-photon table, the recombination-line data and
+ * the bins of the He I 2s triplet and the He II Lyman lines. */
+typedef struct {
+    int nb, ntk, nHI, nHeI, nHeII, ipHI, ipHeI, ipHeII, i0HeI, i0HeII, nfb, nly, j2TsP;
+    const float *nu, *widFlx, *tk, *gHI, *gHeI, *gHeII;
+    const int32_t *eHI, *eHeI, *eHeII;
+    const float *gff, *gffHeII, *sHI, *sHeI, *sHeII;
+    const int32_t *fbm;
+    const float *fbg, *fbs;
+    const float *ydat, *Aydat, *lyData;
+    const int32_t *lyP;
+    const float *HeIc;                   /* HeIrecLineCoeff(34,3,4) */
+    const double *HIrec, *HeIIrec;       /* HIRecLineData(3:30,2:8), HeIIRecLineData(3:30,2:16) */
+} mco_emission;
+
+/* The grains of one cell for setDiffusePDF's dust emission (one chemistry):
+ * absx [nS][nSizes][nbins], grainWeight, grainAbun, TdustSublime per
+ * species, the cell's Tdust(0:nsMax, 0:nSizes) block, Ndust and Hden. */
+typedef struct {
+    int nS, nSizes, nsMax;
+    const float *absx, *weight, *gabun, *tsub, *Tdust;
+    float Ndust, Hden;
+} mco_emission_dust;
+
+/* emissionDriver's gas part for one cell: its recPDF row rec[nb] and
+ * totalLines (D: the cell's grains, or NULL); ion[(stage-1)*nElemUsed + xref-1].  _f32: the reference's
+ * precision throughout; _f64: the level populations in FP64 (the GPU's). */
+int mco_emission_cell_f32(const mco_atomic *A, const mco_emission *E, const mco_emission_dust *D, float Te,
+                          float Ne, const float *ion, float *rec, float *totalLines);
+int mco_emission_cell_f64(const mco_atomic *A, const mco_emission *E, const mco_emission_dust *D, float Te,
+                          float Ne, const float *ion, float *rec, float *totalLines);
+
 /* outputGas's analytic forbidden-line luminosities forbVol of every model ion
  * and HbetaVol (output_mod.f90:302-520) of n cells, accumulated into forb
  * [ncel][17][17] (model ion, upper level, lower level) and *hbeta. */

@@ -226,6 +226,43 @@ class _Grains(C.Structure):
                 ("aweight", C.c_void_p)]
 
 
+class _Emission(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("nb", "ntk", "nHI", "nHeI", "nHeII", "ipHI", "ipHeI", "ipHeII", "i0HeI",
+                                       "i0HeII", "nfb", "nly", "j2TsP")] + \
+               [(n, C.c_void_p) for n in ("nu", "widFlx", "tk", "gHI", "gHeI", "gHeII", "eHI", "eHeI", "eHeII",
+                                          "gff", "gffHeII", "sHI", "sHeI", "sHeII", "fbm", "fbg", "fbs", "ydat",
+                                          "Aydat", "lyData", "lyP", "HeIc", "HIrec", "HeIIrec")]
+
+
+class _EmissionDust(C.Structure):
+    _fields_ = [("nS", C.c_int), ("nSizes", C.c_int), ("nsMax", C.c_int)] + \
+               [(n, C.c_void_p) for n in ("absx", "weight", "gabun", "tsub", "Tdust")] + \
+               [("Ndust", C.c_float), ("Hden", C.c_float)]
+
+
+def emission_tables(g):
+    """(mco_emission, arrays kept alive) from a fixture's em_* arrays."""
+    ei = [int(v) for v in g["em_ints"]]
+    keep = {}
+
+    def a(name, dt):
+        keep[name] = np.ascontiguousarray(g[name], dt)
+        return keep[name].ctypes.data
+    E = _Emission()
+    E.nb = int(g["ints"][4])
+    E.ntk, E.nHI, E.nHeI, E.nHeII, E.ipHI, E.ipHeI, E.ipHeII, E.i0HeI, E.i0HeII, E.nfb, E.nly, E.j2TsP = ei[:12]
+    E.nu, E.widFlx, E.tk = a("nuArray", np.float32), a("widFlx", np.float32), a("em_tk", np.float32)
+    E.gHI, E.gHeI, E.gHeII = a("em_gHI", np.float32), a("em_gHeI", np.float32), a("em_gHeII", np.float32)
+    E.eHI, E.eHeI, E.eHeII = a("em_eHI", np.int32), a("em_eHeI", np.int32), a("em_eHeII", np.int32)
+    E.gff, E.gffHeII = a("em_gff", np.float32), a("em_gffHeII", np.float32)
+    E.sHI, E.sHeI, E.sHeII = a("em_sHI", np.float32), a("em_sHeI", np.float32), a("em_sHeII", np.float32)
+    E.fbm, E.fbg, E.fbs = a("em_fbm", np.int32), a("em_fbg", np.float32), a("em_fbs", np.float32)
+    E.ydat, E.Aydat, E.lyData = a("em_ydat", np.float32), a("em_Aydat", np.float32), a("em_lyData", np.float32)
+    E.lyP, E.HeIc = a("em_lyP", np.int32), a("em_HeIc", np.float32)
+    E.HIrec, E.HeIIrec = a("em_HIrec", np.float64), a("em_HeIIrec", np.float64)
+    return E, keep
+
+
 class Balance:
     """FP32 (the reference's precision) or FP64 restatement of updateCell
     for every cell of a grid, given an atomic bundle (mocassin_amd.atomic)."""
@@ -315,6 +352,49 @@ class Balance:
         hb = np.zeros(1, dt)
         f(C.byref(self.A), len(cells), *(a.ctypes.data for a in args), forb.ctypes.data, hb.ctypes.data)
         return forb, float(hb[0])
+
+    def emission(self, E, Te, Ne, ionDen, cells, precision="f32", dust=None):
+        """emissionDriver's gas part: (recPDF rows [len(cells), nbins],
+        totalLines [len(cells)]) of `cells` from the REAL state Te/Ne
+        (nCells+1,), ionDen (nCells+1, nElemUsed, nstages).  dust: dict
+        absx [nS][nSizes][nbins], weight, gabun, tsub, Tdust (nCells+1,
+        nSizes+1, nsMax+1), Ndust, Hden (nCells+1,), or None."""
+        L = lib()
+        if not hasattr(L, "_em_declared"):
+            for name in ("mco_emission_cell_f32", "mco_emission_cell_f64"):
+                f = getattr(L, name)
+                f.restype = C.c_int
+                f.argtypes = [C.POINTER(_Atomic), C.POINTER(_Emission), C.c_void_p, C.c_float, C.c_float,
+                              C.c_void_p, C.c_void_p, C.c_void_p]
+            L._em_declared = True
+        f = L.mco_emission_cell_f32 if precision == "f32" else L.mco_emission_cell_f64
+        cells = np.asarray(cells, np.int64)
+        rec = np.zeros((len(cells), E.nb), np.float32)
+        tl = np.zeros(len(cells), np.float32)
+        ion = np.asarray(ionDen, np.float32)
+        D, dk = None, {}
+        if dust is not None:
+            for k in ("absx", "weight", "gabun", "tsub"):
+                dk[k] = np.ascontiguousarray(dust[k], np.float32)
+            Td = np.asarray(dust["Tdust"], np.float32)
+            D = _EmissionDust()
+            D.nS, D.nSizes = dk["absx"].shape[0], dk["absx"].shape[1]
+            D.nsMax = Td.shape[2] - 1
+            D.absx, D.weight = dk["absx"].ctypes.data, dk["weight"].ctypes.data
+            D.gabun, D.tsub = dk["gabun"].ctypes.data, dk["tsub"].ctypes.data
+        for r, c in enumerate(cells):
+            sl = np.ascontiguousarray(ion[c].T)          # [stage][elem]
+            t = np.zeros(1, np.float32)
+            row = np.zeros(E.nb, np.float32)
+            if D is not None:
+                blk = np.ascontiguousarray(Td[c])          # [size][species]
+                D.Tdust = blk.ctypes.data
+                D.Ndust, D.Hden = float(np.float32(dust["Ndust"][c])), float(np.float32(dust["Hden"][c]))
+            if f(C.byref(self.A), C.byref(E), None if D is None else C.byref(D), float(np.float32(Te[c])),
+                 float(np.float32(Ne[c])), sl.ctypes.data, row.ctypes.data, t.ctypes.data):
+                raise RuntimeError("coincident gamma edges")
+            rec[r], tl[r] = row, t[0]
+        return rec, tl
 
     def run_grains(self, J, Hden, Te, Ne, ionDen, g, Tdust, precision="f64", cells=None):
         """run() with the grain charge, photoelectric and gas-grain collision

@@ -108,6 +108,7 @@ program mcref_harness
        call run_transport()
        call run_balance()
        call run_lines()
+       call run_emission()
        call run_opacity()
     case ("walk")
        call get_command_argument(5, arg); read(arg, *) nuBin
@@ -543,6 +544,100 @@ contains
        close(86)
        deallocate(L)
     end subroutine run_lines
+
+    ! The re-emission tables of the state after the balance pass: the module
+    ! tables emissionDriver reads (ph_mod/hydro_mod/readData setup), then
+    ! emissionDriver for every cell in iterateMC's order (iteration_mod.f90:
+    ! 229-240) and the resulting recPDF and totalLines.
+    subroutine run_emission()
+       integer :: nc, elem, ion, nElec, outShell, g0, g1, hiP, ipP, xsP, nfb, i0HeI, i0HeII, ii, n
+       integer :: ints(16), lyP(NHeIILyman)
+       integer, allocatable :: fbm(:, :)
+       real, allocatable :: fbg(:), fbs(:), sHI(:), sHeI(:), sHeII(:)
+       nc = grid3D(1)%nCells
+       i0HeI = max(HlevNuP(1), HeINuEdgeP(nlimGammaHeI))
+       i0HeII = max(i0HeI, HeIINuEdgeP(nlimGammaHeII))
+       ! heavy-ion free-bound channels in fb_ff's loop order (emission_mod.f90:473-521)
+       allocate(fbm(4, 200), fbg(200), fbs(200*nbins))
+       nfb = 0
+       n = 0
+       do elem = 3, 19
+          do ion = 1, nstages-1
+             if (.not.lgElementOn(elem)) exit
+             if (ion > elem) exit
+             nElec = elem - ion + 1
+             call getOuterShell(elem, nElec, outShell, g0, g1)
+             hiP = elementP(elem, ion, outShell, 2)
+             ipP = elementP(elem, ion, outShell, 1)
+             xsP = elementP(elem, ion, outShell, 3)
+             nfb = nfb + 1
+             fbm(:, nfb) = (/elem, ion, ipP, hiP/)
+             fbg(nfb) = real(g0)/real(g1)
+             do ii = ipP, hiP-1
+                n = n + 1
+                fbs(n) = xSecArray(ii+xsP-ipP+1-1)
+             end do
+          end do
+       end do
+       allocate(sHI(nbins-HlevNuP(1)+1), sHeI(nbins-i0HeI+1), sHeII(nbins-i0HeII+1))
+       do ii = HlevNuP(1), nbins
+          sHI(ii-HlevNuP(1)+1) = xSecArray(ii-HlevNuP(1)+1+HlevXSecP(1)-1)
+       end do
+       do ii = i0HeI, nbins
+          sHeI(ii-i0HeI+1) = xSecArray(ii-HeIlevNuP(1)+1+HeISingXSecP(1)-1)
+       end do
+       do ii = i0HeII, nbins
+          sHeII(ii-i0HeII+1) = xSecArray(ii-HeIIlevNuP(1)+1+HeIIXSecP(1)-1)
+       end do
+       ints = 0
+       ints(1) = ntkGamma; ints(2) = nlimGammaHI; ints(3) = nlimGammaHeI; ints(4) = nlimGammaHeII
+       ints(5) = HlevNuP(1); ints(6) = HeIlevNuP(1); ints(7) = HeIIlevNuP(1)
+       ints(8) = i0HeI; ints(9) = i0HeII; ints(10) = nfb; ints(11) = NHeIILyman
+       call locate(nuArray, 1.45673, ints(12))
+       do ii = 1, NHeIILyman
+          call locate(nuArray, HeIILymanNu(ii), lyP(ii))
+       end do
+       call dump_i4('em_ints', ints, 16)
+       call dump_i4('em_lyP', lyP, NHeIILyman)
+       call dump_r4('em_tk', tkGamma, ntkGamma)
+       call dump_r4('em_gHI', logGammaHI, ntkGamma*nlimGammaHI)
+       call dump_r4('em_gHeI', logGammaHeI, ntkGamma*nlimGammaHeI)
+       call dump_r4('em_gHeII', logGammaHeII, ntkGamma*nlimGammaHeII)
+       call dump_i4('em_eHI', HINuEdgeP, nlimGammaHI)
+       call dump_i4('em_eHeI', HeINuEdgeP, nlimGammaHeI)
+       call dump_i4('em_eHeII', HeIINuEdgeP, nlimGammaHeII)
+       call dump_r4('em_gff', gauntFF, nbins)
+       call dump_r4('em_gffHeII', gauntFFHeII, nbins)
+       call dump_r4('em_sHI', sHI, size(sHI))
+       call dump_r4('em_sHeI', sHeI, size(sHeI))
+       call dump_r4('em_sHeII', sHeII, size(sHeII))
+       call dump_i4('em_fbm', fbm, 4*nfb)
+       call dump_r4('em_fbg', fbg, nfb)
+       call dump_r4('em_fbs', fbs, max(n, 1))
+       call dump_r4('em_ydat', y_dat, 41)
+       call dump_r4('em_Aydat', Ay_dat, 41)
+       call dump_r4('em_lyData', HeIILymanData, NHeIILyman)
+       call dump_r4('em_lyNu', HeIILymanNu, NHeIILyman)
+       call dump_r4('em_HeIc', HeIrecLineCoeff, 34*3*4)
+       open(unit=86, file=trim(outdir)//'/em_HIrec.bin', access='stream', form='unformatted', status='replace')
+       write(86) HIRecLineData
+       close(86)
+       open(unit=86, file=trim(outdir)//'/em_HeIIrec.bin', access='stream', form='unformatted', status='replace')
+       write(86) HeIIRecLineData
+       close(86)
+       grid3D(1)%recPDF(0:nc, 1:nbins) = 0.
+       grid3D(1)%totalLines(0:nc) = 0.
+       do i = 1, grid3D(1)%nz
+          do j = 1, grid3D(1)%ny
+             do k = 1, grid3D(1)%nx
+                call emissionDriver(grid3D, k, j, i, 1)
+             end do
+          end do
+       end do
+       call dump_r4('em_recPDF', grid3D(1)%recPDF, (nc+1)*nbins)
+       call dump_r4('em_totalLines', grid3D(1)%totalLines, nc+1)
+       deallocate(fbm, fbg, fbs, sHI, sHeI, sHeII)
+    end subroutine run_emission
 
     ! scattering cross-sections of every species and size on nuArray and the
     ! sublimation temperatures (the dust add of iteration_mod.f90:184-222)

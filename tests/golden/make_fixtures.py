@@ -195,6 +195,18 @@ def make_opacity(deck, npk, seed, name, keep_cells=96):
     J = rd("Jste_scaled")
     nzJ = np.nonzero(J)[0]
     out.update(J_idx=nzJ.astype(np.int32), J_val=J[nzJ])
+    # re-emission tables of the balanced state and the module data they use
+    ei = rd("em_ints", np.int32)
+    nfb = int(ei[9])
+    out.update(em_ints=ei, em_lyP=rd("em_lyP", np.int32), em_tk=rd("em_tk"), em_gHI=rd("em_gHI"),
+               em_gHeI=rd("em_gHeI"), em_gHeII=rd("em_gHeII"), em_eHI=rd("em_eHI", np.int32),
+               em_eHeI=rd("em_eHeI", np.int32), em_eHeII=rd("em_eHeII", np.int32), em_gff=rd("em_gff"),
+               em_gffHeII=rd("em_gffHeII"), em_sHI=rd("em_sHI"), em_sHeI=rd("em_sHeI"), em_sHeII=rd("em_sHeII"),
+               em_fbm=rd("em_fbm", np.int32)[:4 * nfb], em_fbg=rd("em_fbg"), em_fbs=rd("em_fbs"),
+               em_ydat=rd("em_ydat"), em_Aydat=rd("em_Aydat"), em_lyData=rd("em_lyData"), em_lyNu=rd("em_lyNu"),
+               em_HeIc=rd("em_HeIc"), em_HIrec=rd("em_HIrec", np.float64), em_HeIIrec=rd("em_HeIIrec", np.float64),
+               widFlx=rd("widFlx"), em_totalLines=rd("em_totalLines"),
+               em_recPDF=pick(rd("em_recPDF")))
     if os.path.exists(os.path.join(d, "op_total.bin")):
         out.update(Tdust=rd("op_Tdust"), op_total=pick(rd("op_total")), op_sca=pick(rd("op_sca")),
                    dust_ints=rd("dust_ints", np.int32), dust_abs=rd("dust_abs"), dust_sca=rd("dust_sca"),
