@@ -121,6 +121,27 @@ def lines_cpu_baseline(bundle, Hden, Te, Ne, ion, dV, cells=500):
             "sample": "%d cells of the workload's state after the balance, scalar C restatement" % n}
 
 
+def emission_cpu_baseline(atom, bundle, nb, nu, Te, Ne, ion, cells=200):
+    """emissionDriver's gas part in the reference's precision (the C
+    restatement, oracle mco_emission_cell_f32) on the first `cells` cells of
+    the same state, one thread: cells per second."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import mc_oracle as O
+    import numpy as np
+    n = min(cells, Te.size - 1)
+    g = dict(atom)
+    g["ints"] = np.array([0, 0, 0, n, nb], np.int32)
+    g["nuArray"] = nu
+    E, keep = O.emission_tables(g)
+    B = O.Balance(bundle)
+    t = time.perf_counter()
+    B.emission(E, np.asarray(Te, np.float32), np.asarray(Ne, np.float32), np.asarray(ion, np.float32),
+               np.arange(1, n + 1), "f32")
+    dt = time.perf_counter() - t
+    return {"value": n / dt, "unit": "cells/s", "cores": 1, "kind": "port",
+            "sample": "%d cells of the workload's state after the balance, scalar C restatement" % n}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,6 +154,7 @@ def main():
     ap.add_argument("--no-balance", action="store_true", help="skip the balance measurement")
     ap.add_argument("--no-grains", action="store_true", help="skip the balance with grains")
     ap.add_argument("--no-lines", action="store_true", help="skip the line luminosities")
+    ap.add_argument("--no-emission", action="store_true", help="skip the re-emission tables")
     ap.add_argument("--no-opacity", action="store_true", help="skip the opacity-table build")
     ap.add_argument("--autotune", action="store_true",
                     help="time the launch-shape candidates on this workload and store the best "
@@ -263,6 +285,18 @@ def main():
             balance["lines"] = lines
             print("bench: line luminosities %.3f ms (%d model ions, %d lines above 1e-3 Hbeta)"
                   % (ms_l, bundle.ncel, lines["lines_above_1e-3_hbeta"]), file=sys.stderr, flush=True)
+        if not a.no_emission and "em_ints" in atom:
+            # the re-emission tables of the balanced state, built where it lies
+            from mocassin_amd.atomic import emission_tables
+            T.emission_setup(**emission_tables(atom))
+            ms_e = min(T.emission_build(host=False)[2] for _ in range(3))
+            em = {"device_ms": ms_e, "cells_per_s": p["nCells"] / (ms_e * 1e-3),
+                  "kernel": "emission_kernel", "free_bound_channels": int(atom["em_ints"][9])}
+            if not a.no_cpu:
+                em["cpu_baseline"] = emission_cpu_baseline(atom, bundle, p["nbins"], p["nuArray"], r[0], r[1], r[2])
+                em["gpu_over_cpu"] = em["cells_per_s"] / em["cpu_baseline"]["value"]
+            balance["emission"] = em
+            print("bench: re-emission tables %.3f ms (%d cells)" % (ms_e, p["nCells"]), file=sys.stderr, flush=True)
         if not a.no_grains:
             # the same balance with silicate grains, their charge and the
             # photoelectric and collision terms, then the grain temperatures

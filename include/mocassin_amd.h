@@ -385,6 +385,72 @@ int mc_opacity_setup(mc_ctx *ctx, const mc_opacity_tables *tables);
 int mc_opacity_build(mc_ctx *ctx, const float *Hden, const float *Te, const float *Ne, const float *ionDen,
                      const float *Tdust, float *opacity, float *scaOpac, double *kernel_ms);
 
+/* The module data emissionDriver reads for the gas re-emission (set up by
+ * the reference's ph_mod, hydro_mod and readData; emission_mod.f90:32-1311).
+ * Bins are 1-based; (a, b) tables are column-major as the host holds them.
+ *   tkGamma(ntk), logGammaHI/HeI/HeII(ntk, nHI/nHeI/nHeII) with the edge
+ *   bins HINuEdgeP/HeINuEdgeP/HeIINuEdgeP (fb_ff);
+ *   gauntFF, gauntFFHeII (nbins);
+ *   HIPnuP, HeIPnuP, HeIIPnuP = HlevNuP(1), HeIlevNuP(1), HeIIlevNuP(1), and
+ *   the ground-state cross-sections the Milne relation reads, from its first
+ *   bin to nbins: xSecHI(i) = xSecArray(i - HIPnuP + HlevXSecP(1)) for i >=
+ *   HIPnuP; xSecHeI(i) = xSecArray(i - HeIPnuP + HeISingXSecP(1)) for i >=
+ *   max(HIPnuP, HeINuEdgeP(nHeI)); xSecHeII(i) = xSecArray(i - HeIIPnuP +
+ *   HeIIXSecP(1)) for i >= max(that, HeIINuEdgeP(nHeII));
+ *   the heavy-ion free-bound channels in fb_ff's loop order (elements 3..19
+ *   switched on, ion 1..min(Z, nstages-1)): fb_meta[nfb][4] = Z, ion,
+ *   IPnuP, highNuP (elementP(Z, ion, outShell, 1:2)), fb_gratio[nfb] =
+ *   real(g0)/real(g1) (getOuterShell), fb_sigma = the cross-sections of
+ *   bins IPnuP..highNuP-1 of every channel, concatenated;
+ *   y_dat, Ay_dat (41), HIRecLineData(3:30, 2:8), HeIIRecLineData(3:30,
+ *   2:16), HeIrecLineCoeff(34, 3, 4), HeIILymanData and HeIILymanNu
+ *   (nHeIILyman = 4), widFlx (nbins). */
+typedef struct {
+    int32_t ntk, nHI, nHeI, nHeII;
+    const float *tkGamma, *logGammaHI, *logGammaHeI, *logGammaHeII;
+    const int32_t *HINuEdgeP, *HeINuEdgeP, *HeIINuEdgeP;
+    const float *gauntFF, *gauntFFHeII;
+    int32_t HIPnuP, HeIPnuP, HeIIPnuP;
+    const float *xSecHI, *xSecHeI, *xSecHeII;
+    int32_t nfb;
+    const int32_t *fb_meta;
+    const float *fb_gratio, *fb_sigma;
+    const float *y_dat, *Ay_dat;
+    const double *HIRecLineData, *HeIIRecLineData;
+    const float *HeIrecLineCoeff;
+    int32_t nHeIILyman;
+    const float *HeIILymanData, *HeIILymanNu;
+    const float *widFlx;
+} mc_emission_tables;
+
+/* Upload the re-emission tables (validated; after mc_balance_setup, whose
+ * model ions give the forbidden lines).  With grains (mc_dust_setup with
+ * TdustSublime), setDiffusePDF's dust term is included.  Replaces: the
+ * module data emissionDriver reads (emission_mod.f90:32-1311). */
+int mc_emission_setup(mc_ctx *ctx, const mc_emission_tables *tables);
+
+/* The re-emission tables of every active cell on the device, written into
+ * the transport's own recPDF and totalLines (the next sweep uses them; pass
+ * recPDF = totalLines = NULL to mc_upload_tables).  Per cell, emissionDriver's
+ * gas part: free-bound and free-free continua of H I, He I, He II and the
+ * heavy ions (fb_ff, free_free), the two-photon continua (twoPhoton), the He
+ * I 2s triplet and He II Lyman lines able to ionise H, the recombination and
+ * forbidden lines (RecLinesEmission, forLines) and setDiffusePDF's sums; with
+ * grains, their thermal emission.  The sums are REALs in the reference's
+ * order, so recPDF follows its rounding; the forbidden lines' level
+ * populations are solved in FP64, and the grains' total is taken in bin
+ * order.  The state is Hden, Te, Ne (0:nCells), ionDen(0:nCells,
+ * 1:nElemUsed, 1:nstages) and Tdust (0:nSpeciesMax, 0:nSizes, 0:nCells) as
+ * the host holds them (REAL); NULL state (and Tdust) take the device state of
+ * the last mc_update_cells / mc_balance_run, rounded to FP32.  recPDF
+ * (0:nCells, 1:nbins) and totalLines (0:nCells), nullable: host copies of
+ * the active cells' rows (row 0 is left as it is).  MC_ETABLE names a cell
+ * whose continuum has no positive total.  kernel_ms (nullable): device
+ * time.  Replaces: emissionDriver per cell in iterateMC (iteration_mod.f90:
+ * 229-240; emission_mod.f90:32-173, 253-903, 905-1311). */
+int mc_emission_build(mc_ctx *ctx, const float *Hden, const float *Te, const float *Ne, const float *ionDen,
+                      const float *Tdust, float *recPDF, float *totalLines, double *kernel_ms);
+
 /* The analytic line luminosities of outputGas for a 3-D grid (output_mod.f90:
  * 302-520), summed over every active cell that is not black: per cell,
  * forLines (1946-2008: each model ion's level populations from equilibrium,

@@ -22,7 +22,7 @@ EXPORTS = ("mc_ctx_create", "mc_upload_tables", "mc_zero_estimators", "mc_transp
            "mc_mlp_set", "mc_mlp_get", "mc_mlp_features", "mc_mlp_predict", "mc_mlp_loss_grad",
            "mc_mlp_train", "mc_count_crossings", "mc_fetch_crossings", "mc_transport_batch",
            "mc_dust_setup", "mc_update_cells", "mc_opacity_setup", "mc_opacity_build",
-           "mc_line_luminosities",
+           "mc_line_luminosities", "mc_emission_setup", "mc_emission_build",
            "mc_ctx_last_error", "mc_ctx_destroy")
 
 MC_SOLVER_REFERENCE, MC_SOLVER_SECANT = 0, 1
@@ -83,6 +83,17 @@ class McBalanceOpts(C.Structure):
                 ("first_slope", C.c_double), ("warm_iter", C.c_int32)]
 
 
+class McEmissionTables(C.Structure):
+    _fields_ = [("ntk", C.c_int32), ("nHI", C.c_int32), ("nHeI", C.c_int32), ("nHeII", C.c_int32)] + \
+               [(n, C.c_void_p) for n in ("tkGamma", "logGammaHI", "logGammaHeI", "logGammaHeII", "HINuEdgeP",
+                                          "HeINuEdgeP", "HeIINuEdgeP", "gauntFF", "gauntFFHeII")] + \
+               [("HIPnuP", C.c_int32), ("HeIPnuP", C.c_int32), ("HeIIPnuP", C.c_int32)] + \
+               [(n, C.c_void_p) for n in ("xSecHI", "xSecHeI", "xSecHeII")] + [("nfb", C.c_int32)] + \
+               [(n, C.c_void_p) for n in ("fb_meta", "fb_gratio", "fb_sigma", "y_dat", "Ay_dat", "HIRecLineData",
+                                          "HeIIRecLineData", "HeIrecLineCoeff")] + \
+               [("nHeIILyman", C.c_int32)] + [(n, C.c_void_p) for n in ("HeIILymanData", "HeIILymanNu", "widFlx")]
+
+
 class McDustTables(C.Structure):
     _fields_ = [("nSpecies", C.c_int32), ("nSpeciesMax", C.c_int32), ("nSizes", C.c_int32),
                 ("nTemps", C.c_int32), ("photoelectric", C.c_int32), ("absXsec", C.c_void_p),
@@ -139,6 +150,8 @@ def load():
     L.mc_opacity_setup.argtypes = [vp, C.POINTER(McOpacityTables)]
     L.mc_opacity_build.argtypes = [vp] * 9
     L.mc_line_luminosities.argtypes = [vp] * 9
+    L.mc_emission_setup.argtypes = [vp, C.POINTER(McEmissionTables)]
+    L.mc_emission_build.argtypes = [vp] * 9
     L.mc_count_crossings.argtypes = [vp, i32]
     L.mc_fetch_crossings.argtypes = [vp, vp]
     L.mc_ctx_last_error.argtypes = [vp]

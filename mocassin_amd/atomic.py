@@ -107,3 +107,21 @@ class AtomicBundle:
                    ap_int=rd("ap_int", np.int32), ap_real=rd("ap_real"),
                    cel_meta=rd("cel_meta", np.int32).reshape(ncel, 6),
                    cel_dbl=rd("cel_dbl", np.float64), cel_flt=rd("cel_flt"))
+
+
+def emission_tables(d):
+    """Keyword arguments of Transport.emission_setup (mc_emission_tables)
+    from arrays saved with an em_ prefix: the module data emissionDriver
+    reads, as the reference's setup leaves them (tkGamma, the log gamma
+    tables and their edge bins, Gaunt factors, the Milne cross-sections from
+    their first bins, the heavy-ion free-bound channels, the two-photon and
+    recombination-line data) and widFlx."""
+    ei = d["em_ints"]
+    nfb = int(ei[9])
+    return dict(tkGamma=d["em_tk"], logGammaHI=d["em_gHI"], logGammaHeI=d["em_gHeI"], logGammaHeII=d["em_gHeII"],
+                HINuEdgeP=d["em_eHI"], HeINuEdgeP=d["em_eHeI"], HeIINuEdgeP=d["em_eHeII"], gauntFF=d["em_gff"],
+                gauntFFHeII=d["em_gffHeII"], HIPnuP=int(ei[4]), HeIPnuP=int(ei[5]), HeIIPnuP=int(ei[6]),
+                xSecHI=d["em_sHI"], xSecHeI=d["em_sHeI"], xSecHeII=d["em_sHeII"], fb_meta=d["em_fbm"][:4 * nfb],
+                fb_gratio=d["em_fbg"][:max(nfb, 1)], fb_sigma=d["em_fbs"], y_dat=d["em_ydat"], Ay_dat=d["em_Aydat"],
+                HIRecLineData=d["em_HIrec"], HeIIRecLineData=d["em_HeIIrec"], HeIrecLineCoeff=d["em_HeIc"],
+                HeIILymanData=d["em_lyData"], HeIILymanNu=d["em_lyNu"], widFlx=d["widFlx"])

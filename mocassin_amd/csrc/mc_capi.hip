@@ -15,6 +15,7 @@
 #include "mc_balance.h"
 #include "mc_device.h"
 #include "mc_dust.h"
+#include "mc_emission.h"
 #include "mc_kernels.h"
 #include "mc_mlp.h"
 #include "mc_opacity.h"
@@ -89,6 +90,12 @@ struct mc_ctx {
     double *d_ln_part = nullptr, *d_ln_st = nullptr, *d_ln_out = nullptr;
     size_t ln_part_n = 0;
     int *d_ln_black = nullptr;
+    // re-emission tables on the device
+    bool emis = false, rec_built = false;
+    int *d_em_i = nullptr, *d_em_err = nullptr;
+    float *d_em_f = nullptr;
+    double *d_em_d = nullptr, *d_em_st = nullptr, *d_em_Td = nullptr;
+    mce::EmArgs em{};
     char err[512] = "";
 };
 
@@ -257,7 +264,9 @@ extern "C" int mc_upload_tables(mc_ctx *c, const float *opacity, const float *sc
     if (!specCDF) return fail(c, MC_EINVAL, "specCDF is required");
     if (!opacity && !c->opac_built)
         return fail(c, MC_EINVAL, "opacity is required (no tables on the device yet)");
-    if (gas && (!recPDF || !totalLines)) return fail(c, MC_EINVAL, "gas needs recPDF and totalLines");
+    if (gas && (!recPDF || !totalLines) && !(!recPDF && !totalLines && c->rec_built))
+        return fail(c, MC_EINVAL, "gas needs recPDF and totalLines (no re-emission tables on the device yet)");
+    const bool rec_host = gas && recPDF;
     if (dust && !scaOpac && !c->opac_sca_built)
         return fail(c, MC_EINVAL, "dust needs scaOpac (no scattering table on the device yet)");
     if (dust && !(G.flags & MC_ISOTROPIC) && !gSca) return fail(c, MC_EINVAL, "HG scattering needs gSca");
@@ -270,7 +279,7 @@ extern "C" int mc_upload_tables(mc_ctx *c, const float *opacity, const float *sc
         if (!cdf_ok(specCDF + (size_t)s * nb, 1, nb))
             return fail(c, MC_ETABLE, "specCDF of source %d not a monotone CDF", s + 1);
     std::vector<float> rec;
-    if (gas) {
+    if (rec_host) {
         rec.resize(nt);
         for (int cell = 0; cell <= nc; ++cell) {
             if (cell > 0 && !cdf_ok(recPDF + cell, (size_t)(nc + 1), nb))
@@ -309,7 +318,7 @@ extern "C" int mc_upload_tables(mc_ctx *c, const float *opacity, const float *sc
     (void)rc;
     if (!c->d_specCDF) HIPCHK(c, dalloc(&c->d_specCDF, (size_t)nb * c->nStars));
     HIPCHK(c, hipMemcpy(c->d_specCDF, specCDF, (size_t)nb * c->nStars * sizeof(float), hipMemcpyHostToDevice));
-    if (gas) {
+    if (rec_host) {
         if (!c->d_recCDF) HIPCHK(c, dalloc(&c->d_recCDF, nt));
         if (!c->d_recGuide) HIPCHK(c, dalloc(&c->d_recGuide, (size_t)(nc + 1) * (mcd::kGuide + 1)));
         if (!c->d_totalLines) HIPCHK(c, dalloc(&c->d_totalLines, (size_t)nc + 1));
@@ -655,8 +664,11 @@ extern "C" int mc_dust_setup(mc_ctx *c, const mc_dust_tables *d)
     if (c->d_op_dust) { (void)hipFree(c->d_op_dust); c->d_op_dust = nullptr; }
     if (d->scaXsec) {
         if (!d->TdustSublime || !d->Ndust) return fail(c, MC_EINVAL, "scaXsec needs TdustSublime and Ndust");
-        std::vector<float> t((size_t)2 * K * nb + 2 * K + nc1);
+        std::vector<float> t((size_t)2 * K * nb + 2 * K + nc1 + nSz + nS);
         float *sc = t.data(), *ab_ = sc + (size_t)K * nb, *dw = ab_ + (size_t)K * nb, *ts = dw + K, *nd = ts + K;
+        float *gw = nd + nc1, *ga = gw + nSz;       // grainWeight, grainAbun (re-emission)
+        for (int ai = 0; ai < nSz; ++ai) gw[ai] = d->grainWeight[ai];
+        for (int s_ = 0; s_ < nS; ++s_) ga[s_] = d->grainAbun[s_];
         for (size_t i = 0; i < (size_t)K * nb; ++i) {
             if (!std::isfinite(d->scaXsec[i])) return fail(c, MC_ETABLE, "scaXsec entry %zu not finite", i);
             sc[i] = d->scaXsec[i];
@@ -1028,6 +1040,281 @@ extern "C" int mc_opacity_build(mc_ctx *c, const float *Hden, const float *Te, c
     }
     c->opac_built = true;
     c->opac_sca_built = dust;
+    return MC_OK;
+}
+
+// ------------------------------------------------------- re-emission tables
+
+// locate (interpolation_mod.f90:48-81) in the increasing frequency mesh
+static int locate_nu(const std::vector<float> &xa, float x)
+{
+    const int n = (int)xa.size();
+    if (x > xa[n - 1]) return n;
+    if (x < xa[0]) return 0;
+    int p = 0;
+    float bd = 0.f;
+    for (int i = 0; i < n; ++i) {
+        const float d = xa[i] - x;
+        if (d > 0.f && (p == 0 || d < bd)) { p = i + 1; bd = d; }
+    }
+    return std::max(p - 1, 1);
+}
+
+extern "C" int mc_emission_setup(mc_ctx *c, const mc_emission_tables *t)
+{
+    if (!c) return MC_EINVAL;
+    if (!t) return fail(c, MC_EINVAL, "no tables");
+    if (!c->balance) return fail(c, MC_ESTATE, "mc_balance_setup has not been called");
+    const mcb::Atomic &A = c->bal.A;
+    const int nb = c->G.nbins;
+    int eH = -1, eHe = -1;
+    for (int e = 0; e < A.nE; ++e) {
+        if (A.el[e].z == 1) eH = e;
+        if (A.el[e].z == 2) eHe = e;
+    }
+    if (eH != 0 || eHe < 0 || A.nstages < 3) return fail(c, MC_EUNSUP, "re-emission needs H and He (3 stages)");
+    if (mce_lds_bytes(nb, true) > 48 * 1024)
+        return fail(c, MC_EUNSUP, "re-emission keeps 4 x nbins partial sums in LDS: %d bins is too many", nb);
+    if (!t->tkGamma || !t->logGammaHI || !t->logGammaHeI || !t->logGammaHeII || !t->HINuEdgeP || !t->HeINuEdgeP ||
+        !t->HeIINuEdgeP || !t->gauntFF || !t->gauntFFHeII || !t->xSecHI || !t->xSecHeI || !t->xSecHeII ||
+        !t->y_dat || !t->Ay_dat || !t->HIRecLineData || !t->HeIIRecLineData || !t->HeIrecLineCoeff ||
+        !t->HeIILymanData || !t->HeIILymanNu || !t->widFlx || (t->nfb > 0 && (!t->fb_meta || !t->fb_gratio || !t->fb_sigma)))
+        return fail(c, MC_EINVAL, "a required table is NULL");
+    const int nl3[3] = { t->nHI, t->nHeI, t->nHeII };
+    if (t->ntk < 1) return fail(c, MC_ETABLE, "no gamma temperatures");
+    for (int i = 0; i + 1 < t->ntk; ++i)
+        if (!(t->tkGamma[i + 1] > t->tkGamma[i])) return fail(c, MC_ETABLE, "tkGamma not increasing");
+    const int32_t *edges[3] = { t->HINuEdgeP, t->HeINuEdgeP, t->HeIINuEdgeP };
+    const float *lg[3] = { t->logGammaHI, t->logGammaHeI, t->logGammaHeII };
+    std::vector<int> pairs((size_t)3 * nb, -1);
+    for (int s_ = 0; s_ < 3; ++s_) {
+        if (nl3[s_] < 2 || nl3[s_] > mce::kMaxLim || nl3[s_] % 2) return fail(c, MC_ETABLE, "gamma table %d: %d edges", s_, nl3[s_]);
+        for (size_t i = 0; i < (size_t)t->ntk * nl3[s_]; ++i)
+            if (!std::isfinite(lg[s_][i])) return fail(c, MC_ETABLE, "log gamma table %d not finite", s_);
+        for (int k = 0; k < nl3[s_] / 2; ++k) {
+            const int lo = edges[s_][2 * k], hi = edges[s_][2 * k + 1];
+            if (lo < 1 || hi > nb || hi < lo) return fail(c, MC_ETABLE, "gamma edge pair %d of series %d outside 1..%d", k, s_, nb);
+            if (hi == lo) return fail(c, MC_ETABLE, "frequency grid too coarse: gamma edges %d of series %d coincide", k, s_);
+            for (int j = lo; j <= hi - 1; ++j) pairs[(size_t)s_ * nb + j - 1] = k;
+        }
+    }
+    const int ip[3] = { t->HIPnuP, t->HeIPnuP, t->HeIIPnuP };
+    for (int s_ = 0; s_ < 3; ++s_)
+        if (ip[s_] < 1 || ip[s_] > nb) return fail(c, MC_ETABLE, "ionisation bin %d of series %d outside 1..%d", ip[s_], s_, nb);
+    const int i0[3] = { t->HIPnuP, std::max(t->HIPnuP, t->HeINuEdgeP[t->nHeI - 1]),
+                        std::max(std::max(t->HIPnuP, t->HeINuEdgeP[t->nHeI - 1]), t->HeIINuEdgeP[t->nHeII - 1]) };
+    const float *xs[3] = { t->xSecHI, t->xSecHeI, t->xSecHeII };
+    // heavy-ion free-bound channels
+    std::vector<int> fbi, fboff;
+    std::vector<float> fbf;
+    size_t nsig = 0;
+    for (int k = 0; k < t->nfb; ++k) {
+        const int32_t *m = t->fb_meta + 4 * k;
+        const int z = m[0], ion = m[1], ipk = m[2], hik = m[3];
+        int e = -1;
+        for (int q = 0; q < A.nE; ++q)
+            if (A.el[q].z == z) e = q;
+        if (e < 0 || ion < 1 || ion + 1 > A.nstages) return fail(c, MC_ETABLE, "free-bound channel %d: (Z=%d, ion %d) not switched on", k, z, ion);
+        if (ipk < 1 || hik > nb || hik < ipk) return fail(c, MC_ETABLE, "free-bound channel %d: bins %d..%d outside 1..%d", k, ipk, hik, nb);
+        fbi.insert(fbi.end(), { e, ion + 1, ipk, hik });
+        fboff.push_back((int)nsig);
+        fbf.push_back(t->fb_gratio[k]);
+        fbf.push_back((float)A.el[e].abun);
+        nsig += (size_t)(hik - ipk);
+    }
+    for (size_t i = 0; i < nsig; ++i)
+        if (!std::isfinite(t->fb_sigma[i])) return fail(c, MC_ETABLE, "free-bound cross-section %zu not finite", i);
+    if (t->nHeIILyman < 4 || t->nHeIILyman > mce::kMaxLy) return fail(c, MC_ETABLE, "nHeIILyman %d outside 4..%d", t->nHeIILyman, mce::kMaxLy);
+    // He I two-photon segments per bin (HeI2photSub's running index)
+    std::vector<int> jHeI(nb, 0);
+    {
+        int j = 1;
+        for (int i = 0; i < nb; ++i) {
+            const float y = c->nuArray[i] / 1.514f;
+            if (y > t->y_dat[j - 1]) j = j + 1;
+            if (j >= 41) break;
+            jHeI[i] = j;
+        }
+    }
+    const int j2TsP = locate_nu(c->nuArray, 1.45673f);
+    if (j2TsP < 1 || j2TsP > nb) return fail(c, MC_ETABLE, "1.45673 Ryd outside the frequency mesh");
+    int lyP[mce::kMaxLy];
+    for (int k = 0; k < t->nHeIILyman; ++k) {
+        lyP[k] = locate_nu(c->nuArray, t->HeIILymanNu[k]);
+        if (lyP[k] < 1 || lyP[k] > nb) return fail(c, MC_ETABLE, "He II Lyman line %d outside the frequency mesh", k + 1);
+    }
+    // device tables
+    std::vector<int> iv(pairs);
+    std::vector<size_t> io(8);
+    io[0] = iv.size(); for (int s_ = 0; s_ < 3; ++s_) iv.insert(iv.end(), edges[s_], edges[s_] + nl3[s_]);
+    io[1] = iv.size(); iv.insert(iv.end(), fbi.begin(), fbi.end());
+    io[2] = iv.size(); iv.insert(iv.end(), fboff.begin(), fboff.end());
+    io[3] = iv.size(); iv.insert(iv.end(), jHeI.begin(), jHeI.end());
+    std::vector<float> fv;
+    std::vector<size_t> fo(16);
+    fo[0] = fv.size(); fv.insert(fv.end(), t->tkGamma, t->tkGamma + t->ntk);
+    for (int s_ = 0; s_ < 3; ++s_) { fo[1 + s_] = fv.size(); fv.insert(fv.end(), lg[s_], lg[s_] + (size_t)t->ntk * nl3[s_]); }
+    fo[4] = fv.size(); fv.insert(fv.end(), t->gauntFF, t->gauntFF + nb);
+    fo[5] = fv.size(); fv.insert(fv.end(), t->gauntFFHeII, t->gauntFFHeII + nb);
+    for (int s_ = 0; s_ < 3; ++s_) { fo[6 + s_] = fv.size(); fv.insert(fv.end(), xs[s_], xs[s_] + (nb - i0[s_] + 1)); }
+    fo[9] = fv.size(); fv.insert(fv.end(), fbf.begin(), fbf.end());
+    fo[10] = fv.size(); fv.insert(fv.end(), t->fb_sigma, t->fb_sigma + nsig);
+    fo[11] = fv.size(); fv.insert(fv.end(), t->y_dat, t->y_dat + 41);
+    fo[12] = fv.size(); fv.insert(fv.end(), t->Ay_dat, t->Ay_dat + 41);
+    fo[13] = fv.size(); fv.insert(fv.end(), t->HeIrecLineCoeff, t->HeIrecLineCoeff + 34 * 3 * 4);
+    fo[14] = fv.size(); fv.insert(fv.end(), t->widFlx, t->widFlx + nb);
+    for (float v : fv)
+        if (!std::isfinite(v)) return fail(c, MC_ETABLE, "a re-emission table holds a value that is not finite");
+    std::vector<double> dv(t->HIRecLineData, t->HIRecLineData + 28 * 7);
+    dv.insert(dv.end(), t->HeIIRecLineData, t->HeIIRecLineData + 28 * 15);
+    HIPCHK(c, hipSetDevice(c->device));
+    c->emis = false;
+    if (c->d_em_i) { (void)hipFree(c->d_em_i); c->d_em_i = nullptr; }
+    if (c->d_em_f) { (void)hipFree(c->d_em_f); c->d_em_f = nullptr; }
+    if (c->d_em_d) { (void)hipFree(c->d_em_d); c->d_em_d = nullptr; }
+    HIPCHK(c, dalloc(&c->d_em_i, iv.size()));
+    HIPCHK(c, dalloc(&c->d_em_f, fv.size()));
+    HIPCHK(c, dalloc(&c->d_em_d, dv.size()));
+    HIPCHK(c, hipMemcpy(c->d_em_i, iv.data(), iv.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_em_f, fv.data(), fv.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_em_d, dv.data(), dv.size() * sizeof(double), hipMemcpyHostToDevice));
+    mce::EmArgs &a = c->em;
+    a = mce::EmArgs{};
+    a.nb = nb; a.nc = c->G.nCells;
+    a.nE = A.nE; a.ns1 = A.nstages + 1; a.eH = eH; a.eHe = eHe;
+    a.abH = (float)A.el[eH].abun; a.abHe = (float)A.el[eHe].abun;
+    a.nu = c->d_nuArray; a.widFlx = c->d_em_f + fo[14];
+    a.ntk = t->ntk; a.tk = c->d_em_f + fo[0];
+    size_t eo = io[0];
+    for (int s_ = 0; s_ < 3; ++s_) {
+        a.nlim[s_] = nl3[s_];
+        a.lg[s_] = c->d_em_f + fo[1 + s_];
+        a.edge[s_] = c->d_em_i + eo;
+        eo += nl3[s_];
+        a.pair[s_] = c->d_em_i + (size_t)s_ * nb;
+        a.ip[s_] = ip[s_];
+        a.i0[s_] = i0[s_];
+        a.sig[s_] = c->d_em_f + fo[6 + s_];
+    }
+    a.gff = c->d_em_f + fo[4]; a.gffHeII = c->d_em_f + fo[5];
+    a.nfb = t->nfb; a.fb_i = c->d_em_i + io[1]; a.fb_off = c->d_em_i + io[2];
+    a.fb_f = c->d_em_f + fo[9]; a.fb_sig = c->d_em_f + fo[10];
+    a.jHeI = c->d_em_i + io[3];
+    a.ydat = c->d_em_f + fo[11]; a.Aydat = c->d_em_f + fo[12];
+    a.j2TsP = j2TsP; a.nly = t->nHeIILyman;
+    for (int k = 0; k < t->nHeIILyman; ++k) { a.lyP[k] = lyP[k]; a.lyData[k] = t->HeIILymanData[k]; }
+    a.HIrec = c->d_em_d; a.HeIIrec = c->d_em_d + 28 * 7;
+    a.HeIc = c->d_em_f + fo[13];
+    c->emis = true;
+    return MC_OK;
+}
+
+extern "C" int mc_emission_build(mc_ctx *c, const float *Hden, const float *Te, const float *Ne,
+                                 const float *ionDen, const float *Tdust, float *recPDF, float *totalLines,
+                                 double *kernel_ms)
+{
+    if (!c) return MC_EINVAL;
+    if (!c->emis) return fail(c, MC_ESTATE, "mc_emission_setup has not been called");
+    const int nc = c->G.nCells, nb = c->G.nbins;
+    const size_t nc1 = (size_t)nc + 1, nt = nc1 * nb;
+    const bool host_state = Hden || Te || Ne || ionDen;
+    if (host_state && !(Hden && Te && Ne && ionDen))
+        return fail(c, MC_EINVAL, "Hden, Te, Ne and ionDen go together (or all NULL for the device state)");
+    if (!host_state && !c->state_dev)
+        return fail(c, MC_ESTATE, "no device state yet: pass the state or run mc_update_cells first");
+    if (c->dust && !c->dust_opac)
+        return fail(c, MC_EUNSUP, "grains need TdustSublime and Ndust (mc_dust_tables.scaXsec) for their re-emission");
+    const bool dust = c->dust && c->dust_opac;
+    if (dust && !Tdust && !c->tdust_dev)
+        return fail(c, MC_ESTATE, "no device Tdust yet: pass Tdust or run mc_update_cells first");
+    HIPCHK(c, hipSetDevice(c->device));
+    mce::EmArgs a = c->em;
+    const mcb::Atomic &A = c->bal.A;
+    const int nE = A.nE, ns = A.nstages;
+    const size_t nion = nc1 * nE * (ns + 1);
+    if (host_state) {
+        for (int cell = 1; cell <= nc; ++cell)
+            if (!(Te[cell] > 0.f) || !(Hden[cell] > 0.f) || !std::isfinite(Ne[cell]))
+                return fail(c, MC_EINVAL, "cell %d: Te and Hden must be > 0, Ne finite", cell);
+        std::vector<double> st(3 * nc1 + nion, 0.0);
+        double *ion = st.data() + 3 * nc1;
+        for (size_t cell = 0; cell < nc1; ++cell) {
+            st[cell] = Te[cell];
+            st[nc1 + cell] = Ne[cell];
+            st[2 * nc1 + cell] = Hden[cell];
+            for (int e = 0; e < nE; ++e) {
+                const size_t xr = (size_t)c->bal.eidx_xref[e] - 1;
+                for (int k = 1; k <= ns; ++k)
+                    ion[(cell * nE + e) * (ns + 1) + k] = ionDen[cell + nc1 * (xr + (size_t)nE * (k - 1))];
+            }
+        }
+        if (!c->d_em_st) HIPCHK(c, dalloc(&c->d_em_st, st.size()));
+        HIPCHK(c, hipMemcpy(c->d_em_st, st.data(), st.size() * sizeof(double), hipMemcpyHostToDevice));
+        a.Te = c->d_em_st; a.Ne = c->d_em_st + nc1; a.Hden = c->d_em_st + 2 * nc1; a.ion = c->d_em_st + 3 * nc1;
+    } else {
+        a.Te = c->d_Te; a.Ne = c->d_Ne; a.Hden = c->d_Hden; a.ion = c->d_ion;
+    }
+    if (dust) {
+        const mcg::DustArgs &da = c->dust_args;
+        const int K = c->dust_rows;
+        a.dS = da.nS; a.dSizes = da.nSizes; a.dnsMax = da.nsMax;
+        a.dabs = c->d_op_dust + (size_t)K * nb;
+        a.dtsub = c->d_op_dust + 2 * (size_t)K * nb + K;
+        a.ndust = a.dtsub + K;
+        a.dweight = a.ndust + nc1;
+        a.dgabun = a.dweight + da.nSizes;
+        const size_t ntd = nc1 * (da.nsMax + 1) * (da.nSizes + 1);
+        if (Tdust && !c->d_em_Td) HIPCHK(c, dalloc(&c->d_em_Td, ntd));
+        if (Tdust) {
+            std::vector<double> td(ntd);
+            for (size_t i = 0; i < ntd; ++i) td[i] = Tdust[i];
+            HIPCHK(c, hipMemcpy(c->d_em_Td, td.data(), ntd * sizeof(double), hipMemcpyHostToDevice));
+            a.Tdust = c->d_em_Td;
+        } else {
+            a.Tdust = c->d_Tdust;
+        }
+    }
+    if (!c->d_recCDF) {
+        HIPCHK(c, dalloc(&c->d_recCDF, nt));
+        HIPCHK(c, hipMemset(c->d_recCDF, 0, nt * sizeof(float)));
+    }
+    if (!c->d_recGuide) HIPCHK(c, dalloc(&c->d_recGuide, nc1 * (mcd::kGuide + 1)));
+    if (!c->d_totalLines) {
+        HIPCHK(c, dalloc(&c->d_totalLines, nc1));
+        HIPCHK(c, hipMemset(c->d_totalLines, 0, nc1 * sizeof(float)));
+    }
+    if (!c->d_em_err) HIPCHK(c, dalloc(&c->d_em_err, nc1));
+    a.rec = c->d_recCDF; a.totalLines = c->d_totalLines; a.err = c->d_em_err;
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, mce_launch(c->d_atomic, a, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, mcd_guides(c->d_recCDF, nc + 1, nb, c->d_recGuide, c->stream));
+    std::vector<int> err(nc1);
+    HIPCHK(c, hipMemcpyAsync(err.data(), c->d_em_err, nc1 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->G.recCDF = c->d_recCDF; c->G.recGuide = c->d_recGuide; c->G.totalLines = c->d_totalLines;
+    for (int cell = 1; cell <= nc; ++cell)
+        if (err[cell]) {
+            c->rec_built = false;
+            return fail(c, MC_ETABLE, "cell %d: the diffuse continuum has no positive total (recPDF undefined)", cell);
+        }
+    c->rec_built = true;
+    if (kernel_ms) {
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        *kernel_ms = ms;
+    }
+    if (recPDF) {
+        std::vector<float> rec(nt);
+        HIPCHK(c, hipMemcpy(rec.data(), c->d_recCDF, nt * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t cell = 1; cell < nc1; ++cell)
+            for (int i = 0; i < nb; ++i) recPDF[cell + nc1 * i] = rec[cell * nb + i];
+    }
+    if (totalLines) {
+        std::vector<float> tl(nc1);
+        HIPCHK(c, hipMemcpy(tl.data(), c->d_totalLines, nc1 * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t cell = 1; cell < nc1; ++cell) totalLines[cell] = tl[cell];
+    }
     return MC_OK;
 }
 
@@ -1624,7 +1911,8 @@ extern "C" void mc_ctx_destroy(mc_ctx *c)
                      c->d_mY, c->d_mX0, c->d_hit, c->d_cross, c->d_Rint, c->d_Hint,
                      c->d_dust_tab, c->d_Tdust, c->d_dust_mono, c->d_pe_tab, c->d_grains, c->d_op_dust,
                      c->d_op_i, c->d_op_f, c->d_op_st, c->d_op_Td, c->d_op_out, c->d_op_ent,
-                     c->d_ln_part, c->d_ln_st, c->d_ln_out, c->d_ln_black };
+                     c->d_ln_part, c->d_ln_st, c->d_ln_out, c->d_ln_black, c->d_em_i, c->d_em_err,
+                     c->d_em_f, c->d_em_d, c->d_em_st, c->d_em_Td };
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

@@ -85,6 +85,23 @@ module mocassin_amd_mod
         real(c_float) :: brems1
     end type mc_opacity_tables
 
+    type, bind(C) :: mc_emission_tables
+        integer(c_int32_t) :: ntk, nHI, nHeI, nHeII
+        type(c_ptr) :: tkGamma, logGammaHI, logGammaHeI, logGammaHeII
+        type(c_ptr) :: HINuEdgeP, HeINuEdgeP, HeIINuEdgeP
+        type(c_ptr) :: gauntFF, gauntFFHeII
+        integer(c_int32_t) :: HIPnuP, HeIPnuP, HeIIPnuP
+        type(c_ptr) :: xSecHI, xSecHeI, xSecHeII
+        integer(c_int32_t) :: nfb
+        type(c_ptr) :: fb_meta, fb_gratio, fb_sigma
+        type(c_ptr) :: y_dat, Ay_dat
+        type(c_ptr) :: HIRecLineData, HeIIRecLineData
+        type(c_ptr) :: HeIrecLineCoeff
+        integer(c_int32_t) :: nHeIILyman
+        type(c_ptr) :: HeIILymanData, HeIILymanNu
+        type(c_ptr) :: widFlx
+    end type mc_emission_tables
+
     interface
         integer(c_int) function mc_ctx_create(desc, ctx) bind(C, name="mc_ctx_create")
             import :: c_int, c_ptr, mc_desc
@@ -172,6 +189,18 @@ module mocassin_amd_mod
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx, Hden, Te, Ne, ionDen, Tdust, opacity, scaOpac, kernel_ms
         end function mc_opacity_build
+
+        integer(c_int) function mc_emission_setup(ctx, tables) bind(C, name="mc_emission_setup")
+            import :: c_int, c_ptr, mc_emission_tables
+            type(c_ptr), value :: ctx
+            type(mc_emission_tables), intent(in) :: tables
+        end function mc_emission_setup
+
+        integer(c_int) function mc_emission_build(ctx, Hden, Te, Ne, ionDen, Tdust, recPDF, totalLines, &
+                                                  kernel_ms) bind(C, name="mc_emission_build")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, Hden, Te, Ne, ionDen, Tdust, recPDF, totalLines, kernel_ms
+        end function mc_emission_build
 
         integer(c_int) function mc_line_luminosities(ctx, Hden, Te, Ne, ionDen, lgBlack, forbVol, HbetaVol, &
                                                      kernel_ms) bind(C, name="mc_line_luminosities")

@@ -288,6 +288,72 @@ class Transport:
             op, sca = op.T.copy(), sca.T.copy()
         return op, sca, ms.value
 
+    def emission_setup(self, tkGamma, logGammaHI, logGammaHeI, logGammaHeII, HINuEdgeP, HeINuEdgeP, HeIINuEdgeP,
+                       gauntFF, gauntFFHeII, HIPnuP, HeIPnuP, HeIIPnuP, xSecHI, xSecHeI, xSecHeII, fb_meta,
+                       fb_gratio, fb_sigma, y_dat, Ay_dat, HIRecLineData, HeIIRecLineData, HeIrecLineCoeff,
+                       HeIILymanData, HeIILymanNu, widFlx):
+        """Upload the module data emissionDriver reads (mc_emission_tables);
+        logGamma* (ntk, nlim), HIRecLineData (28, 7), HeIIRecLineData
+        (28, 15), HeIrecLineCoeff (34, 3, 4) as the host holds them."""
+        keep = {}
+
+        def arr(name, a, dt):
+            keep[name] = np.asfortranarray(a, dt).ravel(order="F")
+            return keep[name].ctypes.data
+        T = _capi.McEmissionTables()
+        lg = [np.asarray(v, np.float32) for v in (logGammaHI, logGammaHeI, logGammaHeII)]
+        T.ntk = int(np.size(tkGamma))
+        T.nHI, T.nHeI, T.nHeII = (int(np.size(v) // T.ntk) for v in lg)
+        T.tkGamma = arr("tk", tkGamma, np.float32)
+        T.logGammaHI, T.logGammaHeI, T.logGammaHeII = (arr("lg%d" % k, v.reshape(T.ntk, -1, order="F"), np.float32)
+                                                       for k, v in enumerate(lg))
+        T.HINuEdgeP, T.HeINuEdgeP, T.HeIINuEdgeP = (arr("e%d" % k, v, np.int32)
+                                                    for k, v in enumerate((HINuEdgeP, HeINuEdgeP, HeIINuEdgeP)))
+        T.gauntFF, T.gauntFFHeII = arr("gff", gauntFF, np.float32), arr("gff2", gauntFFHeII, np.float32)
+        T.HIPnuP, T.HeIPnuP, T.HeIIPnuP = int(HIPnuP), int(HeIPnuP), int(HeIIPnuP)
+        T.xSecHI, T.xSecHeI, T.xSecHeII = (arr("x%d" % k, v, np.float32)
+                                           for k, v in enumerate((xSecHI, xSecHeI, xSecHeII)))
+        fbm = np.ascontiguousarray(fb_meta, np.int32).reshape(-1, 4)
+        T.nfb = fbm.shape[0]
+        keep["fbm"] = fbm
+        T.fb_meta = fbm.ctypes.data if T.nfb else None
+        T.fb_gratio = arr("fbg", np.asarray(fb_gratio, np.float32).reshape(-1)[:max(T.nfb, 1)], np.float32)
+        T.fb_sigma = arr("fbs", np.asarray(fb_sigma, np.float32).reshape(-1), np.float32)
+        T.y_dat, T.Ay_dat = arr("y", y_dat, np.float32), arr("Ay", Ay_dat, np.float32)
+        T.HIRecLineData = arr("hi", np.asarray(HIRecLineData, np.float64).reshape(28, 7, order="F"), np.float64)
+        T.HeIIRecLineData = arr("heii", np.asarray(HeIIRecLineData, np.float64).reshape(28, 15, order="F"),
+                                np.float64)
+        T.HeIrecLineCoeff = arr("hei", np.asarray(HeIrecLineCoeff, np.float32).reshape(34, 3, 4, order="F"),
+                                np.float32)
+        T.nHeIILyman = int(np.size(HeIILymanData))
+        T.HeIILymanData, T.HeIILymanNu = arr("lyd", HeIILymanData, np.float32), arr("lyn", HeIILymanNu, np.float32)
+        T.widFlx = arr("wid", widFlx, np.float32)
+        self._check(self._lib.mc_emission_setup(self._h, C.byref(T)))
+
+    def emission_build(self, Hden=None, Te=None, Ne=None, ionDen=None, Tdust=None, host=True):
+        """The re-emission tables on the device (they become the
+        transport's).  The REAL state, or None for the device state of the
+        last update.  Returns (recPDF (nCells+1, nbins), totalLines
+        (nCells+1,), kernel_ms); the tables are None when host is False."""
+        nc1 = self.nCells + 1
+        st = [None] * 4
+        if Te is not None:
+            st = [np.ascontiguousarray(Hden, np.float32).reshape(nc1), np.ascontiguousarray(Te, np.float32).reshape(nc1),
+                  np.ascontiguousarray(Ne, np.float32).reshape(nc1),
+                  np.asfortranarray(np.asarray(ionDen, np.float32).reshape(nc1, self._nE, self._ns))]
+        td = None if Tdust is None else np.ascontiguousarray(Tdust, np.float32).ravel()
+        rec = tl = None
+        if host:
+            rec = np.zeros((self.nbins, nc1), np.float32)
+            tl = np.zeros(nc1, np.float32)
+        ms = C.c_double()
+        ptr = lambda a: None if a is None else a.ctypes.data
+        self._check(self._lib.mc_emission_build(self._h, *(ptr(a) for a in st), ptr(td), ptr(rec), ptr(tl),
+                                                C.byref(ms)))
+        if host:
+            rec = rec.T.copy()
+        return rec, tl, ms.value
+
     def line_luminosities(self, Hden=None, Te=None, Ne=None, ionDen=None, lgBlack=None):
         """outputGas's analytic line luminosities: (forbVol [nCel, 17, 17]
         (model ion in forLines order, upper level, lower level), HbetaVol,

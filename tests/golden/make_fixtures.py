@@ -235,9 +235,18 @@ def make_atomic(deck, name):
     last = np.maximum(seg[:, 3], np.minimum(seg[:, 4], nb))
     sig = np.concatenate([xs[xp - 1:xp - 1 + l - lo + 1] for (_, _, _, lo, _, xp), l in zip(seg, last)])
     out = os.path.join(HERE, "atomic_%s.npz" % name)
+    # the module data of the re-emission tables on this mesh
+    ei = rd("em_ints", np.int32)
+    em = dict(em_ints=ei, em_lyP=rd("em_lyP", np.int32), em_tk=rd("em_tk"), em_gHI=rd("em_gHI"), em_gHeI=rd("em_gHeI"), em_gHeII=rd("em_gHeII"),
+              em_eHI=rd("em_eHI", np.int32), em_eHeI=rd("em_eHeI", np.int32), em_eHeII=rd("em_eHeII", np.int32),
+              em_gff=rd("em_gff"), em_gffHeII=rd("em_gffHeII"), em_sHI=rd("em_sHI"), em_sHeI=rd("em_sHeI"),
+              em_sHeII=rd("em_sHeII"), em_fbm=rd("em_fbm", np.int32)[:4 * int(ei[9])], em_fbg=rd("em_fbg"),
+              em_fbs=rd("em_fbs"), em_ydat=rd("em_ydat"), em_Aydat=rd("em_Aydat"), em_lyData=rd("em_lyData"),
+              em_lyNu=rd("em_lyNu"), em_HeIc=rd("em_HeIc"), em_HIrec=rd("em_HIrec", np.float64),
+              em_HeIIrec=rd("em_HeIIrec", np.float64), widFlx=rd("widFlx"))
     np.savez_compressed(out, **b.to_npz_dict(), Te0=rd("Te")[1:2], ion0=ion0,
                         op_seg=np.stack([seg[:, 0], seg[:, 1], seg[:, 3], last], axis=1).astype(np.int32),
-                        op_sigma=sig, op_brems1=xs[bp - 1:bp])
+                        op_sigma=sig, op_brems1=xs[bp - 1:bp], **em)
     print("wrote", out, "nbins", nb, "channels", b.nch, "CEL ions", b.ncel)
 
 
