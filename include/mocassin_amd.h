@@ -473,6 +473,40 @@ int mc_emission_build(mc_ctx *ctx, const float *Hden, const float *Te, const flo
 int mc_line_luminosities(mc_ctx *ctx, const float *Hden, const float *Te, const float *Ne, const float *ionDen,
                          const int32_t *lgBlack, double *forbVol, double *HbetaVol, double *kernel_ms);
 
+/* ---- grid state files ---------------------------------------------------
+ * The state a run leaves in output/ and a warm start reads back, in the
+ * reference's layouts: Te, Ne, Hden (0:nCells), ionDen(0:nCells,
+ * 1:nElemUsed, 1:nstages), lgConverged and lgBlack (0:nCells) as INTEGER,
+ * and with grains Ndust (0:nCells) and Tdust(0:nSpeciesMax, 0:nSizes,
+ * 0:nCells).  The geometry and the grid flags come from an mc_desc (as for
+ * mc_ctx_create), the elements switched on and nstages from an mc_atomic
+ * (gas grids).  Neither call touches the GPU. */
+typedef struct {
+    float *Te, *Ne, *Hden;      /* gas; NULL without */
+    float *ionDen;
+    int32_t *lgConverged, *lgBlack;
+    float *Ndust, *Tdust;       /* grains; NULL without */
+    int32_t nSpeciesMax, nSizes;
+    float totalDustMass;        /* [1e45 g], written at the end of dustGrid.out */
+} mc_grid_state;
+
+/* Write grid0.out, and grid1.out/grid2.out with gas and dustGrid.out with
+ * grains, into directory dir, byte for byte as the reference's writeGrid
+ * does (one grid; list-directed records, each REAL as its shortest
+ * round-trip decimal).  photoSource.out and grid3.out hold the run's input
+ * parameters and stay with the host.  Replaces: the state files of
+ * writeGrid (grid_mod.f90:2646-2760). */
+int mc_write_grid(const mc_desc *desc, const mc_atomic *atomic, const char *dir, const mc_grid_state *state);
+
+/* Read those files back into the state arrays as resetGrid does: the
+ * records of inactive lattice points land in row 0 (the last one wins), and
+ * each element's ionDen row takes the whole read buffer, so stages above
+ * min(Z+1, nstages) keep the previous element's values.  The files' grid
+ * (dimensions, axes, active map) must be the descriptor's (MC_ETABLE
+ * otherwise).  Replaces: the state part of resetGrid (grid_mod.f90:
+ * 3187-3470), read by mocassinWarm. */
+int mc_read_grid(const mc_desc *desc, const mc_atomic *atomic, const char *dir, mc_grid_state *state);
+
 /* ---- warm-start network (no counterpart in the reference) -------------
  * A 72 -> 128 -> 128 -> 128 -> 31 ReLU perceptron with BF16 weights and
  * activations and FP32 accumulation on the matrix cores, trained on the

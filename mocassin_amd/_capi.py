@@ -22,7 +22,7 @@ EXPORTS = ("mc_ctx_create", "mc_upload_tables", "mc_zero_estimators", "mc_transp
            "mc_mlp_set", "mc_mlp_get", "mc_mlp_features", "mc_mlp_predict", "mc_mlp_loss_grad",
            "mc_mlp_train", "mc_count_crossings", "mc_fetch_crossings", "mc_transport_batch",
            "mc_dust_setup", "mc_update_cells", "mc_opacity_setup", "mc_opacity_build",
-           "mc_line_luminosities", "mc_emission_setup", "mc_emission_build",
+           "mc_line_luminosities", "mc_emission_setup", "mc_emission_build", "mc_write_grid", "mc_read_grid",
            "mc_ctx_last_error", "mc_ctx_destroy")
 
 MC_SOLVER_REFERENCE, MC_SOLVER_SECANT = 0, 1
@@ -107,6 +107,12 @@ class McOpacityTables(C.Structure):
     _fields_ = [("nseg", C.c_int32), ("seg_meta", C.c_void_p), ("seg_sigma", C.c_void_p), ("brems1", C.c_float)]
 
 
+class McGridState(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("Te", "Ne", "Hden", "ionDen", "lgConverged", "lgBlack", "Ndust",
+                                          "Tdust")] + \
+               [("nSpeciesMax", C.c_int32), ("nSizes", C.c_int32), ("totalDustMass", C.c_float)]
+
+
 class McError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("%s: %s" % (ERRORS.get(code, str(code)), msg))
@@ -151,6 +157,8 @@ def load():
     L.mc_opacity_build.argtypes = [vp] * 9
     L.mc_line_luminosities.argtypes = [vp] * 9
     L.mc_emission_setup.argtypes = [vp, C.POINTER(McEmissionTables)]
+    L.mc_write_grid.argtypes = [C.POINTER(McDesc), vp, C.c_char_p, C.POINTER(McGridState)]
+    L.mc_read_grid.argtypes = [C.POINTER(McDesc), vp, C.c_char_p, C.POINTER(McGridState)]
     L.mc_emission_build.argtypes = [vp] * 9
     L.mc_count_crossings.argtypes = [vp, i32]
     L.mc_fetch_crossings.argtypes = [vp, vp]

@@ -1894,6 +1894,16 @@ extern "C" int mc_walk_replay(mc_ctx *c, int32_t n, const float *pos, const floa
     return rc;
 }
 
+// the message of a call that has no context (grid files)
+__attribute__((visibility("hidden"))) int mc_fail_global(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_create_err, 512, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
 extern "C" const char *mc_ctx_last_error(const mc_ctx *c)
 {
     return c ? c->err : g_create_err;

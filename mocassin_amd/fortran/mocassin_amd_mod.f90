@@ -102,6 +102,12 @@ module mocassin_amd_mod
         type(c_ptr) :: widFlx
     end type mc_emission_tables
 
+    type, bind(C) :: mc_grid_state
+        type(c_ptr) :: Te, Ne, Hden, ionDen, lgConverged, lgBlack, Ndust, Tdust
+        integer(c_int32_t) :: nSpeciesMax, nSizes
+        real(c_float) :: totalDustMass
+    end type mc_grid_state
+
     interface
         integer(c_int) function mc_ctx_create(desc, ctx) bind(C, name="mc_ctx_create")
             import :: c_int, c_ptr, mc_desc
@@ -189,6 +195,22 @@ module mocassin_amd_mod
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx, Hden, Te, Ne, ionDen, Tdust, opacity, scaOpac, kernel_ms
         end function mc_opacity_build
+
+        integer(c_int) function mc_write_grid(desc, atomic, dir, state) bind(C, name="mc_write_grid")
+            import :: c_int, c_char, mc_desc, mc_atomic, mc_grid_state
+            type(mc_desc), intent(in) :: desc
+            type(mc_atomic), intent(in) :: atomic
+            character(kind=c_char), dimension(*), intent(in) :: dir
+            type(mc_grid_state), intent(in) :: state
+        end function mc_write_grid
+
+        integer(c_int) function mc_read_grid(desc, atomic, dir, state) bind(C, name="mc_read_grid")
+            import :: c_int, c_char, mc_desc, mc_atomic, mc_grid_state
+            type(mc_desc), intent(in) :: desc
+            type(mc_atomic), intent(in) :: atomic
+            character(kind=c_char), dimension(*), intent(in) :: dir
+            type(mc_grid_state), intent(in) :: state
+        end function mc_read_grid
 
         integer(c_int) function mc_emission_setup(ctx, tables) bind(C, name="mc_emission_setup")
             import :: c_int, c_ptr, mc_emission_tables

@@ -68,6 +68,15 @@ program mcref_harness
     call mpi_comm_rank(MPI_COMM_WORLD, taskid, ierr)
     call mpi_comm_size(MPI_COMM_WORLD, numtasks, ierr)
 
+    ! the warm start of mocassinWarm.f90:51: the state files a previous run
+    ! wrote under output/, read back by resetGrid
+    if (trim(mode) == "resetgrid") then
+       call resetGrid(grid3D)
+       call dump_reset()
+       call mpi_finalize(ierr)
+       stop
+    end if
+
     ! --- mocassin.f90:66-104
     call readInput()
     do iGrid = 1, nGrids
@@ -110,6 +119,8 @@ program mcref_harness
        call run_lines()
        call run_emission()
        call run_opacity()
+       ! the state files of writeGrid (grid_mod.f90:2646-2872) under output/
+       call writeGrid(grid3D(1:nGrids))
     case ("walk")
        call get_command_argument(5, arg); read(arg, *) nuBin
        call get_command_argument(6, originFile)
@@ -638,6 +649,35 @@ contains
        call dump_r4('em_totalLines', grid3D(1)%totalLines, nc+1)
        deallocate(fbm, fbg, fbs, sHI, sHeI, sHeII)
     end subroutine run_emission
+
+    ! the state resetGrid read back (grid_mod.f90:2967-3568)
+    subroutine dump_reset()
+       integer :: nc, ints(8)
+       nc = grid3D(1)%nCells
+       ints = 0
+       ints(1) = grid3D(1)%nx; ints(2) = grid3D(1)%ny; ints(3) = grid3D(1)%nz; ints(4) = nc
+       ints(5) = nElementsUsed; ints(6) = nstages
+       if (lgDust) then
+          ints(7) = nSpeciesMax; ints(8) = nSizes
+       end if
+       call dump_i4('rg_ints', ints, 8)
+       call dump_r4('rg_xAxis', grid3D(1)%xAxis, grid3D(1)%nx)
+       call dump_r4('rg_yAxis', grid3D(1)%yAxis, grid3D(1)%ny)
+       call dump_r4('rg_zAxis', grid3D(1)%zAxis, grid3D(1)%nz)
+       call dump_i4('rg_active', grid3D(1)%active, grid3D(1)%nx*grid3D(1)%ny*grid3D(1)%nz)
+       call dump_i4('rg_conv', grid3D(1)%lgConverged, nc+1)
+       call dump_i4('rg_black', grid3D(1)%lgBlack, nc+1)
+       if (lgGas) then
+          call dump_r4('rg_Te', grid3D(1)%Te, nc+1)
+          call dump_r4('rg_Ne', grid3D(1)%Ne, nc+1)
+          call dump_r4('rg_Hden', grid3D(1)%Hden, nc+1)
+          call dump_r4('rg_ionDen', grid3D(1)%ionDen, (nc+1)*nElementsUsed*nstages)
+       end if
+       if (lgDust) then
+          call dump_r4('rg_Ndust', grid3D(1)%Ndust, nc+1)
+          call dump_r4('rg_Tdust', grid3D(1)%Tdust, (nc+1)*(nSpeciesMax+1)*(nSizes+1))
+       end if
+    end subroutine dump_reset
 
     ! scattering cross-sections of every species and size on nuArray and the
     ! sublimation temperatures (the dust add of iteration_mod.f90:184-222)

@@ -35,6 +35,7 @@ WORK = os.path.join(ROOT, "oracle", "_ref", "work")
 def run_harness(deck, mode, args, env_extra=None, tag=None):
     wd = os.path.join(WORK, tag or deck)
     os.makedirs(os.path.join(wd, "out"), exist_ok=True)
+    os.makedirs(os.path.join(wd, "output"), exist_ok=True)
     decks.write_deck(wd, deck)
     env = dict(os.environ)
     env.update(env_extra or {})
@@ -195,6 +196,20 @@ def make_opacity(deck, npk, seed, name, keep_cells=96):
     J = rd("Jste_scaled")
     nzJ = np.nonzero(J)[0]
     out.update(J_idx=nzJ.astype(np.int32), J_val=J[nzJ])
+    # the state files writeGrid wrote, and the state resetGrid reads back
+    # from them (a second run of the harness in the same directory)
+    od = os.path.join(os.path.dirname(d), "output")
+    for f in ("grid0", "grid1", "grid2", "dustGrid"):
+        fp = os.path.join(od, f + ".out")
+        if os.path.exists(fp):
+            out["txt_" + f] = np.frombuffer(open(fp, "rb").read(), np.uint8)
+    d2 = run_harness(deck, "resetgrid", [], tag="op_" + name)
+    rg = lambda n, dt=np.float32: np.fromfile(os.path.join(d2, n + ".bin"), dt)
+    out.update(rg_ints=rg("rg_ints", np.int32), rg_active=rg("rg_active", np.int32), rg_conv=rg("rg_conv", np.int32),
+               rg_black=rg("rg_black", np.int32), rg_Te=rg("rg_Te"), rg_Ne=rg("rg_Ne"), rg_Hden=rg("rg_Hden"),
+               rg_ionDen=rg("rg_ionDen"), rg_xAxis=rg("rg_xAxis"))
+    if os.path.exists(os.path.join(d2, "rg_Tdust.bin")):
+        out.update(rg_Ndust=rg("rg_Ndust"), rg_Tdust=rg("rg_Tdust"))
     # re-emission tables of the balanced state and the module data they use
     ei = rd("em_ints", np.int32)
     nfb = int(ei[9])
