@@ -297,6 +297,35 @@ def main():
                 em["gpu_over_cpu"] = em["cells_per_s"] / em["cpu_baseline"]["value"]
             balance["emission"] = em
             print("bench: re-emission tables %.3f ms (%d cells)" % (ms_e, p["nCells"]), file=sys.stderr, flush=True)
+        if "emission" in balance and "opacity" in balance:
+            # one whole iteration with every table resident: the sweep, the
+            # balance of its J (from the state the last balance left), the
+            # opacity and re-emission tables of the new state, and the upload
+            # that keeps them on the device (iterateMC's loop body)
+            stages = {}
+            J_keep = T.fetch()                  # the legs below read the timed sweeps' J
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            T.zero_estimators()
+            c = T.energy_packet_driver(1, npk, step * npk)
+            step += 1
+            stages["transport_ms"] = c["kernel_ms"]
+            rb = T.update_cells(Hd, r[0], r[1], r[2], jscale=js)
+            stages["balance_ms"] = rb[5]
+            stages["opacity_ms"] = T.opacity_build(host=False)[2]
+            stages["emission_ms"] = T.emission_build(host=False)[2]
+            T.upload_tables(None, p["specCDF"])
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            T.set_estimators(*J_keep)
+            del J_keep
+            balance["iteration"] = dict(stages, host_wall_ms=wall * 1e3, packets=npk,
+                                        note="transport, balance (reference schedule from the previous "
+                                             "state), opacity and re-emission builds, table upload without "
+                                             "host tables; device times per stage, wall time of the whole")
+            print("bench: one iteration on the device: %.1f ms wall (transport %.1f, balance %.1f, opacity %.2f, "
+                  "re-emission %.1f ms)" % (wall * 1e3, stages["transport_ms"], stages["balance_ms"],
+                                           stages["opacity_ms"], stages["emission_ms"]), file=sys.stderr, flush=True)
         if not a.no_grains:
             # the same balance with silicate grains, their charge and the
             # photoelectric and collision terms, then the grain temperatures

@@ -444,8 +444,13 @@ int mc_emission_setup(mc_ctx *ctx, const mc_emission_tables *tables);
  * the host holds them (REAL); NULL state (and Tdust) take the device state of
  * the last mc_update_cells / mc_balance_run, rounded to FP32.  recPDF
  * (0:nCells, 1:nbins) and totalLines (0:nCells), nullable: host copies of
- * the active cells' rows (row 0 is left as it is).  MC_ETABLE names a cell
- * whose continuum has no positive total.  kernel_ms (nullable): device
+ * the active cells' rows (row 0 is left as it is).  A cell whose continuum
+ * lies entirely below the reference's 1e-15 cut gets the row 0 .. 0 1 (the
+ * reference's is 0/0; with totalLines = 1 it is never sampled).  A cell at
+ * iterateT's 1 K floor, where the reference's fits give 0/0 and inf*0, emits
+ * nothing: totalLines = 1 (its absorbed packets leave as line photons) and
+ * the row 0 .. 0 1.
+ * MC_ETABLE names a cell whose state or emission is not finite.  kernel_ms (nullable): device
  * time.  Replaces: emissionDriver per cell in iterateMC (iteration_mod.f90:
  * 229-240; emission_mod.f90:32-173, 253-903, 905-1311). */
 int mc_emission_build(mc_ctx *ctx, const float *Hden, const float *Te, const float *Ne, const float *ionDen,

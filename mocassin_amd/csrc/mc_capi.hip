@@ -1294,9 +1294,9 @@ extern "C" int mc_emission_build(mc_ctx *c, const float *Hden, const float *Te, 
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->G.recCDF = c->d_recCDF; c->G.recGuide = c->d_recGuide; c->G.totalLines = c->d_totalLines;
     for (int cell = 1; cell <= nc; ++cell)
-        if (err[cell]) {
+        if (err[cell] == 1) {
             c->rec_built = false;
-            return fail(c, MC_ETABLE, "cell %d: the diffuse continuum has no positive total (recPDF undefined)", cell);
+            return fail(c, MC_ETABLE, "cell %d: the state or its emission is not finite", cell);
         }
     c->rec_built = true;
     if (kernel_ms) {

@@ -136,6 +136,18 @@ __global__ void __launch_bounds__(64) emission_kernel(const mcb::Atomic *__restr
     const float log10Te = log10f(Te);
     if (NeUsed < 1.e-5f) NeUsed = 1.e-5f;
     const float xHp = XION(a.eH, 2), xHe1 = XION(a.eHe, 2), xHe2 = XION(a.eHe, 3);
+    if (Te <= 1.f) {
+        // iterateT's floor (update_mod.f90:454): there the reference's fits
+        // give 0/0 and inf*0; the cell emits nothing, its absorbed packets
+        // leave as lines and its row (never sampled) is 0 .. 0 1
+        float *row = a.rec + (size_t)cell * nb;
+        for (int i = lane; i < nb; i += W) row[i] = i == nb - 1 ? 1.f : 0.f;
+        if (lane == 0) {
+            a.totalLines[cell] = 1.f;
+            a.err[cell] = 2;
+        }
+        return;
+    }
 
     // log gamma rows at Te (fb_ff, emission_mod.f90:306-336)
     for (int s = 0; s < 3; ++s)
@@ -390,15 +402,24 @@ __global__ void __launch_bounds__(64) emission_kernel(const mcb::Atomic *__restr
         }
         const float tl = normRec + normFor;
         normalize = normalize + tl;
-        a.totalLines[cell] = tl / normalize;
+        // a cell that emits nothing (Te at the reference's 1 K floor) has
+        // 0/0 in the reference; its absorbed packets leave as line photons
+        a.totalLines[cell] = normalize > 0.f ? tl / normalize : 1.f;
         red[0] = s1[nb - 1];
-        a.err[cell] = (s1[nb - 1] > 0.f && s1[nb - 1] < INFINITY) ? 0 : 1;
+        // a continuum entirely below the 1e-15 cut leaves the reference a
+        // row of 0/0; with totalLines = 1 the row is never sampled, so it is
+        // written 0 .. 0 1 here.  err: 1 a state that is not finite, 2 a
+        // cell that emits nothing
+        const bool cont = s1[nb - 1] > 0.f && s1[nb - 1] < INFINITY;
+        const bool finite = std::isfinite(Te) && std::isfinite(NeIn) && std::isfinite(tl) && std::isfinite(s1[nb - 1]);
+        a.err[cell] = !finite ? 1 : (!cont && !(tl > 0.f) ? 2 : 0);
     }
     __syncthreads();
     const float last = red[0];
+    const bool cont = last > 0.f && last < INFINITY;
     float *row = a.rec + (size_t)cell * nb;
     for (int i = lane; i < nb; i += W) {
-        float v = s1[i] / last;
+        float v = cont ? s1[i] / last : 0.f;
         if (i == nb - 1) v = 1.f;
         if (v > 0.999998f) v = 1.f;
         row[i] = v;

@@ -117,3 +117,27 @@ def test_device_state_after_balance():
     r = np.max(np.abs(tl[1:] / g["em_totalLines"][1:] - 1))
     print("emission from device state: max |recPDF diff| %.2e, totalLines max rel %.2e, %.3f ms" % (d, r, ms))
     assert d < 1e-4 and r < 1e-4
+
+
+def test_cell_at_the_1K_floor():
+    """A cell the reference's schedule drove to its 1 K floor emits nothing
+    (the reference divides 0 by 0): its absorbed packets leave as lines
+    (totalLines 1) and its never-sampled row is 0 .. 0 1; a state that is
+    not finite is refused."""
+    print("emission of a cell at 1 K", flush=True)
+    from mocassin_amd._capi import McError
+    T, g, b, ion, dust = _ctx("c2mini")
+    Te, Ne = g["Te"].copy(), g["Ne"].copy()
+    Te[5], Ne[5] = 1.0, 1.e-5
+    ion = ion.copy()
+    ion[5, :, :] = 0.0
+    ion[5, :, 0] = 1.0
+    try:
+        rec, tl, _ = T.emission_build(g["Hden"], Te, Ne, ion)
+        assert tl[5] == 1.0 and np.all(rec[5, :-1] == 0) and rec[5, -1] == 1.0
+        assert np.all(tl[6:] < 1)
+        Te[7] = np.nan
+        with pytest.raises(McError):
+            T.emission_build(g["Hden"], Te, Ne, ion)
+    finally:
+        T.close()
