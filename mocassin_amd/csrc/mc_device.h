@@ -243,6 +243,30 @@ __device__ __forceinline__ int act(const Grid &G, int x, int y, int z)
     return G.active[(x - 1) + G.nx * ((y - 1) + G.ny * (z - 1))];
 }
 
+// Distance to the next wall along one axis (photon_mod.f90:1263-1298 for
+// x; y and z alike).  Moving up (v > 1e-10) the wall is the midpoint above
+// cell P, or the outer edge ax(n) in the last cell; moving down it is the
+// midpoint below, or ax(1) in the first cell; otherwise 1e35.  A packet
+// within 1e-10 of an inner wall steps over it (r = wall, P +- 1); at the
+// outer edge moving up it is lost (lost = true, r = the edge); at ax(1)
+// moving down it is put on the edge.  1 <= P <= n on entry.
+__device__ __forceinline__ float wall_distance(const float *ax, const float *mid, int n, float v, float &r,
+                                               int &P, bool &lost)
+{
+    const bool pos = v > 1.e-10f, neg = v < -1.e-10f;
+    const bool inner = pos ? (P < n) : (P > 1);
+    const float m = inner ? mid[pos ? P - 1 : P - 2] : ax[pos ? n - 1 : 0];
+    const float d = (m - r) / v;
+    lost = false;
+    if (!(pos || neg)) return 1.e35f;
+    if (fabsf(d) < 1.e-10f) {
+        if (inner) { r = m; P += pos ? 1 : -1; }
+        else if (pos) { r = m; lost = true; }
+        else r = m;
+    }
+    return d;
+}
+
 // One iteration of pathSegment's "do i" loop (photon_mod.f90:1194-2836) on
 // the mother grid.  ZERO_OPACITY replays geometry only.  kappa is looked up
 // here; on ST_ABSORBED the interaction point is in rVec and (xP,yP,zP).
@@ -265,74 +289,18 @@ __device__ __forceinline__ int walk_step(const Grid &G, const Axes &A, Walker &W
         if (rz <= AX(A.z, 1)) { if (vz < 0.f) vz = -vz; rz = AX(A.z, 1); }
     }
 
-    // wall distances, 1263-1379
-    if (vx > 1.e-10f) {
-        if (xP < nx) {
-            float m = AX(A.mx, xP);
-            dSx = (m - rx) / vx;
-            if (fabsf(dSx) < 1.e-10f) { rx = m; xP = xP + 1; }
-        } else {
-            dSx = (AX(A.x, nx) - rx) / vx;
-            if (fabsf(dSx) < 1.e-10f) { W.rx = AX(A.x, nx); return ST_LOST; }
-        }
-    } else if (vx < -1.e-10f) {
-        if (xP > 1) {
-            float m = AX(A.mx, xP - 1);
-            dSx = (m - rx) / vx;
-            if (fabsf(dSx) < 1.e-10f) { rx = m; xP = xP - 1; }
-        } else {
-            dSx = (AX(A.x, 1) - rx) / vx;
-            if (fabsf(dSx) < 1.e-10f) rx = AX(A.x, 1);
-        }
-    } else {
-        dSx = 1.e35f;
-    }
+    // wall distances, 1263-1379: one division per axis whatever the signs
+    // of the wavefront's directions (the reference's branches select the
+    // wall; they are evaluated here as selects, in the same order)
+    bool lostX, lostY, lostZ;
+    dSx = wall_distance(A.x, A.mx, nx, vx, rx, xP, lostX);
+    dSy = wall_distance(A.y, A.my, ny, vy, ry, yP, lostY);
+    dSz = wall_distance(A.z, A.mz, nz, vz, rz, zP, lostZ);
+    if (lostX) { W.rx = rx; return ST_LOST; }
     if (isnan(dSx)) { err = 34; return ST_ERROR; }
-
-    if (vy > 1.e-10f) {
-        if (yP < ny) {
-            float m = AX(A.my, yP);
-            dSy = (m - ry) / vy;
-            if (fabsf(dSy) < 1.e-10f) { ry = m; yP = yP + 1; }
-        } else {
-            dSy = (AX(A.y, ny) - ry) / vy;
-            if (fabsf(dSy) < 1.e-10f) { W.ry = AX(A.y, ny); return ST_LOST; }
-        }
-    } else if (vy < -1.e-10f) {
-        if (yP > 1) {
-            float m = AX(A.my, yP - 1);
-            dSy = (m - ry) / vy;
-            if (fabsf(dSy) < 1.e-10f) { ry = m; yP = yP - 1; }
-        } else {
-            dSy = (AX(A.y, 1) - ry) / vy;
-            if (fabsf(dSy) < 1.e-10f) ry = AX(A.y, 1);
-        }
-    } else {
-        dSy = 1.e35f;
-    }
+    if (lostY) { W.ry = ry; return ST_LOST; }
     if (isnan(dSy)) { err = 34; return ST_ERROR; }
-
-    if (vz > 1.e-10f) {
-        if (zP < nz) {
-            float m = AX(A.mz, zP);
-            dSz = (m - rz) / vz;
-            if (fabsf(dSz) < 1.e-10f) { rz = m; zP = zP + 1; }
-        } else {
-            dSz = (AX(A.z, nz) - rz) / vz;
-            if (fabsf(dSz) < 1.e-10f) { W.rz = AX(A.z, nz); return ST_LOST; }
-        }
-    } else if (vz < -1.e-10f) {
-        if (zP > 1) {
-            float m = AX(A.mz, zP - 1);
-            dSz = (m - rz) / vz;
-            if (fabsf(dSz) < 1.e-10f) { rz = m; zP = zP - 1; }
-        } else {
-            dSz = (AX(A.z, 1) - rz) / vz;
-            if (fabsf(dSz) < 1.e-10f) rz = AX(A.z, 1);
-        }
-    } else {
-        dSz = 1.e35f;
-    }
+    if (lostZ) { W.rz = rz; return ST_LOST; }
     if (isnan(dSz)) { err = 34; return ST_ERROR; }
 
     // cells on the cell wall, 1382-1384
