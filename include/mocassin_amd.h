@@ -11,7 +11,8 @@
  *   totalLines(0:nCells)                        float
  *   specCDF(1:nbins, 1:nStars)                  float  (inSpectrumProbDen rows)
  *   J(0:nCells,1:nbins)                         double (Jste, unscaled)
- *   esc(0:nCells,0:nbins)                       double (escapedPackets(:,:,0))
+ *   esc(0:nCells,0:nbins[,0:nAngleBins])        double (escapedPackets; planes
+ *                                                after mc_set_view_angles)
  * Row 0 of every (0:nCells,...) array is the inactive-cell sink, as in the
  * reference.
  *
@@ -233,6 +234,30 @@ int mc_walk_replay(mc_ctx *ctx, int32_t n, const float *pos, const float *dir,
  * grid%escapedPackets by iterateMC and updateCell (iteration_mod.f90:700-722,
  * update_mod.f90:104-154). */
 int mc_set_estimators(mc_ctx *ctx, const double *J, const double *esc);
+
+/* Viewing-angle bins of the escapes, as initCartesianGrid sets them from
+ * the "inclination" keyword (grid_mod.f90:416-467, set_input_mod.f90:534-556):
+ * theta and phi bin counts and widths, the maps from a bin to a viewpoint
+ * (0 = none), and the viewpoints' angles in radians (index 0 unused). */
+typedef struct {
+    int32_t nAngleBins;                 /* 0 (none), 1 or 2 */
+    int32_t totAngleBinsTheta, totAngleBinsPhi;   /* 1..64 each */
+    float dTheta, dPhi;                 /* Pi/totAngleBinsTheta, twoPi/totAngleBinsPhi (REAL) */
+    const int32_t *viewPointPtheta;     /* (0:totAngleBinsTheta), values 0..nAngleBins */
+    const int32_t *viewPointPphi;       /* (0:totAngleBinsPhi), values 0..nAngleBins */
+    const float *viewPointTheta;        /* (0:nAngleBins) */
+    const float *viewPointPhi;          /* (0:nAngleBins), < 0: phi switched off */
+} mc_view_angles;
+
+/* Switch the escapes' viewing-angle planes on (v->nAngleBins > 0) or off
+ * (v NULL or nAngleBins 0).  esc then holds escapedPackets(0:nCells,
+ * 0:nbins, 0:nAngleBins) in mc_transport_run, mc_fetch_estimators and
+ * mc_set_estimators, each escape binned by its flight direction
+ * (enPacket%direction) as every escape site does (photon_mod.f90:370-465,
+ * 1582-1666 and the four alike in pathSegment); the escapes are zeroed.
+ * A bin out of range stops the sweep with MC_EPACKET (codes 60, 61), where
+ * the reference stops.  Replaces: those sites' viewing-angle branches. */
+int mc_set_view_angles(mc_ctx *ctx, const mc_view_angles *v);
 
 /* Upload the atomic inputs of the balance (validated; see mc_atomic).
  * Requires H and He switched on, nstages <= 9, nbins <= 2048, at most 16

@@ -40,6 +40,8 @@ struct mc_ctx {
     unsigned short *d_recGuide = nullptr;
     float *d_totalLines = nullptr, *d_specCDF = nullptr, *d_nuArray = nullptr, *d_gSca = nullptr;
     double *d_J3 = nullptr, *d_J = nullptr, *d_esc = nullptr;
+    mcd::View *d_view = nullptr;            // viewing-angle bins (null: escapes in plane 0 only)
+    int esc_planes = 1;                      // nAngleBins + 1
     unsigned long long *d_scratch = nullptr;  // next_packet, counts[8]
     int *d_error = nullptr;
     double *d_qphot = nullptr;
@@ -344,7 +346,7 @@ extern "C" int mc_zero_estimators(mc_ctx *c)
     HIPCHK(c, hipSetDevice(c->device));
     const size_t nc1 = (size_t)c->G.nCells + 1;
     HIPCHK(c, hipMemsetAsync(c->d_J3, 0, mcd::j3_size(c->G.ncell3, c->G.nbins) * sizeof(double), c->stream));
-    HIPCHK(c, hipMemsetAsync(c->d_esc, 0, nc1 * (c->G.nbins + 1) * sizeof(double), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_esc, 0, nc1 * (c->G.nbins + 1) * c->esc_planes * sizeof(double), c->stream));
     if (c->G.cross) HIPCHK(c, hipMemsetAsync(c->G.cross, 0, (size_t)c->G.ncell3 * sizeof(unsigned), c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return MC_OK;
@@ -388,7 +390,7 @@ extern "C" int mc_fetch_estimators(mc_ctx *c, double *J_out, double *esc_out)
                                  hipMemcpyDeviceToHost, c->stream));
     }
     if (esc_out)
-        HIPCHK(c, hipMemcpyAsync(esc_out, c->d_esc, nc1 * (c->G.nbins + 1) * sizeof(double),
+        HIPCHK(c, hipMemcpyAsync(esc_out, c->d_esc, nc1 * (c->G.nbins + 1) * c->esc_planes * sizeof(double),
                                  hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return MC_OK;
@@ -407,7 +409,55 @@ extern "C" int mc_set_estimators(mc_ctx *c, const double *J, const double *esc)
             if (a > 0) dense[mcd::j3_index((int)nb, (int)nu, (int)i)] = J[nu * nc1 + a];
         }
     HIPCHK(c, hipMemcpy(c->d_J3, dense.data(), dense.size() * sizeof(double), hipMemcpyHostToDevice));
-    if (esc) HIPCHK(c, hipMemcpy(c->d_esc, esc, nc1 * (nb + 1) * sizeof(double), hipMemcpyHostToDevice));
+    if (esc) HIPCHK(c, hipMemcpy(c->d_esc, esc, nc1 * (nb + 1) * c->esc_planes * sizeof(double), hipMemcpyHostToDevice));
+    return MC_OK;
+}
+
+extern "C" int mc_set_view_angles(mc_ctx *c, const mc_view_angles *v)
+{
+    if (!c) return MC_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    const int n = v ? v->nAngleBins : 0;
+    mcd::View h{};
+    if (n > 0) {
+        if (n > 2) return fail(c, MC_EINVAL, "nAngleBins %d: at most two viewing angles", n);
+        if (v->totAngleBinsTheta < 1 || v->totAngleBinsTheta > mcd::kViewMax || v->totAngleBinsPhi < 1 ||
+            v->totAngleBinsPhi > mcd::kViewMax)
+            return fail(c, MC_EINVAL, "angle bins %d x %d outside 1..%d", v->totAngleBinsTheta,
+                        v->totAngleBinsPhi, mcd::kViewMax);
+        if (!(v->dTheta > 0.f) || !(v->dPhi > 0.f) || !std::isfinite(v->dTheta) || !std::isfinite(v->dPhi))
+            return fail(c, MC_EINVAL, "dTheta and dPhi must be finite and positive");
+        if (!v->viewPointPtheta || !v->viewPointPphi || !v->viewPointTheta || !v->viewPointPhi)
+            return fail(c, MC_EINVAL, "viewing-angle tables missing");
+        h.nAngle = n; h.totT = v->totAngleBinsTheta; h.totP = v->totAngleBinsPhi;
+        h.dTheta = v->dTheta; h.dPhi = v->dPhi;
+        for (int i = 0; i <= h.totT; ++i) {
+            h.Pt[i] = v->viewPointPtheta[i];
+            if (h.Pt[i] < 0 || h.Pt[i] > n) return fail(c, MC_EINVAL, "viewPointPtheta(%d) = %d", i, h.Pt[i]);
+        }
+        for (int i = 0; i <= h.totP; ++i) {
+            h.Pp[i] = v->viewPointPphi[i];
+            if (h.Pp[i] < 0 || h.Pp[i] > n) return fail(c, MC_EINVAL, "viewPointPphi(%d) = %d", i, h.Pp[i]);
+        }
+        for (int i = 0; i <= n; ++i) { h.vth[i] = v->viewPointTheta[i]; h.vph[i] = v->viewPointPhi[i]; }
+    }
+    const size_t nc1 = (size_t)c->G.nCells + 1, nb1 = (size_t)c->G.nbins + 1;
+    if (n + 1 != c->esc_planes) {
+        double *e = nullptr;
+        HIPCHK(c, dalloc(&e, nc1 * nb1 * (size_t)(n + 1)));
+        (void)hipFree(c->d_esc);
+        c->d_esc = e;
+        c->G.esc = e;
+        c->esc_planes = n + 1;
+    }
+    HIPCHK(c, hipMemset(c->d_esc, 0, nc1 * nb1 * c->esc_planes * sizeof(double)));
+    if (n > 0) {
+        if (!c->d_view) HIPCHK(c, hipMalloc((void **)&c->d_view, sizeof(mcd::View)));
+        HIPCHK(c, hipMemcpy(c->d_view, &h, sizeof h, hipMemcpyHostToDevice));
+        c->G.view = c->d_view;
+    } else {
+        c->G.view = nullptr;
+    }
     return MC_OK;
 }
 
@@ -1673,6 +1723,8 @@ static const char *packet_error_text(int code)
     case 37: return "pathSegment: insane position (NaN)";
     case 39: return "pathSegment: insanity at the scattering/absorption decision";
     case 41: return "dust-only re-emission (dustEmi) is outside the supported path";
+    case 60: return "pathSegment: error in theta direction cosine assignment";
+    case 61: return "pathSegment: error in Phi direction cosine assignment";
     default: return "packet fail-stop";
     }
 }
@@ -1915,7 +1967,7 @@ extern "C" void mc_ctx_destroy(mc_ctx *c)
     (void)hipSetDevice(c->device);
     void *bufs[] = { c->d_axis, c->d_mid, c->d_dfac, c->d_active, c->d_opac3, c->d_sca3,
                      c->d_recCDF, c->d_recGuide, c->d_totalLines, c->d_specCDF, c->d_nuArray,
-                     c->d_gSca, c->d_J3, c->d_J, c->d_esc, c->d_scratch, c->d_error, c->d_qphot,
+                     c->d_gSca, c->d_J3, c->d_J, c->d_esc, c->d_view, c->d_scratch, c->d_error, c->d_qphot,
                      c->d_cell_xyz, c->d_atomic, c->d_bal_tab, c->d_Hden, c->d_Te, c->d_Ne,
                      c->d_ion, c->d_flags, c->d_nIter, c->d_cold, c->d_mlp, c->d_mlpW, c->d_mX,
                      c->d_mY, c->d_mX0, c->d_hit, c->d_cross, c->d_Rint, c->d_Hint,

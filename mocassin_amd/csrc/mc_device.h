@@ -22,6 +22,17 @@ constexpr int kRecursionLimit = 5000;   // constants_mod.f90:56
 enum Flags { F_SYM = 1, F_DUST = 2, F_GAS = 4, F_ISO = 8, F_MULTI = 16 };
 
 // Per-problem constants and device tables.
+// Viewing-angle bins of the escapes as initCartesianGrid sets them
+// (grid_mod.f90:416-467): theta and phi bin widths and the maps from a bin
+// to a viewpoint (0: none); the viewpoints' angles (0:nAngle).
+constexpr int kViewMax = 64;
+struct View {
+    int nAngle, totT, totP;
+    float dTheta, dPhi;
+    int Pt[kViewMax + 1], Pp[kViewMax + 1];
+    float vth[3], vph[3];
+};
+
 struct Grid {
     int nx, ny, nz, nCells, nbins, flags, nStars;
     float R_out, geoCorrX, geoCorrY, geoCorrZ, ionEdge1;
@@ -39,7 +50,8 @@ struct Grid {
     const float *nuArray;    // [nu]
     const float *gSca;       // [nu] or null
     double *J3;              // FP64 path-length estimator, tiled: see j3_index
-    double *esc;             // [nu 0..nbins][cell]
+    double *esc;             // [plane 0..nAngle][nu 0..nbins][cell]
+    const View *view;        // viewing-angle bins, or null (plane 0 only)
     unsigned *cross;         // [x,y,z] segments walked in each cell, or null
     uint32_t seed_lo, seed_hi;
 };

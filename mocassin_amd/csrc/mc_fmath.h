@@ -17,6 +17,11 @@
  *   sin/cos: Cody-Waite reduction by pi/2 with a 33-bit head constant
  *        (so k*head is exact for |k| < 2^20) and a 53-bit tail, then the
  *        Taylor series to r^19 / r^20 on |r| <= pi/4.
+ *   atan: |t| > 1 folded by atan(t) = pi/2 - atan(1/t), then two half-angle
+ *        steps u -> u / (1 + sqrt(1 + u^2)) to |u| <= tan(pi/16), and the
+ *        odd Taylor series to u^25 (truncation below 1e-18); IEEE sqrt is
+ *        the one operation beyond + - * /.
+ *   acos: acos(x) = 2 atan(sqrt((1-x)/(1+x))) on (-1, 1]; acos(-1) = pi.
  *
  * Must be compiled with floating-point contraction disabled.
  */
@@ -24,6 +29,7 @@
 #define MC_FMATH_H
 
 #include <stdint.h>
+#include <math.h>
 
 #if defined(__HIPCC__)
 #define MCF_FN __host__ __device__ static inline
@@ -120,6 +126,44 @@ MCF_FN void mcf_sincosf(float af, float *sp, float *cp)
     }
     *sp = (float)s;
     *cp = (float)c;
+}
+
+MCF_FN double mcf_atan_d(double t)
+{
+    if (t != t) return t;
+    const double sgn = t < 0.0 ? -1.0 : 1.0;
+    double a = t * sgn;
+    const int inv = a > 1.0;
+    if (inv) a = 1.0 / a;
+    a = a / (1.0 + sqrt(1.0 + a * a));
+    a = a / (1.0 + sqrt(1.0 + a * a));
+    const double z = a * a;
+    double p = 1.0 / 25.0;
+    p = 1.0 / 23.0 - p * z;
+    p = 1.0 / 21.0 - p * z;
+    p = 1.0 / 19.0 - p * z;
+    p = 1.0 / 17.0 - p * z;
+    p = 1.0 / 15.0 - p * z;
+    p = 1.0 / 13.0 - p * z;
+    p = 1.0 / 11.0 - p * z;
+    p = 1.0 / 9.0 - p * z;
+    p = 1.0 / 7.0 - p * z;
+    p = 1.0 / 5.0 - p * z;
+    p = 1.0 / 3.0 - p * z;
+    p = 1.0 - p * z;
+    double r = 4.0 * (a * p);
+    if (inv) r = 1.5707963267948966 - r;
+    return sgn * r;
+}
+
+MCF_FN float mcf_atanf(float x) { return (float)mcf_atan_d((double)x); }
+
+MCF_FN float mcf_acosf(float xf)
+{
+    const double x = (double)xf;
+    if (!(x >= -1.0 && x <= 1.0)) return (float)((x - x) / (x - x));   /* NaN */
+    if (x == -1.0) return (float)3.141592653589793;
+    return (float)(2.0 * mcf_atan_d(sqrt((1.0 - x) / (1.0 + x))));
 }
 
 #endif /* MC_FMATH_H */

@@ -101,9 +101,46 @@ __device__ __forceinline__ unsigned long long wcount(bool p)
     return (unsigned long long)__popcll(__ballot(p));
 }
 
-__device__ __forceinline__ void esc_add(const Grid &G, int origin, int nuP, float dE)
+__device__ __forceinline__ void esc_plane(const Grid &G, int a, int origin, int nuP, float dE)
 {
-    unsafeAtomicAdd(&G.esc[(size_t)nuP * (size_t)(G.nCells + 1) + origin], (double)dE);
+    const size_t i = ((size_t)a * (size_t)(G.nbins + 1) + (size_t)nuP) * (size_t)(G.nCells + 1) + origin;
+    unsafeAtomicAdd(&G.esc[i], (double)dE);
+}
+
+// An escape of a packet whose flight direction (enPacket%direction) is d:
+// its theta and phi bins, then the viewing-angle planes they feed, as every
+// escape site of pathSegment and energyPacketRun does
+// (photon_mod.f90:1582-1666, 370-465).  err 60/61: the reference's stops on
+// a bin out of range.  The bin tables were range-checked on the host.
+__device__ __forceinline__ void esc_add(const Grid &G, int origin, int nuP, float dE, float dx, float dy,
+                                        float dz, int &err)
+{
+    const View *V = G.view;
+    if (!V) { esc_plane(G, 0, origin, nuP, dE); return; }
+    const bool sym = (G.flags & F_SYM) != 0;
+    const float q = mcf_acosf(sym ? fabsf(dz) : dz) / V->dTheta;
+    if (isnan(q)) { err = 60; return; }
+    int it = (int)q + 1;
+    if (it > V->totT) it = V->totT;
+    if (it < 1) { err = 60; return; }
+    int ip;
+    if (fabsf(dx) < 1.e-35f) ip = 0;
+    else if (sym) ip = (int)(mcf_atanf(fabsf(dy) / fabsf(dx)) / V->dPhi);
+    else ip = (int)(mcf_atanf(dy / dx) / V->dPhi);
+    if (ip < 0) ip = V->totP + ip;
+    ip = ip + 1;
+    if (ip > V->totP) ip = V->totP;
+    if (ip < 1) { err = 61; return; }
+    const int t = V->Pt[it], ph = V->Pp[ip];
+    if (t > 0 && V->vph[t] < 0.f) {
+        esc_plane(G, t, origin, nuP, dE);
+        esc_plane(G, 0, origin, nuP, dE);
+    } else if (t == ph || V->vth[ph] == V->vth[t] || V->vph[t] == V->vph[ph]) {
+        esc_plane(G, t, origin, nuP, dE);
+        if (t != 0) esc_plane(G, 0, origin, nuP, dE);
+    } else {
+        esc_plane(G, 0, origin, nuP, dE);
+    }
 }
 
 __device__ __forceinline__ void j_add(const Grid &G, int idx, int nuP, float v)
@@ -154,7 +191,7 @@ __device__ __forceinline__ void emit(Lane &L, const Grid &G, const RunArgs &RA, 
     const float nu = G.nuArray[nuP - 1];
     if (stellar && nu > 1.f) qphot += (double)(RA.deltaE / (2.1799153e-11f * nu));
     if (!dust && nu < G.ionEdge1) {                      // escapes at birth (370-465)
-        esc_add(G, a, nuP, RA.deltaE);
+        esc_add(G, a, nuP, RA.deltaE, L.d0x, L.d0y, L.d0z, err);
         esc_end = true;
         L.phase = PH_IDLE;
         return;
@@ -234,7 +271,11 @@ __device__ __forceinline__ void advance(Lane &L, const Grid &G, const Axes &A, c
             if (r == ST_CONTINUE || r == ST_ESCAPED) {
                 j_add(G, L.W.segIdx, L.nuP, L.W.seg * M.dE / dV);
                 seg = true;
-                if (r == ST_ESCAPED) { esc_add(G, L.origin, L.nuP, M.dE); escaped = true; L.phase = PH_IDLE; }
+                if (r == ST_ESCAPED) {
+                    esc_add(G, L.origin, L.nuP, M.dE, L.d0x, L.d0y, L.d0z, err);
+                    escaped = true;
+                    L.phase = PH_IDLE;
+                }
             } else if (r == ST_LOST) {
                 lost = true;
                 L.phase = PH_IDLE;
@@ -243,7 +284,7 @@ __device__ __forceinline__ void advance(Lane &L, const Grid &G, const Axes &A, c
                 seg = true;
                 float rx = L.W.rx / 1.e10f, ry = L.W.ry / 1.e10f, rz = L.W.rz / 1.e10f;
                 if (sqrtf(rx * rx + ry * ry + rz * rz) * 1.e10f >= G.R_out && G.R_out > 0.f) {
-                    esc_add(G, L.origin, L.nuP, M.dE);        // 1577-1669
+                    esc_add(G, L.origin, L.nuP, M.dE, L.d0x, L.d0y, L.d0z, err);   // 1577-1669
                     escaped = true;
                     L.phase = PH_IDLE;
                 } else if (M.dust) {

@@ -102,6 +102,13 @@ module mocassin_amd_mod
         type(c_ptr) :: widFlx
     end type mc_emission_tables
 
+    ! the viewing-angle tables initCartesianGrid sets (grid_mod.f90:416-467)
+    type, bind(C) :: mc_view_angles
+        integer(c_int32_t) :: nAngleBins, totAngleBinsTheta, totAngleBinsPhi
+        real(c_float) :: dTheta, dPhi
+        type(c_ptr) :: viewPointPtheta, viewPointPphi, viewPointTheta, viewPointPhi
+    end type mc_view_angles
+
     type, bind(C) :: mc_grid_state
         type(c_ptr) :: Te, Ne, Hden, ionDen, lgConverged, lgBlack, Ndust, Tdust
         integer(c_int32_t) :: nSpeciesMax, nSizes
@@ -146,6 +153,12 @@ module mocassin_amd_mod
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx, J, esc
         end function mc_set_estimators
+
+        integer(c_int) function mc_set_view_angles(ctx, v) bind(C, name="mc_set_view_angles")
+            import :: c_int, c_ptr, mc_view_angles
+            type(c_ptr), value :: ctx
+            type(mc_view_angles), intent(in) :: v
+        end function mc_set_view_angles
 
         integer(c_int) function mc_walk_replay(ctx, n, pos, dir, cell, cell_seq, ds_seq, &
                                                seq_len, status, max_len) bind(C, name="mc_walk_replay")

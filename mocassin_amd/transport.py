@@ -95,10 +95,36 @@ class Transport:
                                                int(packet_offset), None, None, C.byref(c)))
         return c.as_dict()
 
+    def set_view_angles(self, view=None):
+        """Viewing-angle planes of the escapes (mc_set_view_angles): view is
+        None (plane 0 only) or dict(nAngleBins, totAngleBinsTheta,
+        totAngleBinsPhi, dTheta, dPhi, viewPointPtheta (0:totT), viewPointPphi
+        (0:totP), viewPointTheta, viewPointPhi (0:nAngleBins)).  Zeroes the
+        escapes."""
+        n = 0 if view is None else int(view["nAngleBins"])
+        if n == 0:
+            self._check(self._lib.mc_set_view_angles(self._h, None))
+            self._planes = 1
+            return
+        keep = [np.ascontiguousarray(view[k], np.int32) for k in ("viewPointPtheta", "viewPointPphi")] + \
+               [np.ascontiguousarray(view[k], np.float32) for k in ("viewPointTheta", "viewPointPhi")]
+        v = _capi.McViewAngles()
+        v.nAngleBins = n
+        v.totAngleBinsTheta, v.totAngleBinsPhi = int(view["totAngleBinsTheta"]), int(view["totAngleBinsPhi"])
+        v.dTheta, v.dPhi = float(np.float32(view["dTheta"])), float(np.float32(view["dPhi"]))
+        v.viewPointPtheta, v.viewPointPphi, v.viewPointTheta, v.viewPointPhi = (a.ctypes.data for a in keep)
+        self._check(self._lib.mc_set_view_angles(self._h, C.byref(v)))
+        self._planes = n + 1
+
+    def _esc_shape(self):
+        p = getattr(self, "_planes", 1)
+        return (self.nCells + 1, self.nbins + 1) + ((p,) if p > 1 else ())
+
     def fetch(self):
-        """(J (nCells+1,nbins), esc (nCells+1,nbins+1)) FP64, unscaled, order='F'."""
+        """(J (nCells+1,nbins), esc (nCells+1,nbins+1[,nAngleBins+1])) FP64,
+        unscaled, order='F'."""
         J = np.zeros((self.nCells + 1, self.nbins), np.float64, order="F")
-        E = np.zeros((self.nCells + 1, self.nbins + 1), np.float64, order="F")
+        E = np.zeros(self._esc_shape(), np.float64, order="F")
         self._check(self._lib.mc_fetch_estimators(self._h, J.ctypes.data, E.ctypes.data))
         return J, E
 
@@ -118,7 +144,7 @@ class Transport:
             raise ValueError("J of shape %s" % (J.shape,))
         Jf = np.asfortranarray(J)
         Ef = None if esc is None else np.asfortranarray(np.asarray(esc, np.float64))
-        if Ef is not None and Ef.shape != (self.nCells + 1, self.nbins + 1):
+        if Ef is not None and Ef.shape != self._esc_shape():
             raise ValueError("esc of shape %s" % (Ef.shape,))
         self._check(self._lib.mc_set_estimators(self._h, Jf.ctypes.data,
                                                 None if Ef is None else Ef.ctypes.data))

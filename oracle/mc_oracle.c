@@ -72,6 +72,16 @@ static float draw(rng_t *r)
 }
 
 /* --------------------------------------------------------- elementary fns */
+float mco_acosf(int mode, float x)
+{
+    return mode == MCO_MATH_OWN ? mcf_acosf(x) : acosf(x);
+}
+
+float mco_atanf(int mode, float x)
+{
+    return mode == MCO_MATH_OWN ? mcf_atanf(x) : atanf(x);
+}
+
 float mco_logf(int mode, float x)
 {
     return mode == MCO_MATH_OWN ? mcf_logf(x) : logf(x);
@@ -110,13 +120,50 @@ static void add_J(ctx_t *C, int cell, int nuP, float val)
     else ((double *)C->J)[idx] += (double)val;
 }
 
-static void add_esc(ctx_t *C, int origin, int nuP)
+static void esc_plane(ctx_t *C, int origin, int nuP, int a)
 {
     const mco_problem *P = C->P;
-    size_t idx = (size_t)origin + (size_t)(P->nCells + 1) * (size_t)nuP;
+    size_t idx = (size_t)origin + (size_t)(P->nCells + 1) * ((size_t)nuP + (size_t)(P->nbins + 1) * (size_t)a);
     if (P->j_mode == MCO_J_FP32) ((float *)C->esc)[idx] += P->deltaE;
     else ((double *)C->esc)[idx] += (double)P->deltaE;
+}
+
+/* An escape of a packet whose flight direction is dir (enPacket%direction):
+ * the theta and phi bins and the viewing-angle planes it feeds, as every
+ * escape site of pathSegment and energyPacketRun does
+ * (photon_mod.f90:1582-1666, 370-465).  Returns nonzero on the reference's
+ * "error in ... direction cosine assignment" stops. */
+static int add_esc(ctx_t *C, int origin, int nuP, const float dir[3])
+{
+    const mco_problem *P = C->P;
     C->cnt->n_esc += 1.0;
+    if (P->nAngleBins <= 0) { esc_plane(C, origin, nuP, 0); return 0; }
+    const int sym = P->lgSymmetricXYZ;
+    float q = mco_acosf(P->math_mode, sym ? fabsf(dir[2]) : dir[2]) / P->dTheta;
+    if (!(q == q)) { C->cnt->error = 60; return 1; }
+    int idirT = (int)q + 1;
+    if (idirT > P->totAngleBinsTheta) idirT = P->totAngleBinsTheta;
+    if (idirT < 1 || idirT > P->totAngleBinsTheta) { C->cnt->error = 60; return 1; }
+    int idirP;
+    if (fabsf(dir[0]) < 1.e-35f) idirP = 0;
+    else if (sym) idirP = (int)(mco_atanf(P->math_mode, fabsf(dir[1]) / fabsf(dir[0])) / P->dPhi);
+    else idirP = (int)(mco_atanf(P->math_mode, dir[1] / dir[0]) / P->dPhi);
+    if (idirP < 0) idirP = P->totAngleBinsPhi + idirP;
+    idirP = idirP + 1;
+    if (idirP > P->totAngleBinsPhi) idirP = P->totAngleBinsPhi;
+    if (idirP < 1 || idirP > P->totAngleBinsPhi) { C->cnt->error = 61; return 1; }
+    const int t = P->viewPointPtheta[idirT], ph = P->viewPointPphi[idirP];
+    if (t > 0 && P->viewPointPhi[t] < 0.f) {
+        esc_plane(C, origin, nuP, t);
+        esc_plane(C, origin, nuP, 0);
+    } else if (t == ph || P->viewPointTheta[ph] == P->viewPointTheta[t] ||
+               P->viewPointPhi[t] == P->viewPointPhi[ph]) {
+        esc_plane(C, origin, nuP, t);
+        if (t != 0) esc_plane(C, origin, nuP, 0);
+    } else {
+        esc_plane(C, origin, nuP, 0);
+    }
+    return 0;
 }
 
 /* randomUnitVector, vector_mod.f90:303-314 */
@@ -438,7 +485,7 @@ static int path_segment(ctx_t *C, pkt_t *pk, float rpos[3], int cell[3], seglog_
 
             float rx = rVec[0] / 1.e10f, ry = rVec[1] / 1.e10f, rz = rVec[2] / 1.e10f;
             if (sqrtf(rx * rx + ry * ry + rz * rz) * 1.e10f >= P->R_out && P->R_out > 0.f) {
-                add_esc(C, pk->origin, pk->nuP);
+                if (add_esc(C, pk->origin, pk->nuP, pk->dir)) return PS_ERROR;
                 if (L) L->status = 1;
                 return PS_DONE;
             }
@@ -498,7 +545,7 @@ static int path_segment(ctx_t *C, pkt_t *pk, float rpos[3], int cell[3], seglog_
                 if (rVec[0] >= AX(xA, nx) + P->geoCorrX || xP > nx) { xP = nx; ret = 1; }
                 if (rVec[2] <= AX(zA, 1) - P->geoCorrZ || zP < 1) { zP = 1; ret = 1; }
                 if (rVec[2] >= AX(zA, nz) + P->geoCorrZ || zP > nz) { zP = nz; ret = 1; }
-                if (ret) { add_esc(C, pk->origin, pk->nuP); if (L) L->status = 1; return PS_DONE; }
+                if (ret) { if (add_esc(C, pk->origin, pk->nuP, pk->dir)) return PS_ERROR; if (L) L->status = 1; return PS_DONE; }
             }
             /* escape through the grid boundary, 2421-2554 */
             if ((!P->lgSymmetricXYZ && (rVec[0] <= AX(xA, 1) - P->geoCorrX ||
@@ -508,7 +555,7 @@ static int path_segment(ctx_t *C, pkt_t *pk, float rpos[3], int cell[3], seglog_
                 rVec[1] >= AX(yA, ny) + P->geoCorrY ||
                 rVec[2] >= AX(zA, nz) + P->geoCorrZ ||
                 xP > nx || yP > ny || zP > nz) {
-                add_esc(C, pk->origin, pk->nuP);
+                if (add_esc(C, pk->origin, pk->nuP, pk->dir)) return PS_ERROR;
                 if (L) L->status = 1;
                 return PS_DONE;
             }
@@ -519,7 +566,7 @@ static int path_segment(ctx_t *C, pkt_t *pk, float rpos[3], int cell[3], seglog_
             }
         }
         if (xP > nx || yP > ny || zP > nz) {             /* 2728-2832 */
-            add_esc(C, pk->origin, pk->nuP);
+            if (add_esc(C, pk->origin, pk->nuP, pk->dir)) return PS_ERROR;
             if (L) L->status = 1;
             return PS_DONE;
         }
@@ -568,7 +615,7 @@ static int packet_run(ctx_t *C, int stellar, float pos[3], int cell[3])
     }
     if (!P->lgDust && pk.nu < P->ionEdge1 && !pk.lgLine) {
         if (pk.origin < 0) { C->cnt->error = 16; return PS_ERROR; }
-        add_esc(C, pk.origin, pk.nuP);
+        if (add_esc(C, pk.origin, pk.nuP, pk.dir)) return PS_ERROR;
         return PS_DONE;
     }
     if (pk.lgLine) { C->cnt->n_line += 1.0; return PS_DONE; }
@@ -607,6 +654,7 @@ int mco_walk(const mco_problem *P, const float pos[3], const float dir[3],
     if (!zeros) { *status = -2; return 0; }
     Q.opacity = zeros;
     Q.j_mode = MCO_J_FP64;
+    Q.nAngleBins = 0;
     double *Jd = (double *)calloc(ntab, sizeof(double));
     double *Ed = (double *)calloc((size_t)(P->nCells + 1) * (size_t)(P->nbins + 1), sizeof(double));
     mco_counters cnt;

@@ -7,7 +7,8 @@
  * reads like the reference:
  *   active(1:nx,1:ny,1:nz)           active[(i-1) + nx*((j-1) + ny*(k-1))]
  *   table(0:nCells,1:nbins)          tab[c + (nCells+1)*(nu-1)]
- *   escapedPackets(0:nCells,0:nbins) esc[c + (nCells+1)*nu]     (angle bin 0)
+ *   escapedPackets(0:nCells,0:nbins,0:nAngleBins)
+ *                                    esc[c + (nCells+1)*(nu + (nbins+1)*a)]
  */
 #ifndef MC_ORACLE_H
 #define MC_ORACLE_H
@@ -44,6 +45,14 @@ typedef struct {
     int j_mode;                /* MCO_J_* */
     uint64_t seed;             /* Philox key */
     uint32_t stream;           /* Philox counter word 3 (source index) */
+    /* viewing-angle bins of the escapes as initCartesianGrid sets them
+     * (grid_mod.f90:416-467); nAngleBins = 0 leaves esc with plane 0 only */
+    int nAngleBins, totAngleBinsTheta, totAngleBinsPhi;
+    float dTheta, dPhi;
+    const int32_t *viewPointPtheta;      /* (0:totAngleBinsTheta) */
+    const int32_t *viewPointPphi;        /* (0:totAngleBinsPhi) */
+    const float *viewPointTheta;         /* (0:nAngleBins) */
+    const float *viewPointPhi;           /* (0:nAngleBins) */
 } mco_problem;
 
 typedef struct {
@@ -64,6 +73,8 @@ int mco_getnu2(const float *cdf, int nbins, const float *u, int nu_avail, int *u
 /* Elementary functions in the selected mode. */
 float mco_logf(int mode, float x);
 void mco_sincosf(int mode, float a, float *s, float *c);
+float mco_acosf(int mode, float x);
+float mco_atanf(int mode, float x);
 
 /* Geometry-only walk (pathSegment with zero opacity) from pos/dir in the
  * given start cell.  Writes up to max_len segments (cell x,y,z 1-based and

@@ -36,6 +36,10 @@ class _Problem(C.Structure):
         ("lgIsotropic", C.c_int), ("lgMultiStars", C.c_int),
         ("math_mode", C.c_int), ("j_mode", C.c_int),
         ("seed", C.c_uint64), ("stream", C.c_uint32),
+        ("nAngleBins", C.c_int), ("totAngleBinsTheta", C.c_int), ("totAngleBinsPhi", C.c_int),
+        ("dTheta", C.c_float), ("dPhi", C.c_float),
+        ("viewPointPtheta", C.c_void_p), ("viewPointPphi", C.c_void_p),
+        ("viewPointTheta", C.c_void_p), ("viewPointPhi", C.c_void_p),
     ]
 
 
@@ -59,6 +63,9 @@ def lib():
         L.mco_getnu2.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_int)]
         L.mco_logf.restype = C.c_float
         L.mco_logf.argtypes = [C.c_int, C.c_float]
+        for f in ("mco_acosf", "mco_atanf"):
+            getattr(L, f).restype = C.c_float
+            getattr(L, f).argtypes = [C.c_int, C.c_float]
         L.mco_sincosf.argtypes = [C.c_int, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
         L.mco_walk.restype = C.c_int
         L.mco_walk.argtypes = [C.POINTER(_Problem), C.c_void_p, C.c_void_p, C.c_void_p,
@@ -110,8 +117,19 @@ class Problem:
     def __init__(self, xAxis, yAxis, zAxis, active, opacity, nuArray, specCDF,
                  R_out, geoCorr, ionEdge1, deltaE, starPos, starIdx,
                  recPDF=None, totalLines=None, scaOpac=None, gSca=None,
-                 symmetric=True, dust=False, gas=True, isotropic=False, multistars=False):
+                 symmetric=True, dust=False, gas=True, isotropic=False, multistars=False, view=None):
+        """view: None, or dict(nAngleBins, totAngleBinsTheta, totAngleBinsPhi,
+        dTheta, dPhi, viewPointPtheta (0:totT), viewPointPphi (0:totP),
+        viewPointTheta, viewPointPhi (0:nAngleBins)) as initCartesianGrid
+        sets them (grid_mod.f90:416-467)."""
         f32 = lambda a: None if a is None else np.asarray(a, np.float32)
+        self.view = None
+        if view is not None and int(view["nAngleBins"]) > 0:
+            self.view = dict(view)
+            for k in ("viewPointPtheta", "viewPointPphi"):
+                self.view[k] = np.ascontiguousarray(view[k], np.int32)
+            for k in ("viewPointTheta", "viewPointPhi"):
+                self.view[k] = np.ascontiguousarray(view[k], np.float32)
         self.xAxis, self.yAxis, self.zAxis = f32(xAxis), f32(yAxis), f32(zAxis)
         self.active = np.asfortranarray(active, np.int32)
         self.nx, self.ny, self.nz = self.active.shape
@@ -150,18 +168,32 @@ class Problem:
         s.lgIsotropic, s.lgMultiStars = int(self.isotropic), int(self.multistars)
         s.math_mode, s.j_mode = math_mode, j_mode
         s.seed, s.stream = seed, stream
+        if self.view is not None:
+            v = self.view
+            s.nAngleBins = int(v["nAngleBins"])
+            s.totAngleBinsTheta, s.totAngleBinsPhi = int(v["totAngleBinsTheta"]), int(v["totAngleBinsPhi"])
+            s.dTheta, s.dPhi = float(np.float32(v["dTheta"])), float(np.float32(v["dPhi"]))
+            s.viewPointPtheta, s.viewPointPphi = _ptr(v["viewPointPtheta"]), _ptr(v["viewPointPphi"])
+            s.viewPointTheta, s.viewPointPhi = _ptr(v["viewPointTheta"]), _ptr(v["viewPointPhi"])
         return s
 
+    @property
+    def nplanes(self):
+        return 1 if self.view is None else int(self.view["nAngleBins"]) + 1
+
     def transport(self, first, n, seed, stream=1, math_mode=MATH_OWN, j_mode=J_FP64):
-        """Returns (J, esc, counters dict); J (nCells+1, nbins), esc (nCells+1, nbins+1)."""
+        """Returns (J, esc, counters dict); J (nCells+1, nbins), esc (nCells+1,
+        nbins+1), or (nCells+1, nbins+1, nAngleBins+1) with viewing angles."""
         dt = np.float32 if j_mode == J_FP32 else np.float64
         J = np.zeros((self.nCells + 1, self.nbins), dt, order="F")
-        E = np.zeros((self.nCells + 1, self.nbins + 1), dt, order="F")
+        E = np.zeros((self.nCells + 1, self.nbins + 1, self.nplanes), dt, order="F")
         cnt = _Counters()
         s = self._struct(math_mode, j_mode, seed, stream)
         rc = lib().mco_transport(C.byref(s), first, n, J.ctypes.data, E.ctypes.data, C.byref(cnt))
         if rc:
             raise RuntimeError("oracle transport stopped with code %d" % rc)
+        if self.view is None:
+            E = E[:, :, 0]
         return J, E, {f: getattr(cnt, f) for f, _ in _Counters._fields_}
 
     def walk(self, pos, dirv, cell, max_len=4096):

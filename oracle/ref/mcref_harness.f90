@@ -389,6 +389,16 @@ contains
        call dump_r4('Jste_raw', grid3D(1)%Jste, (grid3D(1)%nCells+1)*nbins)
        call dump_r4('esc_raw', grid3D(1)%escapedPackets(:,:,0), (grid3D(1)%nCells+1)*(nbins+1))
        call dump_r4('transport_counters', q, 4)
+       if (nAngleBins > 0) then
+          ! every viewing-angle plane, and the tables initCartesianGrid set
+          call dump_r4('esc_all', grid3D(1)%escapedPackets, (grid3D(1)%nCells+1)*(nbins+1)*(nAngleBins+1))
+          call dump_i4('view_ints', (/ nAngleBins, totAngleBinsTheta, totAngleBinsPhi /), 3)
+          call dump_r4('view_real', (/ dTheta, dPhi /), 2)
+          call dump_i4('view_Ptheta', viewPointPtheta, totAngleBinsTheta+1)
+          call dump_i4('view_Pphi', viewPointPphi, totAngleBinsPhi+1)
+          call dump_r4('view_theta', viewPointTheta, nAngleBins+1)
+          call dump_r4('view_phi', viewPointPhi, nAngleBins+1)
+       end if
     end subroutine run_transport
 
     ! iteration_mod.f90:700-722 scaling, then 744-909 (balance, one rank)

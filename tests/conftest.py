@@ -47,6 +47,25 @@ def oracle_problem(g, **over):
     return O.Problem(**kw)
 
 
+def view_from_fixture(g):
+    """The viewing-angle tables of a transport fixture (dict for the oracle's
+    Problem(view=...) and Transport.set_view_angles), or None."""
+    if "view_ints" not in g:
+        return None
+    n, tt, tp = (int(v) for v in g["view_ints"])
+    return dict(nAngleBins=n, totAngleBinsTheta=tt, totAngleBinsPhi=tp, dTheta=g["view_real"][0],
+                dPhi=g["view_real"][1], viewPointPtheta=g["view_Ptheta"], viewPointPphi=g["view_Pphi"],
+                viewPointTheta=g["view_theta"], viewPointPhi=g["view_phi"])
+
+
+def reference_escapes_all(g):
+    """Reference raw FP32 escapedPackets (nCells+1, nbins+1, nAngleBins+1)."""
+    nc, nb, na = int(g["ints"][3]), int(g["ints"][4]), int(g["view_ints"][0])
+    E = np.zeros((nc + 1) * (nb + 1) * (na + 1), np.float32)
+    E[g["Eall_idx"]] = g["Eall_val"]
+    return E.reshape(na + 1, nb + 1, nc + 1).transpose(2, 1, 0)
+
+
 def reference_estimators(g):
     """Reference raw FP32 Jste (nCells+1,nbins) and escapedPackets (nCells+1,nbins+1)."""
     nc, nb = int(g["ints"][3]), int(g["ints"][4])

@@ -15,7 +15,7 @@ stores inputs and reference outputs as compressed .npz files:
                         reference's raw FP32 Jste and escapedPackets after
                         N packets with the Philox stream pinned per packet.
 
-usage: python tests/golden/make_fixtures.py [walk|transport|all]
+usage: python tests/golden/make_fixtures.py [walk|transport|view|all]
 """
 import os
 import subprocess
@@ -32,11 +32,11 @@ HARNESS = os.path.join(ROOT, "oracle", "_ref", "mcref_harness")
 WORK = os.path.join(ROOT, "oracle", "_ref", "work")
 
 
-def run_harness(deck, mode, args, env_extra=None, tag=None):
+def run_harness(deck, mode, args, env_extra=None, tag=None, overrides=None):
     wd = os.path.join(WORK, tag or deck)
     os.makedirs(os.path.join(wd, "out"), exist_ok=True)
     os.makedirs(os.path.join(wd, "output"), exist_ok=True)
-    decks.write_deck(wd, deck)
+    decks.write_deck(wd, deck, overrides)
     env = dict(os.environ)
     env.update(env_extra or {})
     with open(os.path.join(wd, "log_%s.txt" % mode), "w") as log:
@@ -121,9 +121,9 @@ def make_walk(deck, npk=60, seed=777):
     print("wrote", out, len(rows), "packets", len(jrows), "J entries")
 
 
-def make_transport(deck, npk, seed, name, opacity_scale=None):
+def make_transport(deck, npk, seed, name, opacity_scale=None, overrides=None):
     env = {"MCREF_OPACITY_SCALE": repr(float(opacity_scale))} if opacity_scale else None
-    d = run_harness(deck, "transport", [seed, npk], env_extra=env, tag=name)
+    d = run_harness(deck, "transport", [seed, npk], env_extra=env, tag=name, overrides=overrides)
     prob, (nx, ny, nz, nc, nb) = read_problem(d)
     J = np.fromfile(os.path.join(d, "Jste_raw.bin"), np.float32)
     E = np.fromfile(os.path.join(d, "esc_raw.bin"), np.float32)
@@ -131,9 +131,18 @@ def make_transport(deck, npk, seed, name, opacity_scale=None):
     nzJ = np.nonzero(J)[0]
     nzE = np.nonzero(E)[0]
     out = os.path.join(HERE, "transport_%s.npz" % name)
+    view = {}
+    if os.path.exists(os.path.join(d, "esc_all.bin")):
+        # viewing-angle planes escapedPackets(:,:,0:nAngleBins) and their tables
+        rd = lambda n, dt=np.float32: np.fromfile(os.path.join(d, n + ".bin"), dt)
+        Ea = rd("esc_all")
+        nzA = np.nonzero(Ea)[0]
+        view = dict(Eall_idx=nzA.astype(np.int32), Eall_val=Ea[nzA], view_ints=rd("view_ints", np.int32),
+                    view_real=rd("view_real"), view_Ptheta=rd("view_Ptheta", np.int32),
+                    view_Pphi=rd("view_Pphi", np.int32), view_theta=rd("view_theta"), view_phi=rd("view_phi"))
     np.savez_compressed(out, seed=np.uint64(seed), npackets=np.int64(npk), **prob,
                         J_idx=nzJ.astype(np.int32), J_val=J[nzJ],
-                        E_idx=nzE.astype(np.int32), E_val=E[nzE], counters=q)
+                        E_idx=nzE.astype(np.int32), E_val=E[nzE], counters=q, **view)
     print("wrote", out, "J nonzero", len(nzJ), "E nonzero", len(nzE), "counters", q)
 
 
@@ -285,6 +294,13 @@ if __name__ == "__main__":
     if what in ("opacity", "all"):
         make_opacity("c2mini", 20000, 4243, "c2mini")
         make_opacity("dust_pe", 20000, 1619, "dust_pe")
+    if what in ("view", "all"):
+        # two inclinations with phi (non-symmetric grid), and phi switched off
+        # by a negative phi on the symmetric octant (grid_mod.f90:419-427)
+        make_transport("full_mini", 8000, 4343, "full_mini_view", opacity_scale=100.0,
+                       overrides={"inclination": "2 0.6 0.9 2.2 5.5"})
+        make_transport("c1mini", 8000, 4444, "c1mini_view", opacity_scale=300.0,
+                       overrides={"inclination": "2 0.4 -1. 1.3 -1."})
     if what in ("transport", "all"):
         make_transport("c1mini", 20000, 12345, "c1mini")
         make_transport("c1mini", 8000, 999, "c1mini_thick", opacity_scale=300.0)

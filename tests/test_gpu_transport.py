@@ -89,6 +89,40 @@ def test_transport_equals_oracle(name):
     _cmp_estimators(J, E, Jo, Eo)
 
 
+@pytest.mark.parametrize("name", ["full_mini_view", "c1mini_view"])
+def test_viewing_angle_planes_equal_oracle(name):
+    """escapedPackets(:,:,0:2) with two inclinations (phi on, on the full
+    grid; phi off, on the symmetric octant): the GPU equals the oracle packet
+    for packet; the oracle equals the reference bit for bit in libm mode
+    (test_oracle_golden.py)."""
+    from mocassin_amd import McError
+    from conftest import reference_escapes_all, view_from_fixture
+    g = golden("transport_" + name)
+    view = view_from_fixture(g)
+    P = oracle_problem(g, view=view)
+    n, seed = int(g["npackets"]), int(g["seed"])
+    print("viewing angles vs oracle", name, n, flush=True)
+    Jo, Eo, co = P.transport(0, n, seed, 1, O.MATH_OWN, O.J_FP64)
+    T = transport_from_problem(P, seed)
+    T.set_view_angles(view)
+    c = T.energy_packet_driver(1, n, 0)
+    J, E = T.fetch()
+    assert E.shape == Eo.shape == (P.nCells + 1, P.nbins + 1, 3)
+    for k in ("n_seg", "n_abs", "n_sca", "n_esc", "n_lost", "n_line", "n_trapped", "n_flights"):
+        assert c[k] == co[k], k
+    _cmp_estimators(J, E, Jo, Eo)
+    Er = reference_escapes_all(g)
+    for a in range(3):
+        assert (E[:, :, a].sum() > 0) == (Er[:, :, a].sum() > 0), a
+    bad = dict(view, viewPointPtheta=np.where(np.asarray(view["viewPointPtheta"]) > 0, 3, 0))
+    with pytest.raises(McError):
+        T.set_view_angles(bad)
+    T.set_view_angles(None)
+    J2, E2 = T.fetch()
+    assert E2.shape == (P.nCells + 1, P.nbins + 1) and not E2.any()
+    T.close()
+
+
 @pytest.mark.parametrize("name", ["c1mini_thick", "full_mini_thick", "dust_mini"])
 def test_transport_vs_reference_within_noise(name):
     g = golden("transport_" + name)

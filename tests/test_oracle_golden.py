@@ -136,6 +136,43 @@ def test_oracle_transport_matches_reference_bit_exact(name):
         assert cnt["n_sca"] > 50            # Henyey-Greenstein scatterings exercised
 
 
+@pytest.mark.parametrize("name", ["full_mini_view", "c1mini_view"])
+def test_oracle_viewing_angle_planes_match_reference_bit_exact(name):
+    """escapedPackets(:,:,0:nAngleBins) of a run with two inclinations: the
+    theta/phi bins and the plane rules of every escape site
+    (photon_mod.f90:1582-1666), bit for bit in libm mode."""
+    from conftest import reference_escapes_all, view_from_fixture
+    g = golden("transport_" + name)
+    P = oracle_problem(g, view=view_from_fixture(g))
+    J, E, cnt = P.transport(0, int(g["npackets"]), int(g["seed"]), 1, O.MATH_LIBM, O.J_FP32)
+    Er = reference_escapes_all(g)
+    assert E.shape == Er.shape
+    assert np.array_equal(E.view(np.uint32), Er.view(np.uint32))
+    Jr, E0 = reference_estimators(g)
+    assert np.array_equal(J.view(np.uint32), Jr.view(np.uint32))
+    assert np.array_equal(E[:, :, 0].view(np.uint32), E0.view(np.uint32))
+    # both viewpoints collect escapes
+    assert (Er[:, :, 1].sum() > 0) and (Er[:, :, 2].sum() > 0)
+
+
+def test_own_acos_atan_are_correctly_rounded():
+    """The deterministic acos/atan (FP64 evaluation, one rounding) against
+    numpy's FP64 functions rounded once, over a sweep of inputs; and within
+    one ulp of the C library's float functions."""
+    rng = np.random.default_rng(5)
+    xs = np.concatenate([rng.uniform(-1, 1, 4000), [-1.0, -0.5, 0.0, 0.5, 1.0, 1e-8, -1e-8]]).astype(np.float32)
+    ts = np.concatenate([rng.standard_normal(2000) * 10.0, rng.uniform(-1e-3, 1e-3, 500),
+                         [0.0, 1.0, -1.0, 1e30, -1e30]]).astype(np.float32)
+    L = O.lib()
+    for f, xsamp, ref in (("mco_acosf", xs, np.arccos), ("mco_atanf", ts, np.arctan)):
+        own = np.array([getattr(L, f)(O.MATH_OWN, float(x)) for x in xsamp], np.float32)
+        want = ref(xsamp.astype(np.float64)).astype(np.float32)
+        assert np.array_equal(own, want), f
+        libm = np.array([getattr(L, f)(O.MATH_LIBM, float(x)) for x in xsamp], np.float32)
+        ulp = np.abs(own.view(np.int32).astype(np.int64) - libm.view(np.int32).astype(np.int64))
+        assert ulp.max() <= 1, f
+
+
 @pytest.mark.parametrize("name", ["c1mini_thick", "full_mini_thick"])
 def test_energy_bookkeeping(name):
     """Every packet ends exactly once: escape, line packet, drop or trap."""
