@@ -259,6 +259,11 @@ typedef struct {
  * the reference stops.  Replaces: those sites' viewing-angle branches. */
 int mc_set_view_angles(mc_ctx *ctx, const mc_view_angles *v);
 
+/* Planes of esc the context returns and takes: nAngleBins + 1 after
+ * mc_set_view_angles, else 1 (0 for a NULL context).  Lets a host check its
+ * escapedPackets array before a copy.  No counterpart in the reference. */
+int mc_esc_planes(const mc_ctx *ctx);
+
 /* Upload the atomic inputs of the balance (validated; see mc_atomic).
  * Requires H and He switched on, nstages <= 9, nbins <= 2048, at most 16
  * elements, 64 channels, 64 CEL model ions of <= 17 levels and <= 64

@@ -413,6 +413,11 @@ extern "C" int mc_set_estimators(mc_ctx *c, const double *J, const double *esc)
     return MC_OK;
 }
 
+extern "C" int mc_esc_planes(const mc_ctx *c)
+{
+    return c ? c->esc_planes : 0;
+}
+
 extern "C" int mc_set_view_angles(mc_ctx *c, const mc_view_angles *v)
 {
     if (!c) return MC_EINVAL;

@@ -160,6 +160,11 @@ module mocassin_amd_mod
             type(mc_view_angles), intent(in) :: v
         end function mc_set_view_angles
 
+        integer(c_int) function mc_esc_planes(ctx) bind(C, name="mc_esc_planes")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function mc_esc_planes
+
         integer(c_int) function mc_walk_replay(ctx, n, pos, dir, cell, cell_seq, ds_seq, &
                                                seq_len, status, max_len) bind(C, name="mc_walk_replay")
             import :: c_int, c_int32_t, c_ptr
@@ -332,6 +337,11 @@ module mocassin_amd_mod
         end function c_strlen
     end interface
 
+    ! energyPacketDriver by the rank of esc: plane 0 only, or with viewing angles
+    interface gpu_energy_packet_driver
+        module procedure gpu_energy_packet_driver_plane, gpu_energy_packet_driver_views
+    end interface gpu_energy_packet_driver
+
 contains
 
     ! message of the last failure on ctx (c_null_ptr: of the last failed create)
@@ -365,16 +375,36 @@ contains
     ! energyPacketDriver(iStar, n, grid): transport n packets of star iStar
     ! (packet indices offset+1 .. offset+n), then copy the running Jste
     ! (0:nCells,1:nbins) and escapedPackets(0:nCells,0:nbins) totals, in
-    ! double, to J and esc.
-    subroutine gpu_energy_packet_driver(ctx, iStar, n, offset, J, esc, cnt)
+    ! double, to J and esc.  Without viewing angles only: with them, pass
+    ! esc(0:nCells,0:nbins,0:nAngleBins) (gpu_energy_packet_driver_views,
+    ! the same generic name).
+    subroutine gpu_energy_packet_driver_plane(ctx, iStar, n, offset, J, esc, cnt)
         type(c_ptr), intent(in) :: ctx
         integer, intent(in) :: iStar
         integer(c_int64_t), intent(in) :: n, offset
         real(c_double), intent(inout), target :: J(:, :), esc(:, :)
         type(mc_counters), intent(out) :: cnt
+        if (mc_esc_planes(ctx) /= 1) then
+            print *, '! energyPacketDriver: viewing angles are on; esc needs its (0:nAngleBins) planes'
+            stop
+        end if
         call mc_check(mc_transport_run(ctx, int(iStar, c_int32_t), n, offset, c_loc(J), &
                                        c_loc(esc), cnt), ctx, 'energyPacketDriver')
-    end subroutine gpu_energy_packet_driver
+    end subroutine gpu_energy_packet_driver_plane
+
+    subroutine gpu_energy_packet_driver_views(ctx, iStar, n, offset, J, esc, cnt)
+        type(c_ptr), intent(in) :: ctx
+        integer, intent(in) :: iStar
+        integer(c_int64_t), intent(in) :: n, offset
+        real(c_double), intent(inout), target :: J(:, :), esc(:, :, :)
+        type(mc_counters), intent(out) :: cnt
+        if (size(esc, 3) /= mc_esc_planes(ctx)) then
+            print *, '! energyPacketDriver: esc has ', size(esc, 3), ' planes; the context ', mc_esc_planes(ctx)
+            stop
+        end if
+        call mc_check(mc_transport_run(ctx, int(iStar, c_int32_t), n, offset, c_loc(J), &
+                                       c_loc(esc), cnt), ctx, 'energyPacketDriver')
+    end subroutine gpu_energy_packet_driver_views
 
     ! The updateCell loop over every active cell: Te, Ne (0:nCells) and
     ! ionDen(0:nCells, nElementsUsed, nstages) are REAL as the host keeps

@@ -105,6 +105,7 @@ def test_viewing_angle_planes_equal_oracle(name):
     Jo, Eo, co = P.transport(0, n, seed, 1, O.MATH_OWN, O.J_FP64)
     T = transport_from_problem(P, seed)
     T.set_view_angles(view)
+    assert T._lib.mc_esc_planes(T._h) == 3
     c = T.energy_packet_driver(1, n, 0)
     J, E = T.fetch()
     assert E.shape == Eo.shape == (P.nCells + 1, P.nbins + 1, 3)
@@ -118,6 +119,7 @@ def test_viewing_angle_planes_equal_oracle(name):
     with pytest.raises(McError):
         T.set_view_angles(bad)
     T.set_view_angles(None)
+    assert T._lib.mc_esc_planes(T._h) == 1
     J2, E2 = T.fetch()
     assert E2.shape == (P.nCells + 1, P.nbins + 1) and not E2.any()
     T.close()
