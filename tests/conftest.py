@@ -12,7 +12,7 @@ for p in (ROOT, os.path.join(ROOT, "oracle"), GOLDEN):
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an MI355X (runs on the GPU box)")
+    config.addinivalue_line("markers", "gpu: needs an MI355X GPU")
 
 
 def golden(name):
