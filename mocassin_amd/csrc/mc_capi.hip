@@ -94,6 +94,10 @@ struct mc_ctx {
     int *d_ln_black = nullptr;
     // re-emission tables on the device
     bool emis = false, rec_built = false;
+    // mc_emission_build writes into these and swaps them with d_recCDF,
+    // d_recGuide and d_totalLines only when every cell's row is good
+    float *d_recCDF_b = nullptr, *d_totalLines_b = nullptr;
+    unsigned short *d_recGuide_b = nullptr;
     int *d_em_i = nullptr, *d_em_err = nullptr;
     float *d_em_f = nullptr;
     double *d_em_d = nullptr, *d_em_st = nullptr, *d_em_Td = nullptr;
@@ -138,6 +142,12 @@ extern "C" int mc_ctx_create(const mc_desc *d, mc_ctx **out)
         !d->xAxis || !d->yAxis || !d->zAxis || !d->active || !d->nuArray || !d->starPos ||
         !d->starIdx || !d->deltaE)
         return fail(nullptr, MC_EINVAL, "descriptor has empty sizes or null arrays");
+    // the energy of a packet of each source (iteration_mod.f90:477-493
+    // divides the luminosity by the packet count): finite and positive
+    for (int s = 0; s < d->nStars; ++s)
+        if (!(d->deltaE[s] > 0.f) || std::isinf(d->deltaE[s]))
+            return fail(nullptr, MC_EINVAL, "deltaE of source %d is %g: must be finite and > 0", s + 1,
+                        (double)d->deltaE[s]);
     mc_ctx *c = new mc_ctx();
     int rc = ctx_init(c, d);
     if (rc) {
@@ -1329,30 +1339,34 @@ extern "C" int mc_emission_build(mc_ctx *c, const float *Hden, const float *Te, 
             a.Tdust = c->d_Tdust;
         }
     }
-    if (!c->d_recCDF) {
-        HIPCHK(c, dalloc(&c->d_recCDF, nt));
-        HIPCHK(c, hipMemset(c->d_recCDF, 0, nt * sizeof(float)));
+    // the rows are written into the back buffers; the tables the transport
+    // samples change only when every row is good
+    if (!c->d_recCDF_b) {
+        HIPCHK(c, dalloc(&c->d_recCDF_b, nt));
+        HIPCHK(c, hipMemset(c->d_recCDF_b, 0, nt * sizeof(float)));
     }
-    if (!c->d_recGuide) HIPCHK(c, dalloc(&c->d_recGuide, nc1 * (mcd::kGuide + 1)));
-    if (!c->d_totalLines) {
-        HIPCHK(c, dalloc(&c->d_totalLines, nc1));
-        HIPCHK(c, hipMemset(c->d_totalLines, 0, nc1 * sizeof(float)));
+    if (!c->d_recGuide_b) HIPCHK(c, dalloc(&c->d_recGuide_b, nc1 * (mcd::kGuide + 1)));
+    if (!c->d_totalLines_b) {
+        HIPCHK(c, dalloc(&c->d_totalLines_b, nc1));
+        HIPCHK(c, hipMemset(c->d_totalLines_b, 0, nc1 * sizeof(float)));
     }
     if (!c->d_em_err) HIPCHK(c, dalloc(&c->d_em_err, nc1));
-    a.rec = c->d_recCDF; a.totalLines = c->d_totalLines; a.err = c->d_em_err;
+    a.rec = c->d_recCDF_b; a.totalLines = c->d_totalLines_b; a.err = c->d_em_err;
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     HIPCHK(c, mce_launch(c->d_atomic, a, c->stream));
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-    HIPCHK(c, mcd_guides(c->d_recCDF, nc + 1, nb, c->d_recGuide, c->stream));
+    HIPCHK(c, mcd_guides(c->d_recCDF_b, nc + 1, nb, c->d_recGuide_b, c->stream));
     std::vector<int> err(nc1);
     HIPCHK(c, hipMemcpyAsync(err.data(), c->d_em_err, nc1 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    c->G.recCDF = c->d_recCDF; c->G.recGuide = c->d_recGuide; c->G.totalLines = c->d_totalLines;
     for (int cell = 1; cell <= nc; ++cell)
-        if (err[cell] == 1) {
-            c->rec_built = false;
-            return fail(c, MC_ETABLE, "cell %d: the state or its emission is not finite", cell);
-        }
+        if (err[cell] == 1)
+            return fail(c, MC_ETABLE, "cell %d: the state is not finite (or Te <= 0); the previous "
+                        "re-emission tables are kept", cell);
+    std::swap(c->d_recCDF, c->d_recCDF_b);
+    std::swap(c->d_recGuide, c->d_recGuide_b);
+    std::swap(c->d_totalLines, c->d_totalLines_b);
+    c->G.recCDF = c->d_recCDF; c->G.recGuide = c->d_recGuide; c->G.totalLines = c->d_totalLines;
     c->rec_built = true;
     if (kernel_ms) {
         float ms = 0.f;
@@ -1979,7 +1993,8 @@ extern "C" void mc_ctx_destroy(mc_ctx *c)
                      c->d_dust_tab, c->d_Tdust, c->d_dust_mono, c->d_pe_tab, c->d_grains, c->d_op_dust,
                      c->d_op_i, c->d_op_f, c->d_op_st, c->d_op_Td, c->d_op_out, c->d_op_ent,
                      c->d_ln_part, c->d_ln_st, c->d_ln_out, c->d_ln_black, c->d_em_i, c->d_em_err,
-                     c->d_em_f, c->d_em_d, c->d_em_st, c->d_em_Td };
+                     c->d_em_f, c->d_em_d, c->d_em_st, c->d_em_Td, c->d_recCDF_b, c->d_recGuide_b,
+                     c->d_totalLines_b };
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

@@ -136,16 +136,9 @@ __global__ void __launch_bounds__(64) emission_kernel(const mcb::Atomic *__restr
     const float log10Te = log10f(Te);
     if (NeUsed < 1.e-5f) NeUsed = 1.e-5f;
     const float xHp = XION(a.eH, 2), xHe1 = XION(a.eHe, 2), xHe2 = XION(a.eHe, 3);
-    if (Te <= 1.f) {
-        // iterateT's floor (update_mod.f90:454): there the reference's fits
-        // give 0/0 and inf*0; the cell emits nothing, its absorbed packets
-        // leave as lines and its row (never sampled) is 0 .. 0 1
-        float *row = a.rec + (size_t)cell * nb;
-        for (int i = lane; i < nb; i += W) row[i] = i == nb - 1 ? 1.f : 0.f;
-        if (lane == 0) {
-            a.totalLines[cell] = 1.f;
-            a.err[cell] = 2;
-        }
+    if (!(Te > 0.f) || !isfinite(Te) || !isfinite(NeIn)) {
+        // not a state the balance leaves (its floor is 1 K): refused
+        if (lane == 0) a.err[cell] = 1;
         return;
     }
 
@@ -402,24 +395,37 @@ __global__ void __launch_bounds__(64) emission_kernel(const mcb::Atomic *__restr
         }
         const float tl = normRec + normFor;
         normalize = normalize + tl;
-        // a cell that emits nothing (Te at the reference's 1 K floor) has
-        // 0/0 in the reference; its absorbed packets leave as line photons
-        a.totalLines[cell] = normalize > 0.f ? tl / normalize : 1.f;
+        // the reference's fraction; at a few tens of K its fits give inf*0
+        // and it is NaN, which newPhotonPacket's test random <= totalLines
+        // never passes (photon_mod.f90:925-929): the same as 0
+        float f = tl / normalize;
+        a.err[cell] = isnan(f) ? 2 : 0;
+        if (isnan(f)) f = 0.f;
+        a.totalLines[cell] = f;
         red[0] = s1[nb - 1];
-        // a continuum entirely below the 1e-15 cut leaves the reference a
-        // row of 0/0; with totalLines = 1 the row is never sampled, so it is
-        // written 0 .. 0 1 here.  err: 1 a state that is not finite, 2 a
-        // cell that emits nothing
-        const bool cont = s1[nb - 1] > 0.f && s1[nb - 1] < INFINITY;
-        const bool finite = std::isfinite(Te) && std::isfinite(NeIn) && std::isfinite(tl) && std::isfinite(s1[nb - 1]);
-        a.err[cell] = !finite ? 1 : (!cont && !(tl > 0.f) ? 2 : 0);
     }
     __syncthreads();
+    // recPDF = cumulative / last (setDiffusePDF).  In barely lit cells the
+    // iteration leaves at a few tens of K, the REAL cumulative sum
+    // overflows (the Milne factor exp((nu_edge - nu) hcRyd/kT) of the bins
+    // between a cross-section's start and its edge bin), so entries from the
+    // overflow on are inf/inf = NaN; no finite total divides 0/0 = NaN
+    // everywhere.  getNu2's scan (photon_mod.f90:745-752) stops at the
+    // first entry random >= probDen fails, which a NaN entry fails as an
+    // entry of 1 does: from the first NaN on the row is written 1, and the
+    // transport draws the bins the reference draws.
     const float last = red[0];
-    const bool cont = last > 0.f && last < INFINITY;
+    int m = nb;
+    for (int i0 = 0; i0 < nb; i0 += W) {
+        const int i = i0 + lane;
+        const uint64_t bad = __ballot(i < nb && isnan(s1[i] / last));
+        if (bad) { m = i0 + __ffsll((unsigned long long)bad) - 1; break; }
+    }
+    if (lane == 0 && m < nb) a.err[cell] = 2;
     float *row = a.rec + (size_t)cell * nb;
     for (int i = lane; i < nb; i += W) {
-        float v = cont ? s1[i] / last : 0.f;
+        float v = s1[i] / last;
+        if (i >= m) v = 1.f;
         if (i == nb - 1) v = 1.f;
         if (v > 0.999998f) v = 1.f;
         row[i] = v;

@@ -153,7 +153,8 @@ def build_problem(n, edge, R_out, nH, Q_H, T_star, nbins, abundances, symmetric=
     phot = (bb / nu64)[ion].sum()
     L_erg = Q_H * 2.1799e-11 * bb.sum() / phot               # erg/s
     L36 = L_erg / 1.e36
-    deltaE = f32(L36 / n_packets)
+    # a model launched with no packets still has a finite packet energy
+    deltaE = f32(L36 / max(n_packets, 1))
 
     # cell-centre radii and densities
     X, Y, Z = np.meshgrid(x, y, z, indexing="ij")

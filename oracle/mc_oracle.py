@@ -103,6 +103,25 @@ def sincosf(a, mode=MATH_OWN):
     return s.value, c.value
 
 
+def sampling_tables(rec, totalLines):
+    """The re-emission tables as the reference samples them when they hold
+    NaN (rows of cells at a few tens of K, where emissionDriver's REAL sums
+    overflow): getNu2's scan (photon_mod.f90:745-752) stops at the first
+    entry u >= probDen fails, which a NaN entry fails as an entry of 1 does,
+    so each row becomes 1 from its first NaN on; newPhotonPacket's test
+    random <= totalLines (925-929) never passes a NaN, as with 0.  rec:
+    rows [..., nbins]; returns copies (rec, totalLines)."""
+    rec = np.array(rec, np.float32, copy=True)
+    tl = np.array(totalLines, np.float32, copy=True)
+    r2 = rec.reshape(-1, rec.shape[-1])
+    for row in r2:
+        bad = np.isnan(row)
+        if bad.any():
+            row[int(np.argmax(bad)):] = 1.0
+    tl[np.isnan(tl)] = 0.0
+    return r2.reshape(rec.shape), tl
+
+
 def getnu2(cdf, u):
     cdf = np.ascontiguousarray(cdf, np.float32)
     u = np.ascontiguousarray(u, np.float32)

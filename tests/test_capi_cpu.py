@@ -44,6 +44,31 @@ def test_create_rejects_bad_descriptors_without_device():
     assert b"empty" in L.mc_ctx_last_error(None)
 
 
+@pytest.mark.parametrize("dE", [float("inf"), float("nan"), 0.0, -1.0])
+def test_create_rejects_bad_packet_energy_without_device(dE):
+    """deltaE = L/N must be finite and positive; the check comes before any
+    device call, so it runs here."""
+    import numpy as np
+    from mocassin_amd import _capi
+    L = _capi.load()
+    ax = np.array([0.0, 1.0, 2.0], np.float32)
+    act = np.ones(27, np.int32)
+    nu = np.array([0.5, 1.0, 2.0], np.float32)
+    pos = np.zeros(3, np.float32)
+    idx = np.ones(3, np.int32)
+    de = np.array([dE], np.float32)
+    d = _capi.McDesc()
+    d.abi_version = _capi.MC_ABI_VERSION
+    d.nx = d.ny = d.nz = 3
+    d.xAxis = d.yAxis = d.zAxis = ax.ctypes.data
+    d.active = act.ctypes.data
+    d.nCells, d.nbins, d.nStars = 27, 3, 1
+    d.nuArray, d.starPos, d.starIdx, d.deltaE = nu.ctypes.data, pos.ctypes.data, idx.ctypes.data, de.ctypes.data
+    h = C.c_void_p()
+    assert L.mc_ctx_create(C.byref(d), C.byref(h)) == 1 and not h.value
+    assert b"deltaE" in L.mc_ctx_last_error(None)
+
+
 def test_calls_on_null_context_fail_cleanly():
     from mocassin_amd import _capi
     L = _capi.load()

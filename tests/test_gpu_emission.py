@@ -119,25 +119,99 @@ def test_device_state_after_balance():
     assert d < 1e-4 and r < 1e-4
 
 
-def test_cell_at_the_1K_floor():
-    """A cell the reference's schedule drove to its 1 K floor emits nothing
-    (the reference divides 0 by 0): its absorbed packets leave as lines
-    (totalLines 1) and its never-sampled row is 0 .. 0 1; a state that is
-    not finite is refused."""
-    print("emission of a cell at 1 K", flush=True)
-    from mocassin_amd._capi import McError
+def test_cold_cells_sample_as_the_reference():
+    """Cells at 1, 5, 19.554 and 60 K (what iterateT's schedule leaves in
+    barely lit cells): the reference's own rows hold NaN from the bin where
+    its REAL sums overflow, and its totalLines is NaN.  The device writes
+    what the reference's sampling does with them (1 from the first NaN on,
+    totalLines 0), equal to the reference's rows mapped so, to the last bits
+    of the device's exp/pow."""
+    print("emission of cold cells", flush=True)
     T, g, b, ion, dust = _ctx("c2mini")
-    Te, Ne = g["Te"].copy(), g["Ne"].copy()
-    Te[5], Ne[5] = 1.0, 1.e-5
-    ion = ion.copy()
-    ion[5, :, :] = 0.0
-    ion[5, :, 0] = 1.0
+    Te = g["Te"].copy()
+    cells = g["cold_cells"]
+    Te[cells] = g["cold_Te"]
+    want, want_tl = O.sampling_tables(g["cold_recPDF"], g["cold_totalLines"])
     try:
-        rec, tl, _ = T.emission_build(g["Hden"], Te, Ne, ion)
-        assert tl[5] == 1.0 and np.all(rec[5, :-1] == 0) and rec[5, -1] == 1.0
-        assert np.all(tl[6:] < 1)
-        Te[7] = np.nan
-        with pytest.raises(McError):
-            T.emission_build(g["Hden"], Te, Ne, ion)
+        rec, tl, _ = T.emission_build(g["Hden"], Te, g["Ne"], ion)
     finally:
         T.close()
+    assert np.all(np.diff(rec[1:], axis=1) >= 0) and np.all(rec[1:, -1] == 1.0)
+    assert np.array_equal(tl[cells], want_tl)
+    assert np.max(np.abs(rec[cells] - want)) < 5e-7
+    other = np.setdiff1d(np.arange(1, T.nCells + 1), cells)
+    assert np.all((tl[other] > 0) & (tl[other] < 1))
+
+
+def test_refused_build_keeps_the_previous_tables():
+    """A state that is not finite is refused (MC_ETABLE) and the tables of
+    the last good build stay in place: a sweep after the refusal equals one
+    right after the good build, packet for packet."""
+    print("refused emission build", flush=True)
+    from mocassin_amd._capi import McError
+    T, g, b, ion, dust = _ctx("c2mini")
+    nb = T.nbins
+    from test_gpu_opacity import _segments
+    spec = np.linspace(0, 1, nb).astype(np.float32)
+    try:
+        T.opacity_setup(*_segments(g))
+        T.opacity_build(g["Hden"], g["Te"], g["Ne"], ion, host=False)
+        rec, tl, _ = T.emission_build(g["Hden"], g["Te"], g["Ne"], ion)
+        T.upload_tables(None, spec)
+        T.zero_estimators()
+        c1 = T.energy_packet_driver(1, 20000, 0)
+        J1, E1 = T.fetch()
+        Te = g["Te"].copy()
+        Te[7] = np.nan
+        with pytest.raises(McError, match="MC_ETABLE"):
+            T.emission_build(g["Hden"], Te, g["Ne"], ion)
+        T.zero_estimators()
+        c2 = T.energy_packet_driver(1, 20000, 0)
+        J2, E2 = T.fetch()
+    finally:
+        T.close()
+    for k in ("n_seg", "n_abs", "n_esc", "n_line", "n_flights"):
+        assert c1[k] == c2[k], k
+    assert np.allclose(J1, J2, rtol=1e-12, atol=0) and np.allclose(E1, E2, rtol=1e-12, atol=0)
+
+
+def test_barely_lit_chain_balance_then_emission():
+    """The chain that stopped a whole-iteration run: a balance from a cold,
+    mostly neutral state on an estimator 1e10 times weaker leaves cells at
+    1 K and at a few K (the reference's schedule: Te/1.2 until the cap);
+    the re-emission tables of that device state are built, equal the
+    restatement's (reference precision, FP64 populations) mapped as the
+    reference samples them, and the sweep runs on them."""
+    print("barely lit chain", flush=True)
+    T, g, b, ion, dust = _ctx("c2mini")
+    bf = balance_fixture("c2mini")
+    J, Hden, Te0, Ne0, ion0 = bf[1], bf[2], bf[3], bf[4], bf[5]
+    nb = T.nbins
+    from test_gpu_opacity import _segments
+    try:
+        T.set_estimators(np.asarray(J, np.float64) * 1e-8)
+        Te1, Ne1, ion1 = T.update_cells(Hden, Te0, Ne0, ion0)[:3]
+        T.set_estimators(np.asarray(J, np.float64) * 1e-10)
+        Te2, Ne2, ion2, fl2, nIt2, _ = T.update_cells(Hden, Te1, Ne1, ion1)
+        t = Te2[1:]
+        assert np.all(np.isfinite(t)) and np.all(t >= 1.0) and np.all(np.isfinite(Ne2[1:]))
+        cold = np.nonzero(Te2 < 60.0)[0]
+        cold = cold[cold > 0]
+        assert cold.size > 20 and np.any(Te2[cold] > 1.0)
+        rec, tl, _ = T.emission_build()
+        T.opacity_setup(*_segments(g))
+        T.opacity_build(host=False)
+        T.upload_tables(None, np.linspace(0, 1, nb).astype(np.float32))
+        T.zero_estimators()
+        c = T.energy_packet_driver(1, 20000, 0)
+    finally:
+        T.close()
+    f32 = lambda a: np.asarray(a, np.float32)
+    ex, tex = O.Balance(b).emission(O.emission_tables(g)[0], f32(Te2), f32(Ne2), f32(ion2),
+                                    np.arange(1, Te2.size), "f64")
+    assert np.isnan(ex[cold - 1]).any()
+    want, want_tl = O.sampling_tables(ex, tex)
+    assert np.all(np.diff(rec[1:], axis=1) >= 0) and np.all(rec[1:, -1] == 1.0)
+    assert np.max(np.abs(rec[1:] - want)) < 5e-7
+    assert np.max(np.abs(tl[1:] - want_tl)) < 1e-6
+    assert c["n_seg"] > 0 and c["n_flights"] >= 20000
