@@ -18,6 +18,15 @@
 !   mcref_harness walk      OUTDIR SEED NPACKETS NUBIN ORIGINFILE
 !   mcref_harness balance   OUTDIR SEED NPACKETS
 !   mcref_harness opacity   OUTDIR SEED NPACKETS
+!   mcref_harness iterate   OUTDIR SEED          (under mpiexec: any rank count)
+!   mcref_harness sweep     OUTDIR SEED NPACKETS (under mpiexec: any rank count)
+!
+! iterate runs mocassin.f90's MCIterationDriver to the deck's convergence
+! criterion with the random stream of rank r pinned to (SEED, 0, r); the
+! state files writeGrid leaves under output/ are the converged state.
+! sweep times one energyPacketDriver call of NPACKETS/ranks packets per rank
+! (the transport phase of iterateMC, iteration_mod.f90:474-496) on the
+! tables of the initial state, with MPI_Wtime, and prints packets/s.
 program mcref_harness
     use common_mod
     use constants_mod
@@ -34,6 +43,7 @@ program mcref_harness
     use photon_mod
     use update_mod
     use readdata_mod
+    use iteration_mod
     use iso_c_binding
     implicit none
     include 'mpif.h'
@@ -59,8 +69,10 @@ program mcref_harness
     call get_command_argument(1, mode)
     call get_command_argument(2, outdir)
     seed = 0; npk = 0
-    if (command_argument_count() >= 4) then
+    if (command_argument_count() >= 3) then
        call get_command_argument(3, arg); read(arg, *) seed
+    end if
+    if (command_argument_count() >= 4) then
        call get_command_argument(4, arg); read(arg, *) npk
     end if
 
@@ -96,7 +108,22 @@ program mcref_harness
        end do
     end if
 
+    if (trim(mode) == "iterate") then
+       ! mocassin.f90:106-124 with the random stream pinned per rank
+       stream = 0
+       p = taskid
+       call mcref_set_stream(seed, stream, p)
+       call MCIterationDriver(grid3D(1:nGrids))
+       call mpi_finalize(ierr)
+       stop
+    end if
+
     call build_tables()
+    if (trim(mode) == "sweep") then
+       call run_sweep()
+       call mpi_finalize(ierr)
+       stop
+    end if
     if (lgGas) call dump_atomic()
     ! optional uniform opacity scaling (stress fixtures with many absorptions)
     call get_environment_variable("MCREF_OPACITY_SCALE", arg, status=ios)
@@ -118,6 +145,7 @@ program mcref_harness
        call run_balance()
        call run_lines()
        call run_emission()
+       if (lgGas .and. .not. lgDust) call run_cold_emission()
        call run_opacity()
        ! the state files of writeGrid (grid_mod.f90:2646-2872) under output/
        call writeGrid(grid3D(1:nGrids))
@@ -401,6 +429,30 @@ contains
        end if
     end subroutine run_transport
 
+    ! the transport phase of iterateMC (iteration_mod.f90:458-496) timed on
+    ! every rank: NPACKETS/ranks packets each, rank r on stream (SEED, 2, r)
+    subroutine run_sweep()
+       double precision :: t0, t1, tmax
+       integer :: load
+       grid3D(1)%Jste(0:grid3D(1)%nCells, 1:nbins) = 0.
+       grid3D(1)%escapedPackets(0:grid3D(1)%nCells, 0:nbins, 0:nAngleBins) = 0.
+       load = int(npk / numtasks)
+       stream = 2
+       p = taskid
+       call mcref_set_stream(seed, stream, p)
+       call mpi_barrier(MPI_COMM_WORLD, ierr)
+       t0 = MPI_Wtime()
+       call energyPacketDriver(1, load, grid3D(1:nGrids))
+       t1 = MPI_Wtime() - t0
+       call mpi_allreduce(t1, tmax, 1, MPI_DOUBLE_PRECISION, MPI_MAX, MPI_COMM_WORLD, ierr)
+       if (taskid == 0) then
+          open(unit=77, file=trim(outdir)//'/sweep.txt', status='replace')
+          write(77, '(A,I6,A,I12,A,ES14.6,A,ES14.6)') 'ranks', numtasks, ' packets', load*numtasks, &
+               & ' seconds', tmax, ' packets_per_s', dble(load)*numtasks/tmax
+          close(77)
+       end if
+    end subroutine run_sweep
+
     ! iteration_mod.f90:700-722 scaling, then 744-909 (balance, one rank)
     subroutine run_balance()
        integer :: nc
@@ -659,6 +711,40 @@ contains
        call dump_r4('em_totalLines', grid3D(1)%totalLines, nc+1)
        deallocate(fbm, fbg, fbs, sHI, sHeI, sHeII)
     end subroutine run_emission
+
+    ! emissionDriver of four cells put at temperatures the reference's
+    ! iterateT reaches in barely lit cells (its 1 K floor, and 5, 19.554 and
+    ! 60 K; each cell keeps its balanced Ne and ionDen): the raw recPDF rows
+    ! and totalLines, NaN entries included; the balanced state is restored
+    subroutine run_cold_emission()
+       integer, parameter :: ncold = 4
+       real, parameter :: tcold(ncold) = (/ 1., 5., 19.554296, 60. /)
+       integer :: cells(ncold), ic, ix, iy, iz
+       real :: tsave(ncold), rows(ncold, nbins), tls(ncold)
+       do ic = 1, ncold
+          cells(ic) = grid3D(1)%nCells - 7 * (ic - 1)
+       end do
+       do ic = 1, ncold
+          tsave(ic) = grid3D(1)%Te(cells(ic))
+          grid3D(1)%Te(cells(ic)) = tcold(ic)
+          do ix = 1, grid3D(1)%nx
+             do iy = 1, grid3D(1)%ny
+                do iz = 1, grid3D(1)%nz
+                   if (grid3D(1)%active(ix,iy,iz) == cells(ic)) call emissionDriver(grid3D, ix, iy, iz, 1)
+                end do
+             end do
+          end do
+          rows(ic, :) = grid3D(1)%recPDF(cells(ic), 1:nbins)
+          tls(ic) = grid3D(1)%totalLines(cells(ic))
+       end do
+       do ic = 1, ncold
+          grid3D(1)%Te(cells(ic)) = tsave(ic)
+       end do
+       call dump_i4('cold_cells', cells, ncold)
+       call dump_r4('cold_Te', tcold, ncold)
+       call dump_r4('cold_recPDF', rows, ncold*nbins)
+       call dump_r4('cold_totalLines', tls, ncold)
+    end subroutine run_cold_emission
 
     ! the state resetGrid read back (grid_mod.f90:2967-3568)
     subroutine dump_reset()

@@ -15,7 +15,7 @@ stores inputs and reference outputs as compressed .npz files:
                         reference's raw FP32 Jste and escapedPackets after
                         N packets with the Philox stream pinned per packet.
 
-usage: python tests/golden/make_fixtures.py [walk|transport|view|all]
+usage: python tests/golden/make_fixtures.py [walk|transport|view|converge|all]
 """
 import os
 import subprocess
@@ -231,6 +231,11 @@ def make_opacity(deck, npk, seed, name, keep_cells=96):
                em_HeIc=rd("em_HeIc"), em_HIrec=rd("em_HIrec", np.float64), em_HeIIrec=rd("em_HeIIrec", np.float64),
                widFlx=rd("widFlx"), em_totalLines=rd("em_totalLines"),
                em_recPDF=pick(rd("em_recPDF")))
+    if os.path.exists(os.path.join(d, "cold_cells.bin")):
+        # emissionDriver of four cells at 1, 5, 19.554 and 60 K (raw rows,
+        # NaN entries included)
+        out.update(cold_cells=rd("cold_cells", np.int32), cold_Te=rd("cold_Te"),
+                   cold_recPDF=rd("cold_recPDF").reshape(nb, -1).T.copy(), cold_totalLines=rd("cold_totalLines"))
     if os.path.exists(os.path.join(d, "op_total.bin")):
         out.update(Tdust=rd("op_Tdust"), op_total=pick(rd("op_total")), op_sca=pick(rd("op_sca")),
                    dust_ints=rd("dust_ints", np.int32), dust_abs=rd("dust_abs"), dust_sca=rd("dust_sca"),
@@ -259,19 +264,74 @@ def make_atomic(deck, name):
     last = np.maximum(seg[:, 3], np.minimum(seg[:, 4], nb))
     sig = np.concatenate([xs[xp - 1:xp - 1 + l - lo + 1] for (_, _, _, lo, _, xp), l in zip(seg, last)])
     out = os.path.join(HERE, "atomic_%s.npz" % name)
-    # the module data of the re-emission tables on this mesh
-    ei = rd("em_ints", np.int32)
-    em = dict(em_ints=ei, em_lyP=rd("em_lyP", np.int32), em_tk=rd("em_tk"), em_gHI=rd("em_gHI"), em_gHeI=rd("em_gHeI"), em_gHeII=rd("em_gHeII"),
-              em_eHI=rd("em_eHI", np.int32), em_eHeI=rd("em_eHeI", np.int32), em_eHeII=rd("em_eHeII", np.int32),
-              em_gff=rd("em_gff"), em_gffHeII=rd("em_gffHeII"), em_sHI=rd("em_sHI"), em_sHeI=rd("em_sHeI"),
-              em_sHeII=rd("em_sHeII"), em_fbm=rd("em_fbm", np.int32)[:4 * int(ei[9])], em_fbg=rd("em_fbg"),
-              em_fbs=rd("em_fbs"), em_ydat=rd("em_ydat"), em_Aydat=rd("em_Aydat"), em_lyData=rd("em_lyData"),
-              em_lyNu=rd("em_lyNu"), em_HeIc=rd("em_HeIc"), em_HIrec=rd("em_HIrec", np.float64),
-              em_HeIIrec=rd("em_HeIIrec", np.float64), widFlx=rd("widFlx"))
     np.savez_compressed(out, **b.to_npz_dict(), Te0=rd("Te")[1:2], ion0=ion0,
                         op_seg=np.stack([seg[:, 0], seg[:, 1], seg[:, 3], last], axis=1).astype(np.int32),
-                        op_sigma=sig, op_brems1=xs[bp - 1:bp], **em)
+                        op_sigma=sig, op_brems1=xs[bp - 1:bp], **_emission_data(rd))
     print("wrote", out, "nbins", nb, "channels", b.nch, "CEL ions", b.ncel)
+
+
+def _emission_data(rd):
+    """The module data of the re-emission tables on this mesh (harness dump)."""
+    ei = rd("em_ints", np.int32)
+    return dict(em_ints=ei, em_lyP=rd("em_lyP", np.int32), em_tk=rd("em_tk"), em_gHI=rd("em_gHI"),
+                em_gHeI=rd("em_gHeI"), em_gHeII=rd("em_gHeII"), em_eHI=rd("em_eHI", np.int32),
+                em_eHeI=rd("em_eHeI", np.int32), em_eHeII=rd("em_eHeII", np.int32), em_gff=rd("em_gff"),
+                em_gffHeII=rd("em_gffHeII"), em_sHI=rd("em_sHI"), em_sHeI=rd("em_sHeI"), em_sHeII=rd("em_sHeII"),
+                em_fbm=rd("em_fbm", np.int32)[:4 * int(ei[9])], em_fbg=rd("em_fbg"), em_fbs=rd("em_fbs"),
+                em_ydat=rd("em_ydat"), em_Aydat=rd("em_Aydat"), em_lyData=rd("em_lyData"), em_lyNu=rd("em_lyNu"),
+                em_HeIc=rd("em_HeIc"), em_HIrec=rd("em_HIrec", np.float64),
+                em_HeIIrec=rd("em_HeIIrec", np.float64), widFlx=rd("widFlx"))
+
+
+def make_converge(deck, name, seeds, overrides, ranks=8):
+    """The reference's MCIterationDriver on `deck` (with `overrides`, e.g.
+    maxIterateMC) to its own stopping point, once per seed under mpiexec with
+    the random stream of rank r pinned to (seed, 0, r): the final Te, Ne,
+    ionDen, lgConverged and lgBlack of every run, as the reference's
+    resetGrid reads them back from the grid files writeGrid left.  With them
+    the grid, source, initial state and atomic data the device loop starts
+    from (one harness pass of the same deck)."""
+    sys.path.insert(0, ROOT)
+    from mocassin_amd.atomic import AtomicBundle
+    d = run_harness(deck, "opacity", [5, 2000], tag="conv_atom_" + name, overrides=overrides)
+    prob, (nx, ny, nz, nc, nb) = read_problem(d)
+    b = AtomicBundle.from_harness_dump(d, "/root/reference/data/radrec.dat")
+    rd = lambda n, dt=np.float32: np.fromfile(os.path.join(d, n + ".bin"), dt)
+    seg = rd("op_seg", np.int32).reshape(-1, 6)
+    xs = rd("op_xsec")
+    bp = int(rd("op_ints", np.int32)[0])
+    last = np.maximum(seg[:, 3], np.minimum(seg[:, 4], nb))
+    sig = np.concatenate([xs[xp - 1:xp - 1 + l - lo + 1] for (_, _, _, lo, _, xp), l in zip(seg, last)])
+    runs = {k: [] for k in ("Te", "Ne", "ionDen", "conv", "black")}
+    for seed in seeds:
+        wd = os.path.join(WORK, "conv_%s_%d" % (name, seed))
+        os.makedirs(os.path.join(wd, "out"), exist_ok=True)
+        os.makedirs(os.path.join(wd, "output"), exist_ok=True)
+        decks.write_deck(wd, deck, overrides)
+        with open(os.path.join(wd, "log_iterate.txt"), "w") as log:
+            # the reference stops in outputGas on its missing e1bx.d data
+            # after writeGrid has written the converged state
+            subprocess.run(["/opt/conda/bin/mpiexec", "-n", str(ranks), HARNESS, "iterate", "out", str(seed)],
+                           cwd=wd, stdout=log, stderr=subprocess.STDOUT, timeout=3600)
+        with open(os.path.join(wd, "log_resetgrid.txt"), "w") as log:
+            subprocess.run([HARNESS, "resetgrid", "out"], cwd=wd, stdout=log, stderr=subprocess.STDOUT, check=True)
+        rg = lambda n, dt=np.float32: np.fromfile(os.path.join(wd, "out", n + ".bin"), dt)
+        runs["Te"].append(rg("rg_Te"))
+        runs["Ne"].append(rg("rg_Ne"))
+        runs["ionDen"].append(rg("rg_ionDen"))
+        runs["conv"].append(rg("rg_conv", np.int32))
+        runs["black"].append(rg("rg_black", np.int32))
+        summ = open(os.path.join(wd, "output", "summary.out")).read().count("energy packets used")
+        print("converged run", name, "seed", seed, "iterations", summ, flush=True)
+    out = os.path.join(HERE, "converge_%s.npz" % name)
+    prob = {k: v for k, v in prob.items() if k != "widFlx"}       # _emission_data holds it
+    np.savez_compressed(out, **prob, **b.to_npz_dict(), Hden=rd("Hden"), Te0=rd("Te"), Ne0=rd("Ne"),
+                        ionDen0=rd("ionDen"), seeds=np.asarray(seeds, np.int64), ranks=np.int32(ranks),
+                        iterations=np.int32(summ),
+                        op_seg=np.stack([seg[:, 0], seg[:, 1], seg[:, 3], last], axis=1).astype(np.int32),
+                        op_sigma=sig, op_brems1=xs[bp - 1:bp], **_emission_data(rd),
+                        **{"ref_" + k: np.stack(v) for k, v in runs.items()})
+    print("wrote", out, "cells", nc, "bins", nb, "seeds", len(seeds))
 
 
 if __name__ == "__main__":
@@ -301,6 +361,12 @@ if __name__ == "__main__":
                        overrides={"inclination": "2 0.6 0.9 2.2 5.5"})
         make_transport("c1mini", 8000, 4444, "c1mini_view", opacity_scale=300.0,
                        overrides={"inclination": "2 0.4 -1. 1.3 -1."})
+    if what in ("converge", "all"):
+        # iterated to the reference's stopping point: 20 iterations of 1e7
+        # packets on the 19^3 H+He octant, 12 of 2e6 on the 9^3 full-abundance
+        # one (per-cell Monte Carlo noise of Te near 0.2 % per run)
+        make_converge("c1", "c1", [101, 202, 303, 404], {"maxIterateMC": "20 95.", "nPhotons": "10000000"})
+        make_converge("c2mini", "c2mini", [101, 202, 303, 404], {"maxIterateMC": "12 95.", "nPhotons": "2000000"})
     if what in ("transport", "all"):
         make_transport("c1mini", 20000, 12345, "c1mini")
         make_transport("c1mini", 8000, 999, "c1mini_thick", opacity_scale=300.0)
