@@ -70,7 +70,7 @@ typedef struct {
     int32_t nStars;             /* nStars */
     const float *starPos;       /* starPosition(1:nStars) as (3,nStars) [cm] */
     const int32_t *starIdx;     /* starIndeces(1:nStars,1:3) as (3,nStars), 1-based */
-    const float *deltaE;        /* deltaE(1:nStars) [1e36 erg/s] */
+    const float *deltaE;        /* deltaE(1:nStars) [1e36 erg/s], finite and > 0 */
     uint64_t seed;              /* key of the counter-based random streams */
     int32_t device;             /* HIP device ordinal */
 } mc_desc;
@@ -474,13 +474,14 @@ int mc_emission_setup(mc_ctx *ctx, const mc_emission_tables *tables);
  * the host holds them (REAL); NULL state (and Tdust) take the device state of
  * the last mc_update_cells / mc_balance_run, rounded to FP32.  recPDF
  * (0:nCells, 1:nbins) and totalLines (0:nCells), nullable: host copies of
- * the active cells' rows (row 0 is left as it is).  A cell whose continuum
- * lies entirely below the reference's 1e-15 cut gets the row 0 .. 0 1 (the
- * reference's is 0/0; with totalLines = 1 it is never sampled).  A cell at
- * iterateT's 1 K floor, where the reference's fits give 0/0 and inf*0, emits
- * nothing: totalLines = 1 (its absorbed packets leave as line photons) and
- * the row 0 .. 0 1.
- * MC_ETABLE names a cell whose state or emission is not finite.  kernel_ms (nullable): device
+ * the active cells' rows (row 0 is left as it is).  Where the reference
+ * forms NaN (a cell at a few tens of K, where its REAL sums overflow and its
+ * fits give inf*0, or a continuum entirely below its 1e-15 cut), the tables
+ * hold what its sampling does with them: getNu2's scan stops at a NaN entry
+ * as at an entry of 1, so the row is 1 from its first NaN on, and a NaN
+ * totalLines, which random <= totalLines never passes, is 0.  MC_ETABLE
+ * names a cell whose state is not finite (or Te <= 0); the tables of the
+ * previous build or upload are then kept.  kernel_ms (nullable): device
  * time.  Replaces: emissionDriver per cell in iterateMC (iteration_mod.f90:
  * 229-240; emission_mod.f90:32-173, 253-903, 905-1311). */
 int mc_emission_build(mc_ctx *ctx, const float *Hden, const float *Te, const float *Ne, const float *ionDen,
@@ -583,6 +584,16 @@ int mc_mlp_train(mc_ctx *ctx, const float *X, const float *T, const float *M, in
 /* Set / read the transport launch shape (see mc_tuning). */
 int mc_set_tuning(mc_ctx *ctx, const mc_tuning *t);
 int mc_get_tuning(const mc_ctx *ctx, mc_tuning *t);
+
+/* Throughput probe of the device's atomic adds (no counterpart in the
+ * reference; the ceiling the transport's J update is measured against):
+ * `requests` FP64 (fp64 != 0) or FP32 atomic adds to uniformly random words
+ * of a fresh array of `words` elements (1 .. 2^32-1), issued by wavefronts
+ * with `lanes` (1..64) active lanes, 8 workgroups of 256 per CU.  One
+ * untimed launch, then one timed: kernel_ms.  Allocates and frees its own
+ * array; needs no context. */
+int mc_probe_atomics(int32_t device, int64_t words, int64_t requests, int32_t lanes, int32_t fp64,
+                     double *kernel_ms);
 
 /* Message of the last failure on this context (never NULL). */
 const char *mc_ctx_last_error(const mc_ctx *ctx);

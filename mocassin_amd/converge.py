@@ -207,3 +207,139 @@ if __name__ == "__main__":
     import json
     import sys
     print(json.dumps(te_convergence(log=lambda m: print(m, file=sys.stderr, flush=True))))
+
+
+def loop_convergence(packets=5e7, ref_iters=12, max_iters=12, train_steps=6000, batch=4096, lr=2e-3,
+                     te_weight=5.0, log=print):
+    """Wall time to converged Te over the iteration, plain versus warm-started,
+    on the clumpy 100^3 H+He workload with the opacity and re-emission tables
+    built on the device from each iteration's state (mocassin_amd/iterate.py).
+
+    Te_ref: the plain iteration run for ref_iters iterations, Te averaged
+    over its last three (the Monte Carlo noise of one sweep shrinks by
+    sqrt(3)); the well-sampled cells are those with at least 1e3 segments in
+    the last sweep.  Each timed loop starts from the reference's cold start
+    and stops at the first iteration after which at least 95 % of the
+    well-sampled cells have |Te - Te_ref| <= 1 % Te_ref; its wall time sums
+    its iterations' host wall times (tables, sweep, balance, state copies).
+
+      plain  each balance (the reference's schedule, iterateT) starts from
+             the state the previous iteration left
+      warm   each balance starts every reached cell from the network's
+             prediction for that sweep's J, falling back to the previous
+             state where that solve fails; the network is trained on the
+             device on (J features, balanced state) pairs of every
+             iteration of another seeded variant of the grid
+    """
+    import os
+    from .atomic import emission_tables
+    from .iterate import iterate, jscale_of
+    here = os.path.dirname(os.path.abspath(__file__))
+    atom = dict(np.load(os.path.join(here, "..", "tests", "golden", "atomic_c1.npz")))
+    bundle = AtomicBundle.from_npz(atom)
+    npk = int(packets)
+
+    def open_variant(seed):
+        t = time.perf_counter()
+        p = synth.workload("c4", n_packets=npk, seed=seed, nu=bundle.nuArray)
+        T = Transport(p["xAxis"], p["yAxis"], p["zAxis"], p["active"], p["nCells"], p["nbins"], p["nuArray"],
+                      p["R_out"], p["geoCorr"], p["ionEdge1"], [p["starPos"]], [p["starIdx"]], [p["deltaE"]],
+                      symmetric=p["symmetric"], seed=1000 + seed, device=0)
+        T.balance_setup(bundle)
+        T.opacity_setup(atom["op_seg"], atom["op_sigma"], float(atom["op_brems1"][0]))
+        T.emission_setup(**emission_tables(atom))
+        Te0, Ne0, ion0, Hd = cold_state(bundle, atom["Te0"][0], atom["ion0"], p["Hden"])
+        log("c4 seed %d: %d cells, built in %.1f s" % (seed, p["nCells"], time.perf_counter() - t))
+        return T, p, (Hd, Te0, Ne0, ion0)
+
+    out = {"workload": "clumpy 100^3 full grid, H+He, 400 bins, %.0e packets per sweep, tables of each "
+                       "iteration's state built on the device" % npk}
+    # ---- training pairs from every iteration of variant 1
+    Ttr, ptr, cold_tr = open_variant(1)
+    js_tr = jscale_of(ptr["symmetric"])
+    Xs, Ts, Ms = [], [], []
+
+    def collect(k, state, flags):
+        X, hit = Ttr.mlp_features(state[0], jscale=js_tr)
+        cells = np.nonzero(hit[1:] & ((flags[1:] & (4 | 8)) == 0))[0] + 1
+        Tg, M = MP.targets(bundle, state[1], state[2], cells)
+        M[:, 0] *= te_weight
+        Xs.append(X[cells - 1])
+        Ts.append(Tg)
+        Ms.append(M)
+        return False
+    try:
+        _, _, rec_tr, _ = iterate(Ttr, cold_tr, npk, ptr["specCDF"], ptr["symmetric"], max_iter=ref_iters // 2,
+                                  min_convergence=101.0, packet_offset=1 << 40, stop=collect)
+        X, Tg, M = np.concatenate(Xs), np.concatenate(Ts), np.concatenate(Ms)
+        if X.shape[0] > 1_000_000:          # a seeded subsample bounds the host copy
+            keep = np.sort(np.random.default_rng(7).choice(X.shape[0], 1_000_000, replace=False))
+            X, Tg, M = X[keep], Tg[keep], M[keep]
+        loss, train_ms = MP.train(Ttr, X, Tg, M, steps=train_steps, batch=batch, lr=lr)
+        params, scale = Ttr.get_mlp()
+        log("network trained on %d (cell, iteration) pairs: loss %.3g -> %.3g in %.0f ms"
+            % (X.shape[0], loss[:20].mean(), loss[-50:].mean(), train_ms))
+        out["training"] = {"pairs": int(X.shape[0]), "iterations": len(rec_tr), "steps": train_steps,
+                           "batch": batch, "device_ms": train_ms, "loss_first": float(loss[:20].mean()),
+                           "loss_last": float(loss[-50:].mean())}
+    finally:
+        Ttr.close()
+    del Xs, Ts, Ms
+
+    # ---- Te_ref on variant 0
+    T, p, cold = open_variant(0)
+    try:
+        js = jscale_of(p["symmetric"])
+        last = []
+
+        def keep_last(k, state, flags):
+            if k > ref_iters - 3:
+                last.append(np.array(state[1]))
+            return False
+        _, fl_ref, rec_ref, cross = iterate(T, cold, npk, p["specCDF"], p["symmetric"], max_iter=ref_iters,
+                                            min_convergence=101.0, packet_offset=1 << 41, stop=keep_last,
+                                            crossings=True)
+        Te_ref = np.mean(last, axis=0)
+        well = np.zeros(Te_ref.size, bool)
+        well[1:] = (cross[1:] >= 1000) & ((fl_ref[1:] & 16) == 0)
+        spread = np.abs(last[-1][well] - Te_ref[well]) / Te_ref[well]
+        out["te_ref"] = {"iterations": ref_iters, "averaged_last": len(last), "well_sampled_cells": int(well.sum()),
+                         "last_iteration_within_1pct_of_ref": float(np.mean(spread <= 0.01)),
+                         "segments_per_packet": rec_ref[-1]["segments_per_packet"],
+                         "transport_ms": rec_ref[-1]["transport_ms"]}
+        log("Te_ref from %d iterations: %d well-sampled cells, last iteration %.3f within 1%%"
+            % (ref_iters, well.sum(), np.mean(spread <= 0.01)))
+
+        def timed(name, balance):
+            hist = []
+
+            def judge(k, state, flags):
+                err = np.abs(state[1][well] - Te_ref[well]) / Te_ref[well]
+                hist.append(float(np.mean(err <= 0.01)))
+                return hist[-1] >= 0.95
+            if balance.get("warm_start"):
+                T.set_mlp(params, scale)
+            _, _, recs, _ = iterate(T, cold, npk, p["specCDF"], p["symmetric"], max_iter=max_iters,
+                                    min_convergence=101.0, packet_offset=1 << 42, balance=balance, stop=judge)
+            walls = np.cumsum([r["wall_ms"] for r in recs])
+            done = hist[-1] >= 0.95
+            res = {"converged": done, "iterations": len(recs), "wall_ms": float(walls[-1]),
+                   "frac_within_1pct_per_iteration": hist,
+                   "balance_ms_per_iteration": [r["balance_ms"] for r in recs],
+                   "transport_ms_per_iteration": [r["transport_ms"] for r in recs],
+                   "wall_ms_per_iteration": [r["wall_ms"] for r in recs],
+                   "mean_T_iterations": [r["mean_T_iterations"] for r in recs]}
+            log("%s: %s after %d iterations, %.0f ms wall (within 1%%: %s)"
+                % (name, "converged" if done else "not converged", len(recs), walls[-1],
+                   ", ".join("%.3f" % h for h in hist)))
+            return res
+        runs = {"plain": timed("plain", {}),
+                "warm": timed("warm", {"warm_start": True})}
+        out["runs"] = runs
+        pl, wa = runs["plain"], runs["warm"]
+        out["speedup_wall"] = pl["wall_ms"] / wa["wall_ms"] if pl["converged"] and wa["converged"] else None
+        out["note"] = ("wall time of the whole loop until converged (every stage: tables, sweep, balance, state "
+                       "copies); training on another variant is not included (training.device_ms)")
+    finally:
+        T.close()
+    return out

@@ -165,6 +165,15 @@ module mocassin_amd_mod
             type(c_ptr), value :: ctx
         end function mc_esc_planes
 
+        integer(c_int) function mc_probe_atomics(device, words, requests, lanes, fp64, kernel_ms) &
+                bind(C, name="mc_probe_atomics")
+            import :: c_int, c_int32_t, c_int64_t, c_double
+            integer(c_int32_t), value :: device
+            integer(c_int64_t), value :: words, requests
+            integer(c_int32_t), value :: lanes, fp64
+            real(c_double), intent(out) :: kernel_ms
+        end function mc_probe_atomics
+
         integer(c_int) function mc_walk_replay(ctx, n, pos, dir, cell, cell_seq, ds_seq, &
                                                seq_len, status, max_len) bind(C, name="mc_walk_replay")
             import :: c_int, c_int32_t, c_ptr

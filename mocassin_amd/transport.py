@@ -530,3 +530,15 @@ def scale_J(J, symmetric):
     if symmetric:
         out = out / 8.
     return out
+
+
+def probe_atomics(words, requests, lanes=64, fp64=True, device=0):
+    """Atomic adds per second of the device (mc_probe_atomics): `requests`
+    adds to random words of a `words`-element array, `lanes` active lanes
+    per wavefront.  Returns (kernel_ms, adds per second)."""
+    L = _capi.load()
+    ms = C.c_double()
+    rc = L.mc_probe_atomics(int(device), int(words), int(requests), int(lanes), 1 if fp64 else 0, C.byref(ms))
+    if rc:
+        raise _capi.McError(rc, "mc_probe_atomics")
+    return ms.value, requests / (ms.value * 1e-3)
