@@ -280,8 +280,16 @@ def main():
     cold = (Hd0[3], Hd0[0], Hd0[1], Hd0[2])
     state, flags, records, _ = iterate(T, cold, npk, spec, p["symmetric"], max_iter=a.setup_iterations,
                                        min_convergence=101.0, packet_offset=1 << 40, log=log)
-    T.opacity_build(host=False)
-    T.emission_build(host=False)
+    # the tables of the state reached, with host copies for the CPU baseline
+    # (built from the state as the host holds it: the same values)
+    host_tables = None
+    if a.no_cpu:
+        T.opacity_build(host=False)
+        T.emission_build(host=False)
+    else:
+        opac_h = T.opacity_build(*_f32_state(state), host=True)[0]
+        rec_h, tl_h, _ = T.emission_build(*_f32_state(state), host=True)
+        host_tables = (opac_h, rec_h, tl_h)
     T.upload_tables(None, spec)
     if a.autotune:
         best, trials = at.autotune(T, max(1, min(npk, 10_000_000)), repeats=2)
@@ -488,11 +496,8 @@ def main():
 
     if not a.no_cpu:
         def cpu_leg():
-            # the same tables on the host: rebuilt from the device state the
-            # setup iterations left, bit for bit the tables of the timed sweeps
-            opac = T.opacity_build(*_f32_state(state), host=True)[0]
-            rec, tl, _ = T.emission_build(*_f32_state(state), host=True)
-            return cpu_baseline(p, (opac, rec, tl))
+            # the tables of the timed sweeps, as the host holds them
+            return cpu_baseline(p, host_tables)
         leg(out, "cpu_baseline", cpu_leg)
         if "value" in out["cpu_baseline"]:
             out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]

@@ -262,7 +262,7 @@ def loop_convergence(packets=5e7, ref_iters=12, max_iters=12, train_steps=6000, 
     def collect(k, state, flags):
         X, hit = Ttr.mlp_features(state[0], jscale=js_tr)
         cells = np.nonzero(hit[1:] & ((flags[1:] & (4 | 8)) == 0))[0] + 1
-        Tg, M = MP.targets(bundle, state[1], state[2], cells)
+        Tg, M = MP.targets(bundle, state[1], state[3], cells)
         M[:, 0] *= te_weight
         Xs.append(X[cells - 1])
         Ts.append(Tg)

@@ -142,6 +142,8 @@ extern "C" int mc_ctx_create(const mc_desc *d, mc_ctx **out)
         !d->xAxis || !d->yAxis || !d->zAxis || !d->active || !d->nuArray || !d->starPos ||
         !d->starIdx || !d->deltaE)
         return fail(nullptr, MC_EINVAL, "descriptor has empty sizes or null arrays");
+    if (d->nbins > 65536)
+        return fail(nullptr, MC_EUNSUP, "nbins %d: the transport's J queue indexes at most 65536 bins", d->nbins);
     // the energy of a packet of each source (iteration_mod.f90:477-493
     // divides the luminosity by the packet count): finite and positive
     for (int s = 0; s < d->nStars; ++s)
@@ -1798,14 +1800,15 @@ extern "C" int mc_transport_run(mc_ctx *c, int32_t istar, int64_t npackets, uint
     return MC_OK;
 }
 
-extern "C" int mc_transport_batch(mc_ctx *const *ctx, int32_t n, const int32_t *istar, const int64_t *npackets,
-                                  const uint64_t *packet_offset, const mc_batch_tuning *tuning, mc_counters *cnt,
-                                  double *kernel_ms)
+// Arguments of a batched launch: the launch shape (tu), the LDS it needs and
+// its chunk count.  Failures are reported on ctx[0].
+static int batch_check(mc_ctx *const *ctx, int32_t n, const int32_t *istar, const int64_t *npackets,
+                       const mc_batch_tuning *tuning, mc_batch_tuning &tu, size_t &lds, size_t &nchunks)
 {
     if (!ctx || n < 1 || !ctx[0]) return fail(nullptr, MC_EINVAL, "no contexts");
     mc_ctx *c0 = ctx[0];
     if (!istar || !npackets) return fail(c0, MC_EINVAL, "istar and npackets are required");
-    mc_batch_tuning tu = { 16, 32, 8, 16384 };
+    tu = { 16, 32, 8, 16384 };
     if (tuning) {
         if (tuning->waves_per_cu < 1 || tuning->waves_per_cu > 32 || tuning->emit_min < 1 ||
             tuning->emit_min > 64 || tuning->walk_steps < 1 || tuning->walk_steps > 64 || tuning->chunk < 64 ||
@@ -1813,7 +1816,8 @@ extern "C" int mc_transport_batch(mc_ctx *const *ctx, int32_t n, const int32_t *
             return fail(c0, MC_EINVAL, "batch tuning out of range");
         tu = *tuning;
     }
-    size_t lds = 0, nchunks = 0;
+    lds = 0;
+    nchunks = 0;
     for (int i = 0; i < n; ++i) {
         mc_ctx *c = ctx[i];
         if (!c) return fail(c0, MC_EINVAL, "context %d is null", i);
@@ -1827,6 +1831,33 @@ extern "C" int mc_transport_batch(mc_ctx *const *ctx, int32_t n, const int32_t *
         nchunks += (size_t)((npackets[i] + tu.chunk - 1) / tu.chunk);
     }
     if (nchunks > (1u << 30)) return fail(c0, MC_EINVAL, "too many chunks");
+    return MC_OK;
+}
+
+// The run arguments of model i of a batched launch (counters set by the caller).
+static void batch_run_args(const mc_ctx *c, int32_t istar, int64_t npackets, uint64_t offset,
+                           const mc_batch_tuning &tu, mcd::RunArgs &R)
+{
+    const int s = istar - 1;
+    R.istar = istar;
+    R.npackets = npackets;
+    R.packet_offset = offset;
+    R.deltaE = c->deltaE[s];
+    R.px = c->starPos[3 * s]; R.py = c->starPos[3 * s + 1]; R.pz = c->starPos[3 * s + 2];
+    R.sx = c->starIdx[3 * s]; R.sy = c->starIdx[3 * s + 1]; R.sz = c->starIdx[3 * s + 2];
+    R.emit_min = tu.emit_min;
+    R.walk_steps = tu.walk_steps;
+}
+
+extern "C" int mc_transport_batch(mc_ctx *const *ctx, int32_t n, const int32_t *istar, const int64_t *npackets,
+                                  const uint64_t *packet_offset, const mc_batch_tuning *tuning, mc_counters *cnt,
+                                  double *kernel_ms)
+{
+    mc_batch_tuning tu;
+    size_t lds = 0, nchunks = 0;
+    int rc0 = batch_check(ctx, n, istar, npackets, tuning, tu, lds, nchunks);
+    if (rc0) return rc0;
+    mc_ctx *c0 = ctx[0];
     HIPCHK(c0, hipSetDevice(c0->device));
     // order every context's earlier work before the launch
     for (int i = 0; i < n; ++i) HIPCHK(c0, hipStreamSynchronize(ctx[i]->stream));
@@ -1843,19 +1874,11 @@ extern "C" int mc_transport_batch(mc_ctx *const *ctx, int32_t n, const int32_t *
         HIPCHK(c0, hipMemsetAsync(c->d_qphot, 0, sizeof(double), c0->stream));
         G[i] = c->G;
         mcd::RunArgs &R = RA[i];
-        const int s = istar[i] - 1;
-        R.istar = istar[i];
-        R.npackets = npackets[i];
-        R.packet_offset = packet_offset ? packet_offset[i] : 0;
+        batch_run_args(c, istar[i], npackets[i], packet_offset ? packet_offset[i] : 0, tu, R);
         R.next_packet = c->d_scratch;
         R.counts = c->d_scratch + 1;
         R.error = c->d_error;
         R.qphot = c->d_qphot;
-        R.deltaE = c->deltaE[s];
-        R.px = c->starPos[3 * s]; R.py = c->starPos[3 * s + 1]; R.pz = c->starPos[3 * s + 2];
-        R.sx = c->starIdx[3 * s]; R.sy = c->starIdx[3 * s + 1]; R.sz = c->starIdx[3 * s + 2];
-        R.emit_min = tu.emit_min;
-        R.walk_steps = tu.walk_steps;
         for (long long f = 0; f < npackets[i]; f += tu.chunk) {
             cm.push_back(i);
             cf.push_back(f);

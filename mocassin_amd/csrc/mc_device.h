@@ -294,6 +294,12 @@ __device__ __forceinline__ int walk_step(const Grid &G, const Axes &A, Walker &W
     float rx = W.rx, ry = W.ry, rz = W.rz, vx = W.vx, vy = W.vy, vz = W.vz;
 
     if (xP > nx || xP < 1 || yP > ny || yP < 1 || zP > nz || zP < 1) { err = 32; return ST_ERROR; }
+    // the cell's opacity, issued first so that its gather overlaps the wall
+    // distances (the indices were just checked); a packet on a wall is
+    // stepped into the next cell there, and then the gather is repeated
+    const int idx0 = (xP - 1) + nx * ((yP - 1) + ny * (zP - 1));
+    float kappa = 0.f;
+    if (!ZERO_OPACITY) kappa = G.opac3[(size_t)(nuP - 1) * (size_t)G.ncell3 + idx0];
 
     if (sym) {                                                   // 1248-1261
         if (rx <= AX(A.x, 1)) { if (vx < 0.f) vx = -vx; rx = AX(A.x, 1); }
@@ -329,8 +335,7 @@ __device__ __forceinline__ int walk_step(const Grid &G, const Axes &A, Walker &W
     // dense opacity: inactive cells hold row 0 (= 0), so "active > 0" of the
     // interaction test (1517) holds whenever tau can exceed passProb
     const int idx = (xP - 1) + nx * ((yP - 1) + ny * (zP - 1));
-    float kappa = 0.f;
-    if (!ZERO_OPACITY) kappa = G.opac3[(size_t)(nuP - 1) * (size_t)G.ncell3 + idx];
+    if (!ZERO_OPACITY && idx != idx0) kappa = G.opac3[(size_t)(nuP - 1) * (size_t)G.ncell3 + idx];
     const float tauCell = dS * kappa;
     const float dV = AX(A.dx, xP) * AX(A.dy, yP) * AX(A.dz, zP);  // 1442-1514
     kappa_out = kappa;

@@ -13,7 +13,10 @@
 // wavefront (or half of its live lanes) are waiting, then all of them emit
 // in one pass; in between the wavefront runs `walk_steps` walk passes.  The
 // per-axis arrays, the source CDF and its guide table live in LDS; J_nu is
-// a dense (x,y,z)-indexed FP64 array updated with hardware FP64 atomics.
+// a dense (x,y,z)-indexed FP64 array.  A walk pass does not update it
+// directly: each wavefront appends its lanes' (lattice point, bin, value)
+// to a queue in LDS and, when the queue is nearly full, flushes it with
+// FP64 atomics of all 64 lanes, off the walk's dependency chain.
 #include "mc_device.h"
 #include "mc_kernels.h"
 
@@ -43,6 +46,59 @@ __device__ __forceinline__ void stage_axes(const Grid &G, float *lds, Axes &A, i
 __host__ __device__ inline size_t lds_floats(int nx, int ny, int nz, int nbins)
 {
     return (size_t)(2 * (nx + ny + nz) + (nx + ny + nz - 3) + nbins) + (kGuide + 2) / 2 + 1;
+}
+
+// ------------------------------------------------ J update queue (per wave)
+// kJQ entries of (lattice point, bin - 1, value) per wavefront at the start
+// of dynamic LDS: xyz u32 [kJQ] | value f32 [kJQ] | bin u16 [kJQ].  n is
+// wave-uniform.
+constexpr int kJQ = 256;
+constexpr int kJQFloats = kJQ * 2 + kJQ / 2;          // 2560 bytes per wavefront
+
+struct JQ {
+    unsigned *xyz;
+    float *val;
+    unsigned short *nu;
+    int n;
+};
+
+__device__ __forceinline__ JQ jq_at(float *lds, int wave)
+{
+    float *b = lds + (size_t)wave * kJQFloats;
+    JQ Q;
+    Q.xyz = reinterpret_cast<unsigned *>(b);
+    Q.val = b + kJQ;
+    Q.nu = reinterpret_cast<unsigned short *>(b + 2 * kJQ);
+    Q.n = 0;
+    return Q;
+}
+
+// every queued update, 64 lanes per atomic instruction
+__device__ __forceinline__ void jq_flush(const Grid &G, JQ &Q, int lane)
+{
+    __builtin_amdgcn_wave_barrier();
+    for (int i = lane; i < Q.n; i += 64) {
+        const unsigned xyz = Q.xyz[i];
+        unsafeAtomicAdd(&G.J3[j3_index(G.nbins, (int)Q.nu[i], (int)xyz)], (double)Q.val[i]);
+        if (G.cross) atomicAdd(&G.cross[xyz], 1u);
+    }
+    __builtin_amdgcn_wave_barrier();
+    Q.n = 0;
+}
+
+// the J update of each lane with `has` (path length times deltaE over dV,
+// photon_mod.f90:1563-1574, 1822-1833), flushed when fewer than 64 slots remain
+__device__ __forceinline__ void jq_push(const Grid &G, JQ &Q, bool has, int xyz, int nuP, float v, int lane)
+{
+    const uint64_t m = __ballot(has);
+    if (has) {
+        const int slot = Q.n + (int)__popcll(m & ((1ull << lane) - 1ull));
+        Q.xyz[slot] = (unsigned)xyz;
+        Q.val[slot] = v;
+        Q.nu[slot] = (unsigned short)(nuP - 1);
+    }
+    Q.n = __builtin_amdgcn_readfirstlane(Q.n + (int)__popcll(m));
+    if (Q.n > kJQ - 64) jq_flush(G, Q, lane);
 }
 
 // ------------------------------------------------------------ replay kernel
@@ -143,12 +199,6 @@ __device__ __forceinline__ void esc_add(const Grid &G, int origin, int nuP, floa
     }
 }
 
-__device__ __forceinline__ void j_add(const Grid &G, int idx, int nuP, float v)
-{
-    unsafeAtomicAdd(&G.J3[j3_index(G.nbins, nuP - 1, idx)], (double)v);
-    if (G.cross) atomicAdd(&G.cross[idx], 1u);
-}
-
 // newPhotonPacket + initPhotonPacket + the start of pathSegment
 // (photon_mod.f90:768-1059, 491-684, 1176-1184) for stellar packets
 // (PH_EMIT_STELLAR) and re-emissions at an absorption point (PH_EMIT_DIFFUSE).
@@ -241,7 +291,8 @@ __device__ __forceinline__ void stage_model(const Grid &G, const RunArgs &RA, fl
 __device__ __forceinline__ void advance(Lane &L, const Grid &G, const Axes &A, const RunArgs &RA,
                                         const Model &M, const float *spec,
                                         const unsigned short *spec_guide, int emit_min, int walk_steps,
-                                        int npend, int nwalk, WaveCounts &C, double &qphot, int &err)
+                                        int npend, int nwalk, WaveCounts &C, double &qphot, int &err, JQ &Q,
+                                        int lane)
 {
     const bool pend = (L.phase == PH_EMIT_STELLAR || L.phase == PH_EMIT_DIFFUSE);
     // ---- emission pass, batched across the wavefront
@@ -264,12 +315,16 @@ __device__ __forceinline__ void advance(Lane &L, const Grid &G, const Axes &A, c
     for (int s = 0; s < walk_steps; ++s) {
         bool seg = false, escaped = false, lost = false, absorbed = false, scattered = false,
              trapped = false;
+        float jv = 0.f;
+        int jxyz = 0, jnu = 1;
         if (L.phase == PH_WALK) {
             L.steps++;
             float kappa, dV;
             int r = walk_step<false>(G, A, L.W, L.nuP, kappa, dV, err);
             if (r == ST_CONTINUE || r == ST_ESCAPED) {
-                j_add(G, L.W.segIdx, L.nuP, L.W.seg * M.dE / dV);
+                jv = L.W.seg * M.dE / dV;
+                jxyz = L.W.segIdx;
+                jnu = L.nuP;
                 seg = true;
                 if (r == ST_ESCAPED) {
                     esc_add(G, L.origin, L.nuP, M.dE, L.d0x, L.d0y, L.d0z, err);
@@ -280,7 +335,9 @@ __device__ __forceinline__ void advance(Lane &L, const Grid &G, const Axes &A, c
                 lost = true;
                 L.phase = PH_IDLE;
             } else if (r == ST_ABSORBED) {
-                j_add(G, L.W.segIdx, L.nuP, L.W.seg * M.dE / dV);
+                jv = L.W.seg * M.dE / dV;
+                jxyz = L.W.segIdx;
+                jnu = L.nuP;
                 seg = true;
                 float rx = L.W.rx / 1.e10f, ry = L.W.ry / 1.e10f, rz = L.W.rz / 1.e10f;
                 if (sqrtf(rx * rx + ry * ry + rz * rz) * 1.e10f >= G.R_out && G.R_out > 0.f) {
@@ -330,6 +387,7 @@ __device__ __forceinline__ void advance(Lane &L, const Grid &G, const Axes &A, c
                 L.phase = PH_IDLE;
             }
         }
+        jq_push(G, Q, seg, jxyz, jnu, jv, lane);
         C.seg += wcount(seg);
         C.esc += wcount(escaped);
         C.lost += wcount(lost);
@@ -368,8 +426,10 @@ __global__ void __launch_bounds__(256) transport_kernel(Grid G, RunArgs RA)
     Axes A;
     float *spec = nullptr;
     unsigned short *spec_guide = nullptr;
-    stage_model(G, RA, lds, A, spec, spec_guide);
+    const int nwaves = (int)(blockDim.x >> 6);
+    stage_model(G, RA, lds + (size_t)nwaves * kJQFloats, A, spec, spec_guide);
     const int lane_id = threadIdx.x & 63;
+    JQ Q = jq_at(lds, (int)(threadIdx.x >> 6));
     const Model M = model_of(G, RA);
 
     Lane L;
@@ -405,8 +465,10 @@ __global__ void __launch_bounds__(256) transport_kernel(Grid G, RunArgs RA)
         const int npend = (int)wcount(pend);
         const int nwalk = (int)wcount(L.phase == PH_WALK);
         if (npend + nwalk == 0) break;
-        advance(L, G, A, RA, M, spec, spec_guide, RA.emit_min, RA.walk_steps, npend, nwalk, C, qphot, err);
+        advance(L, G, A, RA, M, spec, spec_guide, RA.emit_min, RA.walk_steps, npend, nwalk, C, qphot, err, Q,
+                lane_id);
     }
+    jq_flush(G, Q, lane_id);
     flush_counts(RA, C, qphot);
 }
 
@@ -422,6 +484,8 @@ __global__ void __launch_bounds__(64) batch_kernel(BatchArgs B)
 {
     extern __shared__ float lds[];
     const int lane_id = threadIdx.x;
+    JQ Q = jq_at(lds, 0);
+    float *tab = lds + kJQFloats;
     Axes A;
     float *spec = nullptr;
     unsigned short *spec_guide = nullptr;
@@ -474,12 +538,15 @@ __global__ void __launch_bounds__(64) batch_kernel(BatchArgs B)
         if (npend + nwalk == 0) {
             if (pending >= 0) {
                 // all lanes idle: switch to the pending chunk's model
-                if (cur >= 0) flush_counts(*RAp, C, qphot);
+                if (cur >= 0) {
+                    jq_flush(*Gp, Q, lane_id);
+                    flush_counts(*RAp, C, qphot);
+                }
                 cur = __builtin_amdgcn_readfirstlane(B.chunk_model[pending]);
                 Gp = B.G + cur;
                 RAp = B.RA + cur;
                 __syncthreads();
-                stage_model(*Gp, *RAp, lds, A, spec, spec_guide);
+                stage_model(*Gp, *RAp, tab, A, spec, spec_guide);
                 M = model_of(*Gp, *RAp);
                 cnext = B.chunk_first[pending];
                 cend = cnext + B.chunk_count[pending];
@@ -489,9 +556,13 @@ __global__ void __launch_bounds__(64) batch_kernel(BatchArgs B)
             if (done) break;
             continue;
         }
-        advance(L, *Gp, A, *RAp, M, spec, spec_guide, B.emit_min, B.walk_steps, npend, nwalk, C, qphot, err);
+        advance(L, *Gp, A, *RAp, M, spec, spec_guide, B.emit_min, B.walk_steps, npend, nwalk, C, qphot, err, Q,
+                lane_id);
     }
-    if (cur >= 0) flush_counts(*RAp, C, qphot);
+    if (cur >= 0) {
+        jq_flush(*Gp, Q, lane_id);
+        flush_counts(*RAp, C, qphot);
+    }
 }
 
 // ------------------------------------------------------- table preparation
@@ -543,15 +614,16 @@ hipError_t mcd_launch_batch(const mcd::BatchArgs &B, size_t lds_bytes, int block
     return hipGetLastError();
 }
 
+// dynamic LDS of a batch wavefront: its J queue, then the model's tables
 size_t mcd_lds_bytes(int nx, int ny, int nz, int nbins)
 {
-    return sizeof(float) * mcd::lds_floats(nx, ny, nz, nbins);
+    return sizeof(float) * (mcd::kJQFloats + mcd::lds_floats(nx, ny, nz, nbins));
 }
 
 hipError_t mcd_launch_transport(const mcd::Grid &G, const mcd::RunArgs &RA, int blocks,
                                 int threads, hipStream_t s)
 {
-    size_t lds = sizeof(float) * mcd::lds_floats(G.nx, G.ny, G.nz, G.nbins);
+    size_t lds = sizeof(float) * ((size_t)(threads / 64) * mcd::kJQFloats + mcd::lds_floats(G.nx, G.ny, G.nz, G.nbins));
     hipLaunchKernelGGL(mcd::transport_kernel, dim3(blocks), dim3(threads), lds, s, G, RA);
     return hipGetLastError();
 }

@@ -248,18 +248,25 @@ FULL_GAS = {1: 1.0, 2: 0.1, 6: 10 ** -3.6, 7: 1e-4, 8: 10 ** -3.3, 10: 1e-4, 16:
 H_HE = {1: 1.0, 2: 0.1}
 
 
-def clumpy_density(n, seed, mean=300.0, sigma_ln=0.5, corr=8.0, n_clumps=200, contrast=10.0,
-                   clump_width=2.0):
+def lognormal_density(n, seed, mean=300.0, sigma_ln=0.5, corr=8.0, rng=None):
     """Lognormal nH field on an n^3 lattice [x, y, z]: Gaussian-filtered white
     noise (correlation length `corr` cells, periodic), scaled to standard
-    deviation sigma_ln in ln nH and mean `mean`, times `n_clumps` seeded
-    Gaussian clumps of peak contrast `contrast` and width `clump_width` cells."""
+    deviation sigma_ln in ln nH, exp(mu + sigma_ln g) with
+    mu = ln(mean) - sigma_ln^2 / 2 (so that the field's mean is `mean`)."""
     from scipy.ndimage import gaussian_filter
-    rng = np.random.default_rng(seed)
+    rng = np.random.default_rng(seed) if rng is None else rng
     g = gaussian_filter(rng.standard_normal((n, n, n)), corr, mode="wrap")
     g = (g - g.mean()) / g.std()
     mu = np.log(mean) - 0.5 * sigma_ln ** 2
-    dens = np.exp(mu + sigma_ln * g)
+    return np.exp(mu + sigma_ln * g)
+
+
+def clumpy_density(n, seed, mean=300.0, sigma_ln=0.5, corr=8.0, n_clumps=200, contrast=10.0,
+                   clump_width=2.0):
+    """The lognormal field of lognormal_density times `n_clumps` seeded
+    Gaussian clumps of peak contrast `contrast` and width `clump_width` cells."""
+    rng = np.random.default_rng(seed)
+    dens = lognormal_density(n, seed, mean, sigma_ln, corr, rng=rng)
     w = clump_width
     r = int(np.ceil(4 * w))
     off = np.arange(-r, r + 1)

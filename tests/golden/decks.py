@@ -131,6 +131,39 @@ DUST_DECKS = {
 }
 
 
+def density_file(n, edge, seed, mean=300.0, sigma_ln=0.5, corr=8.0):
+    """densityFile contents (set_input_mod.f90:298-310, read by setMotherGrid,
+    grid_mod.f90:956-1010): a '# nx ny nz' header, then 'x y z nH' for every
+    lattice point, x outer and z inner; the axes are the automatic full-grid
+    axes (grid_mod.f90:582-595) and nH the seeded lognormal field of
+    mocassin_amd.synth.lognormal_density."""
+    import sys
+    sys.path.insert(0, os.path.abspath(os.path.join(os.path.dirname(__file__), "..", "..")))
+    from mocassin_amd import synth
+    ax = synth.axes_full(n, edge)
+    dens = synth.lognormal_density(n, seed, mean, sigma_ln, corr)
+    rows = ["# %d %d %d" % (n, n, n)]
+    for i in range(n):
+        for j in range(n):
+            for k in range(n):
+                rows.append("%.9e %.9e %.9e %.9e" % (ax[i], ax[j], ax[k], dens[i, j, k]))
+    return "\n".join(rows) + "\n"
+
+
+# config 3 at reduced size: the 60^3 lognormal field's 8-cell correlation
+# length scaled to the 15^3 grid
+C3MINI_DENSITY = dict(n=15, edge=3.e18, seed=60, corr=2.0)
+DUST_DECKS["c3mini"] = ([
+    "nx 15", "ny 15", "nz 15",
+    "densityFile 'input/density.dat'", "Rin 0.", "Rout 3.e18",
+    "contShape blackbody", "TStellar 50000.",
+    "LPhot 4.e12", "nbins 160", "nuMin 1.e-5", "nuMax 25.",
+    "nPhotons 20000", "maxIterateMC 1 100.", "TeStart 10000.",
+    "dustFile 'input/species.dat' 'input/sizes.dat'", "MdMg constant 0.01",
+], H_HE, {"species.dat": SILICATE, "sizes.dat": None,
+          "density.dat": lambda: density_file(**C3MINI_DENSITY)})
+
+
 def write_deck(dirpath, name, overrides=None):
     files = {}
     if name in DUST_DECKS:
@@ -151,5 +184,5 @@ def write_deck(dirpath, name, overrides=None):
         f.write("\n".join(keys + ['nebComposition "input/abun.in"']) + "\n")
     for fn, text in files.items():
         with open(os.path.join(dirpath, "input", fn), "w") as f:
-            f.write(grain_sizes() if text is None else text)
+            f.write(grain_sizes() if text is None else (text() if callable(text) else text))
     return os.path.join(dirpath, "input", "input.in")

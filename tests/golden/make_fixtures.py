@@ -351,6 +351,8 @@ if __name__ == "__main__":
         make_balance("dust_pe", 20000, 1618, "dust_pe")
         make_transport("dust_mini", 8000, 2718, "dust_mini")
         make_transport("dust_mini", 4000, 3141, "dust_mini_thick", opacity_scale=30.0)
+        # config 3 at reduced size: lognormal density file, dust + gas, HG
+        make_transport("c3mini", 8000, 6060, "c3mini")
     if what in ("opacity", "all"):
         make_opacity("c2mini", 20000, 4243, "c2mini")
         make_opacity("dust_pe", 20000, 1619, "dust_pe")

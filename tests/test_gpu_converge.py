@@ -5,12 +5,21 @@ unmodified reference's MCIterationDriver run to its own stopping point on
 the same deck (tests/golden/converge_*.npz: four seeds of the reference,
 eight MPI ranks each).
 
-Both sides are Monte Carlo estimates; each side's state is the mean over its
-four seeded runs.  In cells with at least 1e3 segments walked in a sweep:
-converged Te within 1 %, ionic fractions within 2 % (every fraction of at
-least 0.01; smaller fractions are reported), and the forbidden-line
-luminosities of outputGas relative to Hbeta (lines above 1e-3 Hbeta) within
-2 %."""
+Both sides are Monte Carlo estimates.  Each side's state is the mean over its
+seeded runs (eight on the device); sd is the standard error of the
+difference of the two means, from each side's seed-to-seed spread.  In the
+cells with at least 1e3 segments walked in a sweep that both sides mark
+converged in every run:
+  * where the noise resolves the bar (sd <= 0.2 % of the value): Te within
+    1 %, every ionic fraction of at least 0.01 within 2 % (the stages an
+    element has; the reference's resetGrid leaves other stages stale);
+  * everywhere: the difference within max(bar, 5 sd) in at least 99 % of the
+    cells; and no bias where the noise resolves it (median signed difference
+    below 0.1 % over ten or more resolved cells);
+  * forbidden-line luminosities of outputGas relative to Hbeta (lines above
+    1e-3 Hbeta): within 2 % where sd <= 0.4 %, within max(2 %, 5 sd) for all.
+The reference's own seeds split in two halves differ by as much as the
+device and the reference do (printed beside each figure)."""
 import numpy as np
 import pytest
 
@@ -18,7 +27,9 @@ from conftest import golden, gpu_available, grid_from_fixture
 
 pytestmark = pytest.mark.gpu
 
-SEEDS = 4
+SEEDS = 8
+MAX_ITER = {"c1": 20, "c2mini": 12}                 # the decks' maxIterateMC
+PACKETS = {"c1": 10000000, "c2mini": 2000000}
 
 
 @pytest.fixture(autouse=True)
@@ -45,8 +56,23 @@ def _open(g):
     return T, b, state, fl["symmetric"], shp
 
 
-def _deck_packets(name):
-    return {"c1": 10000000, "c2mini": 2000000}[name]
+def _compare(r, d, mask, bar, resolve, label):
+    """r [runs_ref][cells], d [runs_dev][cells]: checks and figures."""
+    rm, dm = r.mean(axis=0), d.mean(axis=0)
+    sd = np.sqrt(r.var(axis=0, ddof=1) / r.shape[0] + d.var(axis=0, ddof=1) / d.shape[0]) / np.abs(rm)
+    rel = np.abs(dm / rm - 1)
+    half = np.abs(r[: r.shape[0] // 2].mean(axis=0) / r[r.shape[0] // 2:].mean(axis=0) - 1)
+    res = mask & (sd <= resolve)
+    within = np.mean((rel <= np.maximum(bar, 5 * sd))[mask])
+    bias = np.median((dm / rm - 1)[res]) if res.sum() >= 10 else 0.0
+    print("%s: %d cells; resolved %d, max rel diff there %.4f (bar %.2f); within max(bar, 5 sd) %.4f; "
+          "median signed %.1e; all cells max %.4f (reference split halves %.4f)"
+          % (label, mask.sum(), res.sum(), rel[res].max() if res.any() else 0.0, bar, within, bias,
+             rel[mask].max(), half[mask].max()))
+    assert not res.any() or rel[res].max() <= bar
+    assert within >= 0.99
+    assert abs(bias) <= 1e-3
+    return res.sum()
 
 
 @pytest.mark.parametrize("name", ["c1", "c2mini"])
@@ -56,57 +82,54 @@ def test_iterated_state_matches_reference(name):
     g = golden("converge_" + name)
     T, b, state, sym, shp = _open(g)
     nc = T.nCells
-    npk = _deck_packets(name)
-    iters = int(g["iterations"])
-    Te_d, ion_d, cross, hb_d, forb_d, n_done = [], [], [], [], [], []
+    Te_d, ion_d, flags_d, cross, ratio_d, n_done = [], [], [], [], [], []
+    ratio_r = []
     try:
         for s in range(SEEDS):
-            st, flags, recs, cr = iterate(T, state, npk, g["specCDF"].reshape(-1, 1), sym, max_iter=iters,
-                                          min_convergence=95.0, packet_offset=(s + 1) << 36, crossings=True)
+            st, flags, recs, cr = iterate(T, state, PACKETS[name], g["specCDF"].reshape(-1, 1), sym,
+                                          max_iter=MAX_ITER[name], min_convergence=95.0,
+                                          packet_offset=(s + 1) << 36, crossings=True)
             n_done.append(len(recs))
             Te_d.append(st[1])
             ion_d.append(st[3])
+            flags_d.append(flags)
             cross.append(cr)
             if b.ncel > 0:
                 forb, hb, _ = T.line_luminosities()          # the device state this run left
-                forb_d.append(forb)
-                hb_d.append(hb)
-        # the reference's converged states, through the same line sums
-        ref_forb, ref_hb = [], []
+                ratio_d.append(forb / hb)
         if b.ncel > 0:
+            # the reference's converged states through the same line sums
             for k in range(len(g["seeds"])):
                 f, h, _ = T.line_luminosities(g["Hden"], g["ref_Te"][k], g["ref_Ne"][k], shp(g["ref_ionDen"][k]),
                                               lgBlack=g["ref_black"][k])
-                ref_forb.append(f)
-                ref_hb.append(h)
+                ratio_r.append(f / h)
     finally:
         T.close()
-    ref_Te = g["ref_Te"].astype(np.float64).mean(axis=0)
-    ref_ion = np.mean([shp(a) for a in g["ref_ionDen"]], axis=0)
-    dev_Te = np.mean(Te_d, axis=0)
-    dev_ion = np.mean(ion_d, axis=0)
+    ref_iters = int(g["iterations"])
+    print("%s: device iterations %s, reference %d" % (name, n_done, ref_iters))
+    assert min(n_done) == max(n_done) == ref_iters
     well = np.zeros(nc + 1, bool)
-    well[1:] = np.mean(cross, axis=0)[1:] >= 1000
+    well[1:] = ((np.mean(cross, axis=0)[1:] >= 1000) & (g["ref_conv"][:, 1:].min(axis=0) == 1)
+                & (np.min([(f[1:] & 1) for f in flags_d], axis=0) == 1))
     assert well.sum() > 0.5 * nc
-    sd_ref = g["ref_Te"].astype(np.float64).std(axis=0)[well] / ref_Te[well]
-    sd_dev = np.std(Te_d, axis=0)[well] / dev_Te[well]
-    dT = np.abs(dev_Te[well] / ref_Te[well] - 1)
-    print("%s: %d iterations (reference %d), %d well-sampled cells; Te: max rel diff %.4f, median %.5f; "
-          "seed-to-seed sd reference %.4f, device %.4f (medians)"
-          % (name, max(n_done), iters, well.sum(), dT.max(), np.median(dT), np.median(sd_ref), np.median(sd_dev)))
-    assert dT.max() <= 0.01
-    big = (ref_ion[well] >= 0.01)
-    dI = np.abs(dev_ion[well][big] / ref_ion[well][big] - 1)
-    small = (ref_ion[well] > 1e-6) & ~big
-    dS = np.abs(dev_ion[well][small] / ref_ion[well][small] - 1)
-    print("%s: ionic fractions >= 0.01: max rel diff %.4f (%d values); 1e-6..0.01: median %.4f, p95 %.4f"
-          % (name, dI.max(), big.sum(), np.median(dS) if dS.size else 0, np.percentile(dS, 95) if dS.size else 0))
-    assert dI.max() <= 0.02
+    _compare(g["ref_Te"].astype(np.float64), np.array(Te_d), well, 0.01, 0.002, "%s Te" % name)
+    ref_ion = np.array([shp(a) for a in g["ref_ionDen"]])
+    dev_ion = np.array(ion_d)
+    n_res = 0
+    for z in range(1, 31):
+        if not b.elem_on[z - 1]:
+            continue
+        e = int(b.xref[z - 1]) - 1
+        for st in range(1, min(z + 1, b.nstages) + 1):
+            m = well & (ref_ion[:, :, e, st - 1].mean(axis=0) >= 0.01)
+            if m.sum() < 5:
+                continue
+            n_res += _compare(ref_ion[:, :, e, st - 1], dev_ion[:, :, e, st - 1], m, 0.02, 0.002,
+                              "%s X(Z=%d, stage %d)" % (name, z, st))
+    assert n_res > 0.5 * well.sum()
     if b.ncel > 0:
-        rf = np.mean([f / h for f, h in zip(ref_forb, ref_hb)], axis=0)
-        df = np.mean([f / h for f, h in zip(forb_d, hb_d)], axis=0)
-        lines = rf > 1e-3
-        dL = np.abs(df[lines] / rf[lines] - 1)
-        print("%s: %d forbidden lines above 1e-3 Hbeta: ratio to Hbeta max rel diff %.4f, median %.5f"
-              % (name, lines.sum(), dL.max(), np.median(dL)))
-        assert lines.sum() > 20 and dL.max() <= 0.02
+        rr, dd = np.array(ratio_r), np.array(ratio_d)
+        lines = rr.mean(axis=0) > 1e-3
+        assert lines.sum() > 20
+        _compare(rr.reshape(rr.shape[0], -1), dd.reshape(dd.shape[0], -1), lines.ravel(), 0.02, 0.004,
+                 "%s forbidden lines / Hbeta" % name)

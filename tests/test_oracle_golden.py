@@ -119,7 +119,7 @@ def test_oracle_walk_matches_reference_bit_exact(deck):
 
 # ------------------------------------------------------------ full transport
 @pytest.mark.parametrize("name", ["c1mini", "c1mini_thick", "full_mini_thick", "dust_mini",
-                                  "dust_mini_thick"])
+                                  "dust_mini_thick", "c3mini"])
 def test_oracle_transport_matches_reference_bit_exact(name):
     """libm mode + FP32 accumulation: identical bits to the reference's Jste,
     escapedPackets, Qphot and absInt under the same per-packet stream."""
@@ -132,7 +132,7 @@ def test_oracle_transport_matches_reference_bit_exact(name):
     assert np.float32(cnt["qphot"]) == g["counters"][0]
     assert cnt["n_abs"] == g["counters"][1]
     assert cnt["n_sca"] == g["counters"][2]
-    if name.startswith("dust"):
+    if name.startswith("dust") or name == "c3mini":
         assert cnt["n_sca"] > 50            # Henyey-Greenstein scatterings exercised
 
 
