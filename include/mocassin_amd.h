@@ -214,6 +214,19 @@ int mc_transport_batch(mc_ctx *const *ctx, int32_t n, const int32_t *istar, cons
                        const uint64_t *packet_offset, const mc_batch_tuning *tuning, mc_counters *c,
                        double *kernel_ms);
 
+/* mc_transport_batch without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (serving loops,
+ * code under graph capture).  Same arguments and results.  The launch's
+ * working block (the models' grids, run arguments and counters, the chunk
+ * table) lives in a fixed 8 MiB device array that each GPU the library runs
+ * on holds until the process ends; a launch that needs more (at most 256
+ * models; 16 bytes per chunk of `chunk` packets) is refused with MC_EINVAL
+ * before anything runs.  Every call on a GPU reuses that array: calls that
+ * overlap on one GPU race. */
+int mc_transport_batch_noalloc(mc_ctx *const *ctx, int32_t n, const int32_t *istar, const int64_t *npackets,
+                               const uint64_t *packet_offset, const mc_batch_tuning *tuning, mc_counters *c,
+                               double *kernel_ms);
+
 /* Copy the device estimators to the host (layouts above). */
 int mc_fetch_estimators(mc_ctx *ctx, double *J_out, double *esc_out);
 

@@ -21,6 +21,7 @@ EXPORTS = ("mc_ctx_create", "mc_upload_tables", "mc_zero_estimators", "mc_transp
            "mc_set_estimators", "mc_balance_setup", "mc_balance_run",
            "mc_mlp_set", "mc_mlp_get", "mc_mlp_features", "mc_mlp_predict", "mc_mlp_loss_grad",
            "mc_mlp_train", "mc_count_crossings", "mc_fetch_crossings", "mc_transport_batch",
+           "mc_transport_batch_noalloc",
            "mc_dust_setup", "mc_update_cells", "mc_opacity_setup", "mc_opacity_build",
            "mc_line_luminosities", "mc_emission_setup", "mc_emission_build", "mc_write_grid", "mc_read_grid",
            "mc_set_view_angles", "mc_esc_planes", "mc_probe_atomics", "mc_ctx_last_error", "mc_ctx_destroy")
@@ -158,6 +159,7 @@ def load():
     L.mc_mlp_train.argtypes = [vp, vp, vp, vp, i64, vp, i32, i32, C.c_float, vp, vp]
     L.mc_transport_batch.argtypes = [C.POINTER(vp), i32, vp, vp, vp, C.POINTER(McBatchTuning),
                                      C.POINTER(McCounters), vp]
+    L.mc_transport_batch_noalloc.argtypes = L.mc_transport_batch.argtypes
     L.mc_dust_setup.argtypes = [vp, C.POINTER(McDustTables)]
     L.mc_update_cells.argtypes = [vp, C.c_double, C.POINTER(McBalanceOpts)] + [vp] * 8
     L.mc_opacity_setup.argtypes = [vp, C.POINTER(McOpacityTables)]

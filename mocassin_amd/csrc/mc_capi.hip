@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/mocassin_amd.h"
@@ -131,6 +132,7 @@ static hipError_t dalloc(T **p, size_t n)
 }
 
 static int ctx_init(mc_ctx *c, const mc_desc *d);
+static int noalloc_scratch_init(mc_ctx *c);
 
 extern "C" int mc_ctx_create(const mc_desc *d, mc_ctx **out)
 {
@@ -165,6 +167,10 @@ static int ctx_init(mc_ctx *c, const mc_desc *d)
 {
     c->device = d->device;
     HIPCHK(c, hipSetDevice(d->device));
+    {
+        const int rs = noalloc_scratch_init(c);
+        if (rs) return rs;
+    }
     HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHK(c, hipEventCreate(&c->ev0));
     HIPCHK(c, hipEventCreate(&c->ev1));
@@ -1940,6 +1946,185 @@ extern "C" int mc_transport_batch(mc_ctx *const *ctx, int32_t n, const int32_t *
         }
         if (herr && rc == MC_OK) {
             fail(c, MC_EPACKET, "%s (code %d)", packet_error_text(herr), herr);
+            rc = fail(c0, MC_EPACKET, "context %d: %s (code %d)", i, packet_error_text(herr), herr);
+        }
+    }
+    return rc;
+}
+
+// ------------------------------------------ batched launch, no allocation
+// mc_transport_batch_noalloc: mc_transport_batch for callers that cannot
+// allocate device memory.  The launch's working block (the models' grids,
+// run arguments and counters, the chunk table and the chunk counter) lives
+// in the file-scope device array `scratch` (kNoAllocBytes on every GPU the
+// library runs on, held until the process ends) instead of a per-call
+// allocation.  Its device address is taken once per device when the first
+// context on that device is created (noalloc_scratch_init) and only that
+// address is handed to kernels and copies.  Every call on one GPU reuses the
+// array, which keeps the last call's contents: the counters and the chunk
+// counter are cleared by a kernel at the start of each call, and calls that
+// overlap on one GPU race.
+constexpr size_t kNoAllocBytes = 8u << 20;
+constexpr int kNoAllocModels = 256;
+constexpr int kNoAllocDevices = 64;
+constexpr int kCounterWords = 16;       // next_packet, counts[8], error, qphot per model
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_ptr[kNoAllocDevices];
+
+// With c's device current: record the device address of `scratch` on it.
+static int noalloc_scratch_init(mc_ctx *c)
+{
+    if (c->device < 0 || c->device >= kNoAllocDevices) return MC_OK;   // the launch refuses such a device
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[c->device]) return MC_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(c, MC_ENODEV, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[c->device] = static_cast<unsigned char *>(p);
+    return MC_OK;
+}
+
+namespace {
+
+// chunk k of a batched launch: its model (the last i with cstart[i] <= k),
+// first packet and count, from the models' chunk prefix sums
+__global__ void noalloc_chunks_kernel(int n, const long long *cstart, const long long *npk, long long chunk,
+                                      int *cm, long long *cf, int *cc, long long nchunks)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nchunks) return;
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (cstart[mid] <= k) lo = mid;
+        else hi = mid;
+    }
+    const long long f = (k - cstart[lo]) * chunk;
+    const long long left = npk[lo] - f;
+    cm[k] = lo;
+    cf[k] = f;
+    cc[k] = (int)(left < chunk ? left : chunk);
+}
+
+__global__ void noalloc_clear_kernel(unsigned long long *w, long long nwords)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nwords;
+         i += (long long)gridDim.x * blockDim.x)
+        w[i] = 0ull;
+}
+
+}  // namespace
+
+extern "C" int mc_transport_batch_noalloc(mc_ctx *const *ctx, int32_t n, const int32_t *istar,
+                                          const int64_t *npackets, const uint64_t *packet_offset,
+                                          const mc_batch_tuning *tuning, mc_counters *cnt, double *kernel_ms)
+{
+    mc_batch_tuning tu;
+    size_t lds = 0, nchunks = 0;
+    int rc0 = batch_check(ctx, n, istar, npackets, tuning, tu, lds, nchunks);
+    if (rc0) return rc0;
+    mc_ctx *c0 = ctx[0];
+    if (n > kNoAllocModels)
+        return fail(c0, MC_EINVAL, "%d models: the allocation-free launch takes at most %d", n, kNoAllocModels);
+    if (c0->device < 0 || c0->device >= kNoAllocDevices)
+        return fail(c0, MC_EUNSUP, "device %d: the allocation-free launch serves devices 0..%d", c0->device,
+                    kNoAllocDevices - 1);
+    unsigned char *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        base = g_scratch_ptr[c0->device];
+    }
+    if (!base) return fail(c0, MC_ESTATE, "no scratch address for device %d", c0->device);
+    // layout: grids | run arguments | chunk prefix | packets | counters | chunk counter | chunk table
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t oG = 0, oR = up(n * sizeof(mcd::Grid)), oS = oR + up(n * sizeof(mcd::RunArgs));
+    const size_t oP = oS + up((n + 1) * sizeof(long long)), oK = oP + up(n * sizeof(long long));
+    const size_t oN = oK + up((size_t)n * kCounterWords * sizeof(unsigned long long));
+    const size_t oM = oN + 256, oF = oM + up(nchunks * sizeof(int)), oC = oF + up(nchunks * sizeof(long long));
+    const size_t tot = oC + up(nchunks * sizeof(int));
+    if (tot > kNoAllocBytes)
+        return fail(c0, MC_EINVAL, "the launch needs %zu bytes of scratch (%zu chunks of %lld packets); the "
+                    "allocation-free launch holds %zu: use larger chunks", tot, nchunks, (long long)tu.chunk,
+                    kNoAllocBytes);
+    HIPCHK(c0, hipSetDevice(c0->device));
+    for (int i = 0; i < n; ++i) HIPCHK(c0, hipStreamSynchronize(ctx[i]->stream));
+    // host staging in thread-local arrays (no heap)
+    static thread_local mcd::Grid tG[kNoAllocModels];
+    static thread_local mcd::RunArgs tR[kNoAllocModels];
+    static thread_local long long tS[kNoAllocModels + 1], tP[kNoAllocModels];
+    static thread_local unsigned long long tK[kNoAllocModels * kCounterWords];
+    unsigned long long *dK = reinterpret_cast<unsigned long long *>(base + oK);
+    tS[0] = 0;
+    for (int i = 0; i < n; ++i) {
+        mc_ctx *c = ctx[i];
+        tG[i] = c->G;
+        mcd::RunArgs &R = tR[i];
+        batch_run_args(c, istar[i], npackets[i], packet_offset ? packet_offset[i] : 0, tu, R);
+        unsigned long long *k = dK + (size_t)i * kCounterWords;
+        R.next_packet = k;
+        R.counts = k + 1;
+        R.error = reinterpret_cast<int *>(k + 9);
+        R.qphot = reinterpret_cast<double *>(k + 10);
+        tP[i] = npackets[i];
+        tS[i + 1] = tS[i] + (npackets[i] + tu.chunk - 1) / tu.chunk;
+    }
+    hipStream_t st = c0->stream;
+    HIPCHK(c0, hipMemcpyAsync(base + oG, tG, n * sizeof(mcd::Grid), hipMemcpyHostToDevice, st));
+    HIPCHK(c0, hipMemcpyAsync(base + oR, tR, n * sizeof(mcd::RunArgs), hipMemcpyHostToDevice, st));
+    HIPCHK(c0, hipMemcpyAsync(base + oS, tS, (n + 1) * sizeof(long long), hipMemcpyHostToDevice, st));
+    HIPCHK(c0, hipMemcpyAsync(base + oP, tP, n * sizeof(long long), hipMemcpyHostToDevice, st));
+    // counters and the chunk counter start at zero (a kernel, not a memset)
+    const long long nclear = (long long)(oM - oK) / (long long)sizeof(unsigned long long);
+    hipLaunchKernelGGL(noalloc_clear_kernel, dim3(64), dim3(256), 0, st,
+                       reinterpret_cast<unsigned long long *>(base + oK), nclear);
+    HIPCHK(c0, hipGetLastError());
+    int *cm = reinterpret_cast<int *>(base + oM);
+    long long *cf = reinterpret_cast<long long *>(base + oF);
+    int *cc = reinterpret_cast<int *>(base + oC);
+    if (nchunks > 0) {
+        const unsigned blocks = (unsigned)((nchunks + 255) / 256);
+        hipLaunchKernelGGL(noalloc_chunks_kernel, dim3(blocks), dim3(256), 0, st, (int)n,
+                           reinterpret_cast<const long long *>(base + oS), reinterpret_cast<const long long *>(base + oP),
+                           (long long)tu.chunk, cm, cf, cc, (long long)nchunks);
+        HIPCHK(c0, hipGetLastError());
+    }
+    mcd::BatchArgs B;
+    B.G = reinterpret_cast<const mcd::Grid *>(base + oG);
+    B.RA = reinterpret_cast<const mcd::RunArgs *>(base + oR);
+    B.chunk_model = cm;
+    B.chunk_first = cf;
+    B.chunk_count = cc;
+    B.nchunks = (int)nchunks;
+    B.next_chunk = reinterpret_cast<unsigned *>(base + oN);
+    B.emit_min = tu.emit_min;
+    B.walk_steps = tu.walk_steps;
+    HIPCHK(c0, hipEventRecord(c0->ev0, st));
+    if (B.nchunks > 0) HIPCHK(c0, mcd_launch_batch(B, lds, tu.waves_per_cu * c0->cus, st));
+    HIPCHK(c0, hipEventRecord(c0->ev1, st));
+    HIPCHK(c0, hipMemcpyAsync(tK, dK, (size_t)n * kCounterWords * sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, st));
+    HIPCHK(c0, hipStreamSynchronize(st));
+    float ms = 0.f;
+    HIPCHK(c0, hipEventElapsedTime(&ms, c0->ev0, c0->ev1));
+    if (kernel_ms) *kernel_ms = ms;
+    int rc = MC_OK;
+    for (int i = 0; i < n; ++i) {
+        const unsigned long long *h = tK + (size_t)i * kCounterWords;
+        int herr = 0;
+        double q = 0.0;
+        std::memcpy(&herr, h + 9, sizeof herr);
+        std::memcpy(&q, h + 10, sizeof q);
+        if (cnt) {
+            mc_counters &k = cnt[i];
+            k.n_seg = (double)h[1]; k.n_abs = (double)h[2]; k.n_sca = (double)h[3];
+            k.n_esc = (double)h[4]; k.n_lost = (double)h[5]; k.n_line = (double)h[6];
+            k.n_trapped = (double)h[7]; k.n_flights = (double)h[8];
+            k.qphot = q; k.kernel_ms = ms;
+        }
+        if (herr && rc == MC_OK) {
+            fail(ctx[i], MC_EPACKET, "%s (code %d)", packet_error_text(herr), herr);
             rc = fail(c0, MC_EPACKET, "context %d: %s (code %d)", i, packet_error_text(herr), herr);
         }
     }

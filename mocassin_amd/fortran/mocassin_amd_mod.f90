@@ -307,6 +307,14 @@ module mocassin_amd_mod
             type(c_ptr), value :: istar, npackets, packet_offset, tuning, cnt, kernel_ms
         end function mc_transport_batch
 
+        integer(c_int) function mc_transport_batch_noalloc(ctx, n, istar, npackets, packet_offset, tuning, cnt, &
+                                                           kernel_ms) bind(C, name="mc_transport_batch_noalloc")
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx           ! array of n context handles
+            integer(c_int32_t), value :: n
+            type(c_ptr), value :: istar, npackets, packet_offset, tuning, cnt, kernel_ms
+        end function mc_transport_batch_noalloc
+
         integer(c_int) function mc_count_crossings(ctx, enable) bind(C, name="mc_count_crossings")
             import :: c_int, c_int32_t, c_ptr
             type(c_ptr), value :: ctx
