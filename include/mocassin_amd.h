@@ -277,6 +277,13 @@ int mc_set_view_angles(mc_ctx *ctx, const mc_view_angles *v);
  * escapedPackets array before a copy.  No counterpart in the reference. */
 int mc_esc_planes(const mc_ctx *ctx);
 
+/* nCells and nbins of the context (both 0 for a NULL context; returns
+ * MC_EINVAL then), so that a host can check the extents of the J
+ * (0:nCells, 1:nbins) and escapedPackets (0:nCells, 0:nbins, planes) arrays
+ * it passes.  No counterpart in the reference, whose arrays share the
+ * module globals' extents. */
+int mc_ctx_shape(const mc_ctx *ctx, int32_t *nCells, int32_t *nbins);
+
 /* Upload the atomic inputs of the balance (validated; see mc_atomic).
  * Requires H and He switched on, nstages <= 9, nbins <= 2048, at most 16
  * elements, 64 channels, 64 CEL model ions of <= 17 levels and <= 64

@@ -436,6 +436,13 @@ extern "C" int mc_esc_planes(const mc_ctx *c)
     return c ? c->esc_planes : 0;
 }
 
+extern "C" int mc_ctx_shape(const mc_ctx *c, int32_t *nCells, int32_t *nbins)
+{
+    if (nCells) *nCells = c ? c->G.nCells : 0;
+    if (nbins) *nbins = c ? c->G.nbins : 0;
+    return c ? MC_OK : MC_EINVAL;
+}
+
 extern "C" int mc_set_view_angles(mc_ctx *c, const mc_view_angles *v)
 {
     if (!c) return MC_EINVAL;

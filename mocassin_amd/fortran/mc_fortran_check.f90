@@ -8,6 +8,8 @@
 !   mc_fortran_check selftest         argument checks only (no GPU needed)
 !   mc_fortran_check run DIR          DIR/problem.bin -> DIR/J.bin, DIR/esc.bin,
 !                                     DIR/balance.bin
+!   mc_fortran_check run DIR short    the same, then a sweep into a J one row
+!                                     short, which the module must stop
 ! problem.bin (stream, native endian): int64 header(24) = nx ny nz nCells
 ! nbins flags nStars npackets seed have_balance nstages nElemUsed nch ncel
 ! n_cel_dbl n_cel_flt device, then the grid, the tables and the balance
@@ -29,7 +31,9 @@ program mc_fortran_check
         call selftest()
     else if (trim(mode) == 'run' .and. nargs >= 2) then
         call get_command_argument(2, dir)
-        call run(trim(dir))
+        mode = ' '
+        if (nargs >= 3) call get_command_argument(3, mode)
+        call run(trim(dir), trim(mode) == 'short')
     else
         print *, 'usage: mc_fortran_check selftest | run DIR'
         stop 2
@@ -52,8 +56,9 @@ contains
         print *, 'selftest ok: ', trim(msg)
     end subroutine selftest
 
-    subroutine run(dir)
+    subroutine run(dir, short)
         character(len=*), intent(in) :: dir
+        logical, intent(in) :: short
         integer(c_int64_t) :: h(24)
         integer :: u, nx, ny, nz, nc, nb, ns_, nEl, nch, ncel, nst, iStar
         integer(c_int64_t) :: npk, per, off
@@ -61,7 +66,7 @@ contains
         integer(c_int32_t), allocatable, target :: act(:), sIdx(:)
         real(c_float), target :: sc(5)
         real(c_float), allocatable, target :: opac(:, :), rec(:, :), tl(:), spec(:, :)
-        real(c_double), allocatable, target :: J(:, :), esc(:, :)
+        real(c_double), allocatable, target :: J(:, :), esc(:, :), Jshort(:, :)
         type(mc_desc) :: d
         type(mc_counters) :: cnt
         type(c_ptr) :: ctx
@@ -109,6 +114,13 @@ contains
                          iand(int(h(6)), int(MC_SYMMETRIC)) /= 0, dir)
         end if
         close (u)
+        if (short) then
+            allocate (Jshort(0:nc - 1, nb))
+            print *, 'shape check: a J of ', nc, ' rows'
+            call gpu_energy_packet_driver(ctx, 1, per, off, Jshort, esc, cnt)
+            print *, 'shape check FAILED: the short J was accepted'
+            stop 1
+        end if
         call mc_ctx_destroy(ctx)
     end subroutine run
 

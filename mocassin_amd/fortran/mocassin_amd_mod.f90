@@ -165,6 +165,12 @@ module mocassin_amd_mod
             type(c_ptr), value :: ctx
         end function mc_esc_planes
 
+        integer(c_int) function mc_ctx_shape(ctx, nCells, nbins) bind(C, name="mc_ctx_shape")
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), intent(out) :: nCells, nbins
+        end function mc_ctx_shape
+
         integer(c_int) function mc_probe_atomics(device, words, requests, lanes, fp64, kernel_ms) &
                 bind(C, name="mc_probe_atomics")
             import :: c_int, c_int32_t, c_int64_t, c_double
@@ -389,6 +395,23 @@ contains
         end if
     end subroutine mc_check
 
+    ! J must be (0:nCells, 1:nbins) and esc (0:nCells, 0:nbins, planes) for
+    ! the library to copy into them; anything else stops before the copy
+    subroutine check_estimator_shape(ctx, sJ, sE)
+        type(c_ptr), intent(in) :: ctx
+        integer, intent(in) :: sJ(2), sE(3)
+        integer(c_int32_t) :: nc, nb
+        call mc_check(mc_ctx_shape(ctx, nc, nb), ctx, 'mc_ctx_shape')
+        if (sJ(1) /= nc + 1 .or. sJ(2) /= nb) then
+            print *, '! energyPacketDriver: J is ', sJ, '; the context needs ', nc + 1, nb
+            stop
+        end if
+        if (sE(1) /= nc + 1 .or. sE(2) /= nb + 1) then
+            print *, '! energyPacketDriver: esc is ', sE(1:2), '; the context needs ', nc + 1, nb + 1
+            stop
+        end if
+    end subroutine check_estimator_shape
+
     ! energyPacketDriver(iStar, n, grid): transport n packets of star iStar
     ! (packet indices offset+1 .. offset+n), then copy the running Jste
     ! (0:nCells,1:nbins) and escapedPackets(0:nCells,0:nbins) totals, in
@@ -399,12 +422,13 @@ contains
         type(c_ptr), intent(in) :: ctx
         integer, intent(in) :: iStar
         integer(c_int64_t), intent(in) :: n, offset
-        real(c_double), intent(inout), target :: J(:, :), esc(:, :)
+        real(c_double), intent(inout), contiguous, target :: J(:, :), esc(:, :)
         type(mc_counters), intent(out) :: cnt
         if (mc_esc_planes(ctx) /= 1) then
             print *, '! energyPacketDriver: viewing angles are on; esc needs its (0:nAngleBins) planes'
             stop
         end if
+        call check_estimator_shape(ctx, shape(J), [size(esc, 1), size(esc, 2), 1])
         call mc_check(mc_transport_run(ctx, int(iStar, c_int32_t), n, offset, c_loc(J), &
                                        c_loc(esc), cnt), ctx, 'energyPacketDriver')
     end subroutine gpu_energy_packet_driver_plane
@@ -413,12 +437,13 @@ contains
         type(c_ptr), intent(in) :: ctx
         integer, intent(in) :: iStar
         integer(c_int64_t), intent(in) :: n, offset
-        real(c_double), intent(inout), target :: J(:, :), esc(:, :, :)
+        real(c_double), intent(inout), contiguous, target :: J(:, :), esc(:, :, :)
         type(mc_counters), intent(out) :: cnt
         if (size(esc, 3) /= mc_esc_planes(ctx)) then
             print *, '! energyPacketDriver: esc has ', size(esc, 3), ' planes; the context ', mc_esc_planes(ctx)
             stop
         end if
+        call check_estimator_shape(ctx, shape(J), shape(esc))
         call mc_check(mc_transport_run(ctx, int(iStar, c_int32_t), n, offset, c_loc(J), &
                                        c_loc(esc), cnt), ctx, 'energyPacketDriver')
     end subroutine gpu_energy_packet_driver_views

@@ -74,4 +74,7 @@ def test_calls_on_null_context_fail_cleanly():
     L = _capi.load()
     assert L.mc_transport_run(None, 1, 10, 0, None, None, None) == 1
     assert L.mc_upload_tables(None, None, None, None, None, None, None) == 1
+    nc, nb = C.c_int32(-1), C.c_int32(-1)
+    assert L.mc_ctx_shape(None, C.byref(nc), C.byref(nb)) == 1 and nc.value == nb.value == 0
+    assert L.mc_esc_planes(None) == 0
     L.mc_ctx_destroy(None)

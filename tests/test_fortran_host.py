@@ -3,7 +3,7 @@ its check program links against libmocassin_amd.so, reports C-side errors
 without a device, and on the GPU produces exactly what the Python binding
 of the same C ABI produces (transport J and escapes to the FP64 atomics'
 summation-order rounding, balance outputs on the same J bit for bit after
-narrowing to REAL)."""
+narrowing to REAL), and stops a sweep into a J of the wrong extents."""
 import os
 import subprocess
 
@@ -73,9 +73,12 @@ def test_fortran_host_matches_python_binding(tmp_path):
     assert np.array_equal(np.asarray(b.nuArray, np.float32), np.asarray(P.nuArray, np.float32))
     npk, seed = 200000, 4242
     _write_problem(str(tmp_path / "problem.bin"), P, b, Hden, Te0, Ne0, ion0, npk, seed)
-    r = subprocess.run([_exe(), "run", str(tmp_path)], capture_output=True, text=True, timeout=300)
+    # "short": after the run, a sweep into a J one row short, which the
+    # module's wrapper must stop before the library copies into it
+    r = subprocess.run([_exe(), "run", str(tmp_path), "short"], capture_output=True, text=True, timeout=300)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0 and "balance ok" in r.stdout
+    assert "energyPacketDriver: J is" in r.stdout and "FAILED" not in r.stdout
 
     nc, nb = P.nCells, P.nbins
     Jf = np.fromfile(str(tmp_path / "J.bin"), np.float64).reshape(nb, nc + 1).T
