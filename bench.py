@@ -35,6 +35,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6                 # MI355X dense FP64 vector
 SEED = 20241202
 # packets/s of the compiled reference (MOCASSIN 2.02, amdflang -O2 + MPICH,
 # `mcref_harness sweep`, 8 ranks) over the C restatement's (oracle, FP64 J,
@@ -202,6 +203,19 @@ def emission_cpu_baseline(atom, bundle, nb, nu, Te, Ne, ion, cells=200):
     dt = time.perf_counter() - t
     return {"value": n / dt, "unit": "cells/s", "cores": 1, "kind": "port",
             "sample": "%d cells of the same state, scalar C restatement" % n}
+
+
+def fp64_rate(kind, t_iterations, device_ms):
+    """FP64 rate of a balance stage: the FP64 operations per temperature
+    iteration that rocprofv3 counted for this workload's balance
+    (profiles/balance_c2_fp64.json) times this run's temperature iterations,
+    plus the integrals, over the stage's device time."""
+    prof = json.load(open(os.path.join(ROOT, "profiles", "balance_c2_fp64.json")))[kind]
+    flops = prof["fp64_flops_per_temperature_iteration"] * t_iterations + prof["integrals_kernel_fp64_flops"]
+    achieved = flops / (device_ms * 1e-3) / 1e12
+    return {"achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+            "flops": flops, "source": "profiles/balance_c2_fp64.json (FP64 lane operations issued, inactive "
+                                      "lanes included: an upper bound)"}
 
 
 def leg(out, key, fn):
@@ -404,6 +418,7 @@ def main():
                    "host_wall_ms": wall_b * 1e3, "cells_per_s": float(hit.sum()) / (ms_b * 1e-3),
                    "mean_iterations": float(r[4][1:][hit].mean()),
                    "solver": "reference schedule (iterateT), FP64, from the iterated state on the last sweep's J"}
+            res["fp64"] = fp64_rate("gas", float(r[4][1:][hit].sum()), ms_b)
             Te0, Ne0, ion0, Hdc = Hd0
             rc = T.update_cells(Hdc, Te0, Ne0, ion0, jscale=js)
             hc = (rc[3][1:] & 16) == 0
@@ -490,6 +505,7 @@ def main():
                 return {"device_ms": ms_g, "host_wall_ms": wall_g * 1e3,
                         "cells_per_s": float(hit.sum()) / (ms_g * 1e-3),
                         "mean_iterations": float(r[2][1:][hit].mean()),
+                        "fp64": fp64_rate("grains", float(r[2][1:][hit].sum()), ms_g),
                         "grains": "astronomical silicate, 10 sizes, photoelectric charge and gas-grain collisions",
                         "Tdust_K": [float(Tg.min()), float(np.median(Tg)), float(Tg.max())]}
             leg(out["balance"] if isinstance(out.get("balance"), dict) else out, "with_grains", grains_leg)
