@@ -18,6 +18,8 @@ line with
                 sample of the same workload, on every host core the process
                 may use, with the reference's measured rate relative to it
   stand_in      the same sweep on the synthetic stand-in tables
+  dust_gas      the dust and gas workload at 60^3 (mocassin_amd/dustgas.py):
+                sweeps on its device-built tables and its grain balance
   balance, te_convergence, ...  secondary legs; a leg that fails records
                 {"error": ...} under its key and the line is still printed
 
@@ -244,6 +246,7 @@ def main():
     ap.add_argument("--no-grains", action="store_true", help="skip the balance with grains")
     ap.add_argument("--no-atomics", action="store_true", help="skip the atomic-throughput probe")
     ap.add_argument("--no-stand-in", action="store_true", help="skip the stand-in tables' sweep")
+    ap.add_argument("--no-dust-gas", action="store_true", help="skip the dust and gas 60^3 workload")
     ap.add_argument("--autotune", action="store_true",
                     help="time the launch-shape candidates on this workload and store the best "
                          "in mocassin_amd/tuning_db.json (default: use the stored shape)")
@@ -518,6 +521,50 @@ def main():
         if "value" in out["cpu_baseline"]:
             out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
             out["gpu_over_reference_equivalent"] = value / out["cpu_baseline"]["reference_equivalent"]
+
+    if not a.no_dust_gas:
+        def dust_gas_leg():
+            # the dust and gas workload at its own shape: 60^3 full grid,
+            # lognormal nH, silicate grains with HG scattering, 50,000 K
+            from mocassin_amd import dustgas
+            W = dustgas.open_context(60, packets=npk)
+            T3, p3 = W["T"], W["p"]
+            try:
+                Hd3, Te3, Ne3, ion3 = W["state"]
+                Td3 = W["Tdust0"]
+                ms_o = T3.opacity_build(Hd3, Te3, Ne3, ion3, Tdust=Td3, host=False)[2]
+                ms_e = T3.emission_build(Hd3, Te3, Ne3, ion3, Tdust=Td3, host=False)[2]
+                T3.upload_tables(None, p3["specCDF"], gSca=W["gSca"])
+                cs = []
+                for k in range(3):
+                    T3.zero_estimators()
+                    cs.append(T3.energy_packet_driver(1, npk, k * npk))
+                km = sum(c["kernel_ms"] for c in cs[1:]) / 2
+                bb = sum(algorithmic_bytes(c, p3["nbins"]) for c in cs[1:]) / 2
+                c = cs[-1]
+                rg = T3.update_grains(Td3, jscale=jscale_of(False), Hden=Hd3, Te=Te3, Ne=Ne3, ionDen=ion3)
+                hit = (rg[1][1:] & 16) == 0
+                res = {"workload": "60^3 full grid, lognormal nH (mean 300, sigma_ln 0.5, 8-cell correlation), "
+                                   "H+He with astronomical-silicate grains (10 sizes), Henyey-Greenstein "
+                                   "scattering, 50,000 K source, 160 bins, %d packets per sweep, tables of the "
+                                   "cold start built on the device" % npk,
+                       "cells": int(p3["nCells"]), "kernel_ms": km, "packets_per_s": npk / (km * 1e-3),
+                       "segments_per_packet": c["n_seg"] / npk, "scatterings_per_packet": c["n_sca"] / npk,
+                       "absorptions_per_packet": c["n_abs"] / npk, "J_atomics_per_s": c["n_seg"] / (km * 1e-3),
+                       "hbm_frac": bb / (km * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                       "opacity_ms": ms_o, "emission_ms": ms_e,
+                       "grain_balance": {"device_ms": rg[3], "reached": int(hit.sum()),
+                                         "cells_per_s": float(hit.sum()) / (rg[3] * 1e-3),
+                                         "mean_iterations": float(rg[2][1:][hit].mean()),
+                                         "solver": "reference schedule from the cold start, grain charge, "
+                                                   "photoelectric heating and gas-grain collisions"}}
+                log("dust and gas 60^3: %.3e packets/s (%.1f ms, %.1f seg, %.2f scatterings per packet); "
+                    "grain balance %.1f ms" % (res["packets_per_s"], km, res["segments_per_packet"],
+                                               res["scatterings_per_packet"], rg[3]))
+                return res
+            finally:
+                T3.close()
+        leg(out, "dust_gas", dust_gas_leg)
 
     if not a.no_stand_in:
         def stand_in_leg():

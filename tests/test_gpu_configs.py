@@ -10,14 +10,21 @@ at reduced size, on the device:
           1 % in every cell both converge;
   batch   octant models with edges 20..80 cells (mocassin_amd.nebulae's
           specification, packets scaled down) in one batched launch, equal to
-          their separate launches packet for packet.
+          their separate launches packet for packet;
+  dust    the dust and gas workload (mocassin_amd.dustgas: lognormal nH,
+          astronomical-silicate grains, Henyey-Greenstein scattering, 50,000 K
+          source) at 21^3: the opacity, scattering and re-emission tables built
+          on the device from the cold start feed a sweep that equals the sweep
+          on the same tables through host memory, and the balance with grain
+          charge, photoelectric heating and gas-grain collisions on that J
+          equals the FP64 restatement cell by cell.
 """
 import os
 
 import numpy as np
 import pytest
 
-from conftest import ROOT, gpu_available
+from conftest import ROOT, golden, gpu_available
 
 pytestmark = pytest.mark.gpu
 
@@ -135,3 +142,56 @@ def test_batch_of_edges_20_to_80_equals_separate_runs():
     finally:
         for T in models:
             T.close()
+
+
+def test_dust_gas_lognormal_device_tables_and_grain_balance():
+    import mc_oracle as O
+    from mocassin_amd import dustgas
+    from mocassin_amd.iterate import jscale_of
+    n, npk = 21, 400000
+    print("dust and gas %d^3: device tables, sweep, grain balance" % n, flush=True)
+    W = dustgas.open_context(n, corr=8.0 * n / 60, packets=npk)
+    T, p, Td0 = W["T"], W["p"], W["Tdust0"]
+    Hd, Te0, Ne0, ion0 = W["state"]
+    js = jscale_of(False)
+    try:
+        op, sca, _ = T.opacity_build(Hd, Te0, Ne0, ion0, Tdust=Td0)
+        rec, tl, _ = T.emission_build(Hd, Te0, Ne0, ion0, Tdust=Td0)
+        runs = []
+        for host in (False, True):
+            if host:
+                T.upload_tables(op, p["specCDF"], rec, tl, scaOpac=sca, gSca=W["gSca"])
+            else:
+                T.upload_tables(None, p["specCDF"], gSca=W["gSca"])
+            T.zero_estimators()
+            c = T.energy_packet_driver(1, npk, 0)
+            runs.append((c, T.fetch()[0]))
+        Tg, fl, nI, ms, Te, Ne, ion = T.update_grains(Td0, jscale=js, Hden=Hd, Te=Te0, Ne=Ne0, ionDen=ion0)
+    finally:
+        T.close()
+    (c1, J1), (c2, J2) = runs
+    for k in ("n_seg", "n_abs", "n_sca", "n_esc", "n_line", "n_flights", "n_lost", "n_trapped"):
+        assert c1[k] == c2[k], k
+    print("dust and gas: %d cells, %.1f segments, %.2f scatterings, %.2f absorptions per packet"
+          % (p["nCells"], c1["n_seg"] / npk, c1["n_sca"] / npk, c1["n_abs"] / npk))
+    assert c1["n_sca"] > 0.1 * npk and c1["n_abs"] > 0.1 * npk and c1["n_lost"] == 0
+    assert np.allclose(J1, J2, rtol=1e-12, atol=0)
+    # the FP64 restatement of the balance with grains on the same J
+    g = dict(golden("balance_dust_pe"))
+    g["dust_ndust"] = W["Ndust"]
+    hit = np.nonzero((fl[1:] & 16) == 0)[0] + 1
+    assert hit.size > 0.5 * p["nCells"]
+    cells = hit[np.linspace(0, hit.size - 1, 400).astype(int)]
+    J = J2 * js
+    B = O.Balance(W["bundle"])
+    Teo, Neo, iono, flo, nIo, gdc, peh, pot = B.run_grains(J, Hd, Te0, Ne0, ion0, g, Td0, "f64", cells=cells)
+    To = O.dust_temps(J, Td0, g, "f64", gdc=gdc, peh=peh)
+    assert np.array_equal(fl[cells], flo[cells]) and np.array_equal(nI[cells], nIo[cells])
+    assert np.max(np.abs(Te[cells] / Teo[cells] - 1)) < 1e-9
+    assert np.max(np.abs(Ne[cells] / Neo[cells] - 1)) < 1e-9
+    on = iono[cells] > 0
+    assert np.max(np.abs(ion[cells][on] / iono[cells][on] - 1)) < 1e-8
+    assert np.max(np.abs(Tg[cells] - To[cells]) / To[cells].clip(1)) < 1e-9
+    print("dust and gas: balance of %d sampled cells equals the FP64 restatement; Te %.0f-%.0f K, "
+          "grains %.0f-%.0f K, %.1f ms" % (cells.size, Te[cells].min(), Te[cells].max(),
+                                          Tg[cells][:, 1:, 1].min(), Tg[cells][:, 1:, 1].max(), ms))
