@@ -33,7 +33,7 @@ struct mc_ctx {
     std::vector<int> starIdx;
     bool tables = false;
     int cus = 256;
-    mc_tuning tune = { 8, 256, 16, 4 };
+    mc_tuning tune = { 4, 256, 64, 32 };    // the autotuner's best on the octant, dust and clumpy grids
     // device allocations
     float *d_axis = nullptr, *d_mid = nullptr, *d_dfac = nullptr;
     int *d_active = nullptr;
@@ -835,14 +835,18 @@ static int update_cells(mc_ctx *c, double jscale, const mc_balance_opts *opts, c
     const mcg::DustArgs &da0 = c->dust_args;
     const size_t ntd = dust ? nc1 * (da0.nsMax + 1) * (da0.nSizes + 1) : 0;
     if (gas) {
-        // ionDen(0:nCells, xref, stage) -> [cell][element index][stage 0..ns]
+        // ionDen(0:nCells, xref, stage) -> [cell][element index][stage 0..ns];
+        // cells innermost, so that ionDen is read in order
         ion.assign(nion, 0.0);
-        for (size_t cell = 0; cell < nc1; ++cell)
-            for (int ei = 0; ei < nE; ++ei) {
-                const size_t xr = (size_t)c->bal.eidx_xref[ei] - 1;
-                for (int st = 1; st <= ns; ++st)
-                    ion[(cell * nE + ei) * (ns + 1) + st] = ionDen[cell + nc1 * (xr + (size_t)nE * (st - 1))];
+        for (int ei = 0; ei < nE; ++ei) {
+            const size_t xr = (size_t)c->bal.eidx_xref[ei] - 1;
+            for (int st = 1; st <= ns; ++st) {
+                const double *src = ionDen + nc1 * (xr + (size_t)nE * (st - 1));
+                double *dst = ion.data() + (size_t)ei * (ns + 1) + st;
+                const size_t stride = (size_t)nE * (ns + 1);
+                for (size_t cell = 0; cell < nc1; ++cell) dst[cell * stride] = src[cell];
             }
+        }
         HIPCHK(c, hipMemcpyAsync(c->d_Hden, Hden, nc1 * sizeof(double), hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(c->d_Te, Te, nc1 * sizeof(double), hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(c->d_Ne, Ne, nc1 * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -907,12 +911,15 @@ static int update_cells(mc_ctx *c, double jscale, const mc_balance_opts *opts, c
         *kernel_ms = ms;
     }
     if (gas)
-        for (size_t cell = 0; cell < nc1; ++cell)
-            for (int ei = 0; ei < nE; ++ei) {
-                const size_t xr = (size_t)c->bal.eidx_xref[ei] - 1;
-                for (int st = 1; st <= ns; ++st)
-                    ionDen[cell + nc1 * (xr + (size_t)nE * (st - 1))] = ion[(cell * nE + ei) * (ns + 1) + st];
+        for (int ei = 0; ei < nE; ++ei) {
+            const size_t xr = (size_t)c->bal.eidx_xref[ei] - 1;
+            for (int st = 1; st <= ns; ++st) {
+                double *dst = ionDen + nc1 * (xr + (size_t)nE * (st - 1));
+                const double *src = ion.data() + (size_t)ei * (ns + 1) + st;
+                const size_t stride = (size_t)nE * (ns + 1);
+                for (size_t cell = 0; cell < nc1; ++cell) dst[cell] = src[cell * stride];
             }
+        }
     return MC_OK;
 }
 

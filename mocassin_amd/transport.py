@@ -194,14 +194,16 @@ class Transport:
         solver None: the reference's schedule; "secant": secant iteration to
         |heat-cool| <= tol heat.  warm_start: start reached cells from the
         network's prediction (set_mlp first), falling back to the given state.
-        Returns (Te, Ne, ionDen, flags, nIter, kernel_ms); flags bit0
+        Returns (Te, Ne, ionDen, flags, nIter, kernel_ms), ionDen in
+        column-major order (the layout the host and the library keep); flags bit0
         converged, bit1 black, bit2 no Te balance, bit3 no ion balance, bit4
         not hit, bit5 restarted from the given state."""
         nc1 = self.nCells + 1
         Te = np.array(Te, np.float64).reshape(nc1)
         Ne = np.array(Ne, np.float64).reshape(nc1)
         Hd = np.ascontiguousarray(Hden, np.float64).reshape(nc1)
-        ion = np.asfortranarray(np.array(ionDen, np.float64).reshape(nc1, self._nE, self._ns))
+        # the library's layout is column-major: one copy, kept in that order
+        ion = np.array(np.reshape(ionDen, (nc1, self._nE, self._ns)), np.float64, order="F")
         flags = np.zeros(nc1, np.int32)
         nIt = np.zeros(nc1, np.int32)
         ms = C.c_double()
@@ -210,7 +212,7 @@ class Transport:
                                              Hd.ctypes.data, Te.ctypes.data,
                                              Ne.ctypes.data, ion.ctypes.data, flags.ctypes.data,
                                              nIt.ctypes.data, C.byref(ms)))
-        return Te, Ne, np.ascontiguousarray(ion), flags, nIt, ms.value
+        return Te, Ne, ion, flags, nIt, ms.value
 
     # ---- grains
     def dust_setup(self, absXsec, grainWeight, grainAbun, emIntegral, nTemps, nSpeciesMax=None,
@@ -267,7 +269,7 @@ class Transport:
         if Hden is not None:
             gas = [np.ascontiguousarray(Hden, np.float64).reshape(nc1), np.array(Te, np.float64).reshape(nc1),
                    np.array(Ne, np.float64).reshape(nc1),
-                   np.asfortranarray(np.array(ionDen, np.float64).reshape(nc1, self._nE, self._ns))]
+                   np.array(np.reshape(ionDen, (nc1, self._nE, self._ns)), np.float64, order="F")]
             opts = self._opts(**balance)
         ptr = [None if g is None else g.ctypes.data for g in gas]
         self._check(self._lib.mc_update_cells(self._h, float(jscale), None if opts is None else C.byref(opts),
@@ -275,7 +277,7 @@ class Transport:
                                               C.byref(ms)))
         out = (T, flags, nIt, ms.value)
         if Hden is not None:
-            out += (gas[1], gas[2], np.ascontiguousarray(gas[3]))
+            out += (gas[1], gas[2], gas[3])
         return out
 
     # ---- opacity tables
