@@ -53,9 +53,9 @@ def autotune(transport, npackets, istar=1, grid=None, packet_offset=1 << 40, rep
 
 BATCH_GRID = {
     "waves_per_cu": (8, 16),
-    "emit_min": (32,),
-    "walk_steps": (8,),
-    "chunk": (512, 1024, 4096, 16384),
+    "emit_min": (32, 64),
+    "walk_steps": (8, 32),
+    "chunk": (1024, 4096),
 }
 
 
