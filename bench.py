@@ -523,6 +523,17 @@ def main():
             out["gpu_over_reference_equivalent"] = value / out["cpu_baseline"]["reference_equivalent"]
 
     if not a.no_dust_gas:
+        def dust_gas_atomic(T3, p3, c, km):
+            if a.no_atomics:
+                return None
+            from mocassin_amd.transport import probe_atomics
+            words = int(math.ceil(T3.shape[0] * T3.shape[1] * T3.shape[2] / 64.0) * 64 * p3["nbins"])
+            ms64, rate64 = probe_atomics(words, int(c["n_seg"]), lanes=64)
+            ach = c["n_seg"] / (km * 1e-3)
+            return {"achieved": ach, "peak": rate64, "unit": "atomics/s", "frac": ach / rate64,
+                    "probe": "%d random FP64 adds into %d words (this J3's footprint): %.2f ms"
+                             % (int(c["n_seg"]), words, ms64)}
+
         def dust_gas_leg():
             # the dust and gas workload at its own shape: 60^3 full grid,
             # lognormal nH, silicate grains with HG scattering, 50,000 K
@@ -552,6 +563,7 @@ def main():
                        "segments_per_packet": c["n_seg"] / npk, "scatterings_per_packet": c["n_sca"] / npk,
                        "absorptions_per_packet": c["n_abs"] / npk, "J_atomics_per_s": c["n_seg"] / (km * 1e-3),
                        "hbm_frac": bb / (km * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                       "atomic": dust_gas_atomic(T3, p3, c, km),
                        "opacity_ms": ms_o, "emission_ms": ms_e,
                        "grain_balance": {"device_ms": rg[3], "reached": int(hit.sum()),
                                          "cells_per_s": float(hit.sum()) / (rg[3] * 1e-3),
