@@ -99,11 +99,16 @@ __device__ __forceinline__ void cel_levels(const Atomic &A, const CelIon &ci, Ld
         S.y[1] = 1.0;
     }
     __syncthreads();
-    // lured: Gaussian elimination without pivoting, rows updated across lanes
+    // lured: Gaussian elimination without pivoting.  Each pivot step updates
+    // the (nl-i)^2 entries below and right of the pivot, one entry per lane
+    // at a time; every entry gets the reference's factor and operation, and
+    // column i (the multipliers' numerators reslv reads) is left as it is.
     for (int i = 1; i <= nl - 1; ++i) {
-        for (int k = i + 1 + lane; k <= nl; k += kCelW) {
+        const int m = nl - i;
+        for (int e = lane; e < m * m; e += kCelW) {
+            const int k = i + 1 + e / m, j = i + 1 + e % m;
             const double factor = S.x[k][i] / S.x[i][i];
-            for (int j = i + 1; j <= nl; ++j) S.x[k][j] = S.x[k][j] - S.x[i][j] * factor;
+            S.x[k][j] = S.x[k][j] - S.x[i][j] * factor;
         }
         __syncthreads();
     }

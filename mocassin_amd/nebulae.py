@@ -52,7 +52,7 @@ def build(specs, nu, log=print):
     return models
 
 
-def _balances(models, specs, here, n_train, log):
+def _balances(models, specs, here, n_train, log, steps=6000, decay=(1.0,), te_weight=5.0):
     """Per-model balance of the J each model's last sweep left, from the
     reference's cold start: the reference's schedule for every model; then,
     on the models the network was not trained on, the secant solve at the
@@ -85,12 +85,12 @@ def _balances(models, specs, here, n_train, log):
         X, hit = models[i].mlp_features(colds[i][0], jscale=js)
         cells = np.nonzero(hit[1:] & ((r[3][1:] & (4 | 8)) == 0))[0] + 1
         Tg, M = MP.targets(bundle, r[0], r[2], cells)
-        M[:, 0] *= 5.0
+        M[:, 0] *= te_weight
         Xs.append(X[cells - 1])
         Ts.append(Tg)
         Ms.append(M)
     X, Tg, M = np.concatenate(Xs), np.concatenate(Ts), np.concatenate(Ms)
-    loss, train_ms = MP.train(models[train[0]], X, Tg, M, steps=6000, batch=4096, lr=2e-3)
+    loss, train_ms = MP.train(models[train[0]], X, Tg, M, steps=steps, batch=4096, lr=2e-3, decay=decay)
     params, scale = models[train[0]].get_mlp()
     sec = dict(solver="secant", tol=1e-2, max_iter=60, ion_tol=0.01)
     runs = {"plain": {}, "warm": {}}
@@ -119,7 +119,8 @@ def _balances(models, specs, here, n_train, log):
                                   "host_wall_ms": wall_p * 1e3,
                                   "mean_T_iterations": float(np.concatenate(
                                       [r[4][1:][(r[3][1:] & 16) == 0] for r in plain]).mean())},
-           "training": {"models": train, "pairs": int(X.shape[0]), "device_ms": train_ms,
+           "training": {"models": train, "pairs": int(X.shape[0]), "device_ms": train_ms, "steps": steps,
+                        "lr_decay": list(decay), "te_weight": te_weight,
                         "loss_first": float(loss[:20].mean()), "loss_last": float(loss[-50:].mean())},
            "held_out_models": len(held),
            "prediction_Te_within_1pct": float(np.mean(acc <= 0.01)),
@@ -135,7 +136,7 @@ def _balances(models, specs, here, n_train, log):
     return out
 
 
-def measure(n=64, balance=True, train_models=16, log=print):
+def measure(n=64, balance=True, train_models=16, train_steps=6000, decay=(1.0,), te_weight=5.0, log=print):
     here = os.path.dirname(os.path.abspath(__file__))
     nu = np.load(os.path.join(here, "..", "tests", "golden", "atomic_c1.npz"))["at_nuArray"]
     specs = model_specs(n)
@@ -173,7 +174,7 @@ def measure(n=64, balance=True, train_models=16, log=print):
         log("nebulae: %d models, %.3g packets: batched %.1f ms (%.3g packets/s), separate %.1f ms (%.3g), "
             "counters identical: %s" % (n, tot, ms_b, tot / (ms_b * 1e-3), seq_ms, tot / (seq_ms * 1e-3), same))
         if balance:
-            out["balance"] = _balances(models, specs, here, train_models, log)
+            out["balance"] = _balances(models, specs, here, train_models, log, train_steps, decay, te_weight)
         return out
     finally:
         for m in models:
@@ -181,6 +182,15 @@ def measure(n=64, balance=True, train_models=16, log=print):
 
 
 if __name__ == "__main__":
+    import argparse
     import json
     import sys
-    print(json.dumps(measure(log=lambda m: print(m, file=sys.stderr, flush=True))))
+    ap = argparse.ArgumentParser(description="batched transport and per-model balances of 64 nebulae")
+    ap.add_argument("--train-models", type=int, default=16, help="models the warm-start network is trained on")
+    ap.add_argument("--train-steps", type=int, default=6000)
+    ap.add_argument("--decay", default="1", help="learning-rate factors of equal phases, e.g. 1,0.3,0.1")
+    ap.add_argument("--te-weight", type=float, default=5.0, help="weight of log10 Te in the loss")
+    a = ap.parse_args()
+    print(json.dumps(measure(train_models=a.train_models, train_steps=a.train_steps,
+                             decay=tuple(float(v) for v in a.decay.split(",")), te_weight=a.te_weight,
+                             log=lambda m: print(m, file=sys.stderr, flush=True))))
