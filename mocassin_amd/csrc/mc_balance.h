@@ -104,6 +104,9 @@ struct CellArgs {
     // pointer, with the frequency-bin count for the launch's LDS size
     const Grains *grains;
     int grains_nb;
+    // nullable: block b balances cell order[b] (a permutation of 1..ncells),
+    // else cell b + 1; results do not depend on the order
+    const int *order;
 };
 
 // Line luminosities of a state (outputGas's analytic forbidden lines and

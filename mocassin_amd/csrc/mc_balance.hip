@@ -354,7 +354,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PE ? 2 
     extern __shared__ double dyn[];           // grain variant: photon flux per bin, then GrainLds
     const Atomic &A = *Ap;
     const int lane = threadIdx.x;
-    const int cell = blockIdx.x + 1;
+    const int cell = a.order ? a.order[blockIdx.x] : (int)blockIdx.x + 1;
     if (cell > a.ncells) return;
     const int ns = A.nstages, nE = A.nE, nch = A.nch;
     double *ionOut = a.ion + (size_t)cell * nE * (ns + 1);

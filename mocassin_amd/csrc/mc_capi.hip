@@ -56,6 +56,10 @@ struct mc_ctx {
     double *d_Hden = nullptr, *d_Te = nullptr, *d_Ne = nullptr, *d_ion = nullptr;
     int *d_flags = nullptr, *d_nIter = nullptr;
     double *d_cold = nullptr;                // Te0, Ne0, ion0 of a warm-started run
+    // launch order of the balance: cells by their temperature iterations in
+    // the last balance, most first (d_order[b] = cell of block b)
+    std::vector<int32_t> last_nit, order_h;
+    int *d_order = nullptr;
     double *d_Rint = nullptr, *d_Hint = nullptr;  // [nCells+1][nch] channel integrals
     unsigned *d_cross = nullptr;             // segment counts per lattice point (when enabled)
     // warm-start network
@@ -775,6 +779,8 @@ extern "C" int mc_dust_setup(mc_ctx *c, const mc_dust_tables *d)
 
 // updateCell over every cell: the gas balance when mc_balance_setup was
 // called, then the grain temperatures when mc_dust_setup was called
+constexpr int kOrderBins = 1024;   // iteration counts above this share the first bin
+
 static int update_cells(mc_ctx *c, double jscale, const mc_balance_opts *opts, const double *Hden, double *Te,
                         double *Ne, double *ionDen, double *Tdust, int32_t *flags, int32_t *nIter,
                         double *kernel_ms)
@@ -866,6 +872,21 @@ static int update_cells(mc_ctx *c, double jscale, const mc_balance_opts *opts, c
         a.Ne0 = c->d_cold + nc1;
         a.ion0 = c->d_cold + 2 * nc1;
     }
+    if (gas && c->last_nit.size() == nc1) {
+        // slowest first: a cell that took many iterations last time starts
+        // early instead of holding the launch's tail (counting sort)
+        int cnt[kOrderBins + 1] = {0};
+        for (int cell = 1; cell <= nc; ++cell) ++cnt[std::min(std::max(c->last_nit[cell], 0), kOrderBins)];
+        int pos[kOrderBins + 1];
+        for (int k = kOrderBins, p = 0; k >= 0; --k) { pos[k] = p; p += cnt[k]; }
+        c->order_h.resize(nc);
+        for (int cell = 1; cell <= nc; ++cell)
+            c->order_h[pos[std::min(std::max(c->last_nit[cell], 0), kOrderBins)]++] = cell;
+        if (!c->d_order) HIPCHK(c, dalloc(&c->d_order, (size_t)nc));
+        HIPCHK(c, hipMemcpyAsync(c->d_order, c->order_h.data(), (size_t)nc * sizeof(int), hipMemcpyHostToDevice,
+                                 c->stream));
+        a.order = c->d_order;
+    }
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     if (gas) {
         int rc = integrals(c, a, warm ? c->d_mX : nullptr);
@@ -909,13 +930,15 @@ static int update_cells(mc_ctx *c, double jscale, const mc_balance_opts *opts, c
         HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
         *kernel_ms = ms;
     }
-    if (gas)
+    if (gas) {
         for (size_t cell = 0; cell < nc1; ++cell)
             for (int ei = 0; ei < nE; ++ei) {
                 const size_t xr = (size_t)c->bal.eidx_xref[ei] - 1;
                 for (int st = 1; st <= ns; ++st)
                     ionDen[cell + nc1 * (xr + (size_t)nE * (st - 1))] = ion[(cell * nE + ei) * (ns + 1) + st];
             }
+        c->last_nit.assign(nIter, nIter + nc1);
+    }
     return MC_OK;
 }
 
@@ -2212,7 +2235,7 @@ extern "C" void mc_ctx_destroy(mc_ctx *c)
                      c->d_op_i, c->d_op_f, c->d_op_st, c->d_op_Td, c->d_op_out, c->d_op_ent,
                      c->d_ln_part, c->d_ln_st, c->d_ln_out, c->d_ln_black, c->d_em_i, c->d_em_err,
                      c->d_em_f, c->d_em_d, c->d_em_st, c->d_em_Td, c->d_recCDF_b, c->d_recGuide_b,
-                     c->d_totalLines_b };
+                     c->d_totalLines_b, c->d_order };
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
