@@ -86,11 +86,11 @@ def open_context(n, seed=3, packets=1e8, corr=8.0, device=0, rng_seed=3003):
                      grainVn=bg["dust_vn"], MsurfAtom=bg["dust_msurf"], aWeight=bg["dust_aweight"], Ndust=Ndust,
                      scaXsec=og["dust_sca"].reshape(nS, nSz, -1), TdustSublime=og["dust_tsub"])
         T.emission_setup(**emission_tables(og))
+        ion0 = bg["ionDen0"].reshape(b.nstages, b.nElemUsed, -1)[:, :, 1].T
+        Te0, Ne0, ion, Hd = cold_state(b, float(bg["Te0"][1]), ion0, p["Hden"])
+        Td0 = np.full((nc + 1, nSz + 1, nsm + 1), float(bg["Tdust0"][(nSz + 1) * (nsm + 1)]))
     except Exception:
         T.close()
         raise
-    ion0 = bg["ionDen0"].reshape(b.nstages, b.nElemUsed, -1)[:, :, 1].T
-    Te0, Ne0, ion, Hd = cold_state(b, float(bg["Te0"][1]), ion0, p["Hden"])
-    Td0 = np.full((nc + 1, nSz + 1, nsm + 1), float(bg["Tdust0"][(nSz + 1) * (nsm + 1)]))
     return dict(T=T, p=p, state=(Hd, Te0, Ne0, ion), Tdust0=Td0, gSca=np.asarray(tg["gSca"], np.float32),
                 bundle=b, Ndust=Ndust)
