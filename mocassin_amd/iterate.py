@@ -41,7 +41,7 @@ def converged_percent(flags, active_cells):
 
 
 def iterate(T, state, packets, specCDF, symmetric, max_iter=30, min_convergence=95.0, packet_offset=0,
-            balance=None, crossings=False, stop=None, log=None):
+            balance=None, crossings=False, stop=None, log=None, Tdust=None, gSca=None):
     """Iterate transport and balance on the device from `state` = (Hden, Te,
     Ne, ionDen) (reference layouts, (nCells+1,) and (nCells+1, nElemUsed,
     nstages)).  packets: per sweep of source 1; packet_offset: the first
@@ -49,10 +49,17 @@ def iterate(T, state, packets, specCDF, symmetric, max_iter=30, min_convergence=
     balance: keyword options of Transport.update_cells (solver, tol,
     warm_start, ...).  crossings: count the segments walked in each cell in
     the last sweep.  stop(k, state, flags) -> bool ends the loop early (after
-    its balance).  Returns (state, flags, records, crossings or None); one
-    record per iteration with its device times, host wall time, counters and
-    convergence."""
-    Hden, Te, Ne, ion = state
+    its balance).  Tdust (nCells+1, nSizes+1, nSpeciesMax+1): grains (the
+    context's dust_setup, with gSca for Henyey-Greenstein scattering); the
+    tables then carry the dust terms, and each balance is updateCell with
+    grains (update_grains), Tdust included in the state passed to stop and
+    returned as state[4].  Returns (state, flags, records, crossings or
+    None); one record per iteration with its device times, host wall time,
+    counters and convergence."""
+    Hden, Te, Ne, ion = state[:4]
+    dust = Tdust is not None
+    if dust:
+        Tdust = np.array(Tdust, np.float64)
     Te = np.array(Te, np.float64)
     Ne = np.array(Ne, np.float64)
     ion = np.array(ion, np.float64)
@@ -68,12 +75,17 @@ def iterate(T, state, packets, specCDF, symmetric, max_iter=30, min_convergence=
         t0 = time.perf_counter()
         host = k == 0
         st = (Hden, Te, Ne, ion) if host else (None, None, None, None)
-        ms_o = T.opacity_build(*st, host=False)[2]
-        ms_e = T.emission_build(*st, host=False)[2]
-        T.upload_tables(None, spec)
+        td = Tdust if (dust and host) else None
+        ms_o = T.opacity_build(*st, Tdust=td, host=False)[2]
+        ms_e = T.emission_build(*st, Tdust=td, host=False)[2]
+        T.upload_tables(None, spec, gSca=gSca)
         T.zero_estimators()
         c = T.energy_packet_driver(1, packets, packet_offset + k * packets)
-        Te, Ne, ion, flags, nIt, ms_b = T.update_cells(Hden, Te, Ne, ion, jscale=js, **opts)
+        if dust:
+            Tdust, flags, nIt, ms_b, Te, Ne, ion = T.update_grains(Tdust, jscale=js, Hden=Hden, Te=Te, Ne=Ne,
+                                                                   ionDen=ion, **opts)
+        else:
+            Te, Ne, ion, flags, nIt, ms_b = T.update_cells(Hden, Te, Ne, ion, jscale=js, **opts)
         wall = time.perf_counter() - t0
         hit = (flags[1:] & 16) == 0
         rec = {"iteration": k + 1, "opacity_ms": ms_o, "emission_ms": ms_e, "transport_ms": c["kernel_ms"],
@@ -86,11 +98,12 @@ def iterate(T, state, packets, specCDF, symmetric, max_iter=30, min_convergence=
             log("iteration %d: %.1f%% converged, %.1f ms wall (transport %.1f, balance %.1f, opacity %.2f, "
                 "re-emission %.2f ms)" % (k + 1, rec["converged_pct"], rec["wall_ms"], rec["transport_ms"],
                                          ms_b, ms_o, ms_e))
-        if stop is not None and stop(k + 1, (Hden, Te, Ne, ion), flags):
+        cur = (Hden, Te, Ne, ion, Tdust) if dust else (Hden, Te, Ne, ion)
+        if stop is not None and stop(k + 1, cur, flags):
             break
         if rec["converged_pct"] >= min_convergence:
             break
     if crossings:
         cross = T.crossings()
         T.count_crossings(False)
-    return (Hden, Te, Ne, ion), flags, records, cross
+    return ((Hden, Te, Ne, ion, Tdust) if dust else (Hden, Te, Ne, ion)), flags, records, cross

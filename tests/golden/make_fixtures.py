@@ -303,6 +303,16 @@ def make_converge(deck, name, seeds, overrides, ranks=8):
     last = np.maximum(seg[:, 3], np.minimum(seg[:, 4], nb))
     sig = np.concatenate([xs[xp - 1:xp - 1 + l - lo + 1] for (_, _, _, lo, _, xp), l in zip(seg, last)])
     runs = {k: [] for k in ("Te", "Ne", "ionDen", "conv", "black")}
+    dust = {}
+    if os.path.exists(os.path.join(d, "dust_ints.bin")):
+        # grains: the tables getDustT, the grain charge and the dust add
+        # read, and the deck's starting grain temperatures
+        dust = dict(dust_ints=rd("dust_ints", np.int32), dust_abs=rd("dust_abs"), dust_sca=rd("dust_sca"),
+                    dust_weight=rd("dust_weight"), dust_abun=rd("dust_abun"), dust_tsub=rd("dust_tsub"),
+                    dust_ndust=rd("dust_ndust"), dust_emint=rd("dust_emint"), dust_pe=rd("dust_pe", np.int32),
+                    dust_vn=rd("dust_vn"), dust_msurf=rd("dust_msurf"), dust_radius=rd("dust_radius"),
+                    dust_aweight=rd("dust_aweight"), dust_ipcoll=rd("dust_ipcoll", np.int32), Tdust0=rd("Tdust0"))
+        runs["Tdust"] = []
     for seed in seeds:
         wd = os.path.join(WORK, "conv_%s_%d" % (name, seed))
         os.makedirs(os.path.join(wd, "out"), exist_ok=True)
@@ -321,6 +331,8 @@ def make_converge(deck, name, seeds, overrides, ranks=8):
         runs["ionDen"].append(rg("rg_ionDen"))
         runs["conv"].append(rg("rg_conv", np.int32))
         runs["black"].append(rg("rg_black", np.int32))
+        if dust:
+            runs["Tdust"].append(rg("rg_Tdust"))
         summ = open(os.path.join(wd, "output", "summary.out")).read().count("energy packets used")
         print("converged run", name, "seed", seed, "iterations", summ, flush=True)
     out = os.path.join(HERE, "converge_%s.npz" % name)
@@ -329,7 +341,7 @@ def make_converge(deck, name, seeds, overrides, ranks=8):
                         ionDen0=rd("ionDen"), seeds=np.asarray(seeds, np.int64), ranks=np.int32(ranks),
                         iterations=np.int32(summ),
                         op_seg=np.stack([seg[:, 0], seg[:, 1], seg[:, 3], last], axis=1).astype(np.int32),
-                        op_sigma=sig, op_brems1=xs[bp - 1:bp], **_emission_data(rd),
+                        op_sigma=sig, op_brems1=xs[bp - 1:bp], **_emission_data(rd), **dust,
                         **{"ref_" + k: np.stack(v) for k, v in runs.items()})
     print("wrote", out, "cells", nc, "bins", nb, "seeds", len(seeds))
 
@@ -369,6 +381,11 @@ if __name__ == "__main__":
         # one (per-cell Monte Carlo noise of Te near 0.2 % per run)
         make_converge("c1", "c1", [101, 202, 303, 404], {"maxIterateMC": "20 95.", "nPhotons": "10000000"})
         make_converge("c2mini", "c2mini", [101, 202, 303, 404], {"maxIterateMC": "12 95.", "nPhotons": "2000000"})
+    if what in ("converge", "converge_dust", "all"):
+        # the dust and gas deck (15^3 lognormal densityFile, silicate grains
+        # with the photoelectric effect, Henyey-Greenstein scattering)
+        make_converge("c3mini", "c3mini", [101, 202, 303, 404, 505, 606, 707, 808],
+                      {"maxIterateMC": "12 95.", "nPhotons": "4000000"})
     if what in ("transport", "all"):
         make_transport("c1mini", 20000, 12345, "c1mini")
         make_transport("c1mini", 8000, 999, "c1mini_thick", opacity_scale=300.0)

@@ -3,7 +3,7 @@ tables of the state, sweep, balance from the previous state, as iterateMC,
 iteration_mod.f90:81-1186) from the reference's cold start against the
 unmodified reference's MCIterationDriver run to its own stopping point on
 the same deck (tests/golden/converge_*.npz: four seeds of the reference,
-eight MPI ranks each).
+eight for the dust and gas deck, eight MPI ranks each).
 
 Both sides are Monte Carlo estimates.  Each side's state is the mean over its
 seeded runs (eight on the device); sd is the standard error of the
@@ -17,7 +17,10 @@ converged in every run:
     cells; and no bias where the noise resolves it (median signed difference
     below 0.1 % over ten or more resolved cells);
   * forbidden-line luminosities of outputGas relative to Hbeta (lines above
-    1e-3 Hbeta): within 2 % where sd <= 0.4 %, within max(2 %, 5 sd) for all.
+    1e-3 Hbeta): within 2 % where sd <= 0.4 %, within max(2 %, 5 sd) for all;
+  * with grains (the dust and gas deck, c3mini: lognormal densityFile,
+    silicate grains with the photoelectric effect, Henyey-Greenstein
+    scattering), the grain temperature of every size to the Te bar.
 The reference's own seeds split in two halves differ by as much as the
 device and the reference do (printed beside each figure)."""
 import numpy as np
@@ -28,8 +31,12 @@ from conftest import golden, gpu_available, grid_from_fixture
 pytestmark = pytest.mark.gpu
 
 SEEDS = 8
-MAX_ITER = {"c1": 20, "c2mini": 12}                 # the decks' maxIterateMC
-PACKETS = {"c1": 10000000, "c2mini": 2000000}
+MAX_ITER = {"c1": 20, "c2mini": 12, "c3mini": 12}                 # the decks' maxIterateMC
+PACKETS = {"c1": 10000000, "c2mini": 2000000, "c3mini": 4000000}
+# share of the cells the comparison must cover: on the dust and gas deck the
+# reference's eight runs are all converged in 47 % of the cells (in each run
+# 81-83 % converge, never the deck's 95 %, so all stop at maxIterateMC)
+WELL_MIN = {"c1": 0.5, "c2mini": 0.5, "c3mini": 0.3}
 
 
 @pytest.fixture(autouse=True)
@@ -39,6 +46,9 @@ def _need_gpu():
 
 
 def _open(g):
+    """The context of a converged-state fixture; with grains also their
+    tables (the photoelectric terms and the dust add), the deck's starting
+    Tdust and gSca."""
     from mocassin_amd.atomic import AtomicBundle, emission_tables
     from mocassin_amd.transport import Transport
     axes, active, nc, nb, fl, star_idx = grid_from_fixture(g)
@@ -46,14 +56,24 @@ def _open(g):
     b = AtomicBundle.from_npz(g)
     T = Transport(*axes, active, nc, nb, g["nuArray"], R_out=sc[0], geoCorr=sc[1:4], ionEdge1=sc[4],
                   starPos=sc[6:9], starIdx=star_idx, deltaE=sc[5], symmetric=fl["symmetric"], gas=True,
-                  seed=4242, device=0)
+                  dust=fl["dust"], seed=4242, device=0)
     T.balance_setup(b)
     T.opacity_setup(g["op_seg"], g["op_sigma"], float(g["op_brems1"][0]))
+    dust = None
+    if fl["dust"]:
+        nS, nSz, nT, nsm, nSp = (int(v) for v in g["dust_ints"][:5])
+        T.dust_setup(absXsec=g["dust_abs"].reshape(nS, nSz, -1), grainWeight=g["dust_weight"],
+                     grainAbun=g["dust_abun"], emIntegral=g["dust_emint"].reshape(nT, nSz, nSp).transpose(2, 1, 0),
+                     nTemps=nT, nSpeciesMax=nsm, photoelectric=bool(g["dust_pe"][0]), grainRadius=g["dust_radius"],
+                     grainVn=g["dust_vn"], MsurfAtom=g["dust_msurf"], aWeight=g["dust_aweight"],
+                     Ndust=g["dust_ndust"], scaXsec=g["dust_sca"].reshape(nS, nSz, -1), TdustSublime=g["dust_tsub"])
+        dust = dict(Tdust=g["Tdust0"].reshape(nc + 1, nSz + 1, nsm + 1).astype(np.float64), gSca=g["gSca"],
+                    shape=(nc + 1, nSz + 1, nsm + 1))
     T.emission_setup(**emission_tables(g))
     shp = lambda a: a.reshape(b.nstages, b.nElemUsed, nc + 1).transpose(2, 1, 0).astype(np.float64)
     state = (g["Hden"].astype(np.float64), g["Te0"].astype(np.float64), g["Ne0"].astype(np.float64),
              shp(g["ionDen0"]))
-    return T, b, state, fl["symmetric"], shp
+    return T, b, state, fl["symmetric"], shp, dust
 
 
 def _compare(r, d, mask, bar, resolve, label):
@@ -69,29 +89,37 @@ def _compare(r, d, mask, bar, resolve, label):
           "median signed %.1e; all cells max %.4f (reference split halves %.4f)"
           % (label, mask.sum(), res.sum(), rel[res].max() if res.any() else 0.0, bar, within, bias,
              rel[mask].max(), half[mask].max()))
+    if res.any():
+        w = np.nonzero(res)[0][np.argmax(rel[res])]
+        print("  worst resolved cell %d: reference runs %s, device runs %s, sd %.2e"
+              % (w, np.array2string(r[:, w], precision=5), np.array2string(d[:, w], precision=5), sd[w]))
     assert not res.any() or rel[res].max() <= bar
     assert within >= 0.99
     assert abs(bias) <= 1e-3
     return res.sum()
 
 
-@pytest.mark.parametrize("name", ["c1", "c2mini"])
+@pytest.mark.parametrize("name", ["c1", "c2mini", "c3mini"])
 def test_iterated_state_matches_reference(name):
     from mocassin_amd.iterate import iterate
     print("converged state %s" % name, flush=True)
     g = golden("converge_" + name)
-    T, b, state, sym, shp = _open(g)
+    T, b, state, sym, shp, dust = _open(g)
     nc = T.nCells
-    Te_d, ion_d, flags_d, cross, ratio_d, n_done = [], [], [], [], [], []
+    Te_d, ion_d, flags_d, cross, ratio_d, n_done, Td_d = [], [], [], [], [], [], []
     ratio_r = []
     try:
         for s in range(SEEDS):
             st, flags, recs, cr = iterate(T, state, PACKETS[name], g["specCDF"].reshape(-1, 1), sym,
                                           max_iter=MAX_ITER[name], min_convergence=95.0,
-                                          packet_offset=(s + 1) << 36, crossings=True)
+                                          packet_offset=(s + 1) << 36, crossings=True,
+                                          Tdust=None if dust is None else dust["Tdust"],
+                                          gSca=None if dust is None else dust["gSca"])
             n_done.append(len(recs))
             Te_d.append(st[1])
             ion_d.append(st[3])
+            if dust is not None:
+                Td_d.append(st[4])
             flags_d.append(flags)
             cross.append(cr)
             if b.ncel > 0:
@@ -111,7 +139,7 @@ def test_iterated_state_matches_reference(name):
     well = np.zeros(nc + 1, bool)
     well[1:] = ((np.mean(cross, axis=0)[1:] >= 1000) & (g["ref_conv"][:, 1:].min(axis=0) == 1)
                 & (np.min([(f[1:] & 1) for f in flags_d], axis=0) == 1))
-    assert well.sum() > 0.5 * nc
+    assert well.sum() > WELL_MIN[name] * nc
     _compare(g["ref_Te"].astype(np.float64), np.array(Te_d), well, 0.01, 0.002, "%s Te" % name)
     ref_ion = np.array([shp(a) for a in g["ref_ionDen"]])
     dev_ion = np.array(ion_d)
@@ -127,6 +155,13 @@ def test_iterated_state_matches_reference(name):
             n_res += _compare(ref_ion[:, :, e, st - 1], dev_ion[:, :, e, st - 1], m, 0.02, 0.002,
                               "%s X(Z=%d, stage %d)" % (name, z, st))
     assert n_res > 0.5 * well.sum()
+    if dust is not None:
+        # grain temperatures of every size (species 1), to the Te bar
+        ref_td = np.array([a.reshape(dust["shape"]) for a in g["ref_Tdust"]])
+        dev_td = np.array(Td_d)
+        for a in range(1, dust["shape"][1]):
+            _compare(ref_td[:, :, a, 1].astype(np.float64), dev_td[:, :, a, 1], well, 0.01, 0.002,
+                     "%s Tdust(size %d)" % (name, a))
     if b.ncel > 0:
         rr, dd = np.array(ratio_r), np.array(ratio_d)
         lines = rr.mean(axis=0) > 1e-3
