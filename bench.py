@@ -541,10 +541,15 @@ def main():
             W = dustgas.open_context(60, packets=npk)
             T3, p3 = W["T"], W["p"]
             try:
-                Hd3, Te3, Ne3, ion3 = W["state"]
-                Td3 = W["Tdust0"]
-                ms_o = T3.opacity_build(Hd3, Te3, Ne3, ion3, Tdust=Td3, host=False)[2]
-                ms_e = T3.emission_build(Hd3, Te3, Ne3, ion3, Tdust=Td3, host=False)[2]
+                # four iterations from the cold start (tables with the grain
+                # terms, sweep, balance with grains), then the sweeps are
+                # timed on the tables of the state they reached
+                st3, _, recs3, _ = iterate(T3, W["state"], npk, p3["specCDF"], False, max_iter=4,
+                                           min_convergence=101.0, packet_offset=1 << 43, Tdust=W["Tdust0"],
+                                           gSca=W["gSca"])
+                Hd3, Te3, Ne3, ion3, Td3 = st3
+                ms_o = T3.opacity_build(host=False)[2]
+                ms_e = T3.emission_build(host=False)[2]
                 T3.upload_tables(None, p3["specCDF"], gSca=W["gSca"])
                 cs = []
                 for k in range(3):
@@ -557,8 +562,9 @@ def main():
                 hit = (rg[1][1:] & 16) == 0
                 res = {"workload": "60^3 full grid, lognormal nH (mean 300, sigma_ln 0.5, 8-cell correlation), "
                                    "H+He with astronomical-silicate grains (10 sizes), Henyey-Greenstein "
-                                   "scattering, 50,000 K source, 160 bins, %d packets per sweep, tables of the "
-                                   "cold start built on the device" % npk,
+                                   "scattering, 50,000 K source, 160 bins, %d packets per sweep, tables the device "
+                                   "builds from the state four iterations from the cold start reached" % npk,
+                       "setup_iterations": recs3,
                        "cells": int(p3["nCells"]), "kernel_ms": km, "packets_per_s": npk / (km * 1e-3),
                        "segments_per_packet": c["n_seg"] / npk, "scatterings_per_packet": c["n_sca"] / npk,
                        "absorptions_per_packet": c["n_abs"] / npk, "J_atomics_per_s": c["n_seg"] / (km * 1e-3),
@@ -568,8 +574,8 @@ def main():
                        "grain_balance": {"device_ms": rg[3], "reached": int(hit.sum()),
                                          "cells_per_s": float(hit.sum()) / (rg[3] * 1e-3),
                                          "mean_iterations": float(rg[2][1:][hit].mean()),
-                                         "solver": "reference schedule from the cold start, grain charge, "
-                                                   "photoelectric heating and gas-grain collisions"}}
+                                         "solver": "reference schedule from the iterated state, grain "
+                                                   "charge, photoelectric heating and gas-grain collisions"}}
                 log("dust and gas 60^3: %.3e packets/s (%.1f ms, %.1f seg, %.2f scatterings per packet); "
                     "grain balance %.1f ms" % (res["packets_per_s"], km, res["segments_per_packet"],
                                                res["scatterings_per_packet"], rg[3]))
