@@ -76,3 +76,36 @@ def test_batch_argument_errors():
     finally:
         for T in models:
             T.close()
+
+
+def test_batch_with_dust_and_gas_model_equals_separate_runs():
+    """A dust and gas model (full grid, silicate grains, Henyey-Greenstein
+    scattering, its tables built on the device) between two gas octants in
+    one launch: every model equals its own launch."""
+    from mocassin_amd import dustgas, transport_batch
+    print("batch with a dust and gas model", flush=True)
+    gas = _models([SPECS[0], SPECS[2]])
+    W = dustgas.open_context(15, corr=2.0, packets=40000)
+    models = [gas[0], W["T"], gas[1]]
+    try:
+        Hd, Te, Ne, ion = W["state"]
+        W["T"].opacity_build(Hd, Te, Ne, ion, Tdust=W["Tdust0"], host=False)
+        W["T"].emission_build(Hd, Te, Ne, ion, Tdust=W["Tdust0"], host=False)
+        W["T"].upload_tables(None, W["p"]["specCDF"], gSca=W["gSca"])
+        npk = [SPECS[0][1], 40000, SPECS[2][1]]
+        for T in models:
+            T.zero_estimators()
+        cb, ms = transport_batch(models, [1] * 3, npk, None, None)
+        got = [T.fetch() for T in models]
+        for i, T in enumerate(models):
+            T.zero_estimators()
+            c = T.energy_packet_driver(1, npk[i], 0)
+            J, E = T.fetch()
+            for k in ("n_seg", "n_abs", "n_sca", "n_esc", "n_line", "n_flights", "n_lost", "n_trapped"):
+                assert cb[i][k] == c[k], (i, k, cb[i][k], c[k])
+            assert np.allclose(got[i][0], J, rtol=1e-12, atol=0)
+            assert np.allclose(got[i][1], E, rtol=1e-12, atol=0)
+        assert cb[1]["n_sca"] > 0.1 * npk[1]
+    finally:
+        for T in models:
+            T.close()
