@@ -13,13 +13,17 @@ packets of the source; J_nu and the escape SED stay in HBM.  Prints one JSON
 line with
   roofline      of the transport kernel (HIP-event timed on its own stream):
                 algorithmic HBM bytes and, under "atomic", the J atomics per
-                second against the probe of random-address FP64 atomics
+                second against the probe of random-address FP64 atomics;
+                "hbm_copy" is a 2 GiB device copy (the HBM rate this box
+                reaches)
   cpu_baseline  the C restatement of the same method (oracle/) on a bounded
                 sample of the same workload, on every host core the process
                 may use, with the reference's measured rate relative to it
   stand_in      the same sweep on the synthetic stand-in tables
   dust_gas      the dust and gas workload at 60^3 (mocassin_amd/dustgas.py):
                 sweeps on its device-built tables and its grain balance
+  config1       the 19^3 H+He octant: one sweep of 1e6 packets and one
+                balance
   balance, te_convergence, ...  secondary legs; a leg that fails records
                 {"error": ...} under its key and the line is still printed
 
@@ -402,6 +406,28 @@ def main():
     if not a.no_atomics:
         leg(out["roofline"], "atomic", atomic_leg)
 
+        def copy_leg():
+            # a STREAM-like copy of 2 GiB on the device: the HBM rate this
+            # box reaches, beside the 8 TB/s the peak is quoted at
+            n = (1 << 31) // 4
+            src = torch.ones(n, dtype=torch.float32, device="cuda")
+            dst = torch.empty_like(src)
+            dst.copy_(src)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            best = None
+            for _ in range(3):
+                e0.record()
+                dst.copy_(src)
+                e1.record()
+                torch.cuda.synchronize()
+                ms = e0.elapsed_time(e1)
+                best = ms if best is None else min(best, ms)
+            del src, dst
+            gbs = 2.0 * (1 << 31) / (best * 1e-3) / 1e9
+            return {"bytes_moved": 2 * (1 << 31), "ms": best, "GB_per_s": gbs, "of_peak": gbs / HBM_PEAK_GBS,
+                    "note": "device-to-device copy of 2 GiB (read + write), best of 3"}
+        leg(out["roofline"], "hbm_copy", copy_leg)
+
     # ---- secondary legs on the iterated state
     J_last = None
     if not a.no_balance:
@@ -583,6 +609,53 @@ def main():
             finally:
                 T3.close()
         leg(out, "dust_gas", dust_gas_leg)
+
+    if not a.no_balance:
+        def config1_leg():
+            # the 19^3 H+He octant (400 bins, 1e6 packets): tables of the
+            # reference's cold start built on the device, one sweep, one
+            # balance.  Its FP64 J (22 MB) lies in the Infinity Cache.
+            at1 = dict(np.load(os.path.join(ROOT, "tests", "golden", "atomic_c1.npz")))
+            b1 = AtomicBundle.from_npz(at1)
+            n1 = 1000000
+            p1 = synth.workload("c1", n_packets=n1, nu=b1.nuArray)
+            T1 = Transport(p1["xAxis"], p1["yAxis"], p1["zAxis"], p1["active"], p1["nCells"], p1["nbins"],
+                           p1["nuArray"], p1["R_out"], p1["geoCorr"], p1["ionEdge1"], [p1["starPos"]],
+                           [p1["starIdx"]], [p1["deltaE"]], symmetric=p1["symmetric"], seed=SEED + 1, device=0)
+            try:
+                T1.balance_setup(b1)
+                T1.opacity_setup(at1["op_seg"], at1["op_sigma"], float(at1["op_brems1"][0]))
+                T1.emission_setup(**emission_tables(at1))
+                Te1, Ne1, ion1, Hd1 = cold_state(b1, at1["Te0"][0], at1["ion0"], p1["Hden"])
+                T1.opacity_build(Hd1, Te1, Ne1, ion1, host=False)
+                T1.emission_build(Hd1, Te1, Ne1, ion1, host=False)
+                T1.upload_tables(None, p1["specCDF"])
+                cs = []
+                for k in range(4):
+                    T1.zero_estimators()
+                    cs.append(T1.energy_packet_driver(1, n1, k * n1))
+                km = sum(c["kernel_ms"] for c in cs[1:]) / 3
+                c = cs[-1]
+                r = T1.update_cells(Hd1, Te1, Ne1, ion1, jscale=jscale_of(p1["symmetric"]))
+                hit = (r[3][1:] & 16) == 0
+                res = {"workload": "19^3 H+He octant, 400 bins, %d packets, tables of the cold start built on the "
+                                   "device, one sweep and one balance" % n1,
+                       "cells": int(p1["nCells"]), "kernel_ms": km, "packets_per_s": n1 / (km * 1e-3),
+                       "segments_per_packet": c["n_seg"] / n1, "J_atomics_per_s": c["n_seg"] / (km * 1e-3),
+                       "balance": {"device_ms": r[5], "reached": int(hit.sum()),
+                                   "mean_iterations": float(r[4][1:][hit].mean())}}
+                if not a.no_atomics:
+                    from mocassin_amd.transport import probe_atomics
+                    words = int(math.ceil(T1.shape[0] * T1.shape[1] * T1.shape[2] / 64.0) * 64 * p1["nbins"])
+                    _, rate = probe_atomics(words, int(c["n_seg"]), lanes=64)
+                    res["atomic"] = {"achieved": res["J_atomics_per_s"], "peak": rate, "frac": res["J_atomics_per_s"] / rate,
+                                     "probe": "random FP64 adds into this J3's %d words (Infinity Cache resident)" % words}
+                log("19^3 octant: %.3e packets/s (%.2f ms per 1e6 packets, %.1f seg/packet); balance %.2f ms"
+                    % (res["packets_per_s"], km, res["segments_per_packet"], r[5]))
+                return res
+            finally:
+                T1.close()
+        leg(out, "config1", config1_leg)
 
     if not a.no_stand_in:
         def stand_in_leg():
