@@ -32,6 +32,19 @@ struct CelLds {
     double alev[kLev + 1];
 };
 
+// One dynamic LDS block per wavefront.  Region A serves the forbidden-line
+// solves first (CelLds and the terms of one model ion) and then the per-bin
+// partial sums s1, s2, s3 (and the grains' sD); the log-gamma rows, the
+// recombination-line terms and a reduction slot follow it.  Sharing region
+// A keeps the block at about 12 KB on an 800-bin mesh, so that more
+// wavefronts fit on a CU.
+__host__ __device__ inline size_t region_a_bytes(int nb, bool dust)
+{
+    const size_t cel = sizeof(CelLds) + sizeof(float) * kLev * kLev;
+    const size_t sums = (size_t)(dust ? 4 : 3) * nb * sizeof(float);
+    return ((cel > sums ? cel : sums) + 15) / 16 * 16;
+}
+
 // hydro2phot (emission_mod.f90:553-663), REAL arithmetic
 __device__ double hydro2phot(float nu, int Z, float Te, float Ne)
 {
@@ -113,19 +126,21 @@ __device__ int locate_tk(const float *xa, int n, float x)
 
 __global__ void __launch_bounds__(64) emission_kernel(const mcb::Atomic *__restrict__ Ap, EmArgs a)
 {
-    __shared__ CelLds S;
-    __shared__ float loc[3][kMaxLim];
-    __shared__ float rterm[kMaxRec];
-    __shared__ float fterm[kLev * kLev];
-    __shared__ float red[4];
-    extern __shared__ float dyn[];
+    extern __shared__ __align__(16) unsigned char dyn[];
     const mcb::Atomic &A = *Ap;
     const int cell = blockIdx.x + 1;
     const int lane = threadIdx.x;
     const int nb = a.nb;
     const float *nu = a.nu;
-    float *s1 = dyn, *s2 = dyn + nb, *s3 = dyn + 2 * nb, *sD = dyn + 3 * nb;
     const bool dust = a.dabs != nullptr;
+    // region A: the forbidden-line solves, then the per-bin sums
+    CelLds &S = *reinterpret_cast<CelLds *>(dyn);
+    float *fterm = reinterpret_cast<float *>(dyn + sizeof(CelLds));
+    float *s1 = reinterpret_cast<float *>(dyn), *s2 = s1 + nb, *s3 = s1 + 2 * nb, *sD = s1 + 3 * nb;
+    unsigned char *rest = dyn + region_a_bytes(nb, dust);
+    float (*loc)[kMaxLim] = reinterpret_cast<float (*)[kMaxLim]>(rest);
+    float *rterm = reinterpret_cast<float *>(rest + sizeof(float) * 3 * kMaxLim);
+    float *red = rterm + kMaxRec;
 
     // the state as the host grid stores it (REAL)
     const float Te = (float)a.Te[cell], NeIn = (float)a.Ne[cell];
@@ -141,6 +156,44 @@ __global__ void __launch_bounds__(64) emission_kernel(const mcb::Atomic *__restr
         if (lane == 0) a.err[cell] = 1;
         return;
     }
+
+    float normFor = 0.f;
+    // forbidden lines (forLines), each model ion's terms above 1e-35 summed
+    // lower level outer, upper inner, as setDiffusePDF does
+    const double kL = (double)1.9865e9f;
+    for (int k = 0; k < A.ncel; ++k) {
+        const mcb::CelIon &ci = A.cel[k];
+        const float x = XION(ci.eidx, ci.ion);
+        const float up = (ci.ion < a.ns1 - 1) ? XION(ci.eidx, ci.ion + 1) / x : 0.f;
+        if (up != up) continue;                  // equilibrium's NaN test: no emission
+        mcb::cel_levels(A, ci, S, lane, (double)Te, (double)NeUsed, (double)up);
+        const int nl = ci.nl;
+        const double *E = A.cel_e + ci.off_e;
+        const double *Aa = A.cel_a + ci.off_a;
+        double sumN = 0.0;
+        for (int i = 1; i <= nl; ++i) sumN = sumN + S.y[i];
+        const double ab = (double)(float)A.el[ci.eidx].abun;
+        for (int p = lane; p < kLev * kLev; p += W) {
+            const int lo = p / kLev + 1, hi = p % kLev + 1;
+            float v = 0.f;
+            if (lo < hi && hi <= nl) {
+                const double av = Aa[(hi - 1) + nl * (lo - 1)];
+                if (av != 0.0) {
+                    const double Eji = E[hi - 1] - E[lo - 1];
+                    double f = av * Eji * (S.y[hi] / sumN);
+                    f = f * ab * (double)x;
+                    f = f * kL;
+                    if (f > (double)1.e-35f) v = (float)f;
+                }
+            }
+            fterm[p] = v;
+        }
+        __syncthreads();
+        if (lane == 0)
+            for (int p = 0; p < kLev * kLev; ++p) normFor = normFor + fterm[p];
+        __syncthreads();
+    }
+    __syncthreads();                             // region A now holds the per-bin sums
 
     // log gamma rows at Te (fb_ff, emission_mod.f90:306-336)
     for (int s = 0; s < 3; ++s)
@@ -304,45 +357,9 @@ __global__ void __launch_bounds__(64) emission_kernel(const mcb::Atomic *__restr
         }
     }
     __syncthreads();
-    float normRec = 0.f, normFor = 0.f;
+    float normRec = 0.f;
     if (lane == 0)
         for (int t = 0; t < 70 + 315 + 34; ++t) normRec = normRec + rterm[t];
-
-    // forbidden lines (forLines), each model ion's terms above 1e-35 summed
-    // lower level outer, upper inner, as setDiffusePDF does
-    const double kL = (double)1.9865e9f;
-    for (int k = 0; k < A.ncel; ++k) {
-        const mcb::CelIon &ci = A.cel[k];
-        const float x = XION(ci.eidx, ci.ion);
-        const float up = (ci.ion < a.ns1 - 1) ? XION(ci.eidx, ci.ion + 1) / x : 0.f;
-        if (up != up) continue;                  // equilibrium's NaN test: no emission
-        mcb::cel_levels(A, ci, S, lane, (double)Te, (double)NeUsed, (double)up);
-        const int nl = ci.nl;
-        const double *E = A.cel_e + ci.off_e;
-        const double *Aa = A.cel_a + ci.off_a;
-        double sumN = 0.0;
-        for (int i = 1; i <= nl; ++i) sumN = sumN + S.y[i];
-        const double ab = (double)(float)A.el[ci.eidx].abun;
-        for (int p = lane; p < kLev * kLev; p += W) {
-            const int lo = p / kLev + 1, hi = p % kLev + 1;
-            float v = 0.f;
-            if (lo < hi && hi <= nl) {
-                const double av = Aa[(hi - 1) + nl * (lo - 1)];
-                if (av != 0.0) {
-                    const double Eji = E[hi - 1] - E[lo - 1];
-                    double f = av * Eji * (S.y[hi] / sumN);
-                    f = f * ab * (double)x;
-                    f = f * kL;
-                    if (f > (double)1.e-35f) v = (float)f;
-                }
-            }
-            fterm[p] = v;
-        }
-        __syncthreads();
-        if (lane == 0)
-            for (int p = 0; p < kLev * kLev; ++p) normFor = normFor + fterm[p];
-        __syncthreads();
-    }
 
     // lane 0: the partial sums in bin order, the He lines able to ionise H,
     // totalLines, and the cumulative recPDF (in s1)
@@ -435,7 +452,10 @@ __global__ void __launch_bounds__(64) emission_kernel(const mcb::Atomic *__restr
 
 }  // namespace mce
 
-size_t mce_lds_bytes(int nb, bool dust) { return (size_t)(dust ? 4 : 3) * nb * sizeof(float); }
+size_t mce_lds_bytes(int nb, bool dust)
+{
+    return mce::region_a_bytes(nb, dust) + sizeof(float) * (3 * mce::kMaxLim + mce::kMaxRec + 4);
+}
 
 hipError_t mce_launch(const mcb::Atomic *dA, const mce::EmArgs &a, hipStream_t s)
 {
