@@ -348,7 +348,7 @@ __device__ void grain_terms(const Atomic &A, const Grains &G, const double *pf, 
 }
 
 template <bool PE>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PE ? 2 : 4))) balance_kernel(const Atomic *__restrict__ Ap, CellArgs a)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) balance_kernel(const Atomic *__restrict__ Ap, CellArgs a)
 {
     __shared__ Lds S;
     extern __shared__ double dyn[];           // grain variant: photon flux per bin, then GrainLds
