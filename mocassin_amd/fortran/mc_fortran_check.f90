@@ -10,6 +10,9 @@
 !                                     DIR/balance.bin
 !   mc_fortran_check run DIR short    the same, then a sweep into a J one row
 !                                     short, which the module must stop
+!   mc_fortran_check grains DIR       no sweep: DIR/grains.bin gives J and the
+!                                     grain tables; the updateCell loop with
+!                                     grains -> DIR/grains_out.bin
 ! problem.bin (stream, native endian): int64 header(24) = nx ny nz nCells
 ! nbins flags nStars npackets seed have_balance nstages nElemUsed nch ncel
 ! n_cel_dbl n_cel_flt device, then the grid, the tables and the balance
@@ -20,6 +23,12 @@ program mc_fortran_check
     implicit none
     character(len=512) :: mode, dir
     integer :: nargs
+    ! grains mode: the scaled J and the grain tables of grains.bin
+    integer(c_int64_t) :: gh(8)
+    real(c_float), allocatable, target :: gAbs(:), gWt(:), gAbn(:), gEm(:), gRad(:), gVn(:), gMs(:), gAw(:), gNd(:)
+    real, allocatable :: gTd(:, :, :)
+    real(c_float) :: gXhil
+    real(c_double), allocatable, target :: gJ(:, :)
 
     nargs = command_argument_count()
     if (nargs < 1) then
@@ -33,9 +42,12 @@ program mc_fortran_check
         call get_command_argument(2, dir)
         mode = ' '
         if (nargs >= 3) call get_command_argument(3, mode)
-        call run(trim(dir), trim(mode) == 'short')
+        call run(trim(dir), trim(mode) == 'short', .false.)
+    else if (trim(mode) == 'grains' .and. nargs >= 2) then
+        call get_command_argument(2, dir)
+        call run(trim(dir), .false., .true.)
     else
-        print *, 'usage: mc_fortran_check selftest | run DIR'
+        print *, 'usage: mc_fortran_check selftest | run DIR [short] | grains DIR'
         stop 2
     end if
 
@@ -56,9 +68,9 @@ contains
         print *, 'selftest ok: ', trim(msg)
     end subroutine selftest
 
-    subroutine run(dir, short)
+    subroutine run(dir, short, grains)
         character(len=*), intent(in) :: dir
-        logical, intent(in) :: short
+        logical, intent(in) :: short, grains
         integer(c_int64_t) :: h(24)
         integer :: u, nx, ny, nz, nc, nb, ns_, nEl, nch, ncel, nst, iStar
         integer(c_int64_t) :: npk, per, off
@@ -89,29 +101,35 @@ contains
         d%starPos = c_loc(sPos); d%starIdx = c_loc(sIdx); d%deltaE = c_loc(dE)
         d%seed = h(9); d%device = int(h(17), c_int32_t)
         call mc_check(mc_ctx_create(d, ctx), c_null_ptr, 'mc_ctx_create')
-        call mc_check(mc_upload_tables(ctx, c_loc(opac), c_null_ptr, c_loc(rec), c_loc(tl), &
-                                       c_loc(spec), c_null_ptr), ctx, 'mc_upload_tables')
-        call mc_check(mc_zero_estimators(ctx), ctx, 'mc_zero_estimators')
-
-        ! iterateMC: each star's packets; packet indices continue across
-        ! stars so that every packet has its own stream
         allocate (J(0:nc, nb), esc(0:nc, 0:nb))
         per = npk / nst
         off = 0
-        nseg = 0.d0
-        do iStar = 1, nst
-            call gpu_energy_packet_driver(ctx, iStar, per, off, J, esc, cnt)
-            off = off + per
-            nseg = nseg + cnt%n_seg
-        end do
-        call write_dbl(dir//'/J.bin', J, size(J))
-        call write_dbl(dir//'/esc.bin', esc, size(esc))
-        print *, 'transport ok: packets', off, ' segments', nseg
+        if (grains) then
+            ! the estimator the grain balance reads, already scaled
+            call read_grains(dir, nc, nb)
+            call mc_check(mc_set_estimators(ctx, c_loc(gJ), c_null_ptr), ctx, 'mc_set_estimators')
+        else
+            call mc_check(mc_upload_tables(ctx, c_loc(opac), c_null_ptr, c_loc(rec), c_loc(tl), &
+                                           c_loc(spec), c_null_ptr), ctx, 'mc_upload_tables')
+            call mc_check(mc_zero_estimators(ctx), ctx, 'mc_zero_estimators')
+
+            ! iterateMC: each star's packets; packet indices continue across
+            ! stars so that every packet has its own stream
+            nseg = 0.d0
+            do iStar = 1, nst
+                call gpu_energy_packet_driver(ctx, iStar, per, off, J, esc, cnt)
+                off = off + per
+                nseg = nseg + cnt%n_seg
+            end do
+            call write_dbl(dir//'/J.bin', J, size(J))
+            call write_dbl(dir//'/esc.bin', esc, size(esc))
+            print *, 'transport ok: packets', off, ' segments', nseg
+        end if
 
         if (h(10) /= 0) then
             ns_ = int(h(11)); nEl = int(h(12)); nch = int(h(13)); ncel = int(h(14))
             call balance(u, ctx, nc, nb, ns_, nEl, nch, ncel, int(h(15)), int(h(16)), &
-                         iand(int(h(6)), int(MC_SYMMETRIC)) /= 0, dir)
+                         iand(int(h(6)), int(MC_SYMMETRIC)) /= 0, dir, grains)
         end if
         close (u)
         if (short) then
@@ -124,10 +142,10 @@ contains
         call mc_ctx_destroy(ctx)
     end subroutine run
 
-    subroutine balance(u, ctx, nc, nb, ns_, nEl, nch, ncel, ndbl, nflt, sym, dir)
+    subroutine balance(u, ctx, nc, nb, ns_, nEl, nch, ncel, ndbl, nflt, sym, dir, grains)
         integer, intent(in) :: u, nc, nb, ns_, nEl, nch, ncel, ndbl, nflt
         type(c_ptr), intent(in) :: ctx
-        logical, intent(in) :: sym
+        logical, intent(in) :: sym, grains
         character(len=*), intent(in) :: dir
         integer(c_int32_t), allocatable, target :: elemOn(:), xref(:), chm(:), dri(:), api(:), celm(:)
         real(c_float), allocatable, target :: abun(:), sig(:), rrec(:), rnew(:), fe(:), drr(:), apr(:), celf(:)
@@ -137,6 +155,7 @@ contains
         integer, allocatable :: conv(:), black(:)
         integer :: noHit, noIonBal, noTeBal, w
         type(mc_atomic) :: A
+        type(mc_dust_tables) :: D
         real(c_double) :: jscale
 
         allocate (elemOn(30), xref(30), abun(30), chm(8 * nch), sig(nch * nb), rrec(1800), rnew(3600), fe(30))
@@ -154,6 +173,26 @@ contains
         allocate (Hden(0:nc), Te(0:nc), Ne(0:nc), ion(0:nc, nEl, ns_), conv(0:nc), black(0:nc))
         read (u) Hden, Te, Ne, ion
         conv = 0
+        if (grains) then
+            ! dustDriver's tables, then updateCell with getDustT; grains.bin's
+            ! J is already scaled
+            D%nSpecies = int(gh(1), c_int32_t); D%nSizes = int(gh(2), c_int32_t)
+            D%nTemps = int(gh(3), c_int32_t); D%nSpeciesMax = int(gh(4), c_int32_t)
+            D%photoelectric = int(gh(5), c_int32_t)
+            D%absXsec = c_loc(gAbs); D%grainWeight = c_loc(gWt); D%grainAbun = c_loc(gAbn)
+            D%emIntegral = c_loc(gEm); D%grainRadius = c_loc(gRad); D%grainVn = c_loc(gVn)
+            D%MsurfAtom = c_loc(gMs); D%aWeight = c_loc(gAw); D%Ndust = c_loc(gNd)
+            D%scaXsec = c_null_ptr; D%TdustSublime = c_null_ptr
+            D%XHILimit = gXhil
+            call mc_check(mc_dust_setup(ctx, D), ctx, 'mc_dust_setup')
+            call gpu_update_cells_dust(ctx, 1.d0, .true., Hden, Te, Ne, ion, gTd, conv, black, &
+                                       noHit, noIonBal, noTeBal)
+            open (newunit=w, file=dir//'/grains_out.bin', access='stream', form='unformatted', status='replace')
+            write (w) gTd, Te, Ne, ion, conv, black, noHit, noIonBal, noTeBal
+            close (w)
+            print *, 'grains ok: noHit', noHit, ' noIonBal', noIonBal, ' noTeBal', noTeBal
+            return
+        end if
         ! Jste x 1e-9, / 8 in symmetric mode (iteration_mod.f90:706, 717-722)
         jscale = 1.d-9
         if (sym) jscale = jscale / 8.d0
@@ -163,6 +202,25 @@ contains
         close (w)
         print *, 'balance ok: noHit', noHit, ' noIonBal', noIonBal, ' noTeBal', noTeBal
     end subroutine balance
+
+    ! grains.bin (stream, native endian): int64 header(8) = nSpecies nSizes
+    ! nTemps nSpeciesMax photoelectric; then REAL absXsec [nSpecies][nSizes]
+    ! [nbins], grainWeight(nSizes), grainAbun(nSpecies), dustEmIntegral(
+    ! nSpecies, nSizes, nTemps), grainRadius(nSizes), grainVn(nSpecies),
+    ! MsurfAtom(nSpecies), aWeight(30), Ndust(0:nCells), XHILimit,
+    ! Tdust(0:nSpeciesMax, 0:nSizes, 0:nCells); DOUBLE J(0:nCells, nbins)
+    subroutine read_grains(dir, nc, nb)
+        character(len=*), intent(in) :: dir
+        integer, intent(in) :: nc, nb
+        integer :: g, nS, nSz, nT, nsm
+        open (newunit=g, file=dir//'/grains.bin', access='stream', form='unformatted', status='old')
+        read (g) gh
+        nS = int(gh(1)); nSz = int(gh(2)); nT = int(gh(3)); nsm = int(gh(4))
+        allocate (gAbs(nS * nSz * nb), gWt(nSz), gAbn(nS), gEm(nS * nSz * nT), gRad(nSz), gVn(nS), gMs(nS))
+        allocate (gAw(30), gNd(0:nc), gTd(0:nsm, 0:nSz, 0:nc), gJ(0:nc, nb))
+        read (g) gAbs, gWt, gAbn, gEm, gRad, gVn, gMs, gAw, gNd, gXhil, gTd, gJ
+        close (g)
+    end subroutine read_grains
 
     subroutine write_dbl(path, a, n)
         character(len=*), intent(in) :: path

@@ -3,7 +3,8 @@ its check program links against libmocassin_amd.so, reports C-side errors
 without a device, and on the GPU produces exactly what the Python binding
 of the same C ABI produces (transport J and escapes to the FP64 atomics'
 summation-order rounding, balance outputs on the same J bit for bit after
-narrowing to REAL), and stops a sweep into a J of the wrong extents."""
+narrowing to REAL), and stops a sweep into a J of the wrong extents; the
+updateCell loop with grains likewise."""
 import os
 import subprocess
 
@@ -119,3 +120,71 @@ def test_fortran_host_matches_python_binding(tmp_path):
     assert np.array_equal(out["conv"][1:][hit[1:]], (fl[1:][hit[1:]] & 1))
     assert np.array_equal(out["black"][1:], (fl[1:] & 2) >> 1)
     assert out["counts"][0] == int(np.sum(~hit[1:]))
+
+
+@pytest.mark.gpu
+def test_fortran_grains_matches_python_binding(tmp_path):
+    """gpu_update_cells_dust (updateCell with getDustT, the grain charge and
+    the photoelectric terms) through the Fortran module on the reference's
+    dust deck estimator: Tdust, Te, Ne and the ionic fractions equal the
+    Python binding's after narrowing to REAL, and the flags follow the
+    reference's bookkeeping."""
+    import types
+    if not gpu_available():
+        pytest.skip("no GPU")
+    from test_gpu_balance import _context
+    from test_gpu_dust import _fixture, _pe_tables
+    print("fortran host: grains of the dust deck", flush=True)
+    g, b, J, Hden, Te0, Ne0, ion0, ref, T0, Tref = _fixture("dust_pe")
+    nc, nb = J.shape[0] - 1, J.shape[1]
+    n = 2
+    while n ** 3 < nc + 1:
+        n += 1
+    act = np.zeros(n ** 3, np.int32)
+    act[:nc] = np.arange(1, nc + 1, dtype=np.int32)
+    ax = np.linspace(0.0, 1.e17, n).astype(np.float32)
+    zt = np.zeros((nc + 1, nb), np.float32)
+    P = types.SimpleNamespace(nCells=nc, nbins=nb, active=act.reshape(n, n, n, order="F"), symmetric=False,
+                              xAxis=ax, yAxis=ax, zAxis=ax, R_out=1.e17, geoCorr=(0., 0., 0.), ionEdge1=1.0,
+                              nuArray=b.nuArray, starPos=(0., 0., 0.), starIdx=(1, 1, 1), deltaE=1.0,
+                              opacity=zt, recPDF=zt, totalLines=np.zeros(nc + 1, np.float32),
+                              specCDF=np.linspace(0.0, 1.0, nb, dtype=np.float32))
+    _write_problem(str(tmp_path / "problem.bin"), P, b, Hden, Te0, Ne0, ion0, 0, 1)
+    nS, nSz, nT, nsm, nSp = (int(v) for v in g["dust_ints"][:5])
+    hdr = np.zeros(8, np.int64)
+    hdr[:5] = [nS, nSz, nT, nsm, 1]
+    f32 = lambda a: np.ascontiguousarray(a, np.float32).ravel()
+    with open(str(tmp_path / "grains.bin"), "wb") as f:
+        for a in (hdr, f32(g["dust_abs"]), f32(g["dust_weight"]), f32(g["dust_abun"]), f32(g["dust_emint"]),
+                  f32(g["dust_radius"]), f32(g["dust_vn"]), f32(g["dust_msurf"]), f32(g["dust_aweight"]),
+                  f32(g["dust_ndust"]), f32([b.XHILimit]), f32(g["Tdust0"]),
+                  np.asarray(J, np.float64).ravel(order="F")):
+            f.write(np.ascontiguousarray(a).tobytes())
+    r = subprocess.run([_exe(), "grains", str(tmp_path)], capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0 and "grains ok" in r.stdout
+    raw = open(str(tmp_path / "grains_out.bin"), "rb").read()
+    nE, ns = b.nElemUsed, b.nstages
+    o, out = 0, {}
+    for k, m, dt in (("Td", (nc + 1) * (nSz + 1) * (nsm + 1), np.float32), ("Te", nc + 1, np.float32),
+                     ("Ne", nc + 1, np.float32), ("ion", (nc + 1) * nE * ns, np.float32),
+                     ("conv", nc + 1, np.int32), ("black", nc + 1, np.int32), ("counts", 3, np.int32)):
+        out[k] = np.frombuffer(raw, dt, m, o)
+        o += m * np.dtype(dt).itemsize
+    assert o == len(raw)
+    T = _context(nc, nb, b.nuArray)
+    try:
+        T.set_estimators(J.astype(np.float64))
+        T.balance_setup(b)
+        T.dust_setup(**_pe_tables(g))
+        Tg, fl, nI, ms, Te, Ne, ion = T.update_grains(T0, Hden=Hden, Te=Te0, Ne=Ne0, ionDen=ion0)
+    finally:
+        T.close()
+    assert np.array_equal(out["Td"], Tg.astype(np.float32).ravel())
+    assert np.array_equal(out["Te"], Te.astype(np.float32)) and np.array_equal(out["Ne"], Ne.astype(np.float32))
+    assert np.array_equal(out["ion"].reshape(ns, nE, nc + 1).transpose(2, 1, 0), ion.astype(np.float32))
+    hit = (fl & 16) == 0
+    assert np.array_equal(out["conv"][1:][hit[1:]], fl[1:][hit[1:]] & 1)
+    assert np.array_equal(out["black"][1:], (fl[1:] & 2) >> 1)
+    assert out["counts"][0] == int(np.sum(~hit[1:]))
+    print("fortran grains: %d cells, Tdust %.1f-%.1f K" % (nc, Tg[1:, 1:, 1].min(), Tg[1:, 1:, 1].max()))
