@@ -112,10 +112,17 @@ def j_map_from_segments(cells, ds, axes, active, symmetric, dE=np.float32(1.0)):
 
 
 def gpu_available():
+    """A HIP device is visible to the runtime libmocassin_amd.so links
+    (asked directly: torch need not be loaded, and once the library has
+    loaded the system HIP runtime torch's own reports no device)."""
+    import ctypes
     try:
-        import torch
-        return torch.cuda.is_available()
-    except Exception:
+        from mocassin_amd import _capi
+        _capi.load()
+        hip = ctypes.CDLL("libamdhip64.so.7")
+        n = ctypes.c_int(0)
+        return hip.hipGetDeviceCount(ctypes.byref(n)) == 0 and n.value > 0
+    except OSError:
         return False
 
 

@@ -40,7 +40,7 @@ def test_batch_equals_separate_runs(tuning):
     models = _models(SPECS)
     try:
         npk = [s[1] for s in SPECS]
-        offs = [0, 0, 12345, 7, 0]
+        offs = [0, 0, 12345, 2**33 + 7, 0]                  # one range far above 2^32
         for T in models:
             T.zero_estimators()
         cb, ms = transport_batch(models, [1] * len(models), npk, offs, tuning)

@@ -159,6 +159,35 @@ def test_launch_partition_independence():
     _cmp_estimators(J2, E2, J1, E1)
 
 
+def test_ragged_counts_and_wide_packet_ids_equal_oracle():
+    """Packet counts that do not fill a wavefront (1, 63, 65), none at all,
+    and packet ranges across 2^32 and far above it: every packet id is a
+    64-bit Philox counter, so each range equals the oracle's packet for
+    packet, and a sweep of no packets leaves the estimators zero."""
+    g = golden("transport_dust_mini_thick")
+    P = oracle_problem(g)
+    seed = int(g["seed"])
+    T = transport_from_problem(P, seed)
+    try:
+        c = T.energy_packet_driver(1, 0, 0)
+        J, E = T.fetch()
+        assert c["n_seg"] == 0 and c["n_flights"] == 0 and c["qphot"] == 0.0
+        assert not J.any() and not E.any()
+        for first, n in ((0, 1), (17, 63), (4096, 65), (2**32 - 40, 100), (2**40 + 3, 130),
+                         (2**63 + 11, 70)):
+            print("ragged/wide ids", first, n, flush=True)
+            T.zero_estimators()
+            c = T.energy_packet_driver(1, n, first)
+            J, E = T.fetch()
+            Jo, Eo, co = P.transport(first, n, seed, 1, O.MATH_OWN, O.J_FP64)
+            for k in ("n_seg", "n_abs", "n_sca", "n_esc", "n_lost", "n_line", "n_trapped", "n_flights"):
+                assert c[k] == co[k], (first, n, k)
+            assert c["n_flights"] >= n
+            _cmp_estimators(J, E, Jo, Eo)
+    finally:
+        T.close()
+
+
 @pytest.fixture(scope="module")
 def c2_problem():
     from mocassin_amd import synth
