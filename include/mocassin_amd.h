@@ -97,7 +97,7 @@ typedef struct {
  *                  wavefront wait for one (1..64), or half the live lanes do
  *   walk_steps     walk passes between two emission checks (1..64)
  * Results do not depend on these values (each packet has its own stream).
- * A new context starts with 4, 256, 64, 32. */
+ * A new context starts with 4, 256, 64, 20. */
 typedef struct {
     int32_t blocks_per_cu;
     int32_t threads;

@@ -19,8 +19,8 @@ DB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_db.js
 DEFAULT_GRID = {
     "blocks_per_cu": (2, 4, 8),
     "threads": (256,),
-    "emit_min": (24, 32, 48, 64),
-    "walk_steps": (4, 8, 16, 32),
+    "emit_min": (32, 48, 64),
+    "walk_steps": (8, 12, 16, 20, 24, 32),
 }
 
 

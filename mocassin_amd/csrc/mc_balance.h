@@ -79,7 +79,8 @@ struct Grains {
     const double *Tdust;        // Tdust(0:nsMax, 0:nSizes, 0:nCells) before the update
     double *gdc, *peh;          // [nCells+1][K]: gasDustColl_d, photoelHeat_d of the last T-iteration
     const double *J3;           // the estimator, tiled (mcd::j3_index)
-    const int *cell_xyz;        // lattice point of each cell
+    const int *cell_xyz;        // lattice point of each cell, (x-1) + nx*((y-1) + ny*(z-1))
+    int nx, ny, nz;             // the lattice (J3's tiling)
 };
 
 struct CellArgs {
@@ -147,10 +148,10 @@ int prepare(const mc_atomic *in, int nbins, const float *nuArray, Prepared &P, c
 hipError_t mcb_launch_balance(const mcb::Atomic *dA, const mcb::CellArgs &a, hipStream_t s);
 // channel integrals Rint/Hint [nCells+1][nch], hit flags [nCells+1] and
 // (X non-null) the warm-start network's inputs X [nCells][72] of every
-// active cell, from the dense J3[nu][xyz] times a.jscale
+// active cell, from the tiled J3 (lattice nx x ny x nz) times a.jscale
 hipError_t mcb_launch_integrals(const mcb::Atomic *dA, const mcb::CellArgs &a, const double *J3,
-                                const int *active, int ncell3, double *Rint, double *Hint, int *hit,
-                                float *X, hipStream_t s);
+                                const int *active, int nx, int ny, int nz, double *Rint, double *Hint,
+                                int *hit, float *X, hipStream_t s);
 // Te, Ne, ion of reached cells from the network's outputs Y [nCells][ldy]
 hipError_t mcb_launch_warm(const mcb::Atomic *dA, const mcb::CellArgs &a, const float *Y, int ldy,
                            const int *hit, hipStream_t s);

@@ -380,9 +380,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) ba
     if (PE) {
         // the cell's photon flux J/(h c Ryd nu) (getGrainEmission, setPhotoelHeatCool)
         const Grains &G = *a.grains;
-        const int xyz = G.cell_xyz[cell];
+        const unsigned pt = mcd::j3_point_of(mcd::j3_shape(G.nx, G.ny, G.nz), G.cell_xyz[cell]);
         for (int f = lane; f < A.nbins; f += W)
-            pf[f] = G.J3[mcd::j3_index(A.nbins, f, xyz)] * a.jscale / (kHcRyd * A.nu[f]);
+            pf[f] = G.J3[mcd::j3_index(A.nbins, f, pt)] * a.jscale / (kHcRyd * A.nu[f]);
     }
     __syncthreads();
     // lgConverged compares with the caller's H0 fraction (update_mod.f90:510-515)
@@ -599,7 +599,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) ba
 // elements (-30 when absent).  Channels are accumulated eight at a time.
 __global__ void __launch_bounds__(64) integrals_kernel(const Atomic *__restrict__ Ap, CellArgs a,
                                                        const double *__restrict__ J3,
-                                                       const int *__restrict__ active, int ncell3,
+                                                       const int *__restrict__ active, mcd::J3Shape js,
                                                        double *__restrict__ Rint, double *__restrict__ Hint,
                                                        int *__restrict__ hit, float *__restrict__ X)
 {
@@ -607,12 +607,12 @@ __global__ void __launch_bounds__(64) integrals_kernel(const Atomic *__restrict_
     __shared__ double cr[G][C], ch[G][C];              // coefficients of one chunk of bins
     const Atomic &A = *Ap;
     const int lane = threadIdx.x;
-    const int xyz = blockIdx.x * 64 + lane;
-    const int cell = xyz < ncell3 ? active[xyz] : 0;
+    const int xyz = mcd::j3_lattice(js, blockIdx.x * 64u + (unsigned)lane);
+    const int cell = xyz >= 0 ? active[xyz] : 0;
     if (!__any(cell > 0)) return;
     const int nb = A.nbins, nch = A.nch;
     // this wavefront's tile of J3: [nu][64 points], contiguous
-    const double *Jp = J3 + mcd::j3_index(nb, 0, blockIdx.x * 64) + lane;
+    const double *Jp = J3 + mcd::j3_index(nb, 0, blockIdx.x * 64u) + lane;
     float *brow = (X && cell > 0) ? X + (size_t)(cell - 1) * mcm::kIn : nullptr;
     int any = 0;
     for (int g0 = 0; g0 < nch || g0 == 0; g0 += G) {
@@ -859,10 +859,11 @@ hipError_t mcb_launch_balance(const mcb::Atomic *dA, const mcb::CellArgs &a, hip
 }
 
 hipError_t mcb_launch_integrals(const mcb::Atomic *dA, const mcb::CellArgs &a, const double *J3, const int *active,
-                               int ncell3, double *Rint, double *Hint, int *hit, float *X, hipStream_t s)
+                               int nx, int ny, int nz, double *Rint, double *Hint, int *hit, float *X, hipStream_t s)
 {
-    hipLaunchKernelGGL(mcb::integrals_kernel, dim3((ncell3 + 63) / 64), dim3(64), 0, s, dA, a, J3, active, ncell3,
-                       Rint, Hint, hit, X);
+    const mcd::J3Shape js = mcd::j3_shape(nx, ny, nz);
+    hipLaunchKernelGGL(mcb::integrals_kernel, dim3((unsigned)(mcd::j3_points(js) / mcd::kJTile)), dim3(64), 0, s,
+                       dA, a, J3, active, js, Rint, Hint, hit, X);
     return hipGetLastError();
 }
 

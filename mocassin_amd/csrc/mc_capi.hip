@@ -33,7 +33,7 @@ struct mc_ctx {
     std::vector<int> starIdx;
     bool tables = false;
     int cus = 256;
-    mc_tuning tune = { 4, 256, 64, 32 };    // the autotuner's best on the octant, dust and clumpy grids
+    mc_tuning tune = { 4, 256, 64, 20 };    // the autotuner's best on the octant; within 1.3% on the dust and clumpy grids
     // device allocations
     float *d_axis = nullptr, *d_mid = nullptr, *d_dfac = nullptr;
     int *d_active = nullptr;
@@ -151,6 +151,9 @@ extern "C" int mc_ctx_create(const mc_desc *d, mc_ctx **out)
         return fail(nullptr, MC_EINVAL, "descriptor has empty sizes or null arrays");
     if (d->nbins > 65536)
         return fail(nullptr, MC_EUNSUP, "nbins %d: the transport's J queue indexes at most 65536 bins", d->nbins);
+    if (mcd::j3_points(mcd::j3_shape(d->nx, d->ny, d->nz)) > 0x7fffffffu)
+        return fail(nullptr, MC_EUNSUP, "lattice %d x %d x %d: at most 2^31 points (padded to multiples of 4)",
+                    d->nx, d->ny, d->nz);
     // the energy of a packet of each source (iteration_mod.f90:477-493
     // divides the luminosity by the packet count): finite and positive
     for (int s = 0; s < d->nStars; ++s)
@@ -234,7 +237,7 @@ static int ctx_init(mc_ctx *c, const mc_desc *d)
     HIPCHK(c, dalloc(&c->d_active, (size_t)nx * ny * nz));
     HIPCHK(c, dalloc(&c->d_nuArray, nb));
     const size_t ncell3 = (size_t)nx * ny * nz;
-    HIPCHK(c, dalloc(&c->d_J3, mcd::j3_size((int)ncell3, nb)));
+    HIPCHK(c, dalloc(&c->d_J3, mcd::j3_size(mcd::j3_shape(nx, ny, nz), nb)));
     HIPCHK(c, dalloc(&c->d_esc, (size_t)(nc + 1) * (nb + 1)));
     HIPCHK(c, dalloc(&c->d_scratch, 9));
     HIPCHK(c, dalloc(&c->d_error, 1));
@@ -254,6 +257,7 @@ static int ctx_init(mc_ctx *c, const mc_desc *d)
     G.axis = c->d_axis; G.mid = c->d_mid; G.dfac = c->d_dfac; G.active = c->d_active;
     G.nuArray = c->d_nuArray;
     G.ncell3 = (int)ncell3;
+    G.js = mcd::j3_shape(nx, ny, nz);
     G.J3 = c->d_J3; G.esc = c->d_esc;
     G.seed_lo = (uint32_t)d->seed; G.seed_hi = (uint32_t)(d->seed >> 32);
 
@@ -368,9 +372,9 @@ extern "C" int mc_zero_estimators(mc_ctx *c)
     if (!c) return MC_EINVAL;
     HIPCHK(c, hipSetDevice(c->device));
     const size_t nc1 = (size_t)c->G.nCells + 1;
-    HIPCHK(c, hipMemsetAsync(c->d_J3, 0, mcd::j3_size(c->G.ncell3, c->G.nbins) * sizeof(double), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_J3, 0, mcd::j3_size(c->G.js, c->G.nbins) * sizeof(double), c->stream));
     HIPCHK(c, hipMemsetAsync(c->d_esc, 0, nc1 * (c->G.nbins + 1) * c->esc_planes * sizeof(double), c->stream));
-    if (c->G.cross) HIPCHK(c, hipMemsetAsync(c->G.cross, 0, (size_t)c->G.ncell3 * sizeof(unsigned), c->stream));
+    if (c->G.cross) HIPCHK(c, hipMemsetAsync(c->G.cross, 0, mcd::j3_points(c->G.js) * sizeof(unsigned), c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return MC_OK;
 }
@@ -380,8 +384,8 @@ extern "C" int mc_count_crossings(mc_ctx *c, int32_t enable)
     if (!c) return MC_EINVAL;
     HIPCHK(c, hipSetDevice(c->device));
     if (enable && !c->d_cross) {
-        HIPCHK(c, dalloc(&c->d_cross, (size_t)c->G.ncell3));
-        HIPCHK(c, hipMemset(c->d_cross, 0, (size_t)c->G.ncell3 * sizeof(unsigned)));
+        HIPCHK(c, dalloc(&c->d_cross, mcd::j3_points(c->G.js)));
+        HIPCHK(c, hipMemset(c->d_cross, 0, mcd::j3_points(c->G.js) * sizeof(unsigned)));
     }
     c->G.cross = enable ? c->d_cross : nullptr;
     return MC_OK;
@@ -393,10 +397,10 @@ extern "C" int mc_fetch_crossings(mc_ctx *c, uint32_t *out)
     if (!out) return fail(c, MC_EINVAL, "null output");
     if (!c->d_cross) return fail(c, MC_ESTATE, "crossing counts are not enabled");
     HIPCHK(c, hipSetDevice(c->device));
-    std::vector<uint32_t> dense(c->G.ncell3);
-    HIPCHK(c, hipMemcpy(dense.data(), c->d_cross, dense.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    std::vector<uint32_t> pts(mcd::j3_points(c->G.js));
+    HIPCHK(c, hipMemcpy(pts.data(), c->d_cross, pts.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
     for (int cell = 0; cell <= c->G.nCells; ++cell) out[cell] = 0;
-    for (size_t i = 0; i < dense.size(); ++i) out[c->active[i]] += dense[i];
+    for (int i = 0; i < c->G.ncell3; ++i) out[c->active[i]] += pts[mcd::j3_point_of(c->G.js, i)];
     return MC_OK;
 }
 
@@ -408,7 +412,7 @@ extern "C" int mc_fetch_estimators(mc_ctx *c, double *J_out, double *esc_out)
     if (J_out) {
         if (!c->d_J) HIPCHK(c, dalloc(&c->d_J, nc1 * c->G.nbins));
         HIPCHK(c, hipMemsetAsync(c->d_J, 0, nc1 * c->G.nbins * sizeof(double), c->stream));
-        HIPCHK(c, mcd_compact(c->d_J3, c->d_active, c->G.ncell3, (int)nc1, c->G.nbins, c->d_J, c->stream));
+        HIPCHK(c, mcd_compact(c->d_J3, c->d_active, c->G.js, (int)nc1, c->G.nbins, c->d_J, c->stream));
         HIPCHK(c, hipMemcpyAsync(J_out, c->d_J, nc1 * c->G.nbins * sizeof(double),
                                  hipMemcpyDeviceToHost, c->stream));
     }
@@ -425,12 +429,13 @@ extern "C" int mc_set_estimators(mc_ctx *c, const double *J, const double *esc)
     if (!J) return fail(c, MC_EINVAL, "J is required");
     HIPCHK(c, hipSetDevice(c->device));
     const size_t nc1 = (size_t)c->G.nCells + 1, nb = c->G.nbins, n3 = c->G.ncell3;
-    std::vector<double> dense(mcd::j3_size((int)n3, (int)nb), 0.0);
-    for (size_t nu = 0; nu < nb; ++nu)
-        for (size_t i = 0; i < n3; ++i) {
-            const int a = c->active[i];
-            if (a > 0) dense[mcd::j3_index((int)nb, (int)nu, (int)i)] = J[nu * nc1 + a];
-        }
+    std::vector<double> dense(mcd::j3_size(c->G.js, (int)nb), 0.0);
+    for (size_t i = 0; i < n3; ++i) {
+        const int a = c->active[i];
+        if (a <= 0) continue;
+        const unsigned pt = mcd::j3_point_of(c->G.js, (int)i);
+        for (size_t nu = 0; nu < nb; ++nu) dense[mcd::j3_index((int)nb, (int)nu, pt)] = J[nu * nc1 + a];
+    }
     HIPCHK(c, hipMemcpy(c->d_J3, dense.data(), dense.size() * sizeof(double), hipMemcpyHostToDevice));
     if (esc) HIPCHK(c, hipMemcpy(c->d_esc, esc, nc1 * (nb + 1) * c->esc_planes * sizeof(double), hipMemcpyHostToDevice));
     return MC_OK;
@@ -607,7 +612,7 @@ static mcb::CellArgs cell_args(mc_ctx *c, double jscale)
 static int integrals(mc_ctx *c, const mcb::CellArgs &a, float *X)
 {
     HIPCHK(c, hipMemsetAsync(c->d_hit, 0, ((size_t)c->G.nCells + 1) * sizeof(int), c->stream));
-    HIPCHK(c, mcb_launch_integrals(c->d_atomic, a, c->d_J3, c->d_active, c->G.ncell3, c->d_Rint, c->d_Hint,
+    HIPCHK(c, mcb_launch_integrals(c->d_atomic, a, c->d_J3, c->d_active, c->G.nx, c->G.ny, c->G.nz, c->d_Rint, c->d_Hint,
                                    c->d_hit, X, c->stream));
     return MC_OK;
 }
@@ -721,6 +726,7 @@ extern "C" int mc_dust_setup(mc_ctx *c, const mc_dust_tables *d)
         GH.Tdust = c->d_Tdust;
         GH.J3 = c->d_J3;
         GH.cell_xyz = c->d_cell_xyz;
+        GH.nx = c->G.nx; GH.ny = c->G.ny; GH.nz = c->G.nz;
         HIPCHK(c, dalloc(&c->d_grains, 1));
         HIPCHK(c, hipMemcpy(c->d_grains, &GH, sizeof GH, hipMemcpyHostToDevice));
         c->pe_gdc = GH.gdc;
@@ -734,7 +740,7 @@ extern "C" int mc_dust_setup(mc_ctx *c, const mc_dust_tables *d)
     }
     mcg::DustArgs &a = c->dust_args;
     a = mcg::DustArgs{};
-    a.ncell3 = c->G.ncell3; a.nb = nb; a.nc = c->G.nCells;
+    a.nx = c->G.nx; a.ny = c->G.ny; a.nz = c->G.nz; a.nb = nb; a.nc = c->G.nCells;
     a.nS = nS; a.nsMax = d->nSpeciesMax; a.nSizes = nSz; a.nTemps = nT;
     a.absx = c->d_dust_tab; a.em = c->d_dust_tab + (size_t)K * nb;
     a.weight = a.em + (size_t)K * nT;

@@ -33,6 +33,64 @@ struct View {
     float vth[3], vph[3];
 };
 
+// J3 layout.  The lattice is cut into tiles of 4x4x4 points (each axis
+// padded to a multiple of 4) and a tile holds [nu][64 points].  Inside a
+// tile the points run [2x2x2 brick][2x2x2 point], so the 8 doubles of one
+// 64-byte line are a 2x2x2 brick at one frequency.  Atomic adds of one
+// instruction that fall in one line travel to memory as one request, and a
+// packet's next segment (same frequency, a neighbouring cell) stays in the
+// brick after half of its steps, against 7/24 for 8 points along x.  One
+// wavefront reading a tile's spectra (64 lanes, one point each) streams
+// contiguous memory.
+constexpr int kJTile = 64;
+struct J3Shape {
+    int nx, ny, nz;          // lattice
+    int tx, ty, tz;          // tiles per axis
+};
+__host__ __device__ __forceinline__ J3Shape j3_shape(int nx, int ny, int nz)
+{
+    return J3Shape{ nx, ny, nz, (nx + 3) / 4, (ny + 3) / 4, (nz + 3) / 4 };
+}
+__host__ __device__ __forceinline__ size_t j3_points(const J3Shape &s)
+{
+    return (size_t)s.tx * (size_t)s.ty * (size_t)s.tz * kJTile;
+}
+__host__ __device__ __forceinline__ size_t j3_size(const J3Shape &s, int nb)
+{
+    return j3_points(s) * (size_t)nb;
+}
+// the point of lattice cell (x, y, z), 0-based
+__host__ __device__ __forceinline__ unsigned j3_point(const J3Shape &s, int x, int y, int z)
+{
+    const unsigned tile = ((unsigned)(z >> 2) * (unsigned)s.ty + (unsigned)(y >> 2)) * (unsigned)s.tx +
+                          (unsigned)(x >> 2);
+    const unsigned slot = ((unsigned)(z & 2) << 4) | ((unsigned)(y & 2) << 3) | ((unsigned)(x & 2) << 2) |
+                          ((unsigned)(z & 1) << 2) | ((unsigned)(y & 1) << 1) | (unsigned)(x & 1);
+    return tile * kJTile + slot;
+}
+// the point of dense lattice index xyz = x + nx*(y + ny*z)
+__host__ __device__ __forceinline__ unsigned j3_point_of(const J3Shape &s, int xyz)
+{
+    return j3_point(s, xyz % s.nx, (xyz / s.nx) % s.ny, xyz / (s.nx * s.ny));
+}
+// the dense lattice index of point p, or -1 for a point of the padding
+__host__ __device__ __forceinline__ int j3_lattice(const J3Shape &s, unsigned p)
+{
+    const unsigned tile = p / kJTile, slot = p % kJTile;
+    const int x = (int)(tile % (unsigned)s.tx) * 4 + (int)((slot >> 2) & 2) + (int)(slot & 1);
+    const int y = (int)((tile / (unsigned)s.tx) % (unsigned)s.ty) * 4 + (int)((slot >> 3) & 2) +
+                  (int)((slot >> 1) & 1);
+    const int z = (int)(tile / ((unsigned)s.tx * (unsigned)s.ty)) * 4 + (int)((slot >> 4) & 2) +
+                  (int)((slot >> 2) & 1);
+    if (x >= s.nx || y >= s.ny || z >= s.nz) return -1;
+    return x + s.nx * (y + s.ny * z);
+}
+// the word of bin nu0 (0-based) of point p
+__host__ __device__ __forceinline__ size_t j3_index(int nb, int nu0, unsigned p)
+{
+    return ((size_t)(p / kJTile) * (unsigned)nb + (unsigned)nu0) * kJTile + (size_t)(p % kJTile);
+}
+
 struct Grid {
     int nx, ny, nz, nCells, nbins, flags, nStars;
     float R_out, geoCorrX, geoCorrY, geoCorrZ, ionEdge1;
@@ -50,26 +108,12 @@ struct Grid {
     const float *nuArray;    // [nu]
     const float *gSca;       // [nu] or null
     double *J3;              // FP64 path-length estimator, tiled: see j3_index
+    J3Shape js;              // its tiling
     double *esc;             // [plane 0..nAngle][nu 0..nbins][cell]
     const View *view;        // viewing-angle bins, or null (plane 0 only)
-    unsigned *cross;         // [x,y,z] segments walked in each cell, or null
+    unsigned *cross;         // [J3 point] segments walked in each cell, or null
     uint32_t seed_lo, seed_hi;
 };
-
-// J3 layout: lattice points in tiles of 64 consecutive (x,y,z) indices; a
-// tile holds [nu][64].  A packet's x-steps stay in one tile row, and one
-// wavefront reading a tile's spectra (64 lanes, one point each) streams
-// contiguous memory.
-constexpr int kJTile = 64;
-__host__ __device__ __forceinline__ size_t j3_index(int nb, int nu0, int xyz)
-{
-    const unsigned u = (unsigned)xyz;
-    return ((size_t)(u / kJTile) * (unsigned)nb + (unsigned)nu0) * kJTile + (size_t)(u % kJTile);
-}
-__host__ __device__ __forceinline__ size_t j3_size(int ncell3, int nb)
-{
-    return (size_t)((ncell3 + kJTile - 1) / kJTile) * kJTile * nb;
-}
 
 // Guide table (Chen & Asau 1974, "On generating random variates from an
 // empirical distribution"): g[j] = #{i : cdf[i] <= j/K}.  For u in

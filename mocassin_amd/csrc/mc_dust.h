@@ -10,7 +10,7 @@ constexpr int kMaxRows = 32;          // nSpecies * nSizes
 constexpr int kMaxTemps = 8192;       // length of an emission-integral row
 
 struct DustArgs {
-    int ncell3, nb, nc;               // lattice points, frequency bins, cells
+    int nx, ny, nz, nb, nc;           // lattice, frequency bins, cells
     int nS, nsMax, nSizes, nTemps;    // Tdust is (0:nsMax, 0:nSizes, 0:nc), column-major
     double jscale;                    // J = J3 * jscale (the host's normalisation of Jste)
     const double *absx;               // [nS*nSizes][nb], row (s-1)*nSizes + ai-1

@@ -54,11 +54,12 @@ __global__ void __launch_bounds__(64) dust_kernel(DustArgs a, const double *__re
 {
     __shared__ double sx[G][64];              // cross-sections of 64 bins, G rows
     const int lane = threadIdx.x;
-    const int xyz = blockIdx.x * 64 + lane;
-    const int cell = xyz < a.ncell3 ? active[xyz] : 0;
+    const mcd::J3Shape js = mcd::j3_shape(a.nx, a.ny, a.nz);
+    const int xyz = mcd::j3_lattice(js, blockIdx.x * 64u + (unsigned)lane);
+    const int cell = xyz >= 0 ? active[xyz] : 0;
     if (!__any(cell > 0)) return;             // the whole tile is inactive
     const int nb = a.nb, K = a.nS * a.nSizes;
-    const double *Jp = J3 + mcd::j3_index(nb, 0, blockIdx.x * 64) + lane;
+    const double *Jp = J3 + mcd::j3_index(nb, 0, blockIdx.x * 64u) + lane;
     double acc[G];
 #pragma unroll
     for (int k = 0; k < G; ++k) acc[k] = 0.0;
@@ -118,7 +119,7 @@ __global__ void __launch_bounds__(64) dust_kernel(DustArgs a, const double *__re
 hipError_t mcg_launch_dust(const mcg::DustArgs &a, const double *J3, const int *active, hipStream_t s)
 {
     const int K = a.nS * a.nSizes;
-    const dim3 grid((a.ncell3 + 63) / 64), block(64);
+    const dim3 grid((unsigned)(mcd::j3_points(mcd::j3_shape(a.nx, a.ny, a.nz)) / mcd::kJTile)), block(64);
     if (K <= 8) mcg::dust_kernel<8><<<grid, block, 0, s>>>(a, J3, active);
     else if (K <= 16) mcg::dust_kernel<16><<<grid, block, 0, s>>>(a, J3, active);
     else mcg::dust_kernel<32><<<grid, block, 0, s>>>(a, J3, active);

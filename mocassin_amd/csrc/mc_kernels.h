@@ -6,6 +6,7 @@
 namespace mcd {
 
 struct Grid;
+struct J3Shape;
 
 // Per-launch arguments of the transport kernel.
 struct RunArgs {
@@ -44,7 +45,7 @@ hipError_t mcd_launch_transport(const mcd::Grid &G, const mcd::RunArgs &RA, int 
 hipError_t mcd_densify(const float *compact, const int *active, int ncell3, int nc1, int nbins,
                        float *dense, hipStream_t s);
 hipError_t mcd_guides(const float *cdf, int rows, int nbins, unsigned short *guide, hipStream_t s);
-hipError_t mcd_compact(const double *dense, const int *active, int ncell3, int nc1, int nbins,
+hipError_t mcd_compact(const double *J3, const int *active, const mcd::J3Shape &js, int nc1, int nbins,
                        double *out, hipStream_t s);
 hipError_t mcd_launch_replay(const mcd::Grid &G, int n, const float *pos, const float *dir,
                              const int *cell, int *cell_seq, float *ds_seq, int *seq_len,

@@ -49,56 +49,73 @@ __host__ __device__ inline size_t lds_floats(int nx, int ny, int nz, int nbins)
 }
 
 // ------------------------------------------------ J update queue (per wave)
-// kJQ entries of (lattice point, bin - 1, value) per wavefront at the start
-// of dynamic LDS: xyz u32 [kJQ] | value f32 [kJQ] | bin u16 [kJQ].  n is
-// wave-uniform.
-constexpr int kJQ = 256;
-constexpr int kJQFloats = kJQ * 2 + kJQ / 2;          // 2560 bytes per wavefront
+// Each lane keeps its last kJD updates (J3 point, bin - 1, value) in a ring
+// of its own at the start of dynamic LDS: point u32 [kJQ] | value f32 [kJQ] |
+// bin u16 [kJQ], entry k of lane L at L * kJRow + k (rows padded by one word
+// against bank conflicts).  When a lane's ring is full the wavefront flushes
+// every ring, lane-major: atomic instruction j carries the rings of lanes
+// 8j .. 8j + 7.  Adds of one instruction that fall in one 64-byte line travel
+// to memory as one request, and with J3's 2x2x2 bricks a packet's next
+// segment stays in the line after half of its steps, so keeping a packet's
+// updates in one instruction turns those pairs into single requests.
+constexpr int kJD = 8;                                 // updates per lane
+constexpr int kJRow = kJD + 1;
+constexpr int kJQ = 64 * kJRow;
+constexpr int kJQFloats = kJQ * 2 + kJQ / 2;           // 5760 bytes per wavefront
 
 struct JQ {
-    unsigned *xyz;
+    unsigned *pt;
     float *val;
     unsigned short *nu;
-    int n;
+    int n;                // this lane's queued updates
 };
 
 __device__ __forceinline__ JQ jq_at(float *lds, int wave)
 {
     float *b = lds + (size_t)wave * kJQFloats;
     JQ Q;
-    Q.xyz = reinterpret_cast<unsigned *>(b);
+    Q.pt = reinterpret_cast<unsigned *>(b);
     Q.val = b + kJQ;
     Q.nu = reinterpret_cast<unsigned short *>(b + 2 * kJQ);
     Q.n = 0;
     return Q;
 }
 
-// every queued update, 64 lanes per atomic instruction
+// every queued update: instruction j takes entry (lane % 8) of lane
+// 8j + lane / 8's ring
 __device__ __forceinline__ void jq_flush(const Grid &G, JQ &Q, int lane)
 {
+    static_assert(kJD == 8, "eight rings of eight entries per instruction");
     __builtin_amdgcn_wave_barrier();
-    for (int i = lane; i < Q.n; i += 64) {
-        const unsigned xyz = Q.xyz[i];
-        unsafeAtomicAdd(&G.J3[j3_index(G.nbins, (int)Q.nu[i], (int)xyz)], (double)Q.val[i]);
-        if (G.cross) atomicAdd(&G.cross[xyz], 1u);
+    const int k = lane & 7;
+    for (int j = 0; j < 8; ++j) {
+        const int owner = 8 * j + (lane >> 3);
+        const int cnt = __shfl(Q.n, owner);
+        const bool act = k < cnt;
+        if (!__any(act)) continue;
+        if (act) {
+            const int slot = owner * kJRow + k;
+            const unsigned pt = Q.pt[slot];
+            unsafeAtomicAdd(&G.J3[j3_index(G.nbins, (int)Q.nu[slot], pt)], (double)Q.val[slot]);
+            if (G.cross) atomicAdd(&G.cross[pt], 1u);
+        }
     }
     __builtin_amdgcn_wave_barrier();
     Q.n = 0;
 }
 
 // the J update of each lane with `has` (path length times deltaE over dV,
-// photon_mod.f90:1563-1574, 1822-1833), flushed when fewer than 64 slots remain
-__device__ __forceinline__ void jq_push(const Grid &G, JQ &Q, bool has, int xyz, int nuP, float v, int lane)
+// photon_mod.f90:1563-1574, 1822-1833), flushed when a lane's ring is full
+__device__ __forceinline__ void jq_push(const Grid &G, JQ &Q, bool has, unsigned pt, int nuP, float v, int lane)
 {
-    const uint64_t m = __ballot(has);
     if (has) {
-        const int slot = Q.n + (int)__popcll(m & ((1ull << lane) - 1ull));
-        Q.xyz[slot] = (unsigned)xyz;
+        const int slot = lane * kJRow + Q.n;
+        Q.pt[slot] = pt;
         Q.val[slot] = v;
         Q.nu[slot] = (unsigned short)(nuP - 1);
+        Q.n += 1;
     }
-    Q.n = __builtin_amdgcn_readfirstlane(Q.n + (int)__popcll(m));
-    if (Q.n > kJQ - 64) jq_flush(G, Q, lane);
+    if (__any(Q.n == kJD)) jq_flush(G, Q, lane);
 }
 
 // ------------------------------------------------------------ replay kernel
@@ -316,14 +333,15 @@ __device__ __forceinline__ void advance(Lane &L, const Grid &G, const Axes &A, c
         bool seg = false, escaped = false, lost = false, absorbed = false, scattered = false,
              trapped = false;
         float jv = 0.f;
-        int jxyz = 0, jnu = 1;
+        unsigned jpt = 0;
+        int jnu = 1;
         if (L.phase == PH_WALK) {
             L.steps++;
             float kappa, dV;
             int r = walk_step<false>(G, A, L.W, L.nuP, kappa, dV, err);
             if (r == ST_CONTINUE || r == ST_ESCAPED) {
                 jv = L.W.seg * M.dE / dV;
-                jxyz = L.W.segIdx;
+                jpt = j3_point(G.js, L.W.segX - 1, L.W.segY - 1, L.W.segZ - 1);
                 jnu = L.nuP;
                 seg = true;
                 if (r == ST_ESCAPED) {
@@ -336,7 +354,7 @@ __device__ __forceinline__ void advance(Lane &L, const Grid &G, const Axes &A, c
                 L.phase = PH_IDLE;
             } else if (r == ST_ABSORBED) {
                 jv = L.W.seg * M.dE / dV;
-                jxyz = L.W.segIdx;
+                jpt = j3_point(G.js, L.W.segX - 1, L.W.segY - 1, L.W.segZ - 1);
                 jnu = L.nuP;
                 seg = true;
                 float rx = L.W.rx / 1.e10f, ry = L.W.ry / 1.e10f, rz = L.W.rz / 1.e10f;
@@ -387,7 +405,7 @@ __device__ __forceinline__ void advance(Lane &L, const Grid &G, const Axes &A, c
                 L.phase = PH_IDLE;
             }
         }
-        jq_push(G, Q, seg, jxyz, jnu, jv, lane);
+        jq_push(G, Q, seg, jpt, jnu, jv, lane);
         C.seg += wcount(seg);
         C.esc += wcount(escaped);
         C.lost += wcount(lost);
@@ -591,14 +609,14 @@ __global__ void guide_kernel(const float *cdf, int rows, int nbins, unsigned sho
 
 // J(cell,nu) = value at the (x,y,z) of that cell; row 0 collects every
 // inactive position, as the reference's Jste(0,:) does.  out must be zero.
-__global__ void compact_kernel(const double *dense, const int *active, int ncell3, int nc1,
-                               int nbins, double *out)
+__global__ void compact_kernel(const double *J3, const int *active, J3Shape js, int nc1, int nbins, double *out)
 {
+    const int ncell3 = js.nx * js.ny * js.nz;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t tot = (size_t)ncell3 * nbins;
     for (; i < tot; i += (size_t)gridDim.x * blockDim.x) {
         int nu = (int)(i / ncell3), xyz = (int)(i % ncell3);
-        double v = dense[j3_index(nbins, nu, xyz)];
+        double v = J3[j3_index(nbins, nu, j3_point_of(js, xyz))];
         int a = active[xyz];
         if (a > 0) out[(size_t)nu * nc1 + a] = v;
         else if (v != 0.0) unsafeAtomicAdd(&out[(size_t)nu * nc1], v);
@@ -653,10 +671,9 @@ hipError_t mcd_guides(const float *cdf, int rows, int nbins, unsigned short *gui
     return hipGetLastError();
 }
 
-hipError_t mcd_compact(const double *dense, const int *active, int ncell3, int nc1, int nbins,
+hipError_t mcd_compact(const double *J3, const int *active, const mcd::J3Shape &js, int nc1, int nbins,
                        double *out, hipStream_t s)
 {
-    hipLaunchKernelGGL(mcd::compact_kernel, dim3(2048), dim3(256), 0, s, dense, active, ncell3,
-                       nc1, nbins, out);
+    hipLaunchKernelGGL(mcd::compact_kernel, dim3(2048), dim3(256), 0, s, J3, active, js, nc1, nbins, out);
     return hipGetLastError();
 }
