@@ -136,6 +136,21 @@ static hipError_t dalloc(T **p, size_t n)
     return hipMalloc((void **)p, n * sizeof(T) ? n * sizeof(T) : sizeof(T));
 }
 
+// Every copy and fill of a context goes on its stream, in order with its
+// kernels: the context's stream is non-blocking, so a blocking copy or fill
+// on the null stream is not ordered with its work (a fill of a fresh buffer
+// could land after the kernel that writes it).  A copy returns when it is
+// done, so its host buffer may be a temporary.
+static hipError_t copy_on(hipStream_t s, void *dst, const void *src, size_t bytes, hipMemcpyKind kind)
+{
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, s);
+    return e == hipSuccess ? hipStreamSynchronize(s) : e;
+}
+static hipError_t fill_on(hipStream_t s, void *dst, int value, size_t bytes)
+{
+    return hipMemsetAsync(dst, value, bytes, s);
+}
+
 static int ctx_init(mc_ctx *c, const mc_desc *d);
 static int noalloc_scratch_init(mc_ctx *c);
 
@@ -242,11 +257,11 @@ static int ctx_init(mc_ctx *c, const mc_desc *d)
     HIPCHK(c, dalloc(&c->d_scratch, 9));
     HIPCHK(c, dalloc(&c->d_error, 1));
     HIPCHK(c, dalloc(&c->d_qphot, 1));
-    HIPCHK(c, hipMemcpy(c->d_axis, axis.data(), na * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d_mid, mid.data(), nm * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d_dfac, dfac.data(), na * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d_active, d->active, (size_t)nx * ny * nz * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d_nuArray, d->nuArray, nb * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, copy_on(c->stream, c->d_axis, axis.data(), na * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, copy_on(c->stream, c->d_mid, mid.data(), nm * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, copy_on(c->stream, c->d_dfac, dfac.data(), na * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, copy_on(c->stream, c->d_active, d->active, (size_t)nx * ny * nz * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(c, copy_on(c->stream, c->d_nuArray, d->nuArray, nb * sizeof(float), hipMemcpyHostToDevice));
 
     mcd::Grid &G = c->G;
     G.nx = nx; G.ny = ny; G.nz = nz; G.nCells = nc; G.nbins = nb; G.flags = d->flags;
@@ -328,14 +343,14 @@ extern "C" int mc_upload_tables(mc_ctx *c, const float *opacity, const float *sc
     int rc = MC_OK;
     hipError_t e = hipSuccess;
     if (opacity) {
-        e = hipMemcpy(tmp, opacity, nt * sizeof(float), hipMemcpyHostToDevice);
+        e = copy_on(c->stream, tmp, opacity, nt * sizeof(float), hipMemcpyHostToDevice);
         if (e == hipSuccess && !c->d_opac3) e = dalloc(&c->d_opac3, ncell3 * nb);
         if (e == hipSuccess) e = mcd_densify(tmp, c->d_active, (int)ncell3, nc + 1, nb, c->d_opac3, c->stream);
         c->opac_built = e == hipSuccess;
     }
     if (e == hipSuccess && dust && scaOpac) {
         e = hipStreamSynchronize(c->stream);
-        if (e == hipSuccess) e = hipMemcpy(tmp, scaOpac, nt * sizeof(float), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = copy_on(c->stream, tmp, scaOpac, nt * sizeof(float), hipMemcpyHostToDevice);
         if (e == hipSuccess && !c->d_sca3) e = dalloc(&c->d_sca3, ncell3 * nb);
         if (e == hipSuccess) e = mcd_densify(tmp, c->d_active, (int)ncell3, nc + 1, nb, c->d_sca3, c->stream);
         c->opac_sca_built = e == hipSuccess;
@@ -346,19 +361,19 @@ extern "C" int mc_upload_tables(mc_ctx *c, const float *opacity, const float *sc
         return fail(c, e == hipErrorOutOfMemory ? MC_ENOMEM : MC_ENODEV, "table upload: %s", hipGetErrorString(e));
     (void)rc;
     if (!c->d_specCDF) HIPCHK(c, dalloc(&c->d_specCDF, (size_t)nb * c->nStars));
-    HIPCHK(c, hipMemcpy(c->d_specCDF, specCDF, (size_t)nb * c->nStars * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, copy_on(c->stream, c->d_specCDF, specCDF, (size_t)nb * c->nStars * sizeof(float), hipMemcpyHostToDevice));
     if (rec_host) {
         if (!c->d_recCDF) HIPCHK(c, dalloc(&c->d_recCDF, nt));
         if (!c->d_recGuide) HIPCHK(c, dalloc(&c->d_recGuide, (size_t)(nc + 1) * (mcd::kGuide + 1)));
         if (!c->d_totalLines) HIPCHK(c, dalloc(&c->d_totalLines, (size_t)nc + 1));
-        HIPCHK(c, hipMemcpy(c->d_recCDF, rec.data(), nt * sizeof(float), hipMemcpyHostToDevice));
-        HIPCHK(c, hipMemcpy(c->d_totalLines, totalLines, (nc + 1) * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(c, copy_on(c->stream, c->d_recCDF, rec.data(), nt * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(c, copy_on(c->stream, c->d_totalLines, totalLines, (nc + 1) * sizeof(float), hipMemcpyHostToDevice));
         HIPCHK(c, mcd_guides(c->d_recCDF, nc + 1, nb, c->d_recGuide, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     if (dust && gSca) {
         if (!c->d_gSca) HIPCHK(c, dalloc(&c->d_gSca, nb));
-        HIPCHK(c, hipMemcpy(c->d_gSca, gSca, nb * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(c, copy_on(c->stream, c->d_gSca, gSca, nb * sizeof(float), hipMemcpyHostToDevice));
     }
     G.opac3 = c->d_opac3; G.sca3 = c->d_sca3; G.specCDF = c->d_specCDF;
     G.recCDF = c->d_recCDF; G.recGuide = c->d_recGuide; G.totalLines = c->d_totalLines;
@@ -385,7 +400,7 @@ extern "C" int mc_count_crossings(mc_ctx *c, int32_t enable)
     HIPCHK(c, hipSetDevice(c->device));
     if (enable && !c->d_cross) {
         HIPCHK(c, dalloc(&c->d_cross, mcd::j3_points(c->G.js)));
-        HIPCHK(c, hipMemset(c->d_cross, 0, mcd::j3_points(c->G.js) * sizeof(unsigned)));
+        HIPCHK(c, fill_on(c->stream, c->d_cross, 0, mcd::j3_points(c->G.js) * sizeof(unsigned)));
     }
     c->G.cross = enable ? c->d_cross : nullptr;
     return MC_OK;
@@ -398,7 +413,7 @@ extern "C" int mc_fetch_crossings(mc_ctx *c, uint32_t *out)
     if (!c->d_cross) return fail(c, MC_ESTATE, "crossing counts are not enabled");
     HIPCHK(c, hipSetDevice(c->device));
     std::vector<uint32_t> pts(mcd::j3_points(c->G.js));
-    HIPCHK(c, hipMemcpy(pts.data(), c->d_cross, pts.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_on(c->stream, pts.data(), c->d_cross, pts.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
     for (int cell = 0; cell <= c->G.nCells; ++cell) out[cell] = 0;
     for (int i = 0; i < c->G.ncell3; ++i) out[c->active[i]] += pts[mcd::j3_point_of(c->G.js, i)];
     return MC_OK;
@@ -436,8 +451,8 @@ extern "C" int mc_set_estimators(mc_ctx *c, const double *J, const double *esc)
         const unsigned pt = mcd::j3_point_of(c->G.js, (int)i);
         for (size_t nu = 0; nu < nb; ++nu) dense[mcd::j3_index((int)nb, (int)nu, pt)] = J[nu * nc1 + a];
     }
-    HIPCHK(c, hipMemcpy(c->d_J3, dense.data(), dense.size() * sizeof(double), hipMemcpyHostToDevice));
-    if (esc) HIPCHK(c, hipMemcpy(c->d_esc, esc, nc1 * (nb + 1) * c->esc_planes * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, copy_on(c->stream, c->d_J3, dense.data(), dense.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (esc) HIPCHK(c, copy_on(c->stream, c->d_esc, esc, nc1 * (nb + 1) * c->esc_planes * sizeof(double), hipMemcpyHostToDevice));
     return MC_OK;
 }
 
@@ -490,10 +505,10 @@ extern "C" int mc_set_view_angles(mc_ctx *c, const mc_view_angles *v)
         c->G.esc = e;
         c->esc_planes = n + 1;
     }
-    HIPCHK(c, hipMemset(c->d_esc, 0, nc1 * nb1 * c->esc_planes * sizeof(double)));
+    HIPCHK(c, fill_on(c->stream, c->d_esc, 0, nc1 * nb1 * c->esc_planes * sizeof(double)));
     if (n > 0) {
         if (!c->d_view) HIPCHK(c, hipMalloc((void **)&c->d_view, sizeof(mcd::View)));
-        HIPCHK(c, hipMemcpy(c->d_view, &h, sizeof h, hipMemcpyHostToDevice));
+        HIPCHK(c, copy_on(c->stream, c->d_view, &h, sizeof h, hipMemcpyHostToDevice));
         c->G.view = c->d_view;
     } else {
         c->G.view = nullptr;
@@ -510,7 +525,7 @@ static int cell_positions(mc_ctx *c)
     if (!c->d_cell_xyz) {
         const size_t nc1 = (size_t)c->G.nCells + 1;
         HIPCHK(c, dalloc(&c->d_cell_xyz, nc1));
-        HIPCHK(c, hipMemcpy(c->d_cell_xyz, c->cell_xyz.data(), nc1 * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(c, copy_on(c->stream, c->d_cell_xyz, c->cell_xyz.data(), nc1 * sizeof(int), hipMemcpyHostToDevice));
     }
     return MC_OK;
 }
@@ -525,7 +540,7 @@ extern "C" int mc_balance_setup(mc_ctx *c, const mc_atomic *at)
     }
     c->balance = false;
     std::vector<float> nu(c->G.nbins);
-    HIPCHK(c, hipMemcpy(nu.data(), c->d_nuArray, nu.size() * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_on(c->stream, nu.data(), c->d_nuArray, nu.size() * sizeof(float), hipMemcpyDeviceToHost));
     mcb::Prepared &P = c->bal;
     c->err[0] = 0;
     int rc = mcb::prepare(at, c->G.nbins, nu.data(), P, c->err, sizeof c->err);
@@ -543,7 +558,7 @@ extern "C" int mc_balance_setup(mc_ctx *c, const mc_atomic *at)
     for (int k = 0; k < 10; ++k) {
         ptr[k] = c->d_bal_tab + off;
         if (!parts[k]->empty())
-            HIPCHK(c, hipMemcpy(c->d_bal_tab + off, parts[k]->data(), parts[k]->size() * sizeof(double),
+            HIPCHK(c, copy_on(c->stream, c->d_bal_tab + off, parts[k]->data(), parts[k]->size() * sizeof(double),
                                 hipMemcpyHostToDevice));
         off += parts[k]->size();
     }
@@ -551,7 +566,7 @@ extern "C" int mc_balance_setup(mc_ctx *c, const mc_atomic *at)
     A.sigma = ptr[0]; A.nu = ptr[1]; A.cel_e = ptr[2]; A.cel_a = ptr[3]; A.cel_q = ptr[4];
     A.cel_y2 = ptr[5]; A.cel_t = ptr[6]; A.cel_g = ptr[7]; A.cR = ptr[8]; A.cH = ptr[9];
     if (!c->d_atomic) HIPCHK(c, dalloc(&c->d_atomic, 1));
-    HIPCHK(c, hipMemcpy(c->d_atomic, &A, sizeof A, hipMemcpyHostToDevice));
+    HIPCHK(c, copy_on(c->stream, c->d_atomic, &A, sizeof A, hipMemcpyHostToDevice));
     const size_t nc1 = (size_t)c->G.nCells + 1;
     const size_t nion = nc1 * P.A.nE * (P.A.nstages + 1);
     if (!c->d_Hden) {
@@ -713,8 +728,8 @@ extern "C" int mc_dust_setup(mc_ctx *c, const mc_dust_tables *d)
     if (pe) {
         // qa | yn | Ndust | gdc | peh
         HIPCHK(c, dalloc(&c->d_pe_tab, pet.size() + 2 * nc1 * K));
-        HIPCHK(c, hipMemcpy(c->d_pe_tab, pet.data(), pet.size() * sizeof(double), hipMemcpyHostToDevice));
-        HIPCHK(c, hipMemset(c->d_pe_tab + pet.size(), 0, 2 * nc1 * K * sizeof(double)));
+        HIPCHK(c, copy_on(c->stream, c->d_pe_tab, pet.data(), pet.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(c, fill_on(c->stream, c->d_pe_tab + pet.size(), 0, 2 * nc1 * K * sizeof(double)));
         int rc = cell_positions(c);
         if (rc) return rc;
         GH.absx = c->d_dust_tab;
@@ -728,12 +743,12 @@ extern "C" int mc_dust_setup(mc_ctx *c, const mc_dust_tables *d)
         GH.cell_xyz = c->d_cell_xyz;
         GH.nx = c->G.nx; GH.ny = c->G.ny; GH.nz = c->G.nz;
         HIPCHK(c, dalloc(&c->d_grains, 1));
-        HIPCHK(c, hipMemcpy(c->d_grains, &GH, sizeof GH, hipMemcpyHostToDevice));
+        HIPCHK(c, copy_on(c->stream, c->d_grains, &GH, sizeof GH, hipMemcpyHostToDevice));
         c->pe_gdc = GH.gdc;
         c->pe_peh = GH.peh;
     }
-    HIPCHK(c, hipMemcpy(c->d_dust_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d_dust_mono, mono.data(), K * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(c, copy_on(c->stream, c->d_dust_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, copy_on(c->stream, c->d_dust_mono, mono.data(), K * sizeof(int), hipMemcpyHostToDevice));
     if (!c->d_flags) {
         HIPCHK(c, dalloc(&c->d_flags, nc1));
         HIPCHK(c, dalloc(&c->d_nIter, nc1));
@@ -775,7 +790,7 @@ extern "C" int mc_dust_setup(mc_ctx *c, const mc_dust_tables *d)
             nd[i] = d->Ndust[i];
         }
         HIPCHK(c, dalloc(&c->d_op_dust, t.size()));
-        HIPCHK(c, hipMemcpy(c->d_op_dust, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(c, copy_on(c->stream, c->d_op_dust, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
         c->dust_opac = true;
         c->dust_rows = K;
     }
@@ -1068,10 +1083,10 @@ extern "C" int mc_opacity_setup(mc_ctx *c, const mc_opacity_tables *t)
     HIPCHK(c, dalloc(&c->d_op_i, iv.size()));
     HIPCHK(c, dalloc(&c->d_op_f, fv.size()));
     HIPCHK(c, dalloc(&c->d_op_ent, ent.size()));
-    HIPCHK(c, hipMemcpy(c->d_op_i, iv.data(), iv.size() * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d_op_f, fv.data(), fv.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, copy_on(c->stream, c->d_op_i, iv.data(), iv.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(c, copy_on(c->stream, c->d_op_f, fv.data(), fv.size() * sizeof(float), hipMemcpyHostToDevice));
     if (!ent.empty())
-        HIPCHK(c, hipMemcpy(c->d_op_ent, ent.data(), ent.size() * sizeof(mco::Ent), hipMemcpyHostToDevice));
+        HIPCHK(c, copy_on(c->stream, c->d_op_ent, ent.data(), ent.size() * sizeof(mco::Ent), hipMemcpyHostToDevice));
     const size_t nc1 = (size_t)c->G.nCells + 1;
     const int ldst = 3 + nslot;
     HIPCHK(c, dalloc(&c->d_op_st, nc1 * ldst));
@@ -1137,7 +1152,7 @@ extern "C" int mc_opacity_build(mc_ctx *c, const float *Hden, const float *Te, c
         const mcg::DustArgs &da = c->dust_args;
         const size_t ntd = nc1 * (da.nsMax + 1) * (da.nSizes + 1);
         if (!c->d_op_Td) HIPCHK(c, dalloc(&c->d_op_Td, ntd));
-        if (Tdust) HIPCHK(c, hipMemcpy(c->d_op_Td, Tdust, ntd * sizeof(float), hipMemcpyHostToDevice));
+        if (Tdust) HIPCHK(c, copy_on(c->stream, c->d_op_Td, Tdust, ntd * sizeof(float), hipMemcpyHostToDevice));
         const int K = c->dust_rows;
         o.ndrow = K; o.nSizes = da.nSizes; o.nsMax = da.nsMax;
         o.dsca = c->d_op_dust;
@@ -1318,9 +1333,9 @@ extern "C" int mc_emission_setup(mc_ctx *c, const mc_emission_tables *t)
     HIPCHK(c, dalloc(&c->d_em_i, iv.size()));
     HIPCHK(c, dalloc(&c->d_em_f, fv.size()));
     HIPCHK(c, dalloc(&c->d_em_d, dv.size()));
-    HIPCHK(c, hipMemcpy(c->d_em_i, iv.data(), iv.size() * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d_em_f, fv.data(), fv.size() * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d_em_d, dv.data(), dv.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, copy_on(c->stream, c->d_em_i, iv.data(), iv.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(c, copy_on(c->stream, c->d_em_f, fv.data(), fv.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, copy_on(c->stream, c->d_em_d, dv.data(), dv.size() * sizeof(double), hipMemcpyHostToDevice));
     mce::EmArgs &a = c->em;
     a = mce::EmArgs{};
     a.nb = nb; a.nc = c->G.nCells;
@@ -1392,7 +1407,7 @@ extern "C" int mc_emission_build(mc_ctx *c, const float *Hden, const float *Te, 
             }
         }
         if (!c->d_em_st) HIPCHK(c, dalloc(&c->d_em_st, st.size()));
-        HIPCHK(c, hipMemcpy(c->d_em_st, st.data(), st.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(c, copy_on(c->stream, c->d_em_st, st.data(), st.size() * sizeof(double), hipMemcpyHostToDevice));
         a.Te = c->d_em_st; a.Ne = c->d_em_st + nc1; a.Hden = c->d_em_st + 2 * nc1; a.ion = c->d_em_st + 3 * nc1;
     } else {
         a.Te = c->d_Te; a.Ne = c->d_Ne; a.Hden = c->d_Hden; a.ion = c->d_ion;
@@ -1411,7 +1426,7 @@ extern "C" int mc_emission_build(mc_ctx *c, const float *Hden, const float *Te, 
         if (Tdust) {
             std::vector<double> td(ntd);
             for (size_t i = 0; i < ntd; ++i) td[i] = Tdust[i];
-            HIPCHK(c, hipMemcpy(c->d_em_Td, td.data(), ntd * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(c, copy_on(c->stream, c->d_em_Td, td.data(), ntd * sizeof(double), hipMemcpyHostToDevice));
             a.Tdust = c->d_em_Td;
         } else {
             a.Tdust = c->d_Tdust;
@@ -1421,12 +1436,12 @@ extern "C" int mc_emission_build(mc_ctx *c, const float *Hden, const float *Te, 
     // samples change only when every row is good
     if (!c->d_recCDF_b) {
         HIPCHK(c, dalloc(&c->d_recCDF_b, nt));
-        HIPCHK(c, hipMemset(c->d_recCDF_b, 0, nt * sizeof(float)));
+        HIPCHK(c, fill_on(c->stream, c->d_recCDF_b, 0, nt * sizeof(float)));
     }
     if (!c->d_recGuide_b) HIPCHK(c, dalloc(&c->d_recGuide_b, nc1 * (mcd::kGuide + 1)));
     if (!c->d_totalLines_b) {
         HIPCHK(c, dalloc(&c->d_totalLines_b, nc1));
-        HIPCHK(c, hipMemset(c->d_totalLines_b, 0, nc1 * sizeof(float)));
+        HIPCHK(c, fill_on(c->stream, c->d_totalLines_b, 0, nc1 * sizeof(float)));
     }
     if (!c->d_em_err) HIPCHK(c, dalloc(&c->d_em_err, nc1));
     a.rec = c->d_recCDF_b; a.totalLines = c->d_totalLines_b; a.err = c->d_em_err;
@@ -1453,13 +1468,13 @@ extern "C" int mc_emission_build(mc_ctx *c, const float *Hden, const float *Te, 
     }
     if (recPDF) {
         std::vector<float> rec(nt);
-        HIPCHK(c, hipMemcpy(rec.data(), c->d_recCDF, nt * sizeof(float), hipMemcpyDeviceToHost));
+        HIPCHK(c, copy_on(c->stream, rec.data(), c->d_recCDF, nt * sizeof(float), hipMemcpyDeviceToHost));
         for (size_t cell = 1; cell < nc1; ++cell)
             for (int i = 0; i < nb; ++i) recPDF[cell + nc1 * i] = rec[cell * nb + i];
     }
     if (totalLines) {
         std::vector<float> tl(nc1);
-        HIPCHK(c, hipMemcpy(tl.data(), c->d_totalLines, nc1 * sizeof(float), hipMemcpyDeviceToHost));
+        HIPCHK(c, copy_on(c->stream, tl.data(), c->d_totalLines, nc1 * sizeof(float), hipMemcpyDeviceToHost));
         for (size_t cell = 1; cell < nc1; ++cell) totalLines[cell] = tl[cell];
     }
     return MC_OK;
@@ -1583,9 +1598,9 @@ extern "C" int mc_mlp_set(mc_ctx *c, const float *params, const float *scale)
         HIPCHK(c, dalloc(&c->d_mlp, 4 * P + MC_MLP_NSCALE));
         HIPCHK(c, dalloc(&c->d_mlpW, P));
     }
-    HIPCHK(c, hipMemcpy(c->d_mlp, params, P * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemset(c->d_mlp + P, 0, 3 * P * sizeof(float)));
-    HIPCHK(c, hipMemcpy(c->d_mlp + 4 * P, scale, MC_MLP_NSCALE * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, copy_on(c->stream, c->d_mlp, params, P * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, fill_on(c->stream, c->d_mlp + P, 0, 3 * P * sizeof(float)));
+    HIPCHK(c, copy_on(c->stream, c->d_mlp + 4 * P, scale, MC_MLP_NSCALE * sizeof(float), hipMemcpyHostToDevice));
     HIPCHK(c, mcm_publish(c->d_mlp, c->d_mlpW, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->mlp = true;
@@ -1598,9 +1613,9 @@ extern "C" int mc_mlp_get(mc_ctx *c, float *params, float *scale)
     if (!c) return MC_EINVAL;
     if (!c->mlp) return fail(c, MC_ESTATE, "mc_mlp_set has not been called");
     HIPCHK(c, hipSetDevice(c->device));
-    if (params) HIPCHK(c, hipMemcpy(params, c->d_mlp, mcm::kParams * sizeof(float), hipMemcpyDeviceToHost));
+    if (params) HIPCHK(c, copy_on(c->stream, params, c->d_mlp, mcm::kParams * sizeof(float), hipMemcpyDeviceToHost));
     if (scale)
-        HIPCHK(c, hipMemcpy(scale, c->d_mlp + 4 * (size_t)mcm::kParams, MC_MLP_NSCALE * sizeof(float),
+        HIPCHK(c, copy_on(c->stream, scale, c->d_mlp + 4 * (size_t)mcm::kParams, MC_MLP_NSCALE * sizeof(float),
                             hipMemcpyDeviceToHost));
     return MC_OK;
 }
@@ -2004,9 +2019,9 @@ extern "C" int mc_transport_batch(mc_ctx *const *ctx, int32_t n, const int32_t *
         unsigned long long h[9];
         int herr = 0;
         double q = 0.0;
-        HIPCHK(c0, hipMemcpy(h, c->d_scratch, sizeof h, hipMemcpyDeviceToHost));
-        HIPCHK(c0, hipMemcpy(&herr, c->d_error, sizeof herr, hipMemcpyDeviceToHost));
-        HIPCHK(c0, hipMemcpy(&q, c->d_qphot, sizeof q, hipMemcpyDeviceToHost));
+        HIPCHK(c0, copy_on(c0->stream, h, c->d_scratch, sizeof h, hipMemcpyDeviceToHost));
+        HIPCHK(c0, copy_on(c0->stream, &herr, c->d_error, sizeof herr, hipMemcpyDeviceToHost));
+        HIPCHK(c0, copy_on(c0->stream, &q, c->d_qphot, sizeof q, hipMemcpyDeviceToHost));
         if (cnt) {
             mc_counters &k = cnt[i];
             k.n_seg = (double)h[1]; k.n_abs = (double)h[2]; k.n_sca = (double)h[3];
@@ -2224,18 +2239,18 @@ extern "C" int mc_walk_replay(mc_ctx *c, int32_t n, const float *pos, const floa
         (e = dalloc(&dc, 3 * (size_t)n)) == hipSuccess && (e = dalloc(&dseq, 3 * tot)) == hipSuccess &&
         (e = dalloc(&dds, tot)) == hipSuccess && (e = dalloc(&dlen, (size_t)n)) == hipSuccess &&
         (e = dalloc(&dst, (size_t)n)) == hipSuccess) {
-        e = hipMemcpy(dp, pos, 3 * n * sizeof(float), hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(dd, dir, 3 * n * sizeof(float), hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(dc, cell, 3 * n * sizeof(int), hipMemcpyHostToDevice);
+        e = copy_on(c->stream, dp, pos, 3 * n * sizeof(float), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = copy_on(c->stream, dd, dir, 3 * n * sizeof(float), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = copy_on(c->stream, dc, cell, 3 * n * sizeof(int), hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemsetAsync(dseq, 0, 3 * tot * sizeof(int), c->stream);
         if (e == hipSuccess) e = hipMemsetAsync(dds, 0, tot * sizeof(float), c->stream);
         if (e == hipSuccess)
             e = mcd_launch_replay(c->G, n, dp, dd, dc, dseq, dds, dlen, dst, max_len, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-        if (e == hipSuccess) e = hipMemcpy(cell_seq, dseq, 3 * tot * sizeof(int), hipMemcpyDeviceToHost);
-        if (e == hipSuccess) e = hipMemcpy(ds_seq, dds, tot * sizeof(float), hipMemcpyDeviceToHost);
-        if (e == hipSuccess) e = hipMemcpy(seq_len, dlen, n * sizeof(int), hipMemcpyDeviceToHost);
-        if (e == hipSuccess) e = hipMemcpy(status, dst, n * sizeof(int), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = copy_on(c->stream, cell_seq, dseq, 3 * tot * sizeof(int), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = copy_on(c->stream, ds_seq, dds, tot * sizeof(float), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = copy_on(c->stream, seq_len, dlen, n * sizeof(int), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = copy_on(c->stream, status, dst, n * sizeof(int), hipMemcpyDeviceToHost);
     }
     if (e != hipSuccess) rc = fail(c, e == hipErrorOutOfMemory ? MC_ENOMEM : MC_ENODEV, "replay: %s", hipGetErrorString(e));
     (void)hipFree(dp); (void)hipFree(dd); (void)hipFree(dc); (void)hipFree(dseq);

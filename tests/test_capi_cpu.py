@@ -78,3 +78,21 @@ def test_calls_on_null_context_fail_cleanly():
     assert L.mc_ctx_shape(None, C.byref(nc), C.byref(nb)) == 1 and nc.value == nb.value == 0
     assert L.mc_esc_planes(None) == 0
     L.mc_ctx_destroy(None)
+
+
+def test_copies_and_fills_stay_on_the_context_stream():
+    """A context works on a non-blocking stream, which a blocking hipMemcpy
+    or hipMemset on the null stream is not ordered with: a fill of a fresh
+    re-emission buffer once landed after the kernel that wrote it (seen only
+    when the profiler serialised the queues).  The library's copies and
+    fills therefore all name a stream."""
+    csrc = os.path.join(ROOT, "mocassin_amd", "csrc")
+    bad = []
+    for name in sorted(os.listdir(csrc)):
+        if not name.endswith((".hip", ".h")):
+            continue
+        for i, line in enumerate(open(os.path.join(csrc, name)), 1):
+            code = line.split("//")[0]
+            if re.search(r"\bhipMem(cpy|set)(2D)?\s*\(", code):
+                bad.append("%s:%d: %s" % (name, i, line.strip()))
+    assert not bad, "\n".join(bad)
