@@ -49,19 +49,27 @@ __host__ __device__ inline size_t lds_floats(int nx, int ny, int nz, int nbins)
 }
 
 // ------------------------------------------------ J update queue (per wave)
-// Each lane keeps its last kJD updates (J3 point, bin - 1, value) in a ring
-// of its own at the start of dynamic LDS: point u32 [kJQ] | value f32 [kJQ] |
-// bin u16 [kJQ], entry k of lane L at L * kJRow + k (rows padded by one word
-// against bank conflicts).  When a lane's ring is full the wavefront flushes
-// every ring, lane-major: atomic instruction j carries the rings of lanes
-// 8j .. 8j + 7.  Adds of one instruction that fall in one 64-byte line travel
-// to memory as one request, and with J3's 2x2x2 bricks a packet's next
-// segment stays in the line after half of its steps, so keeping a packet's
-// updates in one instruction turns those pairs into single requests.
-constexpr int kJD = 8;                                 // updates per lane
-constexpr int kJRow = kJD + 1;
-constexpr int kJQ = 64 * kJRow;
-constexpr int kJQFloats = kJQ * 2 + kJQ / 2;           // 5760 bytes per wavefront
+// Each lane keeps its last D updates (J3 point, bin - 1, value) in a ring of
+// its own at the start of the wavefront's dynamic LDS: point u32 [64 x
+// (D + 1)] | value f32 [...] | bin u16 [...], entry k of lane L at
+// L * (D + 1) + k (rows padded by one word against bank conflicts).  When a
+// lane's ring is full the wavefront flushes every ring, lane-major: atomic
+// instruction j carries the rings of lanes (64 / D) j .. (64 / D)(j + 1) - 1.
+// Adds of one instruction that fall in one 64-byte line travel to memory as
+// one request, and with J3's 2x2x2 bricks a packet's next segment stays in
+// the line after half of its steps, so keeping a packet's updates in one
+// instruction turns those pairs into single requests.  The single-model
+// kernel keeps 8 per lane; the batched kernel, whose one-wavefront
+// workgroups each hold a model's tables too, keeps 4 so that 16 fit a CU.
+template <int D>
+struct JRing {
+    static_assert(D == 4 || D == 8, "rings of 4 or 8 updates");
+    static constexpr int kRow = D + 1;
+    static constexpr int kEntries = 64 * kRow;
+    static constexpr int kFloats = kEntries * 2 + kEntries / 2;
+};
+constexpr int kJDepth = 8;                            // single-model kernel
+constexpr int kJDepthBatch = 4;                       // batched kernel
 
 struct JQ {
     unsigned *pt;
@@ -70,31 +78,33 @@ struct JQ {
     int n;                // this lane's queued updates
 };
 
+template <int D>
 __device__ __forceinline__ JQ jq_at(float *lds, int wave)
 {
-    float *b = lds + (size_t)wave * kJQFloats;
+    float *b = lds + (size_t)wave * JRing<D>::kFloats;
     JQ Q;
     Q.pt = reinterpret_cast<unsigned *>(b);
-    Q.val = b + kJQ;
-    Q.nu = reinterpret_cast<unsigned short *>(b + 2 * kJQ);
+    Q.val = b + JRing<D>::kEntries;
+    Q.nu = reinterpret_cast<unsigned short *>(b + 2 * JRing<D>::kEntries);
     Q.n = 0;
     return Q;
 }
 
-// every queued update: instruction j takes entry (lane % 8) of lane
-// 8j + lane / 8's ring
+// every queued update: instruction j takes entry lane % D of lane
+// (64 / D) j + lane / D's ring
+template <int D>
 __device__ __forceinline__ void jq_flush(const Grid &G, JQ &Q, int lane)
 {
-    static_assert(kJD == 8, "eight rings of eight entries per instruction");
+    constexpr int per = 64 / D;                       // rings per instruction
     __builtin_amdgcn_wave_barrier();
-    const int k = lane & 7;
-    for (int j = 0; j < 8; ++j) {
-        const int owner = 8 * j + (lane >> 3);
+    const int k = lane % D;
+    for (int j = 0; j < D; ++j) {
+        const int owner = per * j + lane / D;
         const int cnt = __shfl(Q.n, owner);
         const bool act = k < cnt;
         if (!__any(act)) continue;
         if (act) {
-            const int slot = owner * kJRow + k;
+            const int slot = owner * JRing<D>::kRow + k;
             const unsigned pt = Q.pt[slot];
             unsafeAtomicAdd(&G.J3[j3_index(G.nbins, (int)Q.nu[slot], pt)], (double)Q.val[slot]);
             if (G.cross) atomicAdd(&G.cross[pt], 1u);
@@ -106,16 +116,17 @@ __device__ __forceinline__ void jq_flush(const Grid &G, JQ &Q, int lane)
 
 // the J update of each lane with `has` (path length times deltaE over dV,
 // photon_mod.f90:1563-1574, 1822-1833), flushed when a lane's ring is full
+template <int D>
 __device__ __forceinline__ void jq_push(const Grid &G, JQ &Q, bool has, unsigned pt, int nuP, float v, int lane)
 {
     if (has) {
-        const int slot = lane * kJRow + Q.n;
+        const int slot = lane * JRing<D>::kRow + Q.n;
         Q.pt[slot] = pt;
         Q.val[slot] = v;
         Q.nu[slot] = (unsigned short)(nuP - 1);
         Q.n += 1;
     }
-    if (__any(Q.n == kJD)) jq_flush(G, Q, lane);
+    if (__any(Q.n == D)) jq_flush<D>(G, Q, lane);
 }
 
 // ------------------------------------------------------------ replay kernel
@@ -305,6 +316,7 @@ __device__ __forceinline__ void stage_model(const Grid &G, const RunArgs &RA, fl
 // One scheduling round of a wavefront: an emission pass when at least
 // emit_min lanes (or as many as are walking) wait for one, then walk_steps
 // walk passes.  Counts events into C; a fail-stop code goes to RA.error.
+template <int D>
 __device__ __forceinline__ void advance(Lane &L, const Grid &G, const Axes &A, const RunArgs &RA,
                                         const Model &M, const float *spec,
                                         const unsigned short *spec_guide, int emit_min, int walk_steps,
@@ -405,7 +417,7 @@ __device__ __forceinline__ void advance(Lane &L, const Grid &G, const Axes &A, c
                 L.phase = PH_IDLE;
             }
         }
-        jq_push(G, Q, seg, jpt, jnu, jv, lane);
+        jq_push<D>(G, Q, seg, jpt, jnu, jv, lane);
         C.seg += wcount(seg);
         C.esc += wcount(escaped);
         C.lost += wcount(lost);
@@ -445,9 +457,9 @@ __global__ void __launch_bounds__(256) transport_kernel(Grid G, RunArgs RA)
     float *spec = nullptr;
     unsigned short *spec_guide = nullptr;
     const int nwaves = (int)(blockDim.x >> 6);
-    stage_model(G, RA, lds + (size_t)nwaves * kJQFloats, A, spec, spec_guide);
+    stage_model(G, RA, lds + (size_t)nwaves * JRing<kJDepth>::kFloats, A, spec, spec_guide);
     const int lane_id = threadIdx.x & 63;
-    JQ Q = jq_at(lds, (int)(threadIdx.x >> 6));
+    JQ Q = jq_at<kJDepth>(lds, (int)(threadIdx.x >> 6));
     const Model M = model_of(G, RA);
 
     Lane L;
@@ -483,10 +495,10 @@ __global__ void __launch_bounds__(256) transport_kernel(Grid G, RunArgs RA)
         const int npend = (int)wcount(pend);
         const int nwalk = (int)wcount(L.phase == PH_WALK);
         if (npend + nwalk == 0) break;
-        advance(L, G, A, RA, M, spec, spec_guide, RA.emit_min, RA.walk_steps, npend, nwalk, C, qphot, err, Q,
+        advance<kJDepth>(L, G, A, RA, M, spec, spec_guide, RA.emit_min, RA.walk_steps, npend, nwalk, C, qphot, err, Q,
                 lane_id);
     }
-    jq_flush(G, Q, lane_id);
+    jq_flush<kJDepth>(G, Q, lane_id);
     flush_counts(RA, C, qphot);
 }
 
@@ -502,8 +514,8 @@ __global__ void __launch_bounds__(64) batch_kernel(BatchArgs B)
 {
     extern __shared__ float lds[];
     const int lane_id = threadIdx.x;
-    JQ Q = jq_at(lds, 0);
-    float *tab = lds + kJQFloats;
+    JQ Q = jq_at<kJDepthBatch>(lds, 0);
+    float *tab = lds + JRing<kJDepthBatch>::kFloats;
     Axes A;
     float *spec = nullptr;
     unsigned short *spec_guide = nullptr;
@@ -557,7 +569,7 @@ __global__ void __launch_bounds__(64) batch_kernel(BatchArgs B)
             if (pending >= 0) {
                 // all lanes idle: switch to the pending chunk's model
                 if (cur >= 0) {
-                    jq_flush(*Gp, Q, lane_id);
+                    jq_flush<kJDepthBatch>(*Gp, Q, lane_id);
                     flush_counts(*RAp, C, qphot);
                 }
                 cur = __builtin_amdgcn_readfirstlane(B.chunk_model[pending]);
@@ -574,11 +586,11 @@ __global__ void __launch_bounds__(64) batch_kernel(BatchArgs B)
             if (done) break;
             continue;
         }
-        advance(L, *Gp, A, *RAp, M, spec, spec_guide, B.emit_min, B.walk_steps, npend, nwalk, C, qphot, err, Q,
+        advance<kJDepthBatch>(L, *Gp, A, *RAp, M, spec, spec_guide, B.emit_min, B.walk_steps, npend, nwalk, C, qphot, err, Q,
                 lane_id);
     }
     if (cur >= 0) {
-        jq_flush(*Gp, Q, lane_id);
+        jq_flush<kJDepthBatch>(*Gp, Q, lane_id);
         flush_counts(*RAp, C, qphot);
     }
 }
@@ -635,13 +647,14 @@ hipError_t mcd_launch_batch(const mcd::BatchArgs &B, size_t lds_bytes, int block
 // dynamic LDS of a batch wavefront: its J queue, then the model's tables
 size_t mcd_lds_bytes(int nx, int ny, int nz, int nbins)
 {
-    return sizeof(float) * (mcd::kJQFloats + mcd::lds_floats(nx, ny, nz, nbins));
+    return sizeof(float) * (mcd::JRing<mcd::kJDepthBatch>::kFloats + mcd::lds_floats(nx, ny, nz, nbins));
 }
 
 hipError_t mcd_launch_transport(const mcd::Grid &G, const mcd::RunArgs &RA, int blocks,
                                 int threads, hipStream_t s)
 {
-    size_t lds = sizeof(float) * ((size_t)(threads / 64) * mcd::kJQFloats + mcd::lds_floats(G.nx, G.ny, G.nz, G.nbins));
+    size_t lds = sizeof(float) * ((size_t)(threads / 64) * mcd::JRing<mcd::kJDepth>::kFloats +
+                                  mcd::lds_floats(G.nx, G.ny, G.nz, G.nbins));
     hipLaunchKernelGGL(mcd::transport_kernel, dim3(blocks), dim3(threads), lds, s, G, RA);
     return hipGetLastError();
 }
