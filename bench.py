@@ -42,6 +42,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6                 # MI355X dense FP64 vector
+VALU_CLOCK_HZ = 2.4e9                   # MI355X peak engine clock
 SEED = 20241202
 # packets/s of the compiled reference (MOCASSIN 2.02, amdflang -O2 + MPICH,
 # `mcref_harness sweep`, 8 ranks) over the C restatement's (oracle, FP64 J,
@@ -357,6 +358,7 @@ def main():
     # packet of this workload times this launch, when the profile's launch
     # shape is the one measured now
     traffic, traffic_note = None, "no calibrated counter profile of this workload"
+    valu = None
     tp = os.path.join(ROOT, "profiles", "transport_c2_traffic.json")
     if os.path.exists(tp):
         try:
@@ -365,6 +367,15 @@ def main():
             if tj.get("tuning") == shape and tj.get("tables") == "iterated":
                 traffic = tj["hbm_bytes_per_packet"] * npk
                 traffic_note = "profiles/transport_c2_traffic.json (%s)" % tj.get("tuning")
+                # the walk's instruction issue: wave64 VALU instructions of
+                # one launch (rocprofv3 SQ_INSTS_VALU) over this run's kernel
+                # time, against 4 SIMDs per CU issuing one wave64 FP32
+                # instruction every 4 cycles at the 2.4 GHz peak clock
+                insts = tj["sq_insts_valu_per_packet"] * npk
+                peak = 4 * torch.cuda.get_device_properties(0).multi_processor_count * VALU_CLOCK_HZ / 4
+                valu = {"achieved": insts / (kms * 1e-3), "peak": peak, "unit": "wave64 VALU instructions/s",
+                        "frac": insts / (kms * 1e-3) / peak, "instructions": insts,
+                        "source": "profiles/transport_c2_traffic.json (SQ_INSTS_VALU, %s)" % tj.get("tuning")}
             else:
                 traffic_note = "profile of another launch shape or tables (%s, %s): not used" % (
                     tj.get("tuning"), tj.get("tables"))
@@ -390,10 +401,12 @@ def main():
                      "algorithmic_bytes": byts, "kernel": "transport_kernel", "kernel_ms": kms,
                      "segments": c0["n_seg"], "absorptions": c0["n_abs"], "escapes": c0["n_esc"]},
     }
+    if valu is not None:
+        out["roofline"]["valu"] = valu
 
     def atomic_leg():
         from mocassin_amd.transport import probe_atomics
-        words = int(math.ceil(T.shape[0] * T.shape[1] * T.shape[2] / 64.0) * 64 * p["nbins"])
+        words = int(T.shape[0] * T.shape[1] * T.shape[2] * p["nbins"])
         req = int(c0["n_seg"])
         ms64, rate64 = probe_atomics(words, req, lanes=64)
         ms16, rate16 = probe_atomics(words, req, lanes=16)
@@ -402,7 +415,10 @@ def main():
                 "probe": "%d FP64 atomic adds to random words of a %d-word array (the J3 footprint), "
                          "64 active lanes: %.2f ms; 16 active lanes: %.2f ms (%.3g/s)" % (req, words, ms64, ms16,
                                                                                       rate16),
-                "probe_16_lanes": rate16}
+                "probe_16_lanes": rate16,
+                "note": "adds of one instruction that fall in one 64-byte line go to memory as one request "
+                        "(mc_probe_atomic_lines), and the J rings keep a packet's updates in one instruction, "
+                        "so merged adds run above the random-word rate"}
     if not a.no_atomics:
         leg(out["roofline"], "atomic", atomic_leg)
 
@@ -553,7 +569,7 @@ def main():
             if a.no_atomics:
                 return None
             from mocassin_amd.transport import probe_atomics
-            words = int(math.ceil(T3.shape[0] * T3.shape[1] * T3.shape[2] / 64.0) * 64 * p3["nbins"])
+            words = int(T3.shape[0] * T3.shape[1] * T3.shape[2] * p3["nbins"])
             ms64, rate64 = probe_atomics(words, int(c["n_seg"]), lanes=64)
             ach = c["n_seg"] / (km * 1e-3)
             return {"achieved": ach, "peak": rate64, "unit": "atomics/s", "frac": ach / rate64,
@@ -654,7 +670,7 @@ def main():
                                    "mean_iterations": float(r[4][1:][hit].mean())}}
                 if not a.no_atomics:
                     from mocassin_amd.transport import probe_atomics
-                    words = int(math.ceil(T1.shape[0] * T1.shape[1] * T1.shape[2] / 64.0) * 64 * p1["nbins"])
+                    words = int(T1.shape[0] * T1.shape[1] * T1.shape[2] * p1["nbins"])
                     _, rate = probe_atomics(words, int(c["n_seg"]), lanes=64)
                     res["atomic"] = {"achieved": res["J_atomics_per_s"], "peak": rate, "frac": res["J_atomics_per_s"] / rate,
                                      "probe": "random FP64 adds into this J3's %d words (Infinity Cache resident)" % words}

@@ -616,6 +616,16 @@ int mc_get_tuning(const mc_ctx *ctx, mc_tuning *t);
 int mc_probe_atomics(int32_t device, int64_t words, int64_t requests, int32_t lanes, int32_t fp64,
                      double *kernel_ms);
 
+/* The same probe with the adds of each atomic instruction falling in
+ * 64-byte lines (no counterpart in the reference): `requests` FP64 adds
+ * (a multiple of 64) into an array of `words` (>= 8) doubles; the 64 lanes
+ * form 64 / group sets (group 1, 2, 4 or 8) that add to distinct words of
+ * one random line each, the set's lanes adjacent (spread 0) or spread
+ * across the wavefront (spread 1).  Measures whether the memory side takes
+ * a line's adds as one request.  kernel_ms of the timed launch. */
+int mc_probe_atomic_lines(int32_t device, int64_t words, int64_t requests, int32_t group, int32_t spread,
+                          double *kernel_ms);
+
 /* Message of the last failure on this context (never NULL). */
 const char *mc_ctx_last_error(const mc_ctx *ctx);
 

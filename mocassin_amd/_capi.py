@@ -24,7 +24,8 @@ EXPORTS = ("mc_ctx_create", "mc_upload_tables", "mc_zero_estimators", "mc_transp
            "mc_transport_batch_noalloc",
            "mc_dust_setup", "mc_update_cells", "mc_opacity_setup", "mc_opacity_build",
            "mc_line_luminosities", "mc_emission_setup", "mc_emission_build", "mc_write_grid", "mc_read_grid",
-           "mc_set_view_angles", "mc_esc_planes", "mc_ctx_shape", "mc_probe_atomics", "mc_ctx_last_error", "mc_ctx_destroy")
+           "mc_set_view_angles", "mc_esc_planes", "mc_ctx_shape", "mc_probe_atomics", "mc_probe_atomic_lines",
+           "mc_ctx_last_error", "mc_ctx_destroy")
 
 MC_SOLVER_REFERENCE, MC_SOLVER_SECANT = 0, 1
 MC_MLP_NIN, MC_MLP_NOUT, MC_MLP_NPARAMS, MC_MLP_NSCALE = 72, 31, 49568, 206
@@ -175,6 +176,7 @@ def load():
     L.mc_ctx_shape.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.mc_fetch_crossings.argtypes = [vp, vp]
     L.mc_probe_atomics.argtypes = [i32, C.c_int64, C.c_int64, i32, i32, C.POINTER(C.c_double)]
+    L.mc_probe_atomic_lines.argtypes = [i32, C.c_int64, C.c_int64, i32, i32, C.POINTER(C.c_double)]
     L.mc_ctx_last_error.argtypes = [vp]
     L.mc_ctx_last_error.restype = C.c_char_p
     L.mc_ctx_destroy.argtypes = [vp]

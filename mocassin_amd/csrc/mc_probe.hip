@@ -38,7 +38,68 @@ __global__ void __launch_bounds__(256) atomics_kernel(void *buf, uint32_t words,
     }
 }
 
+// FP64 adds where the lanes of each atomic instruction form 64 / group
+// sets of `group` lanes, each set adding to distinct words of one random
+// 64-byte line: adjacent lanes (lane / group) or lanes spread across the
+// wavefront (lane % (64 / group)).  A request carries the adds of one line.
+__global__ void __launch_bounds__(256) line_kernel(double *buf, uint32_t lines, uint64_t requests, int group,
+                                                   int spread, uint64_t seed)
+{
+    const int lane = threadIdx.x & 63;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const int sets = 64 / group;
+    const int g = spread ? lane % sets : lane / group;      // the lane's line in this instruction
+    const int w = spread ? lane / sets : lane % group;      // its word in the line, < group <= 8
+    for (uint64_t r = wave * 64; r < requests; r += nwaves * 64) {
+        const uint64_t h = mix64((r + (uint64_t)g) ^ seed);
+        const uint32_t line = (uint32_t)(((h >> 32) * (uint64_t)lines) >> 32);     // < lines
+        unsafeAtomicAdd(buf + (size_t)line * 8 + w, 1.0);
+    }
+}
+
 }  // namespace
+
+extern "C" int mc_probe_atomic_lines(int32_t device, int64_t words, int64_t requests, int32_t group,
+                                     int32_t spread, double *kernel_ms)
+{
+    if (words < 8 || words / 8 > 0xffffffffLL || requests < 64 || !kernel_ms ||
+        !(group == 1 || group == 2 || group == 4 || group == 8) || (spread != 0 && spread != 1))
+        return MC_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return MC_ENODEV;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1)
+        return MC_ENODEV;
+    const uint32_t lines = (uint32_t)(words / 8);
+    double *buf = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = MC_OK;
+    if (hipMalloc(&buf, (size_t)lines * 64) != hipSuccess) return MC_ENOMEM;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess || hipMemsetAsync(buf, 0, (size_t)lines * 64, s) != hipSuccess) {
+        rc = MC_ENODEV;
+    } else {
+        const dim3 grid((unsigned)cus * 8), block(256);
+        float ms = 0.f;
+        for (int k = 0; k < 2 && rc == MC_OK; ++k) {
+            if (k == 1 && hipEventRecord(e0, s) != hipSuccess) rc = MC_ENODEV;
+            hipLaunchKernelGGL(line_kernel, grid, block, 0, s, buf, lines, (uint64_t)requests, group, spread,
+                               (uint64_t)(k + 1));
+            if (hipGetLastError() != hipSuccess) rc = MC_ENODEV;
+        }
+        if (rc == MC_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = MC_ENODEV;
+        *kernel_ms = ms;
+    }
+    if (s) (void)hipStreamSynchronize(s);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    (void)hipFree(buf);
+    return rc;
+}
 
 extern "C" int mc_probe_atomics(int32_t device, int64_t words, int64_t requests, int32_t lanes, int32_t fp64,
                                 double *kernel_ms)

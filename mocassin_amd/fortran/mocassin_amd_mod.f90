@@ -180,6 +180,15 @@ module mocassin_amd_mod
             real(c_double), intent(out) :: kernel_ms
         end function mc_probe_atomics
 
+        integer(c_int) function mc_probe_atomic_lines(device, words, requests, group, spread, kernel_ms) &
+                bind(C, name="mc_probe_atomic_lines")
+            import :: c_int, c_int32_t, c_int64_t, c_double
+            integer(c_int32_t), value :: device
+            integer(c_int64_t), value :: words, requests
+            integer(c_int32_t), value :: group, spread
+            real(c_double), intent(out) :: kernel_ms
+        end function mc_probe_atomic_lines
+
         integer(c_int) function mc_walk_replay(ctx, n, pos, dir, cell, cell_seq, ds_seq, &
                                                seq_len, status, max_len) bind(C, name="mc_walk_replay")
             import :: c_int, c_int32_t, c_ptr

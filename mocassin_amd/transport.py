@@ -544,3 +544,18 @@ def probe_atomics(words, requests, lanes=64, fp64=True, device=0):
     if rc:
         raise _capi.McError(rc, "mc_probe_atomics")
     return ms.value, requests / (ms.value * 1e-3)
+
+
+def probe_atomic_lines(words, requests, group, spread=False, device=0):
+    """FP64 atomic adds per second when each instruction's lanes form sets
+    of `group` adding to one 64-byte line (mc_probe_atomic_lines), the sets'
+    lanes adjacent or spread across the wavefront.  Returns (kernel_ms, adds
+    per second, line requests per second)."""
+    L = _capi.load()
+    ms = C.c_double()
+    rc = L.mc_probe_atomic_lines(int(device), int(words), int(requests), int(group), 1 if spread else 0,
+                                 C.byref(ms))
+    if rc:
+        raise _capi.McError(rc, "mc_probe_atomic_lines")
+    rate = requests / (ms.value * 1e-3)
+    return ms.value, rate, rate / group
