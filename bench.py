@@ -656,7 +656,7 @@ def main():
                 km_default, _ = timed3(0)
                 # a launch this short is shaped by its ramp and tail: the
                 # autotuner picks the shape for this workload
-                best, trials = at.autotune(T1, n1, grid={"blocks_per_cu": (1, 2, 4, 8), "threads": (64, 128, 256)},
+                best, trials = at.autotune(T1, n1, grid={"blocks_per_cu": (1, 2, 3, 4, 8), "threads": (64, 128, 256)},
                                            packet_offset=1 << 40)
                 km, c = timed3(4 * n1)
                 r = T1.update_cells(Hd1, Te1, Ne1, ion1, jscale=jscale_of(p1["symmetric"]))

@@ -91,7 +91,11 @@ typedef struct {
 
 /* Launch shape of the transport kernel (chosen by the autotuner,
  * mocassin_amd/autotune.py; no counterpart in the reference):
- *   blocks_per_cu  resident workgroups per compute unit requested (1..32)
+ *   blocks_per_cu  resident workgroups per compute unit requested (1..32),
+ *                  counted in 256-thread units (wavefronts per SIMD); at
+ *                  most 3 runs the kernel with two packets per lane (a lane
+ *                  whose flight ends walks its spare packet while the next
+ *                  emission pass waits), more runs one packet per lane
  *   threads        workgroup size, a multiple of 64 (64..256)
  *   emit_min       an emission pass runs when at least this many lanes of a
  *                  wavefront wait for one (1..64), or half the live lanes do

@@ -3,7 +3,8 @@
 The transport kernel has four free parameters (mc_tuning in
 include/mocassin_amd.h): resident workgroups per CU, workgroup size, the
 number of waiting lanes that triggers a batched emission pass, and the
-number of walk passes between emission checks.  The batched kernel
+number of walk passes between emission checks.  At most 3 workgroups of 256
+per CU select the kernel with two packets per lane.  The batched kernel
 (mc_batch_tuning) has resident wavefronts per CU, the same two scheduling
 parameters, and the packet-batch size handed to a wavefront at a time.
 Results do not depend on any of them (every packet draws from its own
@@ -17,7 +18,7 @@ import os
 DB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_db.json")
 
 DEFAULT_GRID = {
-    "blocks_per_cu": (2, 4, 8),
+    "blocks_per_cu": (2, 3, 4, 8),
     "threads": (256,),
     "emit_min": (32, 48, 64),
     "walk_steps": (8, 12, 16, 20, 24, 32),

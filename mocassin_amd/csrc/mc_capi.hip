@@ -1869,7 +1869,8 @@ extern "C" int mc_transport_run(mc_ctx *c, int32_t istar, int64_t npackets, uint
     RA.walk_steps = c->tune.walk_steps;
     const int blocks = c->tune.blocks_per_cu * c->cus * (256 / c->tune.threads);
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    if (npackets > 0) HIPCHK(c, mcd_launch_transport(c->G, RA, blocks, c->tune.threads, c->stream));
+    if (npackets > 0)
+        HIPCHK(c, mcd_launch_transport(c->G, RA, blocks, c->tune.threads, c->tune.blocks_per_cu <= 3, c->stream));
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     unsigned long long h[9];
     int herr = 0;

@@ -40,8 +40,10 @@ struct BatchArgs {
 
 hipError_t mcd_launch_batch(const mcd::BatchArgs &B, size_t lds_bytes, int blocks, hipStream_t s);
 size_t mcd_lds_bytes(int nx, int ny, int nz, int nbins);
-hipError_t mcd_launch_transport(const mcd::Grid &G, const mcd::RunArgs &RA, int blocks,
-                                int threads, hipStream_t s);
+// two: two packets per lane (at most 3 wavefronts per SIMD, which that
+// kernel's registers allow), else one
+hipError_t mcd_launch_transport(const mcd::Grid &G, const mcd::RunArgs &RA, int blocks, int threads, bool two,
+                                hipStream_t s);
 hipError_t mcd_densify(const float *compact, const int *active, int ncell3, int nc1, int nbins,
                        float *dense, hipStream_t s);
 hipError_t mcd_guides(const float *cdf, int rows, int nbins, unsigned short *guide, hipStream_t s);

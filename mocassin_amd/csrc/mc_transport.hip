@@ -313,9 +313,120 @@ __device__ __forceinline__ void stage_model(const Grid &G, const RunArgs &RA, fl
     __syncthreads();
 }
 
-// One scheduling round of a wavefront: an emission pass when at least
-// emit_min lanes (or as many as are walking) wait for one, then walk_steps
-// walk passes.  Counts events into C; a fail-stop code goes to RA.error.
+// An emission pass of the wavefront: every lane whose packet E waits for an
+// emission runs it together.  Counts events into C.
+__device__ __forceinline__ void emission_pass(Lane &E, const Grid &G, const RunArgs &RA, const Model &M,
+                                              const float *spec, const unsigned short *spec_guide, WaveCounts &C,
+                                              double &qphot, int &err)
+{
+    const bool pend = (E.phase == PH_EMIT_STELLAR || E.phase == PH_EMIT_DIFFUSE);
+    bool line_end = false, esc_end = false, flight = false;
+    if (pend) {
+        emit(E, G, RA, spec, spec_guide, M.fold, M.dust, M.gas, M.star_cell, qphot, line_end, esc_end, flight,
+             err);
+        if (err) E.phase = PH_IDLE;
+    }
+    C.line += wcount(line_end);
+    C.esc += wcount(esc_end);
+    C.flights += wcount(flight);
+    // energyPacketDriver counts the packet whose last run is the
+    // recursionLimit-th as trapped (photon_mod.f90:126)
+    C.trapped += wcount(!err && (line_end || esc_end) && E.runs >= kRecursionLimit);
+}
+
+// One walk pass of the wavefront: every lane whose packet L walks takes one
+// step of pathSegment.  Counts events into C, queues the J updates.
+template <int D>
+__device__ __forceinline__ void walk_pass(Lane &L, const Grid &G, const Axes &A, const Model &M, WaveCounts &C,
+                                          int &err, JQ &Q, int lane)
+{
+    bool seg = false, escaped = false, lost = false, absorbed = false, scattered = false,
+         trapped = false;
+    float jv = 0.f;
+    unsigned jpt = 0;
+    int jnu = 1;
+    if (L.phase == PH_WALK) {
+        L.steps++;
+        float kappa, dV;
+        int r = walk_step<false>(G, A, L.W, L.nuP, kappa, dV, err);
+        if (r == ST_CONTINUE || r == ST_ESCAPED) {
+            jv = L.W.seg * M.dE / dV;
+            jpt = j3_point(G.js, L.W.segX - 1, L.W.segY - 1, L.W.segZ - 1);
+            jnu = L.nuP;
+            seg = true;
+            if (r == ST_ESCAPED) {
+                esc_add(G, L.origin, L.nuP, M.dE, L.d0x, L.d0y, L.d0z, err);
+                escaped = true;
+                L.phase = PH_IDLE;
+            }
+        } else if (r == ST_LOST) {
+            lost = true;
+            L.phase = PH_IDLE;
+        } else if (r == ST_ABSORBED) {
+            jv = L.W.seg * M.dE / dV;
+            jpt = j3_point(G.js, L.W.segX - 1, L.W.segY - 1, L.W.segZ - 1);
+            jnu = L.nuP;
+            seg = true;
+            float rx = L.W.rx / 1.e10f, ry = L.W.ry / 1.e10f, rz = L.W.rz / 1.e10f;
+            if (sqrtf(rx * rx + ry * ry + rz * rz) * 1.e10f >= G.R_out && G.R_out > 0.f) {
+                esc_add(G, L.origin, L.nuP, M.dE, L.d0x, L.d0y, L.d0z, err);   // 1577-1669
+                escaped = true;
+                L.phase = PH_IDLE;
+            } else if (M.dust) {
+                const size_t ti = (size_t)(L.nuP - 1) * (size_t)G.ncell3 + L.W.segIdx;
+                float probSca = G.sca3[ti] / kappa;
+                float u = 1.f - draw(L.R, G);
+                if (u > probSca) {
+                    absorbed = true;
+                    if (M.gas) L.phase = PH_EMIT_DIFFUSE;
+                    else err = 41;                        // dustEmi needs dust PDFs
+                } else if (u <= probSca) {
+                    scattered = true;                     // 1720-1797
+                    L.origin = act(G, L.W.xP, L.W.yP, L.W.zP);
+                    if (M.iso) {
+                        new_direction(L.R, G, false, L.d0x, L.d0y, L.d0z);
+                    } else {
+                        for (int ihg = 1; ihg <= 10; ++ihg)
+                            if (hg(L.R, G, L.nuP, L.d0x, L.d0y, L.d0z) == 0) break;
+                    }
+                    L.W.vx = L.d0x; L.W.vy = L.d0y; L.W.vz = L.d0z;
+                    L.W.absTau = 0.f;
+                    float r2 = draw(L.R, G);
+                    L.W.passProb = -mcf_logf(1.f - r2);
+                } else {
+                    err = 39;
+                }
+            } else {
+                absorbed = true;
+                L.phase = PH_EMIT_DIFFUSE;
+            }
+        } else {
+            if (!err) err = 50;
+        }
+        if (!err) {
+            // safeLimit (2838-2846); recursionLimit (120-126)
+            if ((L.phase == PH_WALK || L.phase == PH_EMIT_DIFFUSE) && L.steps >= kSafeLimit) {
+                lost = true;
+                L.phase = PH_IDLE;
+            }
+            if (L.phase == PH_EMIT_DIFFUSE && L.runs >= kRecursionLimit) L.phase = PH_IDLE;
+            trapped = (L.phase == PH_IDLE) && L.runs >= kRecursionLimit;
+        } else {
+            L.phase = PH_IDLE;
+        }
+    }
+    jq_push<D>(G, Q, seg, jpt, jnu, jv, lane);
+    C.seg += wcount(seg);
+    C.esc += wcount(escaped);
+    C.lost += wcount(lost);
+    C.abs += wcount(absorbed);
+    C.sca += wcount(scattered);
+    C.trapped += wcount(trapped);
+}
+
+// One scheduling round of a wavefront (the batched kernel): an emission pass
+// when at least emit_min lanes (or as many as are walking) wait for one, then
+// walk_steps walk passes.  A fail-stop code goes to RA.error.
 template <int D>
 __device__ __forceinline__ void advance(Lane &L, const Grid &G, const Axes &A, const RunArgs &RA,
                                         const Model &M, const float *spec,
@@ -323,107 +434,56 @@ __device__ __forceinline__ void advance(Lane &L, const Grid &G, const Axes &A, c
                                         int npend, int nwalk, WaveCounts &C, double &qphot, int &err, JQ &Q,
                                         int lane)
 {
-    const bool pend = (L.phase == PH_EMIT_STELLAR || L.phase == PH_EMIT_DIFFUSE);
-    // ---- emission pass, batched across the wavefront
-    if (npend >= emit_min || npend >= nwalk) {
-        bool line_end = false, esc_end = false, flight = false;
-        if (pend) {
-            emit(L, G, RA, spec, spec_guide, M.fold, M.dust, M.gas, M.star_cell, qphot, line_end,
-                 esc_end, flight, err);
-            if (err) L.phase = PH_IDLE;
-        }
-        C.line += wcount(line_end);
-        C.esc += wcount(esc_end);
-        C.flights += wcount(flight);
-        // energyPacketDriver counts the packet whose last run is the
-        // recursionLimit-th as trapped (photon_mod.f90:126)
-        C.trapped += wcount(!err && (line_end || esc_end) && L.runs >= kRecursionLimit);
+    if (npend >= emit_min || npend >= nwalk) emission_pass(L, G, RA, M, spec, spec_guide, C, qphot, err);
+    for (int s = 0; s < walk_steps; ++s) walk_pass<D>(L, G, A, M, C, err, Q, lane);
+    if (err) {
+        atomicMax(RA.error, err);
+        err = 0;
     }
+}
 
-    // ---- walk passes
+__device__ __forceinline__ bool pending(const Lane &L)
+{
+    return L.phase == PH_EMIT_STELLAR || L.phase == PH_EMIT_DIFFUSE;
+}
+
+// Exchange the packets of the lanes where c holds (a wavefront-wide vote
+// first: nothing to do costs one instruction).
+__device__ __forceinline__ void swap_where(bool c, Lane &a, Lane &b)
+{
+    if (!__any(c)) return;
+#define MC_SWAP(f) { const auto t_ = a.f; a.f = c ? b.f : a.f; b.f = c ? t_ : b.f; }
+    MC_SWAP(R.plo) MC_SWAP(R.phi) MC_SWAP(R.stream) MC_SWAP(R.k) MC_SWAP(R.blk)
+    MC_SWAP(R.w0) MC_SWAP(R.w1) MC_SWAP(R.w2) MC_SWAP(R.w3)
+    MC_SWAP(W.rx) MC_SWAP(W.ry) MC_SWAP(W.rz) MC_SWAP(W.vx) MC_SWAP(W.vy) MC_SWAP(W.vz)
+    MC_SWAP(W.xP) MC_SWAP(W.yP) MC_SWAP(W.zP) MC_SWAP(W.absTau) MC_SWAP(W.passProb)
+    MC_SWAP(d0x) MC_SWAP(d0y) MC_SWAP(d0z) MC_SWAP(nuP) MC_SWAP(origin) MC_SWAP(steps) MC_SWAP(runs)
+    MC_SWAP(phase)
+#undef MC_SWAP
+}
+
+// One scheduling round of a wavefront with two packets per lane (the
+// single-model kernel): L walks, S is the spare the emission pass serves.
+// A packet that waits for an emission moves to S first; after the emission
+// pass, and after every walk pass, a lane whose L does not walk takes S's
+// packet when that one does.  A lane then idles in a walk pass only when
+// both of its packets wait, so the walk passes stay fuller than with one
+// packet per lane; the packets and their streams are the same.
+template <int D>
+__device__ __forceinline__ void advance_spare(Lane &L, Lane &S, const Grid &G, const Axes &A, const RunArgs &RA,
+                                              const Model &M, const float *spec, const unsigned short *spec_guide,
+                                              int emit_min, int walk_steps, WaveCounts &C, double &qphot, int &err,
+                                              JQ &Q, int lane)
+{
+    swap_where(pending(L) && !pending(S), L, S);
+    const int npend = (int)wcount(pending(S));
+    const int nwalk = (int)wcount(L.phase == PH_WALK);
+    if (npend > 0 && (npend >= emit_min || npend >= nwalk))
+        emission_pass(S, G, RA, M, spec, spec_guide, C, qphot, err);
+    swap_where(L.phase != PH_WALK && S.phase == PH_WALK, L, S);
     for (int s = 0; s < walk_steps; ++s) {
-        bool seg = false, escaped = false, lost = false, absorbed = false, scattered = false,
-             trapped = false;
-        float jv = 0.f;
-        unsigned jpt = 0;
-        int jnu = 1;
-        if (L.phase == PH_WALK) {
-            L.steps++;
-            float kappa, dV;
-            int r = walk_step<false>(G, A, L.W, L.nuP, kappa, dV, err);
-            if (r == ST_CONTINUE || r == ST_ESCAPED) {
-                jv = L.W.seg * M.dE / dV;
-                jpt = j3_point(G.js, L.W.segX - 1, L.W.segY - 1, L.W.segZ - 1);
-                jnu = L.nuP;
-                seg = true;
-                if (r == ST_ESCAPED) {
-                    esc_add(G, L.origin, L.nuP, M.dE, L.d0x, L.d0y, L.d0z, err);
-                    escaped = true;
-                    L.phase = PH_IDLE;
-                }
-            } else if (r == ST_LOST) {
-                lost = true;
-                L.phase = PH_IDLE;
-            } else if (r == ST_ABSORBED) {
-                jv = L.W.seg * M.dE / dV;
-                jpt = j3_point(G.js, L.W.segX - 1, L.W.segY - 1, L.W.segZ - 1);
-                jnu = L.nuP;
-                seg = true;
-                float rx = L.W.rx / 1.e10f, ry = L.W.ry / 1.e10f, rz = L.W.rz / 1.e10f;
-                if (sqrtf(rx * rx + ry * ry + rz * rz) * 1.e10f >= G.R_out && G.R_out > 0.f) {
-                    esc_add(G, L.origin, L.nuP, M.dE, L.d0x, L.d0y, L.d0z, err);   // 1577-1669
-                    escaped = true;
-                    L.phase = PH_IDLE;
-                } else if (M.dust) {
-                    const size_t ti = (size_t)(L.nuP - 1) * (size_t)G.ncell3 + L.W.segIdx;
-                    float probSca = G.sca3[ti] / kappa;
-                    float u = 1.f - draw(L.R, G);
-                    if (u > probSca) {
-                        absorbed = true;
-                        if (M.gas) L.phase = PH_EMIT_DIFFUSE;
-                        else err = 41;                        // dustEmi needs dust PDFs
-                    } else if (u <= probSca) {
-                        scattered = true;                     // 1720-1797
-                        L.origin = act(G, L.W.xP, L.W.yP, L.W.zP);
-                        if (M.iso) {
-                            new_direction(L.R, G, false, L.d0x, L.d0y, L.d0z);
-                        } else {
-                            for (int ihg = 1; ihg <= 10; ++ihg)
-                                if (hg(L.R, G, L.nuP, L.d0x, L.d0y, L.d0z) == 0) break;
-                        }
-                        L.W.vx = L.d0x; L.W.vy = L.d0y; L.W.vz = L.d0z;
-                        L.W.absTau = 0.f;
-                        float r2 = draw(L.R, G);
-                        L.W.passProb = -mcf_logf(1.f - r2);
-                    } else {
-                        err = 39;
-                    }
-                } else {
-                    absorbed = true;
-                    L.phase = PH_EMIT_DIFFUSE;
-                }
-            } else {
-                if (!err) err = 50;
-            }
-            if (!err) {
-                // safeLimit (2838-2846); recursionLimit (120-126)
-                if ((L.phase == PH_WALK || L.phase == PH_EMIT_DIFFUSE) && L.steps >= kSafeLimit) {
-                    lost = true;
-                    L.phase = PH_IDLE;
-                }
-                if (L.phase == PH_EMIT_DIFFUSE && L.runs >= kRecursionLimit) L.phase = PH_IDLE;
-                trapped = (L.phase == PH_IDLE) && L.runs >= kRecursionLimit;
-            } else {
-                L.phase = PH_IDLE;
-            }
-        }
-        jq_push<D>(G, Q, seg, jpt, jnu, jv, lane);
-        C.seg += wcount(seg);
-        C.esc += wcount(escaped);
-        C.lost += wcount(lost);
-        C.abs += wcount(absorbed);
-        C.sca += wcount(scattered);
-        C.trapped += wcount(trapped);
+        walk_pass<D>(L, G, A, M, C, err, Q, lane);
+        swap_where(L.phase != PH_WALK && S.phase == PH_WALK, L, S);
     }
     if (err) {
         atomicMax(RA.error, err);
@@ -450,6 +510,9 @@ __device__ __forceinline__ void flush_counts(const RunArgs &RA, WaveCounts &C, d
     qphot = 0.0;
 }
 
+// The single-model kernel.  TWO: two packets per lane (advance_spare; 144
+// VGPRs, so at most 3 wavefronts per SIMD), else one (advance; 4 per SIMD).
+template <bool TWO>
 __global__ void __launch_bounds__(256) transport_kernel(Grid G, RunArgs RA)
 {
     extern __shared__ float lds[];
@@ -462,17 +525,20 @@ __global__ void __launch_bounds__(256) transport_kernel(Grid G, RunArgs RA)
     JQ Q = jq_at<kJDepth>(lds, (int)(threadIdx.x >> 6));
     const Model M = model_of(G, RA);
 
-    Lane L;
+    Lane L, S;
     L.phase = PH_IDLE;
     L.runs = 0; L.steps = 0; L.nuP = 1; L.origin = 0;
+    S = L;
     bool exhausted = false;
     WaveCounts C = {0, 0, 0, 0, 0, 0, 0, 0};
     double qphot = 0.0;
     int err = 0;
 
     for (;;) {
-        // ---- refill idle lanes: one atomic per wavefront
-        bool need = (L.phase == PH_IDLE) && !exhausted;
+        // ---- refill idle lanes (their spare with two packets per lane): one
+        // atomic per wavefront
+        Lane &N = TWO ? S : L;
+        bool need = (N.phase == PH_IDLE) && !exhausted;
         uint64_t m = __ballot(need);
         if (m) {
             int leader = __ffsll((unsigned long long)m) - 1;
@@ -484,19 +550,24 @@ __global__ void __launch_bounds__(256) transport_kernel(Grid G, RunArgs RA)
                 unsigned long long rank = __popcll(m & ((1ull << lane_id) - 1ull));
                 unsigned long long id = base + rank;
                 if (id < (unsigned long long)RA.npackets) {
-                    rng_start(L.R, RA.packet_offset + id, (uint32_t)RA.istar);
-                    L.phase = PH_EMIT_STELLAR;
+                    rng_start(N.R, RA.packet_offset + id, (uint32_t)RA.istar);
+                    N.phase = PH_EMIT_STELLAR;
                 } else {
                     exhausted = true;
                 }
             }
         }
-        const bool pend = (L.phase == PH_EMIT_STELLAR || L.phase == PH_EMIT_DIFFUSE);
-        const int npend = (int)wcount(pend);
-        const int nwalk = (int)wcount(L.phase == PH_WALK);
-        if (npend + nwalk == 0) break;
-        advance<kJDepth>(L, G, A, RA, M, spec, spec_guide, RA.emit_min, RA.walk_steps, npend, nwalk, C, qphot, err, Q,
-                lane_id);
+        if (TWO) {
+            if (wcount(L.phase != PH_IDLE || S.phase != PH_IDLE) == 0) break;
+            advance_spare<kJDepth>(L, S, G, A, RA, M, spec, spec_guide, RA.emit_min, RA.walk_steps, C, qphot, err,
+                                   Q, lane_id);
+        } else {
+            const int npend = (int)wcount(pending(L));
+            const int nwalk = (int)wcount(L.phase == PH_WALK);
+            if (npend + nwalk == 0) break;
+            advance<kJDepth>(L, G, A, RA, M, spec, spec_guide, RA.emit_min, RA.walk_steps, npend, nwalk, C, qphot,
+                             err, Q, lane_id);
+        }
     }
     jq_flush<kJDepth>(G, Q, lane_id);
     flush_counts(RA, C, qphot);
@@ -650,12 +721,15 @@ size_t mcd_lds_bytes(int nx, int ny, int nz, int nbins)
     return sizeof(float) * (mcd::JRing<mcd::kJDepthBatch>::kFloats + mcd::lds_floats(nx, ny, nz, nbins));
 }
 
-hipError_t mcd_launch_transport(const mcd::Grid &G, const mcd::RunArgs &RA, int blocks,
-                                int threads, hipStream_t s)
+hipError_t mcd_launch_transport(const mcd::Grid &G, const mcd::RunArgs &RA, int blocks, int threads, bool two,
+                                hipStream_t s)
 {
     size_t lds = sizeof(float) * ((size_t)(threads / 64) * mcd::JRing<mcd::kJDepth>::kFloats +
                                   mcd::lds_floats(G.nx, G.ny, G.nz, G.nbins));
-    hipLaunchKernelGGL(mcd::transport_kernel, dim3(blocks), dim3(threads), lds, s, G, RA);
+    if (two)
+        hipLaunchKernelGGL(mcd::transport_kernel<true>, dim3(blocks), dim3(threads), lds, s, G, RA);
+    else
+        hipLaunchKernelGGL(mcd::transport_kernel<false>, dim3(blocks), dim3(threads), lds, s, G, RA);
     return hipGetLastError();
 }
 
