@@ -53,7 +53,7 @@ def autotune(transport, npackets, istar=1, grid=None, packet_offset=1 << 40, rep
 
 
 BATCH_GRID = {
-    "waves_per_cu": (8, 16),
+    "waves_per_cu": (8, 12, 16),
     "emit_min": (32, 64),
     "walk_steps": (8, 16, 32),
     "chunk": (1024, 4096),

@@ -580,8 +580,10 @@ __global__ void __launch_bounds__(256) transport_kernel(Grid G, RunArgs RA)
 // model is taken only after the wavefront's packets of the current model
 // have finished: then its grid and source are staged in LDS and its
 // counters flushed.  Each packet keeps its own random stream, so results
-// equal those of the models' separate runs.
-__global__ void __launch_bounds__(64) batch_kernel(BatchArgs B)
+// equal those of the models' separate runs.  Compiled for 4 wavefronts per
+// SIMD (128 VGPRs, a few spilled to scratch): the one-wavefront workgroups
+// need the occupancy more than the registers.
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) batch_kernel(BatchArgs B)
 {
     extern __shared__ float lds[];
     const int lane_id = threadIdx.x;
