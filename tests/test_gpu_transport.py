@@ -6,7 +6,8 @@ Bars:
     reference) and equal the oracle's walk bit for bit;
   * transport: same packets and streams as the oracle (deterministic math,
     FP64 accumulation): every event counter equal, J and escapes equal to
-    1e-12 relative (only the FP64 summation order of the atomics differs);
+    1e-12 relative (only the FP64 summation order of the atomics differs),
+    with one packet per lane and with two;
   * against the reference itself: the reference draws glibc's faithfully
     rounded log/sin/cos while the kernel draws correctly rounded ones, so
     ~1-3% of histories differ after a 1-ulp direction change; totals agree
@@ -23,6 +24,10 @@ from conftest import (golden, grid_from_fixture, j_map_from_segments, oracle_pro
                       reference_estimators, walk_packets)
 
 pytestmark = pytest.mark.gpu
+
+# The two transport kernels: one packet per lane (the library default,
+# 4 workgroups of 256 per CU) and two (at most 3 per CU).
+KERNELS = {"one": None, "two": {"blocks_per_cu": 3, "threads": 256, "emit_min": 64, "walk_steps": 16}}
 
 
 def transport_from_problem(P, seed, **kw):
@@ -71,15 +76,18 @@ def _cmp_estimators(J, E, Jo, Eo, rtol=1e-12):
     assert np.array_equal(J > 0, Jo > 0)
 
 
+@pytest.mark.parametrize("kernel", sorted(KERNELS))
 @pytest.mark.parametrize("name", ["c1mini", "c1mini_thick", "full_mini_thick", "dust_mini",
                                   "dust_mini_thick", "c3mini"])
-def test_transport_equals_oracle(name):
+def test_transport_equals_oracle(name, kernel):
     g = golden("transport_" + name)
     P = oracle_problem(g)
     n, seed = int(g["npackets"]), int(g["seed"])
-    print("transport vs oracle", name, n, flush=True)
+    print("transport vs oracle", name, kernel, n, flush=True)
     Jo, Eo, co = P.transport(0, n, seed, 1, O.MATH_OWN, O.J_FP64)
     T = transport_from_problem(P, seed)
+    if KERNELS[kernel]:
+        T.set_tuning(**KERNELS[kernel])
     c = T.energy_packet_driver(1, n, 0)
     J, E = T.fetch()
     T.close()
@@ -89,8 +97,9 @@ def test_transport_equals_oracle(name):
     _cmp_estimators(J, E, Jo, Eo)
 
 
+@pytest.mark.parametrize("kernel", sorted(KERNELS))
 @pytest.mark.parametrize("name", ["full_mini_view", "c1mini_view"])
-def test_viewing_angle_planes_equal_oracle(name):
+def test_viewing_angle_planes_equal_oracle(name, kernel):
     """escapedPackets(:,:,0:2) with two inclinations (phi on, on the full
     grid; phi off, on the symmetric octant): the GPU equals the oracle packet
     for packet; the oracle equals the reference bit for bit in libm mode
@@ -101,9 +110,11 @@ def test_viewing_angle_planes_equal_oracle(name):
     view = view_from_fixture(g)
     P = oracle_problem(g, view=view)
     n, seed = int(g["npackets"]), int(g["seed"])
-    print("viewing angles vs oracle", name, n, flush=True)
+    print("viewing angles vs oracle", name, kernel, n, flush=True)
     Jo, Eo, co = P.transport(0, n, seed, 1, O.MATH_OWN, O.J_FP64)
     T = transport_from_problem(P, seed)
+    if KERNELS[kernel]:
+        T.set_tuning(**KERNELS[kernel])
     T.set_view_angles(view)
     assert T._lib.mc_esc_planes(T._h) == 3
     c = T.energy_packet_driver(1, n, 0)
@@ -159,7 +170,8 @@ def test_launch_partition_independence():
     _cmp_estimators(J2, E2, J1, E1)
 
 
-def test_ragged_counts_and_wide_packet_ids_equal_oracle():
+@pytest.mark.parametrize("kernel", sorted(KERNELS))
+def test_ragged_counts_and_wide_packet_ids_equal_oracle(kernel):
     """Packet counts that do not fill a wavefront (1, 63, 65), none at all,
     and packet ranges across 2^32 and far above it: every packet id is a
     64-bit Philox counter, so each range equals the oracle's packet for
@@ -168,6 +180,8 @@ def test_ragged_counts_and_wide_packet_ids_equal_oracle():
     P = oracle_problem(g)
     seed = int(g["seed"])
     T = transport_from_problem(P, seed)
+    if KERNELS[kernel]:
+        T.set_tuning(**KERNELS[kernel])
     try:
         c = T.energy_packet_driver(1, 0, 0)
         J, E = T.fetch()
