@@ -55,6 +55,14 @@ def log(msg):
     print("bench: " + msg, file=sys.stderr, flush=True)
 
 
+def lanes_per_pass(c):
+    """Busy lanes of a 64-lane wavefront per walk pass and per emission
+    pass of one launch, from the kernel's own pass counters."""
+    return {"walk": c["n_walk_lanes"] / max(c["n_walk_passes"], 1.0),
+            "emit": c["n_emit_lanes"] / max(c["n_emit_passes"], 1.0),
+            "walk_passes": c["n_walk_passes"], "emit_passes": c["n_emit_passes"]}
+
+
 def algorithmic_bytes(c, nbins):
     """24 B per segment (4 active + 4 opacity + 16 FP64 J read-modify-write),
     8 + 4*ceil(log2 nbins) B per absorption (line fraction, u-test operand,
@@ -399,7 +407,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_note,
                      "algorithmic_bytes": byts, "kernel": "transport_kernel", "kernel_ms": kms,
-                     "segments": c0["n_seg"], "absorptions": c0["n_abs"], "escapes": c0["n_esc"]},
+                     "segments": c0["n_seg"], "absorptions": c0["n_abs"], "escapes": c0["n_esc"],
+                     "lanes_per_pass": lanes_per_pass(c0)},
     }
     if valu is not None:
         out["roofline"]["valu"] = valu

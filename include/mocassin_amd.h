@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define MC_ABI_VERSION 1
+#define MC_ABI_VERSION 2
 
 /* flags for mc_desc.flags */
 #define MC_SYMMETRIC   0x01  /* lgSymmetricXYZ  (set_input_mod.f90:333) */
@@ -87,6 +87,11 @@ typedef struct {
     double n_flights;   /* pathSegment calls */
     double qphot;       /* Qphot: sum deltaE/(h nu) over stellar packets with nu > 1 Ryd */
     double kernel_ms;   /* device time of the transport kernel */
+    /* the scheduling of the launch (no counterpart in the reference):
+     * wavefront walk passes and the lanes that walked in them, wavefront
+     * emission passes and the lanes that emitted in them; lanes per pass =
+     * lanes / passes */
+    double n_walk_passes, n_walk_lanes, n_emit_passes, n_emit_lanes;
 } mc_counters;
 
 /* Launch shape of the transport kernel (chosen by the autotuner,

@@ -10,7 +10,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmocassin_amd.so")
 
-MC_ABI_VERSION = 1
+MC_ABI_VERSION = 2
 MC_SYMMETRIC, MC_DUST, MC_GAS, MC_ISOTROPIC, MC_MULTISTARS = 0x01, 0x02, 0x04, 0x08, 0x10
 ERRORS = {0: "MC_OK", 1: "MC_EINVAL", 2: "MC_ENODEV", 3: "MC_ENOMEM", 4: "MC_ESTATE",
           5: "MC_ETABLE", 6: "MC_EPACKET", 7: "MC_EUNSUP"}
@@ -46,7 +46,7 @@ class McDesc(C.Structure):
 class McCounters(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "n_seg", "n_abs", "n_sca", "n_esc", "n_lost", "n_line", "n_trapped", "n_flights",
-        "qphot", "kernel_ms")]
+        "qphot", "kernel_ms", "n_walk_passes", "n_walk_lanes", "n_emit_passes", "n_emit_lanes")]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

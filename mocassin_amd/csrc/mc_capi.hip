@@ -23,6 +23,22 @@
 
 #pragma clang fp contract(off)
 
+// Device event counters of one launch, after the packet counter: the eight
+// event counts of mc_counters (n_seg .. n_flights), then the wavefront walk
+// passes, the lanes that walked in them, the emission passes and the lanes
+// that emitted in them.
+constexpr int kCounts = 12;
+
+static void fill_counters(mc_counters &k, const unsigned long long *h, double q, float ms)
+{
+    k.n_seg = (double)h[1]; k.n_abs = (double)h[2]; k.n_sca = (double)h[3];
+    k.n_esc = (double)h[4]; k.n_lost = (double)h[5]; k.n_line = (double)h[6];
+    k.n_trapped = (double)h[7]; k.n_flights = (double)h[8];
+    k.qphot = q; k.kernel_ms = ms;
+    k.n_walk_passes = (double)h[9]; k.n_walk_lanes = (double)h[10];
+    k.n_emit_passes = (double)h[11]; k.n_emit_lanes = (double)h[12];
+}
+
 struct mc_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -43,7 +59,7 @@ struct mc_ctx {
     double *d_J3 = nullptr, *d_J = nullptr, *d_esc = nullptr;
     mcd::View *d_view = nullptr;            // viewing-angle bins (null: escapes in plane 0 only)
     int esc_planes = 1;                      // nAngleBins + 1
-    unsigned long long *d_scratch = nullptr;  // next_packet, counts[8]
+    unsigned long long *d_scratch = nullptr;  // next_packet, counts[kCounts]
     int *d_error = nullptr;
     double *d_qphot = nullptr;
     // per-cell balance
@@ -254,7 +270,7 @@ static int ctx_init(mc_ctx *c, const mc_desc *d)
     const size_t ncell3 = (size_t)nx * ny * nz;
     HIPCHK(c, dalloc(&c->d_J3, mcd::j3_size(mcd::j3_shape(nx, ny, nz), nb)));
     HIPCHK(c, dalloc(&c->d_esc, (size_t)(nc + 1) * (nb + 1)));
-    HIPCHK(c, dalloc(&c->d_scratch, 9));
+    HIPCHK(c, dalloc(&c->d_scratch, 1 + kCounts));
     HIPCHK(c, dalloc(&c->d_error, 1));
     HIPCHK(c, dalloc(&c->d_qphot, 1));
     HIPCHK(c, copy_on(c->stream, c->d_axis, axis.data(), na * sizeof(float), hipMemcpyHostToDevice));
@@ -1849,7 +1865,7 @@ extern "C" int mc_transport_run(mc_ctx *c, int32_t istar, int64_t npackets, uint
     if (istar < 1 || istar > c->nStars) return fail(c, MC_EINVAL, "istar %d outside 1..%d", istar, c->nStars);
     if (npackets < 0) return fail(c, MC_EINVAL, "negative packet count");
     HIPCHK(c, hipSetDevice(c->device));
-    HIPCHK(c, hipMemsetAsync(c->d_scratch, 0, 9 * sizeof(unsigned long long), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_scratch, 0, (1 + kCounts) * sizeof(unsigned long long), c->stream));
     HIPCHK(c, hipMemsetAsync(c->d_error, 0, sizeof(int), c->stream));
     HIPCHK(c, hipMemsetAsync(c->d_qphot, 0, sizeof(double), c->stream));
     mcd::RunArgs RA;
@@ -1872,7 +1888,7 @@ extern "C" int mc_transport_run(mc_ctx *c, int32_t istar, int64_t npackets, uint
     if (npackets > 0)
         HIPCHK(c, mcd_launch_transport(c->G, RA, blocks, c->tune.threads, c->tune.blocks_per_cu <= 3, c->stream));
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-    unsigned long long h[9];
+    unsigned long long h[1 + kCounts];
     int herr = 0;
     double q = 0.0;
     HIPCHK(c, hipMemcpyAsync(h, c->d_scratch, sizeof h, hipMemcpyDeviceToHost, c->stream));
@@ -1881,12 +1897,7 @@ extern "C" int mc_transport_run(mc_ctx *c, int32_t istar, int64_t npackets, uint
     HIPCHK(c, hipStreamSynchronize(c->stream));
     float ms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    if (cnt) {
-        cnt->n_seg = (double)h[1]; cnt->n_abs = (double)h[2]; cnt->n_sca = (double)h[3];
-        cnt->n_esc = (double)h[4]; cnt->n_lost = (double)h[5]; cnt->n_line = (double)h[6];
-        cnt->n_trapped = (double)h[7]; cnt->n_flights = (double)h[8];
-        cnt->qphot = q; cnt->kernel_ms = ms;
-    }
+    if (cnt) fill_counters(*cnt, h, q, ms);
     if (herr) return fail(c, MC_EPACKET, "%s (code %d)", packet_error_text(herr), herr);
     if (J_out || esc_out) return mc_fetch_estimators(c, J_out, esc_out);
     return MC_OK;
@@ -1961,7 +1972,7 @@ extern "C" int mc_transport_batch(mc_ctx *const *ctx, int32_t n, const int32_t *
     cm.reserve(nchunks); cf.reserve(nchunks); cc.reserve(nchunks);
     for (int i = 0; i < n; ++i) {
         mc_ctx *c = ctx[i];
-        HIPCHK(c0, hipMemsetAsync(c->d_scratch, 0, 9 * sizeof(unsigned long long), c0->stream));
+        HIPCHK(c0, hipMemsetAsync(c->d_scratch, 0, (1 + kCounts) * sizeof(unsigned long long), c0->stream));
         HIPCHK(c0, hipMemsetAsync(c->d_error, 0, sizeof(int), c0->stream));
         HIPCHK(c0, hipMemsetAsync(c->d_qphot, 0, sizeof(double), c0->stream));
         G[i] = c->G;
@@ -2017,19 +2028,13 @@ extern "C" int mc_transport_batch(mc_ctx *const *ctx, int32_t n, const int32_t *
     int rc = MC_OK;
     for (int i = 0; i < n; ++i) {
         mc_ctx *c = ctx[i];
-        unsigned long long h[9];
+        unsigned long long h[1 + kCounts];
         int herr = 0;
         double q = 0.0;
         HIPCHK(c0, copy_on(c0->stream, h, c->d_scratch, sizeof h, hipMemcpyDeviceToHost));
         HIPCHK(c0, copy_on(c0->stream, &herr, c->d_error, sizeof herr, hipMemcpyDeviceToHost));
         HIPCHK(c0, copy_on(c0->stream, &q, c->d_qphot, sizeof q, hipMemcpyDeviceToHost));
-        if (cnt) {
-            mc_counters &k = cnt[i];
-            k.n_seg = (double)h[1]; k.n_abs = (double)h[2]; k.n_sca = (double)h[3];
-            k.n_esc = (double)h[4]; k.n_lost = (double)h[5]; k.n_line = (double)h[6];
-            k.n_trapped = (double)h[7]; k.n_flights = (double)h[8];
-            k.qphot = q; k.kernel_ms = ms;
-        }
+        if (cnt) fill_counters(cnt[i], h, q, ms);
         if (herr && rc == MC_OK) {
             fail(c, MC_EPACKET, "%s (code %d)", packet_error_text(herr), herr);
             rc = fail(c0, MC_EPACKET, "context %d: %s (code %d)", i, packet_error_text(herr), herr);
@@ -2053,7 +2058,8 @@ extern "C" int mc_transport_batch(mc_ctx *const *ctx, int32_t n, const int32_t *
 constexpr size_t kNoAllocBytes = 8u << 20;
 constexpr int kNoAllocModels = 256;
 constexpr int kNoAllocDevices = 64;
-constexpr int kCounterWords = 16;       // next_packet, counts[8], error, qphot per model
+constexpr int kCounterWords = 16;       // next_packet, counts[kCounts], error, qphot per model
+static_assert(kCounterWords >= 1 + kCounts + 2, "counter block too small");
 static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 static std::mutex g_scratch_mu;
 static unsigned char *g_scratch_ptr[kNoAllocDevices];
@@ -2151,8 +2157,8 @@ extern "C" int mc_transport_batch_noalloc(mc_ctx *const *ctx, int32_t n, const i
         unsigned long long *k = dK + (size_t)i * kCounterWords;
         R.next_packet = k;
         R.counts = k + 1;
-        R.error = reinterpret_cast<int *>(k + 9);
-        R.qphot = reinterpret_cast<double *>(k + 10);
+        R.error = reinterpret_cast<int *>(k + 1 + kCounts);
+        R.qphot = reinterpret_cast<double *>(k + 2 + kCounts);
         tP[i] = npackets[i];
         tS[i + 1] = tS[i] + (npackets[i] + tu.chunk - 1) / tu.chunk;
     }
@@ -2200,15 +2206,9 @@ extern "C" int mc_transport_batch_noalloc(mc_ctx *const *ctx, int32_t n, const i
         const unsigned long long *h = tK + (size_t)i * kCounterWords;
         int herr = 0;
         double q = 0.0;
-        std::memcpy(&herr, h + 9, sizeof herr);
-        std::memcpy(&q, h + 10, sizeof q);
-        if (cnt) {
-            mc_counters &k = cnt[i];
-            k.n_seg = (double)h[1]; k.n_abs = (double)h[2]; k.n_sca = (double)h[3];
-            k.n_esc = (double)h[4]; k.n_lost = (double)h[5]; k.n_line = (double)h[6];
-            k.n_trapped = (double)h[7]; k.n_flights = (double)h[8];
-            k.qphot = q; k.kernel_ms = ms;
-        }
+        std::memcpy(&herr, h + 1 + kCounts, sizeof herr);
+        std::memcpy(&q, h + 2 + kCounts, sizeof q);
+        if (cnt) fill_counters(cnt[i], h, q, ms);
         if (herr && rc == MC_OK) {
             fail(ctx[i], MC_EPACKET, "%s (code %d)", packet_error_text(herr), herr);
             rc = fail(c0, MC_EPACKET, "context %d: %s (code %d)", i, packet_error_text(herr), herr);

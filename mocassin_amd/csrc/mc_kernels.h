@@ -15,7 +15,7 @@ struct RunArgs {
     unsigned long long packet_offset;
     unsigned long long *next_packet;    // global refill counter (zeroed per launch)
     int *error;                         // first fail-stop code (atomicMax)
-    unsigned long long *counts;         // 8 event counters
+    unsigned long long *counts;         // 8 event counters, then walk passes and lanes, emission passes and lanes
     double *qphot;
     float deltaE;
     float px, py, pz;                   // source position [cm]

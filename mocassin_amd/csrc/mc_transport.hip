@@ -178,6 +178,7 @@ struct Lane {
 // wave-uniform event counters (scalar registers)
 struct WaveCounts {
     unsigned long long seg, abs, sca, esc, lost, line, trapped, flights;
+    unsigned long long wpass, wlanes, epass, elanes;    // passes and the lanes in them
 };
 
 __device__ __forceinline__ unsigned long long wcount(bool p)
@@ -326,6 +327,8 @@ __device__ __forceinline__ void emission_pass(Lane &E, const Grid &G, const RunA
              err);
         if (err) E.phase = PH_IDLE;
     }
+    C.epass += 1;
+    C.elanes += wcount(pend);
     C.line += wcount(line_end);
     C.esc += wcount(esc_end);
     C.flights += wcount(flight);
@@ -345,6 +348,8 @@ __device__ __forceinline__ void walk_pass(Lane &L, const Grid &G, const Axes &A,
     float jv = 0.f;
     unsigned jpt = 0;
     int jnu = 1;
+    C.wpass += 1;
+    C.wlanes += wcount(L.phase == PH_WALK);
     if (L.phase == PH_WALK) {
         L.steps++;
         float kappa, dV;
@@ -504,9 +509,13 @@ __device__ __forceinline__ void flush_counts(const RunArgs &RA, WaveCounts &C, d
         atomicAdd(&RA.counts[5], C.line);
         atomicAdd(&RA.counts[6], C.trapped);
         atomicAdd(&RA.counts[7], C.flights);
+        atomicAdd(&RA.counts[8], C.wpass);
+        atomicAdd(&RA.counts[9], C.wlanes);
+        atomicAdd(&RA.counts[10], C.epass);
+        atomicAdd(&RA.counts[11], C.elanes);
         unsafeAtomicAdd(RA.qphot, qphot);
     }
-    C = WaveCounts{0, 0, 0, 0, 0, 0, 0, 0};
+    C = WaveCounts{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     qphot = 0.0;
 }
 
@@ -530,7 +539,7 @@ __global__ void __launch_bounds__(256) transport_kernel(Grid G, RunArgs RA)
     L.runs = 0; L.steps = 0; L.nuP = 1; L.origin = 0;
     S = L;
     bool exhausted = false;
-    WaveCounts C = {0, 0, 0, 0, 0, 0, 0, 0};
+    WaveCounts C = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     double qphot = 0.0;
     int err = 0;
 
@@ -605,7 +614,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) ba
     Lane L;
     L.phase = PH_IDLE;
     L.runs = 0; L.steps = 0; L.nuP = 1; L.origin = 0;
-    WaveCounts C = {0, 0, 0, 0, 0, 0, 0, 0};
+    WaveCounts C = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     double qphot = 0.0;
     int err = 0;
 

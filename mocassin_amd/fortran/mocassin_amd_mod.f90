@@ -20,7 +20,7 @@ module mocassin_amd_mod
     use, intrinsic :: iso_c_binding
     implicit none
 
-    integer(c_int32_t), parameter :: MC_ABI_VERSION = 1
+    integer(c_int32_t), parameter :: MC_ABI_VERSION = 2
     integer(c_int32_t), parameter :: MC_SYMMETRIC = 1, MC_DUST = 2, MC_GAS = 4, &
                                      MC_ISOTROPIC = 8, MC_MULTISTARS = 16
     integer(c_int), parameter :: MC_OK = 0, MC_EINVAL = 1, MC_ENODEV = 2, MC_ENOMEM = 3, &
@@ -40,7 +40,7 @@ module mocassin_amd_mod
 
     type, bind(C) :: mc_counters
         real(c_double) :: n_seg, n_abs, n_sca, n_esc, n_lost, n_line, n_trapped, n_flights, &
-                          qphot, kernel_ms
+                          qphot, kernel_ms, n_walk_passes, n_walk_lanes, n_emit_passes, n_emit_lanes
     end type mc_counters
 
     type, bind(C) :: mc_tuning

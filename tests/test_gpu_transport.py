@@ -76,6 +76,19 @@ def _cmp_estimators(J, E, Jo, Eo, rtol=1e-12):
     assert np.array_equal(J > 0, Jo > 0)
 
 
+def _check_pass_counters(c, npackets):
+    """The scheduling counters bound the event counters: a walking lane
+    ends its pass with a segment or as lost (ST_LOST; safeLimit losses
+    follow a segment), an emitting lane starts a flight or ends its packet
+    in a line or an escape, every packet is emitted at least once, and no
+    pass has more than 64 lanes."""
+    wl, wp, el, ep = c["n_walk_lanes"], c["n_walk_passes"], c["n_emit_lanes"], c["n_emit_passes"]
+    assert c["n_seg"] <= wl <= c["n_seg"] + c["n_lost"]
+    assert c["n_flights"] <= el <= c["n_flights"] + c["n_line"] + c["n_esc"]
+    assert el >= npackets
+    assert 0 < wl <= 64 * wp and 0 < el <= 64 * ep
+
+
 @pytest.mark.parametrize("kernel", sorted(KERNELS))
 @pytest.mark.parametrize("name", ["c1mini", "c1mini_thick", "full_mini_thick", "dust_mini",
                                   "dust_mini_thick", "c3mini"])
@@ -94,6 +107,7 @@ def test_transport_equals_oracle(name, kernel):
     for k in ("n_seg", "n_abs", "n_sca", "n_esc", "n_lost", "n_line", "n_trapped", "n_flights"):
         assert c[k] == co[k], k
     assert np.isclose(c["qphot"], co["qphot"], rtol=1e-12)
+    _check_pass_counters(c, n)
     _cmp_estimators(J, E, Jo, Eo)
 
 
