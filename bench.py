@@ -55,6 +55,15 @@ def log(msg):
     print("bench: " + msg, file=sys.stderr, flush=True)
 
 
+def shape_of(t):
+    """A launch shape as text: blocks_per_cu,threads,emit_min,walk_steps and
+    the kernel's name unless the shape picks it (auto)."""
+    from mocassin_amd._capi import KERNELS
+    s = ",".join(str(t[k]) for k in ("blocks_per_cu", "threads", "emit_min", "walk_steps"))
+    name = {v: k for k, v in KERNELS.items()}.get(t.get("kernel", 0), "auto")
+    return s if name == "auto" else s + "," + name
+
+
 def lanes_per_pass(c):
     """Busy lanes of a 64-lane wavefront per walk pass and per emission
     pass of one launch, from the kernel's own pass counters."""
@@ -264,7 +273,8 @@ def main():
                     help="time the launch-shape candidates on this workload and store the best "
                          "in mocassin_amd/tuning_db.json (default: use the stored shape)")
     ap.add_argument("--tuning", default=None,
-                    help="fixed launch shape blocks_per_cu,threads,emit_min,walk_steps")
+                    help="fixed launch shape blocks_per_cu,threads,emit_min,walk_steps[,kernel] "
+                         "(kernel: auto, one, spare, pool)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run one device copy of this many MiB (traffic calibration "
                          "for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -295,12 +305,14 @@ def main():
     T.emission_setup(**emission_tables(atom))
     tuning_src = "default"
     if a.tuning:
-        b, th, e, w = (int(v) for v in a.tuning.split(","))
-        T.set_tuning(blocks_per_cu=b, threads=th, emit_min=e, walk_steps=w)
+        f = a.tuning.split(",")
+        T.set_tuning(blocks_per_cu=int(f[0]), threads=int(f[1]), emit_min=int(f[2]), walk_steps=int(f[3]),
+                     kernel=f[4] if len(f) > 4 else "auto")
         tuning_src = "command line"
     elif at.load("c2") and not a.autotune:
         db = at.load("c2")
-        T.set_tuning(**{k: db[k] for k in ("blocks_per_cu", "threads", "emit_min", "walk_steps")})
+        T.set_tuning(**{k: db[k] for k in ("blocks_per_cu", "threads", "emit_min", "walk_steps", "kernel")
+                        if k in db})
         tuning_src = "mocassin_amd/tuning_db.json"
     js = jscale_of(p["symmetric"])
     spec = p["specCDF"]
@@ -371,7 +383,7 @@ def main():
     if os.path.exists(tp):
         try:
             tj = json.load(open(tp))
-            shape = ",".join(str(tuning[k]) for k in ("blocks_per_cu", "threads", "emit_min", "walk_steps"))
+            shape = shape_of(tuning)
             if tj.get("tuning") == shape and tj.get("tables") == "iterated":
                 traffic = tj["hbm_bytes_per_packet"] * npk
                 traffic_note = "profiles/transport_c2_traffic.json (%s)" % tj.get("tuning")

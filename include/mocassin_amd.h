@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define MC_ABI_VERSION 2
+#define MC_ABI_VERSION 3
 
 /* flags for mc_desc.flags */
 #define MC_SYMMETRIC   0x01  /* lgSymmetricXYZ  (set_input_mod.f90:333) */
@@ -97,21 +97,37 @@ typedef struct {
 /* Launch shape of the transport kernel (chosen by the autotuner,
  * mocassin_amd/autotune.py; no counterpart in the reference):
  *   blocks_per_cu  resident workgroups per compute unit requested (1..32),
- *                  counted in 256-thread units (wavefronts per SIMD); at
- *                  most 3 runs the kernel with two packets per lane (a lane
- *                  whose flight ends walks its spare packet while the next
- *                  emission pass waits), more runs one packet per lane
+ *                  counted in 256-thread units (wavefronts per SIMD)
  *   threads        workgroup size, a multiple of 64 (64..256)
  *   emit_min       an emission pass runs when at least this many lanes of a
  *                  wavefront wait for one (1..64), or half the live lanes do
+ *                  (the pool kernels: when at least this many lanes do not
+ *                  walk, or none does)
  *   walk_steps     walk passes between two emission checks (1..64)
+ *   kernel         MC_KERNEL_AUTO: two packets per lane when blocks_per_cu
+ *                  is at most 3, else the pool kernel where the grid allows
+ *                  it, else one packet per lane;
+ *                  MC_KERNEL_ONE: one packet per lane;
+ *                  MC_KERNEL_SPARE: two packets per lane (a lane whose
+ *                  flight ends walks its spare while the next emission pass
+ *                  waits; 3 wavefronts per SIMD at most);
+ *                  MC_KERNEL_POOL: one packet per lane and a pool of 32
+ *                  packets per wavefront in LDS (a lane whose flight ends
+ *                  takes a walking packet from the pool; emission passes run
+ *                  on nearly full wavefronts); needs a grid of at most
+ *                  2047 x 2047 x 1023 cells and 65535 bins (MC_EUNSUP).
  * Results do not depend on these values (each packet has its own stream).
- * A new context starts with 4, 256, 64, 20. */
+ * A new context starts with 4, 256, 16, 16, MC_KERNEL_AUTO. */
+#define MC_KERNEL_AUTO 0
+#define MC_KERNEL_ONE 1
+#define MC_KERNEL_SPARE 2
+#define MC_KERNEL_POOL 3
 typedef struct {
     int32_t blocks_per_cu;
     int32_t threads;
     int32_t emit_min;
     int32_t walk_steps;
+    int32_t kernel;
 } mc_tuning;
 
 /* Atomic inputs of the per-cell balance, as the host holds them after its

@@ -10,7 +10,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmocassin_amd.so")
 
-MC_ABI_VERSION = 2
+MC_ABI_VERSION = 3
 MC_SYMMETRIC, MC_DUST, MC_GAS, MC_ISOTROPIC, MC_MULTISTARS = 0x01, 0x02, 0x04, 0x08, 0x10
 ERRORS = {0: "MC_OK", 1: "MC_EINVAL", 2: "MC_ENODEV", 3: "MC_ENOMEM", 4: "MC_ESTATE",
           5: "MC_ETABLE", 6: "MC_EPACKET", 7: "MC_EUNSUP"}
@@ -28,6 +28,8 @@ EXPORTS = ("mc_ctx_create", "mc_upload_tables", "mc_zero_estimators", "mc_transp
            "mc_ctx_last_error", "mc_ctx_destroy")
 
 MC_SOLVER_REFERENCE, MC_SOLVER_SECANT = 0, 1
+# mc_tuning.kernel
+KERNELS = {"auto": 0, "one": 1, "spare": 2, "pool": 3}
 MC_MLP_NIN, MC_MLP_NOUT, MC_MLP_NPARAMS, MC_MLP_NSCALE = 72, 31, 49568, 206
 
 
@@ -54,7 +56,7 @@ class McCounters(C.Structure):
 
 class McTuning(C.Structure):
     _fields_ = [("blocks_per_cu", C.c_int32), ("threads", C.c_int32),
-                ("emit_min", C.c_int32), ("walk_steps", C.c_int32)]
+                ("emit_min", C.c_int32), ("walk_steps", C.c_int32), ("kernel", C.c_int32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

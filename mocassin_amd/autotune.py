@@ -1,10 +1,12 @@
 """Autotuner for the transport kernels' launch shapes.
 
-The transport kernel has four free parameters (mc_tuning in
+The transport kernel has five free parameters (mc_tuning in
 include/mocassin_amd.h): resident workgroups per CU, workgroup size, the
-number of waiting lanes that triggers a batched emission pass, and the
-number of walk passes between emission checks.  At most 3 workgroups of 256
-per CU select the kernel with two packets per lane.  The batched kernel
+number of waiting lanes that triggers a batched emission pass, the number
+of walk passes between emission checks, and the kernel (auto: at most 3
+workgroups of 256 per CU select the kernel with two packets per lane; pool:
+one packet per lane and a pool of packets per wavefront in LDS).  The
+default candidates are the auto grid and a pool grid.  The batched kernel
 (mc_batch_tuning) has resident wavefronts per CU, the same two scheduling
 parameters, and the packet-batch size handed to a wavefront at a time.
 Results do not depend on any of them (every packet draws from its own
@@ -22,15 +24,26 @@ DEFAULT_GRID = {
     "threads": (256,),
     "emit_min": (32, 48, 64),
     "walk_steps": (8, 12, 16, 20, 24, 32),
+    "kernel": ("auto",),
+}
+
+POOL_GRID = {
+    "blocks_per_cu": (4,),
+    "threads": (256,),
+    "emit_min": (16, 24, 32),
+    "walk_steps": (8, 16),
+    "kernel": ("pool",),
 }
 
 
 def candidates(grid=None):
-    g = dict(DEFAULT_GRID)
-    g.update(grid or {})
-    keys = list(g)
-    for vals in itertools.product(*(g[k] for k in keys)):
-        yield dict(zip(keys, vals))
+    """The launch shapes to time: DEFAULT_GRID then POOL_GRID, or
+    DEFAULT_GRID with the entries of `grid` replaced."""
+    grids = [DEFAULT_GRID, POOL_GRID] if grid is None else [dict(DEFAULT_GRID, **grid)]
+    for g in grids:
+        keys = list(g)
+        for vals in itertools.product(*(g[k] for k in keys)):
+            yield dict(zip(keys, vals))
 
 
 def autotune(transport, npackets, istar=1, grid=None, packet_offset=1 << 40, repeats=1):

@@ -49,7 +49,7 @@ struct mc_ctx {
     std::vector<int> starIdx;
     bool tables = false;
     int cus = 256;
-    mc_tuning tune = { 4, 256, 64, 20 };    // the autotuner's best on the octant; within 1.3% on the dust and clumpy grids
+    mc_tuning tune = { 4, 256, 16, 16, MC_KERNEL_AUTO };   // the pool kernel where the grid allows    // the autotuner's best on the octant; within 1.3% on the dust and clumpy grids
     // device allocations
     float *d_axis = nullptr, *d_mid = nullptr, *d_dfac = nullptr;
     int *d_active = nullptr;
@@ -353,22 +353,21 @@ extern "C" int mc_upload_tables(mc_ctx *c, const float *opacity, const float *sc
         for (int nu = 0; nu < nb; ++nu)
             if (opacity[(size_t)nu * (nc + 1)] != 0.f)
                 return fail(c, MC_ETABLE, "opacity row 0 (inactive cells) must be zero (bin %d)", nu + 1);
-    const size_t ncell3 = (size_t)G.ncell3;
     float *tmp = nullptr;
     HIPCHK(c, dalloc(&tmp, nt));
     int rc = MC_OK;
     hipError_t e = hipSuccess;
     if (opacity) {
         e = copy_on(c->stream, tmp, opacity, nt * sizeof(float), hipMemcpyHostToDevice);
-        if (e == hipSuccess && !c->d_opac3) e = dalloc(&c->d_opac3, ncell3 * nb);
-        if (e == hipSuccess) e = mcd_densify(tmp, c->d_active, (int)ncell3, nc + 1, nb, c->d_opac3, c->stream);
+        if (e == hipSuccess && !c->d_opac3) e = dalloc(&c->d_opac3, mcd::j3_size(G.js, nb));
+        if (e == hipSuccess) e = mcd_densify(tmp, c->d_active, G.js, nc + 1, nb, c->d_opac3, c->stream);
         c->opac_built = e == hipSuccess;
     }
     if (e == hipSuccess && dust && scaOpac) {
         e = hipStreamSynchronize(c->stream);
         if (e == hipSuccess) e = copy_on(c->stream, tmp, scaOpac, nt * sizeof(float), hipMemcpyHostToDevice);
-        if (e == hipSuccess && !c->d_sca3) e = dalloc(&c->d_sca3, ncell3 * nb);
-        if (e == hipSuccess) e = mcd_densify(tmp, c->d_active, (int)ncell3, nc + 1, nb, c->d_sca3, c->stream);
+        if (e == hipSuccess && !c->d_sca3) e = dalloc(&c->d_sca3, mcd::j3_size(G.js, nb));
+        if (e == hipSuccess) e = mcd_densify(tmp, c->d_active, G.js, nc + 1, nb, c->d_sca3, c->stream);
         c->opac_sca_built = e == hipSuccess;
     }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -1109,6 +1108,7 @@ extern "C" int mc_opacity_setup(mc_ctx *c, const mc_opacity_tables *t)
     mco::OpacArgs &o = c->op;
     o = mco::OpacArgs{};
     o.ncell3 = c->G.ncell3; o.nb = nb; o.nc = c->G.nCells;
+    o.js = c->G.js;
     o.active = c->d_active;
     o.st = c->d_op_st; o.ldst = ldst; o.nslot = nslot;
     o.bin_start = c->d_op_i;
@@ -1133,7 +1133,7 @@ extern "C" int mc_opacity_build(mc_ctx *c, const float *Hden, const float *Te, c
     if (!c) return MC_EINVAL;
     if (!c->opac) return fail(c, MC_ESTATE, "mc_opacity_setup has not been called");
     const int nc = c->G.nCells, nb = c->G.nbins;
-    const size_t nc1 = (size_t)nc + 1, n3 = (size_t)c->G.ncell3;
+    const size_t nc1 = (size_t)nc + 1;
     mco::OpacArgs o = c->op;
     const int nslot = o.nslot, ldst = o.ldst;
     const bool host_state = Hden || Te || Ne || ionDen;
@@ -1177,10 +1177,10 @@ extern "C" int mc_opacity_build(mc_ctx *c, const float *Hden, const float *Te, c
         o.tsub = o.dw + K;
         o.ndust = o.tsub + K;
         o.Tdust = c->d_op_Td;
-        if (!c->d_sca3) HIPCHK(c, dalloc(&c->d_sca3, n3 * nb));
+        if (!c->d_sca3) HIPCHK(c, dalloc(&c->d_sca3, mcd::j3_size(c->G.js, nb)));
         o.sca3 = c->d_sca3;
     }
-    if (!c->d_opac3) HIPCHK(c, dalloc(&c->d_opac3, n3 * nb));
+    if (!c->d_opac3) HIPCHK(c, dalloc(&c->d_opac3, mcd::j3_size(c->G.js, nb)));
     o.opac3 = c->d_opac3;
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     if (!host_state)
@@ -1196,12 +1196,12 @@ extern "C" int mc_opacity_build(mc_ctx *c, const float *Hden, const float *Te, c
     if (opacity || scaOpac) {
         if (!c->d_op_out) HIPCHK(c, dalloc(&c->d_op_out, nc1 * nb));
         if (opacity) {
-            HIPCHK(c, mco_launch_gather(c->d_opac3, n3, c->d_cell_xyz, nc, nb, c->d_op_out, c->stream));
+            HIPCHK(c, mco_launch_gather(c->d_opac3, c->G.js, c->d_cell_xyz, nc, nb, c->d_op_out, c->stream));
             HIPCHK(c, hipMemcpyAsync(opacity, c->d_op_out, nc1 * nb * sizeof(float), hipMemcpyDeviceToHost, c->stream));
             HIPCHK(c, hipStreamSynchronize(c->stream));
         }
         if (scaOpac) {
-            if (dust) HIPCHK(c, mco_launch_gather(c->d_sca3, n3, c->d_cell_xyz, nc, nb, c->d_op_out, c->stream));
+            if (dust) HIPCHK(c, mco_launch_gather(c->d_sca3, c->G.js, c->d_cell_xyz, nc, nb, c->d_op_out, c->stream));
             else HIPCHK(c, hipMemsetAsync(c->d_op_out, 0, nc1 * nb * sizeof(float), c->stream));
             HIPCHK(c, hipMemcpyAsync(scaOpac, c->d_op_out, nc1 * nb * sizeof(float), hipMemcpyDeviceToHost, c->stream));
         }
@@ -1819,13 +1819,34 @@ extern "C" int mc_mlp_train(mc_ctx *c, const float *X, const float *T, const flo
     return MC_OK;
 }
 
+// Grids the pool kernel takes (its packed cell indices and bins).
+static bool pool_fits(const mcd::Grid &G)
+{
+    return G.nx <= 2047 && G.ny <= 2047 && G.nz <= 1023 && G.nbins <= 65535;
+}
+
+// The kernel a launch shape runs (MC_KERNEL_AUTO: two packets per lane at
+// most 3 workgroups of 256 per CU, else the pool kernel where the grid
+// allows it, else one packet per lane).
+static int transport_kernel_of(const mc_tuning &t, const mcd::Grid &G)
+{
+    switch (t.kernel) {
+    case MC_KERNEL_ONE: return MCD_KERNEL_ONE;
+    case MC_KERNEL_SPARE: return MCD_KERNEL_SPARE;
+    case MC_KERNEL_POOL: return MCD_KERNEL_POOL;
+    default: return t.blocks_per_cu <= 3 ? MCD_KERNEL_SPARE : pool_fits(G) ? MCD_KERNEL_POOL : MCD_KERNEL_ONE;
+    }
+}
+
 extern "C" int mc_set_tuning(mc_ctx *c, const mc_tuning *t)
 {
     if (!c || !t) return MC_EINVAL;
     if (t->blocks_per_cu < 1 || t->blocks_per_cu > 32 || t->threads < 64 || t->threads > 256 ||
         t->threads % 64 || t->emit_min < 1 || t->emit_min > 64 || t->walk_steps < 1 ||
-        t->walk_steps > 64)
+        t->walk_steps > 64 || t->kernel < MC_KERNEL_AUTO || t->kernel > MC_KERNEL_POOL)
         return fail(c, MC_EINVAL, "tuning out of range");
+    if (t->kernel == MC_KERNEL_POOL && !pool_fits(c->G))
+        return fail(c, MC_EUNSUP, "the pool kernel takes grids of at most 2047 x 2047 x 1023 cells and 65535 bins");
     c->tune = *t;
     return MC_OK;
 }
@@ -1886,7 +1907,7 @@ extern "C" int mc_transport_run(mc_ctx *c, int32_t istar, int64_t npackets, uint
     const int blocks = c->tune.blocks_per_cu * c->cus * (256 / c->tune.threads);
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     if (npackets > 0)
-        HIPCHK(c, mcd_launch_transport(c->G, RA, blocks, c->tune.threads, c->tune.blocks_per_cu <= 3, c->stream));
+        HIPCHK(c, mcd_launch_transport(c->G, RA, blocks, c->tune.threads, transport_kernel_of(c->tune, c->G), c->stream));
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     unsigned long long h[1 + kCounts];
     int herr = 0;

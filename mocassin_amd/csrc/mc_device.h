@@ -41,7 +41,10 @@ struct View {
 // packet's next segment (same frequency, a neighbouring cell) stays in the
 // brick after half of its steps, against 7/24 for 8 points along x.  One
 // wavefront reading a tile's spectra (64 lanes, one point each) streams
-// contiguous memory.
+// contiguous memory.  The walk's opacity tables use the same layout in
+// FP32: a 64-byte line is a 4x2x2 block of points at one frequency, so a
+// packet's next opacity gather falls in the line of its last one after
+// more than half of its steps (a third with the lattice x-major).
 constexpr int kJTile = 64;
 struct J3Shape {
     int nx, ny, nz;          // lattice
@@ -99,8 +102,8 @@ struct Grid {
     const float *dfac;       // per-axis cell widths /1e15 (getVolume factors)
     int ncell3;              // nx*ny*nz
     const int *active;       // active(x,y,z), x fastest
-    const float *opac3;      // [nu][x,y,z] dense: opacity(active(x,y,z),nu), row 0 = 0
-    const float *sca3;       // [nu][x,y,z] dense scaOpac or null
+    const float *opac3;      // J3 layout, fp32: opacity(active(x,y,z),nu); inactive points 0
+    const float *sca3;       // J3 layout, fp32: scaOpac, or null
     const float *recCDF;     // [cell][nu]   (one row per cell)
     const unsigned short *recGuide;  // [cell][kGuide+1] guide table of recCDF rows
     const float *totalLines; // [cell]
@@ -290,7 +293,7 @@ struct Walker {
     float absTau, passProb;
     // outputs of the last step
     float seg;             // dS or dlLoc of the segment just walked
-    int segIdx;            // dense index (x-1)+nx*((y-1)+ny*(z-1)) of its cell
+    unsigned segPt;        // J3 lattice point of its cell (j3_point)
     int segX, segY, segZ;  // its cell indices (1-based)
 };
 
@@ -341,9 +344,9 @@ __device__ __forceinline__ int walk_step(const Grid &G, const Axes &A, Walker &W
     // the cell's opacity, issued first so that its gather overlaps the wall
     // distances (the indices were just checked); a packet on a wall is
     // stepped into the next cell there, and then the gather is repeated
-    const int idx0 = (xP - 1) + nx * ((yP - 1) + ny * (zP - 1));
+    const unsigned p0 = j3_point(G.js, xP - 1, yP - 1, zP - 1);
     float kappa = 0.f;
-    if (!ZERO_OPACITY) kappa = G.opac3[(size_t)(nuP - 1) * (size_t)G.ncell3 + idx0];
+    if (!ZERO_OPACITY) kappa = G.opac3[j3_index(G.nbins, nuP - 1, p0)];
 
     if (sym) {                                                   // 1248-1261
         if (rx <= AX(A.x, 1)) { if (vx < 0.f) vx = -vx; rx = AX(A.x, 1); }
@@ -378,13 +381,13 @@ __device__ __forceinline__ int walk_step(const Grid &G, const Axes &A, Walker &W
 
     // dense opacity: inactive cells hold row 0 (= 0), so "active > 0" of the
     // interaction test (1517) holds whenever tau can exceed passProb
-    const int idx = (xP - 1) + nx * ((yP - 1) + ny * (zP - 1));
-    if (!ZERO_OPACITY && idx != idx0) kappa = G.opac3[(size_t)(nuP - 1) * (size_t)G.ncell3 + idx];
+    const unsigned p = j3_point(G.js, xP - 1, yP - 1, zP - 1);
+    if (!ZERO_OPACITY && p != p0) kappa = G.opac3[j3_index(G.nbins, nuP - 1, p)];
     const float tauCell = dS * kappa;
     const float dV = AX(A.dx, xP) * AX(A.dy, yP) * AX(A.dz, zP);  // 1442-1514
     kappa_out = kappa;
     dV_out = dV;
-    W.segIdx = idx;
+    W.segPt = p;
     W.segX = xP; W.segY = yP; W.segZ = zP;
 
     if (!ZERO_OPACITY && (W.absTau + tauCell > W.passProb)) {

@@ -40,11 +40,14 @@ struct BatchArgs {
 
 hipError_t mcd_launch_batch(const mcd::BatchArgs &B, size_t lds_bytes, int blocks, hipStream_t s);
 size_t mcd_lds_bytes(int nx, int ny, int nz, int nbins);
-// two: two packets per lane (at most 3 wavefronts per SIMD, which that
-// kernel's registers allow), else one
-hipError_t mcd_launch_transport(const mcd::Grid &G, const mcd::RunArgs &RA, int blocks, int threads, bool two,
+// The transport kernels: one packet per lane; two (at most 3 wavefronts
+// per SIMD, which that kernel's registers allow); one per lane with a pool
+// of 32 packets per wavefront in LDS.  The pool kernel needs grids of at
+// most 2047 x 2047 x 1023 cells and 65535 bins.
+enum { MCD_KERNEL_ONE = 1, MCD_KERNEL_SPARE = 2, MCD_KERNEL_POOL = 3 };
+hipError_t mcd_launch_transport(const mcd::Grid &G, const mcd::RunArgs &RA, int blocks, int threads, int kernel,
                                 hipStream_t s);
-hipError_t mcd_densify(const float *compact, const int *active, int ncell3, int nc1, int nbins,
+hipError_t mcd_densify(const float *compact, const int *active, const mcd::J3Shape &js, int nc1, int nbins,
                        float *dense, hipStream_t s);
 hipError_t mcd_guides(const float *cdf, int rows, int nbins, unsigned short *guide, hipStream_t s);
 hipError_t mcd_compact(const double *J3, const int *active, const mcd::J3Shape &js, int nc1, int nbins,

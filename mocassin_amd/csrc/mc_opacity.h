@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include "mc_device.h"
 
 namespace mco {
 
@@ -20,6 +21,7 @@ struct Ent {
 
 struct OpacArgs {
     int ncell3, nb, nc;
+    mcd::J3Shape js;                 // the tables' layout (J3 tiles of 64 lattice points)
     const int *active;               // lattice point -> cell (0 inactive)
     // per-cell state, FP32: st[cell * ldst + 0..2] = Te, Ne, Hden; then the
     // ionic fractions of slot q at st[cell * ldst + 3 + q]
@@ -41,7 +43,7 @@ struct OpacArgs {
     const float *tsub;               // [ndrow] sublimation temperature of the row's species
     const float *ndust;              // (0:nc)
     const float *Tdust;              // (0:nsMax, 0:nSizes, 0:nc), FP32
-    float *opac3, *sca3;             // [nu][xyz] outputs (sca3 only with grains)
+    float *opac3, *sca3;             // outputs in the J3 layout (sca3 only with grains)
 };
 
 }  // namespace mco
@@ -54,6 +56,6 @@ hipError_t mco_launch_state(const double *Te, const double *Ne, const double *Hd
                             int ldst, hipStream_t s);
 // FP64 Tdust -> FP32
 hipError_t mco_launch_f32(const double *in, size_t n, float *out, hipStream_t s);
-// (0:nc, 1:nb) column-major copy of a [nu][xyz] table (row 0 zero)
-hipError_t mco_launch_gather(const float *t3, size_t n3, const int *cell_xyz, int nc, int nb, float *out,
-                             hipStream_t s);
+// (0:nc, 1:nb) column-major copy of a J3-layout table (row 0 zero)
+hipError_t mco_launch_gather(const float *t3, const mcd::J3Shape &js, const int *cell_xyz, int nc, int nb,
+                             float *out, hipStream_t s);

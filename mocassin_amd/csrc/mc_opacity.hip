@@ -106,8 +106,8 @@ __global__ void __launch_bounds__(64) opacity_kernel(OpacArgs a)
     // kSplit wavefronts share a tile of 64 lattice points, each writing one
     // contiguous range of bins (more wavefronts in flight to hide latency)
     const int tile = blockIdx.x / kSplit, part = blockIdx.x % kSplit;
-    const int xyz = tile * 64 + lane;
-    const bool in = xyz < a.ncell3;
+    const int xyz = mcd::j3_lattice(a.js, (unsigned)(tile * 64 + lane));   // -1: a padding point
+    const bool in = xyz >= 0;
     const int cell = in ? a.active[xyz] : 0;
     float *dens = lds, *wr = lds + a.nslot * 64;
     float ff1 = 0.f;
@@ -146,8 +146,6 @@ __global__ void __launch_bounds__(64) opacity_kernel(OpacArgs a)
             }
         }
     }
-    if (!in) return;
-    const size_t n3 = (size_t)a.ncell3;
     const int per = (a.nb + kSplit - 1) / kSplit;
     const int i0 = part * per, i1 = min(a.nb, i0 + per);
     const Ent *ent = a.ent;
@@ -187,8 +185,9 @@ __global__ void __launch_bounds__(64) opacity_kernel(OpacArgs a)
                 acc = acc + (sca + ab);
             }
         }
-        a.opac3[(size_t)i * n3 + xyz] = acc;
-        if (a.sca3) a.sca3[(size_t)i * n3 + xyz] = sca;
+        const size_t o = ((size_t)tile * a.nb + i) * 64 + lane;   // padding points get 0
+        a.opac3[o] = acc;
+        if (a.sca3) a.sca3[o] = sca;
     }
 }
 
@@ -216,7 +215,8 @@ __global__ void f32_kernel(const double *in, size_t n, float *out)
 
 hipError_t mco_launch_opacity(const mco::OpacArgs &a, size_t lds_bytes, hipStream_t s)
 {
-    hipLaunchKernelGGL(mco::opacity_kernel, dim3((a.ncell3 + 63) / 64 * mco::kSplit), dim3(64), lds_bytes, s, a);
+    const unsigned tiles = (unsigned)(mcd::j3_points(a.js) / 64);
+    hipLaunchKernelGGL(mco::opacity_kernel, dim3(tiles * mco::kSplit), dim3(64), lds_bytes, s, a);
     return hipGetLastError();
 }
 
@@ -237,20 +237,21 @@ hipError_t mco_launch_f32(const double *in, size_t n, float *out, hipStream_t s)
 }
 
 namespace mco {
-__global__ void gather_kernel(const float *t3, size_t n3, const int *cell_xyz, int nc, int nb, float *out)
+__global__ void gather_kernel(const float *t3, mcd::J3Shape js, const int *cell_xyz, int nc, int nb, float *out)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t nc1 = (size_t)nc + 1;
     if (i >= nc1 * nb) return;
     const size_t cell = i % nc1, nu = i / nc1;
-    out[i] = cell == 0 ? 0.f : t3[nu * n3 + (size_t)cell_xyz[cell]];
+    out[i] = cell == 0 ? 0.f : t3[mcd::j3_index(nb, (int)nu, mcd::j3_point_of(js, cell_xyz[cell]))];
 }
 }  // namespace mco
 
-hipError_t mco_launch_gather(const float *t3, size_t n3, const int *cell_xyz, int nc, int nb, float *out, hipStream_t s)
+hipError_t mco_launch_gather(const float *t3, const mcd::J3Shape &js, const int *cell_xyz, int nc, int nb,
+                             float *out, hipStream_t s)
 {
     const size_t n = ((size_t)nc + 1) * nb;
-    hipLaunchKernelGGL(mco::gather_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, t3, n3, cell_xyz, nc, nb,
-                       out);
+    hipLaunchKernelGGL(mco::gather_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, t3, js, cell_xyz, nc,
+                       nb, out);
     return hipGetLastError();
 }

@@ -356,7 +356,7 @@ __device__ __forceinline__ void walk_pass(Lane &L, const Grid &G, const Axes &A,
         int r = walk_step<false>(G, A, L.W, L.nuP, kappa, dV, err);
         if (r == ST_CONTINUE || r == ST_ESCAPED) {
             jv = L.W.seg * M.dE / dV;
-            jpt = j3_point(G.js, L.W.segX - 1, L.W.segY - 1, L.W.segZ - 1);
+            jpt = L.W.segPt;
             jnu = L.nuP;
             seg = true;
             if (r == ST_ESCAPED) {
@@ -369,7 +369,7 @@ __device__ __forceinline__ void walk_pass(Lane &L, const Grid &G, const Axes &A,
             L.phase = PH_IDLE;
         } else if (r == ST_ABSORBED) {
             jv = L.W.seg * M.dE / dV;
-            jpt = j3_point(G.js, L.W.segX - 1, L.W.segY - 1, L.W.segZ - 1);
+            jpt = L.W.segPt;
             jnu = L.nuP;
             seg = true;
             float rx = L.W.rx / 1.e10f, ry = L.W.ry / 1.e10f, rz = L.W.rz / 1.e10f;
@@ -378,8 +378,7 @@ __device__ __forceinline__ void walk_pass(Lane &L, const Grid &G, const Axes &A,
                 escaped = true;
                 L.phase = PH_IDLE;
             } else if (M.dust) {
-                const size_t ti = (size_t)(L.nuP - 1) * (size_t)G.ncell3 + L.W.segIdx;
-                float probSca = G.sca3[ti] / kappa;
+                float probSca = G.sca3[j3_index(G.nbins, L.nuP - 1, L.W.segPt)] / kappa;
                 float u = 1.f - draw(L.R, G);
                 if (u > probSca) {
                     absorbed = true;
@@ -496,6 +495,157 @@ __device__ __forceinline__ void advance_spare(Lane &L, Lane &S, const Grid &G, c
     }
 }
 
+// ------------------------------------------------ packet pool (per wave, LDS)
+// A wavefront keeps a pool of P packets in LDS beside its 64 lanes: packets
+// that walk (the walker stack, slots 0 .. nw-1 from the bottom) and packets
+// that wait for a diffuse emission (the pending stack, slots P-npd .. P-1
+// from the top).  A lane whose flight ends takes a walker from the pool,
+// leaving its own packet on the pending stack if it waits for an emission;
+// an emission round moves walkers into the pool, so that the emission pass
+// runs on a full wavefront, and the walkers come back as flights end.  The
+// walk passes then stay nearly full between emissions.  Word k of slot s is
+// at pw[k * P + s]; the cell indices are packed in one word (11, 11 and 10
+// bits: the host runs this kernel on grids of at most 2047 x 2047 x 1023
+// cells) and the frequency bin, run count and phase in another (16, 14 and
+// 2 bits).  The Rng's stream is the launch's source, not stored.
+constexpr int kPoolWords = 23;
+// Slots per wavefront: with 8-deep J rings, 4 workgroups of 256 per CU keep
+// 32 slots within the CU's 160 KB of LDS (8.5 KB per wavefront, plus the
+// staged tables); 48 slots with 4-deep rings ran slower (the rings merge
+// fewer J adds per instruction).
+constexpr int kPoolSlots = 32;
+
+template <int P>
+__device__ __forceinline__ void pool_put(uint32_t *pw, int s, const Lane &L)
+{
+    uint32_t *q = pw + s;
+    q[0 * P] = L.R.plo; q[1 * P] = L.R.phi; q[2 * P] = L.R.k; q[3 * P] = L.R.blk;
+    q[4 * P] = L.R.w0; q[5 * P] = L.R.w1; q[6 * P] = L.R.w2; q[7 * P] = L.R.w3;
+    q[8 * P] = __float_as_uint(L.W.rx); q[9 * P] = __float_as_uint(L.W.ry); q[10 * P] = __float_as_uint(L.W.rz);
+    q[11 * P] = __float_as_uint(L.W.vx); q[12 * P] = __float_as_uint(L.W.vy); q[13 * P] = __float_as_uint(L.W.vz);
+    q[14 * P] = (uint32_t)L.W.xP | ((uint32_t)L.W.yP << 11) | ((uint32_t)L.W.zP << 22);
+    q[15 * P] = __float_as_uint(L.W.absTau); q[16 * P] = __float_as_uint(L.W.passProb);
+    q[17 * P] = __float_as_uint(L.d0x); q[18 * P] = __float_as_uint(L.d0y); q[19 * P] = __float_as_uint(L.d0z);
+    q[20 * P] = (uint32_t)L.nuP | ((uint32_t)L.runs << 16) | ((uint32_t)L.phase << 30);
+    q[21 * P] = (uint32_t)L.origin; q[22 * P] = (uint32_t)L.steps;
+}
+
+template <int P>
+__device__ __forceinline__ void pool_get(const uint32_t *pw, int s, Lane &L, uint32_t stream)
+{
+    const uint32_t *q = pw + s;
+    L.R.plo = q[0 * P]; L.R.phi = q[1 * P]; L.R.k = q[2 * P]; L.R.blk = q[3 * P];
+    L.R.w0 = q[4 * P]; L.R.w1 = q[5 * P]; L.R.w2 = q[6 * P]; L.R.w3 = q[7 * P];
+    L.R.stream = stream;
+    L.W.rx = __uint_as_float(q[8 * P]); L.W.ry = __uint_as_float(q[9 * P]); L.W.rz = __uint_as_float(q[10 * P]);
+    L.W.vx = __uint_as_float(q[11 * P]); L.W.vy = __uint_as_float(q[12 * P]); L.W.vz = __uint_as_float(q[13 * P]);
+    const uint32_t c = q[14 * P];
+    L.W.xP = (int)(c & 2047u); L.W.yP = (int)((c >> 11) & 2047u); L.W.zP = (int)(c >> 22);
+    L.W.absTau = __uint_as_float(q[15 * P]); L.W.passProb = __uint_as_float(q[16 * P]);
+    L.d0x = __uint_as_float(q[17 * P]); L.d0y = __uint_as_float(q[18 * P]); L.d0z = __uint_as_float(q[19 * P]);
+    const uint32_t f = q[20 * P];
+    L.nuP = (int)(f & 65535u); L.runs = (int)((f >> 16) & 16383u); L.phase = (int)(f >> 30);
+    L.origin = (int)q[21 * P]; L.steps = (int)q[22 * P];
+}
+
+// lanes below this one in m
+__device__ __forceinline__ int lane_rank(uint64_t m, int lane)
+{
+    return __popcll(m & ((1ull << lane) - 1ull));
+}
+
+// Orders the wavefront's LDS accesses on either side (the compiler may not
+// move them across; the LDS serves one wavefront's instructions in order).
+__device__ __forceinline__ void wave_lds_order()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Lanes that do not walk take walkers from the pool (the first nw of them in
+// lane order); a taker whose packet waits for an emission leaves it on the
+// pending stack first, if the pool has a free slot for it.  The pending
+// slots written lie in the free region, the walker slots read below it.
+template <int P>
+__device__ __forceinline__ void pool_settle(Lane &L, uint32_t *pw, int &nw, int &npd, int lane, uint32_t stream)
+{
+    if (nw == 0) return;
+    const bool nonw = L.phase != PH_WALK;
+    if (!__any(nonw)) return;
+    const int F = P - nw - npd;
+    const bool pend = pending(L);
+    const uint64_t mp = __ballot(pend);
+    const bool elig = nonw && (!pend || lane_rank(mp, lane) < F);
+    const uint64_t me = __ballot(elig);
+    const int re = lane_rank(me, lane);
+    const bool take = elig && re < nw;
+    const uint64_t mt = __ballot(take);
+    const uint64_t mtp = mt & mp;
+    if (mtp) {
+        if (take && pend) pool_put<P>(pw, P - npd - 1 - lane_rank(mtp, lane), L);
+        npd += __popcll(mtp);
+        wave_lds_order();
+    }
+    if (take) pool_get<P>(pw, nw - 1 - re, L, stream);
+    nw -= __popcll(mt);
+    wave_lds_order();
+}
+
+// An emission round of the pool kernel: walkers move to the pool (as many
+// as the candidates for an emission exceed the lanes that do not walk, and
+// as fit), idle lanes take packets from the pending stack, then fresh
+// packets (one counter atomic per wavefront), and the emission pass runs.
+template <int P>
+__device__ __forceinline__ void pool_emission_round(Lane &L, uint32_t *pw, int &nw, int &npd, bool &exhausted,
+                                                    const Grid &G, const RunArgs &RA, const Model &M,
+                                                    const float *spec, const unsigned short *spec_guide,
+                                                    WaveCounts &C, double &qphot, int &err, int lane)
+{
+    const bool walk = L.phase == PH_WALK;
+    const uint64_t mw = __ballot(walk);
+    const int nwalk = __popcll(mw);
+    const int npl = (int)wcount(pending(L));
+    const int cand = npl + npd + (exhausted ? 0 : 64);
+    const int k = max(0, min(min(nwalk, P - nw - npd), cand - (64 - nwalk)));
+    if (k > 0) {
+        const int rw = lane_rank(mw, lane);
+        if (walk && rw < k) {
+            pool_put<P>(pw, nw + rw, L);
+            L.phase = PH_IDLE;
+        }
+        nw += k;
+        wave_lds_order();
+    }
+    if (npd > 0) {
+        const bool idle = L.phase == PH_IDLE;
+        const uint64_t mi = __ballot(idle);
+        const int ri = lane_rank(mi, lane);
+        if (idle && ri < npd) pool_get<P>(pw, P - npd + ri, L, (uint32_t)RA.istar);
+        npd -= min(__popcll(mi), npd);
+        wave_lds_order();
+    }
+    const bool need = (L.phase == PH_IDLE) && !exhausted;
+    const uint64_t m = __ballot(need);
+    if (m) {
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(RA.next_packet, (unsigned long long)__popcll(m));
+        base = __shfl(base, leader);
+        bool over = false;
+        if (need) {
+            const unsigned long long id = base + (unsigned long long)lane_rank(m, lane);
+            if (id < (unsigned long long)RA.npackets) {
+                rng_start(L.R, RA.packet_offset + id, (uint32_t)RA.istar);
+                L.phase = PH_EMIT_STELLAR;
+            } else {
+                over = true;
+            }
+        }
+        exhausted = __any(over);
+    }
+    if (__any(pending(L))) emission_pass(L, G, RA, M, spec, spec_guide, C, qphot, err);
+}
+
 // Add a wavefront's counters to its model's totals and clear them.
 __device__ __forceinline__ void flush_counts(const RunArgs &RA, WaveCounts &C, double &qphot)
 {
@@ -579,6 +729,56 @@ __global__ void __launch_bounds__(256) transport_kernel(Grid G, RunArgs RA)
         }
     }
     jq_flush<kJDepth>(G, Q, lane_id);
+    flush_counts(RA, C, qphot);
+}
+
+// The single-model kernel with a packet pool of P slots per wavefront in
+// LDS (one packet per lane in registers, J rings of D updates).  An
+// emission round runs when at least emit_min lanes do not walk (or none
+// does); every walk pass is followed by a settle from the pool.
+template <int D, int P>
+__global__ void __launch_bounds__(256) pool_kernel(Grid G, RunArgs RA)
+{
+    extern __shared__ float lds[];
+    Axes A;
+    float *spec = nullptr;
+    unsigned short *spec_guide = nullptr;
+    const int nwaves = (int)(blockDim.x >> 6);
+    const int wave = (int)(threadIdx.x >> 6);
+    uint32_t *pools = reinterpret_cast<uint32_t *>(lds + (size_t)nwaves * JRing<D>::kFloats);
+    stage_model(G, RA, reinterpret_cast<float *>(pools + (size_t)nwaves * P * kPoolWords), A, spec, spec_guide);
+    uint32_t *pw = pools + (size_t)wave * P * kPoolWords;
+    const int lane_id = threadIdx.x & 63;
+    JQ Q = jq_at<D>(lds, wave);
+    const Model M = model_of(G, RA);
+    const uint32_t stream = (uint32_t)RA.istar;
+
+    Lane L;
+    L.phase = PH_IDLE;
+    L.runs = 0; L.steps = 0; L.nuP = 1; L.origin = 0;
+    int nw = 0, npd = 0;
+    bool exhausted = false;
+    WaveCounts C = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double qphot = 0.0;
+    int err = 0;
+
+    for (;;) {
+        pool_settle<P>(L, pw, nw, npd, lane_id, stream);
+        const int nwalk = (int)wcount(L.phase == PH_WALK);
+        if (nwalk == 0 && nw == 0 && npd == 0 && exhausted && !__any(pending(L))) break;
+        if (64 - nwalk >= RA.emit_min || nwalk == 0)
+            pool_emission_round<P>(L, pw, nw, npd, exhausted, G, RA, M, spec, spec_guide, C, qphot, err, lane_id);
+        for (int s = 0; s < RA.walk_steps; ++s) {
+            if (!__any(L.phase == PH_WALK)) break;
+            walk_pass<D>(L, G, A, M, C, err, Q, lane_id);
+            pool_settle<P>(L, pw, nw, npd, lane_id, stream);
+        }
+        if (err) {
+            atomicMax(RA.error, err);
+            err = 0;
+        }
+    }
+    jq_flush<D>(G, Q, lane_id);
     flush_counts(RA, C, qphot);
 }
 
@@ -678,15 +878,19 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) ba
 }
 
 // ------------------------------------------------------- table preparation
-// opac3[nu][xyz] = opacity(active(xyz), nu)   (reference layout in: [nu][cell])
-__global__ void densify_kernel(const float *compact, const int *active, int ncell3, int nc1,
-                               int nbins, float *dense)
+// opac3 in the J3 layout: the value of lattice point p at bin nu is
+// opacity(active(xyz(p)), nu) (reference layout in: [nu][cell]); padding
+// points hold 0.  Written in layout order (coalesced stores).
+__global__ void densify_kernel(const float *compact, const int *active, J3Shape js, int nc1, int nbins,
+                               float *dense)
 {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t tot = (size_t)ncell3 * nbins;
+    const size_t tot = j3_size(js, nbins);
     for (; i < tot; i += (size_t)gridDim.x * blockDim.x) {
-        int nu = (int)(i / ncell3), xyz = (int)(i % ncell3);
-        dense[i] = compact[(size_t)nu * nc1 + active[xyz]];
+        const size_t tile = i / ((size_t)nbins * kJTile);
+        const int nu = (int)((i / kJTile) % (size_t)nbins);
+        const int xyz = j3_lattice(js, (unsigned)(tile * kJTile + i % kJTile));
+        dense[i] = xyz < 0 ? 0.f : compact[(size_t)nu * nc1 + active[xyz]];
     }
 }
 
@@ -732,15 +936,31 @@ size_t mcd_lds_bytes(int nx, int ny, int nz, int nbins)
     return sizeof(float) * (mcd::JRing<mcd::kJDepthBatch>::kFloats + mcd::lds_floats(nx, ny, nz, nbins));
 }
 
-hipError_t mcd_launch_transport(const mcd::Grid &G, const mcd::RunArgs &RA, int blocks, int threads, bool two,
+template <int D, int P>
+static size_t pool_lds(const mcd::Grid &G, int threads)
+{
+    return sizeof(float) * ((size_t)(threads / 64) * (mcd::JRing<D>::kFloats + (size_t)P * mcd::kPoolWords) +
+                            mcd::lds_floats(G.nx, G.ny, G.nz, G.nbins));
+}
+
+hipError_t mcd_launch_transport(const mcd::Grid &G, const mcd::RunArgs &RA, int blocks, int threads, int kernel,
                                 hipStream_t s)
 {
-    size_t lds = sizeof(float) * ((size_t)(threads / 64) * mcd::JRing<mcd::kJDepth>::kFloats +
-                                  mcd::lds_floats(G.nx, G.ny, G.nz, G.nbins));
-    if (two)
+    const size_t lds = sizeof(float) * ((size_t)(threads / 64) * mcd::JRing<mcd::kJDepth>::kFloats +
+                                        mcd::lds_floats(G.nx, G.ny, G.nz, G.nbins));
+    switch (kernel) {
+    case MCD_KERNEL_SPARE:
         hipLaunchKernelGGL(mcd::transport_kernel<true>, dim3(blocks), dim3(threads), lds, s, G, RA);
-    else
+        break;
+    case MCD_KERNEL_POOL: {
+        const size_t pl = pool_lds<mcd::kJDepth, mcd::kPoolSlots>(G, threads);
+        hipLaunchKernelGGL((mcd::pool_kernel<mcd::kJDepth, mcd::kPoolSlots>), dim3(blocks), dim3(threads), pl, s,
+                           G, RA);
+        break;
+    }
+    default:
         hipLaunchKernelGGL(mcd::transport_kernel<false>, dim3(blocks), dim3(threads), lds, s, G, RA);
+    }
     return hipGetLastError();
 }
 
@@ -755,11 +975,10 @@ hipError_t mcd_launch_replay(const mcd::Grid &G, int n, const float *pos, const 
     return hipGetLastError();
 }
 
-hipError_t mcd_densify(const float *compact, const int *active, int ncell3, int nc1, int nbins,
+hipError_t mcd_densify(const float *compact, const int *active, const mcd::J3Shape &js, int nc1, int nbins,
                        float *dense, hipStream_t s)
 {
-    hipLaunchKernelGGL(mcd::densify_kernel, dim3(2048), dim3(256), 0, s, compact, active, ncell3,
-                       nc1, nbins, dense);
+    hipLaunchKernelGGL(mcd::densify_kernel, dim3(2048), dim3(256), 0, s, compact, active, js, nc1, nbins, dense);
     return hipGetLastError();
 }
 

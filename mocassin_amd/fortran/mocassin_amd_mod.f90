@@ -20,7 +20,7 @@ module mocassin_amd_mod
     use, intrinsic :: iso_c_binding
     implicit none
 
-    integer(c_int32_t), parameter :: MC_ABI_VERSION = 2
+    integer(c_int32_t), parameter :: MC_ABI_VERSION = 3
     integer(c_int32_t), parameter :: MC_SYMMETRIC = 1, MC_DUST = 2, MC_GAS = 4, &
                                      MC_ISOTROPIC = 8, MC_MULTISTARS = 16
     integer(c_int), parameter :: MC_OK = 0, MC_EINVAL = 1, MC_ENODEV = 2, MC_ENOMEM = 3, &
@@ -44,8 +44,10 @@ module mocassin_amd_mod
     end type mc_counters
 
     type, bind(C) :: mc_tuning
-        integer(c_int32_t) :: blocks_per_cu, threads, emit_min, walk_steps
+        integer(c_int32_t) :: blocks_per_cu, threads, emit_min, walk_steps, kernel
     end type mc_tuning
+    integer(c_int32_t), parameter :: MC_KERNEL_AUTO = 0, MC_KERNEL_ONE = 1, MC_KERNEL_SPARE = 2, &
+                                     MC_KERNEL_POOL = 3
 
     integer(c_int32_t), parameter :: MC_SOLVER_REFERENCE = 0, MC_SOLVER_SECANT = 1
 

@@ -473,11 +473,16 @@ class Transport:
                                              ln.ctypes.data, st.ctypes.data, int(max_len)))
         return seq, ds, ln, st
 
-    def set_tuning(self, blocks_per_cu=None, threads=None, emit_min=None, walk_steps=None):
+    def set_tuning(self, blocks_per_cu=None, threads=None, emit_min=None, walk_steps=None, kernel=None):
+        """The transport launch shape (mc_tuning); kernel is a code or one of
+        "auto", "one", "spare", "pool".  Fields left None keep
+        their current values."""
         t = _capi.McTuning()
         self._check(self._lib.mc_get_tuning(self._h, C.byref(t)))
+        if isinstance(kernel, str):
+            kernel = _capi.KERNELS[kernel]
         for k, v in (("blocks_per_cu", blocks_per_cu), ("threads", threads),
-                     ("emit_min", emit_min), ("walk_steps", walk_steps)):
+                     ("emit_min", emit_min), ("walk_steps", walk_steps), ("kernel", kernel)):
             if v is not None:
                 setattr(t, k, int(v))
         self._check(self._lib.mc_set_tuning(self._h, C.byref(t)))

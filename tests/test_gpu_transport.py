@@ -25,9 +25,12 @@ from conftest import (golden, grid_from_fixture, j_map_from_segments, oracle_pro
 
 pytestmark = pytest.mark.gpu
 
-# The two transport kernels: one packet per lane (the library default,
-# 4 workgroups of 256 per CU) and two (at most 3 per CU).
-KERNELS = {"one": None, "two": {"blocks_per_cu": 3, "threads": 256, "emit_min": 64, "walk_steps": 16}}
+# The transport kernels: one packet per lane, two (the automatic choice at
+# most 3 workgroups of 256 per CU), and one per lane with a pool of 32
+# packets per wavefront in LDS (the library default, 4 per CU).
+KERNELS = {"one": {"blocks_per_cu": 4, "threads": 256, "emit_min": 64, "walk_steps": 20, "kernel": "one"},
+           "two": {"blocks_per_cu": 3, "threads": 256, "emit_min": 64, "walk_steps": 16, "kernel": "auto"},
+           "pool": {"blocks_per_cu": 4, "threads": 256, "emit_min": 16, "walk_steps": 4, "kernel": "pool"}}
 
 
 def transport_from_problem(P, seed, **kw):
@@ -227,10 +230,13 @@ def c2_problem():
                      recPDF=p["recPDF"], totalLines=p["totalLines"])
 
 
-def test_full_size_bookkeeping_and_sampled_parity(c2_problem):
+@pytest.mark.parametrize("kernel", ["one", "two", "pool"])
+def test_full_size_bookkeeping_and_sampled_parity(c2_problem, kernel):
     P = c2_problem
-    print("c2 full size", P.nCells, P.nbins, flush=True)
+    print("c2 full size", kernel, P.nCells, P.nbins, flush=True)
     T = transport_from_problem(P, 2024)
+    if KERNELS[kernel]:
+        T.set_tuning(**KERNELS[kernel])
     n = 2_000_000
     c = T.energy_packet_driver(1, n, 0)
     J, E = T.fetch()
@@ -255,9 +261,13 @@ def test_results_independent_of_launch_shape():
     n, seed = int(g["npackets"]), int(g["seed"])
     T = transport_from_problem(P, seed)
     ref = None
-    for tune in ({"blocks_per_cu": 1, "threads": 64, "emit_min": 1, "walk_steps": 1},
-                 {"blocks_per_cu": 8, "threads": 256, "emit_min": 16, "walk_steps": 4},
-                 {"blocks_per_cu": 16, "threads": 128, "emit_min": 64, "walk_steps": 64}):
+    for tune in ({"blocks_per_cu": 1, "threads": 64, "emit_min": 1, "walk_steps": 1, "kernel": "auto"},
+                 {"blocks_per_cu": 8, "threads": 256, "emit_min": 16, "walk_steps": 4, "kernel": "auto"},
+                 {"blocks_per_cu": 16, "threads": 128, "emit_min": 64, "walk_steps": 64, "kernel": "auto"},
+                 {"blocks_per_cu": 8, "threads": 256, "emit_min": 16, "walk_steps": 4, "kernel": "one"},
+                 {"blocks_per_cu": 1, "threads": 64, "emit_min": 1, "walk_steps": 1, "kernel": "pool"},
+                 {"blocks_per_cu": 8, "threads": 128, "emit_min": 64, "walk_steps": 64, "kernel": "pool"},
+                 {"blocks_per_cu": 2, "threads": 192, "emit_min": 33, "walk_steps": 7, "kernel": "pool"}):
         T.set_tuning(**tune)
         T.zero_estimators()
         c = T.energy_packet_driver(1, n, 0)
