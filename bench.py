@@ -72,12 +72,7 @@ def lanes_per_pass(c):
             "walk_passes": c["n_walk_passes"], "emit_passes": c["n_emit_passes"]}
 
 
-def algorithmic_bytes(c, nbins):
-    """24 B per segment (4 active + 4 opacity + 16 FP64 J read-modify-write),
-    8 + 4*ceil(log2 nbins) B per absorption (line fraction, u-test operand,
-    binary search of the re-emission CDF), 16 B per escape (FP64 RMW)."""
-    return (24.0 * c["n_seg"] + (8.0 + 4.0 * math.ceil(math.log2(nbins))) * c["n_abs"]
-            + 16.0 * c["n_esc"])
+from mocassin_amd.transport import algorithmic_bytes  # noqa: E402
 
 
 def host_cores():

@@ -209,6 +209,18 @@ if __name__ == "__main__":
     print(json.dumps(te_convergence(log=lambda m: print(m, file=sys.stderr, flush=True))))
 
 
+def sweep_roofline(rec, npk, nbins):
+    """HBM roofline of one iteration's sweep: its algorithmic bytes (the
+    transport byte model over the sweep's per-packet counts) over the
+    kernel time, against 8 TB/s."""
+    from .transport import algorithmic_bytes
+    c = {"n_seg": rec["segments_per_packet"] * npk, "n_abs": rec["absorptions_per_packet"] * npk,
+         "n_esc": rec["escapes_per_packet"] * npk}
+    gbs = algorithmic_bytes(c, nbins) / (rec["transport_ms"] * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": gbs, "peak": 8000.0, "unit": "GB/s", "frac": gbs / 8000.0,
+            "segments_per_packet": rec["segments_per_packet"], "kernel_ms": rec["transport_ms"]}
+
+
 def loop_convergence(packets=2e8, ref_iters=16, ref_avg=8, max_iters=12, train_steps=6000, batch=4096,
                      lr=2e-3, te_weight=5.0, log=print):
     """Wall time to converged Te over the iteration, plain versus warm-started,
@@ -344,7 +356,10 @@ def loop_convergence(packets=2e8, ref_iters=16, ref_avg=8, max_iters=12, train_s
                    "balance_ms_per_iteration": [r["balance_ms"] for r in recs],
                    "transport_ms_per_iteration": [r["transport_ms"] for r in recs],
                    "wall_ms_per_iteration": [r["wall_ms"] for r in recs],
-                   "mean_T_iterations": [r["mean_T_iterations"] for r in recs]}
+                   "mean_T_iterations": [r["mean_T_iterations"] for r in recs],
+                   "segments_per_packet_per_iteration": [r["segments_per_packet"] for r in recs],
+                   "absorptions_per_packet_per_iteration": [r["absorptions_per_packet"] for r in recs],
+                   "sweep_roofline": sweep_roofline(recs[-1], npk, p["nbins"])}
             log("%s: %s after %d iterations, %.0f ms wall, balance %.1f ms (within 1%%: %s)"
                 % (name, "converged" if done else "not converged", len(recs), walls[-1], res["balance_ms"],
                    ", ".join("%.3f" % h for h in hist)))

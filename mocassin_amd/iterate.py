@@ -92,7 +92,8 @@ def iterate(T, state, packets, specCDF, symmetric, max_iter=30, min_convergence=
                "balance_ms": ms_b, "wall_ms": wall * 1e3, "converged_pct": converged_percent(flags, T.nCells),
                "no_hit_pct": 100.0 * float(np.count_nonzero(~hit)) / T.nCells,
                "mean_T_iterations": float(nIt[1:][hit].mean()) if hit.any() else 0.0,
-               "segments_per_packet": c["n_seg"] / packets, "absorptions_per_packet": c["n_abs"] / packets}
+               "segments_per_packet": c["n_seg"] / packets, "absorptions_per_packet": c["n_abs"] / packets,
+               "escapes_per_packet": c["n_esc"] / packets}
         records.append(rec)
         if log:
             log("iteration %d: %.1f%% converged, %.1f ms wall (transport %.1f, balance %.1f, opacity %.2f, "

@@ -14,6 +14,7 @@ All arrays use the reference's Fortran layouts:
     specCDF (nbins, nStars) float32 order='F'
 """
 import ctypes as C
+import math
 
 import numpy as np
 
@@ -538,6 +539,15 @@ def scale_J(J, symmetric):
         out = out / 8.
     return out
 
+
+
+def algorithmic_bytes(c, nbins):
+    """Algorithmic HBM bytes of one sweep from its counters: 24 B per
+    segment (4 active + 4 opacity + 16 FP64 J read-modify-write), 8 +
+    4*ceil(log2 nbins) B per absorption (line fraction, u-test operand,
+    binary search of the re-emission CDF), 16 B per escape (FP64 RMW)."""
+    return (24.0 * c["n_seg"] + (8.0 + 4.0 * math.ceil(math.log2(nbins))) * c["n_abs"]
+            + 16.0 * c["n_esc"])
 
 def probe_atomics(words, requests, lanes=64, fp64=True, device=0):
     """Atomic adds per second of the device (mc_probe_atomics): `requests`
