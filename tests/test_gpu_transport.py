@@ -281,6 +281,37 @@ def test_results_independent_of_launch_shape():
     T.close()
 
 
+def test_tuning_kernel_field():
+    """mc_tuning.kernel: out-of-range codes are refused and leave the shape
+    unchanged; the names map to the header's codes; every kernel runs the
+    same packets to the same counters."""
+    from mocassin_amd import McError
+    from mocassin_amd._capi import KERNELS as CODES
+    g = golden("transport_c1mini_thick")
+    P = oracle_problem(g)
+    n, seed = 3000, int(g["seed"])
+    T = transport_from_problem(P, seed)
+    try:
+        assert T.get_tuning() == {"blocks_per_cu": 4, "threads": 256, "emit_min": 16, "walk_steps": 16,
+                                  "kernel": CODES["auto"]}
+        for bad in (-1, 4, 99):
+            with pytest.raises(McError):
+                T.set_tuning(kernel=bad)
+        assert T.get_tuning()["kernel"] == CODES["auto"]
+        ref = None
+        for name in ("auto", "one", "spare", "pool"):
+            T.set_tuning(kernel=name)
+            assert T.get_tuning()["kernel"] == CODES[name]
+            T.zero_estimators()
+            c = T.energy_packet_driver(1, n, 0)
+            if ref is None:
+                ref = c
+            for k in ("n_seg", "n_abs", "n_esc", "n_line", "n_flights"):
+                assert c[k] == ref[k], (name, k)
+    finally:
+        T.close()
+
+
 def test_source_on_cell_walls_equals_oracle():
     """A source placed exactly on the corner where the walls of its cell meet
     those of the cells above it: every stellar packet moving up an axis
