@@ -64,6 +64,17 @@ def shape_of(t):
     return s if name == "auto" else s + "," + name
 
 
+def kernel_name(t):
+    """The device kernel a launch shape runs on the 45^3 octant (a grid the
+    pool kernel takes), as rocprofv3 names it."""
+    from mocassin_amd._capi import KERNELS
+    k = t.get("kernel", 0)
+    if k == KERNELS["auto"]:
+        k = KERNELS["spare"] if t["blocks_per_cu"] <= 3 else KERNELS["pool"]
+    return {KERNELS["one"]: "mcd::transport_kernel<false>", KERNELS["spare"]: "mcd::transport_kernel<true>",
+            KERNELS["pool"]: "mcd::pool_kernel<8, 32>"}[k]
+
+
 def lanes_per_pass(c):
     """Busy lanes of a 64-lane wavefront per walk pass and per emission
     pass of one launch, from the kernel's own pass counters."""
@@ -413,7 +424,7 @@ def main():
                    "segments_per_packet": c0["n_seg"] / npk, "absorptions_per_packet": c0["n_abs"] / npk},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_note,
-                     "algorithmic_bytes": byts, "kernel": "transport_kernel", "kernel_ms": kms,
+                     "algorithmic_bytes": byts, "kernel": kernel_name(tuning), "kernel_ms": kms,
                      "segments": c0["n_seg"], "absorptions": c0["n_abs"], "escapes": c0["n_esc"],
                      "lanes_per_pass": lanes_per_pass(c0)},
     }
