@@ -1,22 +1,24 @@
-// mc_transport.hip - persistent packet-transport kernel and geometry replay
+// mc_transport.hip - persistent packet-transport kernels and geometry replay
 // kernel for MI355X (gfx950).
 //
-// Transport: one lane owns one packet at a time; lanes refill from a global
-// packet counter with one atomic per wavefront.  Packet events are split in
-// two classes and scheduled per wavefront so that SIMD lanes stay coherent:
+// Transport: lanes refill from a global packet counter with one atomic per
+// wavefront.  Packet events are split in two classes and scheduled per
+// wavefront so that SIMD lanes stay coherent:
 //   walk      one cell segment of pathSegment (a few FP32 divisions, one
-//             opacity gather, one FP64 atomic): the common event;
+//             opacity gather, one J update): the common event;
 //   emission  creation of a stellar packet or re-emission of an absorbed one
 //             (Philox draws, CDF search, direction, optical-depth draw):
 //             about ten times the instructions of a walk step.
-// Lanes that need an emission park until at least `emit_min` lanes of the
-// wavefront (or half of its live lanes) are waiting, then all of them emit
-// in one pass; in between the wavefront runs `walk_steps` walk passes.  The
-// per-axis arrays, the source CDF and its guide table live in LDS; J_nu is
-// a dense (x,y,z)-indexed FP64 array.  A walk pass does not update it
-// directly: each wavefront appends its lanes' (lattice point, bin, value)
-// to a queue in LDS and, when the queue is nearly full, flushes it with
-// FP64 atomics of all 64 lanes, off the walk's dependency chain.
+// Three kernels share that code and differ in how a wavefront keeps its
+// lanes busy between emission passes: one packet per lane
+// (transport_kernel<false>), a spare packet per lane in registers
+// (transport_kernel<true>), and a pool of packets per wavefront in LDS
+// (pool_kernel).  The per-axis arrays, the source CDF and its guide table
+// live in LDS.  J_nu and the opacity tables are stored in 2x2x2 bricks of
+// 4x4x4 tiles (mc_device.h); each lane queues its J updates in a ring in
+// LDS, and a full ring flushes every lane's ring with FP64 atomics, one
+// packet's consecutive updates to an instruction, so that adds sharing a
+// 64-byte line travel as one request.
 #include "mc_device.h"
 #include "mc_kernels.h"
 
